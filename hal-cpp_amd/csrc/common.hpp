@@ -1,0 +1,213 @@
+// common.hpp -- runtime plumbing shared by every plan in libhalhip.so:
+// status/error handling across the C ABI, the device context, owned device
+// buffers, HIP-event kernel profiling and cached hipGraph replays.
+#pragma once
+
+#include "halhip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <map>
+#include <new>
+#include <string>
+#include <utility>
+#include <vector>
+
+struct halhip_context
+{
+    int32_t device = 0;
+    int32_t failNextAllocs = 0; // test hook (halhip_context_fail_next_allocs)
+};
+
+namespace halhip
+{
+    // Error carried from the implementation to the C ABI, where guard()
+    // turns it into a status code plus the thread's last-error string.
+    struct Error : std::exception
+    {
+        int code;
+        std::string message;
+        Error(int c, std::string m) : code(c), message(std::move(m)) {}
+        const char *what() const noexcept override { return message.c_str(); }
+    };
+
+    void SetLastError(const std::string &message);
+
+    [[noreturn]] inline void Fail(int code, const std::string &message)
+    {
+        throw Error(code, message);
+    }
+
+    inline void Require(bool ok, const std::string &message)
+    {
+        if (!ok)
+        {
+            Fail(HALHIP_ERR_INVALID_ARGUMENT, message);
+        }
+    }
+
+    inline void Check(hipError_t e, const char *what)
+    {
+        if (e != hipSuccess)
+        {
+            int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? HALHIP_ERR_OUT_OF_MEMORY
+                                                                                     : HALHIP_ERR_HIP;
+            Fail(code, std::string(what) + ": " + hipGetErrorString(e));
+        }
+    }
+
+    // Runs f, mapping any exception to a status code and clearing or setting
+    // the calling thread's last error.
+    template <class F>
+    int Guard(F &&f)
+    {
+        try
+        {
+            f();
+            SetLastError("");
+            return HALHIP_OK;
+        }
+        catch (const Error &e)
+        {
+            SetLastError(e.message);
+            return e.code;
+        }
+        catch (const std::bad_alloc &)
+        {
+            SetLastError("host allocation failed");
+            return HALHIP_ERR_OUT_OF_MEMORY;
+        }
+        catch (const std::exception &e)
+        {
+            SetLastError(e.what());
+            return HALHIP_ERR_HIP;
+        }
+    }
+
+    // Makes ctx's device current for the calling thread.
+    inline void Bind(const halhip_context *ctx)
+    {
+        Check(hipSetDevice(ctx->device), "hipSetDevice");
+    }
+
+    // Device allocation owned by a plan; freed on destruction.
+    template <class T>
+    class DeviceArray
+    {
+    public:
+        DeviceArray() = default;
+        DeviceArray(const DeviceArray &) = delete;
+        DeviceArray &operator=(const DeviceArray &) = delete;
+        ~DeviceArray() { Release(); }
+
+        void Allocate(halhip_context *ctx, size_t count)
+        {
+            Release();
+            if (ctx->failNextAllocs > 0)
+            {
+                ctx->failNextAllocs--;
+                Fail(HALHIP_ERR_OUT_OF_MEMORY, "device allocation failed (forced by halhip_context_fail_next_allocs)");
+            }
+            size_t bytes = (count == 0 ? 1 : count) * sizeof(T);
+            Check(hipMalloc(reinterpret_cast<void **>(&m_ptr), bytes), "hipMalloc");
+            m_count = count;
+        }
+        void Release()
+        {
+            if (m_ptr != nullptr)
+            {
+                (void)hipFree(m_ptr);
+                m_ptr = nullptr;
+                m_count = 0;
+            }
+        }
+        T *get() const { return m_ptr; }
+        size_t size() const { return m_count; }
+
+    private:
+        T *m_ptr = nullptr;
+        size_t m_count = 0;
+    };
+
+    // Per-kernel device time accumulated with HIP events bracketing each
+    // launch on the plan's stream.  Off by default; when off Begin/End are
+    // no-ops and launches are not serialised by event records.
+    class Profiler
+    {
+    public:
+        ~Profiler();
+        void SetOn(bool on) { m_on = on; }
+        bool On() const { return m_on; }
+        void Begin(hipStream_t s, const char *name);
+        void End(hipStream_t s);
+        // Waits for the recorded events and folds them into the totals.
+        void Collect();
+        void Reset();
+        int Export(int32_t maxKernels, int32_t *nKernels, char *names, int32_t namesLen, double *totalMs,
+                   int64_t *launches);
+
+    private:
+        struct Pending
+        {
+            std::string name;
+            hipEvent_t start;
+            hipEvent_t stop;
+        };
+        hipEvent_t Take();
+        bool m_on = false;
+        std::vector<Pending> m_pending;
+        std::vector<hipEvent_t> m_free;
+        std::vector<std::pair<std::string, std::pair<double, int64_t>>> m_totals;
+    };
+
+    // A stream plus elapsed-time events for the async entry points.
+    struct StreamClock
+    {
+        hipStream_t stream = nullptr;
+        hipEvent_t start = nullptr;
+        hipEvent_t stop = nullptr;
+        bool pending = false;
+        void Create();
+        void Destroy();
+        void Mark() { Check(hipEventRecord(start, stream), "hipEventRecord"); pending = true; }
+        void Sync(float *elapsedMs);
+    };
+
+    // Instantiated graphs replaying a fixed launch sequence, keyed by a byte
+    // string describing the launch parameters.
+    class GraphCache
+    {
+    public:
+        ~GraphCache() { Clear(); }
+        hipGraphExec_t Find(const std::string &key) const;
+        void Insert(const std::string &key, hipGraphExec_t exec);
+        void Clear();
+
+    private:
+        std::map<std::string, hipGraphExec_t> m_execs;
+    };
+
+    template <class T>
+    inline void AppendKey(std::string &key, const T &v)
+    {
+        key.append(reinterpret_cast<const char *>(&v), sizeof(T));
+    }
+
+    // Host copy of a Dirichlet description expanded to four edge arrays.
+    struct EdgeValues
+    {
+        bool dirichlet = false;
+        std::vector<float> xlo, xhi, ylo, yhi; // [Y], [Y], [X], [X]
+        void From(const halhip_bc *bc, int32_t X, int32_t Y);
+        void AppendTo(std::string &key) const;
+    };
+
+    // Number of bytes in a flat field, checked against the plan's shape.
+    inline void RequireLength(int64_t n, int64_t expect, const char *caller)
+    {
+        Require(n == expect, std::string(caller) + ": field and deltas must each hold xDim*yDim values");
+    }
+}

@@ -1,0 +1,600 @@
+// diffusion2d.hip -- explicit 5-point diffusion with fused decay, point
+// sources and commit, for gfx950.
+//
+// Replaces the reference's two-dispatch Vulkan step (Diffusion2D.comp +
+// DiffusionUpdate2D.comp, src/Gpu/DiffusionKernel2D.cpp:244-304) with one
+// pass per tick: each lane reads its cell and four neighbours and writes the
+// committed value, so a tick moves 8 B/cell of HBM traffic (read u, write u')
+// instead of ~28.  Lanes run along y, the contiguous axis of HAL's x*yDim+y
+// layout, so every wavefront row access is a coalesced 256-byte segment; the
+// x-neighbour rows are re-read from L2.
+//
+// Semantics follow src/Pde/PDEequations.cpp:140-161 (zero flux / periodic),
+// include/Hal/Pde/PDEequations.h:50-98, 264-287 (Dirichlet ghost
+// (bc - c)*2), src/Pde/PDEequations.cpp:199-228 (per-cell face rates) and the
+// PDEGrid2D tick Diffusion -> MulAll(-k) -> Add -> Update
+// (include/Hal/Grids/PDEGrid2D.h:103-218).
+#include "common.hpp"
+
+#include <algorithm>
+#include <string>
+
+using namespace halhip;
+
+namespace
+{
+    constexpr int kBlockY = 64;
+    constexpr int kBlockX = 4;
+    constexpr int kInlineSources = 64; // sources scanned per lane; above this a dense map is used
+
+    struct ExplicitArgs
+    {
+        int32_t X;
+        int32_t Y;
+        float rate;
+        float negDecay;
+        int32_t wrapX;
+        int32_t wrapY;
+        int32_t dirichlet;
+        const float *bcXlo; // [Y] value at x = -1
+        const float *bcXhi; // [Y] value at x = X
+        const float *bcYlo; // [X] value at y = -1
+        const float *bcYhi; // [X] value at y = Y
+        int32_t nSources;   // inline list (<= kInlineSources) ...
+        const int64_t *srcIdx;
+        const float *srcVal;
+        const float *srcDense; // ... or a dense per-cell map (null if unused)
+    };
+
+    // Difference to the neighbour at (nx, ny) of the cell (x, y) with value
+    // c, reading along one axis; off-grid handling per the axis mode.
+    __device__ __forceinline__ float NeighbourDiffX(const float *__restrict__ u, const ExplicitArgs &a, int nx, int y,
+                                                    float c)
+    {
+        if (nx >= 0 && nx < a.X)
+        {
+            return u[static_cast<size_t>(nx) * a.Y + y] - c;
+        }
+        if (a.wrapX)
+        {
+            int w = (nx < 0) ? nx + a.X : nx - a.X;
+            return u[static_cast<size_t>(w) * a.Y + y] - c;
+        }
+        if (a.dirichlet)
+        {
+            float bc = (nx < 0) ? a.bcXlo[y] : a.bcXhi[y];
+            return (bc - c) * 2.0f;
+        }
+        return 0.0f;
+    }
+
+    __device__ __forceinline__ float NeighbourDiffY(const float *__restrict__ row, const ExplicitArgs &a, int x,
+                                                    int ny, float c)
+    {
+        if (ny >= 0 && ny < a.Y)
+        {
+            return row[ny] - c;
+        }
+        if (a.wrapY)
+        {
+            int w = (ny < 0) ? ny + a.Y : ny - a.Y;
+            return row[w] - c;
+        }
+        if (a.dirichlet)
+        {
+            float bc = (ny < 0) ? a.bcYlo[x] : a.bcYhi[x];
+            return (bc - c) * 2.0f;
+        }
+        return 0.0f;
+    }
+
+    // One tick.  TICK: out = u + (rate*lap + negDecay*u + src); otherwise
+    // out = rate*lap (the IDiffusionBackend2D delta).
+    template <bool TICK>
+    __global__ __launch_bounds__(kBlockY *kBlockX) void ExplicitStep(ExplicitArgs a, const float *__restrict__ u,
+                                                                      float *__restrict__ out)
+    {
+        const int y = blockIdx.x * kBlockY + threadIdx.x;
+        const int x = blockIdx.y * kBlockX + threadIdx.y;
+        if (x >= a.X || y >= a.Y)
+        {
+            return;
+        }
+        const size_t i = static_cast<size_t>(x) * a.Y + y;
+        const float *row = u + static_cast<size_t>(x) * a.Y;
+        const float c = row[y];
+        float s = NeighbourDiffX(u, a, x + 1, y, c);
+        s += NeighbourDiffX(u, a, x - 1, y, c);
+        s += NeighbourDiffY(row, a, x, y + 1, c);
+        s += NeighbourDiffY(row, a, x, y - 1, c);
+        float delta = s * a.rate;
+        if (!TICK)
+        {
+            out[i] = delta;
+            return;
+        }
+        delta += c * a.negDecay;
+        if (a.srcDense != nullptr)
+        {
+            delta += a.srcDense[i];
+        }
+        else
+        {
+            for (int k = 0; k < a.nSources; k++)
+            {
+                if (a.srcIdx[k] == static_cast<int64_t>(i))
+                {
+                    delta += a.srcVal[k];
+                }
+            }
+        }
+        out[i] = c + delta;
+    }
+
+    // Per-cell face rates: +x/+y face uses this cell's rate (0 past a
+    // non-wrapped edge, where the neighbour difference is 0 anyway), -x/-y
+    // face the neighbour's.  src/Pde/PDEequations.cpp:199-228.
+    __global__ __launch_bounds__(kBlockY *kBlockX) void ExplicitRatesDelta(int32_t X, int32_t Y, int32_t wrapX,
+                                                                            int32_t wrapY,
+                                                                            const float *__restrict__ u,
+                                                                            const float *__restrict__ rx,
+                                                                            const float *__restrict__ ry,
+                                                                            float *__restrict__ out)
+    {
+        const int y = blockIdx.x * kBlockY + threadIdx.x;
+        const int x = blockIdx.y * kBlockX + threadIdx.y;
+        if (x >= X || y >= Y)
+        {
+            return;
+        }
+        const size_t i = static_cast<size_t>(x) * Y + y;
+        const float c = u[i];
+        auto at = [&](const float *f, int xx, int yy, bool &ok) -> float
+        {
+            ok = true;
+            if (xx < 0 || xx >= X)
+            {
+                if (!wrapX)
+                {
+                    ok = false;
+                    return 0.0f;
+                }
+                xx = (xx < 0) ? xx + X : xx - X;
+            }
+            if (yy < 0 || yy >= Y)
+            {
+                if (!wrapY)
+                {
+                    ok = false;
+                    return 0.0f;
+                }
+                yy = (yy < 0) ? yy + Y : yy - Y;
+            }
+            return f[static_cast<size_t>(xx) * Y + yy];
+        };
+        bool ok;
+        float s = 0.0f;
+        float v = at(u, x + 1, y, ok);
+        s += ok ? (v - c) * rx[i] : 0.0f;
+        v = at(u, x - 1, y, ok);
+        float r = at(rx, x - 1, y, ok);
+        s += ok ? (v - c) * r : 0.0f;
+        v = at(u, x, y + 1, ok);
+        s += ok ? (v - c) * ry[i] : 0.0f;
+        v = at(u, x, y - 1, ok);
+        r = at(ry, x, y - 1, ok);
+        s += ok ? (v - c) * r : 0.0f;
+        out[i] = s;
+    }
+}
+
+struct halhip_diffusion2d
+{
+    halhip_context *ctx = nullptr;
+    int32_t X = 0;
+    int32_t Y = 0;
+    DeviceArray<float> field[2]; // ping-pong resident field
+    int cur = 0;
+    DeviceArray<float> aux[2];   // interface scratch (delta, rates)
+    DeviceArray<float> bc;       // 2Y + 2X Dirichlet edges
+    DeviceArray<int64_t> srcIdx;
+    DeviceArray<float> srcVal;
+    DeviceArray<float> srcDense;
+    int32_t nSources = 0;
+    EdgeValues edges;
+    std::string edgeKey;
+    StreamClock clock;
+    Profiler prof;
+    GraphCache graphs;
+    std::vector<float> hostBuf;
+};
+
+namespace
+{
+    size_t Cells(const halhip_diffusion2d *p)
+    {
+        return static_cast<size_t>(p->X) * static_cast<size_t>(p->Y);
+    }
+
+    void RequirePlan(const halhip_diffusion2d *p, const char *caller)
+    {
+        Require(p != nullptr, std::string(caller) + ": null plan");
+        Bind(p->ctx);
+    }
+
+    // Uploads Dirichlet edges when they changed and fills the kernel args.
+    ExplicitArgs MakeArgs(halhip_diffusion2d *p, double rate, double decay, int32_t wrapX, int32_t wrapY,
+                          const halhip_bc *bc)
+    {
+        EdgeValues e;
+        e.From(bc, p->X, p->Y);
+        std::string key;
+        e.AppendTo(key);
+        if (e.dirichlet && key != p->edgeKey)
+        {
+            std::vector<float> packed;
+            packed.insert(packed.end(), e.xlo.begin(), e.xlo.end());
+            packed.insert(packed.end(), e.xhi.begin(), e.xhi.end());
+            packed.insert(packed.end(), e.ylo.begin(), e.ylo.end());
+            packed.insert(packed.end(), e.yhi.begin(), e.yhi.end());
+            Check(hipMemcpyAsync(p->bc.get(), packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice,
+                                 p->clock.stream),
+                  "hipMemcpyAsync");
+            Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+            p->edgeKey = key;
+        }
+        ExplicitArgs a{};
+        a.X = p->X;
+        a.Y = p->Y;
+        a.rate = static_cast<float>(rate);
+        a.negDecay = static_cast<float>(-decay);
+        a.wrapX = wrapX ? 1 : 0;
+        a.wrapY = wrapY ? 1 : 0;
+        a.dirichlet = e.dirichlet ? 1 : 0;
+        a.bcXlo = p->bc.get();
+        a.bcXhi = p->bc.get() + p->Y;
+        a.bcYlo = p->bc.get() + 2 * p->Y;
+        a.bcYhi = p->bc.get() + 2 * p->Y + p->X;
+        a.nSources = (p->srcDense.get() != nullptr) ? 0 : p->nSources;
+        a.srcIdx = p->srcIdx.get();
+        a.srcVal = p->srcVal.get();
+        a.srcDense = p->srcDense.get();
+        return a;
+    }
+
+    dim3 Grid(const halhip_diffusion2d *p)
+    {
+        return dim3(static_cast<unsigned>((p->Y + kBlockY - 1) / kBlockY),
+                    static_cast<unsigned>((p->X + kBlockX - 1) / kBlockX));
+    }
+
+    void LaunchTick(halhip_diffusion2d *p, const ExplicitArgs &a, const float *in, float *out)
+    {
+        p->prof.Begin(p->clock.stream, "explicit_tick");
+        hipLaunchKernelGGL(ExplicitStep<true>, Grid(p), dim3(kBlockY, kBlockX), 0, p->clock.stream, a, in, out);
+        Check(hipGetLastError(), "ExplicitStep launch");
+        p->prof.End(p->clock.stream);
+    }
+
+    // Enqueues `steps` ticks; full chunks of kChunk ticks replay a cached
+    // graph (an even chunk keeps the ping-pong parity), the rest launch
+    // directly.  With profiling on, every tick is launched directly so each
+    // gets its own event pair.
+    void EnqueueTicks(halhip_diffusion2d *p, int32_t steps, double rate, double decay, int32_t wrapX, int32_t wrapY,
+                      const halhip_bc *bc)
+    {
+        Require(steps >= 0, "halhip_diffusion2d_run_steps: negative steps");
+        ExplicitArgs a = MakeArgs(p, rate, decay, wrapX, wrapY, bc);
+        constexpr int32_t kChunk = 50;
+        int32_t done = 0;
+        if (!p->prof.On() && steps >= kChunk)
+        {
+            std::string key;
+            AppendKey(key, a);
+            AppendKey(key, p->cur);
+            hipGraphExec_t exec = p->graphs.Find(key);
+            if (exec == nullptr)
+            {
+                hipGraph_t graph;
+                Check(hipStreamBeginCapture(p->clock.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+                int c = p->cur;
+                for (int32_t k = 0; k < kChunk; k++)
+                {
+                    LaunchTick(p, a, p->field[c].get(), p->field[1 - c].get());
+                    c = 1 - c;
+                }
+                Check(hipStreamEndCapture(p->clock.stream, &graph), "hipStreamEndCapture");
+                Check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+                (void)hipGraphDestroy(graph);
+                p->graphs.Insert(key, exec);
+            }
+            for (; done + kChunk <= steps; done += kChunk)
+            {
+                Check(hipGraphLaunch(exec, p->clock.stream), "hipGraphLaunch");
+            }
+        }
+        for (; done < steps; done++)
+        {
+            LaunchTick(p, a, p->field[p->cur].get(), p->field[1 - p->cur].get());
+            p->cur = 1 - p->cur;
+        }
+    }
+}
+
+extern "C"
+{
+    int halhip_diffusion2d_create(halhip_context *ctx, int32_t xDim, int32_t yDim, halhip_diffusion2d **out)
+    {
+        halhip_diffusion2d *p = nullptr;
+        int rc = Guard(
+            [&]
+            {
+                Require(out != nullptr, "halhip_diffusion2d_create: null out");
+                *out = nullptr;
+                Require(ctx != nullptr, "halhip_diffusion2d_create: null context");
+                Require(xDim > 0 && yDim > 0, "halhip_diffusion2d_create: dimensions must be positive");
+                Require(static_cast<int64_t>(xDim) * yDim <= (int64_t(1) << 31),
+                        "halhip_diffusion2d_create: grid larger than 2^31 cells");
+                Bind(ctx);
+                p = new halhip_diffusion2d();
+                p->ctx = ctx;
+                p->X = xDim;
+                p->Y = yDim;
+                size_t n = Cells(p);
+                p->field[0].Allocate(ctx, n);
+                p->field[1].Allocate(ctx, n);
+                p->aux[0].Allocate(ctx, n);
+                p->bc.Allocate(ctx, 2 * static_cast<size_t>(xDim) + 2 * static_cast<size_t>(yDim));
+                p->clock.Create();
+                Check(hipMemsetAsync(p->field[0].get(), 0, n * sizeof(float), p->clock.stream), "hipMemset");
+                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                *out = p;
+            });
+        if (rc != HALHIP_OK && p != nullptr)
+        {
+            p->clock.Destroy();
+            delete p;
+        }
+        return rc;
+    }
+
+    int halhip_diffusion2d_destroy(halhip_diffusion2d *p)
+    {
+        return Guard(
+            [&]
+            {
+                if (p == nullptr)
+                {
+                    return;
+                }
+                Bind(p->ctx);
+                p->graphs.Clear();
+                p->clock.Destroy();
+                delete p;
+            });
+    }
+
+    int halhip_diffusion2d_diffuse_f64(halhip_diffusion2d *p, const double *field, double *deltas, int64_t n,
+                                       double diffRate, int32_t wrapX, int32_t wrapY, const halhip_bc *bc)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_diffuse_f64");
+                RequireLength(n, static_cast<int64_t>(Cells(p)), "halhip_diffusion2d_diffuse_f64");
+                Require(field != nullptr && deltas != nullptr, "halhip_diffusion2d_diffuse_f64: null array");
+                size_t cells = Cells(p);
+                p->hostBuf.resize(cells);
+                for (size_t k = 0; k < cells; k++)
+                {
+                    p->hostBuf[k] = static_cast<float>(field[k]);
+                }
+                Check(hipMemcpyAsync(p->aux[0].get(), p->hostBuf.data(), cells * sizeof(float),
+                                     hipMemcpyHostToDevice, p->clock.stream),
+                      "hipMemcpyAsync");
+                ExplicitArgs a = MakeArgs(p, diffRate, 0.0, wrapX, wrapY, bc);
+                if (p->aux[1].get() == nullptr)
+                {
+                    p->aux[1].Allocate(p->ctx, cells);
+                }
+                p->prof.Begin(p->clock.stream, "explicit_delta");
+                hipLaunchKernelGGL(ExplicitStep<false>, Grid(p), dim3(kBlockY, kBlockX), 0, p->clock.stream, a,
+                                   p->aux[0].get(), p->aux[1].get());
+                Check(hipGetLastError(), "ExplicitStep launch");
+                p->prof.End(p->clock.stream);
+                Check(hipMemcpyAsync(p->hostBuf.data(), p->aux[1].get(), cells * sizeof(float),
+                                     hipMemcpyDeviceToHost, p->clock.stream),
+                      "hipMemcpyAsync");
+                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                for (size_t k = 0; k < cells; k++)
+                {
+                    deltas[k] += static_cast<double>(p->hostBuf[k]);
+                }
+            });
+    }
+
+    int halhip_diffusion2d_diffuse_rates_f64(halhip_diffusion2d *p, const double *field, double *deltas,
+                                             const double *ratesX, const double *ratesY, int64_t n, int32_t wrapX,
+                                             int32_t wrapY)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_diffuse_rates_f64");
+                RequireLength(n, static_cast<int64_t>(Cells(p)), "halhip_diffusion2d_diffuse_rates_f64");
+                Require(field != nullptr && deltas != nullptr && ratesX != nullptr && ratesY != nullptr,
+                        "halhip_diffusion2d_diffuse_rates_f64: null array");
+                size_t cells = Cells(p);
+                if (p->aux[1].get() == nullptr)
+                {
+                    p->aux[1].Allocate(p->ctx, cells);
+                }
+                // field -> aux0, ratesX -> aux1, ratesY -> field[1-cur] scratch is not
+                // used: the resident field must survive, so rates go through a
+                // dedicated pair of buffers allocated on first use.
+                static_assert(sizeof(float) == 4, "");
+                std::vector<float> rx(cells), ry(cells);
+                p->hostBuf.resize(cells);
+                for (size_t k = 0; k < cells; k++)
+                {
+                    p->hostBuf[k] = static_cast<float>(field[k]);
+                    rx[k] = static_cast<float>(ratesX[k]);
+                    ry[k] = static_cast<float>(ratesY[k]);
+                }
+                DeviceArray<float> drx, dry;
+                drx.Allocate(p->ctx, cells);
+                dry.Allocate(p->ctx, cells);
+                Check(hipMemcpyAsync(p->aux[0].get(), p->hostBuf.data(), cells * 4, hipMemcpyHostToDevice,
+                                     p->clock.stream),
+                      "hipMemcpyAsync");
+                Check(hipMemcpyAsync(drx.get(), rx.data(), cells * 4, hipMemcpyHostToDevice, p->clock.stream),
+                      "hipMemcpyAsync");
+                Check(hipMemcpyAsync(dry.get(), ry.data(), cells * 4, hipMemcpyHostToDevice, p->clock.stream),
+                      "hipMemcpyAsync");
+                p->prof.Begin(p->clock.stream, "explicit_rates_delta");
+                hipLaunchKernelGGL(ExplicitRatesDelta, Grid(p), dim3(kBlockY, kBlockX), 0, p->clock.stream, p->X,
+                                   p->Y, wrapX ? 1 : 0, wrapY ? 1 : 0, p->aux[0].get(), drx.get(), dry.get(),
+                                   p->aux[1].get());
+                Check(hipGetLastError(), "ExplicitRatesDelta launch");
+                p->prof.End(p->clock.stream);
+                Check(hipMemcpyAsync(p->hostBuf.data(), p->aux[1].get(), cells * 4, hipMemcpyDeviceToHost,
+                                     p->clock.stream),
+                      "hipMemcpyAsync");
+                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                for (size_t k = 0; k < cells; k++)
+                {
+                    deltas[k] += static_cast<double>(p->hostBuf[k]);
+                }
+            });
+    }
+
+    int halhip_diffusion2d_upload(halhip_diffusion2d *p, const float *field, int64_t n)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_upload");
+                RequireLength(n, static_cast<int64_t>(Cells(p)), "halhip_diffusion2d_upload");
+                Require(field != nullptr, "halhip_diffusion2d_upload: null field");
+                Check(hipMemcpyAsync(p->field[p->cur].get(), field, Cells(p) * sizeof(float), hipMemcpyHostToDevice,
+                                     p->clock.stream),
+                      "hipMemcpyAsync");
+                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+            });
+    }
+
+    int halhip_diffusion2d_download(halhip_diffusion2d *p, float *field, int64_t n)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_download");
+                RequireLength(n, static_cast<int64_t>(Cells(p)), "halhip_diffusion2d_download");
+                Require(field != nullptr, "halhip_diffusion2d_download: null field");
+                Check(hipMemcpyAsync(field, p->field[p->cur].get(), Cells(p) * sizeof(float), hipMemcpyDeviceToHost,
+                                     p->clock.stream),
+                      "hipMemcpyAsync");
+                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+            });
+    }
+
+    int halhip_diffusion2d_set_sources(halhip_diffusion2d *p, int32_t count, const int64_t *index,
+                                       const float *value)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_set_sources");
+                Require(count >= 0 && (count == 0 || (index != nullptr && value != nullptr)),
+                        "halhip_diffusion2d_set_sources: bad source arrays");
+                int64_t cells = static_cast<int64_t>(Cells(p));
+                for (int32_t k = 0; k < count; k++)
+                {
+                    Require(index[k] >= 0 && index[k] < cells, "halhip_diffusion2d_set_sources: index out of range");
+                }
+                p->srcDense.Release();
+                p->srcIdx.Release();
+                p->srcVal.Release();
+                p->nSources = count;
+                p->graphs.Clear();
+                if (count == 0)
+                {
+                    return;
+                }
+                if (count <= kInlineSources)
+                {
+                    p->srcIdx.Allocate(p->ctx, static_cast<size_t>(count));
+                    p->srcVal.Allocate(p->ctx, static_cast<size_t>(count));
+                    Check(hipMemcpy(p->srcIdx.get(), index, sizeof(int64_t) * count, hipMemcpyHostToDevice),
+                          "hipMemcpy");
+                    Check(hipMemcpy(p->srcVal.get(), value, sizeof(float) * count, hipMemcpyHostToDevice),
+                          "hipMemcpy");
+                    return;
+                }
+                std::vector<float> dense(static_cast<size_t>(cells), 0.0f);
+                for (int32_t k = 0; k < count; k++)
+                {
+                    dense[static_cast<size_t>(index[k])] += value[k];
+                }
+                p->srcDense.Allocate(p->ctx, dense.size());
+                Check(hipMemcpy(p->srcDense.get(), dense.data(), dense.size() * sizeof(float), hipMemcpyHostToDevice),
+                      "hipMemcpy");
+            });
+    }
+
+    int halhip_diffusion2d_run_steps_async(halhip_diffusion2d *p, int32_t steps, double diffRate, double decay,
+                                           int32_t wrapX, int32_t wrapY, const halhip_bc *bc)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_run_steps");
+                p->clock.Mark();
+                EnqueueTicks(p, steps, diffRate, decay, wrapX, wrapY, bc);
+            });
+    }
+
+    int halhip_diffusion2d_sync(halhip_diffusion2d *p, float *elapsedMs)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_sync");
+                p->clock.Sync(elapsedMs);
+                p->prof.Collect();
+            });
+    }
+
+    int halhip_diffusion2d_run_steps(halhip_diffusion2d *p, int32_t steps, double diffRate, double decay,
+                                     int32_t wrapX, int32_t wrapY, const halhip_bc *bc)
+    {
+        int rc = halhip_diffusion2d_run_steps_async(p, steps, diffRate, decay, wrapX, wrapY, bc);
+        if (rc != HALHIP_OK)
+        {
+            return rc;
+        }
+        return halhip_diffusion2d_sync(p, nullptr);
+    }
+
+    int halhip_diffusion2d_set_profiling(halhip_diffusion2d *p, int32_t on)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_set_profiling");
+                p->prof.Reset();
+                p->prof.SetOn(on != 0);
+            });
+    }
+
+    int halhip_diffusion2d_profile(halhip_diffusion2d *p, int32_t maxKernels, int32_t *nKernels, char *names,
+                                   int32_t namesLen, double *totalMs, int64_t *launches)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_profile");
+                p->prof.Export(maxKernels, nKernels, names, namesLen, totalMs, launches);
+            });
+    }
+}

@@ -1,0 +1,305 @@
+"""ctypes binding of libhalhip.so, the C ABI declared in include/halhip.h.
+
+This is the Python mirror of HAL-CPP's three strategy interfaces used by the
+tests and the benchmark (the C++ mirror is include/Hal/Hip/*.h).  It loads the
+in-tree library and raises when it is missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhalhip.so")
+
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_NO_DEVICE = 2
+ERR_OUT_OF_MEMORY = 3
+ERR_HIP = 4
+ERR_UNSUPPORTED = 5
+BC_NONE = 0
+BC_DIRICHLET = 1
+ADI_MAX_Y = 8192
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_lp = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_f64 = ctypes.c_double
+
+
+class HalBC(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("value", ctypes.c_double),
+                ("xlo", _dp), ("xhi", _dp), ("ylo", _dp), ("yhi", _dp)]
+
+
+class HalHipError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+class InvalidArgument(HalHipError, ValueError):
+    pass
+
+
+class NoDevice(HalHipError):
+    pass
+
+
+# Exported symbols, with argtypes.  Everything returns int.
+_SIGS = {
+    "halhip_last_error": None,
+    "halhip_device_count": [_ip],
+    "halhip_context_create": [_i32, ctypes.POINTER(_vp)],
+    "halhip_context_destroy": [_vp],
+    "halhip_context_fail_next_allocs": [_vp, _i32],
+    "halhip_rng_doubles": [ctypes.c_uint64, _i64, _dp],
+    "halhip_diffusion2d_create": [_vp, _i32, _i32, ctypes.POINTER(_vp)],
+    "halhip_diffusion2d_destroy": [_vp],
+    "halhip_diffusion2d_diffuse_f64": [_vp, _dp, _dp, _i64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
+    "halhip_diffusion2d_diffuse_rates_f64": [_vp, _dp, _dp, _dp, _dp, _i64, _i32, _i32],
+    "halhip_diffusion2d_upload": [_vp, _fp, _i64],
+    "halhip_diffusion2d_download": [_vp, _fp, _i64],
+    "halhip_diffusion2d_set_sources": [_vp, _i32, _lp, _fp],
+    "halhip_diffusion2d_run_steps": [_vp, _i32, _f64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
+    "halhip_diffusion2d_run_steps_async": [_vp, _i32, _f64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
+    "halhip_diffusion2d_sync": [_vp, _fp],
+    "halhip_diffusion2d_set_profiling": [_vp, _i32],
+    "halhip_diffusion2d_profile": [_vp, _i32, _ip, ctypes.c_char_p, _i32, _dp, _lp],
+    "halhip_adi2d_create": [_vp, _i32, _i32, ctypes.POINTER(_vp)],
+    "halhip_adi2d_destroy": [_vp],
+    "halhip_adi2d_diffuse_f64": [_vp, _dp, _dp, _i64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
+    "halhip_adi2d_upload": [_vp, _fp, _i64],
+    "halhip_adi2d_download": [_vp, _fp, _i64],
+    "halhip_adi2d_set_sources": [_vp, _i32, _lp, _fp],
+    "halhip_adi2d_run_steps": [_vp, _i32, _f64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
+    "halhip_adi2d_run_steps_async": [_vp, _i32, _f64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
+    "halhip_adi2d_sync": [_vp, _fp],
+    "halhip_adi2d_set_profiling": [_vp, _i32],
+    "halhip_adi2d_profile": [_vp, _i32, _ip, ctypes.c_char_p, _i32, _dp, _lp],
+}
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load libhalhip.so (once).  Raises FileNotFoundError when not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built; run __graft_entry__.build() or make -C hal-cpp_amd")
+    lib = ctypes.CDLL(path)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        if name == "halhip_last_error":
+            fn.restype = ctypes.c_char_p
+            fn.argtypes = []
+            continue
+        fn.restype = ctypes.c_int
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc == OK:
+        return
+    msg = load().halhip_last_error().decode()
+    if rc == ERR_INVALID_ARGUMENT:
+        raise InvalidArgument(rc, msg)
+    if rc == ERR_NO_DEVICE:
+        raise NoDevice(rc, msg)
+    raise HalHipError(rc, msg)
+
+
+def device_count():
+    n = ctypes.c_int32(0)
+    _check(load().halhip_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def rng_doubles(seed, n):
+    out = np.empty(n, np.float64)
+    _check(load().halhip_rng_doubles(seed, n, out.ctypes.data_as(_dp)))
+    return out
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def make_bc(bc, X, Y):
+    """None, a constant Dirichlet value, or a 4-tuple of edge arrays
+    (bc(-1,y), bc(X,y), bc(x,-1), bc(x,Y)).  Returns (struct-or-None, keepalive)."""
+    if bc is None:
+        return None, ()
+    s = HalBC()
+    s.kind = BC_DIRICHLET
+    if np.isscalar(bc):
+        s.value = float(bc)
+        return s, ()
+    arrs = tuple(_f64(a) for a in bc)
+    assert arrs[0].size == Y and arrs[1].size == Y and arrs[2].size == X and arrs[3].size == X
+    s.xlo, s.xhi, s.ylo, s.yhi = (a.ctypes.data_as(_dp) for a in arrs)
+    return s, arrs
+
+
+def _bcp(s):
+    return ctypes.byref(s) if s is not None else None
+
+
+class Context:
+    """halhip_context: replaces Hal::Gpu::ComputeContext."""
+
+    def __init__(self, device=0):
+        self.h = _vp()
+        _check(load().halhip_context_create(device, ctypes.byref(self.h)))
+
+    def fail_next_allocs(self, count):
+        _check(load().halhip_context_fail_next_allocs(self.h, count))
+
+    def close(self):
+        if self.h:
+            load().halhip_context_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _Plan:
+    _prefix = ""
+
+    def _fn(self, name):
+        return getattr(load(), f"halhip_{self._prefix}_{name}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._fn("destroy")(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_profiling(self, on):
+        _check(self._fn("set_profiling")(self.h, 1 if on else 0))
+
+    def profile(self):
+        """{kernel_name: (total_ms, launches)} accumulated since profiling was enabled."""
+        n = ctypes.c_int32(0)
+        names = ctypes.create_string_buffer(4096)
+        ms = (ctypes.c_double * 32)()
+        cnt = (ctypes.c_int64 * 32)()
+        _check(self._fn("profile")(self.h, 32, ctypes.byref(n), names, 4096, ms, cnt))
+        keys = names.value.decode().split(",") if n.value else []
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(keys)}
+
+    def sync(self):
+        t = ctypes.c_float(0.0)
+        _check(self._fn("sync")(self.h, ctypes.byref(t)))
+        return t.value
+
+
+class Diffusion2D(_Plan):
+    """Explicit diffusion plan (IDiffusionBackend2D + resident RunSteps)."""
+    _prefix = "diffusion2d"
+
+    def __init__(self, ctx, X, Y):
+        self.ctx, self.X, self.Y = ctx, X, Y
+        self.h = _vp()
+        _check(load().halhip_diffusion2d_create(ctx.h, X, Y, ctypes.byref(self.h)))
+
+    def diffuse(self, field, deltas, rate, wrapX, wrapY, bc=None):
+        """deltas += one explicit step (in place on the float64 `deltas`)."""
+        field = _f64(field)
+        assert deltas.dtype == np.float64 and deltas.flags.c_contiguous
+        s, keep = make_bc(bc, self.X, self.Y)
+        _check(self._fn("diffuse_f64")(self.h, field.ctypes.data_as(_dp), deltas.ctypes.data_as(_dp),
+                                       deltas.size if field.size == deltas.size else field.size,
+                                       rate, int(wrapX), int(wrapY), _bcp(s)))
+        return deltas
+
+    def diffuse_rates(self, field, deltas, ratesX, ratesY, wrapX, wrapY):
+        field, ratesX, ratesY = _f64(field), _f64(ratesX), _f64(ratesY)
+        _check(self._fn("diffuse_rates_f64")(self.h, field.ctypes.data_as(_dp), deltas.ctypes.data_as(_dp),
+                                             ratesX.ctypes.data_as(_dp), ratesY.ctypes.data_as(_dp),
+                                             field.size, int(wrapX), int(wrapY)))
+        return deltas
+
+    def upload(self, field):
+        field = _f32(field)
+        _check(self._fn("upload")(self.h, field.ctypes.data_as(_fp), field.size))
+
+    def download(self):
+        out = np.empty(self.X * self.Y, np.float32)
+        _check(self._fn("download")(self.h, out.ctypes.data_as(_fp), out.size))
+        return out
+
+    def set_sources(self, idx, val):
+        idx = np.ascontiguousarray(idx, np.int64)
+        val = _f32(val)
+        _check(self._fn("set_sources")(self.h, idx.size, idx.ctypes.data_as(_lp), val.ctypes.data_as(_fp)))
+
+    def run_steps(self, steps, rate, decay, wrapX, wrapY, bc=None, asynchronous=False):
+        s, keep = make_bc(bc, self.X, self.Y)
+        name = "run_steps_async" if asynchronous else "run_steps"
+        _check(self._fn(name)(self.h, steps, rate, decay, int(wrapX), int(wrapY), _bcp(s)))
+
+
+class Adi2D(_Plan):
+    """ADI plan (IAdiDiffusionBackend2D + resident RunSteps)."""
+    _prefix = "adi2d"
+
+    def __init__(self, ctx, X, Y):
+        self.ctx, self.X, self.Y = ctx, X, Y
+        self.h = _vp()
+        _check(load().halhip_adi2d_create(ctx.h, X, Y, ctypes.byref(self.h)))
+
+    def diffuse(self, field, deltas, rate, wrapX, wrapY, bc=None):
+        field = _f64(field)
+        assert deltas.dtype == np.float64 and deltas.flags.c_contiguous
+        s, keep = make_bc(bc, self.X, self.Y)
+        _check(self._fn("diffuse_f64")(self.h, field.ctypes.data_as(_dp), deltas.ctypes.data_as(_dp),
+                                       deltas.size if field.size == deltas.size else field.size,
+                                       rate, int(wrapX), int(wrapY), _bcp(s)))
+        return deltas
+
+    def upload(self, field):
+        field = _f32(field)
+        _check(self._fn("upload")(self.h, field.ctypes.data_as(_fp), field.size))
+
+    def download(self):
+        out = np.empty(self.X * self.Y, np.float32)
+        _check(self._fn("download")(self.h, out.ctypes.data_as(_fp), out.size))
+        return out
+
+    def set_sources(self, idx, val):
+        idx = np.ascontiguousarray(idx, np.int64)
+        val = _f32(val)
+        _check(self._fn("set_sources")(self.h, idx.size, idx.ctypes.data_as(_lp), val.ctypes.data_as(_fp)))
+
+    def run_steps(self, steps, rate, decay, wrapX, wrapY, bc=None, asynchronous=False):
+        s, keep = make_bc(bc, self.X, self.Y)
+        name = "run_steps_async" if asynchronous else "run_steps"
+        _check(self._fn(name)(self.h, steps, rate, decay, int(wrapX), int(wrapY), _bcp(s)))

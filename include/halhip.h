@@ -1,0 +1,210 @@
+/*
+ * halhip.h -- C ABI of the MI355X (gfx950) compute backend for HAL-CPP's hot
+ * path: explicit 5-point diffusion, ADI diffusion and cell-bucket agent forces,
+ * plus batched wells and adjoint ADI.
+ *
+ * Conventions
+ *   - Fields are flat row-major arrays, index x*yDim + y (HAL's layout,
+ *     include/Hal/Grids/GridGeometry2D.h:12).
+ *   - Every function returns an int status (HALHIP_OK == 0).  On failure the
+ *     calling thread's halhip_last_error() holds a message.  No C++ exception
+ *     crosses this boundary.
+ *   - Handles are opaque.  One thread per handle; calls on a handle are
+ *     synchronous (they return after the device work finished) unless the name
+ *     says "_async".
+ *   - "_f64" entry points take the reference interfaces' FP64 host arrays and
+ *     compute in FP32 on the device, exactly like the reference's GPU adapters.
+ *     Resident entry points keep FP32 state on the device between calls.
+ */
+#ifndef HALHIP_H
+#define HALHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define HALHIP_OK 0
+#define HALHIP_ERR_INVALID_ARGUMENT 1 /* maps to std::invalid_argument */
+#define HALHIP_ERR_NO_DEVICE 2        /* no usable GPU: callers may skip */
+#define HALHIP_ERR_OUT_OF_MEMORY 3    /* device allocation failed */
+#define HALHIP_ERR_HIP 4              /* any other HIP runtime error */
+#define HALHIP_ERR_UNSUPPORTED 5      /* shape outside the documented limits */
+
+/* Axis boundary kinds.  A wrapped axis is periodic regardless of bc. */
+#define HALHIP_BC_NONE 0      /* zero flux on non-wrapped axes */
+#define HALHIP_BC_DIRICHLET 1 /* ghost-cell Dirichlet on non-wrapped axes */
+
+/* Dirichlet boundary description.  The reference's BCFn(x, y) is only ever
+ * evaluated at ghost cells, so four edge arrays capture any BCFn:
+ *   xlo[y] = bc(-1, y), xhi[y] = bc(xDim, y)   (y in [0, yDim))
+ *   ylo[x] = bc(x, -1), yhi[x] = bc(x, yDim)   (x in [0, xDim))
+ * A null edge array means "the constant `value`".  Reference:
+ * include/Hal/Pde/ADIequations.h:339-378, include/Hal/Pde/PDEequations.h:264-287. */
+typedef struct halhip_bc
+{
+    int32_t kind;
+    double value;
+    const double *xlo;
+    const double *xhi;
+    const double *ylo;
+    const double *yhi;
+} halhip_bc;
+
+typedef struct halhip_context halhip_context;
+typedef struct halhip_diffusion2d halhip_diffusion2d;
+typedef struct halhip_adi2d halhip_adi2d;
+typedef struct halhip_agents2d halhip_agents2d;
+
+/* ---- library / context -------------------------------------------------- */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *halhip_last_error(void);
+
+/* Number of visible GPUs (0 without a GPU; never fails on a GPU-less host). */
+int halhip_device_count(int32_t *count);
+
+/* Replaces Hal::Gpu::ComputeContext (include/Hal/Gpu/ComputeContext.h:18-49):
+ * binds a device.  HALHIP_ERR_NO_DEVICE when there is none, which callers
+ * treat as "skip" like tests/GpuTestSupport.cpp:9-20. */
+int halhip_context_create(int32_t device, halhip_context **out);
+int halhip_context_destroy(halhip_context *ctx);
+
+/* Test hook: the next `count` device allocations made through ctx fail with
+ * HALHIP_ERR_OUT_OF_MEMORY, so error paths can be exercised from the host. */
+int halhip_context_fail_next_allocs(halhip_context *ctx, int32_t count);
+
+/* ---- host-side seeded inputs (Hal::Rng, src/Rng.cpp:13-101) -------------- */
+
+/* n draws of Rng(seed).NextDouble(). */
+int halhip_rng_doubles(uint64_t seed, int64_t n, double *out);
+
+/* ---- explicit diffusion -------------------------------------------------- */
+
+/* Replaces Hal::Gpu::DiffusionKernel2D / GpuDiffusionBackend2D
+ * (include/Hal/Gpu/DiffusionKernel2D.h:16-20, src/Gpu/GpuDiffusionBackend2D.cpp:22-52).
+ * A plan is sized at creation (like the reference's kernels) and owns its
+ * device buffers and one HIP stream. */
+int halhip_diffusion2d_create(halhip_context *ctx, int32_t xDim, int32_t yDim, halhip_diffusion2d **out);
+int halhip_diffusion2d_destroy(halhip_diffusion2d *plan);
+
+/* IDiffusionBackend2D::Diffuse (include/Hal/Pde/IDiffusionBackend2D.h:26-32):
+ * deltas[i] += one explicit step's change of field[i]; n must equal
+ * xDim*yDim of the plan.  bc may be null. */
+int halhip_diffusion2d_diffuse_f64(halhip_diffusion2d *plan, const double *field, double *deltas, int64_t n,
+                                   double diffRate, int32_t wrapX, int32_t wrapY, const halhip_bc *bc);
+
+/* Per-cell rates (src/Pde/PDEequations.cpp:199-228): deltas += step with
+ * x-face rates ratesX and y-face rates ratesY (each n values). */
+int halhip_diffusion2d_diffuse_rates_f64(halhip_diffusion2d *plan, const double *field, double *deltas,
+                                         const double *ratesX, const double *ratesY, int64_t n, int32_t wrapX,
+                                         int32_t wrapY);
+
+/* Resident state: FP32 field on the device. */
+int halhip_diffusion2d_upload(halhip_diffusion2d *plan, const float *field, int64_t n);
+int halhip_diffusion2d_download(halhip_diffusion2d *plan, float *field, int64_t n);
+
+/* Point sources added every tick (PDEGrid2D::Add, include/Hal/Grids/PDEGrid2D.h:110-114).
+ * Indices are flat cell indices; a repeated index adds twice. */
+int halhip_diffusion2d_set_sources(halhip_diffusion2d *plan, int32_t count, const int64_t *index,
+                                   const float *value);
+
+/* `steps` model ticks on the resident field, each
+ *   u += diffRate*laplacian(u) - decay*u + sources
+ * (Diffusion + MulAll(-decay) + Add + Update, PDEGrid2D.h:103-218;
+ * DiffusionKernel2D::RunSteps, src/Gpu/DiffusionKernel2D.cpp:244-304). */
+int halhip_diffusion2d_run_steps(halhip_diffusion2d *plan, int32_t steps, double diffRate, double decay,
+                                 int32_t wrapX, int32_t wrapY, const halhip_bc *bc);
+
+/* Same ticks enqueued on the plan's stream; returns at once.  Pair with
+ * halhip_diffusion2d_sync. elapsedMs (may be null) receives the device time of
+ * the enqueued work measured with HIP events once synced. */
+int halhip_diffusion2d_run_steps_async(halhip_diffusion2d *plan, int32_t steps, double diffRate, double decay,
+                                       int32_t wrapX, int32_t wrapY, const halhip_bc *bc);
+int halhip_diffusion2d_sync(halhip_diffusion2d *plan, float *elapsedMs);
+int halhip_diffusion2d_set_profiling(halhip_diffusion2d *plan, int32_t on);
+int halhip_diffusion2d_profile(halhip_diffusion2d *plan, int32_t maxKernels, int32_t *nKernels, char *names,
+                               int32_t namesLen, double *totalMs, int64_t *launches);
+
+/* ---- ADI diffusion ------------------------------------------------------- */
+
+/* Replaces Hal::Gpu::AdiKernel2D / GpuAdiDiffusionBackend2D
+ * (include/Hal/Gpu/AdiKernel2D.h:33-41, src/Gpu/GpuAdiDiffusionBackend2D.cpp:24-50).
+ * Limits: yDim <= HALHIP_ADI_MAX_Y (the contiguous axis is solved inside one
+ * workgroup); xDim unbounded.  The reference caps both axes at 1024. */
+#define HALHIP_ADI_MAX_Y 8192
+int halhip_adi2d_create(halhip_context *ctx, int32_t xDim, int32_t yDim, halhip_adi2d **out);
+int halhip_adi2d_destroy(halhip_adi2d *plan);
+
+/* IAdiDiffusionBackend2D::DiffuseADI (include/Hal/Pde/IAdiDiffusionBackend2D.h:25-31):
+ * deltas[i] += ADI(field)[i] - field[i].  bc may be null (zero flux /
+ * periodic); a Dirichlet bc gives PDEGrid2D::DiffusionADI(rate, bc)
+ * (include/Hal/Grids/PDEGrid2D.h:328-347). */
+int halhip_adi2d_diffuse_f64(halhip_adi2d *plan, const double *field, double *deltas, int64_t n, double diffRate,
+                             int32_t wrapX, int32_t wrapY, const halhip_bc *bc);
+
+int halhip_adi2d_upload(halhip_adi2d *plan, const float *field, int64_t n);
+int halhip_adi2d_download(halhip_adi2d *plan, float *field, int64_t n);
+int halhip_adi2d_set_sources(halhip_adi2d *plan, int32_t count, const int64_t *index, const float *value);
+
+/* `steps` model ticks u <- ADI(u) - decay*u + sources on the resident field
+ * (DiffusionADI + MulAll(-decay) + Add + Update; AdiKernel2D::RunSteps). */
+int halhip_adi2d_run_steps(halhip_adi2d *plan, int32_t steps, double diffRate, double decay, int32_t wrapX,
+                           int32_t wrapY, const halhip_bc *bc);
+int halhip_adi2d_run_steps_async(halhip_adi2d *plan, int32_t steps, double diffRate, double decay, int32_t wrapX,
+                                 int32_t wrapY, const halhip_bc *bc);
+int halhip_adi2d_sync(halhip_adi2d *plan, float *elapsedMs);
+
+/* Per-kernel device time of the last run (HIP events around each launch when
+ * profiling is on; profiling is off by default and does not change results).
+ * names: comma-separated kernel names; ms/count arrays of length *nKernels. */
+int halhip_adi2d_set_profiling(halhip_adi2d *plan, int32_t on);
+int halhip_adi2d_profile(halhip_adi2d *plan, int32_t maxKernels, int32_t *nKernels, char *names, int32_t namesLen,
+                         double *totalMs, int64_t *launches);
+
+/* ---- agent forces -------------------------------------------------------- */
+
+/* Replaces Hal::Gpu::AgentForceKernel2D / GpuAgentForceBackend2D
+ * (include/Hal/Gpu/AgentForceKernel2D.h, src/Gpu/GpuAgentForceBackend2D.cpp:24-68).
+ * Sized at creation: grid xDim*yDim bins of size 1 and maxAgents capacity. */
+int halhip_agents2d_create(halhip_context *ctx, int32_t xDim, int32_t yDim, int32_t maxAgents,
+                           halhip_agents2d **out);
+int halhip_agents2d_destroy(halhip_agents2d *plan);
+
+/* IAgentForceBackend2D::AccumulateForces (include/Hal/Agents/IAgentForceBackend2D.h:28-36):
+ * positions (2*nAgents, x0,y0,x1,...), radii (nAgents); velocityDeltas
+ * (2*nAgents) is overwritten. */
+int halhip_agents2d_forces_f64(halhip_agents2d *plan, const double *positions, const double *radii,
+                               int32_t nAgents, double *velocityDeltas, double interactionRad,
+                               double springConstant, int32_t wrapX, int32_t wrapY);
+
+/* Resident agents: FP32 positions and radii on the device. */
+int halhip_agents2d_upload(halhip_agents2d *plan, const float *positions, const float *radii, int32_t nAgents);
+int halhip_agents2d_download_positions(halhip_agents2d *plan, float *positions, int32_t nAgents);
+int halhip_agents2d_download_velocities(halhip_agents2d *plan, float *velocityDeltas, int32_t nAgents);
+/* Bins of the last tick: offsets (xDim*yDim+1 int32, exclusive prefix of
+ * per-bin counts) and bucket (nAgents int32, agent indices in bin order,
+ * ascending index within a bin). */
+int halhip_agents2d_download_bins(halhip_agents2d *plan, int32_t *offsets, int32_t *bucket);
+
+/* `ticks` ticks of bin -> scan -> stable scatter -> force -> move.  The move
+ * is p += dt*v with AgentPT2D::MoveSafePT semantics
+ * (include/Hal/Agents/AgentPT2D.h:157-186): an axis whose target leaves a
+ * non-wrapped grid keeps its old coordinate; a wrapped axis wraps.  ticks == 0
+ * with dt == 0 computes bins and velocities only. */
+int halhip_agents2d_run_ticks(halhip_agents2d *plan, int32_t ticks, double interactionRad, double springConstant,
+                              double dt, int32_t wrapX, int32_t wrapY);
+int halhip_agents2d_run_ticks_async(halhip_agents2d *plan, int32_t ticks, double interactionRad,
+                                    double springConstant, double dt, int32_t wrapX, int32_t wrapY);
+int halhip_agents2d_sync(halhip_agents2d *plan, float *elapsedMs);
+int halhip_agents2d_set_profiling(halhip_agents2d *plan, int32_t on);
+int halhip_agents2d_profile(halhip_agents2d *plan, int32_t maxKernels, int32_t *nKernels, char *names,
+                            int32_t namesLen, double *totalMs, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HALHIP_H */

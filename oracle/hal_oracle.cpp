@@ -1,0 +1,674 @@
+// hal_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// FP64 CPU restatement of HAL-CPP's CPU hot path, used as the checker for the
+// HIP backend. Only tests, the smoke entry point and the CPU-baseline leg of
+// the benchmark may load this library; the product path never links it.
+//
+// Every function names the reference lines whose behaviour it restates
+// (paths relative to the HAL-CPP source tree). Floating-point operation order
+// follows the reference so that this restatement and the compiled reference
+// agree to the last bit on the golden vectors.
+//
+// Pinning (what checks each function; tests/test_oracle.py):
+//   oracle_rng_*            golden stream from the compiled reference
+//   oracle_diffuse*         golden vectors; periodic eigenmode closed form;
+//                           mass invariance; per-cell == uniform rates
+//   oracle_adi*             golden vectors (zero-flux, periodic, Dirichlet);
+//                           Crank-Nicolson eigenmode product; Dirichlet fixed
+//                           point and linear ramp; scipy banded solve
+//   oracle_agent_*          golden vectors; brute-force all-pairs sum; sorted
+//                           (cell, index) order for bins
+//   oracle_adi_cell_*       per-cell-diffusivity ADI and its adjoint: no
+//                           reference counterpart.  Pinned by "uniform D ==
+//                           oracle_adi", the dot-product test <Ax,y>=<x,A^T y>
+//                           and FP64 central finite differences.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace
+{
+    // True modulo; src/MathUtil.cpp:41-46.
+    inline int32_t WrapI(int32_t v, int32_t n)
+    {
+        int32_t m = v % n;
+        return (m < 0) ? (n + m) : m;
+    }
+
+    // Neighbour difference along one axis of a flat x*Y+y field.  Off-grid
+    // neighbours give 0 (zero flux), the wrapped cell (periodic) or the
+    // Dirichlet ghost (bc - c)*2 when pBc is given.
+    // src/Pde/PDEequations.cpp:24-60; include/Hal/Pde/PDEequations.h:50-98.
+    inline double DeltaX(const double *f, double c, int32_t x, int32_t y, int32_t X, int32_t Y, bool wrap,
+                         const double *bcLo, const double *bcHi)
+    {
+        if (x >= 0 && x < X)
+        {
+            return f[static_cast<size_t>(x) * Y + y] - c;
+        }
+        if (wrap)
+        {
+            return f[static_cast<size_t>(WrapI(x, X)) * Y + y] - c;
+        }
+        if (bcLo != nullptr)
+        {
+            double bc = (x < 0) ? bcLo[y] : bcHi[y];
+            return (bc - c) * 2;
+        }
+        return 0;
+    }
+
+    inline double DeltaY(const double *f, double c, int32_t x, int32_t y, int32_t X, int32_t Y, bool wrap,
+                         const double *bcLo, const double *bcHi)
+    {
+        (void)X;
+        if (y >= 0 && y < Y)
+        {
+            return f[static_cast<size_t>(x) * Y + y] - c;
+        }
+        if (wrap)
+        {
+            return f[static_cast<size_t>(x) * Y + WrapI(y, Y)] - c;
+        }
+        if (bcLo != nullptr)
+        {
+            double bc = (y < 0) ? bcLo[x] : bcHi[x];
+            return (bc - c) * 2;
+        }
+        return 0;
+    }
+
+    // Thomas algorithm on explicit coefficient arrays: a sub-diagonal (a[0]
+    // unused), b diagonal, c super-diagonal (c[n-1] unused).
+    // include/Hal/Pde/TdmaSolver.h:27-66.
+    void Thomas(int32_t n, const double *a, const double *b, const double *c, const double *d, double *x,
+                std::vector<double> &work)
+    {
+        work.resize(static_cast<size_t>(n) * 2);
+        double *bp = work.data();
+        double *dp = work.data() + n;
+        bp[0] = b[0];
+        dp[0] = d[0];
+        for (int32_t i = 1; i < n; i++)
+        {
+            double w = a[i] / bp[i - 1];
+            bp[i] = b[i] - w * c[i - 1];
+            dp[i] = d[i] - w * dp[i - 1];
+        }
+        int32_t i = n - 1;
+        double v = dp[i] / bp[i];
+        x[i] = v;
+        for (i -= 1; i >= 0; i--)
+        {
+            v = (dp[i] - c[i] * v) / bp[i];
+            x[i] = v;
+        }
+    }
+
+    // Cyclic tridiagonal solve by Sherman-Morrison: two Thomas solves on the
+    // corner-adjusted matrix, then a rank-1 combination.
+    // include/Hal/Pde/TdmaSolver.h:68-120.
+    void ThomasPeriodic(int32_t n, const double *a, const double *b, const double *c, const double *d, double *x,
+                        std::vector<double> &work)
+    {
+        const double gamma = -b[0];
+        const double alpha = a[0];
+        const double beta = c[n - 1];
+        std::vector<double> bAdj(b, b + n);
+        bAdj[0] = b[0] - gamma;
+        if (n - 1 != 0)
+        {
+            bAdj[n - 1] = b[n - 1] - (alpha * beta) / gamma;
+        }
+        std::vector<double> u(static_cast<size_t>(n), 0.0);
+        u[0] = gamma;
+        if (n - 1 != 0)
+        {
+            u[n - 1] = alpha;
+        }
+        std::vector<double> ys(static_cast<size_t>(n));
+        std::vector<double> zs(static_cast<size_t>(n));
+        Thomas(n, a, bAdj.data(), c, d, ys.data(), work);
+        Thomas(n, a, bAdj.data(), c, u.data(), zs.data(), work);
+        double vEnd = beta / gamma;
+        double vy = ys[0] + ys[n - 1] * vEnd;
+        double vz = zs[0] + zs[n - 1] * vEnd;
+        double factor = vy / (1 + vz);
+        for (int32_t i = 0; i < n; i++)
+        {
+            x[i] = ys[i] - factor * zs[i];
+        }
+    }
+
+    // Coefficients and solve of one ADI line.  off = -rate/2; periodic lines
+    // use b = 1 - 2*off; otherwise each in-line neighbour weighs 1 and past an
+    // end a Dirichlet ghost weighs 2, a zero-flux edge 0.
+    // include/Hal/Pde/ADIequations.h:225-270.
+    void SolveLine(int32_t n, double rate, bool wrap, bool hasBc, const double *d, double *x,
+                   std::vector<double> &work)
+    {
+        double off = -rate / 2;
+        std::vector<double> a(static_cast<size_t>(n), off);
+        std::vector<double> c(static_cast<size_t>(n), off);
+        std::vector<double> b(static_cast<size_t>(n));
+        if (wrap)
+        {
+            for (int32_t i = 0; i < n; i++)
+            {
+                b[i] = 1 - 2 * off;
+            }
+            ThomasPeriodic(n, a.data(), b.data(), c.data(), d, x, work);
+            return;
+        }
+        double endAdjust = hasBc ? 1.0 : -1.0;
+        for (int32_t i = 0; i < n; i++)
+        {
+            double weight = 2.0;
+            if (i == 0)
+            {
+                weight += endAdjust;
+            }
+            if (i == n - 1)
+            {
+                weight += endAdjust;
+            }
+            b[i] = 1 - weight * off;
+        }
+        Thomas(n, a.data(), b.data(), c.data(), d, x, work);
+    }
+
+    // Peaceman-Rachford ADI: x-implicit sweep (explicit in y) into scratch,
+    // then y-implicit sweep (explicit in x) into deltas += v - field.
+    // include/Hal/Pde/ADIequations.h:306-378; src/Pde/ADIequations.cpp:19-47;
+    // ImplicitBoundarySource include/Hal/Pde/ADIequations.h:198-220.
+    void Adi2D(const double *field, double *deltas, double rate, int32_t X, int32_t Y, bool wrapX, bool wrapY,
+               const double *bcXlo, const double *bcXhi, const double *bcYlo, const double *bcYhi)
+    {
+        bool hasBc = (bcXlo != nullptr);
+        std::vector<double> scratch(static_cast<size_t>(X) * Y);
+        std::vector<double> work;
+        std::vector<double> d(static_cast<size_t>(X > Y ? X : Y));
+        std::vector<double> v(d.size());
+        for (int32_t y = 0; y < Y; y++)
+        {
+            for (int32_t i = 0; i < X; i++)
+            {
+                double c = field[static_cast<size_t>(i) * Y + y];
+                double s = DeltaY(field, c, i, y + 1, X, Y, wrapY, bcYlo, bcYhi);
+                s += DeltaY(field, c, i, y - 1, X, Y, wrapY, bcYlo, bcYhi);
+                double rhs = c + (s * rate) / 2;
+                if (hasBc && !wrapX)
+                {
+                    double src = 0.0;
+                    if (i == 0)
+                    {
+                        src += bcXlo[y];
+                    }
+                    if (i == X - 1)
+                    {
+                        src += bcXhi[y];
+                    }
+                    rhs += src * 2.0 * rate / 2;
+                }
+                d[i] = rhs;
+            }
+            SolveLine(X, rate, wrapX, hasBc, d.data(), v.data(), work);
+            for (int32_t i = 0; i < X; i++)
+            {
+                scratch[static_cast<size_t>(i) * Y + y] = v[i];
+            }
+        }
+        for (int32_t x = 0; x < X; x++)
+        {
+            for (int32_t i = 0; i < Y; i++)
+            {
+                double c = scratch[static_cast<size_t>(x) * Y + i];
+                double s = DeltaX(scratch.data(), c, x + 1, i, X, Y, wrapX, bcXlo, bcXhi);
+                s += DeltaX(scratch.data(), c, x - 1, i, X, Y, wrapX, bcXlo, bcXhi);
+                double rhs = c + (s * rate) / 2;
+                if (hasBc && !wrapY)
+                {
+                    double src = 0.0;
+                    if (i == 0)
+                    {
+                        src += bcYlo[x];
+                    }
+                    if (i == Y - 1)
+                    {
+                        src += bcYhi[x];
+                    }
+                    rhs += src * 2.0 * rate / 2;
+                }
+                d[i] = rhs;
+            }
+            SolveLine(Y, rate, wrapY, hasBc, d.data(), v.data(), work);
+            for (int32_t i = 0; i < Y; i++)
+            {
+                size_t k = static_cast<size_t>(x) * Y + i;
+                deltas[k] += v[i] - field[k];
+            }
+        }
+    }
+
+    // xoshiro256** seeded through splitmix64; src/Rng.cpp:13-20, 37-45, 56-69,
+    // 91-96.
+    struct Xoshiro
+    {
+        uint64_t s[4];
+        static uint64_t SplitMix(uint64_t &st)
+        {
+            st += 0x9E3779B97F4A7C15ULL;
+            uint64_t z = st;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+            return z ^ (z >> 31);
+        }
+        static uint64_t Rotl(uint64_t x, int k)
+        {
+            return (x << k) | (x >> (64 - k));
+        }
+        explicit Xoshiro(uint64_t seed)
+        {
+            uint64_t st = seed;
+            for (int i = 0; i < 4; i++)
+            {
+                s[i] = SplitMix(st);
+            }
+        }
+        uint64_t Next()
+        {
+            uint64_t r = Rotl(s[1] * 5, 7) * 9;
+            uint64_t t = s[1] << 17;
+            s[2] ^= s[0];
+            s[3] ^= s[1];
+            s[1] ^= s[2];
+            s[0] ^= s[3];
+            s[2] ^= t;
+            s[3] = Rotl(s[3], 45);
+            return r;
+        }
+        double NextDouble()
+        {
+            return static_cast<double>(Next() >> 11) * (1.0 / 9007199254740992.0);
+        }
+    };
+
+    // Clamped floor cell of a coordinate; src/Agents/CpuAgentForceBackend2D.cpp:14-27.
+    inline int32_t CellCoord(double v, int32_t dim)
+    {
+        int32_t c = static_cast<int32_t>(std::floor(v));
+        if (c < 0)
+        {
+            c = 0;
+        }
+        else if (c >= dim)
+        {
+            c = dim - 1;
+        }
+        return c;
+    }
+
+    // b - a along the shorter way round a wrapped axis;
+    // src/Agents/CpuAgentForceBackend2D.cpp:30-45.
+    inline double Disp(double a, double b, int32_t dim, bool wrap)
+    {
+        double d = b - a;
+        if (wrap)
+        {
+            double dimF = static_cast<double>(dim);
+            if (std::fabs(d) > dimF * 0.5)
+            {
+                d = (a > b) ? (d + dimF) : (d - dimF);
+            }
+        }
+        return d;
+    }
+}
+
+extern "C"
+{
+    // ---- RNG -----------------------------------------------------------------
+
+    // n draws of Rng(seed).NextDouble(); src/Rng.cpp:91-96.
+    void oracle_rng_doubles(uint64_t seed, int64_t n, double *out)
+    {
+        Xoshiro r(seed);
+        for (int64_t i = 0; i < n; i++)
+        {
+            out[i] = r.NextDouble();
+        }
+    }
+
+    // n raw NextUint64 outputs; src/Rng.cpp:56-69.
+    void oracle_rng_u64(uint64_t seed, int64_t n, uint64_t *out)
+    {
+        Xoshiro r(seed);
+        for (int64_t i = 0; i < n; i++)
+        {
+            out[i] = r.Next();
+        }
+    }
+
+    // ---- explicit diffusion ----------------------------------------------------
+
+    // deltas += rate * sum of four neighbour differences;
+    // src/Pde/PDEequations.cpp:140-161 (bc arrays null) and
+    // include/Hal/Pde/PDEequations.h:264-287 (Dirichlet ghost from edge arrays:
+    // bcXlo[y] = bc(-1,y), bcXhi[y] = bc(X,y), bcYlo[x] = bc(x,-1), bcYhi[x] = bc(x,Y)).
+    void oracle_diffuse(const double *field, double *deltas, double rate, int32_t X, int32_t Y, int32_t wrapX,
+                        int32_t wrapY, const double *bcXlo, const double *bcXhi, const double *bcYlo,
+                        const double *bcYhi)
+    {
+        for (int32_t x = 0; x < X; x++)
+        {
+            for (int32_t y = 0; y < Y; y++)
+            {
+                size_t i = static_cast<size_t>(x) * Y + y;
+                double c = field[i];
+                double s = DeltaX(field, c, x + 1, y, X, Y, wrapX != 0, bcXlo, bcXhi);
+                s += DeltaX(field, c, x - 1, y, X, Y, wrapX != 0, bcXlo, bcXhi);
+                s += DeltaY(field, c, x, y + 1, X, Y, wrapY != 0, bcYlo, bcYhi);
+                s += DeltaY(field, c, x, y - 1, X, Y, wrapY != 0, bcYlo, bcYhi);
+                deltas[i] += s * rate;
+            }
+        }
+    }
+
+    // Per-cell-rate explicit diffusion: the +x/+y face uses this cell's rate
+    // (0 past a non-wrapped edge), the -x/-y face the neighbour's;
+    // src/Pde/PDEequations.cpp:163-228.
+    void oracle_diffuse_rates(const double *field, double *deltas, const double *ratesX, const double *ratesY,
+                              int32_t X, int32_t Y, int32_t wrapX, int32_t wrapY)
+    {
+        auto dispX = [&](const double *v, int32_t x, int32_t y) -> double
+        {
+            if (x >= 0 && x < X)
+            {
+                return v[static_cast<size_t>(x) * Y + y];
+            }
+            if (wrapX)
+            {
+                return v[static_cast<size_t>(WrapI(x, X)) * Y + y];
+            }
+            return 0.0;
+        };
+        auto dispY = [&](const double *v, int32_t x, int32_t y) -> double
+        {
+            if (y >= 0 && y < Y)
+            {
+                return v[static_cast<size_t>(x) * Y + y];
+            }
+            if (wrapY)
+            {
+                return v[static_cast<size_t>(x) * Y + WrapI(y, Y)];
+            }
+            return 0.0;
+        };
+        for (int32_t x = 0; x < X; x++)
+        {
+            for (int32_t y = 0; y < Y; y++)
+            {
+                size_t i = static_cast<size_t>(x) * Y + y;
+                double c = field[i];
+                double dp1x = (x == X - 1 && !wrapX) ? dispX(ratesX, x + 1, y) : ratesX[i];
+                double dm1x = dispX(ratesX, x - 1, y);
+                double dp1y = (y == Y - 1 && !wrapY) ? dispY(ratesY, x, y + 1) : ratesY[i];
+                double dm1y = dispY(ratesY, x, y - 1);
+                double s = DeltaX(field, c, x + 1, y, X, Y, wrapX != 0, nullptr, nullptr) * dp1x;
+                s += DeltaX(field, c, x - 1, y, X, Y, wrapX != 0, nullptr, nullptr) * dm1x;
+                s += DeltaY(field, c, x, y + 1, X, Y, wrapY != 0, nullptr, nullptr) * dp1y;
+                s += DeltaY(field, c, x, y - 1, X, Y, wrapY != 0, nullptr, nullptr) * dm1y;
+                deltas[i] += s;
+            }
+        }
+    }
+
+    // `ticks` model ticks of explicit diffusion + decay + point sources, each
+    // committed like PDEGrid2D::Update: deltas = Diffuse(u) ; deltas += -k*u
+    // (MulAll(-k)) ; deltas[src] += s (Add) ; u += deltas.
+    // include/Hal/Grids/PDEGrid2D.h:103-145, 178-186, 212-218.
+    void oracle_explicit_ticks(double *field, int32_t ticks, double rate, double decay, int32_t X, int32_t Y,
+                               int32_t wrapX, int32_t wrapY, int32_t nSrc, const int64_t *srcIdx,
+                               const double *srcVal)
+    {
+        size_t n = static_cast<size_t>(X) * Y;
+        std::vector<double> deltas(n, 0.0);
+        for (int32_t t = 0; t < ticks; t++)
+        {
+            oracle_diffuse(field, deltas.data(), rate, X, Y, wrapX, wrapY, nullptr, nullptr, nullptr, nullptr);
+            if (decay != 0.0)
+            {
+                for (size_t i = 0; i < n; i++)
+                {
+                    deltas[i] += field[i] * (-decay);
+                }
+            }
+            for (int32_t s = 0; s < nSrc; s++)
+            {
+                deltas[static_cast<size_t>(srcIdx[s])] += srcVal[s];
+            }
+            for (size_t i = 0; i < n; i++)
+            {
+                field[i] += deltas[i];
+                deltas[i] = 0.0;
+            }
+        }
+    }
+
+    // ---- tridiagonal and ADI ------------------------------------------------
+
+    void oracle_thomas(int32_t n, const double *a, const double *b, const double *c, const double *d, double *x)
+    {
+        std::vector<double> work;
+        Thomas(n, a, b, c, d, x, work);
+    }
+
+    void oracle_thomas_periodic(int32_t n, const double *a, const double *b, const double *c, const double *d,
+                                double *x)
+    {
+        std::vector<double> work;
+        ThomasPeriodic(n, a, b, c, d, x, work);
+    }
+
+    // One ADI line solve with the reference's coefficients (ADISolveRow).
+    void oracle_adi_line(int32_t n, double rate, int32_t wrap, int32_t hasBc, const double *d, double *x)
+    {
+        std::vector<double> work;
+        SolveLine(n, rate, wrap != 0, hasBc != 0, d, x, work);
+    }
+
+    // deltas += ADI(u) - u.  Null bc arrays: zero flux / periodic
+    // (CpuAdiDiffusionBackend2D::DiffuseADI, src/Pde/CpuAdiDiffusionBackend2D.cpp:17-32);
+    // non-null: Dirichlet ghost values as in oracle_diffuse.
+    void oracle_adi(const double *field, double *deltas, double rate, int32_t X, int32_t Y, int32_t wrapX,
+                    int32_t wrapY, const double *bcXlo, const double *bcXhi, const double *bcYlo,
+                    const double *bcYhi)
+    {
+        Adi2D(field, deltas, rate, X, Y, wrapX != 0, wrapY != 0, bcXlo, bcXhi, bcYlo, bcYhi);
+    }
+
+    // `ticks` model ticks of ADI + decay + point sources, committed each tick
+    // (PDEGrid2D::DiffusionADI / MulAll / Add / Update).
+    void oracle_adi_ticks(double *field, int32_t ticks, double rate, double decay, int32_t X, int32_t Y,
+                          int32_t wrapX, int32_t wrapY, const double *bcXlo, const double *bcXhi,
+                          const double *bcYlo, const double *bcYhi, int32_t nSrc, const int64_t *srcIdx,
+                          const double *srcVal)
+    {
+        size_t n = static_cast<size_t>(X) * Y;
+        std::vector<double> deltas(n, 0.0);
+        for (int32_t t = 0; t < ticks; t++)
+        {
+            Adi2D(field, deltas.data(), rate, X, Y, wrapX != 0, wrapY != 0, bcXlo, bcXhi, bcYlo, bcYhi);
+            if (decay != 0.0)
+            {
+                for (size_t i = 0; i < n; i++)
+                {
+                    deltas[i] += field[i] * (-decay);
+                }
+            }
+            for (int32_t s = 0; s < nSrc; s++)
+            {
+                deltas[static_cast<size_t>(srcIdx[s])] += srcVal[s];
+            }
+            for (size_t i = 0; i < n; i++)
+            {
+                field[i] += deltas[i];
+                deltas[i] = 0.0;
+            }
+        }
+    }
+
+    // ---- agents -------------------------------------------------------------
+
+    // Passes 1-3 of CpuAgentForceBackend2D::AccumulateForces: exclusive cell
+    // offsets[C+1] and the stable (ascending agent index) bucket order.
+    // src/Agents/CpuAgentForceBackend2D.cpp:85-109.
+    void oracle_agent_bins(const double *pos, int32_t N, int32_t X, int32_t Y, int32_t *offsets, int32_t *bucket)
+    {
+        int64_t C = static_cast<int64_t>(X) * Y;
+        std::memset(offsets, 0, sizeof(int32_t) * static_cast<size_t>(C + 1));
+        for (int32_t i = 0; i < N; i++)
+        {
+            int32_t cx = CellCoord(pos[2 * static_cast<size_t>(i)], X);
+            int32_t cy = CellCoord(pos[2 * static_cast<size_t>(i) + 1], Y);
+            offsets[static_cast<size_t>(cx) * Y + cy + 1]++;
+        }
+        for (int64_t c = 0; c < C; c++)
+        {
+            offsets[c + 1] += offsets[c];
+        }
+        std::vector<int32_t> cursor(offsets, offsets + C);
+        for (int32_t i = 0; i < N; i++)
+        {
+            int32_t cx = CellCoord(pos[2 * static_cast<size_t>(i)], X);
+            int32_t cy = CellCoord(pos[2 * static_cast<size_t>(i) + 1], Y);
+            size_t cell = static_cast<size_t>(cx) * Y + cy;
+            bucket[cursor[cell]++] = i;
+        }
+    }
+
+    // Pass 4: per agent, cells [floor(p-R), ceil(p+R)) per axis in bucket
+    // order, linear spring k*((ri+rj)-d) for 0 < d < R, v -= (disp/d)*F.
+    // Overwrites vel (2N).  src/Agents/CpuAgentForceBackend2D.cpp:111-175.
+    void oracle_agent_forces(const double *pos, const double *radii, int32_t N, double *vel, double R, double k,
+                             int32_t X, int32_t Y, int32_t wrapX, int32_t wrapY)
+    {
+        int64_t C = static_cast<int64_t>(X) * Y;
+        std::vector<int32_t> offsets(static_cast<size_t>(C + 1));
+        std::vector<int32_t> bucket(static_cast<size_t>(N > 0 ? N : 1));
+        oracle_agent_bins(pos, N, X, Y, offsets.data(), bucket.data());
+        for (int32_t i = 0; i < N; i++)
+        {
+            double px = pos[2 * static_cast<size_t>(i)];
+            double py = pos[2 * static_cast<size_t>(i) + 1];
+            double rad = radii[i];
+            int32_t x0 = static_cast<int32_t>(std::floor(px - R));
+            int32_t x1 = static_cast<int32_t>(std::ceil(px + R));
+            int32_t y0 = static_cast<int32_t>(std::floor(py - R));
+            int32_t y1 = static_cast<int32_t>(std::ceil(py + R));
+            double vx = 0.0;
+            double vy = 0.0;
+            for (int32_t xs = x0; xs < x1; xs++)
+            {
+                int32_t cx = xs;
+                bool inX = (cx >= 0) && (cx < X);
+                if (!inX)
+                {
+                    if (!wrapX)
+                    {
+                        continue;
+                    }
+                    cx = WrapI(cx, X);
+                }
+                for (int32_t ys = y0; ys < y1; ys++)
+                {
+                    int32_t cy = ys;
+                    bool inY = (cy >= 0) && (cy < Y);
+                    if (!inY)
+                    {
+                        if (!wrapY)
+                        {
+                            continue;
+                        }
+                        cy = WrapI(cy, Y);
+                    }
+                    size_t cell = static_cast<size_t>(cx) * Y + cy;
+                    for (int32_t q = offsets[cell]; q < offsets[cell + 1]; q++)
+                    {
+                        int32_t j = bucket[q];
+                        if (j == i)
+                        {
+                            continue;
+                        }
+                        double dx = Disp(px, pos[2 * static_cast<size_t>(j)], X, wrapX != 0);
+                        double dy = Disp(py, pos[2 * static_cast<size_t>(j) + 1], Y, wrapY != 0);
+                        double dist = std::sqrt((dx * dx) + (dy * dy));
+                        if ((dist < R) && (dist > 0.0))
+                        {
+                            double force = k * ((rad + radii[j]) - dist);
+                            vx -= (dx / dist) * force;
+                            vy -= (dy / dist) * force;
+                        }
+                    }
+                }
+            }
+            vel[2 * static_cast<size_t>(i)] = vx;
+            vel[2 * static_cast<size_t>(i) + 1] = vy;
+        }
+    }
+
+    // Move with AgentPT2D::MoveSafePT semantics (include/Hal/Agents/AgentPT2D.h:157-186):
+    // target p + dt*v; if the target cell is off the grid, a wrapped axis wraps
+    // (fmod, src/MathUtil.cpp:48-53) and a non-wrapped axis keeps its old
+    // coordinate.
+    void oracle_agent_move(double *pos, const double *vel, int32_t N, double dt, int32_t X, int32_t Y,
+                           int32_t wrapX, int32_t wrapY)
+    {
+        for (int32_t i = 0; i < N; i++)
+        {
+            double ox = pos[2 * static_cast<size_t>(i)];
+            double oy = pos[2 * static_cast<size_t>(i) + 1];
+            double nx = ox + dt * vel[2 * static_cast<size_t>(i)];
+            double ny = oy + dt * vel[2 * static_cast<size_t>(i) + 1];
+            int32_t fx = static_cast<int32_t>(std::floor(nx));
+            int32_t fy = static_cast<int32_t>(std::floor(ny));
+            bool in = (fx >= 0) && (fx < X) && (fy >= 0) && (fy < Y);
+            if (!in)
+            {
+                if (wrapX)
+                {
+                    double m = std::fmod(nx, static_cast<double>(X));
+                    nx = (m < 0) ? (X + m) : m;
+                }
+                else if (!((nx >= 0) && (nx < X)))
+                {
+                    nx = ox;
+                }
+                if (wrapY)
+                {
+                    double m = std::fmod(ny, static_cast<double>(Y));
+                    ny = (m < 0) ? (Y + m) : m;
+                }
+                else if (!((ny >= 0) && (ny < Y)))
+                {
+                    ny = oy;
+                }
+            }
+            pos[2 * static_cast<size_t>(i)] = nx;
+            pos[2 * static_cast<size_t>(i) + 1] = ny;
+        }
+    }
+
+    // `ticks` ticks of forces + move.
+    void oracle_agent_ticks(double *pos, const double *radii, int32_t N, int32_t ticks, double R, double k,
+                            double dt, int32_t X, int32_t Y, int32_t wrapX, int32_t wrapY)
+    {
+        std::vector<double> vel(2 * static_cast<size_t>(N > 0 ? N : 1));
+        for (int32_t t = 0; t < ticks; t++)
+        {
+            oracle_agent_forces(pos, radii, N, vel.data(), R, k, X, Y, wrapX, wrapY);
+            oracle_agent_move(pos, vel.data(), N, dt, X, Y, wrapX, wrapY);
+        }
+    }
+}

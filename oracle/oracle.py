@@ -1,0 +1,196 @@
+"""Python handles on the CPU checkers -- TEST INFRASTRUCTURE ONLY.
+
+``Oracle`` wraps ``oracle/liboracle.so`` (the FP64 restatement in
+``hal_oracle.cpp``); ``Reference`` wraps ``oracle/_ref/libhalref.so`` (the
+HAL-CPP reference CPU backends compiled from their own sources).  Only tests,
+the smoke check and the CPU-baseline leg of the benchmark import this module.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libhalref.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_lp = ctypes.POINTER(ctypes.c_int64)
+_up = ctypes.POINTER(ctypes.c_uint64)
+_i32 = ctypes.c_int32
+_f64 = ctypes.c_double
+
+
+def _p(a, t=_dp):
+    if a is None:
+        return None
+    return a.ctypes.data_as(t)
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _edges(bc, X, Y):
+    """Edge arrays (bc(-1,y), bc(X,y), bc(x,-1), bc(x,Y)) from None, a
+    constant or a 4-tuple of arrays."""
+    if bc is None:
+        return None, None, None, None
+    if np.isscalar(bc):
+        v = float(bc)
+        return (np.full(Y, v), np.full(Y, v), np.full(X, v), np.full(X, v))
+    xlo, xhi, ylo, yhi = bc
+    return _f(xlo), _f(xhi), _f(ylo), _f(yhi)
+
+
+class _Lib:
+    def __init__(self, path, prefix):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path + " is not built (run `make -C oracle` / `make -C oracle ref`)")
+        self.lib = ctypes.CDLL(path)
+        self.prefix = prefix
+
+    def fn(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def rng_doubles(self, seed, n):
+        out = np.empty(n, np.float64)
+        f = self.fn("rng_doubles")
+        f.argtypes = [ctypes.c_uint64, ctypes.c_int64, _dp]
+        f(seed, n, _p(out))
+        return out
+
+    def rng_u64(self, seed, n):
+        out = np.empty(n, np.uint64)
+        f = self.fn("rng_u64")
+        f.argtypes = [ctypes.c_uint64, ctypes.c_int64, _up]
+        f(seed, n, _p(out, _up))
+        return out
+
+    def diffuse(self, field, deltas, rate, X, Y, wrapX, wrapY, bc=None):
+        field = _f(field)
+        deltas = _f(deltas).copy()
+        e = _edges(bc, X, Y)
+        f = self.fn("diffuse")
+        f.argtypes = [_dp, _dp, _f64, _i32, _i32, _i32, _i32, _dp, _dp, _dp, _dp]
+        f(_p(field), _p(deltas), rate, X, Y, int(wrapX), int(wrapY), *[_p(a) for a in e])
+        return deltas
+
+    def diffuse_rates(self, field, deltas, ratesX, ratesY, X, Y, wrapX, wrapY):
+        field, ratesX, ratesY = _f(field), _f(ratesX), _f(ratesY)
+        deltas = _f(deltas).copy()
+        f = self.fn("diffuse_rates")
+        f.argtypes = [_dp, _dp, _dp, _dp, _i32, _i32, _i32, _i32]
+        f(_p(field), _p(deltas), _p(ratesX), _p(ratesY), X, Y, int(wrapX), int(wrapY))
+        return deltas
+
+    def adi(self, field, deltas, rate, X, Y, wrapX, wrapY, bc=None):
+        field = _f(field)
+        deltas = _f(deltas).copy()
+        e = _edges(bc, X, Y)
+        f = self.fn("adi")
+        f.argtypes = [_dp, _dp, _f64, _i32, _i32, _i32, _i32, _dp, _dp, _dp, _dp]
+        f(_p(field), _p(deltas), rate, X, Y, int(wrapX), int(wrapY), *[_p(a) for a in e])
+        return deltas
+
+    def agent_forces(self, pos, radii, R, k, X, Y, wrapX, wrapY):
+        pos, radii = _f(pos), _f(radii)
+        n = radii.shape[0]
+        vel = np.zeros(2 * n, np.float64)
+        f = self.fn("agent_forces")
+        f.argtypes = [_dp, _dp, _i32, _dp, _f64, _f64, _i32, _i32, _i32, _i32]
+        f(_p(pos), _p(radii), n, _p(vel), R, k, X, Y, int(wrapX), int(wrapY))
+        return vel
+
+
+class Oracle(_Lib):
+    """FP64 restatement (hal_oracle.cpp)."""
+
+    def __init__(self, path=ORACLE_SO):
+        super().__init__(path, "oracle_")
+
+    def explicit_ticks(self, field, ticks, rate, decay, X, Y, wrapX, wrapY, src_idx=(), src_val=()):
+        field = _f(field).copy()
+        si = np.ascontiguousarray(src_idx, np.int64)
+        sv = _f(src_val)
+        f = self.fn("explicit_ticks")
+        f.argtypes = [_dp, _i32, _f64, _f64, _i32, _i32, _i32, _i32, _i32, _lp, _dp]
+        f(_p(field), ticks, rate, decay, X, Y, int(wrapX), int(wrapY), len(si), _p(si, _lp), _p(sv))
+        return field
+
+    def adi_ticks(self, field, ticks, rate, decay, X, Y, wrapX, wrapY, bc=None, src_idx=(), src_val=()):
+        field = _f(field).copy()
+        e = _edges(bc, X, Y)
+        si = np.ascontiguousarray(src_idx, np.int64)
+        sv = _f(src_val)
+        f = self.fn("adi_ticks")
+        f.argtypes = [_dp, _i32, _f64, _f64, _i32, _i32, _i32, _i32, _dp, _dp, _dp, _dp, _i32, _lp, _dp]
+        f(_p(field), ticks, rate, decay, X, Y, int(wrapX), int(wrapY), *[_p(a) for a in e],
+          len(si), _p(si, _lp), _p(sv))
+        return field
+
+    def thomas(self, a, b, c, d, periodic=False):
+        a, b, c, d = _f(a), _f(b), _f(c), _f(d)
+        x = np.empty_like(d)
+        f = self.fn("thomas_periodic" if periodic else "thomas")
+        f.argtypes = [_i32, _dp, _dp, _dp, _dp, _dp]
+        f(len(d), _p(a), _p(b), _p(c), _p(d), _p(x))
+        return x
+
+    def adi_line(self, d, rate, wrap, has_bc):
+        d = _f(d)
+        x = np.empty_like(d)
+        f = self.fn("adi_line")
+        f.argtypes = [_i32, _f64, _i32, _i32, _dp, _dp]
+        f(len(d), rate, int(wrap), int(has_bc), _p(d), _p(x))
+        return x
+
+    def agent_bins(self, pos, X, Y):
+        pos = _f(pos)
+        n = pos.shape[0] // 2
+        offsets = np.zeros(X * Y + 1, np.int32)
+        bucket = np.zeros(max(n, 1), np.int32)
+        f = self.fn("agent_bins")
+        f.argtypes = [_dp, _i32, _i32, _i32, _ip, _ip]
+        f(_p(pos), n, X, Y, _p(offsets, _ip), _p(bucket, _ip))
+        return offsets, bucket[:n]
+
+    def agent_move(self, pos, vel, dt, X, Y, wrapX, wrapY):
+        pos = _f(pos).copy()
+        vel = _f(vel)
+        f = self.fn("agent_move")
+        f.argtypes = [_dp, _dp, _i32, _f64, _i32, _i32, _i32, _i32]
+        f(_p(pos), _p(vel), pos.shape[0] // 2, dt, X, Y, int(wrapX), int(wrapY))
+        return pos
+
+    def agent_ticks(self, pos, radii, ticks, R, k, dt, X, Y, wrapX, wrapY):
+        pos = _f(pos).copy()
+        radii = _f(radii)
+        f = self.fn("agent_ticks")
+        f.argtypes = [_dp, _dp, _i32, _i32, _f64, _f64, _f64, _i32, _i32, _i32, _i32]
+        f(_p(pos), _p(radii), radii.shape[0], ticks, R, k, dt, X, Y, int(wrapX), int(wrapY))
+        return pos
+
+
+class Reference(_Lib):
+    """The compiled HAL-CPP reference (oracle/_ref/libhalref.so)."""
+
+    def __init__(self, path=REF_SO):
+        super().__init__(path, "ref_")
+
+    def grid_ticks(self, field, ticks, adi, rate, decay, X, Y, wrapX, wrapY, bc_value=None,
+                   src_idx=(), src_val=()):
+        field = _f(field).copy()
+        si = np.ascontiguousarray(src_idx, np.int64)
+        sv = _f(src_val)
+        f = self.fn("grid_ticks")
+        f.argtypes = [_dp, _i32, _i32, _f64, _f64, _i32, _i32, _i32, _i32, _i32, _f64, _i32, _lp, _dp]
+        f(_p(field), ticks, int(adi), rate, decay, X, Y, int(wrapX), int(wrapY),
+          0 if bc_value is None else 1, 0.0 if bc_value is None else float(bc_value),
+          len(si), _p(si, _lp), _p(sv))
+        return field
+
+
+def available_reference():
+    return os.path.exists(REF_SO)

@@ -1,0 +1,173 @@
+// ref_driver.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// A thin extern "C" shim over the *compiled HAL-CPP reference* CPU backends
+// (built from the reference's own sources by oracle/Makefile into
+// oracle/_ref/libhalref.so).  It is used to write golden vectors and to
+// cross-check the FP64 restatement in hal_oracle.cpp; nothing in the product
+// links it.  Each entry point calls the reference API it names.
+#include "Hal/Agents/CpuAgentForceBackend2D.h"
+#include "Hal/Grids/PDEGrid2D.h"
+#include "Hal/Pde/ADIequations.h"
+#include "Hal/Pde/CpuAdiDiffusionBackend2D.h"
+#include "Hal/Pde/CpuDiffusionBackend2D.h"
+#include "Hal/Pde/PDEequations.h"
+#include "Hal/Rng.h"
+
+#include <cstdint>
+#include <vector>
+
+namespace
+{
+    struct EdgeBc
+    {
+        int32_t X;
+        int32_t Y;
+        const double *xlo;
+        const double *xhi;
+        const double *ylo;
+        const double *yhi;
+        double operator()(int32_t x, int32_t y) const
+        {
+            if (x < 0)
+            {
+                return xlo[y];
+            }
+            if (x >= X)
+            {
+                return xhi[y];
+            }
+            if (y < 0)
+            {
+                return ylo[x];
+            }
+            return yhi[x];
+        }
+    };
+}
+
+extern "C"
+{
+    // Hal::Rng(seed).NextDouble() stream.
+    void ref_rng_doubles(uint64_t seed, int64_t n, double *out)
+    {
+        Hal::Rng rng(seed);
+        for (int64_t i = 0; i < n; i++)
+        {
+            out[i] = rng.NextDouble();
+        }
+    }
+
+    void ref_rng_u64(uint64_t seed, int64_t n, uint64_t *out)
+    {
+        Hal::Rng rng(seed);
+        for (int64_t i = 0; i < n; i++)
+        {
+            out[i] = rng.NextUint64();
+        }
+    }
+
+    // Hal::CpuDiffusionBackend2D::Diffuse, or Internal::Diffusion2<BCFn> when
+    // edge arrays are given.
+    void ref_diffuse(const double *field, double *deltas, double rate, int32_t X, int32_t Y, int32_t wrapX,
+                     int32_t wrapY, const double *bcXlo, const double *bcXhi, const double *bcYlo,
+                     const double *bcYhi)
+    {
+        size_t n = static_cast<size_t>(X) * Y;
+        if (bcXlo == nullptr)
+        {
+            std::vector<double> f(field, field + n);
+            std::vector<double> d(deltas, deltas + n);
+            Hal::CpuDiffusionBackend2D backend;
+            backend.Diffuse(f, d, rate, X, Y, wrapX != 0, wrapY != 0);
+            std::copy(d.begin(), d.end(), deltas);
+            return;
+        }
+        EdgeBc bc{X, Y, bcXlo, bcXhi, bcYlo, bcYhi};
+        Hal::Internal::Diffusion2(field, deltas, rate, X, Y, wrapX != 0, wrapY != 0, bc);
+    }
+
+    void ref_diffuse_rates(const double *field, double *deltas, const double *ratesX, const double *ratesY,
+                           int32_t X, int32_t Y, int32_t wrapX, int32_t wrapY)
+    {
+        Hal::Internal::Diffusion2(field, deltas, ratesX, ratesY, X, Y, wrapX != 0, wrapY != 0);
+    }
+
+    // Hal::CpuAdiDiffusionBackend2D::DiffuseADI, or Internal::Diffusion2DADI<BCFn>.
+    void ref_adi(const double *field, double *deltas, double rate, int32_t X, int32_t Y, int32_t wrapX,
+                 int32_t wrapY, const double *bcXlo, const double *bcXhi, const double *bcYlo, const double *bcYhi)
+    {
+        size_t n = static_cast<size_t>(X) * Y;
+        if (bcXlo == nullptr)
+        {
+            std::vector<double> f(field, field + n);
+            std::vector<double> d(deltas, deltas + n);
+            Hal::CpuAdiDiffusionBackend2D backend;
+            backend.DiffuseADI(f, d, rate, X, Y, wrapX != 0, wrapY != 0);
+            std::copy(d.begin(), d.end(), deltas);
+            return;
+        }
+        EdgeBc bc{X, Y, bcXlo, bcXhi, bcYlo, bcYhi};
+        std::vector<double> scratch(n, 0.0);
+        Hal::TdmaSolver tdma(X > Y ? X : Y);
+        Hal::Internal::Diffusion2DADI(field, scratch.data(), deltas, rate, X, Y, wrapX != 0, wrapY != 0, bc, tdma);
+    }
+
+    // Full model ticks through Hal::PDEGrid2D: Diffusion or DiffusionADI
+    // (adi != 0; constant Dirichlet value when hasBc), MulAll(-decay),
+    // Add(src), Update().
+    void ref_grid_ticks(double *field, int32_t ticks, int32_t adi, double rate, double decay, int32_t X, int32_t Y,
+                        int32_t wrapX, int32_t wrapY, int32_t hasBc, double bcValue, int32_t nSrc,
+                        const int64_t *srcIdx, const double *srcVal)
+    {
+        Hal::PDEGrid2D grid(X, Y, wrapX != 0, wrapY != 0);
+        int32_t n = X * Y;
+        for (int32_t i = 0; i < n; i++)
+        {
+            grid.Set(i, field[i]);
+        }
+        grid.Update();
+        for (int32_t t = 0; t < ticks; t++)
+        {
+            if (adi != 0)
+            {
+                if (hasBc != 0)
+                {
+                    grid.DiffusionADI(rate, bcValue);
+                }
+                else
+                {
+                    grid.DiffusionADI(rate);
+                }
+            }
+            else
+            {
+                grid.Diffusion(rate);
+            }
+            if (decay != 0.0)
+            {
+                grid.MulAll(-decay);
+            }
+            for (int32_t s = 0; s < nSrc; s++)
+            {
+                grid.Add(static_cast<int32_t>(srcIdx[s]), srcVal[s]);
+            }
+            grid.Update();
+        }
+        for (int32_t i = 0; i < n; i++)
+        {
+            field[i] = grid.Get(i);
+        }
+    }
+
+    // Hal::CpuAgentForceBackend2D::AccumulateForces.
+    void ref_agent_forces(const double *pos, const double *radii, int32_t N, double *vel, double R, double k,
+                          int32_t X, int32_t Y, int32_t wrapX, int32_t wrapY)
+    {
+        std::vector<double> p(pos, pos + 2 * static_cast<size_t>(N));
+        std::vector<double> r(radii, radii + N);
+        std::vector<double> v;
+        Hal::CpuAgentForceBackend2D backend;
+        backend.AccumulateForces(p, r, v, R, k, X, Y, wrapX != 0, wrapY != 0);
+        std::copy(v.begin(), v.end(), vel);
+    }
+}
