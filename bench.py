@@ -1,0 +1,143 @@
+#!/usr/bin/env python3
+"""Headline benchmark: implicit ADI diffusion on a 4096 x 4096 FP32 grid.
+
+Workload: D = 5.0 (rate 5 at dt = dx = 1), Dirichlet boundary value 0, 64
+seeded Gaussian blobs (sigma 20) as the initial field (workloads.adi_blobs).
+One step = one ADI tick (x-implicit sweep then y-implicit sweep) over the
+whole grid, inputs resident in HBM.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps 200] [--warmup 20]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hal-cpp_amd"))
+
+HBM_PEAK_GBS = 8000.0            # MI355X HBM3E
+BYTES_PER_CELL_TICK = 16         # read u + write u* (x sweep), read u* + write u' (y sweep)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(field, n, rate, ticks):
+    """The compiled HAL-CPP reference (PDEGrid2D::DiffusionADI + Update, one
+    thread, -O2) timed on this host on `ticks` ticks of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Reference, available_reference
+    import numpy as np
+    if not available_reference():
+        return None
+    secs, _ = Reference().adi_ticks_timed(field.astype(np.float64), 1, ticks, rate, n, n, 0.0)
+    return {"value": round(n * n * ticks / secs / 1e9, 6), "unit": "Gcell-updates/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"{ticks} ticks (after 1 warm-up tick) of the same 4096^2 Dirichlet ADI workload, "
+                      f"{secs:.2f} s; HAL-CPP PDEGrid2D::DiffusionADI compiled -O2, single-threaded "
+                      f"(the reference CPU path has no threading); host {cpu_model()}, nproc {os.cpu_count()}"}
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the strip kernel measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), as
+    recorded under profiles/."""
+    p = os.path.join(ROOT, "profiles", "adi_strip_tick_4096_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-ticks", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this backend runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import halhip
+    import workloads
+
+    w = workloads.adi_blobs()
+    n = w["X"]
+    cells = n * n
+    ctx = halhip.Context(0)
+    plan = halhip.Adi2D(ctx, n, n)
+    plan.upload(w["field"])
+
+    plan.run_steps(args.warmup, w["rate"], 0.0, False, False, bc=w["bc"])
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    plan.run_steps(args.steps, w["rate"], 0.0, False, False, bc=w["bc"], asynchronous=True)
+    event_ms = plan.sync()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+
+    # Per-kernel device time (HIP events around each launch on the plan's
+    # stream), in a separate profiled pass of the same ticks.
+    plan.set_profiling(True)
+    prof_ticks = min(args.steps, 50)
+    plan.run_steps(prof_ticks, w["rate"], 0.0, False, False, bc=w["bc"])
+    prof = plan.profile()
+    plan.set_profiling(False)
+    dom = "adi_strip_tick"
+    dom_ms = prof[dom][0] / prof[dom][1]
+    achieved = BYTES_PER_CELL_TICK * cells / (dom_ms * 1e-3) / 1e9
+    traffic = traffic_from_profile()
+
+    base = None
+    if not args.no_cpu_baseline:
+        base = cpu_baseline(w["field"], n, w["rate"], args.cpu_ticks)
+
+    value = cells * args.steps / wall / 1e9
+    line = {
+        "metric": "Gcell-updates/s (ADI 4096^2 FP32)",
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: 64 Gaussian blobs (sigma 20, amplitude U[0.5,1]) at positions from Hal::Rng(42)",
+        "config": {"workload": "ADI 4096x4096, D=5, dt=1, Dirichlet 0, 200 ticks", "grid": [n, n],
+                   "bc": "dirichlet-0", "rate": w["rate"], "parallelism": "single GPU"},
+        "hbm_frac_wall": round(value * BYTES_PER_CELL_TICK / HBM_PEAK_GBS, 4),
+        "event_ms_per_step": round(event_ms / args.steps, 5),
+        "kernels_ms_per_launch": {k: round(v[0] / v[1], 5) for k, v in prof.items()},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": dom, "algorithmic_bytes_per_launch": BYTES_PER_CELL_TICK * cells},
+        "cpu_baseline": base,
+    }
+    if base is not None:
+        line["speedup_vs_cpu"] = round(value / base["value"], 1)
+    print(json.dumps(line), flush=True)
+    plan.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -289,6 +289,17 @@ extern "C"
         return Guard([&] { delete ctx; });
     }
 
+    int halhip_context_synchronize(halhip_context *ctx)
+    {
+        return Guard(
+            [&]
+            {
+                Require(ctx != nullptr, "halhip_context_synchronize: null context");
+                Bind(ctx);
+                Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+            });
+    }
+
     int halhip_context_fail_next_allocs(halhip_context *ctx, int32_t count)
     {
         return Guard(

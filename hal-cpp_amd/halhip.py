@@ -58,6 +58,7 @@ _SIGS = {
     "halhip_context_create": [_i32, ctypes.POINTER(_vp)],
     "halhip_context_destroy": [_vp],
     "halhip_context_fail_next_allocs": [_vp, _i32],
+    "halhip_context_synchronize": [_vp],
     "halhip_rng_doubles": [ctypes.c_uint64, _i64, _dp],
     "halhip_diffusion2d_create": [_vp, _i32, _i32, ctypes.POINTER(_vp)],
     "halhip_diffusion2d_destroy": [_vp],
@@ -172,6 +173,9 @@ class Context:
 
     def fail_next_allocs(self, count):
         _check(load().halhip_context_fail_next_allocs(self.h, count))
+
+    def synchronize(self):
+        _check(load().halhip_context_synchronize(self.h))
 
     def close(self):
         if self.h:

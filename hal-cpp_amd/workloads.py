@@ -1,0 +1,46 @@
+"""Seeded synthetic inputs of the benchmark workloads.
+
+Every workload draws from Hal::Rng(seed).NextDouble() (src/Rng.cpp:91-96,
+reproduced by halhip_rng_doubles) in a fixed order, evaluates in FP64 and
+rounds to FP32 once; the FP64 checkers receive the same FP32 values widened
+back, so both sides start from identical inputs.
+"""
+import numpy as np
+
+import halhip
+
+
+def explicit_256(seed=42, n=256):
+    """Plain explicit diffusion: n x n field of U[0,1) in flat index order,
+    point source 1.0 at the centre cell, added every tick (PDEGrid2D::Add)."""
+    field = halhip.rng_doubles(seed, n * n).astype(np.float32)
+    c = (n // 2) * n + (n // 2)
+    return dict(X=n, Y=n, field=field, rate=0.2, decay=1e-3, wrapX=True, wrapY=True,
+                src_idx=np.array([c], np.int64), src_val=np.array([1.0], np.float32))
+
+
+def adi_blobs(n=4096, blobs=64, sigma=20.0, seed=42):
+    """ADI workload: sum of `blobs` Gaussian blobs.  Per blob, in order from
+    Rng(seed): cx = n*U, cy = n*U, A = 0.5 + 0.5*U; value
+    A*exp(-((x-cx)^2 + (y-cy)^2) / (2 sigma^2)) at integer cells, truncated at
+    |dx|, |dy| <= 8 sigma.  Dirichlet boundary value 0, rate 5, 200 ticks."""
+    u = halhip.rng_doubles(seed, 3 * blobs)
+    field = np.zeros((n, n), np.float64)
+    half = int(round(8 * sigma))
+    for b in range(blobs):
+        cx, cy, amp = n * u[3 * b], n * u[3 * b + 1], 0.5 + 0.5 * u[3 * b + 2]
+        x0, x1 = max(0, int(np.floor(cx)) - half), min(n, int(np.floor(cx)) + half + 1)
+        y0, y1 = max(0, int(np.floor(cy)) - half), min(n, int(np.floor(cy)) + half + 1)
+        xs = np.arange(x0, x1, dtype=np.float64)[:, None]
+        ys = np.arange(y0, y1, dtype=np.float64)[None, :]
+        field[x0:x1, y0:y1] += amp * np.exp(-((xs - cx) ** 2 + (ys - cy) ** 2) / (2 * sigma * sigma))
+    return dict(X=n, Y=n, field=field.astype(np.float32).ravel(), rate=5.0, bc=0.0, wrapX=False, wrapY=False)
+
+
+def agents_uniform(n_agents=4_000_000, dim=2048, seed=42):
+    """Agent workload: positions (dim*U, dim*U) per agent in order from
+    Rng(seed), radius 0.5, cutoff 1.0, k = 1, dt = 0.1, no wrap."""
+    u = halhip.rng_doubles(seed, 2 * n_agents)
+    pos = (u * dim).astype(np.float32)
+    rad = np.full(n_agents, 0.5, np.float32)
+    return dict(X=dim, Y=dim, pos=pos, rad=rad, R=1.0, k=1.0, dt=0.1, wrapX=False, wrapY=False)

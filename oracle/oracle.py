@@ -192,5 +192,15 @@ class Reference(_Lib):
         return field
 
 
+    def adi_ticks_timed(self, field, warmup, ticks, rate, X, Y, bc_value):
+        """Seconds for `ticks` Dirichlet ADI ticks of the compiled reference."""
+        field = _f(field).copy()
+        f = self.fn("adi_ticks_timed")
+        f.restype = ctypes.c_double
+        f.argtypes = [_dp, _i32, _i32, _f64, _i32, _i32, _f64]
+        secs = f(_p(field), warmup, ticks, rate, X, Y, bc_value)
+        return secs, field
+
+
 def available_reference():
     return os.path.exists(REF_SO)

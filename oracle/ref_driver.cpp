@@ -13,6 +13,7 @@
 #include "Hal/Pde/PDEequations.h"
 #include "Hal/Rng.h"
 
+#include <chrono>
 #include <cstdint>
 #include <vector>
 
@@ -157,6 +158,39 @@ extern "C"
         {
             field[i] = grid.Get(i);
         }
+    }
+
+    // CPU baseline timing: `warmup` untimed then `ticks` timed ticks of
+    // PDEGrid2D::DiffusionADI(rate, bcValue) + Update on one thread, clocked
+    // with std::chrono::steady_clock as examples/BackendBenchmark.cpp:50-54
+    // does.  Returns the seconds of the timed ticks; field is updated.
+    double ref_adi_ticks_timed(double *field, int32_t warmup, int32_t ticks, double rate, int32_t X, int32_t Y,
+                               double bcValue)
+    {
+        Hal::PDEGrid2D grid(X, Y, false, false);
+        int32_t n = X * Y;
+        for (int32_t i = 0; i < n; i++)
+        {
+            grid.Set(i, field[i]);
+        }
+        grid.Update();
+        for (int32_t t = 0; t < warmup; t++)
+        {
+            grid.DiffusionADI(rate, bcValue);
+            grid.Update();
+        }
+        auto t0 = std::chrono::steady_clock::now();
+        for (int32_t t = 0; t < ticks; t++)
+        {
+            grid.DiffusionADI(rate, bcValue);
+            grid.Update();
+        }
+        auto t1 = std::chrono::steady_clock::now();
+        for (int32_t i = 0; i < n; i++)
+        {
+            field[i] = grid.Get(i);
+        }
+        return std::chrono::duration<double>(t1 - t0).count();
     }
 
     // Hal::CpuAgentForceBackend2D::AccumulateForces.

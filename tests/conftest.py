@@ -1,0 +1,35 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+for sub in ("oracle", "hal-cpp_amd", "tests"):
+    p = os.path.join(ROOT, sub)
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built libhalhip.so")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle import Oracle
+    return Oracle()
+
+
+@pytest.fixture(scope="session")
+def golden():
+    return np.load(os.path.join(ROOT, "tests", "golden", "hal_reference_vectors.npz"), allow_pickle=False)
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    """A device context.  GPU tests run on a GPU host: no device is a failure."""
+    import halhip
+    c = halhip.Context(0)
+    yield c
+    c.close()

@@ -1,0 +1,36 @@
+"""Shared helpers for the test suite: golden-case decoding, error norms and
+seeded inputs of the benchmark workloads."""
+import numpy as np
+
+
+def rel_l2(got, ref):
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    den = np.linalg.norm(ref)
+    return np.linalg.norm(got - ref) / (den if den > 0 else 1.0)
+
+
+def golden_cases(g):
+    """Yields dicts for the golden diffusion / ADI cases."""
+    for k in range(int(g["ncases"][0])):
+        pre = f"case{k}_"
+        X, Y, wx, wy, bck, adi = (int(v) for v in g[pre + "meta"])
+        if bck == 0:
+            bc = None
+        elif bck == 1:
+            bc = float(g[pre + "bc"][0])
+        else:
+            bc = tuple(g[pre + s] for s in ("bc_xlo", "bc_xhi", "bc_ylo", "bc_yhi"))
+        yield dict(k=k, X=X, Y=Y, wrapX=wx, wrapY=wy, bc=bc, adi=bool(adi), rate=float(g[pre + "rate"][0]),
+                   field=g[pre + "field"], deltas_in=g[pre + "deltas_in"], deltas_out=g[pre + "deltas_out"])
+
+
+def eigen_field(nx, ny):
+    x = np.arange(nx)[:, None]
+    y = np.arange(ny)[None, :]
+    return (np.sin(2 * np.pi * x / nx) * np.sin(2 * np.pi * y / ny)).ravel()
+
+
+def lambda_cn(rate, n, k=1):
+    s = np.sin(np.pi * k / n) ** 2
+    return (1 - 2 * rate * s) / (1 + 2 * rate * s)

@@ -1,0 +1,197 @@
+"""ADI on the GPU vs the FP64 oracle, the reference's golden vectors and its
+closed forms (IAdiDiffusionBackend2D / AdiKernel2D::RunSteps /
+PDEGrid2D::DiffusionADI with Dirichlet values).
+
+Tolerances: FP32 arithmetic against FP64; the relative L2 bar is 1e-5.
+"""
+import numpy as np
+import pytest
+
+import halhip
+import workloads
+from helpers import eigen_field, golden_cases, lambda_cn, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def adi_ref_ticks(oracle, f, ticks, rate, X, Y, wx, wy, bc=None, decay=0.0, src_idx=(), src_val=()):
+    return oracle.adi_ticks(np.asarray(f, np.float64), ticks, rate, decay, X, Y, wx, wy, bc=bc,
+                            src_idx=src_idx, src_val=np.asarray(src_val, np.float64))
+
+
+def test_interface_matches_golden_cases(ctx, golden):
+    for c in golden_cases(golden):
+        if not c["adi"]:
+            continue
+        plan = halhip.Adi2D(ctx, c["X"], c["Y"])
+        got = plan.diffuse(c["field"], c["deltas_in"].copy(), c["rate"], c["wrapX"], c["wrapY"], c["bc"])
+        scale = max(1.0, np.abs(c["field"]).max())
+        np.testing.assert_allclose(got, c["deltas_out"], rtol=0, atol=3e-6 * scale,
+                                   err_msg=f"case {c['k']} X={c['X']} Y={c['Y']} wrap={c['wrapX']}{c['wrapY']}")
+        plan.close()
+
+
+def test_eigenmode_through_interface(ctx):
+    # tests/IDiffusionBackend2DTests.cpp:76-144 (1e-4 there), called twice on
+    # one plan like the reference test.
+    nx, ny, r = 12, 18, 0.3
+    f = eigen_field(nx, ny)
+    plan = halhip.Adi2D(ctx, nx, ny)
+    lam = lambda_cn(r, nx) * lambda_cn(r, ny)
+    for _ in range(2):
+        got = f + plan.diffuse(f, np.zeros_like(f), r, True, True)
+        np.testing.assert_allclose(got, lam * f, rtol=0, atol=2e-6)
+
+
+def test_zero_flux_mass_and_cell_for_cell(ctx, oracle):
+    # tests/AdiKernel2DTests.cpp:70-141: 20x14, 5 steps, r = 0.6
+    nx, ny = 20, 14
+    x = np.arange(nx)[:, None]
+    y = np.arange(ny)[None, :]
+    f = (1.0 + 0.3 * x - 0.2 * y + 0.5 * np.sin(0.7 * x * y)).ravel().astype(np.float32)
+    plan = halhip.Adi2D(ctx, nx, ny)
+    plan.upload(f)
+    plan.run_steps(5, 0.6, 0.0, False, False)
+    got = plan.download()
+    ref = adi_ref_ticks(oracle, f, 5, 0.6, nx, ny, 0, 0)
+    assert abs(got.astype(np.float64).sum() - f.astype(np.float64).sum()) < 1e-4 * abs(f.sum())
+    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-6)
+
+
+def test_dirichlet_fixed_point_relaxation_and_ramp(ctx):
+    # tests/PDEADITests.cpp:219-264, 297-354
+    plan = halhip.Adi2D(ctx, 7, 5)
+    plan.upload(np.full(35, 4.0, np.float32))
+    plan.run_steps(3, 0.9, 0.0, False, False, bc=4.0)
+    np.testing.assert_allclose(plan.download(), 4.0, rtol=1e-6)
+
+    plan = halhip.Adi2D(ctx, 8, 6)
+    plan.upload(np.zeros(48, np.float32))
+    plan.run_steps(400, 1.5, 0.0, True, False, bc=2.0)
+    np.testing.assert_allclose(plan.download(), 2.0, rtol=1e-5)
+
+    X, Y = 6, 4
+    ramp = lambda x: 1.0 if x < 0 else (3.0 if x >= X else 1.0 + 2.0 * (x + 0.5) / X)
+    f = np.array([ramp(x) for x in range(X) for _ in range(Y)])
+    edges = (np.full(Y, 1.0), np.full(Y, 3.0), np.array([ramp(x) for x in range(X)]),
+             np.array([ramp(x) for x in range(X)]))
+    plan = halhip.Adi2D(ctx, X, Y)
+    d = plan.diffuse(f, np.zeros_like(f), 0.9, False, False, edges)
+    np.testing.assert_allclose(d, 0.0, atol=2e-6)
+
+
+# Shapes that exercise every strip kernel variant (columns per thread 1..8,
+# strip heights 4..16, partial last strips and wavefronts, odd widths).
+SHAPES = [(1, 1), (3, 1), (1, 7), (5, 64), (9, 65), (33, 200), (70, 1000), (17, 1025), (13, 2048), (21, 2049),
+          (11, 4096), (6, 5000), (5, 8192), (300, 300), (1030, 40)]
+
+
+@pytest.mark.parametrize("X,Y", SHAPES)
+def test_shapes_all_boundaries_vs_oracle(ctx, oracle, X, Y):
+    rng = np.random.default_rng(X * 10007 + Y)
+    f = rng.random(X * Y).astype(np.float32)
+    plan = halhip.Adi2D(ctx, X, Y)
+    for wx, wy, bc in ((0, 0, None), (1, 1, None), (1, 0, 0.3), (0, 1, 0.7), (0, 0, 0.0)):
+        plan.upload(f)
+        plan.run_steps(3, 5.0, 0.0, wx, wy, bc=bc)
+        got = plan.download()
+        ref = adi_ref_ticks(oracle, f, 3, 5.0, X, Y, wx, wy, bc)
+        err = rel_l2(got, ref)
+        assert err < 1e-5, (X, Y, wx, wy, bc, err)
+
+
+def test_decay_and_sources_vs_oracle(ctx, oracle):
+    X, Y = 50, 130
+    rng = np.random.default_rng(9)
+    f = rng.random(X * Y).astype(np.float32)
+    idx = [0, 77, 77, X * Y - 1, 2500]
+    val = [1.0, 0.5, 0.25, 2.0, 0.125]
+    plan = halhip.Adi2D(ctx, X, Y)
+    plan.upload(f)
+    plan.set_sources(idx, val)
+    plan.run_steps(25, 2.0, 5e-3, False, True, bc=0.1)
+    got = plan.download()
+    ref = adi_ref_ticks(oracle, f, 25, 2.0, X, Y, 0, 1, 0.1, 5e-3, idx, val)
+    assert rel_l2(got, ref) < 1e-5
+
+
+def test_rate_change_between_runs(ctx, oracle):
+    X, Y = 40, 48
+    f = np.random.default_rng(4).random(X * Y).astype(np.float32)
+    plan = halhip.Adi2D(ctx, X, Y)
+    plan.upload(f)
+    plan.run_steps(4, 5.0, 0.0, False, False)
+    plan.run_steps(4, 0.5, 0.0, False, False, bc=1.0)
+    plan.run_steps(30, 5.0, 0.0, False, False)
+    got = plan.download()
+    ref = adi_ref_ticks(oracle, f, 4, 5.0, X, Y, 0, 0)
+    ref = adi_ref_ticks(oracle, ref, 4, 0.5, X, Y, 0, 0, 1.0)
+    ref = adi_ref_ticks(oracle, ref, 30, 5.0, X, Y, 0, 0)
+    assert rel_l2(got, ref) < 1e-5
+
+
+def test_errors(ctx):
+    plan = halhip.Adi2D(ctx, 3, 4)
+    with pytest.raises(halhip.InvalidArgument):
+        plan.diffuse(np.zeros(12), np.zeros(11), 0.1, False, False)
+    with pytest.raises(halhip.HalHipError) as e:
+        halhip.Adi2D(ctx, 4, halhip.ADI_MAX_Y + 1)
+    assert e.value.code == halhip.ERR_UNSUPPORTED
+    ctx.fail_next_allocs(1)
+    with pytest.raises(halhip.HalHipError) as e:
+        halhip.Adi2D(ctx, 64, 64)
+    assert e.value.code == halhip.ERR_OUT_OF_MEMORY
+
+
+# ---- the 4096^2 workload ------------------------------------------------------
+
+def test_blobs_4096_first_ticks_vs_oracle(ctx, oracle):
+    w = workloads.adi_blobs()
+    plan = halhip.Adi2D(ctx, w["X"], w["Y"])
+    plan.upload(w["field"])
+    plan.run_steps(2, w["rate"], 0.0, False, False, bc=w["bc"])
+    got = plan.download()
+    ref = adi_ref_ticks(oracle, w["field"], 2, w["rate"], w["X"], w["Y"], 0, 0, w["bc"])
+    err = rel_l2(got, ref)
+    print(f"ADI 4096^2 blobs, 2 ticks: rel L2 vs FP64 = {err:.3e}")
+    assert err < 1e-5
+
+
+def test_4096_periodic_eigenmode_200_ticks(ctx):
+    # Closed form at full size: each tick multiplies the mode by the
+    # Crank-Nicolson factor product.
+    n, r, k = 4096, 5.0, 3
+    x = np.arange(n)
+    col = np.sin(2 * np.pi * k * x / n)
+    f = np.outer(col, col).ravel().astype(np.float32)
+    plan = halhip.Adi2D(ctx, n, n)
+    plan.upload(f)
+    plan.run_steps(200, r, 0.0, True, True)
+    got = plan.download()
+    lam = (lambda_cn(r, n, k) ** 2) ** 200
+    err = rel_l2(got, lam * f.astype(np.float64))
+    print(f"4096^2 periodic eigenmode x 200 ticks: rel L2 = {err:.3e}")
+    assert err < 1e-5
+
+
+def test_4096_zero_flux_mass_200_ticks(ctx):
+    w = workloads.adi_blobs()
+    plan = halhip.Adi2D(ctx, w["X"], w["Y"])
+    plan.upload(w["field"])
+    plan.run_steps(200, w["rate"], 0.0, False, False)
+    got = plan.download().astype(np.float64)
+    m0 = w["field"].astype(np.float64).sum()
+    assert abs(got.sum() - m0) < 1e-5 * m0
+
+
+def test_4096_linearity(ctx):
+    n = 4096
+    w = workloads.adi_blobs()
+    g = np.random.default_rng(1).random(n * n).astype(np.float32)
+    plan = halhip.Adi2D(ctx, n, n)
+    outs = []
+    for f in (w["field"], g, 2.0 * w["field"] - 3.0 * g):
+        plan.upload(f)
+        plan.run_steps(20, 5.0, 0.0, False, False, bc=0.0)
+        outs.append(plan.download().astype(np.float64))
+    assert rel_l2(outs[2], 2.0 * outs[0] - 3.0 * outs[1]) < 1e-5

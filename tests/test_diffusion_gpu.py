@@ -1,0 +1,90 @@
+"""Explicit diffusion on the GPU vs the FP64 oracle and the reference's
+closed forms (IDiffusionBackend2D / DiffusionKernel2D::RunSteps)."""
+import numpy as np
+import pytest
+
+import halhip
+import workloads
+from helpers import eigen_field, golden_cases, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def test_interface_matches_golden_cases(ctx, golden):
+    for c in golden_cases(golden):
+        if c["adi"]:
+            continue
+        plan = halhip.Diffusion2D(ctx, c["X"], c["Y"])
+        # The interface computes in FP32 like the reference's GPU adapter:
+        # compare against the reference on the FP32-rounded field.
+        got = plan.diffuse(c["field"], c["deltas_in"].copy(), c["rate"], c["wrapX"], c["wrapY"], c["bc"])
+        scale = max(1.0, np.abs(c["field"]).max())
+        np.testing.assert_allclose(got, c["deltas_out"], rtol=0, atol=2e-6 * scale, err_msg=f"case {c['k']}")
+        plan.close()
+
+
+def test_eigenmode_through_interface(ctx):
+    # tests/IDiffusionBackend2DTests.cpp:23-72 (tolerance 1e-4 relative there)
+    nx, ny, r = 16, 20, 0.05
+    f = eigen_field(nx, ny)
+    plan = halhip.Diffusion2D(ctx, nx, ny)
+    got = f + plan.diffuse(f, np.zeros_like(f), r, True, True)
+    lam = 1 - 4 * r * (np.sin(np.pi / nx) ** 2 + np.sin(np.pi / ny) ** 2)
+    np.testing.assert_allclose(got, lam * f, rtol=0, atol=1e-6)
+
+
+def test_rates_interface(ctx, golden, oracle):
+    f, rx, ry = golden["rates_field"], golden["rates_x"], golden["rates_y"]
+    plan = halhip.Diffusion2D(ctx, 9, 11)
+    for wx in (0, 1):
+        for wy in (0, 1):
+            got = plan.diffuse_rates(f, np.zeros(99), rx, ry, wx, wy)
+            np.testing.assert_allclose(got, golden[f"rates_{wx}{wy}_out"], atol=2e-6)
+
+
+def test_size_mismatch_raises(ctx):
+    # tests/IDiffusionBackend2DTests.cpp:146-177
+    plan = halhip.Diffusion2D(ctx, 3, 4)
+    with pytest.raises(halhip.InvalidArgument):
+        plan.diffuse(np.zeros(12), np.zeros(11), 0.1, False, False)
+
+
+@pytest.mark.parametrize("wrap", [(1, 1), (0, 0), (1, 0)])
+def test_resident_ticks_match_oracle(ctx, oracle, wrap):
+    X, Y = 37, 70
+    rng = np.random.default_rng(11)
+    f = rng.random(X * Y).astype(np.float32)
+    plan = halhip.Diffusion2D(ctx, X, Y)
+    plan.upload(f)
+    plan.set_sources([5, 5, 1000], [1.0, 0.5, 2.0])
+    plan.run_steps(120, 0.2, 2e-3, *wrap)
+    got = plan.download()
+    ref = oracle.explicit_ticks(f.astype(np.float64), 120, 0.2, 2e-3, X, Y, *wrap, [5, 5, 1000], [1.0, 0.5, 2.0])
+    assert rel_l2(got, ref) < 1e-6
+
+
+def test_dirichlet_resident_matches_oracle(ctx, oracle):
+    X, Y = 40, 33
+    f = np.random.default_rng(2).random(X * Y).astype(np.float32)
+    plan = halhip.Diffusion2D(ctx, X, Y)
+    plan.upload(f)
+    plan.run_steps(30, 0.25, 0.0, False, False, bc=0.5)
+    got = plan.download()
+    ref = f.astype(np.float64)
+    for _ in range(30):
+        ref = ref + oracle.diffuse(ref, np.zeros_like(ref), 0.25, X, Y, 0, 0, 0.5)
+    assert rel_l2(got, ref) < 1e-6
+
+
+def test_workload_256_1000_ticks_within_1e5(ctx, oracle):
+    w = workloads.explicit_256()
+    plan = halhip.Diffusion2D(ctx, w["X"], w["Y"])
+    plan.upload(w["field"])
+    plan.set_sources(w["src_idx"], w["src_val"])
+    plan.run_steps(1000, w["rate"], w["decay"], w["wrapX"], w["wrapY"])
+    got = plan.download()
+    ref = oracle.explicit_ticks(w["field"].astype(np.float64), 1000, w["rate"], w["decay"], w["X"], w["Y"],
+                                1, 1, w["src_idx"], w["src_val"].astype(np.float64))
+    err = rel_l2(got, ref)
+    print(f"explicit 256^2 x 1000 ticks: rel L2 vs FP64 = {err:.3e}")
+    assert err < 1e-5

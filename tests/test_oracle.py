@@ -1,0 +1,231 @@
+"""Pins the FP64 oracle (oracle/hal_oracle.cpp) before it is trusted.
+
+Each check is independent of the oracle's own formulas: golden vectors from
+the compiled reference, closed forms the reference's tests use, textbook
+solvers (dense / banded), brute force and conservation laws.
+"""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from helpers import eigen_field, golden_cases, lambda_cn
+
+
+# ---- golden vectors from the compiled reference ------------------------------
+
+def test_rng_matches_reference_stream(oracle, golden):
+    assert np.array_equal(oracle.rng_doubles(42, 64), golden["rng42_doubles"])
+    assert np.array_equal(oracle.rng_u64(1234, 16), golden["rng1234_u64"])
+    assert np.array_equal(oracle.rng_doubles(7, 16), golden["rng7_doubles"])
+
+
+def test_rng_known_answer():
+    # splitmix64 from seed 0 yields 0xE220A8397B1DCDAF first (Vigna's
+    # published reference output); xoshiro256** state = 4 splitmix outputs.
+    from oracle import Oracle
+    o = Oracle()
+    sm = 0
+    state = []
+    for _ in range(4):
+        sm = (sm + 0x9E3779B97F4A7C15) & (2**64 - 1)
+        z = sm
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & (2**64 - 1)
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & (2**64 - 1)
+        state.append(z ^ (z >> 31))
+    assert state[0] == 0xE220A8397B1DCDAF
+    s1 = state[1]
+    rotl = lambda x, k: ((x << k) | (x >> (64 - k))) & (2**64 - 1)
+    first = (rotl((s1 * 5) & (2**64 - 1), 7) * 9) & (2**64 - 1)
+    assert int(o.rng_u64(0, 1)[0]) == first
+
+
+def test_diffusion_and_adi_match_golden(oracle, golden):
+    for c in golden_cases(golden):
+        fn = oracle.adi if c["adi"] else oracle.diffuse
+        got = fn(c["field"], c["deltas_in"], c["rate"], c["X"], c["Y"], c["wrapX"], c["wrapY"], c["bc"])
+        np.testing.assert_array_equal(got, c["deltas_out"], err_msg=f"case {c['k']}")
+
+
+def test_rates_match_golden(oracle, golden):
+    f, rx, ry = golden["rates_field"], golden["rates_x"], golden["rates_y"]
+    for wx in (0, 1):
+        for wy in (0, 1):
+            got = oracle.diffuse_rates(f, np.zeros(99), rx, ry, 9, 11, wx, wy)
+            np.testing.assert_array_equal(got, golden[f"rates_{wx}{wy}_out"])
+
+
+def test_tick_compositions_match_golden(oracle, golden):
+    got = oracle.explicit_ticks(golden["ticks_explicit_field"], 25, 0.2, 1e-3, 16, 16, 1, 1, [8 * 16 + 8], [1.0])
+    np.testing.assert_array_equal(got, golden["ticks_explicit_out"])
+    got = oracle.adi_ticks(golden["ticks_adi_field"], 6, 5.0, 5e-3, 24, 20, 0, 0, bc=0.25,
+                           src_idx=[3, 77, 77, 400], src_val=[1.0, 0.5, 0.25, 2.0])
+    np.testing.assert_array_equal(got, golden["ticks_adi_out"])
+    got = oracle.adi_ticks(golden["ticks_adi_field"], 6, 0.8, 0.0, 24, 20, 0, 1)
+    np.testing.assert_array_equal(got, golden["ticks_adi_zf_out"])
+
+
+def test_agent_forces_match_golden(oracle, golden):
+    pos, rad = golden["agents_pos"], golden["agents_rad"]
+    np.testing.assert_array_equal(oracle.agent_forces(pos, rad, 3.0, 0.4, 16, 16, 1, 1), golden["agents_wrap_out"])
+    np.testing.assert_array_equal(oracle.agent_forces(pos, rad, 3.0, 0.4, 16, 16, 0, 0), golden["agents_nowrap_out"])
+    np.testing.assert_array_equal(oracle.agent_forces(golden["agents2_pos"], np.full(500, 0.5), 1.0, 1.0, 40, 24, 0, 1),
+                                  golden["agents2_out"])
+
+
+# ---- closed forms (the reference's own known answers) -----------------------
+
+def test_explicit_periodic_eigenmode(oracle):
+    # tests/IDiffusionBackend2DTests.cpp:23-72
+    nx, ny, r = 16, 20, 0.05
+    f = eigen_field(nx, ny)
+    got = f + oracle.diffuse(f, np.zeros_like(f), r, nx, ny, 1, 1)
+    lam = 1 - 4 * r * (np.sin(np.pi / nx) ** 2 + np.sin(np.pi / ny) ** 2)
+    np.testing.assert_allclose(got, lam * f, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("r", [0.3, 5.0])
+def test_adi_periodic_eigenmode_is_crank_nicolson_product(oracle, r):
+    # tests/IDiffusionBackend2DTests.cpp:76-144, tests/PDEADITests.cpp:75-130
+    nx, ny = 12, 18
+    f = eigen_field(nx, ny)
+    got = f + oracle.adi(f, np.zeros_like(f), r, nx, ny, 1, 1)
+    lam = lambda_cn(r, nx) * lambda_cn(r, ny)
+    np.testing.assert_allclose(got, lam * f, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("wrap", [(0, 0), (1, 0), (0, 1)])
+def test_mass_conservation_zero_flux(oracle, wrap):
+    # tests/PDEGridTests.cpp:72-104 (explicit), tests/AdiKernel2DTests.cpp:70-141 (ADI)
+    rng = np.random.default_rng(3)
+    X, Y = 20, 14
+    f = rng.random(X * Y)
+    for fn, rate in ((oracle.diffuse, 0.2), (oracle.adi, 0.6), (oracle.adi, 5.0)):
+        d = fn(f, np.zeros_like(f), rate, X, Y, *wrap)
+        assert abs(d.sum()) < 1e-11
+
+
+def test_decay_only_closed_form(oracle):
+    # u = c everywhere, no diffusion flux: u_t = c (1-k)^t exactly.
+    f = np.full(64, 3.0)
+    got = oracle.explicit_ticks(f, 50, 0.2, 1e-2, 8, 8, 1, 1)
+    np.testing.assert_allclose(got, 3.0 * (1 - 1e-2) ** 50, rtol=1e-13)
+
+
+def test_adi_dirichlet_fixed_point_and_relaxation(oracle):
+    # tests/PDEADITests.cpp:219-264
+    f = np.full(7 * 5, 4.0)
+    d = oracle.adi(f, np.zeros_like(f), 0.9, 7, 5, 0, 0, 4.0)
+    np.testing.assert_allclose(d, 0.0, atol=1e-13)
+    got = oracle.adi_ticks(np.zeros(8 * 6), 400, 1.5, 0.0, 8, 6, 1, 0, bc=2.0)
+    np.testing.assert_allclose(got, 2.0, rtol=1e-6)
+
+
+def test_adi_dirichlet_linear_ramp_is_steady(oracle):
+    # tests/PDEADITests.cpp:297-354: bc 1 at x=-1, 3 at x=X, ramp between faces.
+    X, Y = 6, 4
+    ramp = lambda x: 1.0 if x < 0 else (3.0 if x >= X else 1.0 + 2.0 * (x + 0.5) / X)
+    f = np.array([ramp(x) for x in range(X) for _ in range(Y)])
+    bc = (np.full(Y, 1.0), np.full(Y, 3.0), np.array([ramp(x) for x in range(X)]), np.array([ramp(x) for x in range(X)]))
+    d = oracle.adi(f, np.zeros_like(f), 0.9, X, Y, 0, 0, bc)
+    np.testing.assert_allclose(d, 0.0, atol=1e-12)
+
+
+# ---- textbook solvers ------------------------------------------------------
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 33])
+def test_thomas_vs_dense_solve(oracle, n):
+    rng = np.random.default_rng(n)
+    a, c = -rng.random(n), -rng.random(n)
+    b = 3 + rng.random(n)
+    d = rng.random(n)
+    M = np.diag(b) + np.diag(a[1:], -1) + np.diag(c[:-1], 1)
+    np.testing.assert_allclose(oracle.thomas(a, b, c, d), np.linalg.solve(M, d), rtol=1e-12, atol=1e-14)
+    if n >= 3:
+        # TDMAperiodic (TdmaSolver.h:68-120) takes alpha = a[0] and beta =
+        # c[n-1] and pairs them so that the solved cyclic matrix has top-right
+        # corner c[n-1] and bottom-left a[0]; HAL only calls it with equal
+        # off-diagonals, where the pairing does not matter.
+        P = M.copy()
+        P[0, n - 1] = c[n - 1]
+        P[n - 1, 0] = a[0]
+        np.testing.assert_allclose(oracle.thomas(a, b, c, d, periodic=True), np.linalg.solve(P, d),
+                                   rtol=1e-11, atol=1e-13)
+
+
+@pytest.mark.parametrize("n,rate,has_bc", [(5, 5.0, False), (64, 5.0, True), (4096, 5.0, True), (100, 0.3, False)])
+def test_adi_line_vs_banded_solve(oracle, n, rate, has_bc):
+    # ADISolveRow coefficients (ADIequations.h:225-270) assembled by hand.
+    rng = np.random.default_rng(n)
+    d = rng.random(n)
+    off = -rate / 2
+    diag = np.full(n, 1 + rate)
+    adj = rate / 2 if has_bc else -rate / 2
+    diag[0] += adj
+    diag[-1] += adj
+    ab = np.zeros((3, n))
+    ab[0, 1:] = off
+    ab[1] = diag
+    ab[2, :-1] = off
+    want = scipy.linalg.solve_banded((1, 1), ab, d)
+    np.testing.assert_allclose(oracle.adi_line(d, rate, 0, has_bc), want, rtol=1e-12, atol=1e-14)
+
+
+# ---- agents: brute force and sorted bins ----------------------------------
+
+def _brute(pos, rad, R, k, X, Y, wx, wy):
+    n = rad.size
+    out = np.zeros(2 * n)
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                continue
+            d = []
+            for ax, dim, w in ((0, X, wx), (1, Y, wy)):
+                a, b = pos[2 * i + ax], pos[2 * j + ax]
+                v = b - a
+                if w and abs(v) > dim * 0.5:
+                    v = v + dim if a > b else v - dim
+                d.append(v)
+            dist = np.hypot(*d)
+            if 0 < dist < R:
+                F = k * ((rad[i] + rad[j]) - dist)
+                out[2 * i] -= d[0] / dist * F
+                out[2 * i + 1] -= d[1] / dist * F
+    return out
+
+
+def test_agent_forces_vs_brute_force(oracle, golden):
+    # tests/IAgentForceBackend2DTests.cpp:51-121
+    pos, rad = golden["agents_pos"], golden["agents_rad"]
+    for wx, wy in ((1, 1), (0, 0), (1, 0)):
+        np.testing.assert_allclose(oracle.agent_forces(pos, rad, 3.0, 0.4, 16, 16, wx, wy),
+                                   _brute(pos, rad, 3.0, 0.4, 16, 16, wx, wy), rtol=1e-12, atol=1e-12)
+
+
+def test_coincident_agents_give_zero(oracle):
+    # tests/IAgentForceBackend2DTests.cpp:153-165
+    v = oracle.agent_forces([4.0, 4.0, 4.0, 4.0], [0.5, 0.5], 2.0, 1.0, 8, 8, 0, 0)
+    assert np.all(v == 0.0)
+
+
+def test_agent_bins_are_stable_counting_sort(oracle):
+    rng = np.random.default_rng(5)
+    X, Y, n = 13, 7, 400
+    pos = rng.random(2 * n) * np.tile([X + 2.0, Y + 2.0], n) - 1.0   # some off-grid (clamped)
+    offsets, bucket = oracle.agent_bins(pos, X, Y)
+    cx = np.clip(np.floor(pos[0::2]).astype(int), 0, X - 1)
+    cy = np.clip(np.floor(pos[1::2]).astype(int), 0, Y - 1)
+    cell = cx * Y + cy
+    order = np.lexsort((np.arange(n), cell))
+    np.testing.assert_array_equal(bucket, order)
+    np.testing.assert_array_equal(offsets, np.concatenate([[0], np.cumsum(np.bincount(cell, minlength=X * Y))]))
+
+
+def test_agent_move_safe_semantics(oracle):
+    # AgentPT2D::MoveSafePT (include/Hal/Agents/AgentPT2D.h:157-186)
+    pos = np.array([1.0, 1.0, 9.5, 2.0, 5.0, 0.05])
+    vel = np.array([1.0, 1.0, 10.0, 1.0, 0.0, -1.0])
+    got = oracle.agent_move(pos, vel, 0.1, 10, 10, 0, 0)
+    np.testing.assert_allclose(got, [1.1, 1.1, 9.5, 2.1, 5.0, 0.05])
+    got = oracle.agent_move(pos, vel, 0.1, 10, 10, 1, 1)
+    np.testing.assert_allclose(got, [1.1, 1.1, 0.5, 2.1, 5.0, 9.95])
