@@ -83,6 +83,18 @@ _SIGS = {
     "halhip_adi2d_sync": [_vp, _fp],
     "halhip_adi2d_set_profiling": [_vp, _i32],
     "halhip_adi2d_profile": [_vp, _i32, _ip, ctypes.c_char_p, _i32, _dp, _lp],
+    "halhip_agents2d_create": [_vp, _i32, _i32, _i32, ctypes.POINTER(_vp)],
+    "halhip_agents2d_destroy": [_vp],
+    "halhip_agents2d_forces_f64": [_vp, _dp, _dp, _i32, _dp, _f64, _f64, _i32, _i32],
+    "halhip_agents2d_upload": [_vp, _fp, _fp, _i32],
+    "halhip_agents2d_download_positions": [_vp, _fp, _i32],
+    "halhip_agents2d_download_velocities": [_vp, _fp, _i32],
+    "halhip_agents2d_download_bins": [_vp, _ip, _ip],
+    "halhip_agents2d_run_ticks": [_vp, _i32, _f64, _f64, _f64, _i32, _i32],
+    "halhip_agents2d_run_ticks_async": [_vp, _i32, _f64, _f64, _f64, _i32, _i32],
+    "halhip_agents2d_sync": [_vp, _fp],
+    "halhip_agents2d_set_profiling": [_vp, _i32],
+    "halhip_agents2d_profile": [_vp, _i32, _ip, ctypes.c_char_p, _i32, _dp, _lp],
 }
 
 
@@ -307,3 +319,51 @@ class Adi2D(_Plan):
         s, keep = make_bc(bc, self.X, self.Y)
         name = "run_steps_async" if asynchronous else "run_steps"
         _check(self._fn(name)(self.h, steps, rate, decay, int(wrapX), int(wrapY), _bcp(s)))
+
+
+class Agents2D(_Plan):
+    """Agent-force plan (IAgentForceBackend2D + resident ticks)."""
+    _prefix = "agents2d"
+
+    def __init__(self, ctx, X, Y, max_agents):
+        self.ctx, self.X, self.Y, self.cap = ctx, X, Y, max_agents
+        self.n = 0
+        self.h = _vp()
+        _check(load().halhip_agents2d_create(ctx.h, X, Y, max_agents, ctypes.byref(self.h)))
+
+    def accumulate_forces(self, positions, radii, R, k, wrapX, wrapY):
+        """IAgentForceBackend2D::AccumulateForces: returns the float64 (2N) deltas."""
+        positions, radii = _f64(positions), _f64(radii)
+        if positions.size != 2 * radii.size:
+            raise InvalidArgument(ERR_INVALID_ARGUMENT, "positions must hold two values per radius")
+        out = np.zeros(positions.size, np.float64)
+        _check(self._fn("forces_f64")(self.h, positions.ctypes.data_as(_dp), radii.ctypes.data_as(_dp), radii.size,
+                                      out.ctypes.data_as(_dp), R, k, int(wrapX), int(wrapY)))
+        self.n = radii.size
+        return out
+
+    def upload(self, positions, radii):
+        positions, radii = _f32(positions), _f32(radii)
+        assert positions.size == 2 * radii.size
+        _check(self._fn("upload")(self.h, positions.ctypes.data_as(_fp), radii.ctypes.data_as(_fp), radii.size))
+        self.n = radii.size
+
+    def run_ticks(self, ticks, R, k, dt, wrapX, wrapY, asynchronous=False):
+        name = "run_ticks_async" if asynchronous else "run_ticks"
+        _check(self._fn(name)(self.h, ticks, R, k, dt, int(wrapX), int(wrapY)))
+
+    def positions(self):
+        out = np.empty(2 * self.n, np.float32)
+        _check(self._fn("download_positions")(self.h, out.ctypes.data_as(_fp), self.n))
+        return out
+
+    def velocities(self):
+        out = np.empty(2 * self.n, np.float32)
+        _check(self._fn("download_velocities")(self.h, out.ctypes.data_as(_fp), self.n))
+        return out
+
+    def bins(self):
+        offsets = np.empty(self.X * self.Y + 1, np.int32)
+        bucket = np.empty(max(self.n, 1), np.int32)
+        _check(self._fn("download_bins")(self.h, offsets.ctypes.data_as(_ip), bucket.ctypes.data_as(_ip)))
+        return offsets, bucket[:self.n]

@@ -167,6 +167,45 @@ int halhip_adi2d_set_profiling(halhip_adi2d *plan, int32_t on);
 int halhip_adi2d_profile(halhip_adi2d *plan, int32_t maxKernels, int32_t *nKernels, char *names, int32_t namesLen,
                          double *totalMs, int64_t *launches);
 
+/* ---- agent forces -------------------------------------------------------- */
+
+/* Replaces Hal::Gpu::AgentForceKernel2D / GpuAgentForceBackend2D
+ * (include/Hal/Gpu/AgentForceKernel2D.h, src/Gpu/GpuAgentForceBackend2D.cpp:24-68).
+ * Sized at creation: grid xDim*yDim bins of size 1 and maxAgents capacity. */
+int halhip_agents2d_create(halhip_context *ctx, int32_t xDim, int32_t yDim, int32_t maxAgents,
+                           halhip_agents2d **out);
+int halhip_agents2d_destroy(halhip_agents2d *plan);
+
+/* IAgentForceBackend2D::AccumulateForces (include/Hal/Agents/IAgentForceBackend2D.h:28-36):
+ * positions (2*nAgents, x0,y0,x1,...), radii (nAgents); velocityDeltas
+ * (2*nAgents) is overwritten. */
+int halhip_agents2d_forces_f64(halhip_agents2d *plan, const double *positions, const double *radii,
+                               int32_t nAgents, double *velocityDeltas, double interactionRad,
+                               double springConstant, int32_t wrapX, int32_t wrapY);
+
+/* Resident agents: FP32 positions and radii on the device. */
+int halhip_agents2d_upload(halhip_agents2d *plan, const float *positions, const float *radii, int32_t nAgents);
+int halhip_agents2d_download_positions(halhip_agents2d *plan, float *positions, int32_t nAgents);
+int halhip_agents2d_download_velocities(halhip_agents2d *plan, float *velocityDeltas, int32_t nAgents);
+/* Bins of the last tick: offsets (xDim*yDim+1 int32, exclusive prefix of
+ * per-bin counts) and bucket (nAgents int32, agent indices in bin order,
+ * ascending index within a bin). */
+int halhip_agents2d_download_bins(halhip_agents2d *plan, int32_t *offsets, int32_t *bucket);
+
+/* `ticks` ticks of bin -> scan -> stable scatter -> force -> move.  The move
+ * is p += dt*v with AgentPT2D::MoveSafePT semantics
+ * (include/Hal/Agents/AgentPT2D.h:157-186): an axis whose target leaves a
+ * non-wrapped grid keeps its old coordinate; a wrapped axis wraps.  ticks == 0
+ * with dt == 0 computes bins and velocities only. */
+int halhip_agents2d_run_ticks(halhip_agents2d *plan, int32_t ticks, double interactionRad, double springConstant,
+                              double dt, int32_t wrapX, int32_t wrapY);
+int halhip_agents2d_run_ticks_async(halhip_agents2d *plan, int32_t ticks, double interactionRad,
+                                    double springConstant, double dt, int32_t wrapX, int32_t wrapY);
+int halhip_agents2d_sync(halhip_agents2d *plan, float *elapsedMs);
+int halhip_agents2d_set_profiling(halhip_agents2d *plan, int32_t on);
+int halhip_agents2d_profile(halhip_agents2d *plan, int32_t maxKernels, int32_t *nKernels, char *names,
+                            int32_t namesLen, double *totalMs, int64_t *launches);
+
 #ifdef __cplusplus
 }
 #endif
