@@ -67,6 +67,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-ticks", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run the copy kernel (read + write this many MiB, 10 launches) for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this backend runs on one GPU; --gpus {args.gpus} is not supported "
@@ -76,6 +78,14 @@ def main():
     import numpy as np
     import halhip
     import workloads
+
+    if args.calibrate_copy_mib:
+        ctx = halhip.Context(0)
+        nbytes = args.calibrate_copy_mib << 20
+        ms = ctx.calibration_copy(nbytes, 10)
+        print(json.dumps({"calibration_copy_bytes": nbytes, "ms_per_copy": ms,
+                          "GBps_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        return
 
     w = workloads.adi_blobs()
     n = w["X"]

@@ -59,6 +59,7 @@ _SIGS = {
     "halhip_context_destroy": [_vp],
     "halhip_context_fail_next_allocs": [_vp, _i32],
     "halhip_context_synchronize": [_vp],
+    "halhip_calibration_copy": [_vp, _i64, _i32, _fp],
     "halhip_rng_doubles": [ctypes.c_uint64, _i64, _dp],
     "halhip_diffusion2d_create": [_vp, _i32, _i32, ctypes.POINTER(_vp)],
     "halhip_diffusion2d_destroy": [_vp],
@@ -172,6 +173,15 @@ def make_bc(bc, X, Y):
     return s, arrs
 
 
+def _check_field_sizes(field, deltas, X, Y, caller):
+    """Internal::CheckFieldSizes (src/Pde/IDiffusionBackend2D.cpp:20-33): both
+    arrays must hold xDim*yDim values; deltas is updated in place."""
+    if not (isinstance(deltas, np.ndarray) and deltas.dtype == np.float64 and deltas.flags.c_contiguous):
+        raise InvalidArgument(ERR_INVALID_ARGUMENT, caller + ": deltas must be a contiguous float64 array")
+    if field.size != X * Y or deltas.size != X * Y:
+        raise InvalidArgument(ERR_INVALID_ARGUMENT, caller + ": field and deltas must each hold xDim*yDim values")
+
+
 def _bcp(s):
     return ctypes.byref(s) if s is not None else None
 
@@ -185,6 +195,11 @@ class Context:
 
     def fail_next_allocs(self, count):
         _check(load().halhip_context_fail_next_allocs(self.h, count))
+
+    def calibration_copy(self, nbytes, repeats):
+        ms = ctypes.c_float(0.0)
+        _check(load().halhip_calibration_copy(self.h, nbytes, repeats, ctypes.byref(ms)))
+        return ms.value
 
     def synchronize(self):
         _check(load().halhip_context_synchronize(self.h))
@@ -249,15 +264,17 @@ class Diffusion2D(_Plan):
     def diffuse(self, field, deltas, rate, wrapX, wrapY, bc=None):
         """deltas += one explicit step (in place on the float64 `deltas`)."""
         field = _f64(field)
-        assert deltas.dtype == np.float64 and deltas.flags.c_contiguous
+        _check_field_sizes(field, deltas, self.X, self.Y, type(self).__name__ + ".diffuse")
         s, keep = make_bc(bc, self.X, self.Y)
-        _check(self._fn("diffuse_f64")(self.h, field.ctypes.data_as(_dp), deltas.ctypes.data_as(_dp),
-                                       deltas.size if field.size == deltas.size else field.size,
+        _check(self._fn("diffuse_f64")(self.h, field.ctypes.data_as(_dp), deltas.ctypes.data_as(_dp), field.size,
                                        rate, int(wrapX), int(wrapY), _bcp(s)))
         return deltas
 
     def diffuse_rates(self, field, deltas, ratesX, ratesY, wrapX, wrapY):
         field, ratesX, ratesY = _f64(field), _f64(ratesX), _f64(ratesY)
+        _check_field_sizes(field, deltas, self.X, self.Y, "Diffusion2D.diffuse_rates")
+        if ratesX.size != field.size or ratesY.size != field.size:
+            raise InvalidArgument(ERR_INVALID_ARGUMENT, "Diffusion2D.diffuse_rates: rates must hold xDim*yDim values")
         _check(self._fn("diffuse_rates_f64")(self.h, field.ctypes.data_as(_dp), deltas.ctypes.data_as(_dp),
                                              ratesX.ctypes.data_as(_dp), ratesY.ctypes.data_as(_dp),
                                              field.size, int(wrapX), int(wrapY)))
@@ -294,10 +311,9 @@ class Adi2D(_Plan):
 
     def diffuse(self, field, deltas, rate, wrapX, wrapY, bc=None):
         field = _f64(field)
-        assert deltas.dtype == np.float64 and deltas.flags.c_contiguous
+        _check_field_sizes(field, deltas, self.X, self.Y, type(self).__name__ + ".diffuse")
         s, keep = make_bc(bc, self.X, self.Y)
-        _check(self._fn("diffuse_f64")(self.h, field.ctypes.data_as(_dp), deltas.ctypes.data_as(_dp),
-                                       deltas.size if field.size == deltas.size else field.size,
+        _check(self._fn("diffuse_f64")(self.h, field.ctypes.data_as(_dp), deltas.ctypes.data_as(_dp), field.size,
                                        rate, int(wrapX), int(wrapY), _bcp(s)))
         return deltas
 

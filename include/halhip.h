@@ -75,6 +75,11 @@ int halhip_context_destroy(halhip_context *ctx);
 /* Waits for all work on the context's device (hipDeviceSynchronize). */
 int halhip_context_synchronize(halhip_context *ctx);
 
+/* Measurement aid: `repeats` launches of a plain copy kernel moving `bytes`
+ * (read `bytes`, write `bytes`); msPerCopy gets the mean device time.  Used to
+ * calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE counters. */
+int halhip_calibration_copy(halhip_context *ctx, int64_t bytes, int32_t repeats, float *msPerCopy);
+
 /* Test hook: the next `count` device allocations made through ctx fail with
  * HALHIP_ERR_OUT_OF_MEMORY, so error paths can be exercised from the host. */
 int halhip_context_fail_next_allocs(halhip_context *ctx, int32_t count);
