@@ -91,87 +91,188 @@ namespace
         float xzLast;
     };
 
-    // ---- register-tile helpers ------------------------------------------------
+    // ---- wavefront primitives ------------------------------------------------------
 
-    template <int CPT>
-    __device__ __forceinline__ void LoadRow(const float *__restrict__ row, int y0, int Y, bool vec, float (&d)[CPT])
+    // DPP lane moves (GFX9 encodings): row_shr:n = 0x110+n, wave_shr:1 = 0x138,
+    // row_bcast:15 = 0x142, row_bcast:31 = 0x143.  Lanes without a source (or
+    // in rows outside rowMask) receive `old`.
+    template <int CTRL, int ROW_MASK = 0xF>
+    __device__ __forceinline__ float Dpp(float old, float v)
     {
-        if (vec && y0 + CPT <= Y)
+        return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                                      __builtin_bit_cast(int, v), CTRL, ROW_MASK,
+                                                                      0xF, false));
+    }
+
+    // Inclusive left-to-right scan of the affine maps (a_l, b_l) over the 64
+    // lanes of a wavefront, composing x -> a*x + b.  The six stages of the
+    // multiplier scan do not depend on the row, so they are computed once
+    // (ScanStagesForward) and each row only scans its offsets (ScanRowForward).
+    __device__ __forceinline__ float ScanStagesForward(float a, float (&stage)[6])
+    {
+        stage[0] = a;
+        a = Dpp<0x111>(1.0f, a) * a;
+        stage[1] = a;
+        a = Dpp<0x112>(1.0f, a) * a;
+        stage[2] = a;
+        a = Dpp<0x114>(1.0f, a) * a;
+        stage[3] = a;
+        a = Dpp<0x118>(1.0f, a) * a;
+        stage[4] = a;
+        a = Dpp<0x142, 0xA>(1.0f, a) * a;
+        stage[5] = a;
+        a = Dpp<0x143, 0xC>(1.0f, a) * a;
+        return a;
+    }
+
+    __device__ __forceinline__ float ScanRowForward(float b, const float (&stage)[6])
+    {
+        b = fmaf(stage[0], Dpp<0x111>(0.0f, b), b);
+        b = fmaf(stage[1], Dpp<0x112>(0.0f, b), b);
+        b = fmaf(stage[2], Dpp<0x114>(0.0f, b), b);
+        b = fmaf(stage[3], Dpp<0x118>(0.0f, b), b);
+        b = fmaf(stage[4], Dpp<0x142, 0xA>(0.0f, b), b);
+        b = fmaf(stage[5], Dpp<0x143, 0xC>(0.0f, b), b);
+        return b;
+    }
+
+    // Right-to-left counterparts (Kogge-Stone with shuffles down).
+    __device__ __forceinline__ float ScanStagesBackward(float a, int lane, float (&stage)[6])
+    {
+#pragma unroll
+        for (int k = 0; k < 6; k++)
         {
-            if constexpr (CPT % 4 == 0)
+            const int o = 1 << k;
+            stage[k] = a;
+            const float d = __shfl_down(a, o);
+            a = (lane + o < 64) ? d * a : a;
+        }
+        return a;
+    }
+
+    __device__ __forceinline__ float ScanRowBackward(float b, int lane, const float (&stage)[6])
+    {
+#pragma unroll
+        for (int k = 0; k < 6; k++)
+        {
+            const int o = 1 << k;
+            const float d = __shfl_down(b, o);
+            b = (lane + o < 64) ? fmaf(stage[k], d, b) : b;
+        }
+        return b;
+    }
+
+    // ---- register-tile helpers ------------------------------------------------------
+
+    template <int CPT, bool VEC>
+    __device__ __forceinline__ void LoadRow(const float *__restrict__ row, int y0, int Y, float (&d)[CPT])
+    {
+        if constexpr (VEC)
+        {
+            if (y0 + CPT <= Y)
+            {
+                if constexpr (CPT % 4 == 0)
+                {
+#pragma unroll
+                    for (int q = 0; q < CPT / 4; q++)
+                    {
+                        const float4 t = *reinterpret_cast<const float4 *>(row + y0 + 4 * q);
+                        d[4 * q + 0] = t.x;
+                        d[4 * q + 1] = t.y;
+                        d[4 * q + 2] = t.z;
+                        d[4 * q + 3] = t.w;
+                    }
+                }
+                else if constexpr (CPT == 2)
+                {
+                    const float2 t = *reinterpret_cast<const float2 *>(row + y0);
+                    d[0] = t.x;
+                    d[1] = t.y;
+                }
+                else
+                {
+                    d[0] = row[y0];
+                }
+            }
+            else
             {
 #pragma unroll
-                for (int q = 0; q < CPT / 4; q++)
+                for (int c = 0; c < CPT; c++)
                 {
-                    float4 t = *reinterpret_cast<const float4 *>(row + y0 + 4 * q);
-                    d[4 * q + 0] = t.x;
-                    d[4 * q + 1] = t.y;
-                    d[4 * q + 2] = t.z;
-                    d[4 * q + 3] = t.w;
+                    d[c] = 0.0f;
                 }
-                return;
-            }
-            else if constexpr (CPT == 2)
-            {
-                float2 t = *reinterpret_cast<const float2 *>(row + y0);
-                d[0] = t.x;
-                d[1] = t.y;
-                return;
             }
         }
-#pragma unroll
-        for (int c = 0; c < CPT; c++)
+        else
         {
-            d[c] = (y0 + c < Y) ? row[y0 + c] : 0.0f;
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                d[c] = (y0 + c < Y) ? row[y0 + c] : 0.0f;
+            }
         }
     }
 
-    template <int CPT>
-    __device__ __forceinline__ void StoreRow(float *__restrict__ row, int y0, int Y, bool vec, const float (&d)[CPT])
+    template <int CPT, bool VEC>
+    __device__ __forceinline__ void StoreRow(float *__restrict__ row, int y0, int Y, const float (&d)[CPT])
     {
-        if (vec && y0 + CPT <= Y)
+        if constexpr (VEC)
         {
-            if constexpr (CPT % 4 == 0)
+            if (y0 + CPT <= Y)
             {
-#pragma unroll
-                for (int q = 0; q < CPT / 4; q++)
+                if constexpr (CPT % 4 == 0)
                 {
-                    *reinterpret_cast<float4 *>(row + y0 + 4 * q) =
-                        make_float4(d[4 * q + 0], d[4 * q + 1], d[4 * q + 2], d[4 * q + 3]);
+#pragma unroll
+                    for (int q = 0; q < CPT / 4; q++)
+                    {
+                        *reinterpret_cast<float4 *>(row + y0 + 4 * q) =
+                            make_float4(d[4 * q + 0], d[4 * q + 1], d[4 * q + 2], d[4 * q + 3]);
+                    }
                 }
-                return;
-            }
-            else if constexpr (CPT == 2)
-            {
-                *reinterpret_cast<float2 *>(row + y0) = make_float2(d[0], d[1]);
-                return;
+                else if constexpr (CPT == 2)
+                {
+                    *reinterpret_cast<float2 *>(row + y0) = make_float2(d[0], d[1]);
+                }
+                else
+                {
+                    row[y0] = d[0];
+                }
             }
         }
-#pragma unroll
-        for (int c = 0; c < CPT; c++)
+        else
         {
-            if (y0 + c < Y)
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
             {
-                row[y0 + c] = d[c];
+                if (y0 + c < Y)
+                {
+                    row[y0 + c] = d[c];
+                }
             }
         }
     }
 
-    struct LdsEdges
+    struct StripLds
     {
         float *left;  // [kMaxWaves][S] first column of each wavefront
         float *right; // [kMaxWaves][S] last column of each wavefront
         float *first; // [S] column y = 0
         float *last;  // [S] column y = Y-1
+        float *tot;   // [kMaxWaves][S] wavefront scan totals
+        float *aw;    // [kMaxWaves] wavefront multiplier products
+        float *ends;  // [2][S] first / last solution values of each row (periodic y)
     };
 
     // Sweep A right-hand side in place: v <- v + hr*(dv/dy+ + dv/dy-) plus the
     // Dirichlet implicit term r*bc on rows 0 and X-1.  Rows past X and columns
-    // past Y become 0.  ExplicitDiffusionY2ADI (src/Pde/ADIequations.cpp:34-47,
+    // past Y become 0.  An off-grid neighbour is replaced by a ghost value g
+    // with g - c equal to the reference's difference: the wrapped cell
+    // (periodic), 2*bc - c (Dirichlet, whose difference is (bc - c)*2) or c
+    // (zero flux).  ExplicitDiffusionY2ADI (src/Pde/ADIequations.cpp:34-47,
     // ADIequations.h:79-94) + ImplicitBoundarySource (ADIequations.h:198-220).
-    template <int CPT, int S>
+    template <int CPT, int S, bool VEC>
     __device__ __forceinline__ void SweepARhs(const AdiArgs &a, int s0, int y0, int tid, float (&v)[S][CPT],
-                                              const LdsEdges &e)
+                                              const StripLds &L)
     {
         const int lane = tid & 63;
         const int wave = tid >> 6;
@@ -183,7 +284,7 @@ namespace
 #pragma unroll
             for (int r = 0; r < S; r++)
             {
-                e.left[wave * S + r] = v[r][0];
+                L.left[wave * S + r] = v[r][0];
             }
         }
         if (lane == 63)
@@ -191,7 +292,7 @@ namespace
 #pragma unroll
             for (int r = 0; r < S; r++)
             {
-                e.right[wave * S + r] = v[r][CPT - 1];
+                L.right[wave * S + r] = v[r][CPT - 1];
             }
         }
         if (tid == 0)
@@ -199,7 +300,7 @@ namespace
 #pragma unroll
             for (int r = 0; r < S; r++)
             {
-                e.first[r] = v[r][0];
+                L.first[r] = v[r][0];
             }
         }
         if (tid == lastT)
@@ -212,18 +313,19 @@ namespace
                 {
                     if (c == lastC)
                     {
-                        e.last[r] = v[r][c];
+                        L.last[r] = v[r][c];
                     }
                 }
             }
         }
         __syncthreads();
+        const bool dirY = a.dirichlet && !a.wrapY;
 #pragma unroll
         for (int r = 0; r < S; r++)
         {
             const int x = s0 + r;
-            float left = __shfl_up(v[r][CPT - 1], 1);
-            float right = __shfl_down(v[r][0], 1);
+            float left = Dpp<0x138>(0.0f, v[r][CPT - 1]);
+            float right = Dpp<0x130>(0.0f, v[r][0]);
             if (x >= a.X)
             {
 #pragma unroll
@@ -235,187 +337,236 @@ namespace
             }
             if (lane == 0 && wave > 0)
             {
-                left = e.right[(wave - 1) * S + r];
+                left = L.right[(wave - 1) * S + r];
             }
             if (lane == 63 && wave + 1 < nWaves)
             {
-                right = e.left[(wave + 1) * S + r];
+                right = L.left[(wave + 1) * S + r];
             }
-            const float firstV = e.first[r];
-            const float lastV = e.last[r];
-            float bcLo = 0.0f;
-            float bcHi = 0.0f;
-            if (a.dirichlet && !a.wrapY)
+            if (tid == 0)
             {
-                bcLo = a.bcYlo[x];
-                bcHi = a.bcYhi[x];
+                const float c0 = v[r][0];
+                left = a.wrapY ? L.last[r] : (dirY ? 2.0f * a.bcYlo[x] - c0 : c0);
             }
-            const bool srcRowLo = a.dirichlet && !a.wrapX && x == 0;
-            const bool srcRowHi = a.dirichlet && !a.wrapX && x == a.X - 1;
+            float ghostHi = 0.0f;
+            if (tid == lastT)
+            {
+                float cl = 0.0f;
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    cl = (c == lastC) ? v[r][c] : cl;
+                }
+                ghostHi = a.wrapY ? L.first[r] : (dirY ? 2.0f * a.bcYhi[x] - cl : cl);
+                if constexpr (VEC)
+                {
+                    right = ghostHi;
+                }
+            }
+            const bool srcRow = a.dirichlet && !a.wrapX && (x == 0 || x == a.X - 1);
             float prev = left;
 #pragma unroll
             for (int c = 0; c < CPT; c++)
             {
-                const int y = y0 + c;
                 const float cur = v[r][c];
-                const float nxt = (c + 1 < CPT) ? v[r][c + 1] : right;
-                float rhs = 0.0f;
-                if (y < a.Y)
+                float nxt = (c + 1 < CPT) ? v[r][c + 1] : right;
+                if constexpr (!VEC)
                 {
-                    float dp;
-                    float dm;
-                    if (y + 1 < a.Y)
+                    if (tid == lastT && c == lastC)
                     {
-                        dp = nxt - cur;
-                    }
-                    else
-                    {
-                        dp = a.wrapY ? (firstV - cur) : (a.dirichlet ? (bcHi - cur) * 2.0f : 0.0f);
-                    }
-                    if (y > 0)
-                    {
-                        dm = prev - cur;
-                    }
-                    else
-                    {
-                        dm = a.wrapY ? (lastV - cur) : (a.dirichlet ? (bcLo - cur) * 2.0f : 0.0f);
-                    }
-                    rhs = cur + (dp + dm) * a.hr;
-                    if (srcRowLo || srcRowHi)
-                    {
-                        float src = 0.0f;
-                        if (srcRowLo)
-                        {
-                            src += a.bcXlo[y];
-                        }
-                        if (srcRowHi)
-                        {
-                            src += a.bcXhi[y];
-                        }
-                        rhs += src * a.r;
+                        nxt = ghostHi;
                     }
                 }
+                float rhs = fmaf((nxt - cur) + (prev - cur), a.hr, cur);
+                if (srcRow)
+                {
+                    float src = 0.0f;
+                    if (x == 0)
+                    {
+                        src += a.bcXlo[min(y0 + c, a.Y - 1)];
+                    }
+                    if (x == a.X - 1)
+                    {
+                        src += a.bcXhi[min(y0 + c, a.Y - 1)];
+                    }
+                    rhs = fmaf(src, a.r, rhs);
+                }
                 prev = cur;
-                v[r][c] = rhs;
+                v[r][c] = (y0 + c < a.Y) ? rhs : 0.0f;
             }
         }
     }
 
     // Zero-carry local solve of sweep A on the strip (v holds the RHS and is
     // consumed) and write of the strip's two reduction values per column.
-    template <int CPT, int S>
+    template <int CPT, int S, bool VEC>
     __device__ __forceinline__ void ReduceA(const AdiArgs &a, int p, int s0, int y0, float (&v)[S][CPT])
     {
-        float outB[CPT];
-        float outD[CPT];
+        float acc[CPT];
 #pragma unroll
         for (int c = 0; c < CPT; c++)
         {
-            float acc = 0.0f;
+            acc[c] = 0.0f;
+        }
 #pragma unroll
-            for (int r = 0; r < S; r++)
-            {
-                acc = a.xfw[s0 + r] * acc + v[r][c];
-                v[r][c] = acc;
-            }
-            outB[c] = acc;
-            acc = 0.0f;
+        for (int r = 0; r < S; r++)
+        {
+            const float f = a.xfw[s0 + r];
 #pragma unroll
-            for (int r = S - 1; r >= 0; r--)
+            for (int c = 0; c < CPT; c++)
             {
-                acc = a.xbw[s0 + r] * acc + a.xib[s0 + r] * v[r][c];
+                acc[c] = fmaf(f, acc[c], v[r][c]);
+                v[r][c] = acc[c];
             }
-            outD[c] = acc;
         }
         const size_t base = static_cast<size_t>(p) * a.Y;
         const size_t plane = static_cast<size_t>(a.P) * a.Y;
-        StoreRow<CPT>(a.red + base, y0, a.Y, a.vecOk != 0, outB);
-        StoreRow<CPT>(a.red + plane + base, y0, a.Y, a.vecOk != 0, outD);
+        StoreRow<CPT, VEC>(a.red + base, y0, a.Y, acc);
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            acc[c] = 0.0f;
+        }
+#pragma unroll
+        for (int r = S - 1; r >= 0; r--)
+        {
+            const float bwr = a.xbw[s0 + r];
+            const float ibr = a.xib[s0 + r];
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                acc[c] = fmaf(bwr, acc[c], ibr * v[r][c]);
+            }
+        }
+        StoreRow<CPT, VEC>(a.red + plane + base, y0, a.Y, acc);
     }
 
     // Finishes sweep A on the strip from the carries (v: RHS in, u* out) and
     // returns u* of the rows just above (s0-1) and below (s0+S) the strip.
-    template <int CPT, int S>
+    template <int CPT, int S, bool VEC>
     __device__ __forceinline__ void FinishA(const AdiArgs &a, int p, int s0, int y0, float (&v)[S][CPT],
                                             float (&up)[CPT], float (&dn)[CPT])
     {
         const size_t base = static_cast<size_t>(p) * a.Y;
         const size_t plane = static_cast<size_t>(a.P) * a.Y;
-        const bool vec = a.vecOk != 0;
         float cf[CPT];
-        float xb[CPT];
-        float factor[CPT];
-        LoadRow<CPT>(a.carry + base, y0, a.Y, vec, cf);
-        LoadRow<CPT>(a.carry + plane + base, y0, a.Y, vec, xb);
-        LoadRow<CPT>(a.colInfo, y0, a.Y, vec, factor);
-        const bool hasUp = s0 > 0;
-        const bool hasDn = s0 + S < a.X;
-        const float zUp = hasUp ? a.xz[s0 - 1] : 0.0f;
-        const float zDn = hasDn ? a.xz[s0 + S] : 0.0f;
-        const float bwUp = hasUp ? a.xbw[s0 - 1] : 0.0f;
-        const float ibUp = hasUp ? a.xib[s0 - 1] : 0.0f;
+        float acc[CPT];
+        LoadRow<CPT, VEC>(a.carry + base, y0, a.Y, cf);
+        LoadRow<CPT, VEC>(a.carry + plane + base, y0, a.Y, dn);
 #pragma unroll
         for (int c = 0; c < CPT; c++)
         {
-            float acc = cf[c];
+            acc[c] = cf[c];
+        }
 #pragma unroll
-            for (int r = 0; r < S; r++)
-            {
-                acc = a.xfw[s0 + r] * acc + v[r][c];
-                v[r][c] = acc;
-            }
-            acc = xb[c];
+        for (int r = 0; r < S; r++)
+        {
+            const float f = a.xfw[s0 + r];
 #pragma unroll
-            for (int r = S - 1; r >= 0; r--)
+            for (int c = 0; c < CPT; c++)
             {
-                acc = a.xbw[s0 + r] * acc + a.xib[s0 + r] * v[r][c];
-                v[r][c] = acc;
-            }
-            up[c] = bwUp * v[0][c] + ibUp * cf[c] - factor[c] * zUp;
-            dn[c] = xb[c] - factor[c] * zDn;
-#pragma unroll
-            for (int r = 0; r < S; r++)
-            {
-                v[r][c] -= factor[c] * a.xz[s0 + r];
+                acc[c] = fmaf(f, acc[c], v[r][c]);
+                v[r][c] = acc[c];
             }
         }
-    }
-
-    // Sweep B right-hand side in place from u*: v <- v + hr*(dv/dx+ + dv/dx-)
-    // plus the Dirichlet implicit term on columns 0 and Y-1.
-    // ExplicitDiffusionX2ADI (src/Pde/ADIequations.cpp:19-32, ADIequations.h:54-69).
-    template <int CPT, int S>
-    __device__ __forceinline__ void SweepBRhs(const AdiArgs &a, int s0, int y0, float (&v)[S][CPT],
-                                              const float (&up)[CPT], const float (&dn)[CPT])
-    {
-        const bool vec = a.vecOk != 0;
-        float lo[CPT];
-        float hi[CPT];
-        // Off-grid x neighbours: the wrapped row of u* or the Dirichlet value.
-        if (a.wrapX)
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
         {
-            LoadRow<CPT>(a.colInfo + 2 * static_cast<size_t>(a.Y), y0, a.Y, vec, lo);
-            LoadRow<CPT>(a.colInfo + static_cast<size_t>(a.Y), y0, a.Y, vec, hi);
+            acc[c] = dn[c];
         }
-        else if (a.dirichlet)
+#pragma unroll
+        for (int r = S - 1; r >= 0; r--)
         {
-            LoadRow<CPT>(a.bcXlo, y0, a.Y, vec, lo);
-            LoadRow<CPT>(a.bcXhi, y0, a.Y, vec, hi);
+            const float bwr = a.xbw[s0 + r];
+            const float ibr = a.xib[s0 + r];
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                acc[c] = fmaf(bwr, acc[c], ibr * v[r][c]);
+                v[r][c] = acc[c];
+            }
+        }
+        if (s0 > 0)
+        {
+            const float bwUp = a.xbw[s0 - 1];
+            const float ibUp = a.xib[s0 - 1];
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                up[c] = fmaf(bwUp, v[0][c], ibUp * cf[c]);
+            }
         }
         else
         {
 #pragma unroll
             for (int c = 0; c < CPT; c++)
             {
-                lo[c] = 0.0f;
-                hi[c] = 0.0f;
+                up[c] = 0.0f;
             }
         }
-        float prevRow[CPT];
-#pragma unroll
-        for (int c = 0; c < CPT; c++)
+        if (a.wrapX)
         {
-            prevRow[c] = up[c];
+            float factor[CPT];
+            LoadRow<CPT, VEC>(a.colInfo, y0, a.Y, factor);
+            const float zUp = (s0 > 0) ? a.xz[s0 - 1] : 0.0f;
+            const float zDn = (s0 + S < a.X) ? a.xz[s0 + S] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+                const float z = a.xz[s0 + r];
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    v[r][c] = fmaf(-factor[c], z, v[r][c]);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                up[c] = fmaf(-factor[c], zUp, up[c]);
+                dn[c] = fmaf(-factor[c], zDn, dn[c]);
+            }
+        }
+    }
+
+    // Sweep B right-hand side in place from u*: v <- v + hr*(dv/dx+ + dv/dx-)
+    // plus the Dirichlet implicit term on columns 0 and Y-1, with the same
+    // ghost-value treatment of off-grid neighbours as sweep A.
+    // ExplicitDiffusionX2ADI (src/Pde/ADIequations.cpp:19-32, ADIequations.h:54-69).
+    template <int CPT, int S, bool VEC>
+    __device__ __forceinline__ void SweepBRhs(const AdiArgs &a, int s0, int y0, int tid, float (&v)[S][CPT],
+                                              const float (&up)[CPT], const float (&dn)[CPT])
+    {
+        const int lastT = (a.Y - 1) / CPT;
+        const int lastC = (a.Y - 1) % CPT;
+        const bool dirX = a.dirichlet && !a.wrapX;
+        const bool dirY = a.dirichlet && !a.wrapY;
+        float prevRow[CPT];
+        if (s0 == 0)
+        {
+            // Ghost row above row 0.
+            float g[CPT];
+            if (a.wrapX)
+            {
+                LoadRow<CPT, VEC>(a.colInfo + 2 * static_cast<size_t>(a.Y), y0, a.Y, g);
+            }
+            else if (dirX)
+            {
+                LoadRow<CPT, VEC>(a.bcXlo, y0, a.Y, g);
+            }
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                prevRow[c] = a.wrapX ? g[c] : (dirX ? fmaf(2.0f, g[c], -v[0][c]) : v[0][c]);
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                prevRow[c] = up[c];
+            }
         }
 #pragma unroll
         for (int r = 0; r < S; r++)
@@ -425,61 +576,58 @@ namespace
             {
                 break;
             }
+            float next[CPT];
+            if (x + 1 < a.X)
+            {
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    next[c] = (r + 1 < S) ? v[r + 1 < S ? r + 1 : r][c] : dn[c];
+                }
+            }
+            else
+            {
+                float g[CPT];
+                if (a.wrapX)
+                {
+                    LoadRow<CPT, VEC>(a.colInfo + static_cast<size_t>(a.Y), y0, a.Y, g);
+                }
+                else if (dirX)
+                {
+                    LoadRow<CPT, VEC>(a.bcXhi, y0, a.Y, g);
+                }
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    next[c] = a.wrapX ? g[c] : (dirX ? fmaf(2.0f, g[c], -v[r][c]) : v[r][c]);
+                }
+            }
             float srcLo = 0.0f;
             float srcHi = 0.0f;
-            const bool ySrc = a.dirichlet && !a.wrapY;
-            if (ySrc)
+            if (dirY)
             {
-                srcLo = a.bcYlo[x];
-                srcHi = a.bcYhi[x];
+                srcLo = (tid == 0) ? a.bcYlo[x] * a.r : 0.0f;
+                srcHi = (tid == lastT) ? a.bcYhi[x] * a.r : 0.0f;
             }
 #pragma unroll
             for (int c = 0; c < CPT; c++)
             {
-                const int y = y0 + c;
                 const float cur = v[r][c];
-                float dDown;
-                float dUp;
-                if (x + 1 < a.X)
+                float rhs = fmaf((next[c] - cur) + (prevRow[c] - cur), a.hr, cur);
+                if (dirY)
                 {
-                    dDown = ((r + 1 < S) ? v[r + 1][c] : dn[c]) - cur;
-                }
-                else
-                {
-                    dDown = a.wrapX ? (hi[c] - cur) : (a.dirichlet ? (hi[c] - cur) * 2.0f : 0.0f);
-                }
-                if (x > 0)
-                {
-                    dUp = prevRow[c] - cur;
-                }
-                else
-                {
-                    dUp = a.wrapX ? (lo[c] - cur) : (a.dirichlet ? (lo[c] - cur) * 2.0f : 0.0f);
-                }
-                float rhs = cur + (dDown + dUp) * a.hr;
-                if (ySrc)
-                {
-                    float src = 0.0f;
-                    if (y == 0)
-                    {
-                        src += srcLo;
-                    }
-                    if (y == a.Y - 1)
-                    {
-                        src += srcHi;
-                    }
-                    rhs += src * a.r;
+                    rhs += (c == 0) ? srcLo : 0.0f;
+                    rhs += (c == lastC) ? srcHi : 0.0f;
                 }
                 prevRow[c] = cur;
-                v[r][c] = (y < a.Y) ? rhs : 0.0f;
+                v[r][c] = (y0 + c < a.Y) ? rhs : 0.0f;
             }
         }
     }
 
     // Sweep B solve of every row of the strip across the workgroup.
     template <int CPT, int S>
-    __device__ __forceinline__ void SolveB(const AdiArgs &a, int y0, int tid, float (&v)[S][CPT], float *sTot,
-                                           float *sAw, float *sEnds)
+    __device__ __forceinline__ void SolveB(const AdiArgs &a, int y0, int tid, float (&v)[S][CPT], const StripLds &L)
     {
         const int lane = tid & 63;
         const int wave = tid >> 6;
@@ -502,27 +650,12 @@ namespace
         {
             aSeg *= fw[c];
         }
-        float aStage[6];
-        float aInc = aSeg;
-#pragma unroll
-        for (int k = 0; k < 6; k++)
-        {
-            const int o = 1 << k;
-            aStage[k] = aInc;
-            float aUp = __shfl_up(aInc, o);
-            if (lane >= o)
-            {
-                aInc = aUp * aInc;
-            }
-        }
-        float aEx = __shfl_up(aInc, 1);
-        if (lane == 0)
-        {
-            aEx = 1.0f;
-        }
+        float stage[6];
+        const float aInc = ScanStagesForward(aSeg, stage);
+        const float aEx = Dpp<0x138>(1.0f, aInc);
         if (lane == 63)
         {
-            sAw[wave] = aInc;
+            L.aw[wave] = aInc;
         }
         float bEx[S];
 #pragma unroll
@@ -532,39 +665,34 @@ namespace
 #pragma unroll
             for (int c = 0; c < CPT; c++)
             {
-                b = fw[c] * b + v[r][c];
+                b = fmaf(fw[c], b, v[r][c]);
             }
-#pragma unroll
-            for (int k = 0; k < 6; k++)
-            {
-                const int o = 1 << k;
-                float bUp = __shfl_up(b, o);
-                if (lane >= o)
-                {
-                    b = aStage[k] * bUp + b;
-                }
-            }
-            float be = __shfl_up(b, 1);
-            bEx[r] = (lane == 0) ? 0.0f : be;
+            b = ScanRowForward(b, stage);
+            bEx[r] = Dpp<0x138>(0.0f, b);
             if (lane == 63)
             {
-                sTot[wave * S + r] = b;
+                L.tot[wave * S + r] = b;
             }
         }
         __syncthreads();
+        // Lane r folds the totals of the wavefronts before this one for row r.
+        float cwLane = 0.0f;
+        if (lane < S)
+        {
+            for (int w = 0; w < wave; w++)
+            {
+                cwLane = fmaf(L.aw[w], cwLane, L.tot[w * S + lane]);
+            }
+        }
 #pragma unroll
         for (int r = 0; r < S; r++)
         {
-            float cw = 0.0f;
-            for (int w = 0; w < wave; w++)
-            {
-                cw = sAw[w] * cw + sTot[w * S + r];
-            }
-            float acc = aEx * cw + bEx[r];
+            const float cw = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cwLane), r));
+            float acc = fmaf(aEx, cw, bEx[r]);
 #pragma unroll
             for (int c = 0; c < CPT; c++)
             {
-                acc = fw[c] * acc + v[r][c];
+                acc = fmaf(fw[c], acc, v[r][c]);
                 v[r][c] = acc;
             }
         }
@@ -577,26 +705,12 @@ namespace
         {
             aSeg *= bw[c];
         }
-        aInc = aSeg;
-#pragma unroll
-        for (int k = 0; k < 6; k++)
-        {
-            const int o = 1 << k;
-            aStage[k] = aInc;
-            float aDn = __shfl_down(aInc, o);
-            if (lane + o < 64)
-            {
-                aInc = aDn * aInc;
-            }
-        }
-        aEx = __shfl_down(aInc, 1);
-        if (lane == 63)
-        {
-            aEx = 1.0f;
-        }
+        const float aIncB = ScanStagesBackward(aSeg, lane, stage);
+        float aExB = __shfl_down(aIncB, 1);
+        aExB = (lane == 63) ? 1.0f : aExB;
         if (lane == 0)
         {
-            sAw[wave] = aInc;
+            L.aw[wave] = aIncB;
         }
 #pragma unroll
         for (int r = 0; r < S; r++)
@@ -605,48 +719,43 @@ namespace
 #pragma unroll
             for (int c = CPT - 1; c >= 0; c--)
             {
-                b = bw[c] * b + ib[c] * v[r][c];
+                b = fmaf(bw[c], b, ib[c] * v[r][c]);
             }
-#pragma unroll
-            for (int k = 0; k < 6; k++)
-            {
-                const int o = 1 << k;
-                float bDn = __shfl_down(b, o);
-                if (lane + o < 64)
-                {
-                    b = aStage[k] * bDn + b;
-                }
-            }
-            float be = __shfl_down(b, 1);
+            b = ScanRowBackward(b, lane, stage);
+            const float be = __shfl_down(b, 1);
             bEx[r] = (lane == 63) ? 0.0f : be;
             if (lane == 0)
             {
-                sTot[wave * S + r] = b;
+                L.tot[wave * S + r] = b;
             }
         }
         __syncthreads();
+        cwLane = 0.0f;
+        if (lane < S)
+        {
+            for (int w = nWaves - 1; w > wave; w--)
+            {
+                cwLane = fmaf(L.aw[w], cwLane, L.tot[w * S + lane]);
+            }
+        }
         const int lastT = (a.Y - 1) / CPT;
         const int lastC = (a.Y - 1) % CPT;
 #pragma unroll
         for (int r = 0; r < S; r++)
         {
-            float cw = 0.0f;
-            for (int w = nWaves - 1; w > wave; w--)
-            {
-                cw = sAw[w] * cw + sTot[w * S + r];
-            }
-            float acc = aEx * cw + bEx[r];
+            const float cw = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cwLane), r));
+            float acc = fmaf(aExB, cw, bEx[r]);
 #pragma unroll
             for (int c = CPT - 1; c >= 0; c--)
             {
-                acc = bw[c] * acc + ib[c] * v[r][c];
+                acc = fmaf(bw[c], acc, ib[c] * v[r][c]);
                 v[r][c] = acc;
             }
             if (a.wrapY)
             {
                 if (tid == 0)
                 {
-                    sEnds[r] = v[r][0];
+                    L.ends[r] = v[r][0];
                 }
                 if (tid == lastT)
                 {
@@ -655,7 +764,7 @@ namespace
                     {
                         if (c == lastC)
                         {
-                            sEnds[S + r] = v[r][c];
+                            L.ends[S + r] = v[r][c];
                         }
                     }
                 }
@@ -674,11 +783,11 @@ namespace
 #pragma unroll
             for (int r = 0; r < S; r++)
             {
-                const float factor = (sEnds[r] + sEnds[S + r] * a.yvEnd) * a.yInv;
+                const float factor = (L.ends[r] + L.ends[S + r] * a.yvEnd) * a.yInv;
 #pragma unroll
                 for (int c = 0; c < CPT; c++)
                 {
-                    v[r][c] -= factor * z[c];
+                    v[r][c] = fmaf(-factor, z[c], v[r][c]);
                 }
             }
         }
@@ -688,7 +797,8 @@ namespace
     // MODE 1: plain tick in place, then the reduction of u' for the next tick.
     // MODE 2: tick with decay: u' = ADI(u) - k*u (no reduction; the next tick
     //         starts with a MODE 0 pass after point sources are added).
-    template <int CPT, int S, int MODE>
+    // VEC: yDim is a multiple of CPT, so rows move as 16-byte vectors.
+    template <int CPT, int S, int MODE, bool VEC>
     __global__ __launch_bounds__(1024) void AdiStrip(AdiArgs a, float *__restrict__ u)
     {
         __shared__ float sLeft[kMaxWaves * S];
@@ -698,13 +808,12 @@ namespace
         __shared__ float sTot[kMaxWaves * S];
         __shared__ float sAw[kMaxWaves];
         __shared__ float sEnds[2 * S];
-        const LdsEdges edges{sLeft, sRight, sFirst, sLast};
+        const StripLds L{sLeft, sRight, sFirst, sLast, sTot, sAw, sEnds};
 
         const int p = blockIdx.x;
         const int s0 = p * S;
         const int tid = threadIdx.x;
         const int y0 = tid * CPT;
-        const bool vec = a.vecOk != 0;
 
         float v[S][CPT];
 #pragma unroll
@@ -712,7 +821,7 @@ namespace
         {
             if (s0 + r < a.X)
             {
-                LoadRow<CPT>(u + static_cast<size_t>(s0 + r) * a.Y, y0, a.Y, vec, v[r]);
+                LoadRow<CPT, VEC>(u + static_cast<size_t>(s0 + r) * a.Y, y0, a.Y, v[r]);
             }
             else
             {
@@ -723,49 +832,179 @@ namespace
                 }
             }
         }
-        SweepARhs<CPT, S>(a, s0, y0, tid, v, edges);
+        SweepARhs<CPT, S, VEC>(a, s0, y0, tid, v, L);
         if constexpr (MODE == 0)
         {
-            ReduceA<CPT, S>(a, p, s0, y0, v);
+            ReduceA<CPT, S, VEC>(a, p, s0, y0, v);
             return;
         }
-        float up[CPT];
-        float dn[CPT];
-        FinishA<CPT, S>(a, p, s0, y0, v, up, dn);
-        SweepBRhs<CPT, S>(a, s0, y0, v, up, dn);
-        SolveB<CPT, S>(a, y0, tid, v, sTot, sAw, sEnds);
-#pragma unroll
-        for (int r = 0; r < S; r++)
+        else
         {
-            if (s0 + r < a.X)
+            float up[CPT];
+            float dn[CPT];
+            FinishA<CPT, S, VEC>(a, p, s0, y0, v, up, dn);
+            SweepBRhs<CPT, S, VEC>(a, s0, y0, tid, v, up, dn);
+            SolveB<CPT, S>(a, y0, tid, v, L);
+#pragma unroll
+            for (int r = 0; r < S; r++)
             {
-                float *row = u + static_cast<size_t>(s0 + r) * a.Y;
-                if constexpr (MODE == 2)
+                if (s0 + r < a.X)
                 {
-                    float old[CPT];
-                    LoadRow<CPT>(row, y0, a.Y, vec, old);
-#pragma unroll
-                    for (int c = 0; c < CPT; c++)
+                    float *row = u + static_cast<size_t>(s0 + r) * a.Y;
+                    if constexpr (MODE == 2)
                     {
-                        v[r][c] += old[c] * a.negDecay;
+                        float old[CPT];
+                        LoadRow<CPT, VEC>(row, y0, a.Y, old);
+#pragma unroll
+                        for (int c = 0; c < CPT; c++)
+                        {
+                            v[r][c] = fmaf(old[c], a.negDecay, v[r][c]);
+                        }
                     }
+                    StoreRow<CPT, VEC>(row, y0, a.Y, v[r]);
                 }
-                StoreRow<CPT>(row, y0, a.Y, vec, v[r]);
             }
-        }
-        if constexpr (MODE == 1)
-        {
-            __syncthreads();
-            SweepARhs<CPT, S>(a, s0, y0, tid, v, edges);
-            ReduceA<CPT, S>(a, p, s0, y0, v);
+            if constexpr (MODE == 1)
+            {
+                __syncthreads();
+                SweepARhs<CPT, S, VEC>(a, s0, y0, tid, v, L);
+                ReduceA<CPT, S, VEC>(a, p, s0, y0, v);
+            }
         }
     }
 
     // Scan of the strip reductions along each column (sweep A across strips).
+    // Block: 256 threads = 16 columns x 16 segments; lane group k of 16 lanes
+    // takes column y0 + (lane & 15) and segment g = 4*wave + (lane >> 4), which
+    // owns strips [g*Q, (g+1)*Q) with Q <= QMAX.  A thread first loads its
+    // segment's reduction values into registers (all loads in flight
+    // together), folds them, exchanges segment totals through LDS, then runs
+    // the recurrences from registers.  Writes the carry-in values of every
+    // strip and, per column, the periodic Sherman-Morrison factor and the
+    // wrapped rows of u*.
+    constexpr int kScanCols = 16;
+    constexpr int kScanSegs = 16;
+
+    template <int QMAX>
+    __global__ __launch_bounds__(kScanCols *kScanSegs) void AdiStripScan(AdiArgs a, int32_t Q)
+    {
+        __shared__ float sB[kScanSegs][kScanCols];
+        __shared__ float sA[kScanSegs];
+        __shared__ float sB2[kScanSegs][kScanCols];
+        __shared__ float sA2[kScanSegs];
+        __shared__ float sCf[kScanCols];
+        __shared__ float sX0[kScanCols];
+        const int col = threadIdx.x & (kScanCols - 1);
+        const int g = threadIdx.x / kScanCols;
+        const int y = blockIdx.x * kScanCols + col;
+        const bool active = y < a.Y;
+        const int p0 = g * Q;
+        const int cnt = max(0, min(a.P, p0 + Q) - p0);
+        const size_t plane = static_cast<size_t>(a.P) * a.Y;
+        const float *B = a.red;
+        const float *D = a.red + plane;
+        float *cfIn = a.carry;
+        float *xbIn = a.carry + plane;
+
+        float bv[QMAX];
+        float dv[QMAX];
+        float cv[QMAX];
+#pragma unroll
+        for (int q = 0; q < QMAX; q++)
+        {
+            const bool ok = active && q < cnt;
+            const size_t k = ok ? static_cast<size_t>(p0 + q) * a.Y + y : 0;
+            bv[q] = ok ? B[k] : 0.0f;
+            dv[q] = ok ? D[k] : 0.0f;
+        }
+        float acc = 0.0f;
+        float aSeg = 1.0f;
+#pragma unroll
+        for (int q = 0; q < QMAX; q++)
+        {
+            const float A = (q < cnt) ? a.stripA[p0 + q] : 1.0f;
+            acc = fmaf(A, acc, bv[q]);
+            aSeg *= A;
+        }
+        sB[g][col] = acc;
+        if (col == 0)
+        {
+            sA[g] = aSeg;
+        }
+        __syncthreads();
+        float cf = 0.0f;
+        for (int w = 0; w < g; w++)
+        {
+            cf = fmaf(sA[w], cf, sB[w][col]);
+        }
+#pragma unroll
+        for (int q = 0; q < QMAX; q++)
+        {
+            cv[q] = cf;
+            const float A = (q < cnt) ? a.stripA[p0 + q] : 1.0f;
+            cf = fmaf(A, cf, bv[q]);
+        }
+        if (cnt > 0 && p0 + cnt == a.P)
+        {
+            sCf[col] = cf;
+        }
+        acc = 0.0f;
+        aSeg = 1.0f;
+#pragma unroll
+        for (int q = QMAX - 1; q >= 0; q--)
+        {
+            const bool in = q < cnt;
+            const float C = in ? a.stripC[p0 + q] : 1.0f;
+            const float off = in ? fmaf(a.stripE[p0 + q], cv[q], dv[q]) : 0.0f;
+            acc = fmaf(C, acc, off);
+            aSeg *= C;
+        }
+        sB2[g][col] = acc;
+        if (col == 0)
+        {
+            sA2[g] = aSeg;
+        }
+        __syncthreads();
+        float xb = 0.0f;
+        for (int w = kScanSegs - 1; w > g; w--)
+        {
+            xb = fmaf(sA2[w], xb, sB2[w][col]);
+        }
+#pragma unroll
+        for (int q = QMAX - 1; q >= 0; q--)
+        {
+            const bool in = q < cnt;
+            if (in && active)
+            {
+                const size_t k = static_cast<size_t>(p0 + q) * a.Y + y;
+                xbIn[k] = xb;
+                cfIn[k] = cv[q];
+            }
+            const float C = in ? a.stripC[p0 + q] : 1.0f;
+            const float off = in ? fmaf(a.stripE[p0 + q], cv[q], dv[q]) : 0.0f;
+            xb = fmaf(C, xb, off);
+        }
+        if (g == 0)
+        {
+            sX0[col] = xb;
+        }
+        __syncthreads();
+        if (g == 0 && active)
+        {
+            const float x0 = sX0[col];
+            const float xLast = a.xibLast * sCf[col];
+            const float factor = (x0 + xLast * a.xvEnd) * a.xInv;
+            a.colInfo[y] = factor;
+            a.colInfo[a.Y + y] = x0 - factor * a.xz0;
+            a.colInfo[2 * static_cast<size_t>(a.Y) + y] = xLast - factor * a.xzLast;
+        }
+    }
+
+    // Scan of the strip reductions for any strip count (reloads from memory).
     // Block: 64 columns (lanes) x G wavefronts; wavefront g owns strips
     // [g*Q, (g+1)*Q).  Writes the carry-in values of every strip and, per
     // column, the periodic Sherman-Morrison factor and the wrapped rows of u*.
-    __global__ __launch_bounds__(1024) void AdiStripScan(AdiArgs a, int32_t Q)
+    __global__ __launch_bounds__(1024) void AdiStripScanGeneric(AdiArgs a, int32_t Q)
     {
         __shared__ float sB[kMaxWaves][64];
         __shared__ float sA[kMaxWaves];
@@ -983,7 +1222,13 @@ namespace
         int32_t cpt;
         int32_t s;
         int32_t threads;
+        bool vec;
     };
+
+    bool ShapeCompiled(int32_t cpt, int32_t s)
+    {
+        return (cpt == 1 || cpt == 2 || cpt == 4) ? (s == 4 || s == 8 || s == 16) : (cpt == 8 && (s == 4 || s == 8));
+    }
 
     Shape ChooseShape(int32_t X, int32_t Y)
     {
@@ -995,9 +1240,11 @@ namespace
         }
         int32_t lanes = (Y + sh.cpt - 1) / sh.cpt;
         sh.threads = ((lanes + 63) / 64) * 64;
-        // Largest strip height that keeps CPT*S <= 64 register values and still
-        // gives at least one strip per CU; else the smallest.
-        const int32_t maxS = std::min(16, 64 / sh.cpt);
+        sh.vec = (Y % sh.cpt) == 0;
+        // Largest strip height (at most 8: measured fastest at 4096^2, where
+        // 16-row strips spill registers) that keeps CPT*S <= 64 register values
+        // and still gives at least one strip per CU; else the smallest.
+        const int32_t maxS = std::min(8, 64 / sh.cpt);
         sh.s = 4;
         for (int32_t s = maxS; s >= 4; s /= 2)
         {
@@ -1012,46 +1259,52 @@ namespace
 
     using StripKernel = void (*)(AdiArgs, float *);
 
-    template <int CPT, int S>
+    template <int CPT, int S, bool VEC>
     StripKernel PickMode(int mode)
     {
         switch (mode)
         {
             case 0:
-                return AdiStrip<CPT, S, 0>;
+                return AdiStrip<CPT, S, 0, VEC>;
             case 1:
-                return AdiStrip<CPT, S, 1>;
+                return AdiStrip<CPT, S, 1, VEC>;
             default:
-                return AdiStrip<CPT, S, 2>;
+                return AdiStrip<CPT, S, 2, VEC>;
         }
     }
 
-    StripKernel PickStrip(int cpt, int s, int mode)
+    template <int CPT, int S>
+    StripKernel PickVec(bool vec, int mode)
+    {
+        return vec ? PickMode<CPT, S, true>(mode) : PickMode<CPT, S, false>(mode);
+    }
+
+    StripKernel PickStrip(int cpt, int s, bool vec, int mode)
     {
         switch (cpt * 100 + s)
         {
             case 104:
-                return PickMode<1, 4>(mode);
+                return PickVec<1, 4>(vec, mode);
             case 108:
-                return PickMode<1, 8>(mode);
+                return PickVec<1, 8>(vec, mode);
             case 116:
-                return PickMode<1, 16>(mode);
+                return PickVec<1, 16>(vec, mode);
             case 204:
-                return PickMode<2, 4>(mode);
+                return PickVec<2, 4>(vec, mode);
             case 208:
-                return PickMode<2, 8>(mode);
+                return PickVec<2, 8>(vec, mode);
             case 216:
-                return PickMode<2, 16>(mode);
+                return PickVec<2, 16>(vec, mode);
             case 404:
-                return PickMode<4, 4>(mode);
+                return PickVec<4, 4>(vec, mode);
             case 408:
-                return PickMode<4, 8>(mode);
+                return PickVec<4, 8>(vec, mode);
             case 416:
-                return PickMode<4, 16>(mode);
+                return PickVec<4, 16>(vec, mode);
             case 804:
-                return PickMode<8, 4>(mode);
+                return PickVec<8, 4>(vec, mode);
             case 808:
-                return PickMode<8, 8>(mode);
+                return PickVec<8, 8>(vec, mode);
             default:
                 Fail(HALHIP_ERR_UNSUPPORTED, "adi2d: no kernel for the chosen strip shape");
         }
@@ -1093,6 +1346,18 @@ namespace
     {
         Require(p != nullptr, std::string(caller) + ": null plan");
         Bind(p->ctx);
+    }
+
+    // (Re)allocates the strip reduction / carry buffers for the plan's shape.
+    void AllocateStrips(halhip_adi2d *p)
+    {
+        p->P = (p->X + p->shape.s - 1) / p->shape.s;
+        const size_t plane = static_cast<size_t>(p->P) * static_cast<size_t>(p->Y);
+        p->red.Allocate(p->ctx, 2 * plane);
+        p->carry.Allocate(p->ctx, 2 * plane);
+        p->tableKey.clear();
+        p->reducedValid = false;
+        p->graphs.Clear();
     }
 
     // Builds and uploads the line factors and strip constants for a rate and
@@ -1184,7 +1449,7 @@ namespace
         a.wrapX = wrapX ? 1 : 0;
         a.wrapY = wrapY ? 1 : 0;
         a.dirichlet = dir ? 1 : 0;
-        a.vecOk = (p->Y % cpt == 0) ? 1 : 0;
+        a.vecOk = p->shape.vec ? 1 : 0;
         a.xfw = t + oXfw;
         a.xib = t + oXib;
         a.xbw = t + oXbw;
@@ -1217,7 +1482,7 @@ namespace
 
     void LaunchStrip(halhip_adi2d *p, int mode, const char *name)
     {
-        StripKernel k = PickStrip(p->shape.cpt, p->shape.s, mode);
+        StripKernel k = PickStrip(p->shape.cpt, p->shape.s, p->args.vecOk != 0, mode);
         p->prof.Begin(p->clock.stream, name);
         hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(p->P)), dim3(static_cast<unsigned>(p->shape.threads)), 0,
                            p->clock.stream, p->args, p->field.get());
@@ -1227,11 +1492,21 @@ namespace
 
     void LaunchScan(halhip_adi2d *p)
     {
-        const int32_t G = std::min<int32_t>(kMaxWaves, p->P);
-        const int32_t Q = (p->P + G - 1) / G;
         p->prof.Begin(p->clock.stream, "adi_strip_scan");
-        hipLaunchKernelGGL(AdiStripScan, dim3(static_cast<unsigned>((p->Y + 63) / 64)), dim3(static_cast<unsigned>(64 * G)),
-                           0, p->clock.stream, p->args, Q);
+        const int32_t Qs = (p->P + kScanSegs - 1) / kScanSegs;
+        if (Qs <= 32)
+        {
+            auto kernel = (Qs <= 4) ? AdiStripScan<4> : ((Qs <= 16) ? AdiStripScan<16> : AdiStripScan<32>);
+            hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>((p->Y + kScanCols - 1) / kScanCols)),
+                               dim3(kScanCols * kScanSegs), 0, p->clock.stream, p->args, Qs);
+        }
+        else
+        {
+            const int32_t G = std::min<int32_t>(kMaxWaves, p->P);
+            const int32_t Q = (p->P + G - 1) / G;
+            hipLaunchKernelGGL(AdiStripScanGeneric, dim3(static_cast<unsigned>((p->Y + 63) / 64)),
+                               dim3(static_cast<unsigned>(64 * G)), 0, p->clock.stream, p->args, Q);
+        }
         Check(hipGetLastError(), "AdiStripScan launch");
         p->prof.End(p->clock.stream);
     }
@@ -1345,12 +1620,9 @@ extern "C"
                 p->X = xDim;
                 p->Y = yDim;
                 p->shape = ChooseShape(xDim, yDim);
-                p->P = (xDim + p->shape.s - 1) / p->shape.s;
                 const size_t n = Cells(p);
-                const size_t plane = static_cast<size_t>(p->P) * static_cast<size_t>(yDim);
                 p->field.Allocate(ctx, n);
-                p->red.Allocate(ctx, 2 * plane);
-                p->carry.Allocate(ctx, 2 * plane);
+                AllocateStrips(p);
                 p->colInfo.Allocate(ctx, 3 * static_cast<size_t>(yDim));
                 p->clock.Create();
                 Check(hipMemsetAsync(p->field.get(), 0, n * sizeof(float), p->clock.stream), "hipMemset");
@@ -1505,6 +1777,42 @@ extern "C"
             return rc;
         }
         return halhip_adi2d_sync(p, nullptr);
+    }
+
+    int halhip_adi2d_set_strip_shape(halhip_adi2d *p, int32_t columnsPerThread, int32_t stripRows, int32_t vector)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_adi2d_set_strip_shape");
+                Require(ShapeCompiled(columnsPerThread, stripRows),
+                        "halhip_adi2d_set_strip_shape: (columnsPerThread, stripRows) must be one of "
+                        "{1,2,4}x{4,8,16} or 8x{4,8}");
+                const int32_t lanes = (p->Y + columnsPerThread - 1) / columnsPerThread;
+                Require(lanes <= 1024, "halhip_adi2d_set_strip_shape: yDim needs more than 1024 threads at this width");
+                Require(!vector || p->Y % columnsPerThread == 0,
+                        "halhip_adi2d_set_strip_shape: vector rows need yDim to be a multiple of columnsPerThread");
+                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                p->shape.cpt = columnsPerThread;
+                p->shape.s = stripRows;
+                p->shape.threads = ((lanes + 63) / 64) * 64;
+                p->shape.vec = vector != 0;
+                AllocateStrips(p);
+            });
+    }
+
+    int halhip_adi2d_get_strip_shape(halhip_adi2d *p, int32_t *columnsPerThread, int32_t *stripRows, int32_t *vector)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_adi2d_get_strip_shape");
+                Require(columnsPerThread != nullptr && stripRows != nullptr && vector != nullptr,
+                        "halhip_adi2d_get_strip_shape: null output");
+                *columnsPerThread = p->shape.cpt;
+                *stripRows = p->shape.s;
+                *vector = p->shape.vec ? 1 : 0;
+            });
     }
 
     int halhip_adi2d_set_profiling(halhip_adi2d *p, int32_t on)

@@ -82,6 +82,8 @@ _SIGS = {
     "halhip_adi2d_run_steps": [_vp, _i32, _f64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
     "halhip_adi2d_run_steps_async": [_vp, _i32, _f64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
     "halhip_adi2d_sync": [_vp, _fp],
+    "halhip_adi2d_set_strip_shape": [_vp, _i32, _i32, _i32],
+    "halhip_adi2d_get_strip_shape": [_vp, _ip, _ip, _ip],
     "halhip_adi2d_set_profiling": [_vp, _i32],
     "halhip_adi2d_profile": [_vp, _i32, _ip, ctypes.c_char_p, _i32, _dp, _lp],
     "halhip_agents2d_create": [_vp, _i32, _i32, _i32, ctypes.POINTER(_vp)],
@@ -303,6 +305,14 @@ class Diffusion2D(_Plan):
 class Adi2D(_Plan):
     """ADI plan (IAdiDiffusionBackend2D + resident RunSteps)."""
     _prefix = "adi2d"
+
+    def set_strip_shape(self, columns_per_thread, strip_rows, vector=True):
+        _check(self._fn("set_strip_shape")(self.h, columns_per_thread, strip_rows, 1 if vector else 0))
+
+    def strip_shape(self):
+        c, s, v = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(self._fn("get_strip_shape")(self.h, ctypes.byref(c), ctypes.byref(s), ctypes.byref(v)))
+        return c.value, s.value, bool(v.value)
 
     def __init__(self, ctx, X, Y):
         self.ctx, self.X, self.Y = ctx, X, Y

@@ -165,6 +165,14 @@ int halhip_adi2d_run_steps_async(halhip_adi2d *plan, int32_t steps, double diffR
                                  int32_t wrapY, const halhip_bc *bc);
 int halhip_adi2d_sync(halhip_adi2d *plan, float *elapsedMs);
 
+/* Strip decomposition of the plan: each workgroup holds `stripRows` full rows
+ * with `columnsPerThread` adjacent columns per thread (1, 2 or 4 with 4, 8 or
+ * 16 rows; 8 with 4 or 8 rows); `vector` moves rows as 16-byte vectors and
+ * needs yDim % columnsPerThread == 0.  Results do not depend on the shape
+ * beyond FP32 rounding.  The default is chosen from the grid size. */
+int halhip_adi2d_set_strip_shape(halhip_adi2d *plan, int32_t columnsPerThread, int32_t stripRows, int32_t vector);
+int halhip_adi2d_get_strip_shape(halhip_adi2d *plan, int32_t *columnsPerThread, int32_t *stripRows, int32_t *vector);
+
 /* Per-kernel device time of the last run (HIP events around each launch when
  * profiling is on; profiling is off by default and does not change results).
  * names: comma-separated kernel names; ms/count arrays of length *nKernels. */

@@ -195,3 +195,26 @@ def test_4096_linearity(ctx):
         plan.run_steps(20, 5.0, 0.0, False, False, bc=0.0)
         outs.append(plan.download().astype(np.float64))
     assert rel_l2(outs[2], 2.0 * outs[0] - 3.0 * outs[1]) < 1e-5
+
+
+@pytest.mark.parametrize("Y", [4096, 1000])
+def test_every_strip_shape_matches_oracle(ctx, oracle, Y):
+    X = 150
+    f = np.random.default_rng(Y).random(X * Y).astype(np.float32)
+    ref = {}
+    plan = halhip.Adi2D(ctx, X, Y)
+    for cpt in (1, 2, 4, 8):
+        if (Y + cpt - 1) // cpt > 1024:
+            continue
+        for s in ((4, 8) if cpt == 8 else (4, 8, 16)):
+            for vec in ((True, False) if Y % cpt == 0 else (False,)):
+                plan.set_strip_shape(cpt, s, vec)
+                assert plan.strip_shape() == (cpt, s, vec)
+                for wx, wy, bc in ((0, 0, 0.0), (1, 1, None)):
+                    plan.upload(f)
+                    plan.run_steps(3, 5.0, 0.0, wx, wy, bc=bc)
+                    key = (wx, wy, bc)
+                    if key not in ref:
+                        ref[key] = adi_ref_ticks(oracle, f, 3, 5.0, X, Y, wx, wy, bc)
+                    err = rel_l2(plan.download(), ref[key])
+                    assert err < 1e-5, (cpt, s, vec, key, err)
