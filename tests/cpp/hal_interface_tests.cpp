@@ -1,0 +1,293 @@
+// hal_interface_tests.cpp -- model-side C++ code written against HAL's three
+// strategy interfaces, run on the Hal::Hip backends.  It restates the
+// reference's interface tests (tests/IDiffusionBackend2DTests.cpp,
+// tests/IAgentForceBackend2DTests.cpp) with closed-form and brute-force known
+// answers, and a model-owned tick loop (Diffuse, decay, commit as in
+// PDEGrid2D::Update).  Exit status 0 when every check passes; 77 when no GPU
+// is available (skip, like tests/GpuTestSupport.cpp:9-20).
+#include "Hal/Hip/HipBackends.h"
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace
+{
+    int g_failures = 0;
+    int g_checks = 0;
+
+    void Check(bool ok, const std::string &what)
+    {
+        g_checks++;
+        if (!ok)
+        {
+            g_failures++;
+            std::printf("FAIL: %s\n", what.c_str());
+        }
+    }
+
+    bool Near(double a, double b, double tol)
+    {
+        return std::fabs(a - b) <= tol * std::max(1.0, std::fabs(b));
+    }
+
+    std::vector<double> Eigen(int nx, int ny)
+    {
+        std::vector<double> f(static_cast<size_t>(nx * ny));
+        for (int x = 0; x < nx; x++)
+        {
+            for (int y = 0; y < ny; y++)
+            {
+                f[static_cast<size_t>(x * ny + y)] = std::sin(2.0 * M_PI * x / nx) * std::sin(2.0 * M_PI * y / ny);
+            }
+        }
+        return f;
+    }
+
+    // tests/IDiffusionBackend2DTests.cpp:23-55
+    void ExplicitEigenmode(Hal::IDiffusionBackend2D &backend)
+    {
+        const int nx = 16;
+        const int ny = 20;
+        const double r = 0.05;
+        std::vector<double> f = Eigen(nx, ny);
+        std::vector<double> d(f.size(), 0.0);
+        backend.Diffuse(f, d, r, nx, ny, true, true);
+        const double lam = 1.0 - 4.0 * r * (std::pow(std::sin(M_PI / nx), 2) + std::pow(std::sin(M_PI / ny), 2));
+        bool ok = true;
+        for (size_t i = 0; i < f.size(); i++)
+        {
+            ok = ok && Near(f[i] + d[i], lam * f[i], 1e-4);
+        }
+        Check(ok, "explicit eigenmode through IDiffusionBackend2D");
+    }
+
+    // tests/IDiffusionBackend2DTests.cpp:76-144
+    void AdiEigenmode(Hal::IAdiDiffusionBackend2D &backend)
+    {
+        const int nx = 12;
+        const int ny = 18;
+        const double r = 0.3;
+        std::vector<double> f = Eigen(nx, ny);
+        auto cn = [&](int n)
+        {
+            const double s = std::pow(std::sin(M_PI / n), 2);
+            return (1.0 - 2.0 * r * s) / (1.0 + 2.0 * r * s);
+        };
+        const double lam = cn(nx) * cn(ny);
+        for (int call = 0; call < 2; call++)
+        {
+            std::vector<double> d(f.size(), 0.0);
+            backend.DiffuseADI(f, d, r, nx, ny, true, true);
+            bool ok = true;
+            for (size_t i = 0; i < f.size(); i++)
+            {
+                ok = ok && Near(f[i] + d[i], lam * f[i], 1e-4);
+            }
+            Check(ok, "ADI eigenmode through IAdiDiffusionBackend2D, call " + std::to_string(call));
+        }
+    }
+
+    template <class F>
+    bool Throws(F &&f)
+    {
+        try
+        {
+            f();
+        }
+        catch (const std::invalid_argument &)
+        {
+            return true;
+        }
+        return false;
+    }
+
+    // tests/IAgentForceBackend2DTests.cpp:25-121 with Hal::Rng(1234) draws.
+    void AgentBruteForce(Hal::IAgentForceBackend2D &backend)
+    {
+        const int dim = 16;
+        const int n = 60;
+        std::vector<double> u(3 * n);
+        halhip_rng_doubles(1234, 3 * n, u.data());
+        std::vector<double> pos;
+        std::vector<double> rad;
+        for (int i = 0; i < n; i++)
+        {
+            pos.push_back(u[3 * i] * dim);
+            pos.push_back(u[3 * i + 1] * dim);
+            rad.push_back(0.4 + u[3 * i + 2] * 0.4);
+        }
+        auto wrapDisp = [&](double a, double b)
+        {
+            double d = b - a;
+            if (std::fabs(d) > dim * 0.5)
+            {
+                d = (a > b) ? d + dim : d - dim;
+            }
+            return d;
+        };
+        std::vector<double> expect(pos.size(), 0.0);
+        for (int i = 0; i < n; i++)
+        {
+            for (int j = 0; j < n; j++)
+            {
+                if (i == j)
+                {
+                    continue;
+                }
+                const double dx = wrapDisp(pos[2 * i], pos[2 * j]);
+                const double dy = wrapDisp(pos[2 * i + 1], pos[2 * j + 1]);
+                const double dist = std::sqrt(dx * dx + dy * dy);
+                if (dist < 3.0 && dist > 0.0)
+                {
+                    const double force = 0.4 * ((rad[i] + rad[j]) - dist);
+                    expect[2 * i] -= (dx / dist) * force;
+                    expect[2 * i + 1] -= (dy / dist) * force;
+                }
+            }
+        }
+        std::vector<double> got;
+        backend.AccumulateForces(pos, rad, got, 3.0, 0.4, dim, dim, true, true);
+        bool ok = got.size() == expect.size();
+        for (size_t k = 0; ok && k < expect.size(); k++)
+        {
+            ok = Near(got[k], expect[k], 1e-3);
+        }
+        Check(ok, "agent forces vs brute force through IAgentForceBackend2D");
+        std::vector<double> d;
+        Check(Throws([&] { backend.AccumulateForces(pos, rad, d, 1.0, 1.0, 8, 8, true, true); }),
+              "agent dims differing from construction throw");
+        pos.pop_back();
+        Check(Throws([&] { backend.AccumulateForces(pos, rad, d, 1.0, 1.0, dim, dim, true, true); }),
+              "agent positions/radii size mismatch throws");
+    }
+
+    // A model that owns main() and calls HAL each tick: explicit diffusion,
+    // decay and a point source committed like PDEGrid2D::Update, compared with
+    // the same loop in FP64 on the host.
+    void ModelTickLoop(Hal::IDiffusionBackend2D &backend)
+    {
+        const int nx = 32;
+        const int ny = 24;
+        std::vector<double> field(static_cast<size_t>(nx * ny));
+        halhip_rng_doubles(42, nx * ny, field.data());
+        std::vector<double> host = field;
+        std::vector<double> deltas(field.size(), 0.0);
+        const int src = 10 * ny + 7;
+        for (int tick = 0; tick < 50; tick++)
+        {
+            backend.Diffuse(field, deltas, 0.2, nx, ny, false, true);
+            for (size_t i = 0; i < field.size(); i++)
+            {
+                deltas[i] += field[i] * -1e-3;
+            }
+            deltas[src] += 1.0;
+            for (size_t i = 0; i < field.size(); i++)
+            {
+                field[i] += deltas[i];
+                deltas[i] = 0.0;
+            }
+            std::vector<double> hd(host.size(), 0.0);
+            for (int x = 0; x < nx; x++)
+            {
+                for (int y = 0; y < ny; y++)
+                {
+                    const size_t i = static_cast<size_t>(x * ny + y);
+                    const double c = host[i];
+                    double s = 0.0;
+                    s += (x + 1 < nx) ? host[i + ny] - c : 0.0;
+                    s += (x > 0) ? host[i - ny] - c : 0.0;
+                    s += host[static_cast<size_t>(x * ny + (y + 1) % ny)] - c;
+                    s += host[static_cast<size_t>(x * ny + (y + ny - 1) % ny)] - c;
+                    hd[i] = s * 0.2 - c * 1e-3;
+                }
+            }
+            hd[src] += 1.0;
+            for (size_t i = 0; i < host.size(); i++)
+            {
+                host[i] += hd[i];
+            }
+        }
+        double num = 0.0;
+        double den = 0.0;
+        for (size_t i = 0; i < host.size(); i++)
+        {
+            num += (field[i] - host[i]) * (field[i] - host[i]);
+            den += host[i] * host[i];
+        }
+        Check(std::sqrt(num / den) < 1e-5, "model-owned tick loop matches FP64 host loop (rel L2 < 1e-5)");
+    }
+}
+
+int main()
+{
+    std::unique_ptr<Hal::Hip::HipContext> ctx = Hal::Hip::HipContext::TryCreate(0);
+    if (!ctx)
+    {
+        std::printf("SKIP: no GPU device\n");
+        return 77;
+    }
+    {
+        Hal::Hip::HipDiffusionBackend2D fixed(*ctx, 16, 20);
+        ExplicitEigenmode(fixed);
+        Hal::Hip::HipDiffusionBackend2D any(*ctx);
+        ExplicitEigenmode(any);
+        ModelTickLoop(any);
+        std::vector<double> field(12, 0.0);
+        std::vector<double> deltas(11, 0.0);
+        Check(Throws([&] { any.Diffuse(field, deltas, 0.1, 3, 4, false, false); }), "explicit size mismatch throws");
+        std::vector<double> f12(12, 0.0);
+        std::vector<double> d12(12, 0.0);
+        Check(Throws([&] { fixed.Diffuse(f12, d12, 0.1, 3, 4, false, false); }),
+              "explicit dims differing from construction throw");
+    }
+    {
+        Hal::Hip::HipAdiDiffusionBackend2D fixed(*ctx, 12, 18);
+        AdiEigenmode(fixed);
+        Hal::Hip::HipAdiDiffusionBackend2D any(*ctx);
+        AdiEigenmode(any);
+        std::vector<double> field(12, 0.0);
+        std::vector<double> deltas(11, 0.0);
+        Check(Throws([&] { any.DiffuseADI(field, deltas, 0.1, 3, 4, false, false); }), "ADI size mismatch throws");
+        deltas.resize(12);
+        bool threw = false;
+        try
+        {
+            any.DiffuseADI(field, deltas, 0.1, 3, 4, false, false);
+        }
+        catch (...)
+        {
+            threw = true;
+        }
+        Check(!threw, "ADI on matching sizes does not throw");
+        // Dirichlet fixed point (tests/PDEADITests.cpp:219-235).
+        std::vector<double> flat(35, 4.0);
+        std::vector<double> dd(35, 0.0);
+        Hal::Hip::DirichletBoundary bc;
+        bc.value = 4.0;
+        any.DiffuseADI(flat, dd, 0.9, 7, 5, false, false, bc);
+        bool ok = true;
+        for (double v : dd)
+        {
+            ok = ok && std::fabs(v) < 1e-5;
+        }
+        Check(ok, "ADI Dirichlet fixed point");
+    }
+    {
+        Hal::Hip::HipAgentForceBackend2D agents(*ctx, 16, 16, 60);
+        AgentBruteForce(agents);
+        std::vector<double> pos = {4.0, 4.0, 4.0, 4.0};
+        std::vector<double> rad = {0.5, 0.5};
+        std::vector<double> out;
+        Hal::Hip::HipAgentForceBackend2D small(*ctx, 8, 8, 4);
+        small.AccumulateForces(pos, rad, out, 2.0, 1.0, 8, 8, false, false);
+        Check(out.size() == 4 && out[0] == 0.0 && out[1] == 0.0 && out[2] == 0.0 && out[3] == 0.0,
+              "coincident agents give zero force");
+    }
+    std::printf("%d checks, %d failures\n", g_checks, g_failures);
+    return g_failures == 0 ? 0 : 1;
+}
