@@ -1,0 +1,1247 @@
+// adi_device.hpp -- device side of the ADI strip solver, shared by the
+// single-grid plan (adi2d.hip) and the batched-wells plan (batch2d.hip).
+// See adi2d.hip for the method.
+#pragma once
+
+#include "common.hpp"
+
+#include <vector>
+
+namespace
+{
+    using namespace halhip;
+
+    constexpr int kMaxWaves = 16;
+
+    struct AdiArgs
+    {
+        int32_t X;
+        int32_t Y;
+        int32_t P;         // strips
+        float hr;          // rate/2, explicit half-step weight
+        float r;           // rate, weight of a Dirichlet value in the implicit RHS
+        float negDecay;
+        int32_t wrapX;
+        int32_t wrapY;
+        int32_t dirichlet; // ghost-cell Dirichlet on non-wrapped axes
+        int32_t vecOk;     // yDim % CPT == 0: rows load as vectors
+        // sweep A tables, padded to P*S rows (padding: fw=1, ib=bw=z=0)
+        const float *xfw;
+        const float *xib;
+        const float *xbw;
+        const float *xz;
+        // sweep B tables, padded to T*CPT columns (same padding)
+        const float *yfw;
+        const float *yib;
+        const float *ybw;
+        const float *yz;
+        float yvEnd;
+        float yInv;
+        // Dirichlet edge values
+        const float *bcXlo;
+        const float *bcXhi;
+        const float *bcYlo;
+        const float *bcYhi;
+        // strip reduction and carries, [2][P][Y] each
+        float *red;
+        float *carry;
+        float *colInfo; // [3][Y]: SM factor, u*(0,y), u*(X-1,y)
+        const float *stripA;
+        const float *stripC;
+        const float *stripE;
+        float xvEnd;
+        float xInv;
+        float xibLast;
+        float xz0;
+        float xzLast;
+        // point sources applied by MODE 3 (batched wells): pairs (x, y) and
+        // values, entries [srcBegin, srcEnd) belong to the current strip
+        const int32_t *srcXY;
+        const float *srcVal;
+        int32_t srcBegin;
+        int32_t srcEnd;
+    };
+
+    // ---- wavefront primitives ------------------------------------------------------
+
+    // DPP lane moves (GFX9 encodings): row_shr:n = 0x110+n, wave_shr:1 = 0x138,
+    // row_bcast:15 = 0x142, row_bcast:31 = 0x143.  Lanes without a source (or
+    // in rows outside rowMask) receive `old`.
+    template <int CTRL, int ROW_MASK = 0xF>
+    __device__ __forceinline__ float Dpp(float old, float v)
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                                      __builtin_bit_cast(int, v), CTRL, ROW_MASK,
+                                                                      0xF, false));
+    }
+
+    // Inclusive left-to-right scan of the affine maps (a_l, b_l) over the 64
+    // lanes of a wavefront, composing x -> a*x + b.  The six stages of the
+    // multiplier scan do not depend on the row, so they are computed once
+    // (ScanStagesForward) and each row only scans its offsets (ScanRowForward).
+    __device__ __forceinline__ float ScanStagesForward(float a, float (&stage)[6])
+    {
+        stage[0] = a;
+        a = Dpp<0x111>(1.0f, a) * a;
+        stage[1] = a;
+        a = Dpp<0x112>(1.0f, a) * a;
+        stage[2] = a;
+        a = Dpp<0x114>(1.0f, a) * a;
+        stage[3] = a;
+        a = Dpp<0x118>(1.0f, a) * a;
+        stage[4] = a;
+        a = Dpp<0x142, 0xA>(1.0f, a) * a;
+        stage[5] = a;
+        a = Dpp<0x143, 0xC>(1.0f, a) * a;
+        return a;
+    }
+
+    __device__ __forceinline__ float ScanRowForward(float b, const float (&stage)[6])
+    {
+        b = fmaf(stage[0], Dpp<0x111>(0.0f, b), b);
+        b = fmaf(stage[1], Dpp<0x112>(0.0f, b), b);
+        b = fmaf(stage[2], Dpp<0x114>(0.0f, b), b);
+        b = fmaf(stage[3], Dpp<0x118>(0.0f, b), b);
+        b = fmaf(stage[4], Dpp<0x142, 0xA>(0.0f, b), b);
+        b = fmaf(stage[5], Dpp<0x143, 0xC>(0.0f, b), b);
+        return b;
+    }
+
+    // Right-to-left counterparts (Kogge-Stone with shuffles down).
+    __device__ __forceinline__ float ScanStagesBackward(float a, int lane, float (&stage)[6])
+    {
+#pragma unroll
+        for (int k = 0; k < 6; k++)
+        {
+            const int o = 1 << k;
+            stage[k] = a;
+            const float d = __shfl_down(a, o);
+            a = (lane + o < 64) ? d * a : a;
+        }
+        return a;
+    }
+
+    __device__ __forceinline__ float ScanRowBackward(float b, int lane, const float (&stage)[6])
+    {
+#pragma unroll
+        for (int k = 0; k < 6; k++)
+        {
+            const int o = 1 << k;
+            const float d = __shfl_down(b, o);
+            b = (lane + o < 64) ? fmaf(stage[k], d, b) : b;
+        }
+        return b;
+    }
+
+    // ---- register-tile helpers ------------------------------------------------------
+
+    template <int CPT, bool VEC>
+    __device__ __forceinline__ void LoadRow(const float *__restrict__ row, int y0, int Y, float (&d)[CPT])
+    {
+        if constexpr (VEC)
+        {
+            if (y0 + CPT <= Y)
+            {
+                if constexpr (CPT % 4 == 0)
+                {
+#pragma unroll
+                    for (int q = 0; q < CPT / 4; q++)
+                    {
+                        const float4 t = *reinterpret_cast<const float4 *>(row + y0 + 4 * q);
+                        d[4 * q + 0] = t.x;
+                        d[4 * q + 1] = t.y;
+                        d[4 * q + 2] = t.z;
+                        d[4 * q + 3] = t.w;
+                    }
+                }
+                else if constexpr (CPT == 2)
+                {
+                    const float2 t = *reinterpret_cast<const float2 *>(row + y0);
+                    d[0] = t.x;
+                    d[1] = t.y;
+                }
+                else
+                {
+                    d[0] = row[y0];
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    d[c] = 0.0f;
+                }
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                d[c] = (y0 + c < Y) ? row[y0 + c] : 0.0f;
+            }
+        }
+    }
+
+    template <int CPT, bool VEC>
+    __device__ __forceinline__ void StoreRow(float *__restrict__ row, int y0, int Y, const float (&d)[CPT])
+    {
+        if constexpr (VEC)
+        {
+            if (y0 + CPT <= Y)
+            {
+                if constexpr (CPT % 4 == 0)
+                {
+#pragma unroll
+                    for (int q = 0; q < CPT / 4; q++)
+                    {
+                        *reinterpret_cast<float4 *>(row + y0 + 4 * q) =
+                            make_float4(d[4 * q + 0], d[4 * q + 1], d[4 * q + 2], d[4 * q + 3]);
+                    }
+                }
+                else if constexpr (CPT == 2)
+                {
+                    *reinterpret_cast<float2 *>(row + y0) = make_float2(d[0], d[1]);
+                }
+                else
+                {
+                    row[y0] = d[0];
+                }
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                if (y0 + c < Y)
+                {
+                    row[y0 + c] = d[c];
+                }
+            }
+        }
+    }
+
+    struct StripLds
+    {
+        float *left;  // [kMaxWaves][S] first column of each wavefront
+        float *right; // [kMaxWaves][S] last column of each wavefront
+        float *first; // [S] column y = 0
+        float *last;  // [S] column y = Y-1
+        float *tot;   // [kMaxWaves][S] wavefront scan totals
+        float *aw;    // [kMaxWaves] wavefront multiplier products
+        float *ends;  // [2][S] first / last solution values of each row (periodic y)
+    };
+
+    // Sweep A right-hand side in place: v <- v + hr*(dv/dy+ + dv/dy-) plus the
+    // Dirichlet implicit term r*bc on rows 0 and X-1.  Rows past X and columns
+    // past Y become 0.  An off-grid neighbour is replaced by a ghost value g
+    // with g - c equal to the reference's difference: the wrapped cell
+    // (periodic), 2*bc - c (Dirichlet, whose difference is (bc - c)*2) or c
+    // (zero flux).  ExplicitDiffusionY2ADI (src/Pde/ADIequations.cpp:34-47,
+    // ADIequations.h:79-94) + ImplicitBoundarySource (ADIequations.h:198-220).
+    template <int CPT, int S, bool VEC>
+    __device__ __forceinline__ void SweepARhs(const AdiArgs &a, int s0, int y0, int tid, float (&v)[S][CPT],
+                                              const StripLds &L)
+    {
+        const int lane = tid & 63;
+        const int wave = tid >> 6;
+        const int nWaves = blockDim.x >> 6;
+        const int lastT = (a.Y - 1) / CPT;
+        const int lastC = (a.Y - 1) % CPT;
+        if (lane == 0)
+        {
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+                L.left[wave * S + r] = v[r][0];
+            }
+        }
+        if (lane == 63)
+        {
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+                L.right[wave * S + r] = v[r][CPT - 1];
+            }
+        }
+        if (tid == 0)
+        {
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+                L.first[r] = v[r][0];
+            }
+        }
+        if (tid == lastT)
+        {
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    if (c == lastC)
+                    {
+                        L.last[r] = v[r][c];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const bool dirY = a.dirichlet && !a.wrapY;
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            const int x = s0 + r;
+            float left = Dpp<0x138>(0.0f, v[r][CPT - 1]);
+            float right = Dpp<0x130>(0.0f, v[r][0]);
+            if (x >= a.X)
+            {
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    v[r][c] = 0.0f;
+                }
+                continue;
+            }
+            if (lane == 0 && wave > 0)
+            {
+                left = L.right[(wave - 1) * S + r];
+            }
+            if (lane == 63 && wave + 1 < nWaves)
+            {
+                right = L.left[(wave + 1) * S + r];
+            }
+            if (tid == 0)
+            {
+                const float c0 = v[r][0];
+                left = a.wrapY ? L.last[r] : (dirY ? 2.0f * a.bcYlo[x] - c0 : c0);
+            }
+            float ghostHi = 0.0f;
+            if (tid == lastT)
+            {
+                float cl = 0.0f;
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    cl = (c == lastC) ? v[r][c] : cl;
+                }
+                ghostHi = a.wrapY ? L.first[r] : (dirY ? 2.0f * a.bcYhi[x] - cl : cl);
+                if constexpr (VEC)
+                {
+                    right = ghostHi;
+                }
+            }
+            const bool srcRow = a.dirichlet && !a.wrapX && (x == 0 || x == a.X - 1);
+            float prev = left;
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                const float cur = v[r][c];
+                float nxt = (c + 1 < CPT) ? v[r][c + 1] : right;
+                if constexpr (!VEC)
+                {
+                    if (tid == lastT && c == lastC)
+                    {
+                        nxt = ghostHi;
+                    }
+                }
+                float rhs = fmaf((nxt - cur) + (prev - cur), a.hr, cur);
+                if (srcRow)
+                {
+                    float src = 0.0f;
+                    if (x == 0)
+                    {
+                        src += a.bcXlo[min(y0 + c, a.Y - 1)];
+                    }
+                    if (x == a.X - 1)
+                    {
+                        src += a.bcXhi[min(y0 + c, a.Y - 1)];
+                    }
+                    rhs = fmaf(src, a.r, rhs);
+                }
+                prev = cur;
+                v[r][c] = (y0 + c < a.Y) ? rhs : 0.0f;
+            }
+        }
+    }
+
+    // Zero-carry local solve of sweep A on the strip (v holds the RHS and is
+    // consumed) and write of the strip's two reduction values per column.
+    template <int CPT, int S, bool VEC>
+    __device__ __forceinline__ void ReduceA(const AdiArgs &a, int p, int s0, int y0, float (&v)[S][CPT])
+    {
+        float acc[CPT];
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            acc[c] = 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            const float f = a.xfw[s0 + r];
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                acc[c] = fmaf(f, acc[c], v[r][c]);
+                v[r][c] = acc[c];
+            }
+        }
+        const size_t base = static_cast<size_t>(p) * a.Y;
+        const size_t plane = static_cast<size_t>(a.P) * a.Y;
+        StoreRow<CPT, VEC>(a.red + base, y0, a.Y, acc);
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            acc[c] = 0.0f;
+        }
+#pragma unroll
+        for (int r = S - 1; r >= 0; r--)
+        {
+            const float bwr = a.xbw[s0 + r];
+            const float ibr = a.xib[s0 + r];
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                acc[c] = fmaf(bwr, acc[c], ibr * v[r][c]);
+            }
+        }
+        StoreRow<CPT, VEC>(a.red + plane + base, y0, a.Y, acc);
+    }
+
+    // Finishes sweep A on the strip from the carries (v: RHS in, u* out) and
+    // returns u* of the rows just above (s0-1) and below (s0+S) the strip.
+    template <int CPT, int S, bool VEC>
+    __device__ __forceinline__ void FinishA(const AdiArgs &a, int p, int s0, int y0, float (&v)[S][CPT],
+                                            float (&up)[CPT], float (&dn)[CPT])
+    {
+        const size_t base = static_cast<size_t>(p) * a.Y;
+        const size_t plane = static_cast<size_t>(a.P) * a.Y;
+        float cf[CPT];
+        float acc[CPT];
+        LoadRow<CPT, VEC>(a.carry + base, y0, a.Y, cf);
+        LoadRow<CPT, VEC>(a.carry + plane + base, y0, a.Y, dn);
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            acc[c] = cf[c];
+        }
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            const float f = a.xfw[s0 + r];
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                acc[c] = fmaf(f, acc[c], v[r][c]);
+                v[r][c] = acc[c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            acc[c] = dn[c];
+        }
+#pragma unroll
+        for (int r = S - 1; r >= 0; r--)
+        {
+            const float bwr = a.xbw[s0 + r];
+            const float ibr = a.xib[s0 + r];
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                acc[c] = fmaf(bwr, acc[c], ibr * v[r][c]);
+                v[r][c] = acc[c];
+            }
+        }
+        if (s0 > 0)
+        {
+            const float bwUp = a.xbw[s0 - 1];
+            const float ibUp = a.xib[s0 - 1];
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                up[c] = fmaf(bwUp, v[0][c], ibUp * cf[c]);
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                up[c] = 0.0f;
+            }
+        }
+        if (a.wrapX)
+        {
+            float factor[CPT];
+            LoadRow<CPT, VEC>(a.colInfo, y0, a.Y, factor);
+            const float zUp = (s0 > 0) ? a.xz[s0 - 1] : 0.0f;
+            const float zDn = (s0 + S < a.X) ? a.xz[s0 + S] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+                const float z = a.xz[s0 + r];
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    v[r][c] = fmaf(-factor[c], z, v[r][c]);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                up[c] = fmaf(-factor[c], zUp, up[c]);
+                dn[c] = fmaf(-factor[c], zDn, dn[c]);
+            }
+        }
+    }
+
+    // Sweep B right-hand side in place from u*: v <- v + hr*(dv/dx+ + dv/dx-)
+    // plus the Dirichlet implicit term on columns 0 and Y-1, with the same
+    // ghost-value treatment of off-grid neighbours as sweep A.
+    // ExplicitDiffusionX2ADI (src/Pde/ADIequations.cpp:19-32, ADIequations.h:54-69).
+    template <int CPT, int S, bool VEC>
+    __device__ __forceinline__ void SweepBRhs(const AdiArgs &a, int s0, int y0, int tid, float (&v)[S][CPT],
+                                              const float (&up)[CPT], const float (&dn)[CPT])
+    {
+        const int lastT = (a.Y - 1) / CPT;
+        const int lastC = (a.Y - 1) % CPT;
+        const bool dirX = a.dirichlet && !a.wrapX;
+        const bool dirY = a.dirichlet && !a.wrapY;
+        float prevRow[CPT];
+        if (s0 == 0)
+        {
+            // Ghost row above row 0.
+            float g[CPT];
+            if (a.wrapX)
+            {
+                LoadRow<CPT, VEC>(a.colInfo + 2 * static_cast<size_t>(a.Y), y0, a.Y, g);
+            }
+            else if (dirX)
+            {
+                LoadRow<CPT, VEC>(a.bcXlo, y0, a.Y, g);
+            }
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                prevRow[c] = a.wrapX ? g[c] : (dirX ? fmaf(2.0f, g[c], -v[0][c]) : v[0][c]);
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                prevRow[c] = up[c];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            const int x = s0 + r;
+            if (x >= a.X)
+            {
+                break;
+            }
+            float next[CPT];
+            if (x + 1 < a.X)
+            {
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    next[c] = (r + 1 < S) ? v[r + 1 < S ? r + 1 : r][c] : dn[c];
+                }
+            }
+            else
+            {
+                float g[CPT];
+                if (a.wrapX)
+                {
+                    LoadRow<CPT, VEC>(a.colInfo + static_cast<size_t>(a.Y), y0, a.Y, g);
+                }
+                else if (dirX)
+                {
+                    LoadRow<CPT, VEC>(a.bcXhi, y0, a.Y, g);
+                }
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    next[c] = a.wrapX ? g[c] : (dirX ? fmaf(2.0f, g[c], -v[r][c]) : v[r][c]);
+                }
+            }
+            float srcLo = 0.0f;
+            float srcHi = 0.0f;
+            if (dirY)
+            {
+                srcLo = (tid == 0) ? a.bcYlo[x] * a.r : 0.0f;
+                srcHi = (tid == lastT) ? a.bcYhi[x] * a.r : 0.0f;
+            }
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                const float cur = v[r][c];
+                float rhs = fmaf((next[c] - cur) + (prevRow[c] - cur), a.hr, cur);
+                if (dirY)
+                {
+                    rhs += (c == 0) ? srcLo : 0.0f;
+                    rhs += (c == lastC) ? srcHi : 0.0f;
+                }
+                prevRow[c] = cur;
+                v[r][c] = (y0 + c < a.Y) ? rhs : 0.0f;
+            }
+        }
+    }
+
+    // Sweep B solve of every row of the strip across the workgroup.
+    template <int CPT, int S>
+    __device__ __forceinline__ void SolveB(const AdiArgs &a, int y0, int tid, float (&v)[S][CPT], const StripLds &L)
+    {
+        const int lane = tid & 63;
+        const int wave = tid >> 6;
+        const int nWaves = blockDim.x >> 6;
+        float fw[CPT];
+        float bw[CPT];
+        float ib[CPT];
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            fw[c] = a.yfw[y0 + c];
+            bw[c] = a.ybw[y0 + c];
+            ib[c] = a.yib[y0 + c];
+        }
+
+        // Forward recurrence d'_y = fw_y d'_{y-1} + d_y.
+        float aSeg = 1.0f;
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            aSeg *= fw[c];
+        }
+        float stage[6];
+        const float aInc = ScanStagesForward(aSeg, stage);
+        const float aEx = Dpp<0x138>(1.0f, aInc);
+        if (lane == 63)
+        {
+            L.aw[wave] = aInc;
+        }
+        float bEx[S];
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            float b = 0.0f;
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                b = fmaf(fw[c], b, v[r][c]);
+            }
+            b = ScanRowForward(b, stage);
+            bEx[r] = Dpp<0x138>(0.0f, b);
+            if (lane == 63)
+            {
+                L.tot[wave * S + r] = b;
+            }
+        }
+        __syncthreads();
+        // Lane r folds the totals of the wavefronts before this one for row r.
+        float cwLane = 0.0f;
+        if (lane < S)
+        {
+            for (int w = 0; w < wave; w++)
+            {
+                cwLane = fmaf(L.aw[w], cwLane, L.tot[w * S + lane]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            const float cw = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cwLane), r));
+            float acc = fmaf(aEx, cw, bEx[r]);
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                acc = fmaf(fw[c], acc, v[r][c]);
+                v[r][c] = acc;
+            }
+        }
+        __syncthreads();
+
+        // Backward recurrence x_y = bw_y x_{y+1} + ib_y d'_y, scanned from the right.
+        aSeg = 1.0f;
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            aSeg *= bw[c];
+        }
+        const float aIncB = ScanStagesBackward(aSeg, lane, stage);
+        float aExB = __shfl_down(aIncB, 1);
+        aExB = (lane == 63) ? 1.0f : aExB;
+        if (lane == 0)
+        {
+            L.aw[wave] = aIncB;
+        }
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            float b = 0.0f;
+#pragma unroll
+            for (int c = CPT - 1; c >= 0; c--)
+            {
+                b = fmaf(bw[c], b, ib[c] * v[r][c]);
+            }
+            b = ScanRowBackward(b, lane, stage);
+            const float be = __shfl_down(b, 1);
+            bEx[r] = (lane == 63) ? 0.0f : be;
+            if (lane == 0)
+            {
+                L.tot[wave * S + r] = b;
+            }
+        }
+        __syncthreads();
+        cwLane = 0.0f;
+        if (lane < S)
+        {
+            for (int w = nWaves - 1; w > wave; w--)
+            {
+                cwLane = fmaf(L.aw[w], cwLane, L.tot[w * S + lane]);
+            }
+        }
+        const int lastT = (a.Y - 1) / CPT;
+        const int lastC = (a.Y - 1) % CPT;
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            const float cw = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cwLane), r));
+            float acc = fmaf(aExB, cw, bEx[r]);
+#pragma unroll
+            for (int c = CPT - 1; c >= 0; c--)
+            {
+                acc = fmaf(bw[c], acc, ib[c] * v[r][c]);
+                v[r][c] = acc;
+            }
+            if (a.wrapY)
+            {
+                if (tid == 0)
+                {
+                    L.ends[r] = v[r][0];
+                }
+                if (tid == lastT)
+                {
+#pragma unroll
+                    for (int c = 0; c < CPT; c++)
+                    {
+                        if (c == lastC)
+                        {
+                            L.ends[S + r] = v[r][c];
+                        }
+                    }
+                }
+            }
+        }
+        if (a.wrapY)
+        {
+            // Sherman-Morrison combination (TdmaSolver.h:112-119).
+            __syncthreads();
+            float z[CPT];
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                z[c] = a.yz[y0 + c];
+            }
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+                const float factor = (L.ends[r] + L.ends[S + r] * a.yvEnd) * a.yInv;
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    v[r][c] = fmaf(-factor, z[c], v[r][c]);
+                }
+            }
+        }
+    }
+
+    // MODE 0: reduction of the current field only (first tick of a run).
+    // MODE 1: plain tick in place, then the reduction of u' for the next tick.
+    // MODE 2: tick with decay: u' = ADI(u) - k*u (no reduction; the next tick
+    //         starts with a MODE 0 pass after point sources are added).
+    // MODE 3: tick with decay and point sources kept in registers:
+    //         u' = ADI(u) - k*u + sources, then the reduction of u' for the next
+    //         tick (batched wells).
+    // VEC: yDim is a multiple of CPT, so rows move as 16-byte vectors.
+    // One workgroup runs strip p of the grid described by `a`; u is the grid.
+    template <int CPT, int S, int MODE, bool VEC>
+    __device__ __forceinline__ void StripBody(const AdiArgs &a, float *__restrict__ u, int p)
+    {
+        __shared__ float sLeft[kMaxWaves * S];
+        __shared__ float sRight[kMaxWaves * S];
+        __shared__ float sFirst[S];
+        __shared__ float sLast[S];
+        __shared__ float sTot[kMaxWaves * S];
+        __shared__ float sAw[kMaxWaves];
+        __shared__ float sEnds[2 * S];
+        const StripLds L{sLeft, sRight, sFirst, sLast, sTot, sAw, sEnds};
+
+        const int s0 = p * S;
+        const int tid = threadIdx.x;
+        const int y0 = tid * CPT;
+
+        float v[S][CPT];
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            if (s0 + r < a.X)
+            {
+                LoadRow<CPT, VEC>(u + static_cast<size_t>(s0 + r) * a.Y, y0, a.Y, v[r]);
+            }
+            else
+            {
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    v[r][c] = 0.0f;
+                }
+            }
+        }
+        float old[MODE == 3 ? S : 1][CPT];
+        if constexpr (MODE == 3)
+        {
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    old[r][c] = v[r][c];
+                }
+            }
+        }
+        SweepARhs<CPT, S, VEC>(a, s0, y0, tid, v, L);
+        if constexpr (MODE == 0)
+        {
+            ReduceA<CPT, S, VEC>(a, p, s0, y0, v);
+            return;
+        }
+        else
+        {
+            float up[CPT];
+            float dn[CPT];
+            FinishA<CPT, S, VEC>(a, p, s0, y0, v, up, dn);
+            SweepBRhs<CPT, S, VEC>(a, s0, y0, tid, v, up, dn);
+            SolveB<CPT, S>(a, y0, tid, v, L);
+            if constexpr (MODE == 3)
+            {
+#pragma unroll
+                for (int r = 0; r < S; r++)
+                {
+#pragma unroll
+                    for (int c = 0; c < CPT; c++)
+                    {
+                        v[r][c] = fmaf(old[r][c], a.negDecay, v[r][c]);
+                    }
+                }
+                // Point sources of this strip (PDEGrid2D::Add), in list order.
+                for (int k = a.srcBegin; k < a.srcEnd; k++)
+                {
+                    const int sx = a.srcXY[2 * k] - s0;
+                    const int sy = a.srcXY[2 * k + 1] - y0;
+                    const float sv = a.srcVal[k];
+#pragma unroll
+                    for (int r = 0; r < S; r++)
+                    {
+#pragma unroll
+                        for (int c = 0; c < CPT; c++)
+                        {
+                            v[r][c] += (sx == r && sy == c) ? sv : 0.0f;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+                if (s0 + r < a.X)
+                {
+                    float *row = u + static_cast<size_t>(s0 + r) * a.Y;
+                    if constexpr (MODE == 2)
+                    {
+                        float old[CPT];
+                        LoadRow<CPT, VEC>(row, y0, a.Y, old);
+#pragma unroll
+                        for (int c = 0; c < CPT; c++)
+                        {
+                            v[r][c] = fmaf(old[c], a.negDecay, v[r][c]);
+                        }
+                    }
+                    StoreRow<CPT, VEC>(row, y0, a.Y, v[r]);
+                }
+            }
+            if constexpr (MODE == 1 || MODE == 3)
+            {
+                __syncthreads();
+                SweepARhs<CPT, S, VEC>(a, s0, y0, tid, v, L);
+                ReduceA<CPT, S, VEC>(a, p, s0, y0, v);
+            }
+        }
+    }
+
+    template <int CPT, int S, int MODE, bool VEC>
+    __global__ __launch_bounds__(1024) void AdiStrip(AdiArgs a, float *__restrict__ u)
+    {
+        StripBody<CPT, S, MODE, VEC>(a, u, blockIdx.x);
+    }
+
+    // Scan of the strip reductions along each column (sweep A across strips).
+    // Block: 256 threads = 16 columns x 16 segments; lane group k of 16 lanes
+    // takes column y0 + (lane & 15) and segment g = 4*wave + (lane >> 4), which
+    // owns strips [g*Q, (g+1)*Q) with Q <= QMAX.  A thread first loads its
+    // segment's reduction values into registers (all loads in flight
+    // together), folds them, exchanges segment totals through LDS, then runs
+    // the recurrences from registers.  Writes the carry-in values of every
+    // strip and, per column, the periodic Sherman-Morrison factor and the
+    // wrapped rows of u*.
+    constexpr int kScanCols = 16;
+    constexpr int kScanSegs = 16;
+
+    template <int QMAX>
+    __device__ __forceinline__ void ScanBody(const AdiArgs &a, int yBase, int32_t Q)
+    {
+        __shared__ float sB[kScanSegs][kScanCols];
+        __shared__ float sA[kScanSegs];
+        __shared__ float sB2[kScanSegs][kScanCols];
+        __shared__ float sA2[kScanSegs];
+        __shared__ float sCf[kScanCols];
+        __shared__ float sX0[kScanCols];
+        const int col = threadIdx.x & (kScanCols - 1);
+        const int g = threadIdx.x / kScanCols;
+        const int y = yBase + col;
+        const bool active = y < a.Y;
+        const int p0 = g * Q;
+        const int cnt = max(0, min(a.P, p0 + Q) - p0);
+        const size_t plane = static_cast<size_t>(a.P) * a.Y;
+        const float *B = a.red;
+        const float *D = a.red + plane;
+        float *cfIn = a.carry;
+        float *xbIn = a.carry + plane;
+
+        float bv[QMAX];
+        float dv[QMAX];
+        float cv[QMAX];
+#pragma unroll
+        for (int q = 0; q < QMAX; q++)
+        {
+            const bool ok = active && q < cnt;
+            const size_t k = ok ? static_cast<size_t>(p0 + q) * a.Y + y : 0;
+            bv[q] = ok ? B[k] : 0.0f;
+            dv[q] = ok ? D[k] : 0.0f;
+        }
+        float acc = 0.0f;
+        float aSeg = 1.0f;
+#pragma unroll
+        for (int q = 0; q < QMAX; q++)
+        {
+            const float A = (q < cnt) ? a.stripA[p0 + q] : 1.0f;
+            acc = fmaf(A, acc, bv[q]);
+            aSeg *= A;
+        }
+        sB[g][col] = acc;
+        if (col == 0)
+        {
+            sA[g] = aSeg;
+        }
+        __syncthreads();
+        float cf = 0.0f;
+        for (int w = 0; w < g; w++)
+        {
+            cf = fmaf(sA[w], cf, sB[w][col]);
+        }
+#pragma unroll
+        for (int q = 0; q < QMAX; q++)
+        {
+            cv[q] = cf;
+            const float A = (q < cnt) ? a.stripA[p0 + q] : 1.0f;
+            cf = fmaf(A, cf, bv[q]);
+        }
+        if (cnt > 0 && p0 + cnt == a.P)
+        {
+            sCf[col] = cf;
+        }
+        acc = 0.0f;
+        aSeg = 1.0f;
+#pragma unroll
+        for (int q = QMAX - 1; q >= 0; q--)
+        {
+            const bool in = q < cnt;
+            const float C = in ? a.stripC[p0 + q] : 1.0f;
+            const float off = in ? fmaf(a.stripE[p0 + q], cv[q], dv[q]) : 0.0f;
+            acc = fmaf(C, acc, off);
+            aSeg *= C;
+        }
+        sB2[g][col] = acc;
+        if (col == 0)
+        {
+            sA2[g] = aSeg;
+        }
+        __syncthreads();
+        float xb = 0.0f;
+        for (int w = kScanSegs - 1; w > g; w--)
+        {
+            xb = fmaf(sA2[w], xb, sB2[w][col]);
+        }
+#pragma unroll
+        for (int q = QMAX - 1; q >= 0; q--)
+        {
+            const bool in = q < cnt;
+            if (in && active)
+            {
+                const size_t k = static_cast<size_t>(p0 + q) * a.Y + y;
+                xbIn[k] = xb;
+                cfIn[k] = cv[q];
+            }
+            const float C = in ? a.stripC[p0 + q] : 1.0f;
+            const float off = in ? fmaf(a.stripE[p0 + q], cv[q], dv[q]) : 0.0f;
+            xb = fmaf(C, xb, off);
+        }
+        if (g == 0)
+        {
+            sX0[col] = xb;
+        }
+        __syncthreads();
+        if (g == 0 && active)
+        {
+            const float x0 = sX0[col];
+            const float xLast = a.xibLast * sCf[col];
+            const float factor = (x0 + xLast * a.xvEnd) * a.xInv;
+            a.colInfo[y] = factor;
+            a.colInfo[a.Y + y] = x0 - factor * a.xz0;
+            a.colInfo[2 * static_cast<size_t>(a.Y) + y] = xLast - factor * a.xzLast;
+        }
+    }
+
+    template <int QMAX>
+    __global__ __launch_bounds__(kScanCols *kScanSegs) void AdiStripScan(AdiArgs a, int32_t Q)
+    {
+        ScanBody<QMAX>(a, blockIdx.x * kScanCols, Q);
+    }
+
+    // Scan of the strip reductions for any strip count (reloads from memory).
+    // Block: 64 columns (lanes) x G wavefronts; wavefront g owns strips
+    // [g*Q, (g+1)*Q).  Writes the carry-in values of every strip and, per
+    // column, the periodic Sherman-Morrison factor and the wrapped rows of u*.
+    __global__ __launch_bounds__(1024) void AdiStripScanGeneric(AdiArgs a, int32_t Q)
+    {
+        __shared__ float sB[kMaxWaves][64];
+        __shared__ float sA[kMaxWaves];
+        __shared__ float sB2[kMaxWaves][64];
+        __shared__ float sA2[kMaxWaves];
+        __shared__ float sCf[64];
+        __shared__ float sX0[64];
+        const int lane = threadIdx.x & 63;
+        const int g = threadIdx.x >> 6;
+        const int G = blockDim.x >> 6;
+        const int y = blockIdx.x * 64 + lane;
+        const bool active = y < a.Y;
+        const int p0 = g * Q;
+        const int p1 = min(a.P, p0 + Q);
+        const size_t plane = static_cast<size_t>(a.P) * a.Y;
+        const float *B = a.red;
+        const float *D = a.red + plane;
+        float *cfIn = a.carry;
+        float *xbIn = a.carry + plane;
+
+        float acc = 0.0f;
+        float aSeg = 1.0f;
+        for (int p = p0; p < p1; p++)
+        {
+            const float A = a.stripA[p];
+            acc = A * acc + (active ? B[static_cast<size_t>(p) * a.Y + y] : 0.0f);
+            aSeg *= A;
+        }
+        sB[g][lane] = acc;
+        if (lane == 0)
+        {
+            sA[g] = aSeg;
+        }
+        __syncthreads();
+        float cf = 0.0f;
+        for (int w = 0; w < g; w++)
+        {
+            cf = sA[w] * cf + sB[w][lane];
+        }
+        for (int p = p0; p < p1; p++)
+        {
+            const size_t k = static_cast<size_t>(p) * a.Y + y;
+            const float bv = active ? B[k] : 0.0f;
+            if (active)
+            {
+                cfIn[k] = cf;
+            }
+            cf = a.stripA[p] * cf + bv;
+        }
+        if (p0 < p1 && p1 == a.P)
+        {
+            sCf[lane] = cf;
+        }
+
+        acc = 0.0f;
+        aSeg = 1.0f;
+        for (int p = p1 - 1; p >= p0; p--)
+        {
+            const size_t k = static_cast<size_t>(p) * a.Y + y;
+            const float C = a.stripC[p];
+            const float off = active ? (D[k] + a.stripE[p] * cfIn[k]) : 0.0f;
+            acc = C * acc + off;
+            aSeg *= C;
+        }
+        sB2[g][lane] = acc;
+        if (lane == 0)
+        {
+            sA2[g] = aSeg;
+        }
+        __syncthreads();
+        float xb = 0.0f;
+        for (int w = G - 1; w > g; w--)
+        {
+            xb = sA2[w] * xb + sB2[w][lane];
+        }
+        for (int p = p1 - 1; p >= p0; p--)
+        {
+            const size_t k = static_cast<size_t>(p) * a.Y + y;
+            float off = 0.0f;
+            if (active)
+            {
+                xbIn[k] = xb;
+                off = D[k] + a.stripE[p] * cfIn[k];
+            }
+            xb = a.stripC[p] * xb + off;
+        }
+        if (g == 0)
+        {
+            sX0[lane] = xb;
+        }
+        __syncthreads();
+        if (g == 0 && active)
+        {
+            const float x0 = sX0[lane];
+            const float xLast = a.xibLast * sCf[lane];
+            const float factor = (x0 + xLast * a.xvEnd) * a.xInv;
+            a.colInfo[y] = factor;
+            a.colInfo[a.Y + y] = x0 - factor * a.xz0;
+            a.colInfo[2 * static_cast<size_t>(a.Y) + y] = xLast - factor * a.xzLast;
+        }
+    }
+
+    // Point sources, applied serially so repeated indices add in list order.
+    __global__ void AdiAddSources(float *__restrict__ u, int32_t n, const int64_t *__restrict__ idx,
+                                  const float *__restrict__ val)
+    {
+        if (threadIdx.x == 0 && blockIdx.x == 0)
+        {
+            for (int k = 0; k < n; k++)
+            {
+                u[idx[k]] += val[k];
+            }
+        }
+    }
+
+    // ---- host: line factorisation (FP64) ---------------------------------------
+
+    // Thomas factors of one ADI line with the reference's coefficients
+    // (ADISolveRow, ADIequations.h:225-270; TDMA / TDMAperiodic,
+    // TdmaSolver.h:27-120), padded to `padded` entries with identity rows.
+    struct AxisFactors
+    {
+        std::vector<double> fw, ib, bw, z;
+        double vEnd = 0.0;
+        double inv = 0.0;
+    };
+
+    AxisFactors FactorLine(int32_t n, int32_t padded, double rate, bool wrap, bool dirichlet)
+    {
+        AxisFactors f;
+        f.fw.assign(static_cast<size_t>(padded), 1.0);
+        f.ib.assign(static_cast<size_t>(padded), 0.0);
+        f.bw.assign(static_cast<size_t>(padded), 0.0);
+        f.z.assign(static_cast<size_t>(padded), 0.0);
+        const double off = -rate / 2;
+        std::vector<double> b(static_cast<size_t>(n));
+        double gamma = 0.0;
+        if (wrap)
+        {
+            for (int32_t i = 0; i < n; i++)
+            {
+                b[i] = 1 - 2 * off;
+            }
+            gamma = -b[0];
+            const double alpha = off;
+            const double beta = off;
+            b[0] = b[0] - gamma;
+            if (n > 1)
+            {
+                b[n - 1] = b[n - 1] - (alpha * beta) / gamma;
+            }
+        }
+        else
+        {
+            const double endAdjust = dirichlet ? 1.0 : -1.0;
+            for (int32_t i = 0; i < n; i++)
+            {
+                double weight = 2.0;
+                if (i == 0)
+                {
+                    weight += endAdjust;
+                }
+                if (i == n - 1)
+                {
+                    weight += endAdjust;
+                }
+                b[i] = 1 - weight * off;
+            }
+        }
+        std::vector<double> bp(static_cast<size_t>(n));
+        bp[0] = b[0];
+        f.fw[0] = 0.0;
+        for (int32_t i = 1; i < n; i++)
+        {
+            const double w = off / bp[i - 1];
+            bp[i] = b[i] - w * off;
+            f.fw[i] = -w;
+        }
+        for (int32_t i = 0; i < n; i++)
+        {
+            f.ib[i] = 1.0 / bp[i];
+            f.bw[i] = (i < n - 1) ? -off / bp[i] : 0.0;
+        }
+        if (wrap)
+        {
+            // z solves the adjusted system for u = (gamma, 0, ..., 0, alpha).
+            std::vector<double> u(static_cast<size_t>(n), 0.0);
+            u[0] = gamma;
+            if (n > 1)
+            {
+                u[n - 1] = off;
+            }
+            std::vector<double> dp(static_cast<size_t>(n));
+            double acc = 0.0;
+            for (int32_t i = 0; i < n; i++)
+            {
+                acc = ((i == 0) ? 0.0 : f.fw[i]) * acc + u[i];
+                dp[i] = acc;
+            }
+            acc = 0.0;
+            for (int32_t i = n - 1; i >= 0; i--)
+            {
+                acc = f.bw[i] * acc + f.ib[i] * dp[i];
+                f.z[i] = acc;
+            }
+            f.vEnd = off / gamma;
+            const double vz = f.z[0] + f.z[n - 1] * f.vEnd;
+            f.inv = 1.0 / (1 + vz);
+        }
+        return f;
+    }
+
+}

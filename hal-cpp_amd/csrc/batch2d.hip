@@ -1,0 +1,817 @@
+// batch2d.hip -- many independent ADI wells of uneven size advanced together.
+//
+// Reference: the host-side runner MultiWellExperiment::Step / StepMultiThread
+// (include/Hal/Tools/MultiWellExperiment.h:82-103), which steps one model per
+// well, serially or on a thread pool.  Here every well is a HAL grid ticked as
+// DiffusionADI(rate_w) + MulAll(-decay_w) + Add(sources_w) + Update
+// (include/Hal/Grids/PDEGrid2D.h:103-218, 328-347), and a tick of the whole
+// batch is two launches over a work list covering every well:
+//   BatchScan   one block per (well, 16-column group): the cross-strip scan
+//   BatchStrip  one block per (well, strip): sweep A finish, sweep B, decay,
+//               point sources and the next tick's strip reduction, with the
+//               strip body chosen per well (columns per thread 1/2/4, vector
+//               or scalar rows) by a block-uniform switch.
+// Both are captured in one hipGraph per chunk of ticks.  The strip height is
+// picked by an autotuner (timed candidates, cached per batch signature).  The
+// per-instance launch loop (halhip_batch_run_ticks_per_instance) runs the same
+// kernels one well at a time and is the baseline the batch is measured
+// against.
+#include "adi_device.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <numeric>
+#include <sstream>
+#include <string>
+
+using namespace halhip;
+
+namespace
+{
+    constexpr int kBatchThreads = 256;
+    constexpr int kBatchMaxSide = 1024;
+
+    struct BatchWell
+    {
+        AdiArgs args;
+        int64_t fieldOffset;
+        int32_t cls; // 0: 1 col/thread; 1: 2 vec; 2: 2 scalar; 3: 4 vec; 4: 4 scalar
+        int32_t pad;
+    };
+
+    struct StripItem
+    {
+        int32_t well;
+        int32_t strip;
+        int32_t srcBegin;
+        int32_t srcEnd;
+    };
+
+    struct ScanItem
+    {
+        int32_t well;
+        int32_t yBase;
+    };
+
+    template <int S, int MODE>
+    __device__ __forceinline__ void StripByClass(int cls, const AdiArgs &a, float *u, int p)
+    {
+        switch (cls)
+        {
+            case 0:
+                StripBody<1, S, MODE, true>(a, u, p);
+                break;
+            case 1:
+                StripBody<2, S, MODE, true>(a, u, p);
+                break;
+            case 2:
+                StripBody<2, S, MODE, false>(a, u, p);
+                break;
+            case 3:
+                StripBody<4, S, MODE, true>(a, u, p);
+                break;
+            default:
+                StripBody<4, S, MODE, false>(a, u, p);
+                break;
+        }
+    }
+
+    // MODE 0: strip reductions of the current fields; MODE 3: full tick.
+    template <int S, int MODE>
+    __global__ __launch_bounds__(kBatchThreads) void BatchStrip(const BatchWell *__restrict__ wells,
+                                                                const StripItem *__restrict__ items,
+                                                                float *__restrict__ field)
+    {
+        const StripItem it = items[blockIdx.x];
+        const BatchWell *w = wells + it.well;
+        AdiArgs a = w->args;
+        a.srcBegin = it.srcBegin;
+        a.srcEnd = it.srcEnd;
+        StripByClass<S, MODE>(w->cls, a, field + w->fieldOffset, it.strip);
+    }
+
+    __global__ __launch_bounds__(kScanCols *kScanSegs) void BatchScan(const BatchWell *__restrict__ wells,
+                                                                      const ScanItem *__restrict__ items)
+    {
+        const ScanItem it = items[blockIdx.x];
+        const AdiArgs a = wells[it.well].args;
+        ScanBody<16>(a, it.yBase, (a.P + kScanSegs - 1) / kScanSegs);
+    }
+
+    using BatchStripKernel = void (*)(const BatchWell *, const StripItem *, float *);
+
+    BatchStripKernel PickBatchStrip(int rows, int mode)
+    {
+        if (rows == 4)
+        {
+            return mode == 0 ? BatchStrip<4, 0> : BatchStrip<4, 3>;
+        }
+        return mode == 0 ? BatchStrip<8, 0> : BatchStrip<8, 3>;
+    }
+}
+
+struct halhip_batch
+{
+    halhip_context *ctx = nullptr;
+    int32_t nWells = 0;
+    std::vector<int32_t> X, Y;
+    std::vector<int64_t> offset;
+    int64_t cells = 0;
+    std::vector<double> rate, decay;
+    // sources, host copy: (well, cell, value) in list order
+    std::vector<int32_t> srcWell;
+    std::vector<int64_t> srcCell;
+    std::vector<float> srcVal;
+    int32_t rows = 8;
+    DeviceArray<float> field;
+    DeviceArray<float> tables;
+    DeviceArray<float> red;
+    DeviceArray<float> carry;
+    DeviceArray<float> colInfo;
+    DeviceArray<BatchWell> wells;
+    DeviceArray<StripItem> stripItems;
+    DeviceArray<ScanItem> scanItems;
+    DeviceArray<int32_t> srcXY;
+    DeviceArray<float> srcValDev;
+    std::vector<int32_t> wellStripBegin, wellStripCount; // per-instance launches: item ranges
+    std::vector<int32_t> wellScanBegin, wellScanCount;
+    int32_t nStripItems = 0;
+    int32_t nScanItems = 0;
+    std::string builtKey;
+    bool reducedValid = false;
+    StreamClock clock;
+    Profiler prof;
+    GraphCache graphs;
+};
+
+namespace
+{
+    void RequireBatch(const halhip_batch *b, const char *caller)
+    {
+        Require(b != nullptr, std::string(caller) + ": null batch");
+        Bind(b->ctx);
+    }
+
+    int32_t ColsPerThread(int32_t Y)
+    {
+        return (Y <= 256) ? 1 : ((Y <= 512) ? 2 : 4);
+    }
+
+    // Builds per-well factor tables, strip constants, work lists and source
+    // lists for the current parameters, when they changed.
+    void Build(halhip_batch *b, int32_t wrapX, int32_t wrapY, const halhip_bc *bc)
+    {
+        Require(bc == nullptr || bc->kind == HALHIP_BC_NONE ||
+                    (bc->kind == HALHIP_BC_DIRICHLET && bc->xlo == nullptr && bc->xhi == nullptr && bc->ylo == nullptr &&
+                     bc->yhi == nullptr),
+                "halhip_batch: only a constant Dirichlet value is supported for batches");
+        const bool dir = bc != nullptr && bc->kind == HALHIP_BC_DIRICHLET;
+        const double bcValue = dir ? bc->value : 0.0;
+        std::string key;
+        AppendKey(key, b->rows);
+        AppendKey(key, wrapX);
+        AppendKey(key, wrapY);
+        AppendKey(key, dir);
+        AppendKey(key, bcValue);
+        key.append(reinterpret_cast<const char *>(b->rate.data()), b->rate.size() * sizeof(double));
+        key.append(reinterpret_cast<const char *>(b->decay.data()), b->decay.size() * sizeof(double));
+        AppendKey(key, b->srcVal.size());
+        key.append(reinterpret_cast<const char *>(b->srcWell.data()), b->srcWell.size() * sizeof(int32_t));
+        key.append(reinterpret_cast<const char *>(b->srcCell.data()), b->srcCell.size() * sizeof(int64_t));
+        key.append(reinterpret_cast<const char *>(b->srcVal.data()), b->srcVal.size() * sizeof(float));
+        if (key == b->builtKey)
+        {
+            return;
+        }
+        Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
+        const int32_t S = b->rows;
+        // Sources grouped by well then row (stable), as (x, y) pairs.
+        std::vector<int32_t> order(b->srcWell.size());
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int32_t i, int32_t j)
+                         {
+                             if (b->srcWell[i] != b->srcWell[j])
+                             {
+                                 return b->srcWell[i] < b->srcWell[j];
+                             }
+                             const int32_t yi = b->Y[b->srcWell[i]];
+                             return b->srcCell[i] / yi < b->srcCell[j] / yi;
+                         });
+        std::vector<int32_t> xy(2 * order.size());
+        std::vector<float> sval(order.size());
+        for (size_t k = 0; k < order.size(); k++)
+        {
+            const int32_t w = b->srcWell[order[k]];
+            const int64_t cell = b->srcCell[order[k]];
+            xy[2 * k] = static_cast<int32_t>(cell / b->Y[w]);
+            xy[2 * k + 1] = static_cast<int32_t>(cell % b->Y[w]);
+            sval[k] = b->srcVal[order[k]];
+        }
+        std::vector<float> host;
+        std::vector<BatchWell> wells(static_cast<size_t>(b->nWells));
+        std::vector<StripItem> strips;
+        std::vector<ScanItem> scans;
+        std::vector<size_t> tabOff(static_cast<size_t>(b->nWells));
+        struct Offs
+        {
+            size_t xfw, xib, xbw, xz, yfw, yib, ybw, yz, A, C, E, bxl, bxh, byl, byh;
+        };
+        std::vector<Offs> offs(static_cast<size_t>(b->nWells));
+        std::vector<int64_t> redOff(static_cast<size_t>(b->nWells));
+        std::vector<int64_t> colOff(static_cast<size_t>(b->nWells));
+        int64_t redTotal = 0;
+        int64_t colTotal = 0;
+        size_t srcCursor = 0;
+        b->wellStripBegin.assign(b->nWells, 0);
+        b->wellStripCount.assign(b->nWells, 0);
+        b->wellScanBegin.assign(b->nWells, 0);
+        b->wellScanCount.assign(b->nWells, 0);
+        for (int32_t w = 0; w < b->nWells; w++)
+        {
+            const int32_t X = b->X[w];
+            const int32_t Yw = b->Y[w];
+            const int32_t cpt = ColsPerThread(Yw);
+            const int32_t P = (X + S - 1) / S;
+            const int32_t xPad = P * S;
+            const int32_t yPad = kBatchThreads * cpt;
+            AxisFactors fx = FactorLine(X, xPad, b->rate[w], wrapX != 0, dir);
+            AxisFactors fy = FactorLine(Yw, yPad, b->rate[w], wrapY != 0, dir);
+            auto put = [&](const std::vector<double> &v)
+            {
+                size_t at = host.size();
+                for (double d : v)
+                {
+                    host.push_back(static_cast<float>(d));
+                }
+                return at;
+            };
+            std::vector<double> sA(static_cast<size_t>(P)), sC(static_cast<size_t>(P)), sE(static_cast<size_t>(P));
+            for (int32_t q = 0; q < P; q++)
+            {
+                double A = 1.0;
+                double C = 1.0;
+                std::vector<double> dp(static_cast<size_t>(S));
+                double acc = 1.0;
+                for (int32_t r = 0; r < S; r++)
+                {
+                    const double fwv = fx.fw[static_cast<size_t>(q * S + r)];
+                    A *= fwv;
+                    C *= fx.bw[static_cast<size_t>(q * S + r)];
+                    acc = fwv * acc;
+                    dp[r] = acc;
+                }
+                acc = 0.0;
+                for (int32_t r = S - 1; r >= 0; r--)
+                {
+                    acc = fx.bw[static_cast<size_t>(q * S + r)] * acc + fx.ib[static_cast<size_t>(q * S + r)] * dp[r];
+                }
+                sA[q] = A;
+                sC[q] = C;
+                sE[q] = acc;
+            }
+            Offs &o = offs[w];
+            o.xfw = put(fx.fw);
+            o.xib = put(fx.ib);
+            o.xbw = put(fx.bw);
+            o.xz = put(fx.z);
+            o.yfw = put(fy.fw);
+            o.yib = put(fy.ib);
+            o.ybw = put(fy.bw);
+            o.yz = put(fy.z);
+            o.A = put(sA);
+            o.C = put(sC);
+            o.E = put(sE);
+            o.bxl = put(std::vector<double>(static_cast<size_t>(Yw), bcValue));
+            o.bxh = put(std::vector<double>(static_cast<size_t>(Yw), bcValue));
+            o.byl = put(std::vector<double>(static_cast<size_t>(X), bcValue));
+            o.byh = put(std::vector<double>(static_cast<size_t>(X), bcValue));
+            redOff[w] = redTotal;
+            redTotal += 2 * static_cast<int64_t>(P) * Yw;
+            colOff[w] = colTotal;
+            colTotal += 3 * static_cast<int64_t>(Yw);
+
+            BatchWell &bw = wells[w];
+            AdiArgs &a = bw.args;
+            a = AdiArgs{};
+            a.X = X;
+            a.Y = Yw;
+            a.P = P;
+            a.hr = static_cast<float>(b->rate[w] / 2);
+            a.r = static_cast<float>(b->rate[w]);
+            a.negDecay = static_cast<float>(-b->decay[w]);
+            a.wrapX = wrapX ? 1 : 0;
+            a.wrapY = wrapY ? 1 : 0;
+            a.dirichlet = dir ? 1 : 0;
+            a.vecOk = (Yw % cpt == 0) ? 1 : 0;
+            a.yvEnd = static_cast<float>(fy.vEnd);
+            a.yInv = static_cast<float>(fy.inv);
+            a.xvEnd = static_cast<float>(fx.vEnd);
+            a.xInv = static_cast<float>(fx.inv);
+            a.xibLast = static_cast<float>(fx.ib[static_cast<size_t>(X - 1)]);
+            a.xz0 = static_cast<float>(fx.z[0]);
+            a.xzLast = static_cast<float>(fx.z[static_cast<size_t>(X - 1)]);
+            bw.fieldOffset = b->offset[w];
+            bw.cls = (cpt == 1) ? 0 : (cpt == 2 ? (a.vecOk ? 1 : 2) : (a.vecOk ? 3 : 4));
+
+            b->wellScanBegin[w] = static_cast<int32_t>(scans.size());
+            for (int32_t yb = 0; yb < Yw; yb += kScanCols)
+            {
+                scans.push_back(ScanItem{w, yb});
+            }
+            b->wellScanCount[w] = static_cast<int32_t>(scans.size()) - b->wellScanBegin[w];
+            b->wellStripBegin[w] = static_cast<int32_t>(strips.size());
+            for (int32_t q = 0; q < P; q++)
+            {
+                const int32_t begin = static_cast<int32_t>(srcCursor);
+                while (srcCursor < order.size() && b->srcWell[order[srcCursor]] == w &&
+                       xy[2 * srcCursor] < (q + 1) * S)
+                {
+                    srcCursor++;
+                }
+                strips.push_back(StripItem{w, q, begin, static_cast<int32_t>(srcCursor)});
+            }
+            b->wellStripCount[w] = static_cast<int32_t>(strips.size()) - b->wellStripBegin[w];
+        }
+        // Upload tables and fill the device pointers.
+        b->tables.Allocate(b->ctx, host.size());
+        Check(hipMemcpy(b->tables.get(), host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+        b->red.Allocate(b->ctx, static_cast<size_t>(redTotal));
+        b->carry.Allocate(b->ctx, static_cast<size_t>(redTotal));
+        b->colInfo.Allocate(b->ctx, static_cast<size_t>(colTotal));
+        Check(hipMemset(b->colInfo.get(), 0, static_cast<size_t>(colTotal) * sizeof(float)), "hipMemset");
+        b->srcXY.Allocate(b->ctx, std::max<size_t>(1, xy.size()));
+        b->srcValDev.Allocate(b->ctx, std::max<size_t>(1, sval.size()));
+        if (!xy.empty())
+        {
+            Check(hipMemcpy(b->srcXY.get(), xy.data(), xy.size() * sizeof(int32_t), hipMemcpyHostToDevice), "hipMemcpy");
+            Check(hipMemcpy(b->srcValDev.get(), sval.data(), sval.size() * sizeof(float), hipMemcpyHostToDevice),
+                  "hipMemcpy");
+        }
+        float *t = b->tables.get();
+        for (int32_t w = 0; w < b->nWells; w++)
+        {
+            AdiArgs &a = wells[w].args;
+            const Offs &o = offs[w];
+            a.xfw = t + o.xfw;
+            a.xib = t + o.xib;
+            a.xbw = t + o.xbw;
+            a.xz = t + o.xz;
+            a.yfw = t + o.yfw;
+            a.yib = t + o.yib;
+            a.ybw = t + o.ybw;
+            a.yz = t + o.yz;
+            a.stripA = t + o.A;
+            a.stripC = t + o.C;
+            a.stripE = t + o.E;
+            a.bcXlo = t + o.bxl;
+            a.bcXhi = t + o.bxh;
+            a.bcYlo = t + o.byl;
+            a.bcYhi = t + o.byh;
+            a.red = b->red.get() + redOff[w];
+            a.carry = b->carry.get() + redOff[w];
+            a.colInfo = b->colInfo.get() + colOff[w];
+            a.srcXY = b->srcXY.get();
+            a.srcVal = b->srcValDev.get();
+        }
+        b->wells.Allocate(b->ctx, wells.size());
+        Check(hipMemcpy(b->wells.get(), wells.data(), wells.size() * sizeof(BatchWell), hipMemcpyHostToDevice),
+              "hipMemcpy");
+        b->stripItems.Allocate(b->ctx, strips.size());
+        Check(hipMemcpy(b->stripItems.get(), strips.data(), strips.size() * sizeof(StripItem), hipMemcpyHostToDevice),
+              "hipMemcpy");
+        b->scanItems.Allocate(b->ctx, scans.size());
+        Check(hipMemcpy(b->scanItems.get(), scans.data(), scans.size() * sizeof(ScanItem), hipMemcpyHostToDevice),
+              "hipMemcpy");
+        b->nStripItems = static_cast<int32_t>(strips.size());
+        b->nScanItems = static_cast<int32_t>(scans.size());
+        b->builtKey = key;
+        b->reducedValid = false;
+        b->graphs.Clear();
+    }
+
+    void LaunchBatchStrip(halhip_batch *b, int mode, int32_t itemBegin, int32_t itemCount, const char *name)
+    {
+        b->prof.Begin(b->clock.stream, name);
+        hipLaunchKernelGGL(PickBatchStrip(b->rows, mode), dim3(static_cast<unsigned>(itemCount)), dim3(kBatchThreads),
+                           0, b->clock.stream, static_cast<const BatchWell *>(b->wells.get()),
+                           static_cast<const StripItem *>(b->stripItems.get()) + itemBegin, b->field.get());
+        Check(hipGetLastError(), "BatchStrip launch");
+        b->prof.End(b->clock.stream);
+    }
+
+    void LaunchBatchScan(halhip_batch *b, int32_t itemBegin, int32_t itemCount, const char *name)
+    {
+        b->prof.Begin(b->clock.stream, name);
+        hipLaunchKernelGGL(BatchScan, dim3(static_cast<unsigned>(itemCount)), dim3(kScanCols * kScanSegs), 0,
+                           b->clock.stream, static_cast<const BatchWell *>(b->wells.get()),
+                           static_cast<const ScanItem *>(b->scanItems.get()) + itemBegin);
+        Check(hipGetLastError(), "BatchScan launch");
+        b->prof.End(b->clock.stream);
+    }
+
+    void EnqueueBatchTicks(halhip_batch *b, int32_t ticks, int32_t wrapX, int32_t wrapY, const halhip_bc *bc)
+    {
+        Require(ticks >= 0, "halhip_batch_run_ticks: negative ticks");
+        Build(b, wrapX, wrapY, bc);
+        if (ticks == 0)
+        {
+            return;
+        }
+        if (!b->reducedValid)
+        {
+            LaunchBatchStrip(b, 0, 0, b->nStripItems, "batch_strip_reduce");
+        }
+        constexpr int32_t kChunk = 10;
+        int32_t done = 0;
+        if (!b->prof.On() && ticks >= kChunk)
+        {
+            std::string key = b->builtKey;
+            AppendKey(key, kChunk);
+            hipGraphExec_t exec = b->graphs.Find(key);
+            if (exec == nullptr)
+            {
+                hipGraph_t graph;
+                Check(hipStreamBeginCapture(b->clock.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+                for (int32_t k = 0; k < kChunk; k++)
+                {
+                    LaunchBatchScan(b, 0, b->nScanItems, "batch_scan");
+                    LaunchBatchStrip(b, 3, 0, b->nStripItems, "batch_strip_tick");
+                }
+                Check(hipStreamEndCapture(b->clock.stream, &graph), "hipStreamEndCapture");
+                Check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+                (void)hipGraphDestroy(graph);
+                b->graphs.Insert(key, exec);
+            }
+            for (; done + kChunk <= ticks; done += kChunk)
+            {
+                Check(hipGraphLaunch(exec, b->clock.stream), "hipGraphLaunch");
+            }
+        }
+        for (; done < ticks; done++)
+        {
+            LaunchBatchScan(b, 0, b->nScanItems, "batch_scan");
+            LaunchBatchStrip(b, 3, 0, b->nStripItems, "batch_strip_tick");
+        }
+        b->reducedValid = true;
+    }
+
+    // Baseline: the same kernels launched well by well (two launches per well
+    // per tick), the shape of a per-instance loop over single-grid plans.
+    void EnqueuePerInstanceTicks(halhip_batch *b, int32_t ticks, int32_t wrapX, int32_t wrapY, const halhip_bc *bc)
+    {
+        Require(ticks >= 0, "halhip_batch_run_ticks_per_instance: negative ticks");
+        Build(b, wrapX, wrapY, bc);
+        if (ticks == 0)
+        {
+            return;
+        }
+        for (int32_t w = 0; w < b->nWells; w++)
+        {
+            if (!b->reducedValid)
+            {
+                LaunchBatchStrip(b, 0, b->wellStripBegin[w], b->wellStripCount[w], "instance_strip_reduce");
+            }
+        }
+        for (int32_t t = 0; t < ticks; t++)
+        {
+            for (int32_t w = 0; w < b->nWells; w++)
+            {
+                LaunchBatchScan(b, b->wellScanBegin[w], b->wellScanCount[w], "instance_scan");
+                LaunchBatchStrip(b, 3, b->wellStripBegin[w], b->wellStripCount[w], "instance_strip_tick");
+            }
+        }
+        b->reducedValid = true;
+    }
+
+    std::string Signature(const halhip_batch *b)
+    {
+        uint64_t h = 1469598103934665603ULL;
+        auto mix = [&](uint64_t v)
+        {
+            h ^= v;
+            h *= 1099511628211ULL;
+        };
+        mix(static_cast<uint64_t>(b->nWells));
+        for (int32_t w = 0; w < b->nWells; w++)
+        {
+            mix(static_cast<uint64_t>(b->X[w]) << 32 | static_cast<uint32_t>(b->Y[w]));
+        }
+        std::ostringstream s;
+        s << "gfx950:" << std::hex << h;
+        return s.str();
+    }
+}
+
+extern "C"
+{
+    int halhip_batch_create(halhip_context *ctx, int32_t nWells, const int32_t *xDims, const int32_t *yDims,
+                            halhip_batch **out)
+    {
+        halhip_batch *b = nullptr;
+        int rc = Guard(
+            [&]
+            {
+                Require(out != nullptr, "halhip_batch_create: null out");
+                *out = nullptr;
+                Require(ctx != nullptr, "halhip_batch_create: null context");
+                Require(nWells > 0 && xDims != nullptr && yDims != nullptr, "halhip_batch_create: no wells");
+                Bind(ctx);
+                b = new halhip_batch();
+                b->ctx = ctx;
+                b->nWells = nWells;
+                b->X.assign(xDims, xDims + nWells);
+                b->Y.assign(yDims, yDims + nWells);
+                for (int32_t w = 0; w < nWells; w++)
+                {
+                    Require(b->X[w] > 0 && b->Y[w] > 0, "halhip_batch_create: well dimensions must be positive");
+                    if (b->X[w] > kBatchMaxSide || b->Y[w] > kBatchMaxSide)
+                    {
+                        Fail(HALHIP_ERR_UNSUPPORTED, "halhip_batch_create: well sides are limited to 1024 "
+                                                     "(use a single-grid plan for larger grids)");
+                    }
+                    b->offset.push_back(b->cells);
+                    b->cells += static_cast<int64_t>(b->X[w]) * b->Y[w];
+                }
+                b->rate.assign(static_cast<size_t>(nWells), 0.0);
+                b->decay.assign(static_cast<size_t>(nWells), 0.0);
+                b->field.Allocate(ctx, static_cast<size_t>(b->cells));
+                b->clock.Create();
+                Check(hipMemsetAsync(b->field.get(), 0, static_cast<size_t>(b->cells) * sizeof(float), b->clock.stream),
+                      "hipMemset");
+                Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
+                *out = b;
+            });
+        if (rc != HALHIP_OK && b != nullptr)
+        {
+            b->clock.Destroy();
+            delete b;
+        }
+        return rc;
+    }
+
+    int halhip_batch_destroy(halhip_batch *b)
+    {
+        return Guard(
+            [&]
+            {
+                if (b == nullptr)
+                {
+                    return;
+                }
+                Bind(b->ctx);
+                b->graphs.Clear();
+                b->clock.Destroy();
+                delete b;
+            });
+    }
+
+    int halhip_batch_set_wells(halhip_batch *b, const double *rates, const double *decays)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_set_wells");
+                Require(rates != nullptr, "halhip_batch_set_wells: null rates");
+                for (int32_t w = 0; w < b->nWells; w++)
+                {
+                    Require(std::isfinite(rates[w]) && rates[w] >= 0.0, "halhip_batch_set_wells: bad rate");
+                    Require(decays == nullptr || std::isfinite(decays[w]), "halhip_batch_set_wells: bad decay");
+                    b->rate[w] = rates[w];
+                    b->decay[w] = decays != nullptr ? decays[w] : 0.0;
+                }
+            });
+    }
+
+    int halhip_batch_set_sources(halhip_batch *b, int32_t count, const int32_t *well, const int64_t *cell,
+                                 const float *value)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_set_sources");
+                Require(count >= 0 && (count == 0 || (well != nullptr && cell != nullptr && value != nullptr)),
+                        "halhip_batch_set_sources: bad source arrays");
+                for (int32_t k = 0; k < count; k++)
+                {
+                    Require(well[k] >= 0 && well[k] < b->nWells, "halhip_batch_set_sources: well out of range");
+                    Require(cell[k] >= 0 && cell[k] < static_cast<int64_t>(b->X[well[k]]) * b->Y[well[k]],
+                            "halhip_batch_set_sources: cell out of range");
+                }
+                b->srcWell.assign(well, well + count);
+                b->srcCell.assign(cell, cell + count);
+                b->srcVal.assign(value, value + count);
+            });
+    }
+
+    int halhip_batch_upload(halhip_batch *b, const float *fields, int64_t n)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_upload");
+                Require(fields != nullptr && n == b->cells, "halhip_batch_upload: need the concatenated fields");
+                Check(hipMemcpyAsync(b->field.get(), fields, static_cast<size_t>(n) * sizeof(float),
+                                     hipMemcpyHostToDevice, b->clock.stream),
+                      "hipMemcpyAsync");
+                Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
+                b->reducedValid = false;
+            });
+    }
+
+    int halhip_batch_download(halhip_batch *b, float *fields, int64_t n)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_download");
+                Require(fields != nullptr && n == b->cells, "halhip_batch_download: need room for the concatenated fields");
+                Check(hipMemcpyAsync(fields, b->field.get(), static_cast<size_t>(n) * sizeof(float),
+                                     hipMemcpyDeviceToHost, b->clock.stream),
+                      "hipMemcpyAsync");
+                Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
+            });
+    }
+
+    int halhip_batch_run_ticks_async(halhip_batch *b, int32_t ticks, int32_t wrapX, int32_t wrapY,
+                                     const halhip_bc *bc)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_run_ticks");
+                b->clock.Mark();
+                EnqueueBatchTicks(b, ticks, wrapX, wrapY, bc);
+            });
+    }
+
+    int halhip_batch_run_ticks_per_instance_async(halhip_batch *b, int32_t ticks, int32_t wrapX, int32_t wrapY,
+                                                  const halhip_bc *bc)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_run_ticks_per_instance");
+                b->clock.Mark();
+                EnqueuePerInstanceTicks(b, ticks, wrapX, wrapY, bc);
+            });
+    }
+
+    int halhip_batch_sync(halhip_batch *b, float *elapsedMs)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_sync");
+                b->clock.Sync(elapsedMs);
+                b->prof.Collect();
+            });
+    }
+
+    int halhip_batch_run_ticks(halhip_batch *b, int32_t ticks, int32_t wrapX, int32_t wrapY, const halhip_bc *bc)
+    {
+        int rc = halhip_batch_run_ticks_async(b, ticks, wrapX, wrapY, bc);
+        return rc != HALHIP_OK ? rc : halhip_batch_sync(b, nullptr);
+    }
+
+    int halhip_batch_run_ticks_per_instance(halhip_batch *b, int32_t ticks, int32_t wrapX, int32_t wrapY,
+                                            const halhip_bc *bc)
+    {
+        int rc = halhip_batch_run_ticks_per_instance_async(b, ticks, wrapX, wrapY, bc);
+        return rc != HALHIP_OK ? rc : halhip_batch_sync(b, nullptr);
+    }
+
+    int halhip_batch_set_strip_rows(halhip_batch *b, int32_t rows)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_set_strip_rows");
+                Require(rows == 4 || rows == 8, "halhip_batch_set_strip_rows: rows must be 4 or 8");
+                if (rows != b->rows)
+                {
+                    Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
+                    b->rows = rows;
+                    b->builtKey.clear();
+                    b->reducedValid = false;
+                }
+            });
+    }
+
+    int halhip_batch_get_strip_rows(halhip_batch *b, int32_t *rows)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_get_strip_rows");
+                Require(rows != nullptr, "halhip_batch_get_strip_rows: null output");
+                *rows = b->rows;
+            });
+    }
+
+    int halhip_batch_autotune(halhip_batch *b, int32_t ticks, int32_t wrapX, int32_t wrapY, const halhip_bc *bc,
+                              const char *cachePath, int32_t *chosenRows)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_autotune");
+                Require(ticks > 0, "halhip_batch_autotune: ticks must be positive");
+                const std::string sig = Signature(b);
+                std::map<std::string, int32_t> cache;
+                if (cachePath != nullptr)
+                {
+                    std::ifstream in(cachePath);
+                    std::string k;
+                    int32_t v;
+                    while (in >> k >> v)
+                    {
+                        cache[k] = v;
+                    }
+                }
+                auto it = cache.find(sig);
+                int32_t best = 0;
+                if (it != cache.end() && (it->second == 4 || it->second == 8))
+                {
+                    best = it->second;
+                }
+                else
+                {
+                    // Time each candidate on a scratch copy of the fields so the
+                    // batch's state is unchanged.
+                    DeviceArray<float> saved;
+                    saved.Allocate(b->ctx, static_cast<size_t>(b->cells));
+                    Check(hipMemcpyAsync(saved.get(), b->field.get(), static_cast<size_t>(b->cells) * sizeof(float),
+                                         hipMemcpyDeviceToDevice, b->clock.stream),
+                          "hipMemcpyAsync");
+                    float bestMs = 0.0f;
+                    for (int32_t rows : {4, 8})
+                    {
+                        b->rows = rows;
+                        b->builtKey.clear();
+                        b->reducedValid = false;
+                        EnqueueBatchTicks(b, 2, wrapX, wrapY, bc);
+                        Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
+                        b->clock.Mark();
+                        EnqueueBatchTicks(b, ticks, wrapX, wrapY, bc);
+                        float ms = 0.0f;
+                        b->clock.Sync(&ms);
+                        if (best == 0 || ms < bestMs)
+                        {
+                            best = rows;
+                            bestMs = ms;
+                        }
+                        Check(hipMemcpyAsync(b->field.get(), saved.get(), static_cast<size_t>(b->cells) * sizeof(float),
+                                             hipMemcpyDeviceToDevice, b->clock.stream),
+                              "hipMemcpyAsync");
+                    }
+                    Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
+                    if (cachePath != nullptr)
+                    {
+                        cache[sig] = best;
+                        std::ofstream outFile(cachePath, std::ios::trunc);
+                        for (auto &kv : cache)
+                        {
+                            outFile << kv.first << " " << kv.second << "\n";
+                        }
+                    }
+                }
+                if (best != b->rows)
+                {
+                    b->rows = best;
+                    b->builtKey.clear();
+                }
+                b->reducedValid = false;
+                if (chosenRows != nullptr)
+                {
+                    *chosenRows = best;
+                }
+            });
+    }
+
+    int halhip_batch_set_profiling(halhip_batch *b, int32_t on)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_set_profiling");
+                b->prof.Reset();
+                b->prof.SetOn(on != 0);
+            });
+    }
+
+    int halhip_batch_profile(halhip_batch *b, int32_t maxKernels, int32_t *nKernels, char *names, int32_t namesLen,
+                             double *totalMs, int64_t *launches)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_profile");
+                b->prof.Export(maxKernels, nKernels, names, namesLen, totalMs, launches);
+            });
+    }
+}

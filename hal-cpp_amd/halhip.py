@@ -86,6 +86,22 @@ _SIGS = {
     "halhip_adi2d_get_strip_shape": [_vp, _ip, _ip, _ip],
     "halhip_adi2d_set_profiling": [_vp, _i32],
     "halhip_adi2d_profile": [_vp, _i32, _ip, ctypes.c_char_p, _i32, _dp, _lp],
+    "halhip_batch_create": [_vp, _i32, _ip, _ip, ctypes.POINTER(_vp)],
+    "halhip_batch_destroy": [_vp],
+    "halhip_batch_set_wells": [_vp, _dp, _dp],
+    "halhip_batch_set_sources": [_vp, _i32, _ip, _lp, _fp],
+    "halhip_batch_upload": [_vp, _fp, _i64],
+    "halhip_batch_download": [_vp, _fp, _i64],
+    "halhip_batch_run_ticks": [_vp, _i32, _i32, _i32, ctypes.POINTER(HalBC)],
+    "halhip_batch_run_ticks_async": [_vp, _i32, _i32, _i32, ctypes.POINTER(HalBC)],
+    "halhip_batch_run_ticks_per_instance": [_vp, _i32, _i32, _i32, ctypes.POINTER(HalBC)],
+    "halhip_batch_run_ticks_per_instance_async": [_vp, _i32, _i32, _i32, ctypes.POINTER(HalBC)],
+    "halhip_batch_sync": [_vp, _fp],
+    "halhip_batch_set_strip_rows": [_vp, _i32],
+    "halhip_batch_get_strip_rows": [_vp, _ip],
+    "halhip_batch_autotune": [_vp, _i32, _i32, _i32, ctypes.POINTER(HalBC), ctypes.c_char_p, _ip],
+    "halhip_batch_set_profiling": [_vp, _i32],
+    "halhip_batch_profile": [_vp, _i32, _ip, ctypes.c_char_p, _i32, _dp, _lp],
     "halhip_agents2d_create": [_vp, _i32, _i32, _i32, ctypes.POINTER(_vp)],
     "halhip_agents2d_destroy": [_vp],
     "halhip_agents2d_forces_f64": [_vp, _dp, _dp, _i32, _dp, _f64, _f64, _i32, _i32],
@@ -393,3 +409,66 @@ class Agents2D(_Plan):
         bucket = np.empty(max(self.n, 1), np.int32)
         _check(self._fn("download_bins")(self.h, offsets.ctypes.data_as(_ip), bucket.ctypes.data_as(_ip)))
         return offsets, bucket[:self.n]
+
+
+class Batch(_Plan):
+    """Batched wells of uneven size (one batched launch per phase per tick)."""
+    _prefix = "batch"
+
+    def __init__(self, ctx, xdims, ydims):
+        xdims = np.ascontiguousarray(xdims, np.int32)
+        ydims = np.ascontiguousarray(ydims, np.int32)
+        self.ctx = ctx
+        self.xdims, self.ydims = xdims, ydims
+        self.cells = int((xdims.astype(np.int64) * ydims).sum())
+        self.offsets = np.concatenate([[0], np.cumsum(xdims.astype(np.int64) * ydims)])
+        self.h = _vp()
+        _check(load().halhip_batch_create(ctx.h, xdims.size, xdims.ctypes.data_as(_ip), ydims.ctypes.data_as(_ip),
+                                          ctypes.byref(self.h)))
+
+    def set_wells(self, rates, decays=None):
+        rates = _f64(rates)
+        decays = None if decays is None else _f64(decays)
+        _check(self._fn("set_wells")(self.h, rates.ctypes.data_as(_dp),
+                                     None if decays is None else decays.ctypes.data_as(_dp)))
+
+    def set_sources(self, well, cell, value):
+        well = np.ascontiguousarray(well, np.int32)
+        cell = np.ascontiguousarray(cell, np.int64)
+        value = _f32(value)
+        _check(self._fn("set_sources")(self.h, well.size, well.ctypes.data_as(_ip), cell.ctypes.data_as(_lp),
+                                       value.ctypes.data_as(_fp)))
+
+    def upload(self, fields):
+        fields = _f32(fields)
+        _check(self._fn("upload")(self.h, fields.ctypes.data_as(_fp), fields.size))
+
+    def download(self):
+        out = np.empty(self.cells, np.float32)
+        _check(self._fn("download")(self.h, out.ctypes.data_as(_fp), out.size))
+        return out
+
+    def well(self, fields, w):
+        return fields[self.offsets[w]:self.offsets[w + 1]]
+
+    def run_ticks(self, ticks, wrapX=False, wrapY=False, bc=None, asynchronous=False, per_instance=False):
+        s, keep = make_bc(bc, 1, 1) if (bc is None or np.isscalar(bc)) else (None, ())
+        name = "run_ticks_per_instance" if per_instance else "run_ticks"
+        if asynchronous:
+            name += "_async"
+        _check(self._fn(name)(self.h, ticks, int(wrapX), int(wrapY), _bcp(s)))
+
+    def set_strip_rows(self, rows):
+        _check(self._fn("set_strip_rows")(self.h, rows))
+
+    def strip_rows(self):
+        r = ctypes.c_int32()
+        _check(self._fn("get_strip_rows")(self.h, ctypes.byref(r)))
+        return r.value
+
+    def autotune(self, ticks, wrapX=False, wrapY=False, bc=None, cache_path=None):
+        s, keep = make_bc(bc, 1, 1) if (bc is None or np.isscalar(bc)) else (None, ())
+        r = ctypes.c_int32()
+        _check(self._fn("autotune")(self.h, ticks, int(wrapX), int(wrapY), _bcp(s),
+                                    None if cache_path is None else cache_path.encode(), ctypes.byref(r)))
+        return r.value

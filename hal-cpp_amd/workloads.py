@@ -44,3 +44,22 @@ def agents_uniform(n_agents=4_000_000, dim=2048, seed=42):
     pos = (u * dim).astype(np.float32)
     rad = np.full(n_agents, 0.5, np.float32)
     return dict(X=dim, Y=dim, pos=pos, rad=rad, R=1.0, k=1.0, dt=0.1, wrapX=False, wrapY=False)
+
+
+def wells_batch(n_wells=512, seed=42, lo=64, hi=1024, sources_per_well=32):
+    """Batched wells: per well, in order from Rng(seed): side =
+    round(exp(U(ln lo, ln hi))), D ~ U[0.1, 5], decay ~ U[0, 1e-2], then
+    `sources_per_well` point sources (x = floor(side*U), y = floor(side*U),
+    strength U[0,1]).  Square wells, zero flux, initial field 0."""
+    per = 3 + 3 * sources_per_well
+    u = halhip.rng_doubles(seed, n_wells * per).reshape(n_wells, per)
+    side = np.rint(np.exp(np.log(lo) + u[:, 0] * (np.log(hi) - np.log(lo)))).astype(np.int32)
+    rate = 0.1 + 4.9 * u[:, 1]
+    decay = 1e-2 * u[:, 2]
+    src = u[:, 3:].reshape(n_wells, sources_per_well, 3)
+    sx = np.floor(src[:, :, 0] * side[:, None]).astype(np.int64)
+    sy = np.floor(src[:, :, 1] * side[:, None]).astype(np.int64)
+    well = np.repeat(np.arange(n_wells, dtype=np.int32), sources_per_well)
+    cell = (sx * side[:, None] + sy).ravel()
+    value = src[:, :, 2].ravel().astype(np.float32)
+    return dict(xdims=side, ydims=side.copy(), rate=rate, decay=decay, src_well=well, src_cell=cell, src_val=value)

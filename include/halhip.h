@@ -57,6 +57,7 @@ typedef struct halhip_context halhip_context;
 typedef struct halhip_diffusion2d halhip_diffusion2d;
 typedef struct halhip_adi2d halhip_adi2d;
 typedef struct halhip_agents2d halhip_agents2d;
+typedef struct halhip_batch halhip_batch;
 
 /* ---- library / context -------------------------------------------------- */
 
@@ -218,6 +219,51 @@ int halhip_agents2d_sync(halhip_agents2d *plan, float *elapsedMs);
 int halhip_agents2d_set_profiling(halhip_agents2d *plan, int32_t on);
 int halhip_agents2d_profile(halhip_agents2d *plan, int32_t maxKernels, int32_t *nKernels, char *names,
                             int32_t namesLen, double *totalMs, int64_t *launches);
+
+/* ---- batched wells --------------------------------------------------------- */
+
+/* Many independent ADI grids ("wells") of uneven size ticked together, each
+ * tick u_w <- ADI(u_w; rate_w) - decay_w*u_w + sources_w, i.e. per well
+ * PDEGrid2D::DiffusionADI + MulAll(-decay) + Add + Update
+ * (include/Hal/Grids/PDEGrid2D.h:103-218, 328-347).  Replaces the host-side
+ * MultiWellExperiment::Step / StepMultiThread loop
+ * (include/Hal/Tools/MultiWellExperiment.h:82-103).  Well sides <= 1024.
+ * Fields are uploaded and downloaded concatenated in well order, each well in
+ * HAL's x*yDim+y layout. */
+int halhip_batch_create(halhip_context *ctx, int32_t nWells, const int32_t *xDims, const int32_t *yDims,
+                        halhip_batch **out);
+int halhip_batch_destroy(halhip_batch *batch);
+/* Per-well diffusion rate and decay (decays may be null: 0). */
+int halhip_batch_set_wells(halhip_batch *batch, const double *rates, const double *decays);
+/* Point sources added every tick: well index, flat cell index in that well,
+ * value.  Repeated cells add in list order. */
+int halhip_batch_set_sources(halhip_batch *batch, int32_t count, const int32_t *well, const int64_t *cell,
+                             const float *value);
+int halhip_batch_upload(halhip_batch *batch, const float *fields, int64_t n);
+int halhip_batch_download(halhip_batch *batch, float *fields, int64_t n);
+/* `ticks` ticks of every well: two batched launches per tick, replayed from a
+ * hipGraph.  bc: null, or a constant Dirichlet value for every well. */
+int halhip_batch_run_ticks(halhip_batch *batch, int32_t ticks, int32_t wrapX, int32_t wrapY, const halhip_bc *bc);
+int halhip_batch_run_ticks_async(halhip_batch *batch, int32_t ticks, int32_t wrapX, int32_t wrapY,
+                                 const halhip_bc *bc);
+/* The same ticks as a per-instance launch loop (the same kernels, one well at
+ * a time): the baseline the batched launch is measured against. */
+int halhip_batch_run_ticks_per_instance(halhip_batch *batch, int32_t ticks, int32_t wrapX, int32_t wrapY,
+                                        const halhip_bc *bc);
+int halhip_batch_run_ticks_per_instance_async(halhip_batch *batch, int32_t ticks, int32_t wrapX, int32_t wrapY,
+                                              const halhip_bc *bc);
+int halhip_batch_sync(halhip_batch *batch, float *elapsedMs);
+/* Strip height of the batched kernels (4 or 8 rows). */
+int halhip_batch_set_strip_rows(halhip_batch *batch, int32_t rows);
+int halhip_batch_get_strip_rows(halhip_batch *batch, int32_t *rows);
+/* Times every strip height on `ticks` ticks (state is restored afterwards)
+ * and keeps the fastest; with a cache file, results are stored per batch
+ * signature (well shapes, gfx target) and reused. */
+int halhip_batch_autotune(halhip_batch *batch, int32_t ticks, int32_t wrapX, int32_t wrapY, const halhip_bc *bc,
+                          const char *cachePath, int32_t *chosenRows);
+int halhip_batch_set_profiling(halhip_batch *batch, int32_t on);
+int halhip_batch_profile(halhip_batch *batch, int32_t maxKernels, int32_t *nKernels, char *names, int32_t namesLen,
+                         double *totalMs, int64_t *launches);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,105 @@
+"""Batched wells on the GPU vs the FP64 oracle run well by well
+(PDEGrid2D::DiffusionADI + MulAll + Add + Update per well), and vs the
+per-instance launch loop of the same kernels."""
+import numpy as np
+import pytest
+
+import halhip
+import workloads
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_wells(oracle, batch, fields, w, ticks, wrap=(0, 0), bc=None):
+    out = []
+    for k in range(len(batch["xdims"])):
+        X, Y = int(batch["xdims"][k]), int(batch["ydims"][k])
+        sel = batch["src_well"] == k
+        f = w.well(fields, k).astype(np.float64)
+        out.append(oracle.adi_ticks(f, ticks, float(batch["rate"][k]), float(batch["decay"][k]), X, Y, *wrap, bc=bc,
+                                    src_idx=batch["src_cell"][sel], src_val=batch["src_val"][sel].astype(np.float64)))
+    return out
+
+
+def make(ctx, spec, rows):
+    b = halhip.Batch(ctx, spec["xdims"], spec["ydims"])
+    b.set_wells(spec["rate"], spec["decay"])
+    b.set_sources(spec["src_well"], spec["src_cell"], spec["src_val"])
+    b.set_strip_rows(rows)
+    return b
+
+
+@pytest.mark.parametrize("rows", [4, 8])
+def test_small_batch_vs_oracle(ctx, oracle, rows):
+    spec = workloads.wells_batch(n_wells=12, seed=7, lo=5, hi=300, sources_per_well=6)
+    b = make(ctx, spec, rows)
+    rng = np.random.default_rng(1)
+    f0 = rng.random(b.cells).astype(np.float32)
+    b.upload(f0)
+    b.run_ticks(12)
+    got = b.download()
+    ref = oracle_wells(oracle, spec, f0, b, 12)
+    for k in range(len(spec["xdims"])):
+        err = rel_l2(b.well(got, k), ref[k])
+        assert err < 1e-5, (k, spec["xdims"][k], err)
+
+
+def test_wide_wells_and_boundaries(ctx, oracle):
+    # widths that select 1, 2 and 4 columns per thread, vector and scalar rows
+    xd = np.array([3, 40, 17, 70, 9], np.int32)
+    yd = np.array([256, 300, 512, 1000, 1024], np.int32)
+    spec = dict(xdims=xd, ydims=yd, rate=np.array([0.3, 5.0, 2.0, 1.0, 4.0]),
+                decay=np.array([0.0, 1e-3, 5e-3, 0.0, 1e-2]),
+                src_well=np.array([0, 1, 1, 3, 4], np.int32),
+                src_cell=np.array([5, 0, 299 + 300 * 39, 70 * 1000 - 1, 4000], np.int64),
+                src_val=np.array([1.0, 0.5, 0.25, 2.0, 1.5], np.float32))
+    b = make(ctx, spec, 8)
+    f0 = np.random.default_rng(2).random(b.cells).astype(np.float32)
+    for wrap, bc in (((0, 0), None), ((1, 0), 0.2), ((0, 1), None)):
+        b.upload(f0)
+        b.run_ticks(7, *wrap, bc=bc)
+        got = b.download()
+        ref = oracle_wells(oracle, spec, f0, b, 7, wrap, bc)
+        for k in range(5):
+            assert rel_l2(b.well(got, k), ref[k]) < 1e-5, (k, wrap, bc)
+
+
+def test_per_instance_loop_matches_batched(ctx):
+    spec = workloads.wells_batch(n_wells=40, seed=3, lo=16, hi=512)
+    b = make(ctx, spec, 8)
+    b.upload(np.zeros(b.cells, np.float32))
+    b.run_ticks(25)
+    batched = b.download()
+    b.upload(np.zeros(b.cells, np.float32))
+    b.run_ticks(25, per_instance=True)
+    single = b.download()
+    np.testing.assert_array_equal(batched, single)
+
+
+def test_autotune_keeps_state_and_caches(ctx, tmp_path):
+    spec = workloads.wells_batch(n_wells=20, seed=5, lo=32, hi=400)
+    b = make(ctx, spec, 8)
+    f0 = np.random.default_rng(3).random(b.cells).astype(np.float32)
+    b.upload(f0)
+    cache = str(tmp_path / "tune.txt")
+    rows = b.autotune(5, cache_path=cache)
+    assert rows in (4, 8)
+    np.testing.assert_array_equal(b.download(), f0)
+    assert open(cache).read().strip().endswith(str(rows))
+    assert b.autotune(5, cache_path=cache) == rows
+
+
+def test_workload_512_wells_first_ticks(ctx, oracle):
+    spec = workloads.wells_batch()
+    b = make(ctx, spec, 8)
+    b.run_ticks(3)
+    got = b.download()
+    # check a sample of wells (small, median, largest) against the oracle
+    order = np.argsort(spec["xdims"])
+    for k in (int(order[0]), int(order[len(order) // 2]), int(order[-1])):
+        X = int(spec["xdims"][k])
+        sel = spec["src_well"] == k
+        ref = oracle.adi_ticks(np.zeros(X * X), 3, float(spec["rate"][k]), float(spec["decay"][k]), X, X, 0, 0,
+                               src_idx=spec["src_cell"][sel], src_val=spec["src_val"][sel].astype(np.float64))
+        assert rel_l2(b.well(got, k), ref) < 1e-5, (k, X)
