@@ -1,0 +1,827 @@
+// adjoint2d.hip -- per-cell-diffusivity ADI, its discrete adjoint and the
+// gradient of a field misfit with respect to every cell's diffusivity and to
+// the decay rate, with checkpoint-and-recompute.  No reference counterpart:
+// the forward model generalises Diffusion2DADI (include/Hal/Pde/ADIequations.h:306-337)
+// to a per-cell rate with the face convention of the per-cell explicit
+// stencil (src/Pde/PDEequations.cpp:214-219; the face between cells i and i+1
+// carries D_i), zero-flux boundaries, and appends MulAll(-k) + Update.
+//
+// Tick (N = X*Y cells, u in HAL's x*Y+y layout, uT its transpose):
+//   s1 = (I + L_y/2) u;  s2 = (I - L_x/2)^{-1} s1;  s3 = (I + L_x/2) s2;
+//   s4 = (I - L_y/2)^{-1} s3;  u' = s4 - k u.
+// Every factor is symmetric, so the adjoint tick applies them transposed in
+// reverse order:  m1 = (I - L_y/2)^{-1} lam;  m2 = (I - L_x/2)^{-1} (I + L_x/2) m1;
+//   lam' = (I + L_y/2) m2 - k lam,
+// and the per-tick gradient contributions are
+//   dJ/dD_c -= 1/2 (dy m1 . dy s4 + dx m1 . dx s2 + dx m2 . dx s2 + dy m2 . dy u)
+//   dJ/dk   -= <lam, u>
+// with dx p = p(x+1,y) - p(x,y) (0 on the last row), dy likewise.
+//
+// Kernels: a line solve (one wavefront per line, lane-local recurrences plus a
+// wavefront scan of affine maps; the per-cell Thomas factors are computed once
+// per diffusivity field on the host in FP64), an LDS-tiled transpose so each
+// sweep reads its lines contiguously, and a fused gradient / adjoint-update
+// kernel.  Checkpoints every K ticks; the backward pass recomputes each
+// segment from its checkpoint, keeping u, s2, s4 of the segment.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+using namespace halhip;
+
+namespace
+{
+    constexpr int kLineThreads = 256; // four lines per block
+
+    template <int CTRL, int ROW_MASK = 0xF>
+    __device__ __forceinline__ float DppF(float old, float v)
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                                      __builtin_bit_cast(int, v), CTRL, ROW_MASK,
+                                                                      0xF, false));
+    }
+
+    struct LineArgs
+    {
+        int32_t nLines;
+        int32_t L;
+        const float *in;   // [nLines][L]
+        const float *rate; // [nLines][L] face rates of this line family
+        const float *fw;   // [nLines][L] Thomas factors
+        const float *ib;
+        const float *bw;
+        float *out;        // [nLines][L]
+        const float *prev; // decay: out = x - k * prev
+        float negK;
+        float *stash;      // optional copy of x before decay
+    };
+
+    // RHS 0: d = c + 1/2 (rate[l] (in[l+1] - c) + rate[l-1] (in[l-1] - c)) across lines.
+    // RHS 1: the same along the line (neighbours j+-1 of the same line).
+    // RHS 2: d = in.
+    template <int EPL, int RHS, bool DECAY, bool STASH>
+    __global__ __launch_bounds__(kLineThreads) void CellLineSolve(LineArgs a)
+    {
+        const int lane = threadIdx.x & 63;
+        const int line = blockIdx.x * (kLineThreads / 64) + (threadIdx.x >> 6);
+        if (line >= a.nLines)
+        {
+            return;
+        }
+        const int L = a.L;
+        const size_t base = static_cast<size_t>(line) * L;
+        const int j0 = lane * EPL;
+        float d[EPL];
+        float fw[EPL];
+#pragma unroll
+        for (int e = 0; e < EPL; e++)
+        {
+            const int j = j0 + e;
+            const bool in = j < L;
+            const float c = in ? a.in[base + j] : 0.0f;
+            d[e] = c;
+            fw[e] = in ? a.fw[base + j] : 1.0f;
+        }
+        if constexpr (RHS == 0)
+        {
+#pragma unroll
+            for (int e = 0; e < EPL; e++)
+            {
+                const int j = j0 + e;
+                if (j < L)
+                {
+                    const float c = d[e];
+                    float s = 0.0f;
+                    if (line + 1 < a.nLines)
+                    {
+                        s += a.rate[base + j] * (a.in[base + L + j] - c);
+                    }
+                    if (line > 0)
+                    {
+                        s += a.rate[base - L + j] * (a.in[base - L + j] - c);
+                    }
+                    d[e] = fmaf(0.5f, s, c);
+                }
+            }
+        }
+        else if constexpr (RHS == 1)
+        {
+            float r[EPL];
+#pragma unroll
+            for (int e = 0; e < EPL; e++)
+            {
+                r[e] = (j0 + e < L) ? a.rate[base + j0 + e] : 0.0f;
+            }
+            const float cLeft = DppF<0x138>(0.0f, d[EPL - 1]); // previous lane's last value
+            const float rLeft = DppF<0x138>(0.0f, r[EPL - 1]); // and its face rate
+            const float cRight = DppF<0x130>(0.0f, d[0]);      // next lane's first value
+            float out[EPL];
+#pragma unroll
+            for (int e = 0; e < EPL; e++)
+            {
+                const int j = j0 + e;
+                const float c = d[e];
+                const float cp = (e + 1 < EPL) ? d[e + 1] : cRight;
+                const float cm = (e > 0) ? d[e - 1] : cLeft;
+                const float rm = (e > 0) ? r[e - 1] : rLeft;
+                float s = 0.0f;
+                s += (j + 1 < L) ? r[e] * (cp - c) : 0.0f;
+                s += (j > 0) ? rm * (cm - c) : 0.0f;
+                out[e] = (j < L) ? fmaf(0.5f, s, c) : 0.0f;
+            }
+#pragma unroll
+            for (int e = 0; e < EPL; e++)
+            {
+                d[e] = out[e];
+            }
+        }
+
+        // Forward recurrence d'_j = fw_j d'_{j-1} + d_j: lane-local, then a
+        // wavefront scan of (A, B) with the carry into the line equal to 0.
+        float A = 1.0f;
+        float B = 0.0f;
+#pragma unroll
+        for (int e = 0; e < EPL; e++)
+        {
+            B = fmaf(fw[e], B, d[e]);
+            A *= fw[e];
+        }
+        {
+            float t;
+            t = DppF<0x111>(0.0f, B);
+            B = fmaf(A, t, B);
+            A *= DppF<0x111>(1.0f, A);
+            t = DppF<0x112>(0.0f, B);
+            B = fmaf(A, t, B);
+            A *= DppF<0x112>(1.0f, A);
+            t = DppF<0x114>(0.0f, B);
+            B = fmaf(A, t, B);
+            A *= DppF<0x114>(1.0f, A);
+            t = DppF<0x118>(0.0f, B);
+            B = fmaf(A, t, B);
+            A *= DppF<0x118>(1.0f, A);
+            t = DppF<0x142, 0xA>(0.0f, B);
+            B = fmaf(A, t, B);
+            A *= DppF<0x142, 0xA>(1.0f, A);
+            t = DppF<0x143, 0xC>(0.0f, B);
+            B = fmaf(A, t, B);
+        }
+        float acc = DppF<0x138>(0.0f, B); // carry into this lane
+#pragma unroll
+        for (int e = 0; e < EPL; e++)
+        {
+            acc = fmaf(fw[e], acc, d[e]);
+            d[e] = acc;
+        }
+
+        // Backward recurrence x_j = bw_j x_{j+1} + ib_j d'_j from the right.
+        float bw[EPL];
+        float ib[EPL];
+#pragma unroll
+        for (int e = 0; e < EPL; e++)
+        {
+            const int j = j0 + e;
+            const bool in = j < L;
+            bw[e] = in ? a.bw[base + j] : 0.0f;
+            ib[e] = in ? a.ib[base + j] : 0.0f;
+        }
+        A = 1.0f;
+        B = 0.0f;
+#pragma unroll
+        for (int e = EPL - 1; e >= 0; e--)
+        {
+            B = fmaf(bw[e], B, ib[e] * d[e]);
+            A *= bw[e];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; k++)
+        {
+            const int o = 1 << k;
+            const float Bd = __shfl_down(B, o);
+            const float Ad = __shfl_down(A, o);
+            if (lane + o < 64)
+            {
+                B = fmaf(A, Bd, B);
+                A *= Ad;
+            }
+        }
+        float carry = __shfl_down(B, 1);
+        acc = (lane == 63) ? 0.0f : carry;
+#pragma unroll
+        for (int e = EPL - 1; e >= 0; e--)
+        {
+            acc = fmaf(bw[e], acc, ib[e] * d[e]);
+            d[e] = acc;
+        }
+#pragma unroll
+        for (int e = 0; e < EPL; e++)
+        {
+            const int j = j0 + e;
+            if (j < L)
+            {
+                if constexpr (STASH)
+                {
+                    a.stash[base + j] = d[e];
+                }
+                float v = d[e];
+                if constexpr (DECAY)
+                {
+                    v = fmaf(a.negK, a.prev[base + j], v);
+                }
+                a.out[base + j] = v;
+            }
+        }
+    }
+
+    // out[c][r] = in[r][c] for an R x C matrix, through 64 x 64 LDS tiles.
+    __global__ __launch_bounds__(256) void Transpose(const float *__restrict__ in, float *__restrict__ out, int32_t R,
+                                                     int32_t C)
+    {
+        __shared__ float tile[64][65];
+        const int c0 = blockIdx.x * 64;
+        const int r0 = blockIdx.y * 64;
+        const int tx = threadIdx.x & 63;
+        const int ty = threadIdx.x >> 6;
+        for (int k = ty; k < 64; k += 4)
+        {
+            const int r = r0 + k;
+            const int c = c0 + tx;
+            tile[k][tx] = (r < R && c < C) ? in[static_cast<size_t>(r) * C + c] : 0.0f;
+        }
+        __syncthreads();
+        for (int k = ty; k < 64; k += 4)
+        {
+            const int c = c0 + k;
+            const int r = r0 + tx;
+            if (c < C && r < R)
+            {
+                out[static_cast<size_t>(c) * R + r] = tile[tx][k];
+            }
+        }
+    }
+
+    struct GradArgs
+    {
+        int32_t X;
+        int32_t Y;
+        const float *D;   // [X][Y]
+        const float *m1;  // [X][Y]
+        const float *m2;  // [X][Y]
+        const float *s2;  // [X][Y]
+        const float *s4;  // [X][Y]
+        const float *u;   // [X][Y]
+        const float *lam; // [X][Y]
+        float *lamNext;   // [X][Y]
+        float *gradD;     // [X][Y], accumulated
+        float *partialK;  // one per block
+        float negK;
+    };
+
+    // Per-tick gradient contributions and lam' = (I + L_y/2) m2 - k lam.
+    __global__ __launch_bounds__(256) void CellGradient(GradArgs a)
+    {
+        __shared__ float red[4];
+        const int y = blockIdx.x * 64 + (threadIdx.x & 63);
+        const int x = blockIdx.y * 4 + (threadIdx.x >> 6);
+        float kPart = 0.0f;
+        if (x < a.X && y < a.Y)
+        {
+            const size_t i = static_cast<size_t>(x) * a.Y + y;
+            const bool hx = x + 1 < a.X;
+            const bool hy = y + 1 < a.Y;
+            const size_t ix = hx ? i + a.Y : i;
+            const size_t iy = hy ? i + 1 : i;
+            const float dyM1 = a.m1[iy] - a.m1[i];
+            const float dyS4 = a.s4[iy] - a.s4[i];
+            const float dxM1 = a.m1[ix] - a.m1[i];
+            const float dxS2 = a.s2[ix] - a.s2[i];
+            const float dxM2 = a.m2[ix] - a.m2[i];
+            const float dyM2 = a.m2[iy] - a.m2[i];
+            const float dyU = a.u[iy] - a.u[i];
+            const float g = dyM1 * dyS4 + (dxM1 + dxM2) * dxS2 + dyM2 * dyU;
+            a.gradD[i] = fmaf(-0.5f, g, a.gradD[i]);
+            const float lam = a.lam[i];
+            kPart = -lam * a.u[i];
+            // (I + L_y/2) m2 along the row
+            const float c = a.m2[i];
+            float s = 0.0f;
+            if (hy)
+            {
+                s += a.D[i] * (a.m2[i + 1] - c);
+            }
+            if (y > 0)
+            {
+                s += a.D[i - 1] * (a.m2[i - 1] - c);
+            }
+            a.lamNext[i] = fmaf(a.negK, lam, fmaf(0.5f, s, c));
+        }
+        // Block sum of the decay-gradient part (deterministic: fixed tree).
+        for (int o = 32; o > 0; o >>= 1)
+        {
+            kPart += __shfl_down(kPart, o);
+        }
+        if ((threadIdx.x & 63) == 0)
+        {
+            red[threadIdx.x >> 6] = kPart;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            a.partialK[blockIdx.y * gridDim.x + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+        }
+    }
+
+    // loss partials: 1/2 (u - target)^2 per block, and lam = u - target.
+    __global__ __launch_bounds__(256) void LossSeed(const float *__restrict__ u, const float *__restrict__ target,
+                                                    float *__restrict__ lam, int64_t n, double *__restrict__ partial)
+    {
+        __shared__ double red[4];
+        double s = 0.0;
+        for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+             i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        {
+            const float r = u[i] - target[i];
+            lam[i] = r;
+            s += 0.5 * static_cast<double>(r) * static_cast<double>(r);
+        }
+        for (int o = 32; o > 0; o >>= 1)
+        {
+            s += __shfl_down(s, o);
+        }
+        if ((threadIdx.x & 63) == 0)
+        {
+            red[threadIdx.x >> 6] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+        }
+    }
+
+    // Thomas factors of the line family: lines of length L with face rates
+    // rate[line][j] (face j..j+1), zero flux.
+    void FactorFamily(int32_t nLines, int32_t L, const std::vector<double> &rate, std::vector<float> &fw,
+                      std::vector<float> &ib, std::vector<float> &bw)
+    {
+        const size_t n = static_cast<size_t>(nLines) * L;
+        fw.assign(n, 0.0f);
+        ib.assign(n, 0.0f);
+        bw.assign(n, 0.0f);
+        for (int32_t l = 0; l < nLines; l++)
+        {
+            const size_t base = static_cast<size_t>(l) * L;
+            double bpPrev = 0.0;
+            for (int32_t j = 0; j < L; j++)
+            {
+                const double dm = (j > 0) ? rate[base + j - 1] : 0.0;
+                const double dp = (j < L - 1) ? rate[base + j] : 0.0;
+                const double aj = -0.5 * dm;
+                const double cj = -0.5 * dp;
+                const double cPrev = (j > 0) ? -0.5 * rate[base + j - 1] : 0.0;
+                double bp = 1.0 + 0.5 * (dm + dp);
+                double w = 0.0;
+                if (j > 0)
+                {
+                    w = aj / bpPrev;
+                    bp -= w * cPrev;
+                }
+                fw[base + j] = static_cast<float>(-w);
+                ib[base + j] = static_cast<float>(1.0 / bp);
+                bw[base + j] = static_cast<float>((j < L - 1) ? -cj / bp : 0.0);
+                bpPrev = bp;
+            }
+        }
+    }
+
+    int Epl(int32_t L)
+    {
+        int e = 1;
+        while (64 * e < L)
+        {
+            e *= 2;
+        }
+        return e;
+    }
+}
+
+struct halhip_adjoint2d
+{
+    halhip_context *ctx = nullptr;
+    int32_t X = 0;
+    int32_t Y = 0;
+    int32_t ticks = 0;
+    int32_t every = 1;
+    float decay = 0.0f;
+    bool ready = false;
+    DeviceArray<float> D, DT;              // rates in both layouts
+    DeviceArray<float> fA, iA, bA;         // x-line factors, [Y][X]
+    DeviceArray<float> fB, iB, bB;         // y-line factors, [X][Y]
+    DeviceArray<float> u, uT, s2T, work;   // field work buffers
+    DeviceArray<float> ckpt;               // (ticks/every + 1) checkpoints
+    DeviceArray<float> stashU, stashS2, stashS4; // one segment
+    DeviceArray<float> lam, lamNext, m1, m1T, m2T, m2, grad;
+    DeviceArray<float> partialK;
+    DeviceArray<double> partialJ;
+    StreamClock clock;
+    Profiler prof;
+};
+
+namespace
+{
+    size_t Cells(const halhip_adjoint2d *p)
+    {
+        return static_cast<size_t>(p->X) * p->Y;
+    }
+
+    void RequireAdj(const halhip_adjoint2d *p, const char *caller)
+    {
+        Require(p != nullptr, std::string(caller) + ": null plan");
+        Bind(p->ctx);
+    }
+
+    template <int RHS, bool DECAY, bool STASH>
+    void LaunchLine(halhip_adjoint2d *p, const LineArgs &a, const char *name)
+    {
+        const unsigned blocks = static_cast<unsigned>((a.nLines + 3) / 4);
+        p->prof.Begin(p->clock.stream, name);
+        switch (Epl(a.L))
+        {
+            case 1:
+                hipLaunchKernelGGL((CellLineSolve<1, RHS, DECAY, STASH>), dim3(blocks), dim3(kLineThreads), 0,
+                                   p->clock.stream, a);
+                break;
+            case 2:
+                hipLaunchKernelGGL((CellLineSolve<2, RHS, DECAY, STASH>), dim3(blocks), dim3(kLineThreads), 0,
+                                   p->clock.stream, a);
+                break;
+            case 4:
+                hipLaunchKernelGGL((CellLineSolve<4, RHS, DECAY, STASH>), dim3(blocks), dim3(kLineThreads), 0,
+                                   p->clock.stream, a);
+                break;
+            case 8:
+                hipLaunchKernelGGL((CellLineSolve<8, RHS, DECAY, STASH>), dim3(blocks), dim3(kLineThreads), 0,
+                                   p->clock.stream, a);
+                break;
+            case 16:
+                hipLaunchKernelGGL((CellLineSolve<16, RHS, DECAY, STASH>), dim3(blocks), dim3(kLineThreads), 0,
+                                   p->clock.stream, a);
+                break;
+            case 32:
+                hipLaunchKernelGGL((CellLineSolve<32, RHS, DECAY, STASH>), dim3(blocks), dim3(kLineThreads), 0,
+                                   p->clock.stream, a);
+                break;
+            default:
+                hipLaunchKernelGGL((CellLineSolve<64, RHS, DECAY, STASH>), dim3(blocks), dim3(kLineThreads), 0,
+                                   p->clock.stream, a);
+                break;
+        }
+        Check(hipGetLastError(), name);
+        p->prof.End(p->clock.stream);
+    }
+
+    void LaunchTranspose(halhip_adjoint2d *p, const float *in, float *out, int32_t R, int32_t C)
+    {
+        p->prof.Begin(p->clock.stream, "adjoint_transpose");
+        hipLaunchKernelGGL(Transpose, dim3(static_cast<unsigned>((C + 63) / 64), static_cast<unsigned>((R + 63) / 64)),
+                           dim3(256), 0, p->clock.stream, in, out, R, C);
+        Check(hipGetLastError(), "Transpose launch");
+        p->prof.End(p->clock.stream);
+    }
+
+    LineArgs FamilyA(halhip_adjoint2d *p)
+    {
+        LineArgs a{};
+        a.nLines = p->Y;
+        a.L = p->X;
+        a.rate = p->DT.get();
+        a.fw = p->fA.get();
+        a.ib = p->iA.get();
+        a.bw = p->bA.get();
+        return a;
+    }
+
+    LineArgs FamilyB(halhip_adjoint2d *p)
+    {
+        LineArgs a{};
+        a.nLines = p->X;
+        a.L = p->Y;
+        a.rate = p->D.get();
+        a.fw = p->fB.get();
+        a.ib = p->iB.get();
+        a.bw = p->bB.get();
+        return a;
+    }
+
+    // One forward tick from `in` to `out` (both [X][Y]); with stash pointers,
+    // keeps s2 and s4 ([X][Y]).
+    void ForwardTick(halhip_adjoint2d *p, const float *in, float *out, float *s2, float *s4)
+    {
+        const int32_t X = p->X;
+        const int32_t Y = p->Y;
+        LaunchTranspose(p, in, p->uT.get(), X, Y);
+        LineArgs a = FamilyA(p);
+        a.in = p->uT.get();
+        a.out = p->s2T.get();
+        LaunchLine<0, false, false>(p, a, "adjoint_fwd_solve_x");
+        float *s2dst = (s2 != nullptr) ? s2 : p->work.get();
+        LaunchTranspose(p, p->s2T.get(), s2dst, Y, X);
+        LineArgs b = FamilyB(p);
+        b.in = s2dst;
+        b.out = out;
+        b.prev = in;
+        b.negK = -p->decay;
+        if (s4 != nullptr)
+        {
+            b.stash = s4;
+            LaunchLine<0, true, true>(p, b, "adjoint_fwd_solve_y");
+        }
+        else
+        {
+            LaunchLine<0, true, false>(p, b, "adjoint_fwd_solve_y");
+        }
+    }
+
+    // Runs the forward model from u0 (device) storing checkpoints; result in p->u.
+    void Forward(halhip_adjoint2d *p)
+    {
+        const size_t n = Cells(p);
+        Check(hipMemcpyAsync(p->ckpt.get(), p->u.get(), n * sizeof(float), hipMemcpyDeviceToDevice, p->clock.stream),
+              "hipMemcpyAsync");
+        for (int32_t t = 0; t < p->ticks; t++)
+        {
+            ForwardTick(p, p->u.get(), p->lamNext.get(), nullptr, nullptr);
+            p->u.SwapWith(p->lamNext);
+            if ((t + 1) % p->every == 0 && t + 1 < p->ticks)
+            {
+                const size_t slot = static_cast<size_t>((t + 1) / p->every);
+                Check(hipMemcpyAsync(p->ckpt.get() + slot * n, p->u.get(), n * sizeof(float), hipMemcpyDeviceToDevice,
+                                     p->clock.stream),
+                      "hipMemcpyAsync");
+            }
+        }
+    }
+}
+
+extern "C"
+{
+    int halhip_adjoint2d_create(halhip_context *ctx, int32_t xDim, int32_t yDim, halhip_adjoint2d **out)
+    {
+        halhip_adjoint2d *p = nullptr;
+        int rc = Guard(
+            [&]
+            {
+                Require(out != nullptr, "halhip_adjoint2d_create: null out");
+                *out = nullptr;
+                Require(ctx != nullptr, "halhip_adjoint2d_create: null context");
+                Require(xDim > 0 && yDim > 0, "halhip_adjoint2d_create: dimensions must be positive");
+                if (xDim > 4096 || yDim > 4096)
+                {
+                    Fail(HALHIP_ERR_UNSUPPORTED, "halhip_adjoint2d_create: sides are limited to 4096");
+                }
+                Bind(ctx);
+                p = new halhip_adjoint2d();
+                p->ctx = ctx;
+                p->X = xDim;
+                p->Y = yDim;
+                const size_t n = Cells(p);
+                for (DeviceArray<float> *b : {&p->D, &p->DT, &p->fA, &p->iA, &p->bA, &p->fB, &p->iB, &p->bB, &p->u,
+                                              &p->uT, &p->s2T, &p->work, &p->lam, &p->lamNext, &p->m1, &p->m1T,
+                                              &p->m2T, &p->m2, &p->grad})
+                {
+                    b->Allocate(ctx, n);
+                }
+                p->partialJ.Allocate(ctx, 1024);
+                p->clock.Create();
+                *out = p;
+            });
+        if (rc != HALHIP_OK && p != nullptr)
+        {
+            p->clock.Destroy();
+            delete p;
+        }
+        return rc;
+    }
+
+    int halhip_adjoint2d_destroy(halhip_adjoint2d *p)
+    {
+        return Guard(
+            [&]
+            {
+                if (p == nullptr)
+                {
+                    return;
+                }
+                Bind(p->ctx);
+                p->clock.Destroy();
+                delete p;
+            });
+    }
+
+    int halhip_adjoint2d_set_problem(halhip_adjoint2d *p, const double *diffusivity, double decay, int32_t ticks,
+                                     int32_t checkpointEvery)
+    {
+        return Guard(
+            [&]
+            {
+                RequireAdj(p, "halhip_adjoint2d_set_problem");
+                Require(diffusivity != nullptr, "halhip_adjoint2d_set_problem: null diffusivity");
+                Require(ticks >= 1 && checkpointEvery >= 1, "halhip_adjoint2d_set_problem: ticks and checkpoint >= 1");
+                Require(std::isfinite(decay), "halhip_adjoint2d_set_problem: bad decay");
+                const int32_t X = p->X;
+                const int32_t Y = p->Y;
+                const size_t n = Cells(p);
+                std::vector<double> Dxy(diffusivity, diffusivity + n);
+                std::vector<double> Dyx(n);
+                for (int32_t x = 0; x < X; x++)
+                {
+                    for (int32_t y = 0; y < Y; y++)
+                    {
+                        const double v = Dxy[static_cast<size_t>(x) * Y + y];
+                        Require(std::isfinite(v) && v >= 0.0, "halhip_adjoint2d_set_problem: diffusivity must be finite "
+                                                               "and non-negative");
+                        Dyx[static_cast<size_t>(y) * X + x] = v;
+                    }
+                }
+                std::vector<float> f, i, b;
+                auto up = [&](DeviceArray<float> &dst, const std::vector<float> &src)
+                {
+                    Check(hipMemcpy(dst.get(), src.data(), src.size() * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+                };
+                std::vector<float> Df(Dxy.begin(), Dxy.end()), DTf(Dyx.begin(), Dyx.end());
+                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                up(p->D, Df);
+                up(p->DT, DTf);
+                FactorFamily(Y, X, Dyx, f, i, b);
+                up(p->fA, f);
+                up(p->iA, i);
+                up(p->bA, b);
+                FactorFamily(X, Y, Dxy, f, i, b);
+                up(p->fB, f);
+                up(p->iB, i);
+                up(p->bB, b);
+                p->decay = static_cast<float>(decay);
+                p->ticks = ticks;
+                p->every = checkpointEvery;
+                const size_t nCk = static_cast<size_t>((ticks + checkpointEvery - 1) / checkpointEvery);
+                p->ckpt.Allocate(p->ctx, nCk * n);
+                const size_t seg = static_cast<size_t>(checkpointEvery);
+                p->stashU.Allocate(p->ctx, seg * n);
+                p->stashS2.Allocate(p->ctx, seg * n);
+                p->stashS4.Allocate(p->ctx, seg * n);
+                const size_t gblocks = static_cast<size_t>((Y + 63) / 64) * static_cast<size_t>((X + 3) / 4);
+                p->partialK.Allocate(p->ctx, gblocks * static_cast<size_t>(ticks));
+                p->ready = true;
+            });
+    }
+
+    int halhip_adjoint2d_forward(halhip_adjoint2d *p, const float *u0, float *uT)
+    {
+        return Guard(
+            [&]
+            {
+                RequireAdj(p, "halhip_adjoint2d_forward");
+                Require(p->ready, "halhip_adjoint2d_forward: call halhip_adjoint2d_set_problem first");
+                Require(u0 != nullptr && uT != nullptr, "halhip_adjoint2d_forward: null array");
+                const size_t n = Cells(p);
+                Check(hipMemcpyAsync(p->u.get(), u0, n * sizeof(float), hipMemcpyHostToDevice, p->clock.stream),
+                      "hipMemcpyAsync");
+                Forward(p);
+                Check(hipMemcpyAsync(uT, p->u.get(), n * sizeof(float), hipMemcpyDeviceToHost, p->clock.stream),
+                      "hipMemcpyAsync");
+                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                p->prof.Collect();
+            });
+    }
+
+    int halhip_adjoint2d_gradient(halhip_adjoint2d *p, const float *u0, const float *target, double *loss,
+                                  float *gradDiffusivity, double *gradDecay)
+    {
+        return Guard(
+            [&]
+            {
+                RequireAdj(p, "halhip_adjoint2d_gradient");
+                Require(p->ready, "halhip_adjoint2d_gradient: call halhip_adjoint2d_set_problem first");
+                Require(u0 != nullptr && target != nullptr && loss != nullptr && gradDiffusivity != nullptr &&
+                            gradDecay != nullptr,
+                        "halhip_adjoint2d_gradient: null array");
+                const int32_t X = p->X;
+                const int32_t Y = p->Y;
+                const size_t n = Cells(p);
+                hipStream_t s = p->clock.stream;
+                Check(hipMemcpyAsync(p->u.get(), u0, n * sizeof(float), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+                Forward(p);
+                // Seed: lam = u_T - target, J = 1/2 |lam|^2.  The target goes
+                // through the work buffer.
+                Check(hipMemcpyAsync(p->work.get(), target, n * sizeof(float), hipMemcpyHostToDevice, s),
+                      "hipMemcpyAsync");
+                const unsigned lossBlocks = 1024;
+                hipLaunchKernelGGL(LossSeed, dim3(lossBlocks), dim3(256), 0, s, p->u.get(), p->work.get(), p->lam.get(),
+                                   static_cast<int64_t>(n), p->partialJ.get());
+                Check(hipGetLastError(), "LossSeed launch");
+                Check(hipMemsetAsync(p->grad.get(), 0, n * sizeof(float), s), "hipMemsetAsync");
+                const dim3 gGrid(static_cast<unsigned>((Y + 63) / 64), static_cast<unsigned>((X + 3) / 4));
+                const size_t gBlocks = static_cast<size_t>(gGrid.x) * gGrid.y;
+                std::vector<float> hostK(gBlocks * static_cast<size_t>(p->ticks));
+                double gk = 0.0;
+                const int32_t nSeg = (p->ticks + p->every - 1) / p->every;
+                for (int32_t seg = nSeg - 1; seg >= 0; seg--)
+                {
+                    const int32_t t0 = seg * p->every;
+                    const int32_t t1 = std::min(p->ticks, t0 + p->every);
+                    // Recompute the segment from its checkpoint, keeping u, s2, s4.
+                    Check(hipMemcpyAsync(p->stashU.get(), p->ckpt.get() + static_cast<size_t>(seg) * n,
+                                         n * sizeof(float), hipMemcpyDeviceToDevice, s),
+                          "hipMemcpyAsync");
+                    for (int32_t t = t0; t < t1; t++)
+                    {
+                        const size_t k = static_cast<size_t>(t - t0);
+                        float *next = (t + 1 < t1) ? p->stashU.get() + (k + 1) * n : p->work.get();
+                        ForwardTick(p, p->stashU.get() + k * n, next, p->stashS2.get() + k * n,
+                                    p->stashS4.get() + k * n);
+                    }
+                    // Adjoint ticks backwards through the segment.
+                    for (int32_t t = t1 - 1; t >= t0; t--)
+                    {
+                        const size_t k = static_cast<size_t>(t - t0);
+                        LineArgs b = FamilyB(p);
+                        b.in = p->lam.get();
+                        b.out = p->m1.get();
+                        LaunchLine<2, false, false>(p, b, "adjoint_solve_y");
+                        LaunchTranspose(p, p->m1.get(), p->m1T.get(), X, Y);
+                        LineArgs a = FamilyA(p);
+                        a.in = p->m1T.get();
+                        a.out = p->m2T.get();
+                        LaunchLine<1, false, false>(p, a, "adjoint_solve_x");
+                        LaunchTranspose(p, p->m2T.get(), p->m2.get(), Y, X);
+                        GradArgs g{};
+                        g.X = X;
+                        g.Y = Y;
+                        g.D = p->D.get();
+                        g.m1 = p->m1.get();
+                        g.m2 = p->m2.get();
+                        g.s2 = p->stashS2.get() + k * n;
+                        g.s4 = p->stashS4.get() + k * n;
+                        g.u = p->stashU.get() + k * n;
+                        g.lam = p->lam.get();
+                        g.lamNext = p->lamNext.get();
+                        g.gradD = p->grad.get();
+                        g.partialK = p->partialK.get() + static_cast<size_t>(t) * gBlocks;
+                        g.negK = -p->decay;
+                        p->prof.Begin(s, "adjoint_gradient");
+                        hipLaunchKernelGGL(CellGradient, gGrid, dim3(256), 0, s, g);
+                        Check(hipGetLastError(), "CellGradient launch");
+                        p->prof.End(s);
+                        p->lam.SwapWith(p->lamNext);
+                    }
+                }
+                std::vector<double> hostJ(lossBlocks);
+                Check(hipMemcpyAsync(hostJ.data(), p->partialJ.get(), lossBlocks * sizeof(double),
+                                     hipMemcpyDeviceToHost, s),
+                      "hipMemcpyAsync");
+                Check(hipMemcpyAsync(gradDiffusivity, p->grad.get(), n * sizeof(float), hipMemcpyDeviceToHost, s),
+                      "hipMemcpyAsync");
+                Check(hipMemcpyAsync(hostK.data(), p->partialK.get(), hostK.size() * sizeof(float),
+                                     hipMemcpyDeviceToHost, s),
+                      "hipMemcpyAsync");
+                Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+                double J = 0.0;
+                for (double v : hostJ)
+                {
+                    J += v;
+                }
+                for (float v : hostK)
+                {
+                    gk += static_cast<double>(v);
+                }
+                *loss = J;
+                *gradDecay = gk;
+                p->prof.Collect();
+            });
+    }
+
+    int halhip_adjoint2d_set_profiling(halhip_adjoint2d *p, int32_t on)
+    {
+        return Guard(
+            [&]
+            {
+                RequireAdj(p, "halhip_adjoint2d_set_profiling");
+                p->prof.Reset();
+                p->prof.SetOn(on != 0);
+            });
+    }
+
+    int halhip_adjoint2d_profile(halhip_adjoint2d *p, int32_t maxKernels, int32_t *nKernels, char *names,
+                                 int32_t namesLen, double *totalMs, int64_t *launches)
+    {
+        return Guard(
+            [&]
+            {
+                RequireAdj(p, "halhip_adjoint2d_profile");
+                p->prof.Export(maxKernels, nKernels, names, namesLen, totalMs, launches);
+            });
+    }
+}

@@ -126,6 +126,11 @@ namespace halhip
         }
         T *get() const { return m_ptr; }
         size_t size() const { return m_count; }
+        void SwapWith(DeviceArray &other)
+        {
+            std::swap(m_ptr, other.m_ptr);
+            std::swap(m_count, other.m_count);
+        }
 
     private:
         T *m_ptr = nullptr;

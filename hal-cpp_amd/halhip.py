@@ -102,6 +102,13 @@ _SIGS = {
     "halhip_batch_autotune": [_vp, _i32, _i32, _i32, ctypes.POINTER(HalBC), ctypes.c_char_p, _ip],
     "halhip_batch_set_profiling": [_vp, _i32],
     "halhip_batch_profile": [_vp, _i32, _ip, ctypes.c_char_p, _i32, _dp, _lp],
+    "halhip_adjoint2d_create": [_vp, _i32, _i32, ctypes.POINTER(_vp)],
+    "halhip_adjoint2d_destroy": [_vp],
+    "halhip_adjoint2d_set_problem": [_vp, _dp, _f64, _i32, _i32],
+    "halhip_adjoint2d_forward": [_vp, _fp, _fp],
+    "halhip_adjoint2d_gradient": [_vp, _fp, _fp, _dp, _fp, _dp],
+    "halhip_adjoint2d_set_profiling": [_vp, _i32],
+    "halhip_adjoint2d_profile": [_vp, _i32, _ip, ctypes.c_char_p, _i32, _dp, _lp],
     "halhip_agents2d_create": [_vp, _i32, _i32, _i32, ctypes.POINTER(_vp)],
     "halhip_agents2d_destroy": [_vp],
     "halhip_agents2d_forces_f64": [_vp, _dp, _dp, _i32, _dp, _f64, _f64, _i32, _i32],
@@ -472,3 +479,34 @@ class Batch(_Plan):
         _check(self._fn("autotune")(self.h, ticks, int(wrapX), int(wrapY), _bcp(s),
                                     None if cache_path is None else cache_path.encode(), ctypes.byref(r)))
         return r.value
+
+
+class Adjoint2D(_Plan):
+    """Per-cell-diffusivity ADI with adjoint gradients (zero flux)."""
+    _prefix = "adjoint2d"
+
+    def __init__(self, ctx, X, Y):
+        self.ctx, self.X, self.Y = ctx, X, Y
+        self.h = _vp()
+        _check(load().halhip_adjoint2d_create(ctx.h, X, Y, ctypes.byref(self.h)))
+
+    def set_problem(self, diffusivity, decay, ticks, checkpoint_every):
+        D = _f64(diffusivity)
+        assert D.size == self.X * self.Y
+        _check(self._fn("set_problem")(self.h, D.ctypes.data_as(_dp), decay, ticks, checkpoint_every))
+
+    def forward(self, u0):
+        u0 = _f32(u0)
+        out = np.empty(self.X * self.Y, np.float32)
+        _check(self._fn("forward")(self.h, u0.ctypes.data_as(_fp), out.ctypes.data_as(_fp)))
+        return out
+
+    def gradient(self, u0, target):
+        """(J, dJ/dD float32 array, dJ/dk) for J = 1/2 |u_T - target|^2."""
+        u0, target = _f32(u0), _f32(target)
+        J = ctypes.c_double(0.0)
+        gk = ctypes.c_double(0.0)
+        gD = np.empty(self.X * self.Y, np.float32)
+        _check(self._fn("gradient")(self.h, u0.ctypes.data_as(_fp), target.ctypes.data_as(_fp), ctypes.byref(J),
+                                    gD.ctypes.data_as(_fp), ctypes.byref(gk)))
+        return J.value, gD, gk.value

@@ -63,3 +63,29 @@ def wells_batch(n_wells=512, seed=42, lo=64, hi=1024, sources_per_well=32):
     cell = (sx * side[:, None] + sy).ravel()
     value = src[:, :, 2].ravel().astype(np.float32)
     return dict(xdims=side, ydims=side.copy(), rate=rate, decay=decay, src_well=well, src_cell=cell, src_val=value)
+
+
+def mode_mixture(n, u, modes=8):
+    """Sum of `modes` plane sine waves w*sin(2 pi (kx x + ky y)/n + phi) with
+    kx, ky = 1 + floor(8U), w = 0.5 + 0.5U, phi = 2 pi U drawn in that order
+    from u (5 values per mode), scaled to a maximum magnitude of 1."""
+    x = np.arange(n, dtype=np.float64)[:, None]
+    y = np.arange(n, dtype=np.float64)[None, :]
+    m = np.zeros((n, n))
+    for k in range(modes):
+        kx, ky, w, ph = 1 + np.floor(8 * u[5 * k]), 1 + np.floor(8 * u[5 * k + 1]), 0.5 + 0.5 * u[5 * k + 2], \
+            2 * np.pi * u[5 * k + 3]
+        m += w * np.sin(2 * np.pi * (kx * x + ky * y) / n + ph)
+    return m / np.abs(m).max()
+
+
+def adjoint_problem(n=1024, ticks=200, every=20, seed=42):
+    """Adjoint workload: D = 1 + 0.5 M1 (M1 an 8-mode mixture from Rng(seed),
+    so D in [0.5, 1.5]), decay 5e-3, 200 ticks with a checkpoint every 20;
+    the target diffusivity is D * (1 + 0.05 M2) with a second mixture, and
+    the initial field 16 Gaussian blobs (sigma 20) from Rng(seed + 1)."""
+    u = halhip.rng_doubles(seed, 80)
+    D = 1.0 + 0.5 * mode_mixture(n, u[:40])
+    Dt = D * (1.0 + 0.05 * mode_mixture(n, u[40:]))
+    u0 = adi_blobs(n=n, blobs=16, sigma=20.0, seed=seed + 1)["field"]
+    return dict(X=n, Y=n, D=D.ravel(), D_target=Dt.ravel(), u0=u0, decay=5e-3, ticks=ticks, every=every)

@@ -58,6 +58,7 @@ typedef struct halhip_diffusion2d halhip_diffusion2d;
 typedef struct halhip_adi2d halhip_adi2d;
 typedef struct halhip_agents2d halhip_agents2d;
 typedef struct halhip_batch halhip_batch;
+typedef struct halhip_adjoint2d halhip_adjoint2d;
 
 /* ---- library / context -------------------------------------------------- */
 
@@ -264,6 +265,31 @@ int halhip_batch_autotune(halhip_batch *batch, int32_t ticks, int32_t wrapX, int
 int halhip_batch_set_profiling(halhip_batch *batch, int32_t on);
 int halhip_batch_profile(halhip_batch *batch, int32_t maxKernels, int32_t *nKernels, char *names, int32_t namesLen,
                          double *totalMs, int64_t *launches);
+
+/* ---- adjoint ADI --------------------------------------------------------------- */
+
+/* Per-cell-diffusivity ADI on an xDim x yDim grid with zero-flux boundaries,
+ * its discrete adjoint and gradients.  No reference counterpart: the forward
+ * tick generalises PDEGrid2D::DiffusionADI (include/Hal/Pde/ADIequations.h:306-337)
+ * to a per-cell rate D with the face convention of the per-cell explicit
+ * stencil (src/Pde/PDEequations.cpp:214-219) and appends MulAll(-decay) +
+ * Update.  Sides <= 4096. */
+int halhip_adjoint2d_create(halhip_context *ctx, int32_t xDim, int32_t yDim, halhip_adjoint2d **out);
+int halhip_adjoint2d_destroy(halhip_adjoint2d *plan);
+/* Diffusivity field (xDim*yDim, HAL layout), decay rate, number of ticks
+ * and checkpoint interval (the backward pass recomputes each interval from
+ * its checkpoint). */
+int halhip_adjoint2d_set_problem(halhip_adjoint2d *plan, const double *diffusivity, double decay, int32_t ticks,
+                                 int32_t checkpointEvery);
+/* u_T from u0 (host FP32 arrays of xDim*yDim). */
+int halhip_adjoint2d_forward(halhip_adjoint2d *plan, const float *u0, float *uT);
+/* J = 1/2 |u_T - target|^2 and its gradients with respect to every cell's
+ * diffusivity (gradDiffusivity, xDim*yDim) and to the decay rate. */
+int halhip_adjoint2d_gradient(halhip_adjoint2d *plan, const float *u0, const float *target, double *loss,
+                              float *gradDiffusivity, double *gradDecay);
+int halhip_adjoint2d_set_profiling(halhip_adjoint2d *plan, int32_t on);
+int halhip_adjoint2d_profile(halhip_adjoint2d *plan, int32_t maxKernels, int32_t *nKernels, char *names,
+                             int32_t namesLen, double *totalMs, int64_t *launches);
 
 #ifdef __cplusplus
 }

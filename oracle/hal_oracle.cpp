@@ -672,3 +672,216 @@ extern "C"
         }
     }
 }
+
+// ---- per-cell-diffusivity ADI and its adjoint (no reference counterpart) ----
+//
+// Forward model, generalising Diffusion2DADI (include/Hal/Pde/ADIequations.h:306-337)
+// to a per-cell rate D with the face convention of the per-cell explicit
+// stencil (src/Pde/PDEequations.cpp:214-219): the face between cells i and
+// i+1 along an axis carries D_i.  Zero-flux boundaries.  With
+//   (L_x u)_i = D_i (u_{i+1} - u_i) + D_{i-1} (u_{i-1} - u_i)   (absent faces dropped)
+// a tick is
+//   s1 = (I + L_y/2) u ;  s2 = (I - L_x/2)^{-1} s1 ;
+//   s3 = (I + L_x/2) s2 ; s4 = (I - L_y/2)^{-1} s3 ;  u' = s4 - k u,
+// which for uniform D = r is exactly the reference's zero-flux ADI tick plus
+// MulAll(-k).  The loss is J = 1/2 |u_T - target|^2.
+namespace
+{
+    // Line solve (I - L/2) x = d along one line of a strided field:
+    // element j of the line is at base + j*stride; the line's face rates are
+    // Dline[j] (face j..j+1) at the same addressing.
+    void CellLineSolve(int32_t n, const double *D, size_t base, size_t stride, const double *d, double *x,
+                       std::vector<double> &work)
+    {
+        work.resize(3 * static_cast<size_t>(n));
+        double *a = work.data();
+        double *b = a + n;
+        double *c = b + n;
+        for (int32_t j = 0; j < n; j++)
+        {
+            const double dm = (j > 0) ? D[base + (j - 1) * stride] : 0.0;
+            const double dp = (j < n - 1) ? D[base + j * stride] : 0.0;
+            a[j] = -0.5 * dm;
+            c[j] = -0.5 * dp;
+            b[j] = 1.0 + 0.5 * (dm + dp);
+        }
+        std::vector<double> dd(static_cast<size_t>(n)), xx(static_cast<size_t>(n)), w2;
+        for (int32_t j = 0; j < n; j++)
+        {
+            dd[j] = d[base + j * stride];
+        }
+        Thomas(n, a, b, c, dd.data(), xx.data(), w2);
+        for (int32_t j = 0; j < n; j++)
+        {
+            x[base + j * stride] = xx[j];
+        }
+    }
+
+    // out = (I + sign*L/2) in along x (axis 0) or y (axis 1).
+    void CellExplicit(int32_t X, int32_t Y, const double *D, const double *in, double *out, int axis, double sign)
+    {
+        for (int32_t x = 0; x < X; x++)
+        {
+            for (int32_t y = 0; y < Y; y++)
+            {
+                const size_t i = static_cast<size_t>(x) * Y + y;
+                const double c = in[i];
+                double s = 0.0;
+                if (axis == 0)
+                {
+                    if (x + 1 < X)
+                    {
+                        s += D[i] * (in[i + Y] - c);
+                    }
+                    if (x > 0)
+                    {
+                        s += D[i - Y] * (in[i - Y] - c);
+                    }
+                }
+                else
+                {
+                    if (y + 1 < Y)
+                    {
+                        s += D[i] * (in[i + 1] - c);
+                    }
+                    if (y > 0)
+                    {
+                        s += D[i - 1] * (in[i - 1] - c);
+                    }
+                }
+                out[i] = c + sign * 0.5 * s;
+            }
+        }
+    }
+
+    // x-implicit (axis 0: lines along x, one per y) or y-implicit solve.
+    void CellSolve(int32_t X, int32_t Y, const double *D, const double *d, double *x, int axis)
+    {
+        std::vector<double> work;
+        if (axis == 0)
+        {
+            for (int32_t y = 0; y < Y; y++)
+            {
+                CellLineSolve(X, D, static_cast<size_t>(y), static_cast<size_t>(Y), d, x, work);
+            }
+        }
+        else
+        {
+            for (int32_t xx = 0; xx < X; xx++)
+            {
+                CellLineSolve(Y, D, static_cast<size_t>(xx) * Y, 1, d, x, work);
+            }
+        }
+    }
+
+    // One forward tick; s2 and s4 receive the intermediate states if non-null.
+    void CellTick(int32_t X, int32_t Y, const double *D, double k, const double *u, double *uNext, double *s2,
+                  double *s4)
+    {
+        const size_t n = static_cast<size_t>(X) * Y;
+        std::vector<double> t1(n), t2(n), t3(n), t4(n);
+        CellExplicit(X, Y, D, u, t1.data(), 1, 1.0);
+        CellSolve(X, Y, D, t1.data(), t2.data(), 0);
+        CellExplicit(X, Y, D, t2.data(), t3.data(), 0, 1.0);
+        CellSolve(X, Y, D, t3.data(), t4.data(), 1);
+        for (size_t i = 0; i < n; i++)
+        {
+            uNext[i] = t4[i] - k * u[i];
+        }
+        if (s2 != nullptr)
+        {
+            std::copy(t2.begin(), t2.end(), s2);
+        }
+        if (s4 != nullptr)
+        {
+            std::copy(t4.begin(), t4.end(), s4);
+        }
+    }
+}
+
+extern "C"
+{
+    // `ticks` forward ticks of the per-cell-diffusivity model; u updated in place.
+    void oracle_adi_cell_forward(double *u, const double *D, double k, int32_t ticks, int32_t X, int32_t Y)
+    {
+        const size_t n = static_cast<size_t>(X) * Y;
+        std::vector<double> next(n);
+        for (int32_t t = 0; t < ticks; t++)
+        {
+            CellTick(X, Y, D, k, u, next.data(), nullptr, nullptr);
+            std::copy(next.begin(), next.end(), u);
+        }
+    }
+
+    // Transposed tick: lambdaOut = M^T lambdaIn with M the tick's linear map
+    // at fixed D (the factors are symmetric, so M^T applies them in reverse
+    // order: (I - L_y/2)^{-1}, (I + L_x/2), (I - L_x/2)^{-1}, (I + L_y/2), - k).
+    void oracle_adi_cell_tick_transpose(const double *lambdaIn, double *lambdaOut, const double *D, double k,
+                                        int32_t X, int32_t Y)
+    {
+        const size_t n = static_cast<size_t>(X) * Y;
+        std::vector<double> m1(n), w(n), m2(n);
+        CellSolve(X, Y, D, lambdaIn, m1.data(), 1);
+        CellExplicit(X, Y, D, m1.data(), w.data(), 0, 1.0);
+        CellSolve(X, Y, D, w.data(), m2.data(), 0);
+        CellExplicit(X, Y, D, m2.data(), lambdaOut, 1, 1.0);
+        for (size_t i = 0; i < n; i++)
+        {
+            lambdaOut[i] -= k * lambdaIn[i];
+        }
+    }
+
+    // Loss J = 1/2 |u_T - target|^2 after `ticks` ticks from u0, and its
+    // gradient with respect to every cell's D (gradD, X*Y) and to k (gradK),
+    // by the discrete adjoint with every intermediate stored.  Returns J.
+    double oracle_adi_cell_gradient(const double *u0, const double *D, double k, int32_t ticks, int32_t X,
+                                    int32_t Y, const double *target, double *gradD, double *gradK)
+    {
+        const size_t n = static_cast<size_t>(X) * Y;
+        std::vector<std::vector<double>> u(static_cast<size_t>(ticks) + 1, std::vector<double>(n));
+        std::vector<std::vector<double>> s2(static_cast<size_t>(ticks), std::vector<double>(n));
+        std::vector<std::vector<double>> s4(static_cast<size_t>(ticks), std::vector<double>(n));
+        std::copy(u0, u0 + n, u[0].begin());
+        for (int32_t t = 0; t < ticks; t++)
+        {
+            CellTick(X, Y, D, k, u[t].data(), u[t + 1].data(), s2[t].data(), s4[t].data());
+        }
+        double J = 0.0;
+        std::vector<double> lam(n), m1(n), w(n), m2(n), next(n);
+        for (size_t i = 0; i < n; i++)
+        {
+            const double r = u[ticks][i] - target[i];
+            J += 0.5 * r * r;
+            lam[i] = r;
+        }
+        std::fill(gradD, gradD + n, 0.0);
+        double gk = 0.0;
+        auto dx = [&](const std::vector<double> &p, size_t i, int32_t x) { return (x + 1 < X) ? p[i + Y] - p[i] : 0.0; };
+        auto dy = [&](const std::vector<double> &p, size_t i, int32_t y) { return (y + 1 < Y) ? p[i + 1] - p[i] : 0.0; };
+        for (int32_t t = ticks - 1; t >= 0; t--)
+        {
+            CellSolve(X, Y, D, lam.data(), m1.data(), 1);
+            CellExplicit(X, Y, D, m1.data(), w.data(), 0, 1.0);
+            CellSolve(X, Y, D, w.data(), m2.data(), 0);
+            for (int32_t x = 0; x < X; x++)
+            {
+                for (int32_t y = 0; y < Y; y++)
+                {
+                    const size_t i = static_cast<size_t>(x) * Y + y;
+                    const double g = dy(m1, i, y) * dy(s4[t], i, y) + dx(m1, i, x) * dx(s2[t], i, x) +
+                                     dx(m2, i, x) * dx(s2[t], i, x) + dy(m2, i, y) * dy(u[t], i, y);
+                    gradD[i] -= 0.5 * g;
+                    gk -= lam[i] * u[t][i];
+                }
+            }
+            CellExplicit(X, Y, D, m2.data(), next.data(), 1, 1.0);
+            for (size_t i = 0; i < n; i++)
+            {
+                next[i] -= k * lam[i];
+            }
+            lam.swap(next);
+        }
+        *gradK = gk;
+        return J;
+    }
+}

@@ -173,6 +173,35 @@ class Oracle(_Lib):
         return pos
 
 
+    def adi_cell_forward(self, u0, D, k, ticks, X, Y):
+        u = _f(u0).copy()
+        D = _f(D)
+        f = self.fn("adi_cell_forward")
+        f.argtypes = [_dp, _dp, _f64, _i32, _i32, _i32]
+        f(_p(u), _p(D), k, ticks, X, Y)
+        return u
+
+    def adi_cell_tick_transpose(self, lam, D, k, X, Y):
+        lam = _f(lam)
+        D = _f(D)
+        out = np.empty_like(lam)
+        f = self.fn("adi_cell_tick_transpose")
+        f.argtypes = [_dp, _dp, _dp, _f64, _i32, _i32]
+        f(_p(lam), _p(out), _p(D), k, X, Y)
+        return out
+
+    def adi_cell_gradient(self, u0, D, k, ticks, X, Y, target):
+        """(J, dJ/dD (X*Y), dJ/dk) for J = 1/2 |u_T - target|^2."""
+        u0, D, target = _f(u0), _f(D), _f(target)
+        gD = np.zeros(X * Y)
+        gk = ctypes.c_double(0.0)
+        f = self.fn("adi_cell_gradient")
+        f.restype = ctypes.c_double
+        f.argtypes = [_dp, _dp, _f64, _i32, _i32, _i32, _dp, _dp, ctypes.POINTER(ctypes.c_double)]
+        J = f(_p(u0), _p(D), k, ticks, X, Y, _p(target), _p(gD), ctypes.byref(gk))
+        return J, gD, gk.value
+
+
 class Reference(_Lib):
     """The compiled HAL-CPP reference (oracle/_ref/libhalref.so)."""
 

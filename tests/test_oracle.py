@@ -229,3 +229,48 @@ def test_agent_move_safe_semantics(oracle):
     np.testing.assert_allclose(got, [1.1, 1.1, 9.5, 2.1, 5.0, 0.05])
     got = oracle.agent_move(pos, vel, 0.1, 10, 10, 1, 1)
     np.testing.assert_allclose(got, [1.1, 1.1, 0.5, 2.1, 5.0, 9.95])
+
+
+# ---- per-cell-diffusivity ADI and adjoint (no reference counterpart) ------
+
+def test_cell_adi_uniform_equals_reference_adi(oracle):
+    # Uniform D reduces to the reference's zero-flux ADI tick + MulAll(-k).
+    rng = np.random.default_rng(11)
+    X, Y = 13, 9
+    u0 = rng.random(X * Y)
+    got = oracle.adi_cell_forward(u0, np.full(X * Y, 2.5), 3e-3, 4, X, Y)
+    ref = oracle.adi_ticks(u0, 4, 2.5, 3e-3, X, Y, 0, 0)
+    np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-14)
+
+
+def test_cell_adi_tick_transpose_dot_product(oracle):
+    # <M x, y> = <x, M^T y> for the tick's linear map M at fixed D.
+    rng = np.random.default_rng(12)
+    X, Y = 11, 7
+    D = 0.5 + rng.random(X * Y)
+    x, y = rng.random(X * Y), rng.random(X * Y)
+    Mx = oracle.adi_cell_forward(x, D, 0.02, 1, X, Y)
+    MTy = oracle.adi_cell_tick_transpose(y, D, 0.02, X, Y)
+    assert abs(Mx @ y - x @ MTy) < 1e-12 * abs(Mx @ y)
+
+
+def test_cell_adi_gradient_vs_central_differences(oracle):
+    rng = np.random.default_rng(13)
+    X, Y, ticks, k = 10, 8, 12, 5e-3
+    D = 0.5 + rng.random(X * Y)
+    u0 = rng.random(X * Y)
+    target = oracle.adi_cell_forward(u0, D * (1 + 0.1 * rng.random(X * Y)), k, ticks, X, Y)
+    J, gD, gk = oracle.adi_cell_gradient(u0, D, k, ticks, X, Y, target)
+    h = 1e-6
+    def loss(Dv, kv):
+        uT = oracle.adi_cell_forward(u0, Dv, kv, ticks, X, Y)
+        return 0.5 * np.sum((uT - target) ** 2)
+    assert abs(J - loss(D, k)) < 1e-14
+    for c in rng.choice(X * Y, 12, replace=False):
+        Dp, Dm = D.copy(), D.copy()
+        Dp[c] += h
+        Dm[c] -= h
+        fd = (loss(Dp, k) - loss(Dm, k)) / (2 * h)
+        assert abs(fd - gD[c]) < 1e-6 * max(1.0, abs(fd)), (c, fd, gD[c])
+    fdk = (loss(D, k + h) - loss(D, k - h)) / (2 * h)
+    assert abs(fdk - gk) < 1e-6 * max(1.0, abs(fdk))
