@@ -60,6 +60,61 @@ def traffic_from_profile():
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def bench_batched(ctx, ticks=50):
+    """512 uneven wells (workloads.wells_batch): batched ticks vs the same
+    kernels launched well by well."""
+    import numpy as np
+    import halhip
+    import workloads
+    spec = workloads.wells_batch()
+    b = halhip.Batch(ctx, spec["xdims"], spec["ydims"])
+    b.set_wells(spec["rate"], spec["decay"])
+    b.set_sources(spec["src_well"], spec["src_cell"], spec["src_val"])
+    zeros = np.zeros(b.cells, np.float32)
+    b.upload(zeros)
+    b.run_ticks(5)
+    b.run_ticks(ticks, asynchronous=True)
+    batched_ms = b.sync() / ticks
+    b.upload(zeros)
+    b.run_ticks(2, per_instance=True)
+    loop_ticks = 10
+    b.run_ticks(loop_ticks, per_instance=True, asynchronous=True)
+    loop_ms = b.sync() / loop_ticks
+    out = {"workload": "512 wells, sides log-uniform 64-1024, D~U[0.1,5], decay~U[0,1e-2], 32 sources/well, "
+                       "zero flux", "cells": int(b.cells), "strip_rows": b.strip_rows(),
+           "batched_ms_per_tick": round(batched_ms, 4), "per_instance_ms_per_tick": round(loop_ms, 4),
+           "batched_Gcell_s": round(b.cells / (batched_ms * 1e-3) / 1e9, 2),
+           "per_instance_Gcell_s": round(b.cells / (loop_ms * 1e-3) / 1e9, 2),
+           "speedup": round(loop_ms / batched_ms, 2)}
+    b.close()
+    return out
+
+
+def bench_adjoint(ctx):
+    """1024^2 per-cell-diffusivity ADI (workloads.adjoint_problem): forward
+    200 ticks vs forward + checkpointed adjoint gradient."""
+    import halhip
+    import workloads
+    w = workloads.adjoint_problem()
+    p = halhip.Adjoint2D(ctx, w["X"], w["Y"])
+    p.set_problem(w["D_target"], w["decay"], w["ticks"], w["every"])
+    target = p.forward(w["u0"])
+    p.set_problem(w["D"], w["decay"], w["ticks"], w["every"])
+    p.forward(w["u0"])
+    t0 = time.perf_counter()
+    p.forward(w["u0"])
+    fwd = time.perf_counter() - t0
+    p.gradient(w["u0"], target)
+    t0 = time.perf_counter()
+    p.gradient(w["u0"], target)
+    grad = time.perf_counter() - t0
+    p.close()
+    return {"workload": "1024^2, D = 1 + 0.5 * 8-mode mixture, decay 5e-3, 200 ticks, checkpoint every 20",
+            "forward_ms": round(fwd * 1e3, 3), "gradient_ms": round(grad * 1e3, 3),
+            "gradient_over_forward": round(grad / fwd, 3),
+            "forward_Gcell_s": round(w["X"] * w["Y"] * w["ticks"] / fwd / 1e9, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +122,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-ticks", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--adi-only", action="store_true", help="skip the batched and adjoint comparisons")
+    ap.add_argument("--shape", default=None,
+                    help="strip shape columns_per_thread,strip_rows,vector(0|1) instead of the automatic one")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run the copy kernel (read + write this many MiB, 10 launches) for counter calibration")
     args = ap.parse_args()
@@ -92,6 +150,9 @@ def main():
     cells = n * n
     ctx = halhip.Context(0)
     plan = halhip.Adi2D(ctx, n, n)
+    if args.shape:
+        cpt, rows, vec = (int(v) for v in args.shape.split(","))
+        plan.set_strip_shape(cpt, rows, bool(vec))
     plan.upload(w["field"])
 
     plan.run_steps(args.warmup, w["rate"], 0.0, False, False, bc=w["bc"])
@@ -114,6 +175,11 @@ def main():
     achieved = BYTES_PER_CELL_TICK * cells / (dom_ms * 1e-3) / 1e9
     traffic = traffic_from_profile()
 
+    extras = {}
+    if not args.adi_only:
+        extras["batched"] = bench_batched(ctx)
+        extras["adjoint"] = bench_adjoint(ctx)
+
     base = None
     if not args.no_cpu_baseline:
         base = cpu_baseline(w["field"], n, w["rate"], args.cpu_ticks)
@@ -133,7 +199,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic: 64 Gaussian blobs (sigma 20, amplitude U[0.5,1]) at positions from Hal::Rng(42)",
         "config": {"workload": "ADI 4096x4096, D=5, dt=1, Dirichlet 0, 200 ticks", "grid": [n, n],
-                   "bc": "dirichlet-0", "rate": w["rate"], "parallelism": "single GPU"},
+                   "bc": "dirichlet-0", "rate": w["rate"], "parallelism": "single GPU",
+                   "strip_shape": list(plan.strip_shape())},
         "hbm_frac_wall": round(value * BYTES_PER_CELL_TICK / HBM_PEAK_GBS, 4),
         "event_ms_per_step": round(event_ms / args.steps, 5),
         "kernels_ms_per_launch": {k: round(v[0] / v[1], 5) for k, v in prof.items()},
@@ -142,6 +209,7 @@ def main():
                      "kernel": dom, "algorithmic_bytes_per_launch": BYTES_PER_CELL_TICK * cells},
         "cpu_baseline": base,
     }
+    line.update(extras)
     if base is not None:
         line["speedup_vs_cpu"] = round(value / base["value"], 1)
     print(json.dumps(line), flush=True)
