@@ -141,6 +141,198 @@ namespace
     }
 }
 
+namespace
+{
+    using LdsKernel = void (*)(AdiArgs, float *, int32_t, RowSolve);
+
+    // Windows up to 8 strips; up to 4 with 8 columns per thread, where the
+    // larger window's carries would not fit in registers.
+    template <int CPT, int S, int E>
+    LdsKernel PickLdsWindow(int32_t K)
+    {
+        if (K == 0)
+        {
+            return AdiStripLds<CPT, S, E, 0>;
+        }
+        if (K <= 4)
+        {
+            return AdiStripLds<CPT, S, E, 4>;
+        }
+        if constexpr (CPT < 8)
+        {
+            return AdiStripLds<CPT, S, E, kMaxWindow>;
+        }
+        return nullptr;
+    }
+
+    // Kernel of the LDS row solve for a strip shape and E with windowed
+    // carries of size K (0: carries from the scan kernel), or null.
+    LdsKernel PickLds(int cpt, int s, int e, int32_t window)
+    {
+        switch (cpt * 10000 + s * 100 + e)
+        {
+            case 40832:
+                return PickLdsWindow<4, 8, 32>(window);
+            case 20816:
+                return PickLdsWindow<2, 8, 16>(window);
+            case 10808:
+                return PickLdsWindow<1, 8, 8>(window);
+            case 40416:
+                return PickLdsWindow<4, 4, 16>(window);
+            case 20408:
+                return PickLdsWindow<2, 4, 8>(window);
+            case 80432:
+                return PickLdsWindow<8, 4, 32>(window);
+            default:
+                return nullptr;
+        }
+    }
+
+    // Elements per lane E of the LDS row solve for this shape (0: the shape
+    // does not fit it).  Needs whole 16-byte rows, yDim = threads * CPT with a
+    // whole number W of wavefronts per row, and a compiled (CPT, S, E).
+    int32_t LdsElements(const Shape &sh, int32_t Y)
+    {
+        if (!sh.vec || Y % 64 != 0 || sh.threads * sh.cpt != Y)
+        {
+            return 0;
+        }
+        const int32_t waves = sh.threads / 64;
+        if (waves % sh.s != 0)
+        {
+            return 0;
+        }
+        const int32_t W = waves / sh.s;
+        const int32_t e = Y / (64 * W);
+        if (64 * W * e != Y || PickLds(sh.cpt, sh.s, e, 0) == nullptr)
+        {
+            return 0;
+        }
+        return e;
+    }
+
+    // Smallest K <= kMaxWindow such that every product of K consecutive
+    // strip multipliers, forward (A) and backward (C), that a windowed carry
+    // drops is at most 2^-28; 0 when there is none.  Windows reaching past the
+    // first or last strip drop nothing.
+    int32_t WindowSize(const std::vector<double> &A, const std::vector<double> &C)
+    {
+        const double tau = std::ldexp(1.0, -28);
+        const int32_t P = static_cast<int32_t>(A.size());
+        for (int32_t k = 1; k <= kMaxWindow; k++)
+        {
+            bool ok = true;
+            for (int32_t p = 0; p < P && ok; p++)
+            {
+                if (p - k >= 0)
+                {
+                    double prod = 1.0;
+                    for (int32_t q = p - k; q < p; q++)
+                    {
+                        prod *= std::fabs(A[static_cast<size_t>(q)]);
+                    }
+                    ok = prod <= tau;
+                }
+                if (ok && p + k <= P - 1)
+                {
+                    double prod = 1.0;
+                    for (int32_t q = p + 1; q <= p + k; q++)
+                    {
+                        prod *= std::fabs(C[static_cast<size_t>(q)]);
+                    }
+                    ok = prod <= tau;
+                }
+            }
+            if (ok)
+            {
+                return k;
+            }
+        }
+        return 0;
+    }
+
+    // Constants of the LDS row solve (RowSolve) for lines of n = 64 W E
+    // elements, computed in FP64.  End diagonals follow ADISolveRow
+    // (ADIequations.h:225-270): 1 + r/2 (zero flux) or 1 + 3r/2 (Dirichlet).
+    struct RowTables
+    {
+        RowSolve solve{};
+        std::vector<double> data;
+        size_t oPw = 0, oLane = 0, oChunk = 0, oPowE = 0;
+    };
+
+    RowTables BuildRowTables(int32_t n, int32_t E, int32_t W, double rate, bool dirichlet)
+    {
+        RowTables t;
+        const double hr = rate / 2;
+        const double beta = ((1 + rate) + std::sqrt(1 + 2 * rate)) / 2;
+        const double phi = hr / beta;
+        const double ib = 1 / beta;
+        const double bEnd = dirichlet ? 1 + 1.5 * rate : 1 + 0.5 * rate;
+        const double d0 = bEnd - beta;
+        const double d1 = bEnd - (1 + rate);
+        auto pw = [&](double e) { return std::pow(phi, e); };
+        // G = U^T T^-1 U (symmetric), Kmat = (I + diag(d) G)^-1 diag(d)
+        const double g00 = ib * (1 - pw(2.0 * n)) / (1 - phi * phi);
+        const double g01 = ib * pw(n - 1.0);
+        const double g11 = ib;
+        const double m00 = 1 + d0 * g00, m01 = d0 * g01, m10 = d1 * g01, m11 = 1 + d1 * g11;
+        const double det = m00 * m11 - m01 * m10;
+        const double k00 = m11 * d0 / det, k01 = -m01 * d1 / det;
+        const double k10 = -m10 * d0 / det, k11 = m00 * d1 / det;
+        // (alpha, gamma) = Kmat ib (X_0, X_{n-1}); h = alpha / (1 - phi^2);
+        // t = gamma - h phi^(n+1)
+        const double q00 = k00 * ib / (1 - phi * phi), q01 = k01 * ib / (1 - phi * phi);
+        const double q10 = k10 * ib - q00 * pw(n + 1.0), q11 = k11 * ib - q01 * pw(n + 1.0);
+        RowSolve &r = t.solve;
+        r.phi = static_cast<float>(phi);
+        r.ib = static_cast<float>(ib);
+        r.phiWave = static_cast<float>(pw(64.0 * E));
+        r.q00 = static_cast<float>(q00);
+        r.q01 = static_cast<float>(q01);
+        r.q10 = static_cast<float>(q10);
+        r.q11 = static_cast<float>(q11);
+        r.W = W;
+        t.oPw = t.data.size();
+        for (int k = 0; k < 6; k++)
+        {
+            t.data.push_back(pw(static_cast<double>(E) * (1 << k)));
+        }
+        t.oLane = t.data.size();
+        for (int l = 0; l < 64; l++)
+        {
+            t.data.push_back(pw(static_cast<double>(E) * (l % 16 + 1)));
+        }
+        for (int l = 0; l < 64; l++)
+        {
+            t.data.push_back(pw(static_cast<double>(E) * (l % 32 + 1)));
+        }
+        for (int l = 0; l < 64; l++)
+        {
+            t.data.push_back(pw(static_cast<double>(E) * l));
+        }
+        for (int l = 0; l < 64; l++)
+        {
+            t.data.push_back(pw(static_cast<double>(E) * (63 - l)));
+        }
+        t.oChunk = t.data.size();
+        for (int k = 0; k < 64 * W; k++)
+        {
+            t.data.push_back(pw(static_cast<double>(E) * k));
+        }
+        for (int k = 0; k < 64 * W; k++)
+        {
+            t.data.push_back(pw(static_cast<double>(n) - static_cast<double>(E) * k - E));
+        }
+        t.oPowE = t.data.size();
+        for (int j = 0; j < E; j++)
+        {
+            t.data.push_back(pw(j));
+        }
+        return t;
+    }
+}
+
 struct halhip_adi2d
 {
     halhip_context *ctx = nullptr;
@@ -157,7 +349,15 @@ struct halhip_adi2d
     DeviceArray<float> srcVal;
     int32_t nSources = 0;
     std::string tableKey;
-    bool reducedValid = false; // red[] holds the reduction of the current field for tableKey
+    bool reducedValid = false; // red half redCur holds the reduction of the current field for tableKey
+    int32_t redCur = 0;        // which half of red[] holds the current reductions
+    int32_t wantRow = HALHIP_ADI_AUTO;
+    int32_t wantCarry = HALHIP_ADI_AUTO;
+    int32_t useRow = HALHIP_ADI_ROWS_WORKGROUP_SCAN;
+    int32_t useCarry = HALHIP_ADI_CARRIES_SCAN;
+    int32_t window = 0;        // K of the windowed carries
+    int32_t ldsE = 0;          // elements per lane of the LDS row solve
+    RowSolve rs{};
     AdiArgs args{};
     StreamClock clock;
     Profiler prof;
@@ -183,7 +383,8 @@ namespace
     {
         p->P = (p->X + p->shape.s - 1) / p->shape.s;
         const size_t plane = static_cast<size_t>(p->P) * static_cast<size_t>(p->Y);
-        p->red.Allocate(p->ctx, 2 * plane);
+        p->red.Allocate(p->ctx, 4 * plane); // two halves of (B, D) for windowed ticks
+        p->redCur = 0;
         p->carry.Allocate(p->ctx, 2 * plane);
         p->tableKey.clear();
         p->reducedValid = false;
@@ -201,6 +402,8 @@ namespace
         AppendKey(key, rate);
         AppendKey(key, wrapX);
         AppendKey(key, wrapY);
+        AppendKey(key, p->wantRow);
+        AppendKey(key, p->wantCarry);
         e.AppendTo(key);
         if (key == p->tableKey)
         {
@@ -260,6 +463,21 @@ namespace
         size_t oA = put(sA), oC = put(sC), oE = put(sE);
         size_t oXlo = putf(e.xlo), oXhi = putf(e.xhi), oYlo = putf(e.ylo), oYhi = putf(e.yhi);
 
+        // Row solver and carries for plain ticks.
+        const int32_t ldsE = (wrapY || p->wantRow == HALHIP_ADI_ROWS_WORKGROUP_SCAN) ? 0 : LdsElements(p->shape, p->Y);
+        int32_t K = (ldsE == 0 || wrapX || p->wantCarry == HALHIP_ADI_CARRIES_SCAN) ? 0 : WindowSize(sA, sC);
+        if (K > 0 && PickLds(p->shape.cpt, p->shape.s, ldsE, K) == nullptr)
+        {
+            K = 0;
+        }
+        size_t oRow = 0;
+        RowTables rt;
+        if (ldsE > 0)
+        {
+            rt = BuildRowTables(p->Y, ldsE, p->shape.threads / 64 / p->shape.s, rate, dir);
+            oRow = put(rt.data);
+        }
+
         if (p->tables.size() < host.size())
         {
             p->tables.Allocate(p->ctx, host.size());
@@ -305,6 +523,18 @@ namespace
         a.xibLast = static_cast<float>(fx.ib[static_cast<size_t>(p->X - 1)]);
         a.xz0 = static_cast<float>(fx.z[0]);
         a.xzLast = static_cast<float>(fx.z[static_cast<size_t>(p->X - 1)]);
+        p->ldsE = ldsE;
+        p->window = K;
+        p->useRow = (ldsE > 0) ? HALHIP_ADI_ROWS_LDS : HALHIP_ADI_ROWS_WORKGROUP_SCAN;
+        p->useCarry = (K > 0) ? HALHIP_ADI_CARRIES_WINDOW : HALHIP_ADI_CARRIES_SCAN;
+        if (ldsE > 0)
+        {
+            p->rs = rt.solve;
+            p->rs.pw = t + oRow + rt.oPw;
+            p->rs.lane = t + oRow + rt.oLane;
+            p->rs.chunk = t + oRow + rt.oChunk;
+            p->rs.powE = t + oRow + rt.oPowE;
+        }
         p->tableKey = key;
         p->reducedValid = false;
         p->graphs.Clear();
@@ -317,6 +547,23 @@ namespace
         hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(p->P)), dim3(static_cast<unsigned>(p->shape.threads)), 0,
                            p->clock.stream, p->args, p->field.get());
         Check(hipGetLastError(), "AdiStrip launch");
+        p->prof.End(p->clock.stream);
+    }
+
+    float *RedHalf(halhip_adi2d *p, int32_t half)
+    {
+        return p->red.get() + static_cast<size_t>(half) * 2 * static_cast<size_t>(p->P) * static_cast<size_t>(p->Y);
+    }
+
+    void LaunchLds(halhip_adi2d *p)
+    {
+        const int32_t window = (p->useCarry == HALHIP_ADI_CARRIES_WINDOW) ? p->window : 0;
+        LdsKernel k = PickLds(p->shape.cpt, p->shape.s, p->ldsE, window);
+        Require(k != nullptr, "adi2d: no LDS kernel for the strip shape");
+        p->prof.Begin(p->clock.stream, "adi_strip_tick");
+        hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(p->P)), dim3(static_cast<unsigned>(p->shape.threads)), 0,
+                           p->clock.stream, p->args, p->field.get(), p->window, p->rs);
+        Check(hipGetLastError(), "AdiStripLds launch");
         p->prof.End(p->clock.stream);
     }
 
@@ -354,11 +601,28 @@ namespace
         p->prof.End(p->clock.stream);
     }
 
-    // One plain tick's launches (the reduction of the field is resident).
+    // One plain tick's launches (the reduction of the field is resident in
+    // red half redCur).  Windowed ticks read that half and write the other.
     void PlainTick(halhip_adi2d *p)
     {
+        if (p->useCarry == HALHIP_ADI_CARRIES_WINDOW)
+        {
+            p->args.redIn = RedHalf(p, p->redCur);
+            p->args.red = RedHalf(p, 1 - p->redCur);
+            LaunchLds(p);
+            p->redCur = 1 - p->redCur;
+            return;
+        }
+        p->args.red = RedHalf(p, p->redCur);
         LaunchScan(p);
-        LaunchStrip(p, 1, "adi_strip_tick");
+        if (p->useRow == HALHIP_ADI_ROWS_LDS)
+        {
+            LaunchLds(p);
+        }
+        else
+        {
+            LaunchStrip(p, 1, "adi_strip_tick");
+        }
     }
 
     void EnqueueTicks(halhip_adi2d *p, int32_t steps, double rate, double decay, int32_t wrapX, int32_t wrapY,
@@ -372,18 +636,20 @@ namespace
             return;
         }
         const bool plain = (decay == 0.0) && (!withSources || p->nSources == 0);
+        p->args.red = RedHalf(p, p->redCur);
         if (plain)
         {
             if (!p->reducedValid)
             {
                 LaunchStrip(p, 0, "adi_strip_reduce");
             }
-            constexpr int32_t kChunk = 20;
+            constexpr int32_t kChunk = 20; // even: a windowed chunk leaves redCur as it found it
             int32_t done = 0;
             if (!p->prof.On() && steps >= kChunk)
             {
                 std::string key = p->tableKey;
                 AppendKey(key, kChunk);
+                AppendKey(key, p->redCur);
                 hipGraphExec_t exec = p->graphs.Find(key);
                 if (exec == nullptr)
                 {
@@ -628,6 +894,39 @@ extern "C"
                 p->shape.threads = ((lanes + 63) / 64) * 64;
                 p->shape.vec = vector != 0;
                 AllocateStrips(p);
+            });
+    }
+
+    int halhip_adi2d_set_methods(halhip_adi2d *p, int32_t rowSolver, int32_t carries)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_adi2d_set_methods");
+                Require(rowSolver >= HALHIP_ADI_AUTO && rowSolver <= HALHIP_ADI_ROWS_LDS,
+                        "halhip_adi2d_set_methods: unknown row solver");
+                Require(carries >= HALHIP_ADI_AUTO && carries <= HALHIP_ADI_CARRIES_WINDOW,
+                        "halhip_adi2d_set_methods: unknown carry method");
+                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                p->wantRow = rowSolver;
+                p->wantCarry = carries;
+                p->tableKey.clear();
+                p->reducedValid = false;
+                p->graphs.Clear();
+            });
+    }
+
+    int halhip_adi2d_get_methods(halhip_adi2d *p, int32_t *rowSolver, int32_t *carries, int32_t *window)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_adi2d_get_methods");
+                Require(rowSolver != nullptr && carries != nullptr && window != nullptr,
+                        "halhip_adi2d_get_methods: null output");
+                *rowSolver = p->useRow;
+                *carries = p->useCarry;
+                *window = p->window;
             });
     }
 

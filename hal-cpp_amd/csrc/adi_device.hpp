@@ -42,8 +42,10 @@ namespace
         const float *bcXhi;
         const float *bcYlo;
         const float *bcYhi;
-        // strip reduction and carries, [2][P][Y] each
+        // strip reduction and carries, [2][P][Y] each (red: written by this
+        // launch; redIn: the current tick's reductions, read by windowed ticks)
         float *red;
+        const float *redIn;
         float *carry;
         float *colInfo; // [3][Y]: SM factor, u*(0,y), u*(X-1,y)
         const float *stripA;
@@ -412,18 +414,15 @@ namespace
         StoreRow<CPT, VEC>(a.red + plane + base, y0, a.Y, acc);
     }
 
-    // Finishes sweep A on the strip from the carries (v: RHS in, u* out) and
-    // returns u* of the rows just above (s0-1) and below (s0+S) the strip.
+    // Finishes sweep A on the strip from its carries (v: RHS in, u* out): cf
+    // is d' of row s0-1 (forward carry-in), dn on entry u* of row s0+S
+    // (backward carry-in).  Returns u* of the rows just above (s0-1, in up)
+    // and below (s0+S, in dn) the strip.
     template <int CPT, int S, bool VEC>
-    __device__ __forceinline__ void FinishA(const AdiArgs &a, int p, int s0, int y0, float (&v)[S][CPT],
-                                            float (&up)[CPT], float (&dn)[CPT])
+    __device__ __forceinline__ void FinishACore(const AdiArgs &a, int s0, int y0, float (&v)[S][CPT],
+                                                const float (&cf)[CPT], float (&up)[CPT], float (&dn)[CPT])
     {
-        const size_t base = static_cast<size_t>(p) * a.Y;
-        const size_t plane = static_cast<size_t>(a.P) * a.Y;
-        float cf[CPT];
         float acc[CPT];
-        LoadRow<CPT, VEC>(a.carry + base, y0, a.Y, cf);
-        LoadRow<CPT, VEC>(a.carry + plane + base, y0, a.Y, dn);
 #pragma unroll
         for (int c = 0; c < CPT; c++)
         {
@@ -498,6 +497,19 @@ namespace
                 dn[c] = fmaf(-factor[c], zDn, dn[c]);
             }
         }
+    }
+
+    // FinishACore with the carries written by the cross-strip scan kernel.
+    template <int CPT, int S, bool VEC>
+    __device__ __forceinline__ void FinishA(const AdiArgs &a, int p, int s0, int y0, float (&v)[S][CPT],
+                                            float (&up)[CPT], float (&dn)[CPT])
+    {
+        const size_t base = static_cast<size_t>(p) * a.Y;
+        const size_t plane = static_cast<size_t>(a.P) * a.Y;
+        float cf[CPT];
+        LoadRow<CPT, VEC>(a.carry + base, y0, a.Y, cf);
+        LoadRow<CPT, VEC>(a.carry + plane + base, y0, a.Y, dn);
+        FinishACore<CPT, S, VEC>(a, s0, y0, v, cf, up, dn);
     }
 
     // Sweep B right-hand side in place from u*: v <- v + hr*(dv/dx+ + dv/dx-)
@@ -892,6 +904,491 @@ namespace
     __global__ __launch_bounds__(1024) void AdiStrip(AdiArgs a, float *__restrict__ u)
     {
         StripBody<CPT, S, MODE, VEC>(a, u, blockIdx.x);
+    }
+
+    // ---- windowed carries ------------------------------------------------------------
+    //
+    // Sweep A's lines are diagonally dominant, so the carry a strip receives
+    // from strip q decays like the product of the strip multipliers in
+    // between.  When the host has found K with every product of K consecutive
+    // strip multipliers A (forward) and C (backward) below 2^-28, each strip
+    // composes its carries from the K strips on either side and the
+    // cross-strip scan kernel is not needed:
+    //   F_{q+1} = A_q F_q + B_q            (d' of the row before strip q+1)
+    //   X_q     = D_q + E_q F_q + C_q X_{q+1}   (u* of the first row of strip q)
+    // starting from F = 0 K strips up (exact at the top of the grid) and
+    // X = 0 K strips down (exact at the bottom).
+    constexpr int kMaxWindow = 8;
+
+    template <int CPT, bool VEC, int KW>
+    __device__ __forceinline__ void WindowCarries(const AdiArgs &a, int p, int y0, int K, float (&cf)[CPT],
+                                                  float (&xb)[CPT])
+    {
+        const size_t plane = static_cast<size_t>(a.P) * a.Y;
+        const float *B = a.redIn;
+        const float *D = a.redIn + plane;
+        const int qLo = max(0, p - K);
+        const int nUp = p - qLo;                 // strips above folded into cf
+        const int nDn = min(a.P - 1, p + K) - p; // strips below folded into xb
+        float bUp[KW][CPT];
+        float bDn[KW][CPT];
+        float dDn[KW][CPT];
+#pragma unroll
+        for (int i = 0; i < KW; i++)
+        {
+            if (i < nUp)
+            {
+                LoadRow<CPT, VEC>(B + static_cast<size_t>(qLo + i) * a.Y, y0, a.Y, bUp[i]);
+            }
+            if (i < nDn)
+            {
+                // B of strips p .. p+nDn-1 and D of strips p+1 .. p+nDn
+                LoadRow<CPT, VEC>(B + static_cast<size_t>(p + i) * a.Y, y0, a.Y, bDn[i]);
+                LoadRow<CPT, VEC>(D + static_cast<size_t>(p + 1 + i) * a.Y, y0, a.Y, dDn[i]);
+            }
+        }
+        float F[CPT];
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            F[c] = 0.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < KW; i++)
+        {
+            if (i < nUp)
+            {
+                const float A = a.stripA[qLo + i];
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    F[c] = fmaf(A, F[c], bUp[i][c]);
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            cf[c] = F[c];
+        }
+        // bDn[i] <- F_{p+1+i}
+#pragma unroll
+        for (int i = 0; i < KW; i++)
+        {
+            if (i < nDn)
+            {
+                const float A = a.stripA[p + i];
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    F[c] = fmaf(A, F[c], bDn[i][c]);
+                    bDn[i][c] = F[c];
+                }
+            }
+        }
+        float X[CPT];
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            X[c] = 0.0f;
+        }
+#pragma unroll
+        for (int i = KW - 1; i >= 0; i--)
+        {
+            if (i < nDn)
+            {
+                const int q = p + 1 + i;
+                const float C = a.stripC[q];
+                const float Eq = a.stripE[q];
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    X[c] = fmaf(C, X[c], fmaf(Eq, bDn[i][c], dDn[i][c]));
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            xb[c] = X[c];
+        }
+    }
+
+    // ---- sweep B through LDS with constant coefficients -----------------------------
+    //
+    // A non-periodic sweep-B line is M x = d with off-diagonals -r/2,
+    // interior diagonal 1+r and end diagonals b_0, b_{n-1}.  Let T be M with
+    // its first diagonal replaced by beta = ((1+r) + sqrt(1+2r))/2, the fixed
+    // point of Thomas's b'_i = (1+r) - (r/2)^2 / b'_{i-1}, and its last by
+    // 1+r: T's Thomas factors are constant (b' = beta, multipliers
+    // phi = (r/2)/beta), and M = T + U diag(b_0 - beta, b_{n-1} - 1 - r) U^T
+    // with U = [e_0, e_{n-1}].  By the Woodbury identity
+    //     x = w - Z K U^T w,   w = T^-1 d,   Z = T^-1 U,
+    // where in the scaled variable X = beta * w
+    //     X_i = d'_i + phi X_{i+1},   d'_i = d_i + phi d'_{i-1}
+    // and Z's columns are closed-form geometric sequences, so
+    //     x_i = (X_i - h phi^i - t phi^{n-1-i}) / beta
+    // with (h, t) = Q (X_0, X_{n-1}) for a host-computed 2x2 Q.
+    //
+    // Layout: the strip's S rows sit in LDS; W wavefronts share a row, lane l
+    // of wavefront h owns the E consecutive elements of chunk k = 64h + l,
+    // stored with a one-float pad per chunk (stride E+1) so that lanes reading
+    // element j of their chunks hit distinct banks.
+    struct RowSolve
+    {
+        float phi;        // multiplier of both recurrences
+        float ib;         // 1 / beta
+        float phiWave;    // phi^(64 E): one wavefront's product
+        float q00, q01, q10, q11;
+        int32_t W;        // wavefronts per row
+        const float *pw;  // [6]: phi^(E 2^k), uniform scan stages
+        const float *lane; // [4][64]: phi^(E(l%16+1)), phi^(E(l%32+1)), phi^(E l), phi^(E(63-l))
+        const float *chunk; // [2][64 W]: phi^(E k), phi^(n - E k - E)
+        const float *powE;  // [E]: phi^j
+    };
+
+    template <int E>
+    __device__ __forceinline__ int ChunkAddr(int k, int j)
+    {
+        return k * (E + 1) + j;
+    }
+
+    template <int E>
+    __device__ __forceinline__ int RowAddr(int y)
+    {
+        return ChunkAddr<E>(y / E, y % E);
+    }
+
+    // Sweep A right-hand side of the strip (v: u in, RHS out) with the row
+    // neighbours read from the copy of the same rows staged in LDS.  yDim is
+    // non-periodic here: the ghost value at y = -1 / Y is the cell itself
+    // (zero flux) or 2 bc - c (Dirichlet).  Only the first and last strips
+    // (edge) add the Dirichlet implicit term of rows 0 and X-1 and zero the
+    // rows past X.  Same arithmetic as SweepARhs.
+    template <int CPT, int S, int E, bool EDGE>
+    __device__ __forceinline__ void RhsAFromLds(const AdiArgs &a, const float *rows, int rowStride, int s0, int y0,
+                                                bool first, bool last, float (&v)[S][CPT])
+    {
+        const int yl = (y0 > 0) ? y0 - 1 : 0;
+        const int yr = (y0 + CPT < a.Y) ? y0 + CPT : a.Y - 1;
+        const int al = RowAddr<E>(yl);
+        const int ar = RowAddr<E>(yr);
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            const int x = min(s0 + r, a.X - 1);
+            float L = rows[r * rowStride + al];
+            float R = rows[r * rowStride + ar];
+            const float c0 = v[r][0];
+            const float cl = v[r][CPT - 1];
+            const float gl = a.dirichlet ? fmaf(2.0f, a.bcYlo[x], -c0) : c0;
+            const float gr = a.dirichlet ? fmaf(2.0f, a.bcYhi[x], -cl) : cl;
+            L = first ? gl : L;
+            R = last ? gr : R;
+            float prev = L;
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                const float cur = v[r][c];
+                const float nxt = (c + 1 < CPT) ? v[r][c + 1 < CPT ? c + 1 : c] : R;
+                v[r][c] = fmaf((nxt - cur) + (prev - cur), a.hr, cur);
+                prev = cur;
+            }
+        }
+        if constexpr (EDGE)
+        {
+            const bool dirX = a.dirichlet && !a.wrapX;
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+                const int x = s0 + r;
+                if (x >= a.X)
+                {
+#pragma unroll
+                    for (int c = 0; c < CPT; c++)
+                    {
+                        v[r][c] = 0.0f;
+                    }
+                }
+                else if (dirX && (x == 0 || x == a.X - 1))
+                {
+#pragma unroll
+                    for (int c = 0; c < CPT; c++)
+                    {
+                        float src = 0.0f;
+                        if (x == 0)
+                        {
+                            src += a.bcXlo[y0 + c];
+                        }
+                        if (x == a.X - 1)
+                        {
+                            src += a.bcXhi[y0 + c];
+                        }
+                        v[r][c] = fmaf(src, a.r, v[r][c]);
+                    }
+                }
+            }
+        }
+    }
+
+    // Sweep B right-hand side for a strip away from the x boundaries (rows
+    // s0-1 and s0+S exist and are up / dn).  Same arithmetic as SweepBRhs.
+    template <int CPT, int S>
+    __device__ __forceinline__ void RhsBInterior(const AdiArgs &a, int s0, bool first, bool last, float (&v)[S][CPT],
+                                                 const float (&up)[CPT], const float (&dn)[CPT])
+    {
+        float prevRow[CPT];
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            prevRow[c] = up[c];
+        }
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            const int x = s0 + r;
+            const float srcLo = (a.dirichlet && first) ? a.bcYlo[x] * a.r : 0.0f;
+            const float srcHi = (a.dirichlet && last) ? a.bcYhi[x] * a.r : 0.0f;
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                const float cur = v[r][c];
+                const float nxt = (r + 1 < S) ? v[r + 1 < S ? r + 1 : r][c] : dn[c];
+                float rhs = fmaf((nxt - cur) + (prevRow[c] - cur), a.hr, cur);
+                if (c == 0)
+                {
+                    rhs += srcLo;
+                }
+                if (c == CPT - 1)
+                {
+                    rhs += srcHi;
+                }
+                prevRow[c] = cur;
+                v[r][c] = rhs;
+            }
+        }
+    }
+
+    // KW: 0 takes the carries from the scan kernel, else windowed carries
+    // with K <= KW.  EDGE: the strip holds row 0 or row X-1 (or rows past X).
+    // rows: blockDim.x * (E+1) floats (S rows x 64W chunks x (E+1)).
+    template <int CPT, int S, int E, int KW, bool EDGE>
+    __device__ __forceinline__ void StripLdsBody(const AdiArgs &a, float *__restrict__ u, int p, int K,
+                                                 const RowSolve &rs, float *rows, float *sTotF, float *sTotB)
+    {
+
+        const int s0 = p * S;
+        const int tid = threadIdx.x;
+        const int y0 = tid * CPT;
+        const int lane = tid & 63;
+        const int wave = tid >> 6;
+        const int W = rs.W;
+        const int rowStride = 64 * W * (E + 1);
+        const bool first = tid == 0;
+        const bool last = tid == static_cast<int>(blockDim.x) - 1;
+
+        float cf[CPT];
+        float up[CPT];
+        float dn[CPT];
+        if constexpr (KW > 0)
+        {
+            WindowCarries<CPT, true, KW>(a, p, y0, K, cf, dn);
+        }
+        else
+        {
+            const size_t base = static_cast<size_t>(p) * a.Y;
+            const size_t plane = static_cast<size_t>(a.P) * a.Y;
+            LoadRow<CPT, true>(a.carry + base, y0, a.Y, cf);
+            LoadRow<CPT, true>(a.carry + plane + base, y0, a.Y, dn);
+        }
+        float v[S][CPT];
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            if (s0 + r < a.X)
+            {
+                LoadRow<CPT, true>(u + static_cast<size_t>(s0 + r) * a.Y, y0, a.Y, v[r]);
+            }
+            else
+            {
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    v[r][c] = 0.0f;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                rows[r * rowStride + RowAddr<E>(y0 + c)] = v[r][c];
+            }
+        }
+        __syncthreads();
+        RhsAFromLds<CPT, S, E, EDGE>(a, rows, rowStride, s0, y0, first, last, v);
+        FinishACore<CPT, S, true>(a, s0, y0, v, cf, up, dn);
+        if constexpr (EDGE)
+        {
+            SweepBRhs<CPT, S, true>(a, s0, y0, tid, v, up, dn);
+        }
+        else
+        {
+            RhsBInterior<CPT, S>(a, s0, first, last, v, up, dn);
+        }
+        __syncthreads(); // every read of the staged u is done
+
+        // Right-hand sides to LDS, row-major by chunk.
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                rows[r * rowStride + RowAddr<E>(y0 + c)] = v[r][c];
+            }
+        }
+        __syncthreads();
+
+        // Row phase: wavefront `wave` solves part h of row `row`.
+        const int row = wave / W;
+        const int h = wave - row * W;
+        const int k = h * 64 + lane;
+        const bool active = row < S && s0 + row < a.X;
+        float *line = rows + row * rowStride;
+        const float phi = rs.phi;
+        float d[E];
+        float b = 0.0f;
+        if (active)
+        {
+#pragma unroll
+            for (int j = 0; j < E; j++)
+            {
+                d[j] = line[ChunkAddr<E>(k, j)];
+                b = fmaf(phi, b, d[j]);
+            }
+            // inclusive scan over the wavefront's lanes
+            b = fmaf(rs.pw[0], Dpp<0x111>(0.0f, b), b);
+            b = fmaf(rs.pw[1], Dpp<0x112>(0.0f, b), b);
+            b = fmaf(rs.pw[2], Dpp<0x114>(0.0f, b), b);
+            b = fmaf(rs.pw[3], Dpp<0x118>(0.0f, b), b);
+            b = fmaf(rs.lane[lane], Dpp<0x142, 0xA>(0.0f, b), b);
+            b = fmaf(rs.lane[64 + lane], Dpp<0x143, 0xC>(0.0f, b), b);
+            if (lane == 63)
+            {
+                sTotF[wave] = b;
+            }
+        }
+        __syncthreads();
+        float xLast = 0.0f; // d'_{n-1} = X_{n-1}
+        if (active)
+        {
+            float cw = 0.0f;
+            for (int g = 0; g < W; g++)
+            {
+                const float t = sTotF[row * W + g];
+                cw = (g < h) ? fmaf(rs.phiWave, cw, t) : cw;
+                xLast = fmaf(rs.phiWave, xLast, t);
+            }
+            float acc = fmaf(rs.lane[128 + lane], cw, Dpp<0x138>(0.0f, b));
+#pragma unroll
+            for (int j = 0; j < E; j++)
+            {
+                acc = fmaf(phi, acc, d[j]);
+                d[j] = acc;
+            }
+            b = 0.0f;
+#pragma unroll
+            for (int j = E - 1; j >= 0; j--)
+            {
+                b = fmaf(phi, b, d[j]);
+            }
+            // inclusive suffix scan over the wavefront's lanes
+#pragma unroll
+            for (int st = 0; st < 6; st++)
+            {
+                const int o = 1 << st;
+                const float nb = __shfl_down(b, o);
+                b = (lane + o < 64) ? fmaf(rs.pw[st], nb, b) : b;
+            }
+            if (lane == 0)
+            {
+                sTotB[wave] = b;
+            }
+        }
+        __syncthreads();
+        if (active)
+        {
+            float cw = 0.0f;
+            float x0 = 0.0f;
+            for (int g = W - 1; g >= 0; g--)
+            {
+                const float t = sTotB[row * W + g];
+                cw = (g > h) ? fmaf(rs.phiWave, cw, t) : cw;
+                x0 = fmaf(rs.phiWave, x0, t);
+            }
+            const float hc = fmaf(rs.q00, x0, rs.q01 * xLast);
+            const float tc = fmaf(rs.q10, x0, rs.q11 * xLast);
+            const float hl = hc * rs.chunk[k];
+            const float tl = tc * rs.chunk[64 * W + k];
+            const float nb = __shfl_down(b, 1);
+            float acc = fmaf(rs.lane[192 + lane], cw, (lane == 63) ? 0.0f : nb);
+#pragma unroll
+            for (int j = E - 1; j >= 0; j--)
+            {
+                acc = fmaf(phi, acc, d[j]);
+                const float corr = fmaf(hl, rs.powE[j], tl * rs.powE[E - 1 - j]);
+                line[ChunkAddr<E>(k, j)] = rs.ib * (acc - corr);
+            }
+        }
+        __syncthreads();
+
+        // Column phase: u' from LDS, store, and next tick's strip reduction.
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                v[r][c] = (s0 + r < a.X) ? rows[r * rowStride + RowAddr<E>(y0 + c)] : 0.0f;
+            }
+            if (s0 + r < a.X)
+            {
+                StoreRow<CPT, true>(u + static_cast<size_t>(s0 + r) * a.Y, y0, a.Y, v[r]);
+            }
+        }
+        RhsAFromLds<CPT, S, E, EDGE>(a, rows, rowStride, s0, y0, first, last, v);
+        ReduceA<CPT, S, true>(a, p, s0, y0, v);
+    }
+
+    // Strip index of workgroup b: with windowed carries the strips of one
+    // contiguous range go to one XCD (workgroups are dealt to the 8 XCDs in
+    // turn), so the reductions a strip reads from its neighbours are mostly
+    // in that XCD's L2.
+    __device__ __forceinline__ int StripOfBlock(int b, int P, bool xcdMajor)
+    {
+        if (!xcdMajor)
+        {
+            return b;
+        }
+        const int per = P / 8;
+        return (b % 8) * per + b / 8;
+    }
+
+    template <int CPT, int S, int E, int KW>
+    __global__ __launch_bounds__(1024) void AdiStripLds(AdiArgs a, float *__restrict__ u, int32_t K, RowSolve rs)
+    {
+        static_assert(1024 * (E + 1) * 4 <= 140 * 1024, "LDS row buffer too large");
+        __shared__ float rows[1024 * (E + 1)];
+        __shared__ float sTotF[kMaxWaves];
+        __shared__ float sTotB[kMaxWaves];
+        const int p = StripOfBlock(blockIdx.x, a.P, KW > 0 && (a.P % 8 == 0));
+        if (p == 0 || (p + 1) * S >= a.X)
+        {
+            StripLdsBody<CPT, S, E, KW, true>(a, u, p, K, rs, rows, sTotF, sTotB);
+        }
+        else
+        {
+            StripLdsBody<CPT, S, E, KW, false>(a, u, p, K, rs, rows, sTotF, sTotB);
+        }
     }
 
     // Scan of the strip reductions along each column (sweep A across strips).

@@ -21,6 +21,11 @@ ERR_UNSUPPORTED = 5
 BC_NONE = 0
 BC_DIRICHLET = 1
 ADI_MAX_Y = 8192
+ADI_AUTO = 0
+ADI_ROWS_WORKGROUP_SCAN = 1
+ADI_ROWS_LDS = 2
+ADI_CARRIES_SCAN = 1
+ADI_CARRIES_WINDOW = 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -84,6 +89,8 @@ _SIGS = {
     "halhip_adi2d_sync": [_vp, _fp],
     "halhip_adi2d_set_strip_shape": [_vp, _i32, _i32, _i32],
     "halhip_adi2d_get_strip_shape": [_vp, _ip, _ip, _ip],
+    "halhip_adi2d_set_methods": [_vp, _i32, _i32],
+    "halhip_adi2d_get_methods": [_vp, _ip, _ip, _ip],
     "halhip_adi2d_set_profiling": [_vp, _i32],
     "halhip_adi2d_profile": [_vp, _i32, _ip, ctypes.c_char_p, _i32, _dp, _lp],
     "halhip_batch_create": [_vp, _i32, _ip, _ip, ctypes.POINTER(_vp)],
@@ -336,6 +343,17 @@ class Adi2D(_Plan):
         c, s, v = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         _check(self._fn("get_strip_shape")(self.h, ctypes.byref(c), ctypes.byref(s), ctypes.byref(v)))
         return c.value, s.value, bool(v.value)
+
+    def set_methods(self, rows=ADI_AUTO, carries=ADI_AUTO):
+        """Plain-tick methods (halhip_adi2d_set_methods); a method that does not
+        apply falls back to the other."""
+        _check(self._fn("set_methods")(self.h, rows, carries))
+
+    def methods(self):
+        """(row solver, carries, window K) used by the last plain run."""
+        r, c, k = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(self._fn("get_methods")(self.h, ctypes.byref(r), ctypes.byref(c), ctypes.byref(k)))
+        return r.value, c.value, k.value
 
     def __init__(self, ctx, X, Y):
         self.ctx, self.X, self.Y = ctx, X, Y

@@ -175,6 +175,27 @@ int halhip_adi2d_sync(halhip_adi2d *plan, float *elapsedMs);
 int halhip_adi2d_set_strip_shape(halhip_adi2d *plan, int32_t columnsPerThread, int32_t stripRows, int32_t vector);
 int halhip_adi2d_get_strip_shape(halhip_adi2d *plan, int32_t *columnsPerThread, int32_t *stripRows, int32_t *vector);
 
+/* Methods of the plain ADI tick (no decay, no point sources), for testing
+ * and benchmarking; both are HALHIP_ADI_AUTO by default.
+ *   rowSolver: HALHIP_ADI_ROWS_WORKGROUP_SCAN solves each contiguous line as a
+ *     scan across the workgroup's registers; HALHIP_ADI_ROWS_LDS stages the
+ *     strip in LDS and solves each line with constant Thomas factors plus an
+ *     exact rank-2 end correction (non-periodic yDim, supported shapes only).
+ *   carries: HALHIP_ADI_CARRIES_SCAN runs a cross-strip scan kernel per tick;
+ *     HALHIP_ADI_CARRIES_WINDOW composes each strip's carries from the K
+ *     strips on either side when the line factors decay fast enough that the
+ *     dropped terms are below 2^-28 (non-periodic xDim, LDS rows only).
+ * A method that does not apply falls back to the other one;
+ * halhip_adi2d_get_methods reports what the last plain run used and the
+ * window K (0 without windowed carries).  No counterpart in the reference. */
+#define HALHIP_ADI_AUTO 0
+#define HALHIP_ADI_ROWS_WORKGROUP_SCAN 1
+#define HALHIP_ADI_ROWS_LDS 2
+#define HALHIP_ADI_CARRIES_SCAN 1
+#define HALHIP_ADI_CARRIES_WINDOW 2
+int halhip_adi2d_set_methods(halhip_adi2d *plan, int32_t rowSolver, int32_t carries);
+int halhip_adi2d_get_methods(halhip_adi2d *plan, int32_t *rowSolver, int32_t *carries, int32_t *window);
+
 /* Per-kernel device time of the last run (HIP events around each launch when
  * profiling is on; profiling is off by default and does not change results).
  * names: comma-separated kernel names; ms/count arrays of length *nKernels. */

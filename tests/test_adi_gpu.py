@@ -218,3 +218,70 @@ def test_every_strip_shape_matches_oracle(ctx, oracle, Y):
                         ref[key] = adi_ref_ticks(oracle, f, 3, 5.0, X, Y, wx, wy, bc)
                     err = rel_l2(plan.download(), ref[key])
                     assert err < 1e-5, (cpt, s, vec, key, err)
+
+
+# ---- plain-tick methods: workgroup-scan vs LDS rows, scan vs windowed carries ----
+
+METHOD_CASES = [
+    # X, Y, (cpt, rows), rate, wrapX, bc
+    (300, 1024, (1, 8), 5.0, 0, 0.2),
+    (256, 2048, (2, 8), 5.0, 0, None),
+    (200, 4096, (4, 8), 0.5, 0, 1.0),
+    (200, 4096, (4, 8), 50.0, 0, 0.0),
+    (96, 512, (1, 8), 5.0, 0, None),
+    (128, 4096, (4, 8), 5.0, 1, None),
+    (61, 4096, (4, 4), 2.0, 0, 0.5),
+    (70, 2048, (2, 4), 5.0, 0, None),
+    (40, 8192, (8, 4), 5.0, 0, 0.0),
+    (2, 1024, (1, 8), 5.0, 0, 0.0),
+]
+
+
+@pytest.mark.parametrize("X,Y,shape,rate,wx,bc", METHOD_CASES)
+def test_methods_match_oracle(ctx, oracle, X, Y, shape, rate, wx, bc):
+    f = np.random.default_rng(X + Y).random(X * Y).astype(np.float32)
+    ref = adi_ref_ticks(oracle, f, 6, rate, X, Y, wx, 0, bc)
+    plan = halhip.Adi2D(ctx, X, Y)
+    plan.set_strip_shape(shape[0], shape[1], True)
+    used = set()
+    for rows, carries in ((halhip.ADI_ROWS_WORKGROUP_SCAN, halhip.ADI_CARRIES_SCAN),
+                          (halhip.ADI_ROWS_LDS, halhip.ADI_CARRIES_SCAN),
+                          (halhip.ADI_ROWS_LDS, halhip.ADI_CARRIES_WINDOW)):
+        plan.set_methods(rows, carries)
+        plan.upload(f)
+        plan.run_steps(6, rate, 0.0, wx, 0, bc=bc)
+        err = rel_l2(plan.download(), ref)
+        m = plan.methods()
+        used.add(m[:2])
+        assert err < 1e-5, (rows, carries, m, err)
+        # run 2 more ticks after a pause: the resident reductions carry over
+        plan.run_steps(2, rate, 0.0, wx, 0, bc=bc)
+    # the LDS row solve applies to every case here; windowed carries to the
+    # non-periodic ones whose factors decay fast enough (with 8 columns per
+    # thread only windows up to 4 strips are compiled)
+    assert (halhip.ADI_ROWS_LDS, halhip.ADI_CARRIES_SCAN) in used
+    if not wx and rate <= 5.0 and shape[0] < 8:
+        assert (halhip.ADI_ROWS_LDS, halhip.ADI_CARRIES_WINDOW) in used
+
+
+def test_methods_odd_tick_counts_keep_reductions(ctx, oracle):
+    # Windowed ticks alternate two reduction buffers; runs of odd length and
+    # graph-sized chunks must chain correctly.
+    X, Y = 160, 4096
+    f = np.random.default_rng(5).random(X * Y).astype(np.float32)
+    plan = halhip.Adi2D(ctx, X, Y)
+    plan.upload(f)
+    for n in (1, 3, 20, 21, 2):
+        plan.run_steps(n, 5.0, 0.0, False, False, bc=0.0)
+    assert plan.methods() == (halhip.ADI_ROWS_LDS, halhip.ADI_CARRIES_WINDOW, plan.methods()[2])
+    ref = adi_ref_ticks(oracle, f, 47, 5.0, X, Y, 0, 0, 0.0)
+    assert rel_l2(plan.download(), ref) < 1e-5
+
+
+def test_blobs_4096_uses_lds_rows_and_windowed_carries(ctx):
+    w = workloads.adi_blobs()
+    plan = halhip.Adi2D(ctx, w["X"], w["Y"])
+    plan.upload(w["field"])
+    plan.run_steps(1, w["rate"], 0.0, False, False, bc=w["bc"])
+    rows, carries, k = plan.methods()
+    assert (rows, carries) == (halhip.ADI_ROWS_LDS, halhip.ADI_CARRIES_WINDOW) and 1 <= k <= 8
