@@ -60,6 +60,57 @@ def traffic_from_profile():
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def bench_explicit(ctx, ticks=1000):
+    """Explicit 256^2 (workloads.explicit_256): r = 0.2, decay 1e-3, centre
+    source 1.0, periodic, 1000 ticks resident; 8 B/cell-tick algorithmic."""
+    import halhip
+    import workloads
+    w = workloads.explicit_256()
+    p = halhip.Diffusion2D(ctx, w["X"], w["Y"])
+    p.set_sources(w["src_idx"], w["src_val"])
+    p.upload(w["field"])
+    p.run_steps(100, w["rate"], w["decay"], True, True)
+    p.upload(w["field"])
+    p.run_steps(ticks, w["rate"], w["decay"], True, True, asynchronous=True)
+    ms = p.sync()
+    cells = w["X"] * w["Y"]
+    p.set_profiling(True)
+    p.run_steps(100, w["rate"], w["decay"], True, True)
+    prof = p.profile()
+    p.close()
+    k_us = prof["explicit_tick"][0] / prof["explicit_tick"][1] * 1e3
+    return {"workload": "explicit 256^2, r=0.2, decay 1e-3, centre source 1.0, periodic, 1000 ticks",
+            "ms_total": round(ms, 4), "us_per_tick": round(ms * 1e3 / ticks, 3),
+            "Gcell_s": round(cells * ticks / (ms * 1e-3) / 1e9, 2), "kernel_us": round(k_us, 3),
+            "note": "256 KiB field stays in L2: launch-latency bound (50-tick hipGraph chunks)"}
+
+
+def bench_agents(ctx, ticks=100):
+    """4M agents on 2048^2 (workloads.agents_uniform): bin, scan, stable
+    scatter, forces, move per tick; 52 N + 20 C bytes algorithmic per tick."""
+    import halhip
+    import workloads
+    w = workloads.agents_uniform()
+    n = w["rad"].size
+    cells = w["X"] * w["Y"]
+    p = halhip.Agents2D(ctx, w["X"], w["Y"], n)
+    p.upload(w["pos"], w["rad"])
+    p.run_ticks(10, w["R"], w["k"], w["dt"], False, False)
+    p.upload(w["pos"], w["rad"])
+    p.run_ticks(ticks, w["R"], w["k"], w["dt"], False, False, asynchronous=True)
+    ms = p.sync()
+    p.set_profiling(True)
+    p.run_ticks(10, w["R"], w["k"], w["dt"], False, False)
+    prof = p.profile()
+    p.close()
+    per_tick = ms / ticks
+    alg = 52 * n + 20 * cells
+    return {"workload": "4M agents, 2048^2 bins, r=0.5, R=1, k=1, dt=0.1, no wrap, 100 ticks",
+            "ms_per_tick": round(per_tick, 4), "Magent_ticks_s": round(n / (per_tick * 1e-3) / 1e6, 1),
+            "algorithmic_bytes_per_tick": alg, "hbm_frac": round(alg / (per_tick * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+            "kernels_us_per_launch": {k: round(v[0] / v[1] * 1e3, 2) for k, v in prof.items()}}
+
+
 def bench_batched(ctx, ticks=50):
     """512 uneven wells (workloads.wells_batch): batched ticks vs the same
     kernels launched well by well."""
@@ -177,6 +228,8 @@ def main():
 
     extras = {}
     if not args.adi_only:
+        extras["explicit"] = bench_explicit(ctx)
+        extras["agents"] = bench_agents(ctx)
         extras["batched"] = bench_batched(ctx)
         extras["adjoint"] = bench_adjoint(ctx)
 

@@ -145,44 +145,42 @@ namespace
 {
     using LdsKernel = void (*)(AdiArgs, float *, int32_t, RowSolve);
 
-    // Windows up to 8 strips; up to 4 with 8 columns per thread, where the
-    // larger window's carries would not fit in registers.
-    template <int CPT, int S, int E>
+    template <int CPT, int S, int E, int T>
     LdsKernel PickLdsWindow(int32_t K)
     {
         if (K == 0)
         {
-            return AdiStripLds<CPT, S, E, 0>;
+            return AdiStripLds<CPT, S, E, 0, T>;
         }
-        if (K <= 4)
-        {
-            return AdiStripLds<CPT, S, E, 4>;
-        }
-        if constexpr (CPT < 8)
-        {
-            return AdiStripLds<CPT, S, E, kMaxWindow>;
-        }
-        return nullptr;
+        return (K <= 4) ? AdiStripLds<CPT, S, E, 4, T> : AdiStripLds<CPT, S, E, kMaxWindow, T>;
     }
 
-    // Kernel of the LDS row solve for a strip shape and E with windowed
-    // carries of size K (0: carries from the scan kernel), or null.
-    LdsKernel PickLds(int cpt, int s, int e, int32_t window)
+    template <int CPT, int S, int E>
+    LdsKernel PickLdsThreads(int threads, int32_t K)
+    {
+        return threads == 1024 ? PickLdsWindow<CPT, S, E, 1024>(K)
+                               : (threads == 512 ? PickLdsWindow<CPT, S, E, 512>(K) : nullptr);
+    }
+
+    // Kernel of the LDS row solve for a strip shape, E and workgroup size
+    // (512 or 1024) with windowed carries of size K (0: carries from the scan
+    // kernel), or null when that combination is not compiled.
+    LdsKernel PickLds(int cpt, int s, int e, int32_t window, int threads)
     {
         switch (cpt * 10000 + s * 100 + e)
         {
             case 40832:
-                return PickLdsWindow<4, 8, 32>(window);
-            case 20816:
-                return PickLdsWindow<2, 8, 16>(window);
-            case 10808:
-                return PickLdsWindow<1, 8, 8>(window);
-            case 40416:
-                return PickLdsWindow<4, 4, 16>(window);
-            case 20408:
-                return PickLdsWindow<2, 4, 8>(window);
+                return PickLdsThreads<4, 8, 32>(threads, window);
             case 80432:
-                return PickLdsWindow<8, 4, 32>(window);
+                return PickLdsThreads<8, 4, 32>(threads, window);
+            case 20816:
+                return PickLdsThreads<2, 8, 16>(threads, window);
+            case 40416:
+                return PickLdsThreads<4, 4, 16>(threads, window);
+            case 10808:
+                return PickLdsThreads<1, 8, 8>(threads, window);
+            case 20408:
+                return PickLdsThreads<2, 4, 8>(threads, window);
             default:
                 return nullptr;
         }
@@ -204,7 +202,7 @@ namespace
         }
         const int32_t W = waves / sh.s;
         const int32_t e = Y / (64 * W);
-        if (64 * W * e != Y || PickLds(sh.cpt, sh.s, e, 0) == nullptr)
+        if (64 * W * e != Y || PickLds(sh.cpt, sh.s, e, 0, sh.threads) == nullptr)
         {
             return 0;
         }
@@ -314,6 +312,10 @@ namespace
         for (int l = 0; l < 64; l++)
         {
             t.data.push_back(pw(static_cast<double>(E) * (63 - l)));
+        }
+        for (int l = 0; l < 64; l++)
+        {
+            t.data.push_back(pw(static_cast<double>(E) * (16 - l % 16)));
         }
         t.oChunk = t.data.size();
         for (int k = 0; k < 64 * W; k++)
@@ -458,15 +460,48 @@ namespace
             sC[q] = C;
             sE[q] = acc;
         }
+        // Reduction weights: B = sum_j wB_j rhs_j with wB_j = prod_{m>j} fw_m;
+        // D = sum_j wD_j rhs_j with wD_j = sum_{i>=j} (prod_{m<i} bw_m) ib_i
+        // (prod_{j<m<=i} fw_m), indices within the strip.
+        std::vector<double> redW(static_cast<size_t>(p->P) * 2 * S);
+        for (int32_t q = 0; q < p->P; q++)
+        {
+            const size_t o = static_cast<size_t>(q) * S;
+            for (int32_t j = 0; j < S; j++)
+            {
+                double wb = 1.0;
+                for (int32_t m = j + 1; m < S; m++)
+                {
+                    wb *= fx.fw[o + m];
+                }
+                double wd = 0.0;
+                double bwProd = 1.0; // prod_{m<i} bw_m
+                for (int32_t i = 0; i < S; i++)
+                {
+                    if (i >= j)
+                    {
+                        double fwProd = 1.0;
+                        for (int32_t m = j + 1; m <= i; m++)
+                        {
+                            fwProd *= fx.fw[o + m];
+                        }
+                        wd += bwProd * fx.ib[o + i] * fwProd;
+                    }
+                    bwProd *= fx.bw[o + i];
+                }
+                redW[2 * o + j] = wb;
+                redW[2 * o + S + j] = wd;
+            }
+        }
         size_t oXfw = put(fx.fw), oXib = put(fx.ib), oXbw = put(fx.bw), oXz = put(fx.z);
         size_t oYfw = put(fy.fw), oYib = put(fy.ib), oYbw = put(fy.bw), oYz = put(fy.z);
-        size_t oA = put(sA), oC = put(sC), oE = put(sE);
+        size_t oA = put(sA), oC = put(sC), oE = put(sE), oW = put(redW);
         size_t oXlo = putf(e.xlo), oXhi = putf(e.xhi), oYlo = putf(e.ylo), oYhi = putf(e.yhi);
 
         // Row solver and carries for plain ticks.
         const int32_t ldsE = (wrapY || p->wantRow == HALHIP_ADI_ROWS_WORKGROUP_SCAN) ? 0 : LdsElements(p->shape, p->Y);
         int32_t K = (ldsE == 0 || wrapX || p->wantCarry == HALHIP_ADI_CARRIES_SCAN) ? 0 : WindowSize(sA, sC);
-        if (K > 0 && PickLds(p->shape.cpt, p->shape.s, ldsE, K) == nullptr)
+        if (K > 0 && PickLds(p->shape.cpt, p->shape.s, ldsE, K, p->shape.threads) == nullptr)
         {
             K = 0;
         }
@@ -518,6 +553,7 @@ namespace
         a.stripA = t + oA;
         a.stripC = t + oC;
         a.stripE = t + oE;
+        a.redW = t + oW;
         a.xvEnd = static_cast<float>(fx.vEnd);
         a.xInv = static_cast<float>(fx.inv);
         a.xibLast = static_cast<float>(fx.ib[static_cast<size_t>(p->X - 1)]);
@@ -558,7 +594,7 @@ namespace
     void LaunchLds(halhip_adi2d *p)
     {
         const int32_t window = (p->useCarry == HALHIP_ADI_CARRIES_WINDOW) ? p->window : 0;
-        LdsKernel k = PickLds(p->shape.cpt, p->shape.s, p->ldsE, window);
+        LdsKernel k = PickLds(p->shape.cpt, p->shape.s, p->ldsE, window, p->shape.threads);
         Require(k != nullptr, "adi2d: no LDS kernel for the strip shape");
         p->prof.Begin(p->clock.stream, "adi_strip_tick");
         hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(p->P)), dim3(static_cast<unsigned>(p->shape.threads)), 0,

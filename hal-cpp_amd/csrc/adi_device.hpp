@@ -51,6 +51,7 @@ namespace
         const float *stripA;
         const float *stripC;
         const float *stripE;
+        const float *redW; // [P][2][S]: per-row weights of the strip reductions B and D
         float xvEnd;
         float xInv;
         float xibLast;
@@ -66,8 +67,9 @@ namespace
 
     // ---- wavefront primitives ------------------------------------------------------
 
-    // DPP lane moves (GFX9 encodings): row_shr:n = 0x110+n, wave_shr:1 = 0x138,
-    // row_bcast:15 = 0x142, row_bcast:31 = 0x143.  Lanes without a source (or
+    // DPP lane moves (GFX9 encodings): row_shl:n = 0x100+n, row_shr:n =
+    // 0x110+n, wave_shl:1 = 0x130, wave_shr:1 = 0x138, row_bcast:15 = 0x142,
+    // row_bcast:31 = 0x143.  Lanes without a source (or
     // in rows outside rowMask) receive `old`.
     template <int CTRL, int ROW_MASK = 0xF>
     __device__ __forceinline__ float Dpp(float old, float v)
@@ -930,39 +932,26 @@ namespace
         const int qLo = max(0, p - K);
         const int nUp = p - qLo;                 // strips above folded into cf
         const int nDn = min(a.P - 1, p + K) - p; // strips below folded into xb
-        float bUp[KW][CPT];
-        float bDn[KW][CPT];
-        float dDn[KW][CPT];
-#pragma unroll
-        for (int i = 0; i < KW; i++)
-        {
-            if (i < nUp)
-            {
-                LoadRow<CPT, VEC>(B + static_cast<size_t>(qLo + i) * a.Y, y0, a.Y, bUp[i]);
-            }
-            if (i < nDn)
-            {
-                // B of strips p .. p+nDn-1 and D of strips p+1 .. p+nDn
-                LoadRow<CPT, VEC>(B + static_cast<size_t>(p + i) * a.Y, y0, a.Y, bDn[i]);
-                LoadRow<CPT, VEC>(D + static_cast<size_t>(p + 1 + i) * a.Y, y0, a.Y, dDn[i]);
-            }
-        }
         float F[CPT];
+        float X[CPT];
 #pragma unroll
         for (int c = 0; c < CPT; c++)
         {
             F[c] = 0.0f;
+            X[c] = 0.0f;
         }
 #pragma unroll
         for (int i = 0; i < KW; i++)
         {
             if (i < nUp)
             {
+                float bq[CPT];
+                LoadRow<CPT, VEC>(B + static_cast<size_t>(qLo + i) * a.Y, y0, a.Y, bq);
                 const float A = a.stripA[qLo + i];
 #pragma unroll
                 for (int c = 0; c < CPT; c++)
                 {
-                    F[c] = fmaf(A, F[c], bUp[i][c]);
+                    F[c] = fmaf(A, F[c], bq[c]);
                 }
             }
         }
@@ -971,40 +960,28 @@ namespace
         {
             cf[c] = F[c];
         }
-        // bDn[i] <- F_{p+1+i}
+        // X_{p+1} = sum_q (C_{p+1} ... C_{q-1}) (D_q + E_q F_q), q = p+1 .. p+nDn,
+        // with F_q = A_{q-1} F_{q-1} + B_{q-1}.
+        float coef = 1.0f;
 #pragma unroll
         for (int i = 0; i < KW; i++)
         {
             if (i < nDn)
             {
-                const float A = a.stripA[p + i];
-#pragma unroll
-                for (int c = 0; c < CPT; c++)
-                {
-                    F[c] = fmaf(A, F[c], bDn[i][c]);
-                    bDn[i][c] = F[c];
-                }
-            }
-        }
-        float X[CPT];
-#pragma unroll
-        for (int c = 0; c < CPT; c++)
-        {
-            X[c] = 0.0f;
-        }
-#pragma unroll
-        for (int i = KW - 1; i >= 0; i--)
-        {
-            if (i < nDn)
-            {
                 const int q = p + 1 + i;
-                const float C = a.stripC[q];
+                float bq[CPT];
+                float dq[CPT];
+                LoadRow<CPT, VEC>(B + static_cast<size_t>(q - 1) * a.Y, y0, a.Y, bq);
+                LoadRow<CPT, VEC>(D + static_cast<size_t>(q) * a.Y, y0, a.Y, dq);
+                const float A = a.stripA[q - 1];
                 const float Eq = a.stripE[q];
 #pragma unroll
                 for (int c = 0; c < CPT; c++)
                 {
-                    X[c] = fmaf(C, X[c], fmaf(Eq, bDn[i][c], dDn[i][c]));
+                    F[c] = fmaf(A, F[c], bq[c]);
+                    X[c] = fmaf(coef, fmaf(Eq, F[c], dq[c]), X[c]);
                 }
+                coef *= a.stripC[q];
             }
         }
 #pragma unroll
@@ -1042,7 +1019,8 @@ namespace
         float q00, q01, q10, q11;
         int32_t W;        // wavefronts per row
         const float *pw;  // [6]: phi^(E 2^k), uniform scan stages
-        const float *lane; // [4][64]: phi^(E(l%16+1)), phi^(E(l%32+1)), phi^(E l), phi^(E(63-l))
+        const float *lane; // [5][64]: phi^(E(l%16+1)), phi^(E(l%32+1)), phi^(E l), phi^(E(63-l)),
+                           //        phi^(E(16-l%16))
         const float *chunk; // [2][64 W]: phi^(E k), phi^(n - E k - E)
         const float *powE;  // [E]: phi^j
     };
@@ -1085,14 +1063,16 @@ namespace
             const float gr = a.dirichlet ? fmaf(2.0f, a.bcYhi[x], -cl) : cl;
             L = first ? gl : L;
             R = last ? gr : R;
-            float prev = L;
+            // (nxt - cur) + (prev - cur) = e_c - e_{c-1} with e_c = nxt - cur
+            float eLeft = v[r][0] - L;
 #pragma unroll
             for (int c = 0; c < CPT; c++)
             {
                 const float cur = v[r][c];
                 const float nxt = (c + 1 < CPT) ? v[r][c + 1 < CPT ? c + 1 : c] : R;
-                v[r][c] = fmaf((nxt - cur) + (prev - cur), a.hr, cur);
-                prev = cur;
+                const float e = nxt - cur;
+                v[r][c] = fmaf(e - eLeft, a.hr, cur);
+                eLeft = e;
             }
         }
         if constexpr (EDGE)
@@ -1137,11 +1117,12 @@ namespace
     __device__ __forceinline__ void RhsBInterior(const AdiArgs &a, int s0, bool first, bool last, float (&v)[S][CPT],
                                                  const float (&up)[CPT], const float (&dn)[CPT])
     {
-        float prevRow[CPT];
+        // (nxt - cur) + (prev - cur) = e_r - e_{r-1} with e_r = nxt - cur
+        float eUp[CPT];
 #pragma unroll
         for (int c = 0; c < CPT; c++)
         {
-            prevRow[c] = up[c];
+            eUp[c] = v[0][c] - up[c];
         }
 #pragma unroll
         for (int r = 0; r < S; r++)
@@ -1154,7 +1135,8 @@ namespace
             {
                 const float cur = v[r][c];
                 const float nxt = (r + 1 < S) ? v[r + 1 < S ? r + 1 : r][c] : dn[c];
-                float rhs = fmaf((nxt - cur) + (prevRow[c] - cur), a.hr, cur);
+                const float e = nxt - cur;
+                float rhs = fmaf(e - eUp[c], a.hr, cur);
                 if (c == 0)
                 {
                     rhs += srcLo;
@@ -1163,10 +1145,44 @@ namespace
                 {
                     rhs += srcHi;
                 }
-                prevRow[c] = cur;
+                eUp[c] = e;
                 v[r][c] = rhs;
             }
         }
+    }
+
+    // The strip's two reduction values as dot products of the sweep-A
+    // right-hand side with per-row weights (host FP64): B = sum_r wB_r rhs_r
+    // (forward end value, zero carry-in) and D = sum_r wD_r rhs_r (backward
+    // start value, zero carries), the same linear maps as ReduceA.
+    template <int CPT, int S>
+    __device__ __forceinline__ void ReduceADot(const AdiArgs &a, int p, int y0, const float (&v)[S][CPT])
+    {
+        float bAcc[CPT];
+        float dAcc[CPT];
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            bAcc[c] = 0.0f;
+            dAcc[c] = 0.0f;
+        }
+        const float *wB = a.redW + static_cast<size_t>(p) * 2 * S;
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            const float b = wB[r];
+            const float d = wB[S + r];
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                bAcc[c] = fmaf(b, v[r][c], bAcc[c]);
+                dAcc[c] = fmaf(d, v[r][c], dAcc[c]);
+            }
+        }
+        const size_t base = static_cast<size_t>(p) * a.Y;
+        const size_t plane = static_cast<size_t>(a.P) * a.Y;
+        StoreRow<CPT, true>(a.red + base, y0, a.Y, bAcc);
+        StoreRow<CPT, true>(a.red + plane + base, y0, a.Y, dAcc);
     }
 
     // KW: 0 takes the carries from the scan kernel, else windowed carries
@@ -1187,20 +1203,8 @@ namespace
         const bool first = tid == 0;
         const bool last = tid == static_cast<int>(blockDim.x) - 1;
 
-        float cf[CPT];
-        float up[CPT];
-        float dn[CPT];
-        if constexpr (KW > 0)
-        {
-            WindowCarries<CPT, true, KW>(a, p, y0, K, cf, dn);
-        }
-        else
-        {
-            const size_t base = static_cast<size_t>(p) * a.Y;
-            const size_t plane = static_cast<size_t>(a.P) * a.Y;
-            LoadRow<CPT, true>(a.carry + base, y0, a.Y, cf);
-            LoadRow<CPT, true>(a.carry + plane + base, y0, a.Y, dn);
-        }
+        // The strip's rows are requested first so that their latency overlaps
+        // the carries' loads and arithmetic.
         float v[S][CPT];
 #pragma unroll
         for (int r = 0; r < S; r++)
@@ -1217,6 +1221,24 @@ namespace
                     v[r][c] = 0.0f;
                 }
             }
+        }
+        float cf[CPT];
+        float up[CPT];
+        float dn[CPT];
+        if constexpr (KW > 0)
+        {
+            WindowCarries<CPT, true, KW>(a, p, y0, K, cf, dn);
+        }
+        else
+        {
+            const size_t base = static_cast<size_t>(p) * a.Y;
+            const size_t plane = static_cast<size_t>(a.P) * a.Y;
+            LoadRow<CPT, true>(a.carry + base, y0, a.Y, cf);
+            LoadRow<CPT, true>(a.carry + plane + base, y0, a.Y, dn);
+        }
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
 #pragma unroll
             for (int c = 0; c < CPT; c++)
             {
@@ -1301,13 +1323,23 @@ namespace
             {
                 b = fmaf(phi, b, d[j]);
             }
-            // inclusive suffix scan over the wavefront's lanes
-#pragma unroll
-            for (int st = 0; st < 6; st++)
+            // inclusive suffix scan over the wavefront's lanes: within each
+            // row of 16 lanes by DPP row_shl, then the row totals (lanes 16,
+            // 32, 48) folded in from the rows below
+            b = fmaf(rs.pw[0], Dpp<0x101>(0.0f, b), b);
+            b = fmaf(rs.pw[1], Dpp<0x102>(0.0f, b), b);
+            b = fmaf(rs.pw[2], Dpp<0x104>(0.0f, b), b);
+            b = fmaf(rs.pw[3], Dpp<0x108>(0.0f, b), b);
             {
-                const int o = 1 << st;
-                const float nb = __shfl_down(b, o);
-                b = (lane + o < 64) ? fmaf(rs.pw[st], nb, b) : b;
+                const float t1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, b), 16));
+                const float t2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, b), 32));
+                const float t3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, b), 48));
+                const float s2 = t3;
+                const float s1 = fmaf(rs.pw[4], s2, t2);
+                const float s0 = fmaf(rs.pw[4], s1, t1);
+                const int rw = lane >> 4;
+                const float below = (rw == 0) ? s0 : ((rw == 1) ? s1 : ((rw == 2) ? s2 : 0.0f));
+                b = fmaf(rs.lane[256 + lane], below, b);
             }
             if (lane == 0)
             {
@@ -1327,16 +1359,14 @@ namespace
             }
             const float hc = fmaf(rs.q00, x0, rs.q01 * xLast);
             const float tc = fmaf(rs.q10, x0, rs.q11 * xLast);
-            const float hl = hc * rs.chunk[k];
-            const float tl = tc * rs.chunk[64 * W + k];
-            const float nb = __shfl_down(b, 1);
-            float acc = fmaf(rs.lane[192 + lane], cw, (lane == 63) ? 0.0f : nb);
+            const float hl = -rs.ib * hc * rs.chunk[k];
+            const float tl = -rs.ib * tc * rs.chunk[64 * W + k];
+            float acc = fmaf(rs.lane[192 + lane], cw, Dpp<0x130>(0.0f, b));
 #pragma unroll
             for (int j = E - 1; j >= 0; j--)
             {
                 acc = fmaf(phi, acc, d[j]);
-                const float corr = fmaf(hl, rs.powE[j], tl * rs.powE[E - 1 - j]);
-                line[ChunkAddr<E>(k, j)] = rs.ib * (acc - corr);
+                line[ChunkAddr<E>(k, j)] = fmaf(hl, rs.powE[j], fmaf(tl, rs.powE[E - 1 - j], rs.ib * acc));
             }
         }
         __syncthreads();
@@ -1356,7 +1386,7 @@ namespace
             }
         }
         RhsAFromLds<CPT, S, E, EDGE>(a, rows, rowStride, s0, y0, first, last, v);
-        ReduceA<CPT, S, true>(a, p, s0, y0, v);
+        ReduceADot<CPT, S>(a, p, y0, v);
     }
 
     // Strip index of workgroup b: with windowed carries the strips of one
@@ -1373,11 +1403,13 @@ namespace
         return (b % 8) * per + b / 8;
     }
 
-    template <int CPT, int S, int E, int KW>
-    __global__ __launch_bounds__(1024) void AdiStripLds(AdiArgs a, float *__restrict__ u, int32_t K, RowSolve rs)
+    // T: threads per workgroup (= yDim / CPT); the LDS row buffer holds
+    // T * (E+1) floats.
+    template <int CPT, int S, int E, int KW, int T>
+    __global__ __launch_bounds__(T) void AdiStripLds(AdiArgs a, float *__restrict__ u, int32_t K, RowSolve rs)
     {
-        static_assert(1024 * (E + 1) * 4 <= 140 * 1024, "LDS row buffer too large");
-        __shared__ float rows[1024 * (E + 1)];
+        static_assert(T * (E + 1) * 4 <= 140 * 1024, "LDS row buffer too large");
+        __shared__ float rows[T * (E + 1)];
         __shared__ float sTotF[kMaxWaves];
         __shared__ float sTotB[kMaxWaves];
         const int p = StripOfBlock(blockIdx.x, a.P, KW > 0 && (a.P % 8 == 0));

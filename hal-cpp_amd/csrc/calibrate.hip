@@ -37,6 +37,10 @@ extern "C" int halhip_calibration_copy(halhip_context *ctx, int64_t bytes, int32
             Check(hipEventCreate(&e0), "hipEventCreate");
             Check(hipEventCreate(&e1), "hipEventCreate");
             const unsigned blocks = 256 * 8 * 4;
+            // one untimed launch first: the first touch of fresh allocations
+            // is slower than steady state
+            hipLaunchKernelGGL(CopyKernel, dim3(blocks), dim3(256), 0, nullptr, a.get(), b.get(), n);
+            Check(hipGetLastError(), "CopyKernel launch");
             Check(hipEventRecord(e0, nullptr), "hipEventRecord");
             for (int32_t r = 0; r < repeats; r++)
             {

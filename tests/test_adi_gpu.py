@@ -233,6 +233,8 @@ METHOD_CASES = [
     (61, 4096, (4, 4), 2.0, 0, 0.5),
     (70, 2048, (2, 4), 5.0, 0, None),
     (40, 8192, (8, 4), 5.0, 0, 0.0),
+    (100, 4096, (8, 4), 5.0, 0, 0.0),
+    (90, 2048, (4, 8), 1.0, 0, None),
     (2, 1024, (1, 8), 5.0, 0, 0.0),
 ]
 
@@ -257,10 +259,9 @@ def test_methods_match_oracle(ctx, oracle, X, Y, shape, rate, wx, bc):
         # run 2 more ticks after a pause: the resident reductions carry over
         plan.run_steps(2, rate, 0.0, wx, 0, bc=bc)
     # the LDS row solve applies to every case here; windowed carries to the
-    # non-periodic ones whose factors decay fast enough (with 8 columns per
-    # thread only windows up to 4 strips are compiled)
+    # non-periodic ones whose factors decay fast enough
     assert (halhip.ADI_ROWS_LDS, halhip.ADI_CARRIES_SCAN) in used
-    if not wx and rate <= 5.0 and shape[0] < 8:
+    if not wx and rate <= 5.0:
         assert (halhip.ADI_ROWS_LDS, halhip.ADI_CARRIES_WINDOW) in used
 
 
