@@ -9,22 +9,28 @@
 // (:111-175).  The move applies AgentPT2D::MoveSafePT
 // (include/Hal/Agents/AgentPT2D.h:157-186) to p + dt*v.
 //
-// A tick is ten launches, all captured in one hipGraph for multi-tick runs:
-//   fill     counts <- 0
-//   count    cell_i = clamped floor cell; ticket_i = atomicAdd(count[cell_i], 1)
+// Agents stay resident in slot order: sorted by (cell, agent index) as of
+// the previous binning, with id[k] the agent index held in slot k (after an
+// upload, slot k holds agent k).  Agents move little per tick, so slot order
+// stays close to spatial order and every pass below touches memory almost in
+// order.  A tick is seven launches, captured in one hipGraph for multi-tick
+// runs:
+//   count    per slot: cell = clamped floor cell; ticket = atomicAdd(count[cell], 1)
 //   scan     offsets = exclusive prefix of counts (three passes: per-block
 //            sums, scan of the block sums, block-local scan + block prefix;
-//            wavefront shuffles + LDS inside a block)
-//   scatter  bucket[offsets[cell_i] + ticket_i] = i   (a permutation)
-//   order    each cell's slice sorted by agent index, which makes the bucket
-//            identical to the reference's stable scatter whatever order the
-//            atomics handed out tickets in
-//   gather   bucket-ordered copies of positions and radii, so the force pass
-//            reads its neighbours from a few contiguous slices
-//   force    one lane per bucket slot; neighbour cells from the offsets
-//   move     p += dt*v with the MoveSafePT rules
-// The bins (offsets, bucket) are therefore bit-identical to the reference's
-// for the same FP32 positions; forces are FP32 sums in the reference's order.
+//            wavefront shuffles + LDS inside a block); the last pass also
+//            resets the counts to 0 for the next binning
+//   scatter  (id, position, radius) of each slot to offsets[cell] + ticket
+//            in the other buffer set (a permutation)
+//   order    each cell's slice sorted by agent index, which makes the ids in
+//            slot order identical to the reference's bucket (stable scatter)
+//            whatever order the atomics handed out tickets in
+//   force    one lane per slot; neighbour cells from the offsets, their
+//            agents read from contiguous slices
+//   move     p += dt*v with the MoveSafePT rules, in slot order
+// The bins (offsets, bucket = ids in slot order) are therefore bit-identical
+// to the reference's for the same FP32 positions; forces are FP32 sums in
+// the reference's order.  Downloads by agent index scatter through id.
 #include "common.hpp"
 
 #include <algorithm>
@@ -36,24 +42,14 @@ using namespace halhip;
 namespace
 {
     constexpr int kScanThreads = 256;
-    constexpr int kScanItems = 16;                              // per thread
-    constexpr int kScanTile = kScanThreads * kScanItems;        // per block
+    constexpr int kScanRounds = 4;                              // int4 loads per thread
+    constexpr int kScanTile = kScanThreads * 4 * kScanRounds;   // counts per block
     constexpr int kThreads = 256;
 
     __device__ __forceinline__ int ClampCell(float v, int dim)
     {
         int c = static_cast<int>(floorf(v));
         return c < 0 ? 0 : (c >= dim ? dim - 1 : c);
-    }
-
-    __global__ __launch_bounds__(kThreads) void FillZero(int32_t *__restrict__ p, int64_t n)
-    {
-        int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-        int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-        for (; i < n; i += stride)
-        {
-            p[i] = 0;
-        }
     }
 
     __global__ __launch_bounds__(kThreads) void AgentCount(int32_t N, int32_t X, int32_t Y,
@@ -110,17 +106,22 @@ namespace
         return before + inc - v;
     }
 
-    // Pass 1: per-tile totals of counts[0..C).
-    __global__ __launch_bounds__(kScanThreads) void ScanTileSums(const int32_t *__restrict__ counts, int64_t C,
+    // The counts and offsets arrays are padded to whole tiles (padding counts
+    // stay 0), so every access below is a whole, aligned int4.  In round r
+    // thread t covers counts [4(256 r + t), 4(256 r + t) + 4) of its tile.
+
+    // Pass 1: per-tile totals of the counts.
+    __global__ __launch_bounds__(kScanThreads) void ScanTileSums(const int4 *__restrict__ counts,
                                                                  int32_t *__restrict__ tileSums)
     {
         __shared__ int sWave[kScanThreads / 64];
-        const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+        const int4 *tile = counts + static_cast<int64_t>(blockIdx.x) * (kScanTile / 4);
         int s = 0;
 #pragma unroll
-        for (int k = 0; k < kScanItems; k++)
+        for (int r = 0; r < kScanRounds; r++)
         {
-            s += (base + k < C) ? counts[base + k] : 0;
+            const int4 v = tile[r * kScanThreads + threadIdx.x];
+            s += (v.x + v.y) + (v.z + v.w);
         }
         int total;
         BlockExclusiveSum(s, sWave, total);
@@ -149,55 +150,59 @@ namespace
         }
     }
 
-    // Pass 3: offsets[c] = tile prefix + in-tile exclusive prefix; offsets[C] = N.
-    __global__ __launch_bounds__(kScanThreads) void ScanApply(const int32_t *__restrict__ counts, int64_t C,
+    // Pass 3: offsets = tile prefix + in-tile exclusive prefix (offsets past
+    // the last real cell read N), and the counts reset to 0 for the next
+    // binning.
+    __global__ __launch_bounds__(kScanThreads) void ScanApply(int4 *__restrict__ counts,
                                                               const int32_t *__restrict__ tileOffsets,
-                                                              int32_t *__restrict__ offsets, int32_t N)
+                                                              int4 *__restrict__ offsets, int32_t *__restrict__ offsetsTail,
+                                                              int32_t N)
     {
         __shared__ int sWave[kScanThreads / 64];
-        const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
-        int v[kScanItems];
-        int s = 0;
+        const int64_t tileBase = static_cast<int64_t>(blockIdx.x) * (kScanTile / 4);
+        int run = tileOffsets[blockIdx.x];
 #pragma unroll
-        for (int k = 0; k < kScanItems; k++)
+        for (int r = 0; r < kScanRounds; r++)
         {
-            v[k] = (base + k < C) ? counts[base + k] : 0;
-            s += v[k];
-        }
-        int total;
-        int run = tileOffsets[blockIdx.x] + BlockExclusiveSum(s, sWave, total);
-#pragma unroll
-        for (int k = 0; k < kScanItems; k++)
-        {
-            if (base + k < C)
-            {
-                offsets[base + k] = run;
-            }
-            run += v[k];
+            const int64_t i = tileBase + r * kScanThreads + threadIdx.x;
+            const int4 v = counts[i];
+            int total;
+            const int o = run + BlockExclusiveSum((v.x + v.y) + (v.z + v.w), sWave, total);
+            offsets[i] = make_int4(o, o + v.x, o + v.x + v.y, o + v.x + v.y + v.z);
+            counts[i] = make_int4(0, 0, 0, 0);
+            run += total;
         }
         if (blockIdx.x == 0 && threadIdx.x == 0)
         {
-            offsets[C] = N;
+            *offsetsTail = N;
         }
     }
 
+    // Moves slot k's (id, position, radius) to slot offsets[cell] + ticket.
     __global__ __launch_bounds__(kThreads) void AgentScatter(int32_t N, const int32_t *__restrict__ cellOf,
                                                              const int32_t *__restrict__ ticketOf,
                                                              const int32_t *__restrict__ offsets,
-                                                             int32_t *__restrict__ bucket)
+                                                             const int32_t *__restrict__ id,
+                                                             const float2 *__restrict__ pos,
+                                                             const float *__restrict__ rad, int32_t *__restrict__ nId,
+                                                             float2 *__restrict__ nPos, float *__restrict__ nRad)
     {
-        const int i = blockIdx.x * blockDim.x + threadIdx.x;
-        if (i >= N)
+        const int k = blockIdx.x * blockDim.x + threadIdx.x;
+        if (k >= N)
         {
             return;
         }
-        bucket[offsets[cellOf[i]] + ticketOf[i]] = i;
+        const int dst = offsets[cellOf[k]] + ticketOf[k];
+        nId[dst] = id[k];
+        nPos[dst] = pos[k];
+        nRad[dst] = rad[k];
     }
 
-    // Sorts every cell's slice of the bucket by agent index (insertion sort:
-    // slices hold a handful of agents).
+    // Sorts every cell's slice of slots by agent index, moving position and
+    // radius along (insertion sort: slices hold a handful of agents).
     __global__ __launch_bounds__(kThreads) void AgentOrderCells(int64_t C, const int32_t *__restrict__ offsets,
-                                                                int32_t *__restrict__ bucket)
+                                                                int32_t *__restrict__ id, float2 *__restrict__ pos,
+                                                                float *__restrict__ rad)
     {
         const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
         if (c >= C)
@@ -208,30 +213,47 @@ namespace
         const int e = offsets[c + 1];
         for (int a = s + 1; a < e; a++)
         {
-            const int key = bucket[a];
-            int b = a - 1;
-            while (b >= s && bucket[b] > key)
+            const int key = id[a];
+            if (id[a - 1] <= key)
             {
-                bucket[b + 1] = bucket[b];
+                continue;
+            }
+            const float2 kp = pos[a];
+            const float kr = rad[a];
+            int b = a - 1;
+            while (b >= s && id[b] > key)
+            {
+                id[b + 1] = id[b];
+                pos[b + 1] = pos[b];
+                rad[b + 1] = rad[b];
                 b--;
             }
-            bucket[b + 1] = key;
+            id[b + 1] = key;
+            pos[b + 1] = kp;
+            rad[b + 1] = kr;
         }
     }
 
-    __global__ __launch_bounds__(kThreads) void AgentGather(int32_t N, const int32_t *__restrict__ bucket,
-                                                            const float2 *__restrict__ pos,
-                                                            const float *__restrict__ rad, float2 *__restrict__ spos,
-                                                            float *__restrict__ srad)
+    // out[id[k]] = v[k]: slot order to agent-index order.
+    __global__ __launch_bounds__(kThreads) void AgentUnscatter(int32_t N, const int32_t *__restrict__ id,
+                                                               const float2 *__restrict__ v, float2 *__restrict__ out)
     {
         const int k = blockIdx.x * blockDim.x + threadIdx.x;
         if (k >= N)
         {
             return;
         }
-        const int j = bucket[k];
-        spos[k] = pos[j];
-        srad[k] = rad[j];
+        out[id[k]] = v[k];
+    }
+
+    // Slot k holds agent k (after an upload).
+    __global__ __launch_bounds__(kThreads) void AgentIota(int32_t N, int32_t *__restrict__ id)
+    {
+        const int k = blockIdx.x * blockDim.x + threadIdx.x;
+        if (k < N)
+        {
+            id[k] = k;
+        }
     }
 
     // Minimum-image displacement b - a on one axis (CpuAgentForceBackend2D.cpp:30-45).
@@ -251,11 +273,42 @@ namespace
         return m < 0 ? m + dim : m;
     }
 
-    // One lane per bucket slot k; writes the velocity delta of agent bucket[k].
+    // Interaction of the agent in slot k (position p, radius ri) with the
+    // agents of slots [s, e), accumulated into (vx, vy) in slot order.
+    __device__ __forceinline__ void AccumulateRange(int s, int e, int k, float2 p, float ri, float R, float kSpring,
+                                                    float fX, float fY, bool wrapX, bool wrapY,
+                                                    const float2 *__restrict__ spos, const float *__restrict__ srad,
+                                                    float &vx, float &vy)
+    {
+        for (int q = s; q < e; q++)
+        {
+            if (q == k)
+            {
+                continue;
+            }
+            const float2 o = spos[q];
+            const float dx = Disp(p.x, o.x, fX, wrapX);
+            const float dy = Disp(p.y, o.y, fY, wrapY);
+            // hardware square root and reciprocal (within 1 ulp): the FP32
+            // sums stay within 1e-5 of the reference's FP64 ones
+            const float dist = __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
+            if (dist < R && dist > 0.0f)
+            {
+                const float f = kSpring * ((ri + srad[q]) - dist) * __builtin_amdgcn_rcpf(dist);
+                vx -= dx * f;
+                vy -= dy * f;
+            }
+        }
+    }
+
+    // One lane per slot k; writes the velocity delta of slot k's agent to
+    // vel[k] (slot order).  Slots are sorted by cell x*Y + y, so the cells
+    // (cx, ya .. yb-1) of one neighbour row are the contiguous slot range
+    // [offsets[cx*Y+ya], offsets[cx*Y+yb]); visiting the rows x-major and the
+    // wrapped y segments in increasing ys keeps the reference's order.
+    template <bool WX, bool WY>
     __global__ __launch_bounds__(kThreads) void AgentForces(int32_t N, int32_t X, int32_t Y, float R, float kSpring,
-                                                            int32_t wrapX, int32_t wrapY,
                                                             const int32_t *__restrict__ offsets,
-                                                            const int32_t *__restrict__ bucket,
                                                             const float2 *__restrict__ spos,
                                                             const float *__restrict__ srad, float2 *__restrict__ vel)
     {
@@ -264,7 +317,6 @@ namespace
         {
             return;
         }
-        const int i = bucket[k];
         const float2 p = spos[k];
         const float ri = srad[k];
         const float fX = static_cast<float>(X);
@@ -280,46 +332,34 @@ namespace
             int cx = xs;
             if (cx < 0 || cx >= X)
             {
-                if (!wrapX)
+                if (!WX)
                 {
                     continue;
                 }
                 cx = WrapIndex(cx, X);
             }
-            for (int ys = y0; ys < y1; ys++)
+            const int64_t rowBase = static_cast<int64_t>(cx) * Y;
+            if (!WY)
             {
-                int cy = ys;
-                if (cy < 0 || cy >= Y)
+                const int ya = max(y0, 0);
+                const int yb = min(y1, Y);
+                if (ya < yb)
                 {
-                    if (!wrapY)
-                    {
-                        continue;
-                    }
-                    cy = WrapIndex(cy, Y);
+                    AccumulateRange(offsets[rowBase + ya], offsets[rowBase + yb], k, p, ri, R, kSpring, fX, fY, WX,
+                                    WY, spos, srad, vx, vy);
                 }
-                const int64_t cell = static_cast<int64_t>(cx) * Y + cy;
-                const int s = offsets[cell];
-                const int e = offsets[cell + 1];
-                for (int q = s; q < e; q++)
-                {
-                    if (q == k)
-                    {
-                        continue;
-                    }
-                    const float2 o = spos[q];
-                    const float dx = Disp(p.x, o.x, fX, wrapX != 0);
-                    const float dy = Disp(p.y, o.y, fY, wrapY != 0);
-                    const float dist = sqrtf(dx * dx + dy * dy);
-                    if (dist < R && dist > 0.0f)
-                    {
-                        const float force = kSpring * ((ri + srad[q]) - dist);
-                        vx -= (dx / dist) * force;
-                        vy -= (dy / dist) * force;
-                    }
-                }
+                continue;
+            }
+            for (int ys = y0; ys < y1;)
+            {
+                const int cy = WrapIndex(ys, Y);
+                const int len = min(y1 - ys, Y - cy);
+                AccumulateRange(offsets[rowBase + cy], offsets[rowBase + cy + len], k, p, ri, R, kSpring, fX, fY, WX,
+                                WY, spos, srad, vx, vy);
+                ys += len;
             }
         }
-        vel[i] = make_float2(vx, vy);
+        vel[k] = make_float2(vx, vy);
     }
 
     // p += dt*v with MoveSafePT: off-grid target -> wrap a wrapped axis, keep
@@ -381,17 +421,19 @@ struct halhip_agents2d
     int32_t capacity = 0;
     int32_t N = 0;
     int32_t nTiles = 0;
-    DeviceArray<float2> pos;
-    DeviceArray<float> rad;
-    DeviceArray<float2> vel;
-    DeviceArray<float2> spos;
-    DeviceArray<float> srad;
+    // Two buffer sets of (id, position, radius) in slot order; `cur` holds
+    // the resident agents, the other receives the next binning's scatter.
+    DeviceArray<int32_t> id[2];
+    DeviceArray<float2> pos[2];
+    DeviceArray<float> rad[2];
+    int32_t cur = 0;
+    DeviceArray<float2> vel;    // slot order of set `cur`
+    DeviceArray<float2> outBuf; // agent-index order, for downloads
     DeviceArray<int32_t> counts;
     DeviceArray<int32_t> offsets;
     DeviceArray<int32_t> tileSums;
     DeviceArray<int32_t> cellOf;
     DeviceArray<int32_t> ticketOf;
-    DeviceArray<int32_t> bucket;
     bool binsValid = false;
     StreamClock clock;
     Profiler prof;
@@ -416,34 +458,38 @@ namespace
         p->prof.End(p->clock.stream);
     }
 
-    // Bins of the current positions: counts, offsets, stable bucket, and the
-    // bucket-ordered position / radius copies.
+    // Bins of the resident positions: counts, offsets, and the agents moved
+    // to the other buffer set in bucket order, which becomes the resident set.
     void EnqueueBins(halhip_agents2d *p)
     {
         const int32_t N = p->N;
-        Launch(p, "agents_fill", FillZero, Blocks(p->C, kThreads * 8), kThreads, p->counts.get(), p->C);
+        const int c = p->cur;
+        const int n = 1 - c;
         if (N > 0)
         {
-            Launch(p, "agents_count", AgentCount, Blocks(N, kThreads), kThreads, N, p->X, p->Y, p->pos.get(),
-                   p->counts.get(), p->cellOf.get(), p->ticketOf.get());
+            Launch(p, "agents_count", AgentCount, Blocks(N, kThreads), kThreads, N, p->X, p->Y,
+                   static_cast<const float2 *>(p->pos[c].get()), p->counts.get(), p->cellOf.get(), p->ticketOf.get());
         }
+        int4 *counts4 = reinterpret_cast<int4 *>(p->counts.get());
+        const int64_t cPad = static_cast<int64_t>(p->nTiles) * kScanTile;
         Launch(p, "agents_scan_tiles", ScanTileSums, static_cast<unsigned>(p->nTiles), kScanThreads,
-               static_cast<const int32_t *>(p->counts.get()), p->C, p->tileSums.get());
+               static_cast<const int4 *>(counts4), p->tileSums.get());
         Launch(p, "agents_scan_tile_offsets", ScanTileOffsets, 1u, 1024u, p->tileSums.get(), p->nTiles);
-        Launch(p, "agents_scan_apply", ScanApply, static_cast<unsigned>(p->nTiles), kScanThreads,
-               static_cast<const int32_t *>(p->counts.get()), p->C, static_cast<const int32_t *>(p->tileSums.get()),
-               p->offsets.get(), N);
+        Launch(p, "agents_scan_apply", ScanApply, static_cast<unsigned>(p->nTiles), kScanThreads, counts4,
+               static_cast<const int32_t *>(p->tileSums.get()), reinterpret_cast<int4 *>(p->offsets.get()),
+               p->offsets.get() + cPad, N);
         if (N > 0)
         {
             Launch(p, "agents_scatter", AgentScatter, Blocks(N, kThreads), kThreads, N,
                    static_cast<const int32_t *>(p->cellOf.get()), static_cast<const int32_t *>(p->ticketOf.get()),
-                   static_cast<const int32_t *>(p->offsets.get()), p->bucket.get());
+                   static_cast<const int32_t *>(p->offsets.get()), static_cast<const int32_t *>(p->id[c].get()),
+                   static_cast<const float2 *>(p->pos[c].get()), static_cast<const float *>(p->rad[c].get()),
+                   p->id[n].get(), p->pos[n].get(), p->rad[n].get());
             Launch(p, "agents_order", AgentOrderCells, Blocks(p->C, kThreads), kThreads, p->C,
-                   static_cast<const int32_t *>(p->offsets.get()), p->bucket.get());
-            Launch(p, "agents_gather", AgentGather, Blocks(N, kThreads), kThreads, N,
-                   static_cast<const int32_t *>(p->bucket.get()), static_cast<const float2 *>(p->pos.get()),
-                   static_cast<const float *>(p->rad.get()), p->spos.get(), p->srad.get());
+                   static_cast<const int32_t *>(p->offsets.get()), p->id[n].get(), p->pos[n].get(),
+                   p->rad[n].get());
         }
+        p->cur = n;
     }
 
     void EnqueueForces(halhip_agents2d *p, float R, float k, int32_t wrapX, int32_t wrapY)
@@ -452,10 +498,12 @@ namespace
         {
             return;
         }
-        Launch(p, "agents_force", AgentForces, Blocks(p->N, kThreads), kThreads, p->N, p->X, p->Y, R, k,
-               wrapX ? 1 : 0, wrapY ? 1 : 0, static_cast<const int32_t *>(p->offsets.get()),
-               static_cast<const int32_t *>(p->bucket.get()), static_cast<const float2 *>(p->spos.get()),
-               static_cast<const float *>(p->srad.get()), p->vel.get());
+        const int c = p->cur;
+        auto kernel = wrapX ? (wrapY ? AgentForces<true, true> : AgentForces<true, false>)
+                            : (wrapY ? AgentForces<false, true> : AgentForces<false, false>);
+        Launch(p, "agents_force", kernel, Blocks(p->N, kThreads), kThreads, p->N, p->X, p->Y, R, k,
+               static_cast<const int32_t *>(p->offsets.get()), static_cast<const float2 *>(p->pos[c].get()),
+               static_cast<const float *>(p->rad[c].get()), p->vel.get());
     }
 
     void EnqueueTick(halhip_agents2d *p, float R, float k, float dt, int32_t wrapX, int32_t wrapY)
@@ -465,8 +513,44 @@ namespace
         if (p->N > 0)
         {
             Launch(p, "agents_move", AgentMove, Blocks(p->N, kThreads), kThreads, p->N, p->X, p->Y, dt,
-                   wrapX ? 1 : 0, wrapY ? 1 : 0, p->pos.get(), static_cast<const float2 *>(p->vel.get()));
+                   wrapX ? 1 : 0, wrapY ? 1 : 0, p->pos[p->cur].get(), static_cast<const float2 *>(p->vel.get()));
         }
+    }
+
+    // Copies the resident agents in and resets slot k to agent k.
+    void UploadAgents(halhip_agents2d *p, const float *positions, const float *radii, int32_t nAgents)
+    {
+        if (nAgents != p->N)
+        {
+            p->graphs.Clear();
+        }
+        p->N = nAgents;
+        p->binsValid = false;
+        if (nAgents == 0)
+        {
+            return;
+        }
+        const int c = p->cur;
+        Check(hipMemcpyAsync(p->pos[c].get(), positions, sizeof(float2) * nAgents, hipMemcpyHostToDevice,
+                             p->clock.stream),
+              "hipMemcpyAsync");
+        Check(hipMemcpyAsync(p->rad[c].get(), radii, sizeof(float) * nAgents, hipMemcpyHostToDevice, p->clock.stream),
+              "hipMemcpyAsync");
+        hipLaunchKernelGGL(AgentIota, dim3(Blocks(nAgents, kThreads)), dim3(kThreads), 0, p->clock.stream, nAgents,
+                           p->id[c].get());
+        Check(hipGetLastError(), "AgentIota launch");
+    }
+
+    // Slot-order float2 values of the resident set to host memory in
+    // agent-index order.
+    void DownloadByIndex(halhip_agents2d *p, const float2 *slotValues, float *out)
+    {
+        hipLaunchKernelGGL(AgentUnscatter, dim3(Blocks(p->N, kThreads)), dim3(kThreads), 0, p->clock.stream, p->N,
+                           static_cast<const int32_t *>(p->id[p->cur].get()), slotValues, p->outBuf.get());
+        Check(hipGetLastError(), "AgentUnscatter launch");
+        Check(hipMemcpyAsync(out, p->outBuf.get(), sizeof(float2) * p->N, hipMemcpyDeviceToHost, p->clock.stream),
+              "hipMemcpyAsync");
+        Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
     }
 
     void CheckParams(const halhip_agents2d *p, double R, double k, double dt)
@@ -491,12 +575,13 @@ namespace
             p->binsValid = true;
             return;
         }
-        constexpr int32_t kChunk = 10;
+        constexpr int32_t kChunk = 10; // even: a chunk leaves `cur` as it found it
         int32_t done = 0;
         if (!p->prof.On() && ticks >= kChunk)
         {
             std::string key;
             AppendKey(key, p->N);
+            AppendKey(key, p->cur);
             AppendKey(key, fR);
             AppendKey(key, fk);
             AppendKey(key, fdt);
@@ -555,18 +640,25 @@ extern "C"
                 p->capacity = maxAgents;
                 p->nTiles = static_cast<int32_t>((p->C + kScanTile - 1) / kScanTile);
                 const size_t cap = static_cast<size_t>(std::max(1, maxAgents));
-                p->pos.Allocate(ctx, cap);
-                p->rad.Allocate(ctx, cap);
+                for (int b = 0; b < 2; b++)
+                {
+                    p->id[b].Allocate(ctx, cap);
+                    p->pos[b].Allocate(ctx, cap);
+                    p->rad[b].Allocate(ctx, cap);
+                }
                 p->vel.Allocate(ctx, cap);
-                p->spos.Allocate(ctx, cap);
-                p->srad.Allocate(ctx, cap);
+                p->outBuf.Allocate(ctx, cap);
                 p->cellOf.Allocate(ctx, cap);
                 p->ticketOf.Allocate(ctx, cap);
-                p->bucket.Allocate(ctx, cap);
-                p->counts.Allocate(ctx, static_cast<size_t>(p->C));
-                p->offsets.Allocate(ctx, static_cast<size_t>(p->C) + 1);
+                // counts and offsets padded to whole scan tiles; counts start
+                // at 0 and the scan resets them after every binning
+                const size_t cPad = static_cast<size_t>(p->nTiles) * kScanTile;
+                p->counts.Allocate(ctx, cPad);
+                p->offsets.Allocate(ctx, cPad + 4);
                 p->tileSums.Allocate(ctx, static_cast<size_t>(p->nTiles));
                 p->clock.Create();
+                Check(hipMemsetAsync(p->counts.get(), 0, cPad * sizeof(int32_t), p->clock.stream), "hipMemsetAsync");
+                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
                 *out = p;
             });
         if (rc != HALHIP_OK && p != nullptr)
@@ -603,22 +695,7 @@ extern "C"
                         "halhip_agents2d_upload: agent count outside [0, construction-time capacity]");
                 Require(nAgents == 0 || (positions != nullptr && radii != nullptr),
                         "halhip_agents2d_upload: null array");
-                if (nAgents != p->N)
-                {
-                    p->graphs.Clear();
-                }
-                p->N = nAgents;
-                p->binsValid = false;
-                if (nAgents == 0)
-                {
-                    return;
-                }
-                Check(hipMemcpyAsync(p->pos.get(), positions, sizeof(float2) * nAgents, hipMemcpyHostToDevice,
-                                     p->clock.stream),
-                      "hipMemcpyAsync");
-                Check(hipMemcpyAsync(p->rad.get(), radii, sizeof(float) * nAgents, hipMemcpyHostToDevice,
-                                     p->clock.stream),
-                      "hipMemcpyAsync");
+                UploadAgents(p, positions, radii, nAgents);
                 Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
             });
     }
@@ -635,10 +712,7 @@ extern "C"
                 {
                     return;
                 }
-                Check(hipMemcpyAsync(positions, p->pos.get(), sizeof(float2) * nAgents, hipMemcpyDeviceToHost,
-                                     p->clock.stream),
-                      "hipMemcpyAsync");
-                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                DownloadByIndex(p, p->pos[p->cur].get(), positions);
             });
     }
 
@@ -655,10 +729,7 @@ extern "C"
                 {
                     return;
                 }
-                Check(hipMemcpyAsync(velocityDeltas, p->vel.get(), sizeof(float2) * nAgents, hipMemcpyDeviceToHost,
-                                     p->clock.stream),
-                      "hipMemcpyAsync");
-                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                DownloadByIndex(p, p->vel.get(), velocityDeltas);
             });
     }
 
@@ -675,7 +746,7 @@ extern "C"
                       "hipMemcpyAsync");
                 if (p->N > 0)
                 {
-                    Check(hipMemcpyAsync(bucket, p->bucket.get(), sizeof(int32_t) * p->N, hipMemcpyDeviceToHost,
+                    Check(hipMemcpyAsync(bucket, p->id[p->cur].get(), sizeof(int32_t) * p->N, hipMemcpyDeviceToHost,
                                          p->clock.stream),
                           "hipMemcpyAsync");
                 }
@@ -706,28 +777,13 @@ extern "C"
                 {
                     p->hostBuf[2 * n + i] = static_cast<float>(radii[i]);
                 }
-                if (nAgents != p->N)
-                {
-                    p->graphs.Clear();
-                }
-                p->N = nAgents;
-                if (n > 0)
-                {
-                    Check(hipMemcpyAsync(p->pos.get(), p->hostBuf.data(), sizeof(float2) * n, hipMemcpyHostToDevice,
-                                         p->clock.stream),
-                          "hipMemcpyAsync");
-                    Check(hipMemcpyAsync(p->rad.get(), p->hostBuf.data() + 2 * n, sizeof(float) * n,
-                                         hipMemcpyHostToDevice, p->clock.stream),
-                          "hipMemcpyAsync");
-                }
+                UploadAgents(p, p->hostBuf.data(), p->hostBuf.data() + 2 * n, nAgents);
                 EnqueueBins(p);
                 EnqueueForces(p, static_cast<float>(interactionRad), static_cast<float>(springConstant), wrapX, wrapY);
                 p->binsValid = true;
                 if (n > 0)
                 {
-                    Check(hipMemcpyAsync(p->hostBuf.data(), p->vel.get(), sizeof(float2) * n, hipMemcpyDeviceToHost,
-                                         p->clock.stream),
-                          "hipMemcpyAsync");
+                    DownloadByIndex(p, p->vel.get(), p->hostBuf.data());
                 }
                 Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
                 p->prof.Collect();
