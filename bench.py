@@ -176,6 +176,8 @@ def main():
     ap.add_argument("--adi-only", action="store_true", help="skip the batched and adjoint comparisons")
     ap.add_argument("--shape", default=None,
                     help="strip shape columns_per_thread,strip_rows,vector(0|1) instead of the automatic one")
+    ap.add_argument("--companion", choices=["explicit", "agents", "batched", "adjoint"], default=None,
+                    help="run only this companion workload and print its JSON (for profiling)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run the copy kernel (read + write this many MiB, 10 launches) for counter calibration")
     args = ap.parse_args()
@@ -194,6 +196,14 @@ def main():
         ms = ctx.calibration_copy(nbytes, 10)
         print(json.dumps({"calibration_copy_bytes": nbytes, "ms_per_copy": ms,
                           "GBps_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        return
+
+    if args.companion:
+        ctx = halhip.Context(0)
+        run = {"explicit": bench_explicit, "agents": bench_agents, "batched": bench_batched,
+               "adjoint": bench_adjoint}[args.companion]
+        print(json.dumps({args.companion: run(ctx)}), flush=True)
+        ctx.close()
         return
 
     w = workloads.adi_blobs()
