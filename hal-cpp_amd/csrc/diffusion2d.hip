@@ -131,6 +131,184 @@ namespace
         out[i] = c + delta;
     }
 
+    // ---- several ticks per launch (temporal blocking) --------------------------
+    //
+    // Workgroup g owns rows [x0, x0+B), x0 = g*B, and holds rows
+    // [x0-T, x0+B+T) of the field in LDS (two copies, ping-pong); rows wrap
+    // on a periodic x axis and do not exist past a non-wrapped edge.  Tick t
+    // updates local rows [t, B+2T-t): every row it reads was updated at tick
+    // t-1 or is past the grid edge, where the boundary rule replaces it.
+    // After T ticks rows [T, T+B) are exact and are stored.  The arithmetic
+    // is ExplicitStep's, term for term, so results are bit-identical to T
+    // single-tick launches.
+    constexpr int kMultiThreads = 1024;
+    constexpr int kMultiRowsPerThread = 8; // rows of the held B + 2T one thread covers
+
+    __device__ __forceinline__ float EdgeDiffX(const ExplicitArgs &a, bool low, int y, float c)
+    {
+        if (a.dirichlet)
+        {
+            const float bc = low ? a.bcXlo[y] : a.bcXhi[y];
+            return (bc - c) * 2.0f;
+        }
+        return 0.0f;
+    }
+
+    // Threads: thread `tid` works on column y = tid % Y of rows
+    // tid / Y, tid / Y + R, ... (R = 1024 / Y rows per pass; Y <= 1024), so
+    // its y neighbours and y-edge rules are fixed for the whole launch.
+    template <bool WX, bool WY, bool DIR>
+    __global__ __launch_bounds__(kMultiThreads) void ExplicitMulti(ExplicitArgs a, int32_t B, int32_t T,
+                                                                   const float *__restrict__ in,
+                                                                   float *__restrict__ out)
+    {
+        extern __shared__ float lds[];
+        __shared__ int sRow[64]; // global row of each local row, -1 past a non-wrapped edge
+        const int rowsHeld = B + 2 * T;
+        float *buf[2] = {lds, lds + static_cast<size_t>(rowsHeld) * a.Y};
+        const int x0 = blockIdx.x * B;
+        if (threadIdx.x < rowsHeld)
+        {
+            int gx = x0 - T + static_cast<int>(threadIdx.x);
+            if (WX)
+            {
+                gx %= a.X;
+                gx = gx < 0 ? gx + a.X : gx;
+            }
+            else if (gx < 0 || gx >= a.X)
+            {
+                gx = -1;
+            }
+            sRow[threadIdx.x] = gx;
+        }
+        __syncthreads();
+        const int rowsPerPass = kMultiThreads / a.Y;
+        const int y = static_cast<int>(threadIdx.x) % a.Y;
+        const int r0 = static_cast<int>(threadIdx.x) / a.Y;
+        const bool lane = r0 < rowsPerPass;
+        // y neighbours: in-row index, or an edge rule (zero flux: difference
+        // 0; Dirichlet: (bc - c) * 2 with bc at (x, -1) / (x, Y))
+        const bool hiIn = WY || y + 1 < a.Y;
+        const bool loIn = WY || y > 0;
+        const int yHi = (y + 1 < a.Y) ? y + 1 : 0;
+        const int yLo = (y > 0) ? y - 1 : a.Y - 1;
+        const float bcXlo = DIR && !WX ? a.bcXlo[y] : 0.0f;
+        const float bcXhi = DIR && !WX ? a.bcXhi[y] : 0.0f;
+        // Source term of each of this thread's rows (list order), gathered
+        // once per launch: row j of the thread is local row r0 + j*R.
+        float srcAdd[kMultiRowsPerThread];
+        if (lane)
+        {
+#pragma unroll
+            for (int j = 0; j < kMultiRowsPerThread; j++)
+            {
+                const int l = r0 + j * rowsPerPass;
+                srcAdd[j] = 0.0f;
+                const int gx = (l < rowsHeld) ? sRow[l] : -1;
+                if (gx >= 0)
+                {
+                    const size_t i = static_cast<size_t>(gx) * a.Y + y;
+                    if (a.srcDense != nullptr)
+                    {
+                        srcAdd[j] = a.srcDense[i];
+                    }
+                    else
+                    {
+                        for (int k = 0; k < a.nSources; k++)
+                        {
+                            if (a.srcIdx[k] == static_cast<int64_t>(i))
+                            {
+                                srcAdd[j] += a.srcVal[k];
+                            }
+                        }
+                    }
+                }
+            }
+            for (int l = r0; l < rowsHeld; l += rowsPerPass)
+            {
+                const int gx = sRow[l];
+                buf[0][l * a.Y + y] = (gx >= 0) ? in[static_cast<size_t>(gx) * a.Y + y] : 0.0f;
+            }
+        }
+        __syncthreads();
+        int cur = 0;
+        for (int t = 1; t <= T; t++)
+        {
+            const float *src = buf[cur];
+            float *dst = buf[1 - cur];
+            if (lane)
+            {
+#pragma unroll
+                for (int j = 0; j < kMultiRowsPerThread; j++)
+                {
+                    const int l = r0 + j * rowsPerPass;
+                    if (l < t || l >= rowsHeld - t)
+                    {
+                        continue;
+                    }
+                    const int gx = sRow[l];
+                    if (gx < 0)
+                    {
+                        continue;
+                    }
+                    const float *row = src + l * a.Y;
+                    const float c = row[y];
+                    // x+1, x-1, y+1, y-1 in ExplicitStep's order
+                    float s;
+                    if (WX || gx + 1 < a.X)
+                    {
+                        s = row[a.Y + y] - c;
+                    }
+                    else
+                    {
+                        s = DIR ? (bcXhi - c) * 2.0f : 0.0f;
+                    }
+                    if (WX || gx > 0)
+                    {
+                        s += row[y - a.Y] - c;
+                    }
+                    else
+                    {
+                        s += DIR ? (bcXlo - c) * 2.0f : 0.0f;
+                    }
+                    if (hiIn)
+                    {
+                        s += row[yHi] - c;
+                    }
+                    else
+                    {
+                        s += DIR ? (a.bcYhi[gx] - c) * 2.0f : 0.0f;
+                    }
+                    if (loIn)
+                    {
+                        s += row[yLo] - c;
+                    }
+                    else
+                    {
+                        s += DIR ? (a.bcYlo[gx] - c) * 2.0f : 0.0f;
+                    }
+                    float delta = s * a.rate;
+                    delta += c * a.negDecay;
+                    delta += srcAdd[j];
+                    dst[l * a.Y + y] = c + delta;
+                }
+            }
+            __syncthreads();
+            cur = 1 - cur;
+        }
+        if (lane)
+        {
+            for (int l = T + r0; l < T + B; l += rowsPerPass)
+            {
+                const int gx = x0 + (l - T);
+                if (gx < a.X)
+                {
+                    out[static_cast<size_t>(gx) * a.Y + y] = buf[cur][l * a.Y + y];
+                }
+            }
+        }
+    }
+
     // Per-cell face rates: +x/+y face uses this cell's rate (0 past a
     // non-wrapped edge, where the neighbour difference is 0 anyway), -x/-y
     // face the neighbour's.  src/Pde/PDEequations.cpp:199-228.
@@ -268,6 +446,52 @@ namespace
                     static_cast<unsigned>((p->X + kBlockX - 1) / kBlockX));
     }
 
+    // Rows per workgroup B and ticks per launch T of the multi-tick kernel
+    // (both 0 when yDim is too long for its LDS rows).
+    struct MultiShape
+    {
+        int32_t B = 0;
+        int32_t T = 0;
+    };
+
+    constexpr size_t kMultiLds = 64 * 1024; // dynamic LDS per workgroup
+
+    MultiShape ChooseMulti(const halhip_diffusion2d *p)
+    {
+        MultiShape m;
+        if (p->Y > kMultiThreads)
+        {
+            return m;
+        }
+        // (B, T): 8 ticks over 4-row workgroups where the rows fit, else fewer
+        const int32_t rowsPerPass = kMultiThreads / p->Y;
+        for (const MultiShape c : {MultiShape{4, 8}, MultiShape{4, 4}, MultiShape{2, 2}})
+        {
+            const int32_t held = c.B + 2 * c.T;
+            const size_t bytes = 2 * static_cast<size_t>(held) * static_cast<size_t>(p->Y) * sizeof(float);
+            if (bytes <= kMultiLds && (held + rowsPerPass - 1) / rowsPerPass <= kMultiRowsPerThread)
+            {
+                return c;
+            }
+        }
+        return m;
+    }
+
+    void LaunchMulti(halhip_diffusion2d *p, const ExplicitArgs &a, const MultiShape &m, int32_t ticks,
+                     const float *in, float *out)
+    {
+        const size_t lds = 2 * static_cast<size_t>(m.B + 2 * ticks) * static_cast<size_t>(p->Y) * sizeof(float);
+        auto kernel = a.wrapX ? (a.wrapY ? (a.dirichlet ? ExplicitMulti<true, true, true> : ExplicitMulti<true, true, false>)
+                                         : (a.dirichlet ? ExplicitMulti<true, false, true> : ExplicitMulti<true, false, false>))
+                              : (a.wrapY ? (a.dirichlet ? ExplicitMulti<false, true, true> : ExplicitMulti<false, true, false>)
+                                         : (a.dirichlet ? ExplicitMulti<false, false, true> : ExplicitMulti<false, false, false>));
+        p->prof.Begin(p->clock.stream, "explicit_multi");
+        hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>((p->X + m.B - 1) / m.B)), dim3(kMultiThreads), lds,
+                           p->clock.stream, a, m.B, ticks, in, out);
+        Check(hipGetLastError(), "ExplicitMulti launch");
+        p->prof.End(p->clock.stream);
+    }
+
     void LaunchTick(halhip_diffusion2d *p, const ExplicitArgs &a, const float *in, float *out)
     {
         p->prof.Begin(p->clock.stream, "explicit_tick");
@@ -285,6 +509,50 @@ namespace
     {
         Require(steps >= 0, "halhip_diffusion2d_run_steps: negative steps");
         ExplicitArgs a = MakeArgs(p, rate, decay, wrapX, wrapY, bc);
+        const MultiShape m = ChooseMulti(p);
+        if (m.T > 0)
+        {
+            // Graph chunks of six launches (an even count keeps the ping-pong
+            // parity) of T ticks each; the rest launch directly.
+            const int32_t chunk = 6 * m.T;
+            int32_t done = 0;
+            if (!p->prof.On() && steps >= chunk)
+            {
+                std::string key;
+                AppendKey(key, a);
+                AppendKey(key, m);
+                AppendKey(key, p->cur);
+                hipGraphExec_t exec = p->graphs.Find(key);
+                if (exec == nullptr)
+                {
+                    hipGraph_t graph;
+                    Check(hipStreamBeginCapture(p->clock.stream, hipStreamCaptureModeThreadLocal),
+                          "hipStreamBeginCapture");
+                    int c = p->cur;
+                    for (int32_t k = 0; k < 6; k++)
+                    {
+                        LaunchMulti(p, a, m, m.T, p->field[c].get(), p->field[1 - c].get());
+                        c = 1 - c;
+                    }
+                    Check(hipStreamEndCapture(p->clock.stream, &graph), "hipStreamEndCapture");
+                    Check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+                    (void)hipGraphDestroy(graph);
+                    p->graphs.Insert(key, exec);
+                }
+                for (; done + chunk <= steps; done += chunk)
+                {
+                    Check(hipGraphLaunch(exec, p->clock.stream), "hipGraphLaunch");
+                }
+            }
+            while (done < steps)
+            {
+                const int32_t t = std::min(m.T, steps - done);
+                LaunchMulti(p, a, m, t, p->field[p->cur].get(), p->field[1 - p->cur].get());
+                p->cur = 1 - p->cur;
+                done += t;
+            }
+            return;
+        }
         constexpr int32_t kChunk = 50;
         int32_t done = 0;
         if (!p->prof.On() && steps >= kChunk)

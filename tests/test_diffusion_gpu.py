@@ -88,3 +88,49 @@ def test_workload_256_1000_ticks_within_1e5(ctx, oracle):
     err = rel_l2(got, ref)
     print(f"explicit 256^2 x 1000 ticks: rel L2 vs FP64 = {err:.3e}")
     assert err < 1e-5
+
+
+def _explicit_ref(oracle, f, ticks, rate, decay, X, Y, wx, wy, bc, idx, val):
+    """FP64 tick composition Diffusion -> MulAll(-k) -> Add -> Update with a
+    Dirichlet boundary (the oracle's explicit_ticks covers bc=None)."""
+    if bc is None:
+        return oracle.explicit_ticks(f.astype(np.float64), ticks, rate, decay, X, Y, wx, wy, idx,
+                                     np.asarray(val, np.float64))
+    u = f.astype(np.float64)
+    for _ in range(ticks):
+        d = oracle.diffuse(u, np.zeros_like(u), rate, X, Y, wx, wy, bc)
+        d -= decay * u
+        np.add.at(d, np.asarray(idx, np.int64), np.asarray(val, np.float64))
+        u = u + d
+    return u
+
+
+# Several ticks per launch (yDim <= 1024) and the single-tick kernel (longer
+# rows): tiny periodic grids whose halo wraps more than once, partial last
+# workgroups, remainder launches, a dense source map (> 64 sources) and
+# Dirichlet edges.
+MULTI_CASES = [
+    (3, 50, 1, 1, None, 3, 37),
+    (5, 64, 0, 0, 0.3, 0, 13),
+    (70, 257, 1, 0, None, 100, 50),
+    (33, 1024, 0, 1, 0.7, 2, 9),
+    (9, 1500, 1, 1, None, 2, 11),
+    (300, 17, 0, 0, 0.0, 5, 101),
+    (1, 1, 1, 1, None, 1, 20),
+]
+
+
+@pytest.mark.parametrize("X,Y,wx,wy,bc,nsrc,ticks", MULTI_CASES)
+def test_multi_tick_launches_match_oracle(ctx, oracle, X, Y, wx, wy, bc, nsrc, ticks):
+    rng = np.random.default_rng(X * 31 + Y)
+    f = rng.random(X * Y).astype(np.float32)
+    idx = rng.integers(0, X * Y, nsrc)
+    val = rng.random(nsrc).astype(np.float32)
+    plan = halhip.Diffusion2D(ctx, X, Y)
+    plan.upload(f)
+    if nsrc:
+        plan.set_sources(idx, val)
+    plan.run_steps(ticks, 0.2, 1e-3, wx, wy, bc=bc)
+    got = plan.download()
+    ref = _explicit_ref(oracle, f, ticks, 0.2, 1e-3, X, Y, wx, wy, bc, idx, val)
+    assert rel_l2(got, ref) < 1e-6, (X, Y, rel_l2(got, ref))
