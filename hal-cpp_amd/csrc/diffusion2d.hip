@@ -12,11 +12,15 @@
 // Semantics follow src/Pde/PDEequations.cpp:140-161 (zero flux / periodic),
 // include/Hal/Pde/PDEequations.h:50-98, 264-287 (Dirichlet ghost
 // (bc - c)*2), src/Pde/PDEequations.cpp:199-228 (per-cell face rates) and the
-// PDEGrid2D tick Diffusion -> MulAll(-k) -> Add -> Update
-// (include/Hal/Grids/PDEGrid2D.h:103-218).
+// PDEGrid2D tick Diffusion -> Advection -> MulAll(-k) -> Add -> Update
+// (include/Hal/Grids/PDEGrid2D.h:103-218, 289-375).  First-order upwind
+// advection: uniform velocity src/Pde/PDEequations.cpp:380-413 and
+// include/Hal/Pde/PDEequations.h:413-447; per-cell velocities in flux form
+// src/Pde/PDEequations.cpp:535-573 and include/Hal/Pde/PDEequations.h:721-783.
 #include "common.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 
 using namespace halhip;
@@ -32,6 +36,8 @@ namespace
         int32_t X;
         int32_t Y;
         float rate;
+        float xVel; // uniform advection velocity (0: none)
+        float yVel;
         float negDecay;
         int32_t wrapX;
         int32_t wrapY;
@@ -88,8 +94,32 @@ namespace
         return 0.0f;
     }
 
-    // One tick.  TICK: out = u + (rate*lap + negDecay*u + src); otherwise
-    // out = rate*lap (the IDiffusionBackend2D delta).
+    // Upwind advection term from the four neighbour differences (d+x, d-x,
+    // d+y, d-y) of a cell: x axis then y axis, as Advection2 adds them.
+    __device__ __forceinline__ float AdvectTerm(float delta, float xVel, float yVel, float dxp, float dxm, float dyp,
+                                                float dym)
+    {
+        if (xVel > 0.0f)
+        {
+            delta += dxm * xVel;
+        }
+        else if (xVel < 0.0f)
+        {
+            delta -= dxp * xVel;
+        }
+        if (yVel > 0.0f)
+        {
+            delta += dym * yVel;
+        }
+        else if (yVel < 0.0f)
+        {
+            delta -= dyp * yVel;
+        }
+        return delta;
+    }
+
+    // One tick.  TICK: out = u + (rate*lap + advection + negDecay*u + src);
+    // otherwise out = rate*lap + advection (the interface deltas).
     template <bool TICK>
     __global__ __launch_bounds__(kBlockY *kBlockX) void ExplicitStep(ExplicitArgs a, const float *__restrict__ u,
                                                                       float *__restrict__ out)
@@ -103,11 +133,16 @@ namespace
         const size_t i = static_cast<size_t>(x) * a.Y + y;
         const float *row = u + static_cast<size_t>(x) * a.Y;
         const float c = row[y];
-        float s = NeighbourDiffX(u, a, x + 1, y, c);
-        s += NeighbourDiffX(u, a, x - 1, y, c);
-        s += NeighbourDiffY(row, a, x, y + 1, c);
-        s += NeighbourDiffY(row, a, x, y - 1, c);
+        const float dxp = NeighbourDiffX(u, a, x + 1, y, c);
+        const float dxm = NeighbourDiffX(u, a, x - 1, y, c);
+        const float dyp = NeighbourDiffY(row, a, x, y + 1, c);
+        const float dym = NeighbourDiffY(row, a, x, y - 1, c);
+        float s = dxp;
+        s += dxm;
+        s += dyp;
+        s += dym;
         float delta = s * a.rate;
+        delta = AdvectTerm(delta, a.xVel, a.yVel, dxp, dxm, dyp, dym);
         if (!TICK)
         {
             out[i] = delta;
@@ -254,40 +289,16 @@ namespace
                     const float *row = src + l * a.Y;
                     const float c = row[y];
                     // x+1, x-1, y+1, y-1 in ExplicitStep's order
-                    float s;
-                    if (WX || gx + 1 < a.X)
-                    {
-                        s = row[a.Y + y] - c;
-                    }
-                    else
-                    {
-                        s = DIR ? (bcXhi - c) * 2.0f : 0.0f;
-                    }
-                    if (WX || gx > 0)
-                    {
-                        s += row[y - a.Y] - c;
-                    }
-                    else
-                    {
-                        s += DIR ? (bcXlo - c) * 2.0f : 0.0f;
-                    }
-                    if (hiIn)
-                    {
-                        s += row[yHi] - c;
-                    }
-                    else
-                    {
-                        s += DIR ? (a.bcYhi[gx] - c) * 2.0f : 0.0f;
-                    }
-                    if (loIn)
-                    {
-                        s += row[yLo] - c;
-                    }
-                    else
-                    {
-                        s += DIR ? (a.bcYlo[gx] - c) * 2.0f : 0.0f;
-                    }
+                    const float dxp = (WX || gx + 1 < a.X) ? row[a.Y + y] - c : (DIR ? (bcXhi - c) * 2.0f : 0.0f);
+                    const float dxm = (WX || gx > 0) ? row[y - a.Y] - c : (DIR ? (bcXlo - c) * 2.0f : 0.0f);
+                    const float dyp = hiIn ? row[yHi] - c : (DIR ? (a.bcYhi[gx] - c) * 2.0f : 0.0f);
+                    const float dym = loIn ? row[yLo] - c : (DIR ? (a.bcYlo[gx] - c) * 2.0f : 0.0f);
+                    float s = dxp;
+                    s += dxm;
+                    s += dyp;
+                    s += dym;
                     float delta = s * a.rate;
+                    delta = AdvectTerm(delta, a.xVel, a.yVel, dxp, dxm, dyp, dym);
                     delta += c * a.negDecay;
                     delta += srcAdd[j];
                     dst[l * a.Y + y] = c + delta;
@@ -307,6 +318,105 @@ namespace
                 }
             }
         }
+    }
+
+    // Per-cell-velocity upwind advection delta (flux form).  Off-grid field
+    // values and velocities are 0 without edge arrays (bc / vbc null), the
+    // edge values otherwise; vbc's x edges hold the x velocity at x = -1 / X,
+    // its y edges the y velocity at y = -1 / Y.
+    struct AdvectCellsArgs
+    {
+        int32_t X;
+        int32_t Y;
+        int32_t wrapX;
+        int32_t wrapY;
+        const float *bcXlo, *bcXhi, *bcYlo, *bcYhi;
+        const float *vXlo, *vXhi, *vYlo, *vYhi;
+    };
+
+    __device__ __forceinline__ float ShiftX(const float *v, const AdvectCellsArgs &a, int x, int y, const float *lo,
+                                            const float *hi)
+    {
+        if (x >= 0 && x < a.X)
+        {
+            return v[static_cast<size_t>(x) * a.Y + y];
+        }
+        if (a.wrapX)
+        {
+            return v[static_cast<size_t>(x < 0 ? x + a.X : x - a.X) * a.Y + y];
+        }
+        return lo != nullptr ? (x < 0 ? lo[y] : hi[y]) : 0.0f;
+    }
+
+    __device__ __forceinline__ float ShiftY(const float *v, const AdvectCellsArgs &a, int x, int y, const float *lo,
+                                            const float *hi)
+    {
+        if (y >= 0 && y < a.Y)
+        {
+            return v[static_cast<size_t>(x) * a.Y + y];
+        }
+        if (a.wrapY)
+        {
+            return v[static_cast<size_t>(x) * a.Y + (y < 0 ? y + a.Y : y - a.Y)];
+        }
+        return lo != nullptr ? (y < 0 ? lo[x] : hi[x]) : 0.0f;
+    }
+
+    __global__ __launch_bounds__(kBlockY *kBlockX) void AdvectCellsDelta(AdvectCellsArgs a,
+                                                                          const float *__restrict__ u,
+                                                                          const float *__restrict__ xv,
+                                                                          const float *__restrict__ yv,
+                                                                          float *__restrict__ out)
+    {
+        const int y = blockIdx.x * kBlockY + threadIdx.x;
+        const int x = blockIdx.y * kBlockX + threadIdx.y;
+        if (x >= a.X || y >= a.Y)
+        {
+            return;
+        }
+        const size_t i = static_cast<size_t>(x) * a.Y + y;
+        const float xVelP = (x == a.X - 1 && !a.wrapX) ? ShiftX(xv, a, x + 1, y, a.vXlo, a.vXhi) : xv[i];
+        const float yVelP = (y == a.Y - 1 && !a.wrapY) ? ShiftY(yv, a, x, y + 1, a.vYlo, a.vYhi) : yv[i];
+        const float xVelM = ShiftX(xv, a, x - 1, y, a.vXlo, a.vXhi);
+        const float yVelM = ShiftY(yv, a, x, y - 1, a.vYlo, a.vYhi);
+        const float c = u[i];
+        float xFluxP = 0.0f;
+        float xFluxM = 0.0f;
+        float yFluxP = 0.0f;
+        float yFluxM = 0.0f;
+        if (xVelP > 0.0f)
+        {
+            xFluxP = xVelP * c;
+        }
+        else if (xVelP < 0.0f)
+        {
+            xFluxP = xVelP * ShiftX(u, a, x + 1, y, a.bcXlo, a.bcXhi);
+        }
+        if (xVelM > 0.0f)
+        {
+            xFluxM = xVelM * ShiftX(u, a, x - 1, y, a.bcXlo, a.bcXhi);
+        }
+        else if (xVelM < 0.0f)
+        {
+            xFluxM = xVelM * c;
+        }
+        if (yVelP > 0.0f)
+        {
+            yFluxP = yVelP * c;
+        }
+        else if (yVelP < 0.0f)
+        {
+            yFluxP = yVelP * ShiftY(u, a, x, y + 1, a.bcYlo, a.bcYhi);
+        }
+        if (yVelM > 0.0f)
+        {
+            yFluxM = yVelM * ShiftY(u, a, x, y - 1, a.bcYlo, a.bcYhi);
+        }
+        else if (yVelM < 0.0f)
+        {
+            yFluxM = yVelM * c;
+        }
+        out[i] = -xFluxP + xFluxM - yFluxP + yFluxM;
     }
 
     // Per-cell face rates: +x/+y face uses this cell's rate (0 past a
@@ -379,6 +489,8 @@ struct halhip_diffusion2d
     DeviceArray<float> srcVal;
     DeviceArray<float> srcDense;
     int32_t nSources = 0;
+    double xVel = 0.0; // uniform advection of the resident ticks
+    double yVel = 0.0;
     EdgeValues edges;
     std::string edgeKey;
     StreamClock clock;
@@ -402,7 +514,7 @@ namespace
 
     // Uploads Dirichlet edges when they changed and fills the kernel args.
     ExplicitArgs MakeArgs(halhip_diffusion2d *p, double rate, double decay, int32_t wrapX, int32_t wrapY,
-                          const halhip_bc *bc)
+                          const halhip_bc *bc, double xVel = 0.0, double yVel = 0.0)
     {
         EdgeValues e;
         e.From(bc, p->X, p->Y);
@@ -425,6 +537,8 @@ namespace
         a.X = p->X;
         a.Y = p->Y;
         a.rate = static_cast<float>(rate);
+        a.xVel = static_cast<float>(xVel);
+        a.yVel = static_cast<float>(yVel);
         a.negDecay = static_cast<float>(-decay);
         a.wrapX = wrapX ? 1 : 0;
         a.wrapY = wrapY ? 1 : 0;
@@ -508,7 +622,7 @@ namespace
                       const halhip_bc *bc)
     {
         Require(steps >= 0, "halhip_diffusion2d_run_steps: negative steps");
-        ExplicitArgs a = MakeArgs(p, rate, decay, wrapX, wrapY, bc);
+        ExplicitArgs a = MakeArgs(p, rate, decay, wrapX, wrapY, bc, p->xVel, p->yVel);
         const MultiShape m = ChooseMulti(p);
         if (m.T > 0)
         {
@@ -733,6 +847,149 @@ extern "C"
                 {
                     deltas[k] += static_cast<double>(p->hostBuf[k]);
                 }
+            });
+    }
+
+    int halhip_diffusion2d_advect_f64(halhip_diffusion2d *p, const double *field, double *deltas, int64_t n,
+                                      double xVel, double yVel, int32_t wrapX, int32_t wrapY, const halhip_bc *bc)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_advect_f64");
+                RequireLength(n, static_cast<int64_t>(Cells(p)), "halhip_diffusion2d_advect_f64");
+                Require(field != nullptr && deltas != nullptr, "halhip_diffusion2d_advect_f64: null array");
+                Require(std::isfinite(xVel) && std::isfinite(yVel), "halhip_diffusion2d_advect_f64: bad velocity");
+                const size_t cells = Cells(p);
+                p->hostBuf.resize(cells);
+                for (size_t k = 0; k < cells; k++)
+                {
+                    p->hostBuf[k] = static_cast<float>(field[k]);
+                }
+                Check(hipMemcpyAsync(p->aux[0].get(), p->hostBuf.data(), cells * sizeof(float),
+                                     hipMemcpyHostToDevice, p->clock.stream),
+                      "hipMemcpyAsync");
+                // rate 0: the kernel's delta is the advection term alone
+                ExplicitArgs a = MakeArgs(p, 0.0, 0.0, wrapX, wrapY, bc, xVel, yVel);
+                if (p->aux[1].get() == nullptr)
+                {
+                    p->aux[1].Allocate(p->ctx, cells);
+                }
+                p->prof.Begin(p->clock.stream, "advect_delta");
+                hipLaunchKernelGGL(ExplicitStep<false>, Grid(p), dim3(kBlockY, kBlockX), 0, p->clock.stream, a,
+                                   p->aux[0].get(), p->aux[1].get());
+                Check(hipGetLastError(), "ExplicitStep launch");
+                p->prof.End(p->clock.stream);
+                Check(hipMemcpyAsync(p->hostBuf.data(), p->aux[1].get(), cells * sizeof(float),
+                                     hipMemcpyDeviceToHost, p->clock.stream),
+                      "hipMemcpyAsync");
+                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                for (size_t k = 0; k < cells; k++)
+                {
+                    deltas[k] += static_cast<double>(p->hostBuf[k]);
+                }
+            });
+    }
+
+    int halhip_diffusion2d_advect_cells_f64(halhip_diffusion2d *p, const double *field, double *deltas,
+                                            const double *xVels, const double *yVels, int64_t n, int32_t wrapX,
+                                            int32_t wrapY, const halhip_bc *bc, const halhip_bc *velBc)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_advect_cells_f64");
+                RequireLength(n, static_cast<int64_t>(Cells(p)), "halhip_diffusion2d_advect_cells_f64");
+                Require(field != nullptr && deltas != nullptr && xVels != nullptr && yVels != nullptr,
+                        "halhip_diffusion2d_advect_cells_f64: null array");
+                EdgeValues e;
+                e.From(bc, p->X, p->Y);
+                EdgeValues v;
+                v.From(velBc, p->X, p->Y);
+                Require(!v.dirichlet || e.dirichlet,
+                        "halhip_diffusion2d_advect_cells_f64: velocity edges need field edges (Dirichlet bc)");
+                const size_t cells = Cells(p);
+                // field, x and y velocities, then the edge arrays, in one upload
+                std::vector<float> host(3 * cells);
+                for (size_t k = 0; k < cells; k++)
+                {
+                    host[k] = static_cast<float>(field[k]);
+                    host[cells + k] = static_cast<float>(xVels[k]);
+                    host[2 * cells + k] = static_cast<float>(yVels[k]);
+                }
+                const size_t edgeLen = 2 * static_cast<size_t>(p->X) + 2 * static_cast<size_t>(p->Y);
+                if (e.dirichlet)
+                {
+                    for (const EdgeValues *ev : {&e, &v})
+                    {
+                        if (ev->dirichlet)
+                        {
+                            host.insert(host.end(), ev->xlo.begin(), ev->xlo.end());
+                            host.insert(host.end(), ev->xhi.begin(), ev->xhi.end());
+                            host.insert(host.end(), ev->ylo.begin(), ev->ylo.end());
+                            host.insert(host.end(), ev->yhi.begin(), ev->yhi.end());
+                        }
+                    }
+                }
+                DeviceArray<float> dev;
+                dev.Allocate(p->ctx, host.size() + cells);
+                Check(hipMemcpyAsync(dev.get(), host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice,
+                                     p->clock.stream),
+                      "hipMemcpyAsync");
+                float *d = dev.get();
+                AdvectCellsArgs a{};
+                a.X = p->X;
+                a.Y = p->Y;
+                a.wrapX = wrapX ? 1 : 0;
+                a.wrapY = wrapY ? 1 : 0;
+                if (e.dirichlet)
+                {
+                    float *eb = d + 3 * cells;
+                    a.bcXlo = eb;
+                    a.bcXhi = eb + p->Y;
+                    a.bcYlo = eb + 2 * p->Y;
+                    a.bcYhi = eb + 2 * p->Y + p->X;
+                    if (v.dirichlet)
+                    {
+                        float *vb = eb + edgeLen;
+                        a.vXlo = vb;
+                        a.vXhi = vb + p->Y;
+                        a.vYlo = vb + 2 * p->Y;
+                        a.vYhi = vb + 2 * p->Y + p->X;
+                    }
+                }
+                float *out = d + host.size();
+                p->prof.Begin(p->clock.stream, "advect_cells_delta");
+                hipLaunchKernelGGL(AdvectCellsDelta, Grid(p), dim3(kBlockY, kBlockX), 0, p->clock.stream, a,
+                                   static_cast<const float *>(d), static_cast<const float *>(d + cells),
+                                   static_cast<const float *>(d + 2 * cells), out);
+                Check(hipGetLastError(), "AdvectCellsDelta launch");
+                p->prof.End(p->clock.stream);
+                p->hostBuf.resize(cells);
+                Check(hipMemcpyAsync(p->hostBuf.data(), out, cells * sizeof(float), hipMemcpyDeviceToHost,
+                                     p->clock.stream),
+                      "hipMemcpyAsync");
+                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                for (size_t k = 0; k < cells; k++)
+                {
+                    deltas[k] += static_cast<double>(p->hostBuf[k]);
+                }
+            });
+    }
+
+    int halhip_diffusion2d_set_advection(halhip_diffusion2d *p, double xVel, double yVel)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_set_advection");
+                Require(std::isfinite(xVel) && std::isfinite(yVel), "halhip_diffusion2d_set_advection: bad velocity");
+                // PDEGrid2D::CheckAdvectionStable (include/Hal/Grids/PDEGrid2D.h:444-451)
+                Require(std::fabs(xVel) + std::fabs(yVel) <= 0.5,
+                        "halhip_diffusion2d_set_advection: advection rate component sum above stable maximum value "
+                        "of 0.5");
+                p->xVel = xVel;
+                p->yVel = yVel;
             });
     }
 

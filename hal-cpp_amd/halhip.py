@@ -70,6 +70,10 @@ _SIGS = {
     "halhip_diffusion2d_destroy": [_vp],
     "halhip_diffusion2d_diffuse_f64": [_vp, _dp, _dp, _i64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
     "halhip_diffusion2d_diffuse_rates_f64": [_vp, _dp, _dp, _dp, _dp, _i64, _i32, _i32],
+    "halhip_diffusion2d_advect_f64": [_vp, _dp, _dp, _i64, _f64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
+    "halhip_diffusion2d_advect_cells_f64": [_vp, _dp, _dp, _dp, _dp, _i64, _i32, _i32, ctypes.POINTER(HalBC),
+                                            ctypes.POINTER(HalBC)],
+    "halhip_diffusion2d_set_advection": [_vp, _f64, _f64],
     "halhip_diffusion2d_upload": [_vp, _fp, _i64],
     "halhip_diffusion2d_download": [_vp, _fp, _i64],
     "halhip_diffusion2d_set_sources": [_vp, _i32, _lp, _fp],
@@ -311,6 +315,31 @@ class Diffusion2D(_Plan):
                                              ratesX.ctypes.data_as(_dp), ratesY.ctypes.data_as(_dp),
                                              field.size, int(wrapX), int(wrapY)))
         return deltas
+
+    def advect(self, field, deltas, xVel, yVel, wrapX, wrapY, bc=None):
+        """deltas += uniform-velocity upwind advection (in place on the float64 `deltas`)."""
+        field = _f64(field)
+        _check_field_sizes(field, deltas, self.X, self.Y, "Diffusion2D.advect")
+        s, keep = make_bc(bc, self.X, self.Y)
+        _check(self._fn("advect_f64")(self.h, field.ctypes.data_as(_dp), deltas.ctypes.data_as(_dp), field.size,
+                                      xVel, yVel, int(wrapX), int(wrapY), _bcp(s)))
+        return deltas
+
+    def advect_cells(self, field, deltas, xVels, yVels, wrapX, wrapY, bc=None, vel_bc=None):
+        """deltas += per-cell-velocity upwind advection (flux form)."""
+        field, xVels, yVels = _f64(field), _f64(xVels), _f64(yVels)
+        _check_field_sizes(field, deltas, self.X, self.Y, "Diffusion2D.advect_cells")
+        if xVels.size != field.size or yVels.size != field.size:
+            raise InvalidArgument(ERR_INVALID_ARGUMENT, "Diffusion2D.advect_cells: velocities must hold xDim*yDim values")
+        s, keep = make_bc(bc, self.X, self.Y)
+        v, keepv = make_bc(vel_bc, self.X, self.Y)
+        _check(self._fn("advect_cells_f64")(self.h, field.ctypes.data_as(_dp), deltas.ctypes.data_as(_dp),
+                                            xVels.ctypes.data_as(_dp), yVels.ctypes.data_as(_dp), field.size,
+                                            int(wrapX), int(wrapY), _bcp(s), _bcp(v)))
+        return deltas
+
+    def set_advection(self, xVel, yVel):
+        _check(self._fn("set_advection")(self.h, xVel, yVel))
 
     def upload(self, field):
         field = _f32(field)

@@ -269,6 +269,76 @@ namespace Hal
             ThrowIfFailed(halhip_diffusion2d_download(plan, field.data(), static_cast<int64_t>(field.size())), caller);
         }
 
+        void HipDiffusionBackend2D::Advect(const std::vector<double> &field,
+                                           std::vector<double>       &deltas,
+                                           double                     xVel,
+                                           double                     yVel,
+                                           int32_t                    xDim,
+                                           int32_t                    yDim,
+                                           bool                       wrapX,
+                                           bool                       wrapY)
+        {
+            const char *caller = "HipDiffusionBackend2D::Advect";
+            Internal::CheckFieldSizes(field, deltas, xDim, yDim, caller);
+            halhip_diffusion2d *plan = Plan(xDim, yDim, caller);
+            ThrowIfFailed(halhip_diffusion2d_advect_f64(plan, field.data(), deltas.data(),
+                                                        static_cast<int64_t>(field.size()), xVel, yVel, wrapX, wrapY,
+                                                        nullptr),
+                          caller);
+        }
+
+        void HipDiffusionBackend2D::Advect(const std::vector<double> &field,
+                                           std::vector<double>       &deltas,
+                                           double                     xVel,
+                                           double                     yVel,
+                                           int32_t                    xDim,
+                                           int32_t                    yDim,
+                                           bool                       wrapX,
+                                           bool                       wrapY,
+                                           const DirichletBoundary   &bc)
+        {
+            const char *caller = "HipDiffusionBackend2D::Advect (Dirichlet)";
+            Internal::CheckFieldSizes(field, deltas, xDim, yDim, caller);
+            halhip_diffusion2d *plan = Plan(xDim, yDim, caller);
+            halhip_bc b = ToBc(bc, xDim, yDim, caller);
+            ThrowIfFailed(halhip_diffusion2d_advect_f64(plan, field.data(), deltas.data(),
+                                                        static_cast<int64_t>(field.size()), xVel, yVel, wrapX, wrapY,
+                                                        &b),
+                          caller);
+        }
+
+        void HipDiffusionBackend2D::AdvectCells(const std::vector<double> &field,
+                                                std::vector<double>       &deltas,
+                                                const std::vector<double> &xVels,
+                                                const std::vector<double> &yVels,
+                                                int32_t                    xDim,
+                                                int32_t                    yDim,
+                                                bool                       wrapX,
+                                                bool                       wrapY)
+        {
+            const char *caller = "HipDiffusionBackend2D::AdvectCells";
+            Internal::CheckFieldSizes(field, deltas, xDim, yDim, caller);
+            if (xVels.size() != field.size() || yVels.size() != field.size())
+            {
+                Throw(true, std::string(caller) + ": velocities must hold xDim*yDim values");
+            }
+            halhip_diffusion2d *plan = Plan(xDim, yDim, caller);
+            ThrowIfFailed(halhip_diffusion2d_advect_cells_f64(plan, field.data(), deltas.data(), xVels.data(),
+                                                              yVels.data(), static_cast<int64_t>(field.size()), wrapX,
+                                                              wrapY, nullptr, nullptr),
+                          caller);
+        }
+
+        void HipDiffusionBackend2D::SetAdvection(double xVel, double yVel)
+        {
+            const char *caller = "HipDiffusionBackend2D::SetAdvection";
+            if (!m_fixed)
+            {
+                Throw(true, std::string(caller) + " needs a backend constructed with dimensions");
+            }
+            ThrowIfFailed(halhip_diffusion2d_set_advection(Plan(m_xDim, m_yDim, caller), xVel, yVel), caller);
+        }
+
         // ---- ADI diffusion ---------------------------------------------------------------
 
         HipAdiDiffusionBackend2D::HipAdiDiffusionBackend2D(HipContext &context)

@@ -92,6 +92,42 @@ namespace Hal
                               bool                       wrapX,
                               bool                       wrapY);
 
+            // PDEGrid2D::Advection(xVel, yVel[, boundary]) on the interface's
+            // terms: deltas += first-order upwind advection
+            // (src/Pde/PDEequations.cpp:380-413, include/Hal/Pde/PDEequations.h:413-447).
+            void Advect(const std::vector<double> &field,
+                        std::vector<double>       &deltas,
+                        double                     xVel,
+                        double                     yVel,
+                        int32_t                    xDim,
+                        int32_t                    yDim,
+                        bool                       wrapX,
+                        bool                       wrapY);
+            void Advect(const std::vector<double> &field,
+                        std::vector<double>       &deltas,
+                        double                     xVel,
+                        double                     yVel,
+                        int32_t                    xDim,
+                        int32_t                    yDim,
+                        bool                       wrapX,
+                        bool                       wrapY,
+                        const DirichletBoundary   &bc);
+
+            // PDEGrid2D::Advection(xVels, yVels): per-cell velocities in flux
+            // form with zero-flux walls (src/Pde/PDEequations.cpp:535-573).
+            void AdvectCells(const std::vector<double> &field,
+                             std::vector<double>       &deltas,
+                             const std::vector<double> &xVels,
+                             const std::vector<double> &yVels,
+                             int32_t                    xDim,
+                             int32_t                    yDim,
+                             bool                       wrapX,
+                             bool                       wrapY);
+
+            // Uniform advection added to every RunSteps tick (Diffusion +
+            // Advection + MulAll + Update); 0, 0 turns it off.
+            void SetAdvection(double xVel, double yVel);
+
             // DiffusionKernel2D::RunSteps (include/Hal/Gpu/DiffusionKernel2D.h:34-38):
             // `steps` full ticks on the device, one upload and one readback;
             // decay and point sources are fused into each tick.

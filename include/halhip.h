@@ -113,6 +113,33 @@ int halhip_diffusion2d_diffuse_rates_f64(halhip_diffusion2d *plan, const double 
                                          int32_t wrapY);
 
 /* Resident state: FP32 field on the device. */
+/* deltas += first-order upwind advection with the uniform velocity
+ * (xVel, yVel): Hal::Internal::Advection2 (src/Pde/PDEequations.cpp:380-413;
+ * boundary form include/Hal/Pde/PDEequations.h:413-447), as called by
+ * PDEGrid2D::Advection (include/Hal/Grids/PDEGrid2D.h:349-375).  Off-grid
+ * neighbours follow bc as in diffusion. */
+int halhip_diffusion2d_advect_f64(halhip_diffusion2d *plan, const double *field, double *deltas, int64_t n,
+                                  double xVel, double yVel, int32_t wrapX, int32_t wrapY, const halhip_bc *bc);
+
+/* deltas += conservative upwind advection with per-cell velocities
+ * (xVels, yVels: n values each): Hal::Internal::Advection2
+ * (src/Pde/PDEequations.cpp:535-573; boundary form
+ * include/Hal/Pde/PDEequations.h:721-783; PDEGrid2D::Advection with vectors,
+ * include/Hal/Grids/PDEGrid2D.h:377-399).  Without bc, off-grid values and
+ * velocities are 0 (walls); with a Dirichlet bc the field's off-grid values
+ * are its edges and velBc (optional) gives the off-grid velocities: its x
+ * edges the x velocity at x = -1 / xDim, its y edges the y velocity at
+ * y = -1 / yDim. */
+int halhip_diffusion2d_advect_cells_f64(halhip_diffusion2d *plan, const double *field, double *deltas,
+                                        const double *xVels, const double *yVels, int64_t n, int32_t wrapX,
+                                        int32_t wrapY, const halhip_bc *bc, const halhip_bc *velBc);
+
+/* Uniform advection velocity included in every resident tick
+ * (Diffusion + Advection + MulAll + Add + Update); 0, 0 (the default)
+ * disables it.  Rejects |xVel| + |yVel| > 0.5 like
+ * PDEGrid2D::CheckAdvectionStable (include/Hal/Grids/PDEGrid2D.h:444-451). */
+int halhip_diffusion2d_set_advection(halhip_diffusion2d *plan, double xVel, double yVel);
+
 int halhip_diffusion2d_upload(halhip_diffusion2d *plan, const float *field, int64_t n);
 int halhip_diffusion2d_download(halhip_diffusion2d *plan, float *field, int64_t n);
 

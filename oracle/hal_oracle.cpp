@@ -18,6 +18,11 @@
 //                           point and linear ramp; scipy banded solve
 //   oracle_agent_*          golden vectors; brute-force all-pairs sum; sorted
 //                           (cell, index) order for bins
+//   oracle_advect*          golden vectors (uniform and per-cell velocity,
+//                           zero flux, periodic, Dirichlet); mass invariance
+//                           of the flux form; uniform velocity fields give
+//                           the uniform-velocity form in the interior
+//   oracle_model_ticks      golden tick compositions
 //   oracle_adi_cell_*       per-cell-diffusivity ADI and its adjoint: no
 //                           reference counterpart.  Pinned by "uniform D ==
 //                           oracle_adi", the dot-product test <Ax,y>=<x,A^T y>
@@ -77,6 +82,47 @@ namespace
             return (bc - c) * 2;
         }
         return 0;
+    }
+
+    // Value of a per-cell array (field or velocity) at (x, y) along x: the
+    // cell, the wrapped cell, or past a non-wrapped edge the boundary value
+    // (lo at x = -1, hi at x = X), 0 without one.
+    // src/Pde/PDEequations.cpp:163-198; include/Hal/Pde/PDEequations.h:579-615.
+    inline double ShiftX(const double *v, int32_t x, int32_t y, int32_t X, int32_t Y, bool wrap, const double *lo,
+                         const double *hi)
+    {
+        if (x >= 0 && x < X)
+        {
+            return v[static_cast<size_t>(x) * Y + y];
+        }
+        if (wrap)
+        {
+            return v[static_cast<size_t>(WrapI(x, X)) * Y + y];
+        }
+        if (lo != nullptr)
+        {
+            return (x < 0) ? lo[y] : hi[y];
+        }
+        return 0.0;
+    }
+
+    inline double ShiftY(const double *v, int32_t x, int32_t y, int32_t X, int32_t Y, bool wrap, const double *lo,
+                         const double *hi)
+    {
+        (void)X;
+        if (y >= 0 && y < Y)
+        {
+            return v[static_cast<size_t>(x) * Y + y];
+        }
+        if (wrap)
+        {
+            return v[static_cast<size_t>(x) * Y + WrapI(y, Y)];
+        }
+        if (lo != nullptr)
+        {
+            return (y < 0) ? lo[x] : hi[x];
+        }
+        return 0.0;
     }
 
     // Thomas algorithm on explicit coefficient arrays: a sub-diagonal (a[0]
@@ -437,6 +483,142 @@ extern "C"
         for (int32_t t = 0; t < ticks; t++)
         {
             oracle_diffuse(field, deltas.data(), rate, X, Y, wrapX, wrapY, nullptr, nullptr, nullptr, nullptr);
+            if (decay != 0.0)
+            {
+                for (size_t i = 0; i < n; i++)
+                {
+                    deltas[i] += field[i] * (-decay);
+                }
+            }
+            for (int32_t s = 0; s < nSrc; s++)
+            {
+                deltas[static_cast<size_t>(srcIdx[s])] += srcVal[s];
+            }
+            for (size_t i = 0; i < n; i++)
+            {
+                field[i] += deltas[i];
+                deltas[i] = 0.0;
+            }
+        }
+    }
+
+    // deltas += first-order upwind advection with a uniform velocity: the
+    // upwind neighbour difference times the velocity along each axis, with
+    // the same off-grid rules as diffusion (Dirichlet ghost (bc - c)*2 when
+    // edges are given).  src/Pde/PDEequations.cpp:380-413 (zero flux /
+    // periodic), include/Hal/Pde/PDEequations.h:413-447 (boundary function).
+    void oracle_advect(const double *field, double *deltas, double xVel, double yVel, int32_t X, int32_t Y,
+                       int32_t wrapX, int32_t wrapY, const double *bcXlo, const double *bcXhi, const double *bcYlo,
+                       const double *bcYhi)
+    {
+        for (int32_t x = 0; x < X; x++)
+        {
+            for (int32_t y = 0; y < Y; y++)
+            {
+                const size_t i = static_cast<size_t>(x) * Y + y;
+                const double c = field[i];
+                if (xVel > 0)
+                {
+                    deltas[i] += DeltaX(field, c, x - 1, y, X, Y, wrapX != 0, bcXlo, bcXhi) * xVel;
+                }
+                else if (xVel < 0)
+                {
+                    deltas[i] -= DeltaX(field, c, x + 1, y, X, Y, wrapX != 0, bcXlo, bcXhi) * xVel;
+                }
+                if (yVel > 0)
+                {
+                    deltas[i] += DeltaY(field, c, x, y - 1, X, Y, wrapY != 0, bcYlo, bcYhi) * yVel;
+                }
+                else if (yVel < 0)
+                {
+                    deltas[i] -= DeltaY(field, c, x, y + 1, X, Y, wrapY != 0, bcYlo, bcYhi) * yVel;
+                }
+            }
+        }
+    }
+
+    // deltas += conservative first-order upwind advection with per-cell
+    // velocities: the flux through each face takes the velocity of the cell
+    // on its low side (the cell itself on the +x/+y face, except at a
+    // non-wrapped high edge where the off-grid velocity is used) and the
+    // upwind cell's value.  Without boundary arrays off-grid values and
+    // velocities are 0 (nothing crosses the walls); with them (bc for the
+    // field, vbc for the velocity components: xlo/xhi the x velocity at
+    // x = -1 / X, ylo/yhi the y velocity at y = -1 / Y) they are the given
+    // values.  src/Pde/PDEequations.cpp:535-573, include/Hal/Pde/PDEequations.h:721-783.
+    void oracle_advect_cells(const double *field, double *deltas, const double *xVels, const double *yVels, int32_t X,
+                             int32_t Y, int32_t wrapX, int32_t wrapY, const double *bcXlo, const double *bcXhi,
+                             const double *bcYlo, const double *bcYhi, const double *vXlo, const double *vXhi,
+                             const double *vYlo, const double *vYhi)
+    {
+        const bool wx = wrapX != 0;
+        const bool wy = wrapY != 0;
+        for (int32_t x = 0; x < X; x++)
+        {
+            for (int32_t y = 0; y < Y; y++)
+            {
+                const size_t i = static_cast<size_t>(x) * Y + y;
+                const double xVelP = (x == X - 1 && !wx) ? ShiftX(xVels, x + 1, y, X, Y, wx, vXlo, vXhi) : xVels[i];
+                const double yVelP = (y == Y - 1 && !wy) ? ShiftY(yVels, x, y + 1, X, Y, wy, vYlo, vYhi) : yVels[i];
+                const double xVelM = ShiftX(xVels, x - 1, y, X, Y, wx, vXlo, vXhi);
+                const double yVelM = ShiftY(yVels, x, y - 1, X, Y, wy, vYlo, vYhi);
+                const double c = field[i];
+                double xFluxP = 0.0;
+                double xFluxM = 0.0;
+                double yFluxP = 0.0;
+                double yFluxM = 0.0;
+                if (xVelP > 0)
+                {
+                    xFluxP = xVelP * c;
+                }
+                else if (xVelP < 0)
+                {
+                    xFluxP = xVelP * ShiftX(field, x + 1, y, X, Y, wx, bcXlo, bcXhi);
+                }
+                if (xVelM > 0)
+                {
+                    xFluxM = xVelM * ShiftX(field, x - 1, y, X, Y, wx, bcXlo, bcXhi);
+                }
+                else if (xVelM < 0)
+                {
+                    xFluxM = xVelM * c;
+                }
+                if (yVelP > 0)
+                {
+                    yFluxP = yVelP * c;
+                }
+                else if (yVelP < 0)
+                {
+                    yFluxP = yVelP * ShiftY(field, x, y + 1, X, Y, wy, bcYlo, bcYhi);
+                }
+                if (yVelM > 0)
+                {
+                    yFluxM = yVelM * ShiftY(field, x, y - 1, X, Y, wy, bcYlo, bcYhi);
+                }
+                else if (yVelM < 0)
+                {
+                    yFluxM = yVelM * c;
+                }
+                deltas[i] += -xFluxP + xFluxM - yFluxP + yFluxM;
+            }
+        }
+    }
+
+    // `ticks` model ticks Diffusion(rate) + Advection(xVel, yVel) +
+    // MulAll(-decay) + Add(sources) + Update, with Dirichlet edges for both
+    // stencils when given (PDEGrid2D::Diffusion / Advection with a boundary,
+    // include/Hal/Grids/PDEGrid2D.h:289-375).
+    void oracle_model_ticks(double *field, int32_t ticks, double rate, double xVel, double yVel, double decay,
+                            int32_t X, int32_t Y, int32_t wrapX, int32_t wrapY, const double *bcXlo,
+                            const double *bcXhi, const double *bcYlo, const double *bcYhi, int32_t nSrc,
+                            const int64_t *srcIdx, const double *srcVal)
+    {
+        const size_t n = static_cast<size_t>(X) * Y;
+        std::vector<double> deltas(n, 0.0);
+        for (int32_t t = 0; t < ticks; t++)
+        {
+            oracle_diffuse(field, deltas.data(), rate, X, Y, wrapX, wrapY, bcXlo, bcXhi, bcYlo, bcYhi);
+            oracle_advect(field, deltas.data(), xVel, yVel, X, Y, wrapX, wrapY, bcXlo, bcXhi, bcYlo, bcYhi);
             if (decay != 0.0)
             {
                 for (size_t i = 0; i < n; i++)

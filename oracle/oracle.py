@@ -104,11 +104,48 @@ class _Lib:
         return vel
 
 
+    def advect(self, field, deltas, xVel, yVel, X, Y, wrapX, wrapY, bc=None):
+        """deltas + uniform-velocity upwind advection (returns a new array)."""
+        field = _f(field)
+        deltas = _f(deltas).copy()
+        e = _edges(bc, X, Y)
+        f = self.fn("advect")
+        f.argtypes = [_dp, _dp, _f64, _f64, _i32, _i32, _i32, _i32, _dp, _dp, _dp, _dp]
+        f(_p(field), _p(deltas), xVel, yVel, X, Y, int(wrapX), int(wrapY), *[_p(a) for a in e])
+        return deltas
+
+    def advect_cells(self, field, deltas, xVels, yVels, X, Y, wrapX, wrapY, bc=None, vbc=None):
+        """deltas + per-cell-velocity upwind advection; bc / vbc: field and
+        velocity edge values (both or neither)."""
+        field, xVels, yVels = _f(field), _f(xVels), _f(yVels)
+        deltas = _f(deltas).copy()
+        e = _edges(bc, X, Y)
+        v = _edges(vbc, X, Y)
+        f = self.fn("advect_cells")
+        f.argtypes = [_dp, _dp, _dp, _dp, _i32, _i32, _i32, _i32] + [_dp] * 8
+        f(_p(field), _p(deltas), _p(xVels), _p(yVels), X, Y, int(wrapX), int(wrapY), *[_p(a) for a in e],
+          *[_p(a) for a in v])
+        return deltas
+
+
 class Oracle(_Lib):
     """FP64 restatement (hal_oracle.cpp)."""
 
     def __init__(self, path=ORACLE_SO):
         super().__init__(path, "oracle_")
+
+    def model_ticks(self, field, ticks, rate, xVel, yVel, decay, X, Y, wrapX, wrapY, bc=None, src_idx=(),
+                    src_val=()):
+        """Diffusion + Advection + MulAll + Add + Update ticks (FP64)."""
+        field = _f(field).copy()
+        e = _edges(bc, X, Y)
+        si = np.ascontiguousarray(src_idx, np.int64)
+        sv = _f(src_val)
+        f = self.fn("model_ticks")
+        f.argtypes = [_dp, _i32, _f64, _f64, _f64, _f64, _i32, _i32, _i32, _i32, _dp, _dp, _dp, _dp, _i32, _lp, _dp]
+        f(_p(field), ticks, rate, xVel, yVel, decay, X, Y, int(wrapX), int(wrapY), *[_p(a) for a in e], len(si),
+          _p(si, _lp), _p(sv))
+        return field
 
     def explicit_ticks(self, field, ticks, rate, decay, X, Y, wrapX, wrapY, src_idx=(), src_val=()):
         field = _f(field).copy()
@@ -220,6 +257,40 @@ class Reference(_Lib):
           len(si), _p(si, _lp), _p(sv))
         return field
 
+
+    def model_ticks(self, field, ticks, rate, xVel, yVel, decay, X, Y, wrapX, wrapY, bc_value=None, src_idx=(),
+                    src_val=()):
+        """PDEGrid2D Diffusion + Advection + MulAll + Add + Update ticks."""
+        field = _f(field).copy()
+        si = np.ascontiguousarray(src_idx, np.int64)
+        sv = _f(src_val)
+        f = self.fn("model_ticks")
+        f.argtypes = [_dp, _i32, _f64, _f64, _f64, _f64, _i32, _i32, _i32, _i32, _i32, _f64, _i32, _lp, _dp]
+        f(_p(field), ticks, rate, xVel, yVel, decay, X, Y, int(wrapX), int(wrapY), 0 if bc_value is None else 1,
+          0.0 if bc_value is None else float(bc_value), len(si), _p(si, _lp), _p(sv))
+        return field
+
+    def grid_ticks_timed(self, field, warmup, ticks, adi, rate, decay, X, Y, wrapX, wrapY, bc_value=None,
+                         src_idx=(), src_val=()):
+        """Seconds for `ticks` PDEGrid2D model ticks of the compiled reference."""
+        field = _f(field).copy()
+        si = np.ascontiguousarray(src_idx, np.int64)
+        sv = _f(src_val)
+        f = self.fn("grid_ticks_timed")
+        f.restype = ctypes.c_double
+        f.argtypes = [_dp, _i32, _i32, _i32, _f64, _f64, _i32, _i32, _i32, _i32, _i32, _f64, _i32, _lp, _dp]
+        secs = f(_p(field), warmup, ticks, int(adi), rate, decay, X, Y, int(wrapX), int(wrapY),
+                 0 if bc_value is None else 1, 0.0 if bc_value is None else float(bc_value),
+                 len(si), _p(si, _lp), _p(sv))
+        return secs, field
+
+    def agent_forces_timed(self, pos, radii, calls, R, k, X, Y, wrapX, wrapY):
+        """Seconds for `calls` CpuAgentForceBackend2D::AccumulateForces calls."""
+        pos, radii = _f(pos), _f(radii)
+        f = self.fn("agent_forces_timed")
+        f.restype = ctypes.c_double
+        f.argtypes = [_dp, _dp, _i32, _i32, _f64, _f64, _i32, _i32, _i32, _i32]
+        return f(_p(pos), _p(radii), radii.size, calls, R, k, X, Y, int(wrapX), int(wrapY))
 
     def adi_ticks_timed(self, field, warmup, ticks, rate, X, Y, bc_value):
         """Seconds for `ticks` Dirichlet ADI ticks of the compiled reference."""

@@ -193,6 +193,156 @@ extern "C"
         return std::chrono::duration<double>(t1 - t0).count();
     }
 
+    // Hal::Internal::Advection2 with a uniform velocity (zero flux / periodic,
+    // or Dirichlet edge values when xlo is non-null).
+    void ref_advect(const double *field, double *deltas, double xVel, double yVel, int32_t X, int32_t Y,
+                    int32_t wrapX, int32_t wrapY, const double *bcXlo, const double *bcXhi, const double *bcYlo,
+                    const double *bcYhi)
+    {
+        if (bcXlo == nullptr)
+        {
+            Hal::Internal::Advection2(field, deltas, xVel, yVel, X, Y, wrapX != 0, wrapY != 0);
+            return;
+        }
+        EdgeBc bc{X, Y, bcXlo, bcXhi, bcYlo, bcYhi};
+        Hal::Internal::Advection2(field, deltas, xVel, yVel, X, Y, wrapX != 0, wrapY != 0, bc);
+    }
+
+    // Hal::Internal::Advection2 with per-cell velocities (zero flux /
+    // periodic, or field and velocity edge values when bcXlo is non-null).
+    void ref_advect_cells(const double *field, double *deltas, const double *xVels, const double *yVels, int32_t X,
+                          int32_t Y, int32_t wrapX, int32_t wrapY, const double *bcXlo, const double *bcXhi,
+                          const double *bcYlo, const double *bcYhi, const double *vXlo, const double *vXhi,
+                          const double *vYlo, const double *vYhi)
+    {
+        if (bcXlo == nullptr)
+        {
+            Hal::Internal::Advection2(field, deltas, xVels, yVels, X, Y, wrapX != 0, wrapY != 0);
+            return;
+        }
+        EdgeBc bc{X, Y, bcXlo, bcXhi, bcYlo, bcYhi};
+        EdgeBc vbc{X, Y, vXlo, vXhi, vYlo, vYhi};
+        Hal::Internal::Advection2(field, deltas, xVels, yVels, X, Y, wrapX != 0, wrapY != 0, bc, vbc, vbc);
+    }
+
+    // Model ticks through Hal::PDEGrid2D: Diffusion + Advection (with a
+    // constant boundary value when hasBc) + MulAll(-decay) + Add + Update.
+    void ref_model_ticks(double *field, int32_t ticks, double rate, double xVel, double yVel, double decay, int32_t X,
+                         int32_t Y, int32_t wrapX, int32_t wrapY, int32_t hasBc, double bcValue, int32_t nSrc,
+                         const int64_t *srcIdx, const double *srcVal)
+    {
+        Hal::PDEGrid2D grid(X, Y, wrapX != 0, wrapY != 0);
+        int32_t n = X * Y;
+        for (int32_t i = 0; i < n; i++)
+        {
+            grid.Set(i, field[i]);
+        }
+        grid.Update();
+        for (int32_t t = 0; t < ticks; t++)
+        {
+            if (hasBc != 0)
+            {
+                grid.Diffusion(rate, bcValue);
+                grid.Advection(xVel, yVel, bcValue);
+            }
+            else
+            {
+                grid.Diffusion(rate);
+                grid.Advection(xVel, yVel);
+            }
+            if (decay != 0.0)
+            {
+                grid.MulAll(-decay);
+            }
+            for (int32_t s = 0; s < nSrc; s++)
+            {
+                grid.Add(static_cast<int32_t>(srcIdx[s]), srcVal[s]);
+            }
+            grid.Update();
+        }
+        for (int32_t i = 0; i < n; i++)
+        {
+            field[i] = grid.Get(i);
+        }
+    }
+
+    // CPU baseline timing of whole model ticks through Hal::PDEGrid2D
+    // (as ref_grid_ticks): `warmup` untimed then `ticks` timed ticks, one
+    // thread, std::chrono::steady_clock.  Returns the timed seconds.
+    double ref_grid_ticks_timed(double *field, int32_t warmup, int32_t ticks, int32_t adi, double rate, double decay,
+                                int32_t X, int32_t Y, int32_t wrapX, int32_t wrapY, int32_t hasBc, double bcValue,
+                                int32_t nSrc, const int64_t *srcIdx, const double *srcVal)
+    {
+        Hal::PDEGrid2D grid(X, Y, wrapX != 0, wrapY != 0);
+        int32_t n = X * Y;
+        for (int32_t i = 0; i < n; i++)
+        {
+            grid.Set(i, field[i]);
+        }
+        grid.Update();
+        auto tick = [&]()
+        {
+            if (adi != 0)
+            {
+                if (hasBc != 0)
+                {
+                    grid.DiffusionADI(rate, bcValue);
+                }
+                else
+                {
+                    grid.DiffusionADI(rate);
+                }
+            }
+            else
+            {
+                grid.Diffusion(rate);
+            }
+            if (decay != 0.0)
+            {
+                grid.MulAll(-decay);
+            }
+            for (int32_t s = 0; s < nSrc; s++)
+            {
+                grid.Add(static_cast<int32_t>(srcIdx[s]), srcVal[s]);
+            }
+            grid.Update();
+        };
+        for (int32_t t = 0; t < warmup; t++)
+        {
+            tick();
+        }
+        auto t0 = std::chrono::steady_clock::now();
+        for (int32_t t = 0; t < ticks; t++)
+        {
+            tick();
+        }
+        auto t1 = std::chrono::steady_clock::now();
+        for (int32_t i = 0; i < n; i++)
+        {
+            field[i] = grid.Get(i);
+        }
+        return std::chrono::duration<double>(t1 - t0).count();
+    }
+
+    // CPU baseline timing of `calls` Hal::CpuAgentForceBackend2D::AccumulateForces
+    // calls (after one untimed call, as examples/BackendBenchmark.cpp:75-77).
+    double ref_agent_forces_timed(const double *pos, const double *radii, int32_t N, int32_t calls, double R,
+                                  double k, int32_t X, int32_t Y, int32_t wrapX, int32_t wrapY)
+    {
+        std::vector<double> p(pos, pos + 2 * static_cast<size_t>(N));
+        std::vector<double> r(radii, radii + N);
+        std::vector<double> v;
+        Hal::CpuAgentForceBackend2D backend;
+        backend.AccumulateForces(p, r, v, R, k, X, Y, wrapX != 0, wrapY != 0);
+        auto t0 = std::chrono::steady_clock::now();
+        for (int32_t c = 0; c < calls; c++)
+        {
+            backend.AccumulateForces(p, r, v, R, k, X, Y, wrapX != 0, wrapY != 0);
+        }
+        auto t1 = std::chrono::steady_clock::now();
+        return std::chrono::duration<double>(t1 - t0).count();
+    }
+
     // Hal::CpuAgentForceBackend2D::AccumulateForces.
     void ref_agent_forces(const double *pos, const double *radii, int32_t N, double *vel, double R, double k,
                           int32_t X, int32_t Y, int32_t wrapX, int32_t wrapY)

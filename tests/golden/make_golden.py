@@ -100,6 +100,41 @@ def main():
     out["agents2_pos"] = pos2
     out["agents2_out"] = ref.agent_forces(pos2, rad2, 1.0, 1.0, 40, 24, 0, 1)
 
+    # Upwind advection, uniform and per-cell velocity
+    # (src/Pde/PDEequations.cpp:380-413, 535-573; include/Hal/Pde/PDEequations.h:413-447, 721-783),
+    # and model ticks Diffusion + Advection + MulAll + Add + Update.
+    adv = 0
+    for (X, Y) in [(1, 1), (1, 6), (5, 1), (7, 9), (16, 12)]:
+        for wx in (0, 1):
+            for wy in (0, 1):
+                for bck in (0, 2):
+                    f = rng.random(X * Y)
+                    d = rng.random(X * Y) * 0.1
+                    vel = rng.uniform(-0.25, 0.25, 2)
+                    xv = rng.uniform(-0.25, 0.25, X * Y)
+                    yv = rng.uniform(-0.25, 0.25, X * Y)
+                    if bck:
+                        bc = (rng.random(Y), rng.random(Y), rng.random(X), rng.random(X))
+                        vbc = (rng.uniform(-0.25, 0.25, Y), rng.uniform(-0.25, 0.25, Y),
+                               rng.uniform(-0.25, 0.25, X), rng.uniform(-0.25, 0.25, X))
+                    else:
+                        bc = vbc = None
+                    pre = f"adv{adv}_"
+                    out[pre + "meta"] = np.array([X, Y, wx, wy, bck], np.int64)
+                    out[pre + "field"], out[pre + "deltas_in"], out[pre + "vel"] = f, d, vel
+                    out[pre + "xv"], out[pre + "yv"] = xv, yv
+                    if bck:
+                        out[pre + "bc_xlo"], out[pre + "bc_xhi"], out[pre + "bc_ylo"], out[pre + "bc_yhi"] = bc
+                        out[pre + "vbc_xlo"], out[pre + "vbc_xhi"], out[pre + "vbc_ylo"], out[pre + "vbc_yhi"] = vbc
+                    out[pre + "uniform_out"] = ref.advect(f, d, vel[0], vel[1], X, Y, wx, wy, bc)
+                    out[pre + "cells_out"] = ref.advect_cells(f, d, xv, yv, X, Y, wx, wy, bc, vbc)
+                    adv += 1
+    out["nadv"] = np.array([adv])
+    f = rng.random(20 * 18)
+    out["ticks_model_field"] = f
+    out["ticks_model_out"] = ref.model_ticks(f, 15, 0.15, 0.2, -0.1, 2e-3, 20, 18, 1, 0, None, [5, 200], [1.0, 0.5])
+    out["ticks_model_bc_out"] = ref.model_ticks(f, 15, 0.15, -0.05, 0.25, 0.0, 20, 18, 0, 0, 0.4, [7], [0.3])
+
     np.savez_compressed(os.path.join(HERE, "hal_reference_vectors.npz"), **out)
     print("wrote", len(out), "arrays")
 

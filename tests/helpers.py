@@ -25,6 +25,20 @@ def golden_cases(g):
                    field=g[pre + "field"], deltas_in=g[pre + "deltas_in"], deltas_out=g[pre + "deltas_out"])
 
 
+def advection_cases(g):
+    """Yields dicts for the golden upwind-advection cases."""
+    for k in range(int(g["nadv"][0])):
+        pre = f"adv{k}_"
+        X, Y, wx, wy, bck = (int(v) for v in g[pre + "meta"])
+        bc = vbc = None
+        if bck:
+            bc = tuple(g[pre + s] for s in ("bc_xlo", "bc_xhi", "bc_ylo", "bc_yhi"))
+            vbc = tuple(g[pre + s] for s in ("vbc_xlo", "vbc_xhi", "vbc_ylo", "vbc_yhi"))
+        yield dict(k=k, X=X, Y=Y, wrapX=wx, wrapY=wy, bc=bc, vbc=vbc, field=g[pre + "field"],
+                   deltas_in=g[pre + "deltas_in"], vel=tuple(float(v) for v in g[pre + "vel"]), xv=g[pre + "xv"],
+                   yv=g[pre + "yv"], uniform_out=g[pre + "uniform_out"], cells_out=g[pre + "cells_out"])
+
+
 def eigen_field(nx, ny):
     x = np.arange(nx)[:, None]
     y = np.arange(ny)[None, :]

@@ -274,3 +274,64 @@ def test_cell_adi_gradient_vs_central_differences(oracle):
         assert abs(fd - gD[c]) < 1e-6 * max(1.0, abs(fd)), (c, fd, gD[c])
     fdk = (loss(D, k + h) - loss(D, k - h)) / (2 * h)
     assert abs(fdk - gk) < 1e-6 * max(1.0, abs(fdk))
+
+
+# ---- upwind advection ----------------------------------------------------------
+
+def test_advection_matches_golden_bit_for_bit(oracle, golden):
+    from helpers import advection_cases
+    n = 0
+    for c in advection_cases(golden):
+        got = oracle.advect(c["field"], c["deltas_in"], *c["vel"], c["X"], c["Y"], c["wrapX"], c["wrapY"], c["bc"])
+        np.testing.assert_array_equal(got, c["uniform_out"], err_msg=str(c["k"]))
+        got = oracle.advect_cells(c["field"], c["deltas_in"], c["xv"], c["yv"], c["X"], c["Y"], c["wrapX"],
+                                  c["wrapY"], c["bc"], c["vbc"])
+        np.testing.assert_array_equal(got, c["cells_out"], err_msg=str(c["k"]))
+        n += 1
+    assert n == 40
+
+
+def test_model_ticks_match_golden(oracle, golden):
+    f = golden["ticks_model_field"]
+    got = oracle.model_ticks(f, 15, 0.15, 0.2, -0.1, 2e-3, 20, 18, 1, 0, None, [5, 200], [1.0, 0.5])
+    np.testing.assert_array_equal(got, golden["ticks_model_out"])
+    got = oracle.model_ticks(f, 15, 0.15, -0.05, 0.25, 0.0, 20, 18, 0, 0, 0.4, [7], [0.3])
+    np.testing.assert_array_equal(got, golden["ticks_model_bc_out"])
+
+
+def test_flux_form_conserves_mass_with_walls(oracle):
+    # Zero-flux walls: every face flux leaves one cell and enters its
+    # neighbour, so the deltas sum to zero; also on a periodic grid.
+    rng = np.random.default_rng(3)
+    X, Y = 13, 17
+    f = rng.random(X * Y)
+    xv, yv = rng.uniform(-0.3, 0.3, X * Y), rng.uniform(-0.3, 0.3, X * Y)
+    for wx, wy in ((0, 0), (1, 1), (1, 0)):
+        d = oracle.advect_cells(f, np.zeros(X * Y), xv, yv, X, Y, wx, wy)
+        assert abs(d.sum()) < 1e-13 * np.abs(d).sum()
+
+
+def test_uniform_velocity_field_matches_uniform_form(oracle):
+    # A uniform velocity field reduces the flux form to u_upwind - u times
+    # the velocity: equal to the uniform-velocity form everywhere on a
+    # periodic grid and away from the walls otherwise.
+    rng = np.random.default_rng(4)
+    X, Y = 11, 9
+    f = rng.random(X * Y)
+    for vx, vy in ((0.2, -0.15), (-0.1, 0.3), (0.0, 0.25)):
+        a = oracle.advect(f, np.zeros(X * Y), vx, vy, X, Y, 1, 1)
+        b = oracle.advect_cells(f, np.zeros(X * Y), np.full(X * Y, vx), np.full(X * Y, vy), X, Y, 1, 1)
+        np.testing.assert_allclose(b, a, rtol=0, atol=1e-15)
+        a = oracle.advect(f, np.zeros(X * Y), vx, vy, X, Y, 0, 0).reshape(X, Y)
+        b = oracle.advect_cells(f, np.zeros(X * Y), np.full(X * Y, vx), np.full(X * Y, vy), X, Y, 0, 0).reshape(X, Y)
+        np.testing.assert_allclose(b[1:-1, 1:-1], a[1:-1, 1:-1], rtol=0, atol=1e-15)
+
+
+def test_upwind_shift_of_a_step(oracle):
+    # Closed form: with velocity 1 along +x on a periodic grid one explicit
+    # step moves every row to the next x exactly (u + (u[x-1] - u) * 1).
+    rng = np.random.default_rng(5)
+    X, Y = 8, 5
+    f = rng.random(X * Y)
+    d = oracle.advect(f, np.zeros(X * Y), 1.0, 0.0, X, Y, 1, 1)
+    np.testing.assert_allclose((f + d).reshape(X, Y), np.roll(f.reshape(X, Y), 1, axis=0), rtol=0, atol=1e-15)
