@@ -60,7 +60,18 @@ def traffic_from_profile():
         return json.load(f).get("hbm_bytes_per_launch")
 
 
-def bench_explicit(ctx, ticks=1000):
+def reference_lib():
+    """The compiled HAL-CPP reference (oracle/_ref), or None when not built."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Reference, available_reference
+    return Reference() if available_reference() else None
+
+
+def host_note():
+    return f"host {cpu_model()}, nproc {os.cpu_count()}, one thread"
+
+
+def bench_explicit(ctx, ticks=1000, cpu=True):
     """Explicit 256^2 (workloads.explicit_256): r = 0.2, decay 1e-3, centre
     source 1.0, periodic, 1000 ticks resident; 8 B/cell-tick algorithmic."""
     import halhip
@@ -79,15 +90,27 @@ def bench_explicit(ctx, ticks=1000):
     prof = p.profile()
     p.close()
     name, (total_ms, launches) = max(prof.items(), key=lambda kv: kv[1][0])
-    return {"workload": "explicit 256^2, r=0.2, decay 1e-3, centre source 1.0, periodic, 1000 ticks",
-            "ms_total": round(ms, 4), "us_per_tick": round(ms * 1e3 / ticks, 3),
-            "Gcell_s": round(cells * ticks / (ms * 1e-3) / 1e9, 2), "kernel": name,
-            "kernel_us_per_launch": round(total_ms / launches * 1e3, 3),
-            "ticks_per_launch": round(100 / launches, 2),
-            "note": "256 KiB field stays in L2: several ticks per launch in LDS (temporal blocking)"}
+    out = {"workload": "explicit 256^2, r=0.2, decay 1e-3, centre source 1.0, periodic, 1000 ticks",
+           "ms_total": round(ms, 4), "us_per_tick": round(ms * 1e3 / ticks, 3),
+           "Gcell_s": round(cells * ticks / (ms * 1e-3) / 1e9, 2), "kernel": name,
+           "kernel_us_per_launch": round(total_ms / launches * 1e3, 3),
+           "ticks_per_launch": round(100 / launches, 2),
+           "note": "256 KiB field stays in L2: several ticks per launch in LDS (temporal blocking)"}
+    ref = reference_lib() if cpu else None
+    if ref is not None:
+        import numpy as np
+        secs, _ = ref.grid_ticks_timed(w["field"].astype(np.float64), 1, ticks, False, w["rate"], w["decay"],
+                                       w["X"], w["Y"], True, True, None, w["src_idx"],
+                                       w["src_val"].astype(np.float64))
+        out["cpu_baseline"] = {"value": round(cells * ticks / secs / 1e9, 5), "unit": "Gcell-updates/s", "cores": 1,
+                               "kind": "reference",
+                               "sample": f"the full 1000 ticks, {secs:.2f} s: PDEGrid2D Diffusion + MulAll + Add + "
+                                         f"Update compiled -O2; {host_note()}"}
+        out["speedup_vs_cpu"] = round(out["Gcell_s"] / out["cpu_baseline"]["value"], 1)
+    return out
 
 
-def bench_agents(ctx, ticks=100):
+def bench_agents(ctx, ticks=100, cpu=True):
     """4M agents on 2048^2 (workloads.agents_uniform): bin, scan, stable
     scatter, forces, move per tick; 52 N + 20 C bytes algorithmic per tick."""
     import halhip
@@ -107,13 +130,24 @@ def bench_agents(ctx, ticks=100):
     p.close()
     per_tick = ms / ticks
     alg = 52 * n + 20 * cells
-    return {"workload": "4M agents, 2048^2 bins, r=0.5, R=1, k=1, dt=0.1, no wrap, 100 ticks",
-            "ms_per_tick": round(per_tick, 4), "Magent_ticks_s": round(n / (per_tick * 1e-3) / 1e6, 1),
-            "algorithmic_bytes_per_tick": alg, "hbm_frac": round(alg / (per_tick * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
-            "kernels_us_per_launch": {k: round(v[0] / v[1] * 1e3, 2) for k, v in prof.items()}}
+    out = {"workload": "4M agents, 2048^2 bins, r=0.5, R=1, k=1, dt=0.1, no wrap, 100 ticks",
+           "ms_per_tick": round(per_tick, 4), "Magent_ticks_s": round(n / (per_tick * 1e-3) / 1e6, 1),
+           "algorithmic_bytes_per_tick": alg, "hbm_frac": round(alg / (per_tick * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+           "kernels_us_per_launch": {k: round(v[0] / v[1] * 1e3, 2) for k, v in prof.items()}}
+    ref = reference_lib() if cpu else None
+    if ref is not None:
+        import numpy as np
+        secs = ref.agent_forces_timed(w["pos"].astype(np.float64), w["rad"].astype(np.float64), 1, w["R"], w["k"],
+                                      w["X"], w["Y"], False, False)
+        out["cpu_baseline"] = {"value": round(n / secs / 1e6, 3), "unit": "Magent-ticks/s", "cores": 1,
+                               "kind": "reference",
+                               "sample": f"one AccumulateForces call on the same 4M agents after one untimed call, "
+                                         f"{secs:.2f} s (binning + forces; the move is not timed); {host_note()}"}
+        out["speedup_vs_cpu"] = round(out["Magent_ticks_s"] / out["cpu_baseline"]["value"], 1)
+    return out
 
 
-def bench_batched(ctx, ticks=50):
+def bench_batched(ctx, ticks=50, cpu=True):
     """512 uneven wells (workloads.wells_batch): batched ticks vs the same
     kernels launched well by well."""
     import numpy as np
@@ -140,10 +174,27 @@ def bench_batched(ctx, ticks=50):
            "per_instance_Gcell_s": round(b.cells / (loop_ms * 1e-3) / 1e9, 2),
            "speedup": round(loop_ms / batched_ms, 2)}
     b.close()
+    ref = reference_lib() if cpu else None
+    if ref is not None:
+        import numpy as np
+        secs = 0.0
+        for wi in range(len(spec["xdims"])):
+            side = int(spec["xdims"][wi])
+            m = spec["src_well"] == wi
+            s, _ = ref.grid_ticks_timed(np.zeros(side * side), 0, 1, True, spec["rate"][wi], spec["decay"][wi], side,
+                                        side, False, False, None, spec["src_cell"][m],
+                                        spec["src_val"][m].astype(np.float64))
+            secs += s
+        out["cpu_baseline"] = {"value": round(out["cells"] / secs / 1e9, 5), "unit": "Gcell-updates/s", "cores": 1,
+                               "kind": "reference",
+                               "sample": f"one tick of every well in turn (MultiWellExperiment::Step shape), "
+                                         f"{secs:.2f} s: PDEGrid2D DiffusionADI + MulAll + Add + Update; "
+                                         f"{host_note()}"}
+        out["speedup_vs_cpu"] = round(out["batched_Gcell_s"] / out["cpu_baseline"]["value"], 1)
     return out
 
 
-def bench_adjoint(ctx):
+def bench_adjoint(ctx, cpu=True):
     """1024^2 per-cell-diffusivity ADI (workloads.adjoint_problem): forward
     200 ticks vs forward + checkpointed adjoint gradient."""
     import halhip
@@ -162,10 +213,26 @@ def bench_adjoint(ctx):
     p.gradient(w["u0"], target)
     grad = time.perf_counter() - t0
     p.close()
-    return {"workload": "1024^2, D = 1 + 0.5 * 8-mode mixture, decay 5e-3, 200 ticks, checkpoint every 20",
-            "forward_ms": round(fwd * 1e3, 3), "gradient_ms": round(grad * 1e3, 3),
-            "gradient_over_forward": round(grad / fwd, 3),
-            "forward_Gcell_s": round(w["X"] * w["Y"] * w["ticks"] / fwd / 1e9, 2)}
+    out = {"workload": "1024^2, D = 1 + 0.5 * 8-mode mixture, decay 5e-3, 200 ticks, checkpoint every 20",
+           "forward_ms": round(fwd * 1e3, 3), "gradient_ms": round(grad * 1e3, 3),
+           "gradient_over_forward": round(grad / fwd, 3),
+           "forward_Gcell_s": round(w["X"] * w["Y"] * w["ticks"] / fwd / 1e9, 2)}
+    if cpu:
+        # The reference has no adjoint: the FP64 restatement's adjoint is
+        # the CPU figure (10 ticks, scaled to the 200-tick problem).
+        import numpy as np
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle
+        sample = 10
+        t0 = time.perf_counter()
+        Oracle().adi_cell_gradient(w["u0"].astype(np.float64), w["D"], w["decay"], sample, w["X"], w["Y"],
+                                   target.astype(np.float64))
+        secs = (time.perf_counter() - t0) * w["ticks"] / sample
+        out["cpu_baseline"] = {"value": round(secs * 1e3, 1), "unit": "ms per gradient", "cores": 1, "kind": "port",
+                               "sample": f"FP64 CPU adjoint (forward with all intermediates stored + adjoint sweep) "
+                                         f"over {sample} ticks, scaled x{w['ticks'] // sample}; {host_note()}"}
+        out["speedup_vs_cpu"] = round(secs / grad, 1)
+    return out
 
 
 def main():
@@ -204,7 +271,7 @@ def main():
         ctx = halhip.Context(0)
         run = {"explicit": bench_explicit, "agents": bench_agents, "batched": bench_batched,
                "adjoint": bench_adjoint}[args.companion]
-        print(json.dumps({args.companion: run(ctx)}), flush=True)
+        print(json.dumps({args.companion: run(ctx, cpu=not args.no_cpu_baseline)}), flush=True)
         ctx.close()
         return
 
@@ -240,10 +307,11 @@ def main():
 
     extras = {}
     if not args.adi_only:
-        extras["explicit"] = bench_explicit(ctx)
-        extras["agents"] = bench_agents(ctx)
-        extras["batched"] = bench_batched(ctx)
-        extras["adjoint"] = bench_adjoint(ctx)
+        cpu = not args.no_cpu_baseline
+        extras["explicit"] = bench_explicit(ctx, cpu=cpu)
+        extras["agents"] = bench_agents(ctx, cpu=cpu)
+        extras["batched"] = bench_batched(ctx, cpu=cpu)
+        extras["adjoint"] = bench_adjoint(ctx, cpu=cpu)
 
     base = None
     if not args.no_cpu_baseline:

@@ -209,46 +209,6 @@ namespace
         return e;
     }
 
-    // Smallest K <= kMaxWindow such that every product of K consecutive
-    // strip multipliers, forward (A) and backward (C), that a windowed carry
-    // drops is at most 2^-28; 0 when there is none.  Windows reaching past the
-    // first or last strip drop nothing.
-    int32_t WindowSize(const std::vector<double> &A, const std::vector<double> &C)
-    {
-        const double tau = std::ldexp(1.0, -28);
-        const int32_t P = static_cast<int32_t>(A.size());
-        for (int32_t k = 1; k <= kMaxWindow; k++)
-        {
-            bool ok = true;
-            for (int32_t p = 0; p < P && ok; p++)
-            {
-                if (p - k >= 0)
-                {
-                    double prod = 1.0;
-                    for (int32_t q = p - k; q < p; q++)
-                    {
-                        prod *= std::fabs(A[static_cast<size_t>(q)]);
-                    }
-                    ok = prod <= tau;
-                }
-                if (ok && p + k <= P - 1)
-                {
-                    double prod = 1.0;
-                    for (int32_t q = p + 1; q <= p + k; q++)
-                    {
-                        prod *= std::fabs(C[static_cast<size_t>(q)]);
-                    }
-                    ok = prod <= tau;
-                }
-            }
-            if (ok)
-            {
-                return k;
-            }
-        }
-        return 0;
-    }
-
     // Constants of the LDS row solve (RowSolve) for lines of n = 64 W E
     // elements, computed in FP64.  End diagonals follow ADISolveRow
     // (ADIequations.h:225-270): 1 + r/2 (zero flux) or 1 + 3r/2 (Dirichlet).

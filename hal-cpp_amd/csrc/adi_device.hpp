@@ -5,6 +5,7 @@
 
 #include "common.hpp"
 
+#include <cmath>
 #include <vector>
 
 namespace
@@ -63,6 +64,7 @@ namespace
         const float *srcVal;
         int32_t srcBegin;
         int32_t srcEnd;
+        int32_t window; // K of windowed carries (0: carries from the scan kernel)
     };
 
     // ---- wavefront primitives ------------------------------------------------------
@@ -778,136 +780,6 @@ namespace
         }
     }
 
-    // MODE 0: reduction of the current field only (first tick of a run).
-    // MODE 1: plain tick in place, then the reduction of u' for the next tick.
-    // MODE 2: tick with decay: u' = ADI(u) - k*u (no reduction; the next tick
-    //         starts with a MODE 0 pass after point sources are added).
-    // MODE 3: tick with decay and point sources kept in registers:
-    //         u' = ADI(u) - k*u + sources, then the reduction of u' for the next
-    //         tick (batched wells).
-    // VEC: yDim is a multiple of CPT, so rows move as 16-byte vectors.
-    // One workgroup runs strip p of the grid described by `a`; u is the grid.
-    template <int CPT, int S, int MODE, bool VEC>
-    __device__ __forceinline__ void StripBody(const AdiArgs &a, float *__restrict__ u, int p)
-    {
-        __shared__ float sLeft[kMaxWaves * S];
-        __shared__ float sRight[kMaxWaves * S];
-        __shared__ float sFirst[S];
-        __shared__ float sLast[S];
-        __shared__ float sTot[kMaxWaves * S];
-        __shared__ float sAw[kMaxWaves];
-        __shared__ float sEnds[2 * S];
-        const StripLds L{sLeft, sRight, sFirst, sLast, sTot, sAw, sEnds};
-
-        const int s0 = p * S;
-        const int tid = threadIdx.x;
-        const int y0 = tid * CPT;
-
-        float v[S][CPT];
-#pragma unroll
-        for (int r = 0; r < S; r++)
-        {
-            if (s0 + r < a.X)
-            {
-                LoadRow<CPT, VEC>(u + static_cast<size_t>(s0 + r) * a.Y, y0, a.Y, v[r]);
-            }
-            else
-            {
-#pragma unroll
-                for (int c = 0; c < CPT; c++)
-                {
-                    v[r][c] = 0.0f;
-                }
-            }
-        }
-        float old[MODE == 3 ? S : 1][CPT];
-        if constexpr (MODE == 3)
-        {
-#pragma unroll
-            for (int r = 0; r < S; r++)
-            {
-#pragma unroll
-                for (int c = 0; c < CPT; c++)
-                {
-                    old[r][c] = v[r][c];
-                }
-            }
-        }
-        SweepARhs<CPT, S, VEC>(a, s0, y0, tid, v, L);
-        if constexpr (MODE == 0)
-        {
-            ReduceA<CPT, S, VEC>(a, p, s0, y0, v);
-            return;
-        }
-        else
-        {
-            float up[CPT];
-            float dn[CPT];
-            FinishA<CPT, S, VEC>(a, p, s0, y0, v, up, dn);
-            SweepBRhs<CPT, S, VEC>(a, s0, y0, tid, v, up, dn);
-            SolveB<CPT, S>(a, y0, tid, v, L);
-            if constexpr (MODE == 3)
-            {
-#pragma unroll
-                for (int r = 0; r < S; r++)
-                {
-#pragma unroll
-                    for (int c = 0; c < CPT; c++)
-                    {
-                        v[r][c] = fmaf(old[r][c], a.negDecay, v[r][c]);
-                    }
-                }
-                // Point sources of this strip (PDEGrid2D::Add), in list order.
-                for (int k = a.srcBegin; k < a.srcEnd; k++)
-                {
-                    const int sx = a.srcXY[2 * k] - s0;
-                    const int sy = a.srcXY[2 * k + 1] - y0;
-                    const float sv = a.srcVal[k];
-#pragma unroll
-                    for (int r = 0; r < S; r++)
-                    {
-#pragma unroll
-                        for (int c = 0; c < CPT; c++)
-                        {
-                            v[r][c] += (sx == r && sy == c) ? sv : 0.0f;
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < S; r++)
-            {
-                if (s0 + r < a.X)
-                {
-                    float *row = u + static_cast<size_t>(s0 + r) * a.Y;
-                    if constexpr (MODE == 2)
-                    {
-                        float old[CPT];
-                        LoadRow<CPT, VEC>(row, y0, a.Y, old);
-#pragma unroll
-                        for (int c = 0; c < CPT; c++)
-                        {
-                            v[r][c] = fmaf(old[c], a.negDecay, v[r][c]);
-                        }
-                    }
-                    StoreRow<CPT, VEC>(row, y0, a.Y, v[r]);
-                }
-            }
-            if constexpr (MODE == 1 || MODE == 3)
-            {
-                __syncthreads();
-                SweepARhs<CPT, S, VEC>(a, s0, y0, tid, v, L);
-                ReduceA<CPT, S, VEC>(a, p, s0, y0, v);
-            }
-        }
-    }
-
-    template <int CPT, int S, int MODE, bool VEC>
-    __global__ __launch_bounds__(1024) void AdiStrip(AdiArgs a, float *__restrict__ u)
-    {
-        StripBody<CPT, S, MODE, VEC>(a, u, blockIdx.x);
-    }
-
     // ---- windowed carries ------------------------------------------------------------
     //
     // Sweep A's lines are diagonally dominant, so the carry a strip receives
@@ -989,6 +861,154 @@ namespace
         {
             xb[c] = X[c];
         }
+    }
+
+    // MODE 0: reduction of the current field only (first tick of a run).
+    // MODE 1: plain tick in place, then the reduction of u' for the next tick.
+    // MODE 2: tick with decay: u' = ADI(u) - k*u (no reduction; the next tick
+    //         starts with a MODE 0 pass after point sources are added).
+    // MODE 3: tick with decay and point sources kept in registers:
+    //         u' = ADI(u) - k*u + sources, then the reduction of u' for the next
+    //         tick (batched wells).
+    // VEC: yDim is a multiple of CPT, so rows move as 16-byte vectors.
+    // One workgroup runs strip p of the grid described by `a`; u is the grid.
+    // KW > 0: the carries are composed from the neighbouring strips'
+    // reductions in a.redIn (windowed carries, K = a.window <= KW) instead of
+    // read from the scan kernel's carry arrays.
+    template <int CPT, int S, int MODE, bool VEC, int KW = 0>
+    __device__ __forceinline__ void StripBody(const AdiArgs &a, float *__restrict__ u, int p)
+    {
+        __shared__ float sLeft[kMaxWaves * S];
+        __shared__ float sRight[kMaxWaves * S];
+        __shared__ float sFirst[S];
+        __shared__ float sLast[S];
+        __shared__ float sTot[kMaxWaves * S];
+        __shared__ float sAw[kMaxWaves];
+        __shared__ float sEnds[2 * S];
+        const StripLds L{sLeft, sRight, sFirst, sLast, sTot, sAw, sEnds};
+
+        const int s0 = p * S;
+        const int tid = threadIdx.x;
+        const int y0 = tid * CPT;
+
+        // Windowed carries first: their loads are in flight while the
+        // strip's rows load, and their arithmetic waits only for them.
+        float cf[CPT];
+        float dn[CPT];
+        if constexpr (KW > 0 && MODE != 0)
+        {
+            WindowCarries<CPT, VEC, KW>(a, p, y0, a.window, cf, dn);
+        }
+
+        float v[S][CPT];
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            if (s0 + r < a.X)
+            {
+                LoadRow<CPT, VEC>(u + static_cast<size_t>(s0 + r) * a.Y, y0, a.Y, v[r]);
+            }
+            else
+            {
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    v[r][c] = 0.0f;
+                }
+            }
+        }
+        float old[MODE == 3 ? S : 1][CPT];
+        if constexpr (MODE == 3)
+        {
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    old[r][c] = v[r][c];
+                }
+            }
+        }
+        SweepARhs<CPT, S, VEC>(a, s0, y0, tid, v, L);
+        if constexpr (MODE == 0)
+        {
+            ReduceA<CPT, S, VEC>(a, p, s0, y0, v);
+            return;
+        }
+        else
+        {
+            float up[CPT];
+            if constexpr (KW > 0)
+            {
+                FinishACore<CPT, S, VEC>(a, s0, y0, v, cf, up, dn);
+            }
+            else
+            {
+                FinishA<CPT, S, VEC>(a, p, s0, y0, v, up, dn);
+            }
+            SweepBRhs<CPT, S, VEC>(a, s0, y0, tid, v, up, dn);
+            SolveB<CPT, S>(a, y0, tid, v, L);
+            if constexpr (MODE == 3)
+            {
+#pragma unroll
+                for (int r = 0; r < S; r++)
+                {
+#pragma unroll
+                    for (int c = 0; c < CPT; c++)
+                    {
+                        v[r][c] = fmaf(old[r][c], a.negDecay, v[r][c]);
+                    }
+                }
+                // Point sources of this strip (PDEGrid2D::Add), in list order.
+                for (int k = a.srcBegin; k < a.srcEnd; k++)
+                {
+                    const int sx = a.srcXY[2 * k] - s0;
+                    const int sy = a.srcXY[2 * k + 1] - y0;
+                    const float sv = a.srcVal[k];
+#pragma unroll
+                    for (int r = 0; r < S; r++)
+                    {
+#pragma unroll
+                        for (int c = 0; c < CPT; c++)
+                        {
+                            v[r][c] += (sx == r && sy == c) ? sv : 0.0f;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+                if (s0 + r < a.X)
+                {
+                    float *row = u + static_cast<size_t>(s0 + r) * a.Y;
+                    if constexpr (MODE == 2)
+                    {
+                        float old[CPT];
+                        LoadRow<CPT, VEC>(row, y0, a.Y, old);
+#pragma unroll
+                        for (int c = 0; c < CPT; c++)
+                        {
+                            v[r][c] = fmaf(old[c], a.negDecay, v[r][c]);
+                        }
+                    }
+                    StoreRow<CPT, VEC>(row, y0, a.Y, v[r]);
+                }
+            }
+            if constexpr (MODE == 1 || MODE == 3)
+            {
+                __syncthreads();
+                SweepARhs<CPT, S, VEC>(a, s0, y0, tid, v, L);
+                ReduceA<CPT, S, VEC>(a, p, s0, y0, v);
+            }
+        }
+    }
+
+    template <int CPT, int S, int MODE, bool VEC>
+    __global__ __launch_bounds__(1024) void AdiStrip(AdiArgs a, float *__restrict__ u)
+    {
+        StripBody<CPT, S, MODE, VEC>(a, u, blockIdx.x);
     }
 
     // ---- sweep B through LDS with constant coefficients -----------------------------
@@ -1687,6 +1707,46 @@ namespace
         double vEnd = 0.0;
         double inv = 0.0;
     };
+
+    // Smallest K <= kMaxWindow such that every product of K consecutive
+    // strip multipliers, forward (A) and backward (C), that a windowed carry
+    // drops is at most 2^-28; 0 when there is none.  Windows reaching past the
+    // first or last strip drop nothing.
+    int32_t WindowSize(const std::vector<double> &A, const std::vector<double> &C)
+    {
+        const double tau = std::ldexp(1.0, -28);
+        const int32_t P = static_cast<int32_t>(A.size());
+        for (int32_t k = 1; k <= kMaxWindow; k++)
+        {
+            bool ok = true;
+            for (int32_t p = 0; p < P && ok; p++)
+            {
+                if (p - k >= 0)
+                {
+                    double prod = 1.0;
+                    for (int32_t q = p - k; q < p; q++)
+                    {
+                        prod *= std::fabs(A[static_cast<size_t>(q)]);
+                    }
+                    ok = prod <= tau;
+                }
+                if (ok && p + k <= P - 1)
+                {
+                    double prod = 1.0;
+                    for (int32_t q = p + 1; q <= p + k; q++)
+                    {
+                        prod *= std::fabs(C[static_cast<size_t>(q)]);
+                    }
+                    ok = prod <= tau;
+                }
+            }
+            if (ok)
+            {
+                return k;
+            }
+        }
+        return 0;
+    }
 
     AxisFactors FactorLine(int32_t n, int32_t padded, double rate, bool wrap, bool dirichlet)
     {

@@ -41,6 +41,7 @@ namespace
         int64_t fieldOffset;
         int32_t cls; // 0: 1 col/thread; 1: 2 vec; 2: 2 scalar; 3: 4 vec; 4: 4 scalar
         int32_t pad;
+        float *redHalf[2]; // windowed carries: the two halves of this well's reductions
     };
 
     struct StripItem
@@ -57,41 +58,53 @@ namespace
         int32_t yBase;
     };
 
-    template <int S, int MODE>
+    template <int S, int MODE, int KW>
     __device__ __forceinline__ void StripByClass(int cls, const AdiArgs &a, float *u, int p)
     {
         switch (cls)
         {
             case 0:
-                StripBody<1, S, MODE, true>(a, u, p);
+                StripBody<1, S, MODE, true, KW>(a, u, p);
                 break;
             case 1:
-                StripBody<2, S, MODE, true>(a, u, p);
+                StripBody<2, S, MODE, true, KW>(a, u, p);
                 break;
             case 2:
-                StripBody<2, S, MODE, false>(a, u, p);
+                StripBody<2, S, MODE, false, KW>(a, u, p);
                 break;
             case 3:
-                StripBody<4, S, MODE, true>(a, u, p);
+                StripBody<4, S, MODE, true, KW>(a, u, p);
                 break;
             default:
-                StripBody<4, S, MODE, false>(a, u, p);
+                StripBody<4, S, MODE, false, KW>(a, u, p);
                 break;
         }
     }
 
     // MODE 0: strip reductions of the current fields; MODE 3: full tick.
-    template <int S, int MODE>
+    // KW > 0 (windowed carries): the current reductions are in redHalf[cur];
+    // a tick reads them and writes the next ones to redHalf[1 - cur], a
+    // MODE 0 pass writes redHalf[cur].
+    template <int S, int MODE, int KW>
     __global__ __launch_bounds__(kBatchThreads) void BatchStrip(const BatchWell *__restrict__ wells,
                                                                 const StripItem *__restrict__ items,
-                                                                float *__restrict__ field)
+                                                                float *__restrict__ field, int32_t cur)
     {
         const StripItem it = items[blockIdx.x];
+        if (it.well < 0)
+        {
+            return; // padding of the XCD-major work list
+        }
         const BatchWell *w = wells + it.well;
         AdiArgs a = w->args;
         a.srcBegin = it.srcBegin;
         a.srcEnd = it.srcEnd;
-        StripByClass<S, MODE>(w->cls, a, field + w->fieldOffset, it.strip);
+        if constexpr (KW > 0)
+        {
+            a.redIn = w->redHalf[cur];
+            a.red = (MODE == 0) ? w->redHalf[cur] : w->redHalf[1 - cur];
+        }
+        StripByClass<S, MODE, KW>(w->cls, a, field + w->fieldOffset, it.strip);
     }
 
     __global__ __launch_bounds__(kScanCols *kScanSegs) void BatchScan(const BatchWell *__restrict__ wells,
@@ -102,15 +115,21 @@ namespace
         ScanBody<16>(a, it.yBase, (a.P + kScanSegs - 1) / kScanSegs);
     }
 
-    using BatchStripKernel = void (*)(const BatchWell *, const StripItem *, float *);
+    using BatchStripKernel = void (*)(const BatchWell *, const StripItem *, float *, int32_t);
 
-    BatchStripKernel PickBatchStrip(int rows, int mode)
+    template <int S>
+    BatchStripKernel PickBatchStripRows(int mode, bool window)
     {
-        if (rows == 4)
+        if (window)
         {
-            return mode == 0 ? BatchStrip<4, 0> : BatchStrip<4, 3>;
+            return mode == 0 ? BatchStrip<S, 0, kMaxWindow> : BatchStrip<S, 3, kMaxWindow>;
         }
-        return mode == 0 ? BatchStrip<8, 0> : BatchStrip<8, 3>;
+        return mode == 0 ? BatchStrip<S, 0, 0> : BatchStrip<S, 3, 0>;
+    }
+
+    BatchStripKernel PickBatchStrip(int rows, int mode, bool window)
+    {
+        return rows == 4 ? PickBatchStripRows<4>(mode, window) : PickBatchStripRows<8>(mode, window);
     }
 }
 
@@ -133,16 +152,21 @@ struct halhip_batch
     DeviceArray<float> carry;
     DeviceArray<float> colInfo;
     DeviceArray<BatchWell> wells;
-    DeviceArray<StripItem> stripItems;
+    DeviceArray<StripItem> stripItems;    // natural order (per-instance ranges)
+    DeviceArray<StripItem> stripItemsXcd; // XCD-major order for batched launches
     DeviceArray<ScanItem> scanItems;
     DeviceArray<int32_t> srcXY;
     DeviceArray<float> srcValDev;
     std::vector<int32_t> wellStripBegin, wellStripCount; // per-instance launches: item ranges
     std::vector<int32_t> wellScanBegin, wellScanCount;
     int32_t nStripItems = 0;
+    int32_t nStripItemsXcd = 0;
     int32_t nScanItems = 0;
     std::string builtKey;
     bool reducedValid = false;
+    bool wantWindow = false; // windowed carries requested (halhip_batch_set_carries / autotune)
+    bool window = false;     // every well uses windowed carries (no scan launches)
+    int32_t cur = 0;     // windowed carries: half of red[] with the current reductions
     StreamClock clock;
     Profiler prof;
     GraphCache graphs;
@@ -173,6 +197,7 @@ namespace
         const double bcValue = dir ? bc->value : 0.0;
         std::string key;
         AppendKey(key, b->rows);
+        AppendKey(key, b->wantWindow);
         AppendKey(key, wrapX);
         AppendKey(key, wrapY);
         AppendKey(key, dir);
@@ -226,6 +251,7 @@ namespace
         std::vector<int64_t> colOff(static_cast<size_t>(b->nWells));
         int64_t redTotal = 0;
         int64_t colTotal = 0;
+        std::vector<int32_t> windows(static_cast<size_t>(b->nWells), 0);
         size_t srcCursor = 0;
         b->wellStripBegin.assign(b->nWells, 0);
         b->wellStripCount.assign(b->nWells, 0);
@@ -274,6 +300,7 @@ namespace
                 sC[q] = C;
                 sE[q] = acc;
             }
+            windows[w] = wrapX ? 0 : WindowSize(sA, sC);
             Offs &o = offs[w];
             o.xfw = put(fx.fw);
             o.xib = put(fx.ib);
@@ -337,10 +364,14 @@ namespace
             }
             b->wellStripCount[w] = static_cast<int32_t>(strips.size()) - b->wellStripBegin[w];
         }
+        // Windowed carries when every well allows them: two halves of red[].
+        b->window = b->wantWindow && b->nWells > 0 &&
+                    std::all_of(windows.begin(), windows.end(), [](int32_t k) { return k > 0; });
+        b->cur = 0;
         // Upload tables and fill the device pointers.
         b->tables.Allocate(b->ctx, host.size());
         Check(hipMemcpy(b->tables.get(), host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
-        b->red.Allocate(b->ctx, static_cast<size_t>(redTotal));
+        b->red.Allocate(b->ctx, static_cast<size_t>(b->window ? 2 * redTotal : redTotal));
         b->carry.Allocate(b->ctx, static_cast<size_t>(redTotal));
         b->colInfo.Allocate(b->ctx, static_cast<size_t>(colTotal));
         Check(hipMemset(b->colInfo.get(), 0, static_cast<size_t>(colTotal) * sizeof(float)), "hipMemset");
@@ -374,6 +405,9 @@ namespace
             a.bcYhi = t + o.byh;
             a.red = b->red.get() + redOff[w];
             a.carry = b->carry.get() + redOff[w];
+            a.window = b->window ? windows[w] : 0;
+            wells[w].redHalf[0] = b->red.get() + redOff[w];
+            wells[w].redHalf[1] = b->window ? b->red.get() + redTotal + redOff[w] : wells[w].redHalf[0];
             a.colInfo = b->colInfo.get() + colOff[w];
             a.srcXY = b->srcXY.get();
             a.srcVal = b->srcValDev.get();
@@ -384,6 +418,27 @@ namespace
         b->stripItems.Allocate(b->ctx, strips.size());
         Check(hipMemcpy(b->stripItems.get(), strips.data(), strips.size() * sizeof(StripItem), hipMemcpyHostToDevice),
               "hipMemcpy");
+        // Workgroups are dealt to the 8 XCDs in turn; give XCD x the
+        // contiguous run [x*per, (x+1)*per) of the work list, so that a
+        // strip's neighbours (whose reductions the windowed carries read) ran
+        // on the same XCD and are likely still in its L2.
+        {
+            const size_t n = strips.size();
+            const size_t per = (n + 7) / 8;
+            std::vector<StripItem> xcd(8 * per, StripItem{-1, 0, 0, 0});
+            for (size_t j = 0; j < xcd.size(); j++)
+            {
+                const size_t src = (j % 8) * per + j / 8;
+                if (src < n)
+                {
+                    xcd[j] = strips[src];
+                }
+            }
+            b->stripItemsXcd.Allocate(b->ctx, xcd.size());
+            Check(hipMemcpy(b->stripItemsXcd.get(), xcd.data(), xcd.size() * sizeof(StripItem), hipMemcpyHostToDevice),
+                  "hipMemcpy");
+            b->nStripItemsXcd = static_cast<int32_t>(xcd.size());
+        }
         b->scanItems.Allocate(b->ctx, scans.size());
         Check(hipMemcpy(b->scanItems.get(), scans.data(), scans.size() * sizeof(ScanItem), hipMemcpyHostToDevice),
               "hipMemcpy");
@@ -394,12 +449,17 @@ namespace
         b->graphs.Clear();
     }
 
-    void LaunchBatchStrip(halhip_batch *b, int mode, int32_t itemBegin, int32_t itemCount, const char *name)
+    // Strip kernel over items [itemBegin, itemBegin + itemCount) of the
+    // natural work list, or (all == true) over the whole XCD-major list.
+    void LaunchBatchStrip(halhip_batch *b, int mode, int32_t itemBegin, int32_t itemCount, const char *name,
+                          bool all = false)
     {
+        const StripItem *items = all ? b->stripItemsXcd.get() : b->stripItems.get() + itemBegin;
+        const int32_t count = all ? b->nStripItemsXcd : itemCount;
         b->prof.Begin(b->clock.stream, name);
-        hipLaunchKernelGGL(PickBatchStrip(b->rows, mode), dim3(static_cast<unsigned>(itemCount)), dim3(kBatchThreads),
-                           0, b->clock.stream, static_cast<const BatchWell *>(b->wells.get()),
-                           static_cast<const StripItem *>(b->stripItems.get()) + itemBegin, b->field.get());
+        hipLaunchKernelGGL(PickBatchStrip(b->rows, mode, b->window), dim3(static_cast<unsigned>(count)),
+                           dim3(kBatchThreads), 0, b->clock.stream, static_cast<const BatchWell *>(b->wells.get()),
+                           items, b->field.get(), b->cur);
         Check(hipGetLastError(), "BatchStrip launch");
         b->prof.End(b->clock.stream);
     }
@@ -414,6 +474,21 @@ namespace
         b->prof.End(b->clock.stream);
     }
 
+    // One tick of every well: the cross-strip scan then the strip kernel, or
+    // with windowed carries the strip kernel alone (flipping `cur`).
+    void BatchTick(halhip_batch *b)
+    {
+        if (!b->window)
+        {
+            LaunchBatchScan(b, 0, b->nScanItems, "batch_scan");
+        }
+        LaunchBatchStrip(b, 3, 0, b->nStripItems, "batch_strip_tick", true);
+        if (b->window)
+        {
+            b->cur = 1 - b->cur;
+        }
+    }
+
     void EnqueueBatchTicks(halhip_batch *b, int32_t ticks, int32_t wrapX, int32_t wrapY, const halhip_bc *bc)
     {
         Require(ticks >= 0, "halhip_batch_run_ticks: negative ticks");
@@ -424,14 +499,15 @@ namespace
         }
         if (!b->reducedValid)
         {
-            LaunchBatchStrip(b, 0, 0, b->nStripItems, "batch_strip_reduce");
+            LaunchBatchStrip(b, 0, 0, b->nStripItems, "batch_strip_reduce", true);
         }
-        constexpr int32_t kChunk = 10;
+        constexpr int32_t kChunk = 10; // even: a windowed chunk leaves `cur` as it found it
         int32_t done = 0;
         if (!b->prof.On() && ticks >= kChunk)
         {
             std::string key = b->builtKey;
             AppendKey(key, kChunk);
+            AppendKey(key, b->cur);
             hipGraphExec_t exec = b->graphs.Find(key);
             if (exec == nullptr)
             {
@@ -439,8 +515,7 @@ namespace
                 Check(hipStreamBeginCapture(b->clock.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
                 for (int32_t k = 0; k < kChunk; k++)
                 {
-                    LaunchBatchScan(b, 0, b->nScanItems, "batch_scan");
-                    LaunchBatchStrip(b, 3, 0, b->nStripItems, "batch_strip_tick");
+                    BatchTick(b);
                 }
                 Check(hipStreamEndCapture(b->clock.stream, &graph), "hipStreamEndCapture");
                 Check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
@@ -454,14 +529,14 @@ namespace
         }
         for (; done < ticks; done++)
         {
-            LaunchBatchScan(b, 0, b->nScanItems, "batch_scan");
-            LaunchBatchStrip(b, 3, 0, b->nStripItems, "batch_strip_tick");
+            BatchTick(b);
         }
         b->reducedValid = true;
     }
 
-    // Baseline: the same kernels launched well by well (two launches per well
-    // per tick), the shape of a per-instance loop over single-grid plans.
+    // Baseline: the same kernels launched well by well (one or two launches
+    // per well per tick), the shape of a per-instance loop over single-grid
+    // plans.
     void EnqueuePerInstanceTicks(halhip_batch *b, int32_t ticks, int32_t wrapX, int32_t wrapY, const halhip_bc *bc)
     {
         Require(ticks >= 0, "halhip_batch_run_ticks_per_instance: negative ticks");
@@ -481,8 +556,15 @@ namespace
         {
             for (int32_t w = 0; w < b->nWells; w++)
             {
-                LaunchBatchScan(b, b->wellScanBegin[w], b->wellScanCount[w], "instance_scan");
+                if (!b->window)
+                {
+                    LaunchBatchScan(b, b->wellScanBegin[w], b->wellScanCount[w], "instance_scan");
+                }
                 LaunchBatchStrip(b, 3, b->wellStripBegin[w], b->wellStripCount[w], "instance_strip_tick");
+            }
+            if (b->window)
+            {
+                b->cur = 1 - b->cur;
             }
         }
         b->reducedValid = true;
@@ -733,9 +815,11 @@ extern "C"
                         cache[k] = v;
                     }
                 }
+                // Candidates: strip rows 4 / 8, each with the scan kernel and
+                // with windowed carries; cached as rows + 100 * windowed.
                 auto it = cache.find(sig);
                 int32_t best = 0;
-                if (it != cache.end() && (it->second == 4 || it->second == 8))
+                if (it != cache.end() && (it->second % 100 == 4 || it->second % 100 == 8) && it->second / 100 <= 1)
                 {
                     best = it->second;
                 }
@@ -749,9 +833,11 @@ extern "C"
                                          hipMemcpyDeviceToDevice, b->clock.stream),
                           "hipMemcpyAsync");
                     float bestMs = 0.0f;
-                    for (int32_t rows : {4, 8})
+                    for (int32_t code : {4, 8, 104, 108})
                     {
+                        const int32_t rows = code % 100;
                         b->rows = rows;
+                        b->wantWindow = code >= 100;
                         b->builtKey.clear();
                         b->reducedValid = false;
                         EnqueueBatchTicks(b, 2, wrapX, wrapY, bc);
@@ -762,7 +848,7 @@ extern "C"
                         b->clock.Sync(&ms);
                         if (best == 0 || ms < bestMs)
                         {
-                            best = rows;
+                            best = code;
                             bestMs = ms;
                         }
                         Check(hipMemcpyAsync(b->field.get(), saved.get(), static_cast<size_t>(b->cells) * sizeof(float),
@@ -780,16 +866,40 @@ extern "C"
                         }
                     }
                 }
-                if (best != b->rows)
-                {
-                    b->rows = best;
-                    b->builtKey.clear();
-                }
+                b->rows = best % 100;
+                b->wantWindow = best >= 100;
+                b->builtKey.clear();
                 b->reducedValid = false;
                 if (chosenRows != nullptr)
                 {
-                    *chosenRows = best;
+                    *chosenRows = b->rows;
                 }
+            });
+    }
+
+    int halhip_batch_set_carries(halhip_batch *b, int32_t method)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_set_carries");
+                Require(method == HALHIP_ADI_CARRIES_SCAN || method == HALHIP_ADI_CARRIES_WINDOW,
+                        "halhip_batch_set_carries: unknown carry method");
+                Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
+                b->wantWindow = method == HALHIP_ADI_CARRIES_WINDOW;
+                b->builtKey.clear();
+                b->reducedValid = false;
+            });
+    }
+
+    int halhip_batch_get_carries(halhip_batch *b, int32_t *method)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_get_carries");
+                Require(method != nullptr, "halhip_batch_get_carries: null output");
+                *method = b->window ? HALHIP_ADI_CARRIES_WINDOW : HALHIP_ADI_CARRIES_SCAN;
             });
     }
 

@@ -110,6 +110,8 @@ _SIGS = {
     "halhip_batch_sync": [_vp, _fp],
     "halhip_batch_set_strip_rows": [_vp, _i32],
     "halhip_batch_get_strip_rows": [_vp, _ip],
+    "halhip_batch_set_carries": [_vp, _i32],
+    "halhip_batch_get_carries": [_vp, _ip],
     "halhip_batch_autotune": [_vp, _i32, _i32, _i32, ctypes.POINTER(HalBC), ctypes.c_char_p, _ip],
     "halhip_batch_set_profiling": [_vp, _i32],
     "halhip_batch_profile": [_vp, _i32, _ip, ctypes.c_char_p, _i32, _dp, _lp],
@@ -519,6 +521,14 @@ class Batch(_Plan):
         r = ctypes.c_int32()
         _check(self._fn("get_strip_rows")(self.h, ctypes.byref(r)))
         return r.value
+
+    def set_carries(self, method):
+        _check(self._fn("set_carries")(self.h, method))
+
+    def carries(self):
+        m = ctypes.c_int32()
+        _check(self._fn("get_carries")(self.h, ctypes.byref(m)))
+        return m.value
 
     def autotune(self, ticks, wrapX=False, wrapY=False, bc=None, cache_path=None):
         s, keep = make_bc(bc, 1, 1) if (bc is None or np.isscalar(bc)) else (None, ())

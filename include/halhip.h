@@ -305,11 +305,21 @@ int halhip_batch_sync(halhip_batch *batch, float *elapsedMs);
 /* Strip height of the batched kernels (4 or 8 rows). */
 int halhip_batch_set_strip_rows(halhip_batch *batch, int32_t rows);
 int halhip_batch_get_strip_rows(halhip_batch *batch, int32_t *rows);
-/* Times every strip height on `ticks` ticks (state is restored afterwards)
- * and keeps the fastest; with a cache file, results are stored per batch
- * signature (well shapes, gfx target) and reused. */
+/* Times every strip height, each with the scan kernel and with windowed
+ * carries, on `ticks` ticks (state is restored afterwards) and keeps the
+ * fastest; with a cache file, results are stored per batch signature (well
+ * shapes, gfx target) as strip rows + 100 when windowed, and reused. */
 int halhip_batch_autotune(halhip_batch *batch, int32_t ticks, int32_t wrapX, int32_t wrapY, const halhip_bc *bc,
                           const char *cachePath, int32_t *chosenRows);
+/* Carry method of the batch's sweep A across strips: HALHIP_ADI_CARRIES_SCAN
+ * (the default: a scan launch per tick) or HALHIP_ADI_CARRIES_WINDOW (each
+ * strip composes its carries from its neighbours' reductions, as in
+ * halhip_adi2d_set_methods; used only when every well allows it).
+ * halhip_batch_get_carries reports the method the last build used;
+ * halhip_batch_autotune times both.  No counterpart in the reference. */
+int halhip_batch_set_carries(halhip_batch *batch, int32_t method);
+int halhip_batch_get_carries(halhip_batch *batch, int32_t *method);
+
 int halhip_batch_set_profiling(halhip_batch *batch, int32_t on);
 int halhip_batch_profile(halhip_batch *batch, int32_t maxKernels, int32_t *nKernels, char *names, int32_t namesLen,
                          double *totalMs, int64_t *launches);

@@ -22,30 +22,37 @@ def oracle_wells(oracle, batch, fields, w, ticks, wrap=(0, 0), bc=None):
     return out
 
 
-def make(ctx, spec, rows):
+CARRIES = [halhip.ADI_CARRIES_SCAN, halhip.ADI_CARRIES_WINDOW]
+
+
+def make(ctx, spec, rows, carries=halhip.ADI_CARRIES_SCAN):
     b = halhip.Batch(ctx, spec["xdims"], spec["ydims"])
     b.set_wells(spec["rate"], spec["decay"])
     b.set_sources(spec["src_well"], spec["src_cell"], spec["src_val"])
     b.set_strip_rows(rows)
+    b.set_carries(carries)
     return b
 
 
+@pytest.mark.parametrize("carries", CARRIES)
 @pytest.mark.parametrize("rows", [4, 8])
-def test_small_batch_vs_oracle(ctx, oracle, rows):
+def test_small_batch_vs_oracle(ctx, oracle, rows, carries):
     spec = workloads.wells_batch(n_wells=12, seed=7, lo=5, hi=300, sources_per_well=6)
-    b = make(ctx, spec, rows)
+    b = make(ctx, spec, rows, carries)
     rng = np.random.default_rng(1)
     f0 = rng.random(b.cells).astype(np.float32)
     b.upload(f0)
     b.run_ticks(12)
     got = b.download()
     ref = oracle_wells(oracle, spec, f0, b, 12)
+    assert b.carries() == carries
     for k in range(len(spec["xdims"])):
         err = rel_l2(b.well(got, k), ref[k])
         assert err < 1e-5, (k, spec["xdims"][k], err)
 
 
-def test_wide_wells_and_boundaries(ctx, oracle):
+@pytest.mark.parametrize("carries", CARRIES)
+def test_wide_wells_and_boundaries(ctx, oracle, carries):
     # widths that select 1, 2 and 4 columns per thread, vector and scalar rows
     xd = np.array([3, 40, 17, 70, 9], np.int32)
     yd = np.array([256, 300, 512, 1000, 1024], np.int32)
@@ -54,20 +61,24 @@ def test_wide_wells_and_boundaries(ctx, oracle):
                 src_well=np.array([0, 1, 1, 3, 4], np.int32),
                 src_cell=np.array([5, 0, 299 + 300 * 39, 70 * 1000 - 1, 4000], np.int64),
                 src_val=np.array([1.0, 0.5, 0.25, 2.0, 1.5], np.float32))
-    b = make(ctx, spec, 8)
+    b = make(ctx, spec, 8, carries)
     f0 = np.random.default_rng(2).random(b.cells).astype(np.float32)
     for wrap, bc in (((0, 0), None), ((1, 0), 0.2), ((0, 1), None)):
         b.upload(f0)
         b.run_ticks(7, *wrap, bc=bc)
         got = b.download()
+        # a periodic x axis has no windowed carries: the batch keeps the scan
+        want = halhip.ADI_CARRIES_SCAN if wrap[0] else carries
+        assert b.carries() == want, (wrap, b.carries())
         ref = oracle_wells(oracle, spec, f0, b, 7, wrap, bc)
         for k in range(5):
             assert rel_l2(b.well(got, k), ref[k]) < 1e-5, (k, wrap, bc)
 
 
-def test_per_instance_loop_matches_batched(ctx):
+@pytest.mark.parametrize("carries", CARRIES)
+def test_per_instance_loop_matches_batched(ctx, carries):
     spec = workloads.wells_batch(n_wells=40, seed=3, lo=16, hi=512)
-    b = make(ctx, spec, 8)
+    b = make(ctx, spec, 8, carries)
     b.upload(np.zeros(b.cells, np.float32))
     b.run_ticks(25)
     batched = b.download()
@@ -86,8 +97,11 @@ def test_autotune_keeps_state_and_caches(ctx, tmp_path):
     rows = b.autotune(5, cache_path=cache)
     assert rows in (4, 8)
     np.testing.assert_array_equal(b.download(), f0)
-    assert open(cache).read().strip().endswith(str(rows))
+    code = int(open(cache).read().split()[-1])
+    assert code % 100 == rows and code // 100 in (0, 1)
     assert b.autotune(5, cache_path=cache) == rows
+    b.run_ticks(1)
+    assert b.carries() == (halhip.ADI_CARRIES_WINDOW if code >= 100 else halhip.ADI_CARRIES_SCAN)
 
 
 def test_workload_512_wells_first_ticks(ctx, oracle):
