@@ -210,6 +210,16 @@ namespace halhip
         void AppendTo(std::string &key) const;
     };
 
+    // HAL's PDEGrid2D file layout (src/Grids/PDEGridSerialization.cpp:33-61,
+    // raw little-endian as BinaryWriter writes it): int32 xDim, int32 yDim,
+    // bool wrapX, bool wrapY, int32 updateCt, xDim*yDim FP64 field values,
+    // xDim*yDim FP64 deltas.  Resident plans hold committed state, so they
+    // save zero deltas and refuse files whose deltas are not all zero.
+    void SavePdeGrid2D(const char *path, int32_t X, int32_t Y, bool wrapX, bool wrapY, int32_t updateCt,
+                       const std::vector<float> &field);
+    void LoadPdeGrid2D(const char *path, int32_t X, int32_t Y, bool &wrapX, bool &wrapY, int32_t &updateCt,
+                       std::vector<float> &field);
+
     // Number of bytes in a flat field, checked against the plan's shape.
     inline void RequireLength(int64_t n, int64_t expect, const char *caller)
     {

@@ -1008,6 +1008,60 @@ extern "C"
             });
     }
 
+    int halhip_diffusion2d_save_grid(halhip_diffusion2d *p, const char *path, int32_t wrapX, int32_t wrapY,
+                                  int32_t updateCt)
+    {
+        std::vector<float> host;
+        int rc = Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_save_grid");
+                host.resize(Cells(p));
+            });
+        if (rc != HALHIP_OK)
+        {
+            return rc;
+        }
+        rc = halhip_diffusion2d_download(p, host.data(), static_cast<int64_t>(host.size()));
+        if (rc != HALHIP_OK)
+        {
+            return rc;
+        }
+        return Guard([&] { SavePdeGrid2D(path, p->X, p->Y, wrapX != 0, wrapY != 0, updateCt, host); });
+    }
+
+    int halhip_diffusion2d_load_grid(halhip_diffusion2d *p, const char *path, int32_t *wrapX, int32_t *wrapY,
+                                  int32_t *updateCt)
+    {
+        std::vector<float> host;
+        int rc = Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_load_grid");
+                bool wx = false;
+                bool wy = false;
+                int32_t ct = 0;
+                LoadPdeGrid2D(path, p->X, p->Y, wx, wy, ct, host);
+                if (wrapX != nullptr)
+                {
+                    *wrapX = wx ? 1 : 0;
+                }
+                if (wrapY != nullptr)
+                {
+                    *wrapY = wy ? 1 : 0;
+                }
+                if (updateCt != nullptr)
+                {
+                    *updateCt = ct;
+                }
+            });
+        if (rc != HALHIP_OK)
+        {
+            return rc;
+        }
+        return halhip_diffusion2d_upload(p, host.data(), static_cast<int64_t>(host.size()));
+    }
+
     int halhip_diffusion2d_download(halhip_diffusion2d *p, float *field, int64_t n)
     {
         return Guard(

@@ -2,6 +2,7 @@
 // libhalhip.so, and the host-side seeded input generator.
 #include "common.hpp"
 
+#include <cstdint>
 #include <cstdio>
 
 namespace halhip
@@ -14,6 +15,93 @@ namespace halhip
     void SetLastError(const std::string &message)
     {
         t_lastError = message;
+    }
+
+    void SavePdeGrid2D(const char *path, int32_t X, int32_t Y, bool wrapX, bool wrapY, int32_t updateCt,
+                       const std::vector<float> &field)
+    {
+        Require(path != nullptr, "save_grid: null path");
+        std::FILE *f = std::fopen(path, "wb");
+        Require(f != nullptr, std::string("save_grid: cannot open ") + path + " for writing");
+        const unsigned char wx = wrapX ? 1 : 0;
+        const unsigned char wy = wrapY ? 1 : 0;
+        bool ok = std::fwrite(&X, sizeof(X), 1, f) == 1 && std::fwrite(&Y, sizeof(Y), 1, f) == 1 &&
+                  std::fwrite(&wx, 1, 1, f) == 1 && std::fwrite(&wy, 1, 1, f) == 1 &&
+                  std::fwrite(&updateCt, sizeof(updateCt), 1, f) == 1;
+        std::vector<double> row(static_cast<size_t>(Y));
+        const size_t n = static_cast<size_t>(X) * static_cast<size_t>(Y);
+        for (size_t base = 0; ok && base < n; base += row.size())
+        {
+            for (size_t k = 0; k < row.size(); k++)
+            {
+                row[k] = static_cast<double>(field[base + k]);
+            }
+            ok = std::fwrite(row.data(), sizeof(double), row.size(), f) == row.size();
+        }
+        std::fill(row.begin(), row.end(), 0.0);
+        for (size_t base = 0; ok && base < n; base += row.size())
+        {
+            ok = std::fwrite(row.data(), sizeof(double), row.size(), f) == row.size();
+        }
+        ok = (std::fclose(f) == 0) && ok;
+        Require(ok, std::string("save_grid: write failed: ") + path);
+    }
+
+    void LoadPdeGrid2D(const char *path, int32_t X, int32_t Y, bool &wrapX, bool &wrapY, int32_t &updateCt,
+                       std::vector<float> &field)
+    {
+        Require(path != nullptr, "load_grid: null path");
+        std::FILE *f = std::fopen(path, "rb");
+        Require(f != nullptr, std::string("load_grid: cannot open ") + path);
+        int32_t fx = 0;
+        int32_t fy = 0;
+        unsigned char wx = 0;
+        unsigned char wy = 0;
+        int32_t ct = 0;
+        bool ok = std::fread(&fx, sizeof(fx), 1, f) == 1 && std::fread(&fy, sizeof(fy), 1, f) == 1 &&
+                  std::fread(&wx, 1, 1, f) == 1 && std::fread(&wy, 1, 1, f) == 1 &&
+                  std::fread(&ct, sizeof(ct), 1, f) == 1;
+        // Internal::CheckLoadedDims (src/Grids/GridSerialization.cpp:14-32)
+        ok = ok && fx > 0 && fy > 0 && static_cast<int64_t>(fx) * fy <= INT32_MAX;
+        if (!ok)
+        {
+            std::fclose(f);
+            Fail(HALHIP_ERR_INVALID_ARGUMENT, "load_grid: failed read or invalid dimensions in saved data");
+        }
+        if (fx != X || fy != Y)
+        {
+            std::fclose(f);
+            Fail(HALHIP_ERR_INVALID_ARGUMENT, "load_grid: saved grid is " + std::to_string(fx) + "x" +
+                                                  std::to_string(fy) + ", the plan " + std::to_string(X) + "x" +
+                                                  std::to_string(Y));
+        }
+        const size_t n = static_cast<size_t>(X) * static_cast<size_t>(Y);
+        field.resize(n);
+        std::vector<double> row(static_cast<size_t>(Y));
+        for (size_t base = 0; ok && base < n; base += row.size())
+        {
+            ok = std::fread(row.data(), sizeof(double), row.size(), f) == row.size();
+            for (size_t k = 0; ok && k < row.size(); k++)
+            {
+                field[base + k] = static_cast<float>(row[k]);
+            }
+        }
+        bool zeroDeltas = true;
+        for (size_t base = 0; ok && base < n; base += row.size())
+        {
+            ok = std::fread(row.data(), sizeof(double), row.size(), f) == row.size();
+            for (size_t k = 0; ok && k < row.size(); k++)
+            {
+                zeroDeltas = zeroDeltas && row[k] == 0.0;
+            }
+        }
+        std::fclose(f);
+        Require(ok, "load_grid: failed read or invalid dimensions in saved data");
+        Require(zeroDeltas, "load_grid: the saved grid has pending deltas (saved between a stencil call and "
+                            "Update); resident plans hold committed state only");
+        wrapX = wx != 0;
+        wrapY = wy != 0;
+        updateCt = ct;
     }
 
     Profiler::~Profiler()

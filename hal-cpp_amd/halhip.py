@@ -74,6 +74,10 @@ _SIGS = {
     "halhip_diffusion2d_advect_cells_f64": [_vp, _dp, _dp, _dp, _dp, _i64, _i32, _i32, ctypes.POINTER(HalBC),
                                             ctypes.POINTER(HalBC)],
     "halhip_diffusion2d_set_advection": [_vp, _f64, _f64],
+    "halhip_diffusion2d_save_grid": [_vp, ctypes.c_char_p, _i32, _i32, _i32],
+    "halhip_diffusion2d_load_grid": [_vp, ctypes.c_char_p, _ip, _ip, _ip],
+    "halhip_adi2d_save_grid": [_vp, ctypes.c_char_p, _i32, _i32, _i32],
+    "halhip_adi2d_load_grid": [_vp, ctypes.c_char_p, _ip, _ip, _ip],
     "halhip_diffusion2d_upload": [_vp, _fp, _i64],
     "halhip_diffusion2d_download": [_vp, _fp, _i64],
     "halhip_diffusion2d_set_sources": [_vp, _i32, _lp, _fp],
@@ -283,6 +287,16 @@ class _Plan:
         _check(self._fn("profile")(self.h, 32, ctypes.byref(n), names, 4096, ms, cnt))
         keys = names.value.decode().split(",") if n.value else []
         return {k: (ms[i], cnt[i]) for i, k in enumerate(keys)}
+
+    def save_grid(self, path, wrapX, wrapY, update_ct=0):
+        """Resident field to a HAL PDEGrid2D file (zero deltas)."""
+        _check(self._fn("save_grid")(self.h, str(path).encode(), int(wrapX), int(wrapY), update_ct))
+
+    def load_grid(self, path):
+        """Resident field from a HAL PDEGrid2D file; returns (wrapX, wrapY, updateCt)."""
+        wx, wy, ct = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(self._fn("load_grid")(self.h, str(path).encode(), ctypes.byref(wx), ctypes.byref(wy), ctypes.byref(ct)))
+        return bool(wx.value), bool(wy.value), ct.value
 
     def sync(self):
         t = ctypes.c_float(0.0)

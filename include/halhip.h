@@ -142,6 +142,16 @@ int halhip_diffusion2d_set_advection(halhip_diffusion2d *plan, double xVel, doub
 
 int halhip_diffusion2d_upload(halhip_diffusion2d *plan, const float *field, int64_t n);
 int halhip_diffusion2d_download(halhip_diffusion2d *plan, float *field, int64_t n);
+/* The resident field saved as / loaded from a HAL PDEGrid2D file
+ * (Hal::SavePDEGrid2D / LoadPDEGrid2D, src/Grids/PDEGridSerialization.cpp:33-61:
+ * int32 xDim, int32 yDim, bool wrapX, bool wrapY, int32 updateCt, FP64
+ * field, FP64 deltas).  A resident plan holds committed state: saving writes
+ * zero deltas, loading refuses a file with pending deltas or other
+ * dimensions, and returns the file's wrap flags and update count. */
+int halhip_diffusion2d_save_grid(halhip_diffusion2d *plan, const char *path, int32_t wrapX, int32_t wrapY,
+                              int32_t updateCt);
+int halhip_diffusion2d_load_grid(halhip_diffusion2d *plan, const char *path, int32_t *wrapX, int32_t *wrapY,
+                              int32_t *updateCt);
 
 /* Point sources added every tick (PDEGrid2D::Add, include/Hal/Grids/PDEGrid2D.h:110-114).
  * Indices are flat cell indices; a repeated index adds twice. */
@@ -184,6 +194,16 @@ int halhip_adi2d_diffuse_f64(halhip_adi2d *plan, const double *field, double *de
 
 int halhip_adi2d_upload(halhip_adi2d *plan, const float *field, int64_t n);
 int halhip_adi2d_download(halhip_adi2d *plan, float *field, int64_t n);
+/* The resident field saved as / loaded from a HAL PDEGrid2D file
+ * (Hal::SavePDEGrid2D / LoadPDEGrid2D, src/Grids/PDEGridSerialization.cpp:33-61:
+ * int32 xDim, int32 yDim, bool wrapX, bool wrapY, int32 updateCt, FP64
+ * field, FP64 deltas).  A resident plan holds committed state: saving writes
+ * zero deltas, loading refuses a file with pending deltas or other
+ * dimensions, and returns the file's wrap flags and update count. */
+int halhip_adi2d_save_grid(halhip_adi2d *plan, const char *path, int32_t wrapX, int32_t wrapY,
+                              int32_t updateCt);
+int halhip_adi2d_load_grid(halhip_adi2d *plan, const char *path, int32_t *wrapX, int32_t *wrapY,
+                              int32_t *updateCt);
 int halhip_adi2d_set_sources(halhip_adi2d *plan, int32_t count, const int64_t *index, const float *value);
 
 /* `steps` model ticks u <- ADI(u) - decay*u + sources on the resident field

@@ -270,6 +270,28 @@ class Reference(_Lib):
           0.0 if bc_value is None else float(bc_value), len(si), _p(si, _lp), _p(sv))
         return field
 
+    def save_pdegrid2d(self, path, field, deltas, X, Y, wrapX, wrapY, update_ct):
+        """Hal::SavePDEGrid2D of a grid holding field / deltas."""
+        f = self.fn("save_pdegrid2d")
+        f.restype = ctypes.c_int32
+        f.argtypes = [ctypes.c_char_p, _dp, _dp, _i32, _i32, _i32, _i32, _i32]
+        rc = f(path.encode(), _p(_f(field)), _p(_f(deltas)), X, Y, int(wrapX), int(wrapY), update_ct)
+        assert rc == 0, path
+
+    def load_pdegrid2d(self, path, capacity):
+        """Hal::LoadPDEGrid2D: (X, Y, wrapX, wrapY, updateCt, field, deltas) or None when it throws."""
+        dims = np.zeros(5, np.int32)
+        field = np.zeros(capacity)
+        deltas = np.zeros(capacity)
+        f = self.fn("load_pdegrid2d")
+        f.restype = ctypes.c_int32
+        f.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), _dp, _dp, ctypes.c_int64]
+        if f(path.encode(), dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _p(field), _p(deltas),
+             capacity) != 0:
+            return None
+        n = int(dims[0]) * int(dims[1])
+        return (*(int(v) for v in dims), field[:n], deltas[:n])
+
     def grid_ticks_timed(self, field, warmup, ticks, adi, rate, decay, X, Y, wrapX, wrapY, bc_value=None,
                          src_idx=(), src_val=()):
         """Seconds for `ticks` PDEGrid2D model ticks of the compiled reference."""

@@ -5,6 +5,7 @@
 // oracle/_ref/libhalref.so).  It is used to write golden vectors and to
 // cross-check the FP64 restatement in hal_oracle.cpp; nothing in the product
 // links it.  Each entry point calls the reference API it names.
+#include "Hal/Grids/PDEGridSerialization.h"
 #include "Hal/Agents/CpuAgentForceBackend2D.h"
 #include "Hal/Grids/PDEGrid2D.h"
 #include "Hal/Pde/ADIequations.h"
@@ -341,6 +342,48 @@ extern "C"
         }
         auto t1 = std::chrono::steady_clock::now();
         return std::chrono::duration<double>(t1 - t0).count();
+    }
+
+    // Hal::SavePDEGrid2D of a grid holding `field` and `deltas`
+    // (src/Grids/PDEGridSerialization.cpp:33-43) to `path`.
+    int32_t ref_save_pdegrid2d(const char *path, const double *field, const double *deltas, int32_t X, int32_t Y,
+                               int32_t wrapX, int32_t wrapY, int32_t updateCt)
+    {
+        Hal::PDEGrid2D grid(X, Y, wrapX != 0, wrapY != 0);
+        const size_t n = static_cast<size_t>(X) * Y;
+        grid.SetState(std::vector<double>(field, field + n), std::vector<double>(deltas, deltas + n), updateCt);
+        Hal::BinaryWriter writer(path);
+        Hal::SavePDEGrid2D(writer, grid);
+        return writer.IsGood() ? 0 : 1;
+    }
+
+    // Hal::LoadPDEGrid2D (src/Grids/PDEGridSerialization.cpp:45-61): fills
+    // dims, flags, update count, field and deltas (n = capacity); returns 0,
+    // or 1 when the reference throws.
+    int32_t ref_load_pdegrid2d(const char *path, int32_t *dims, double *field, double *deltas, int64_t n)
+    {
+        try
+        {
+            Hal::BinaryReader reader(path);
+            Hal::PDEGrid2D grid = Hal::LoadPDEGrid2D(reader);
+            dims[0] = grid.Xdim();
+            dims[1] = grid.Ydim();
+            dims[2] = grid.IsWrapX() ? 1 : 0;
+            dims[3] = grid.IsWrapY() ? 1 : 0;
+            dims[4] = grid.UpdateCt();
+            const size_t len = static_cast<size_t>(grid.Xdim()) * grid.Ydim();
+            if (static_cast<int64_t>(len) > n)
+            {
+                return 1;
+            }
+            std::copy(grid.GetField().begin(), grid.GetField().end(), field);
+            std::copy(grid.GetDeltas().begin(), grid.GetDeltas().end(), deltas);
+            return 0;
+        }
+        catch (const std::exception &)
+        {
+            return 1;
+        }
     }
 
     // Hal::CpuAgentForceBackend2D::AccumulateForces.

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+GOLDEN_DIR = os.path.join(ROOT, "tests", "golden")
 for sub in ("oracle", "hal-cpp_amd", "tests"):
     p = os.path.join(ROOT, sub)
     if p not in sys.path:

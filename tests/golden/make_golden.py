@@ -136,6 +136,13 @@ def main():
     out["ticks_model_bc_out"] = ref.model_ticks(f, 15, 0.15, -0.05, 0.25, 0.0, 20, 18, 0, 0, 0.4, [7], [0.3])
 
     np.savez_compressed(os.path.join(HERE, "hal_reference_vectors.npz"), **out)
+
+    # PDEGrid2D files written by Hal::SavePDEGrid2D (src/Grids/PDEGridSerialization.cpp:33-43):
+    # FP32-exact field values (a resident FP32 plan saves them bit for bit),
+    # zero deltas; and the same grid with pending deltas.
+    f = rng.random(20 * 18).astype(np.float32).astype(np.float64)
+    ref.save_pdegrid2d(os.path.join(HERE, "pdegrid2d_20x18.bin"), f, np.zeros(f.size), 20, 18, 1, 0, 37)
+    ref.save_pdegrid2d(os.path.join(HERE, "pdegrid2d_20x18_pending.bin"), f, np.full(f.size, 0.5), 20, 18, 1, 0, 37)
     print("wrote", len(out), "arrays")
 
 
