@@ -20,6 +20,8 @@
 
 #include <cstdint>
 #include <memory>
+#include <string>
+#include <utility>
 #include <vector>
 
 namespace Hal
@@ -196,6 +198,95 @@ namespace Hal
             int32_t m_xDim;
             int32_t m_yDim;
             halhip_adi2d *m_plan;
+        };
+
+        // A PDEGrid2D whose field stays on the GPU, for model code written
+        // against Hal::PDEGrid2D's tick calls (include/Hal/Grids/PDEGrid2D.h):
+        // Diffusion / DiffusionADI / Advection / MulAll / Add accumulate a
+        // pending tick, and Update runs it as one fused device tick and
+        // commits, like PDEGrid2D::Update (include/Hal/Grids/PDEGrid2D.h:178-186).
+        // Ticks are queued asynchronously; GetField, Get and Save wait for them.
+        // A tick holds at most one diffusion call (explicit or ADI); Advection
+        // combines with explicit diffusion (or stands alone), and the
+        // boundary value, when given, must be the same for every call of a
+        // tick.  Other combinations throw std::invalid_argument.  Get(i)
+        // downloads the whole field.  One thread per grid object.
+        class HipPDEGrid2D
+        {
+        public:
+            HipPDEGrid2D(HipContext &context, int32_t xDim, int32_t yDim, bool wrapX = false, bool wrapY = false);
+            ~HipPDEGrid2D();
+
+            HipPDEGrid2D(const HipPDEGrid2D &other) = delete;
+            HipPDEGrid2D &operator=(const HipPDEGrid2D &other) = delete;
+
+            int32_t Xdim() const { return m_xDim; }
+            int32_t Ydim() const { return m_yDim; }
+            bool IsWrapX() const { return m_wrapX; }
+            bool IsWrapY() const { return m_wrapY; }
+            int32_t UpdateCt() const { return m_updateCt; }
+
+            // Committed field (PDEGrid2D::SetState with zero deltas / GetField).
+            void SetField(const std::vector<double> &field);
+            std::vector<double> GetField();
+            double Get(int32_t i);
+            double Get(int32_t x, int32_t y) { return Get(x * m_yDim + y); }
+
+            // PDEGrid2D::Diffusion (include/Hal/Grids/PDEGrid2D.h:289-310; rate <= 0.25)
+            void Diffusion(double diffCoef);
+            void Diffusion(double diffCoef, double boundaryValue);
+            // PDEGrid2D::DiffusionADI (include/Hal/Grids/PDEGrid2D.h:328-347)
+            void DiffusionADI(double diffRate);
+            void DiffusionADI(double diffRate, double boundaryValue);
+            // PDEGrid2D::Advection (include/Hal/Grids/PDEGrid2D.h:349-375; |xVel|+|yVel| <= 0.5)
+            void Advection(double xVel, double yVel);
+            void Advection(double xVel, double yVel, double boundaryValue);
+            // PDEGrid2D::MulAll / Add (include/Hal/Grids/PDEGrid2D.h:103-130, 212-218)
+            void MulAll(double val);
+            void Add(int32_t i, double val);
+            void Add(int32_t x, int32_t y, double val) { Add(x * m_yDim + y, val); }
+            // PDEGrid2D::Update: runs and commits the pending tick.
+            void Update();
+
+            // SavePDEGrid2D / LoadPDEGrid2D file layout (src/Grids/PDEGridSerialization.cpp:33-61).
+            void Save(const std::string &path);
+            void Load(const std::string &path);
+
+        private:
+            enum class Stencil
+            {
+                None,
+                Explicit,
+                Adi
+            };
+            void SetBoundary(bool has, double value, const char *caller);
+            void SetStencil(Stencil s, double rate, const char *caller);
+            void MoveFieldTo(Stencil s);
+
+            HipContext &m_context;
+            int32_t m_xDim;
+            int32_t m_yDim;
+            bool m_wrapX;
+            bool m_wrapY;
+            int32_t m_updateCt;
+            halhip_diffusion2d *m_explicit;
+            halhip_adi2d *m_adi;
+            Stencil m_resident; // plan holding the committed field
+            // pending tick
+            Stencil m_stencil;
+            double m_rate;
+            bool m_advect;
+            double m_xVel;
+            double m_yVel;
+            bool m_hasBc;
+            double m_bc;
+            bool m_bcSet;
+            double m_mulAll;
+            std::vector<std::pair<int64_t, float>> m_adds;
+            // sources last handed to each plan
+            std::vector<std::pair<int64_t, float>> m_explicitSources;
+            std::vector<std::pair<int64_t, float>> m_adiSources;
+            double m_explicitVel[2];
         };
 
         class HipAgentForceBackend2D : public IAgentForceBackend2D

@@ -3,7 +3,8 @@
 // reference's interface tests (tests/IDiffusionBackend2DTests.cpp,
 // tests/IAgentForceBackend2DTests.cpp) with closed-form and brute-force known
 // answers, and a model-owned tick loop (Diffuse, decay, commit as in
-// PDEGrid2D::Update).  Exit status 0 when every check passes; 77 when no GPU
+// PDEGrid2D::Update), and the same model code on the device-resident
+// Hal::Hip::HipPDEGrid2D.  Exit status 0 when every check passes; 77 when no GPU
 // is available (skip, like tests/GpuTestSupport.cpp:9-20).
 #include "Hal/Hip/HipBackends.h"
 
@@ -262,10 +263,140 @@ namespace
         }
         Check(ok, "flux-form advection with a uniform velocity field equals the uniform form");
     }
+
+    // One explicit model tick on the host in FP64, periodic on both axes:
+    // Diffusion(r) + Advection(vx > 0, vy < 0) + MulAll(m) + Add(src, a) + Update.
+    void HostExplicitTick(std::vector<double> &u, int nx, int ny, double r, double vx, double vy, double m, int src,
+                          double a)
+    {
+        std::vector<double> d(u.size(), 0.0);
+        for (int x = 0; x < nx; x++)
+        {
+            for (int y = 0; y < ny; y++)
+            {
+                const size_t i = static_cast<size_t>(x * ny + y);
+                const double c = u[i];
+                const double xp = u[static_cast<size_t>(((x + 1) % nx) * ny + y)];
+                const double xm = u[static_cast<size_t>(((x + nx - 1) % nx) * ny + y)];
+                const double yp = u[static_cast<size_t>(x * ny + (y + 1) % ny)];
+                const double ym = u[static_cast<size_t>(x * ny + (y + ny - 1) % ny)];
+                d[i] = r * (xp + xm + yp + ym - 4.0 * c) + (xm - c) * vx - (yp - c) * vy + m * c;
+            }
+        }
+        d[static_cast<size_t>(src)] += a;
+        for (size_t i = 0; i < u.size(); i++)
+        {
+            u[i] += d[i];
+        }
+    }
+
+    double RelL2(const std::vector<double> &a, const std::vector<double> &b)
+    {
+        double num = 0.0;
+        double den = 0.0;
+        for (size_t i = 0; i < a.size(); i++)
+        {
+            num += (a[i] - b[i]) * (a[i] - b[i]);
+            den += b[i] * b[i];
+        }
+        return std::sqrt(num / den);
+    }
+
+    // Model code written against PDEGrid2D's tick calls, run on the
+    // device-resident grid: explicit ticks with advection, decay and a
+    // source against the FP64 host loop; ADI ticks (and switching between
+    // the stencils mid-run) against the array adapters; misuse throws; and
+    // a save/load round trip in HAL's PDEGrid2D file layout.
+    void ResidentGrid(Hal::Hip::HipContext &ctx, const std::string &scratchDir)
+    {
+        const int nx = 24;
+        const int ny = 20;
+        const int src = 5 * ny + 9;
+        std::vector<double> u0(static_cast<size_t>(nx * ny));
+        halhip_rng_doubles(7, nx * ny, u0.data());
+        Hal::Hip::HipPDEGrid2D grid(ctx, nx, ny, true, true);
+        grid.SetField(u0);
+        std::vector<double> host = u0;
+        for (int tick = 0; tick < 40; tick++)
+        {
+            grid.Diffusion(0.15);
+            grid.Advection(0.2, -0.1);
+            grid.MulAll(-2e-3);
+            grid.Add(src, 0.5);
+            grid.Update();
+            HostExplicitTick(host, nx, ny, 0.15, 0.2, -0.1, -2e-3, src, 0.5);
+        }
+        Check(grid.UpdateCt() == 40, "resident grid counts its ticks");
+        Check(RelL2(grid.GetField(), host) < 1e-5,
+              "resident explicit + advection + decay + source ticks match FP64 host loop (rel L2 < 1e-5)");
+        Check(Near(grid.Get(3, 4), grid.GetField()[static_cast<size_t>(3 * ny + 4)], 0.0), "Get(x, y) = x*yDim + y");
+
+        // ADI, explicit and ADI ticks in a row with a Dirichlet boundary.
+        Hal::Hip::HipPDEGrid2D walls(ctx, nx, ny, false, false);
+        walls.SetField(u0);
+        Hal::Hip::HipAdiDiffusionBackend2D adi(ctx);
+        Hal::Hip::HipDiffusionBackend2D expl(ctx);
+        Hal::Hip::DirichletBoundary bc;
+        bc.value = 0.25;
+        std::vector<double> ref = u0;
+        for (int tick = 0; tick < 6; tick++)
+        {
+            std::vector<double> d(ref.size(), 0.0);
+            if (tick == 2 || tick == 3)
+            {
+                walls.Diffusion(0.2, 0.25);
+                expl.Diffuse(ref, d, 0.2, nx, ny, false, false, bc);
+            }
+            else
+            {
+                walls.DiffusionADI(3.0, 0.25);
+                adi.DiffuseADI(ref, d, 3.0, nx, ny, false, false, bc);
+            }
+            walls.MulAll(-0.01);
+            walls.Add(src, 1.0);
+            walls.Update();
+            for (size_t i = 0; i < ref.size(); i++)
+            {
+                ref[i] += d[i] - 0.01 * ref[i];
+            }
+            ref[static_cast<size_t>(src)] += 1.0;
+        }
+        Check(RelL2(walls.GetField(), ref) < 1e-5, "resident ADI / explicit ticks match the array adapters");
+
+        // A tick with no calls commits nothing but counts.
+        const std::vector<double> before = walls.GetField();
+        walls.Update();
+        Check(walls.GetField() == before && walls.UpdateCt() == 7, "empty tick keeps the field and counts");
+
+        Check(Throws([&] { walls.Diffusion(0.3); }), "unstable explicit rate throws");
+        Check(Throws([&] { walls.Advection(0.4, 0.2); }), "unstable advection velocity throws");
+        walls.Diffusion(0.1);
+        Check(Throws([&] { walls.DiffusionADI(1.0); }), "two diffusion calls in one tick throw");
+        Check(Throws([&] { walls.Advection(0.1, 0.1, 2.0); }), "differing boundary values in one tick throw");
+        Check(Throws([&] { walls.Save(scratchDir + "/pending.bin"); }), "saving with pending calls throws");
+        walls.Update();
+        walls.DiffusionADI(1.0);
+        Check(Throws([&] { walls.Advection(0.1, 0.1); }), "advection with ADI in one tick throws");
+        walls.Update();
+        Check(Throws([&] { walls.Add(nx * ny, 1.0); }), "out-of-range Add throws");
+
+        const std::string path = scratchDir + "/resident_grid.bin";
+        walls.Save(path);
+        Hal::Hip::HipPDEGrid2D back(ctx, nx, ny, false, false);
+        back.Load(path);
+        Check(back.UpdateCt() == walls.UpdateCt() && back.GetField() == walls.GetField(),
+              "resident grid save/load round trip is exact");
+        Hal::Hip::HipPDEGrid2D wrapped(ctx, nx, ny, true, false);
+        Check(Throws([&] { wrapped.Load(path); }), "loading a file with other wrap flags throws");
+        Hal::Hip::HipPDEGrid2D other(ctx, nx, ny + 1, false, false);
+        Check(Throws([&] { other.Load(path); }), "loading a file with other dimensions throws");
+    }
 }
 
-int main()
+int main(int argc, char **argv)
 {
+    // argv[1]: a writable folder for the file round trip (default /tmp)
+    const std::string scratchDir = argc > 1 ? argv[1] : "/tmp";
     std::unique_ptr<Hal::Hip::HipContext> ctx = Hal::Hip::HipContext::TryCreate(0);
     if (!ctx)
     {
@@ -331,6 +462,7 @@ int main()
         Check(out.size() == 4 && out[0] == 0.0 && out[1] == 0.0 && out[2] == 0.0 && out[3] == 0.0,
               "coincident agents give zero force");
     }
+    ResidentGrid(*ctx, scratchDir);
     std::printf("%d checks, %d failures\n", g_checks, g_failures);
     return g_failures == 0 ? 0 : 1;
 }

@@ -16,7 +16,8 @@ def test_backends_library_exports_interface_classes():
     out = subprocess.run(["nm", "-DC", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
     for sym in ("Hal::Hip::HipDiffusionBackend2D::Diffuse", "Hal::Hip::HipAdiDiffusionBackend2D::DiffuseADI",
                 "Hal::Hip::HipAgentForceBackend2D::AccumulateForces", "Hal::Internal::CheckFieldSizes",
-                "Hal::Hip::HipContext::TryCreate", "Hal::Logger::Instance"):
+                "Hal::Hip::HipContext::TryCreate", "Hal::Logger::Instance", "Hal::Hip::HipPDEGrid2D::Update",
+                "Hal::Hip::HipPDEGrid2D::DiffusionADI"):
         assert sym in out, sym
     # vtables of the interface implementations are present
     assert re.search(r"vtable for Hal::Hip::HipAdiDiffusionBackend2D", out)
@@ -31,8 +32,8 @@ def test_cpp_program_skips_cleanly_without_gpu():
 
 
 @pytest.mark.gpu
-def test_cpp_model_code_through_interfaces():
-    r = subprocess.run([BIN], capture_output=True, text=True, timeout=120)
+def test_cpp_model_code_through_interfaces(tmp_path):
+    r = subprocess.run([BIN, str(tmp_path)], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failures" in r.stdout
