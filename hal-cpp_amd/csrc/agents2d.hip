@@ -184,8 +184,10 @@ namespace
                                                              const int32_t *__restrict__ offsets,
                                                              const int32_t *__restrict__ id,
                                                              const float2 *__restrict__ pos,
-                                                             const float *__restrict__ rad, int32_t *__restrict__ nId,
-                                                             float2 *__restrict__ nPos, float *__restrict__ nRad)
+                                                             const float *__restrict__ rad,
+                                                             const float2 *__restrict__ carry,
+                                                             int32_t *__restrict__ nId, float2 *__restrict__ nPos,
+                                                             float *__restrict__ nRad, float2 *__restrict__ nCarry)
     {
         const int k = blockIdx.x * blockDim.x + threadIdx.x;
         if (k >= N)
@@ -196,13 +198,18 @@ namespace
         nId[dst] = id[k];
         nPos[dst] = pos[k];
         nRad[dst] = rad[k];
+        if (carry != nullptr)
+        {
+            nCarry[dst] = carry[k];
+        }
     }
 
-    // Sorts every cell's slice of slots by agent index, moving position and
-    // radius along (insertion sort: slices hold a handful of agents).
+    // Sorts every cell's slice of slots by agent index, moving position,
+    // radius and (when kept) carried velocity along (insertion sort: slices
+    // hold a handful of agents).
     __global__ __launch_bounds__(kThreads) void AgentOrderCells(int64_t C, const int32_t *__restrict__ offsets,
                                                                 int32_t *__restrict__ id, float2 *__restrict__ pos,
-                                                                float *__restrict__ rad)
+                                                                float *__restrict__ rad, float2 *__restrict__ carry)
     {
         const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
         if (c >= C)
@@ -220,17 +227,37 @@ namespace
             }
             const float2 kp = pos[a];
             const float kr = rad[a];
+            const float2 kc = (carry != nullptr) ? carry[a] : make_float2(0.0f, 0.0f);
             int b = a - 1;
             while (b >= s && id[b] > key)
             {
                 id[b + 1] = id[b];
                 pos[b + 1] = pos[b];
                 rad[b + 1] = rad[b];
+                if (carry != nullptr)
+                {
+                    carry[b + 1] = carry[b];
+                }
                 b--;
             }
             id[b + 1] = key;
             pos[b + 1] = kp;
             rad[b + 1] = kr;
+            if (carry != nullptr)
+            {
+                carry[b + 1] = kc;
+            }
+        }
+    }
+
+    // out[k] = v[id[k]]: agent-index order to slot order.
+    __global__ __launch_bounds__(kThreads) void AgentGather(int32_t N, const int32_t *__restrict__ id,
+                                                            const float2 *__restrict__ v, float2 *__restrict__ out)
+    {
+        const int k = blockIdx.x * blockDim.x + threadIdx.x;
+        if (k < N)
+        {
+            out[k] = v[id[k]];
         }
     }
 
@@ -364,9 +391,16 @@ namespace
 
     // p += dt*v with MoveSafePT: off-grid target -> wrap a wrapped axis, keep
     // the old coordinate on a non-wrapped one.
+    // With a carried velocity (friction or a speed cap set), the agent's
+    // velocity is carry + this tick's force delta, as SphericalAgent2D
+    // accumulates SumForces into xVel/yVel (include/Hal/Agents/SphericalAgent2D.h:100-128);
+    // its norm is capped at maxVel when maxVel > 0 (CapVelocity, :139-150),
+    // the move uses it (ForceMove, :153-156), and friction * velocity is
+    // carried to the next tick (ApplyFriction, :133-137).
     __global__ __launch_bounds__(kThreads) void AgentMove(int32_t N, int32_t X, int32_t Y, float dt, int32_t wrapX,
                                                           int32_t wrapY, float2 *__restrict__ pos,
-                                                          const float2 *__restrict__ vel)
+                                                          const float2 *__restrict__ vel, float2 *__restrict__ carry,
+                                                          float friction, float maxVel)
     {
         const int i = blockIdx.x * blockDim.x + threadIdx.x;
         if (i >= N)
@@ -374,7 +408,24 @@ namespace
             return;
         }
         const float2 o = pos[i];
-        const float2 v = vel[i];
+        float2 v = vel[i];
+        if (carry != nullptr)
+        {
+            const float2 c = carry[i];
+            v.x += c.x;
+            v.y += c.y;
+            if (maxVel > 0.0f)
+            {
+                const float normSq = v.x * v.x + v.y * v.y;
+                if (normSq > maxVel * maxVel)
+                {
+                    const float f = maxVel / sqrtf(normSq);
+                    v.x *= f;
+                    v.y *= f;
+                }
+            }
+            carry[i] = make_float2(friction * v.x, friction * v.y);
+        }
         float nx = o.x + dt * v.x;
         float ny = o.y + dt * v.y;
         const float fX = static_cast<float>(X);
@@ -428,6 +479,12 @@ struct halhip_agents2d
     DeviceArray<float> rad[2];
     int32_t cur = 0;
     DeviceArray<float2> vel;    // slot order of set `cur`
+    // Carried velocity (slot order, per buffer set), kept while friction or
+    // a speed cap is set; allocated on first use.
+    DeviceArray<float2> carry[2];
+    bool motion = false;
+    float friction = 0.0f;
+    float maxVel = 0.0f;
     DeviceArray<float2> outBuf; // agent-index order, for downloads
     DeviceArray<int32_t> counts;
     DeviceArray<int32_t> offsets;
@@ -484,10 +541,11 @@ namespace
                    static_cast<const int32_t *>(p->cellOf.get()), static_cast<const int32_t *>(p->ticketOf.get()),
                    static_cast<const int32_t *>(p->offsets.get()), static_cast<const int32_t *>(p->id[c].get()),
                    static_cast<const float2 *>(p->pos[c].get()), static_cast<const float *>(p->rad[c].get()),
-                   p->id[n].get(), p->pos[n].get(), p->rad[n].get());
+                   static_cast<const float2 *>(p->motion ? p->carry[c].get() : nullptr), p->id[n].get(),
+                   p->pos[n].get(), p->rad[n].get(), p->motion ? p->carry[n].get() : nullptr);
             Launch(p, "agents_order", AgentOrderCells, Blocks(p->C, kThreads), kThreads, p->C,
                    static_cast<const int32_t *>(p->offsets.get()), p->id[n].get(), p->pos[n].get(),
-                   p->rad[n].get());
+                   p->rad[n].get(), p->motion ? p->carry[n].get() : nullptr);
         }
         p->cur = n;
     }
@@ -513,7 +571,8 @@ namespace
         if (p->N > 0)
         {
             Launch(p, "agents_move", AgentMove, Blocks(p->N, kThreads), kThreads, p->N, p->X, p->Y, dt,
-                   wrapX ? 1 : 0, wrapY ? 1 : 0, p->pos[p->cur].get(), static_cast<const float2 *>(p->vel.get()));
+                   wrapX ? 1 : 0, wrapY ? 1 : 0, p->pos[p->cur].get(), static_cast<const float2 *>(p->vel.get()),
+                   p->motion ? p->carry[p->cur].get() : nullptr, p->friction, p->maxVel);
         }
     }
 
@@ -539,6 +598,10 @@ namespace
         hipLaunchKernelGGL(AgentIota, dim3(Blocks(nAgents, kThreads)), dim3(kThreads), 0, p->clock.stream, nAgents,
                            p->id[c].get());
         Check(hipGetLastError(), "AgentIota launch");
+        if (p->motion)
+        {
+            Check(hipMemsetAsync(p->carry[c].get(), 0, sizeof(float2) * nAgents, p->clock.stream), "hipMemsetAsync");
+        }
     }
 
     // Slot-order float2 values of the resident set to host memory in
@@ -587,6 +650,9 @@ namespace
             AppendKey(key, fdt);
             AppendKey(key, wrapX);
             AppendKey(key, wrapY);
+            AppendKey(key, p->motion ? 1 : 0);
+            AppendKey(key, p->friction);
+            AppendKey(key, p->maxVel);
             hipGraphExec_t exec = p->graphs.Find(key);
             if (exec == nullptr)
             {
@@ -730,6 +796,80 @@ extern "C"
                     return;
                 }
                 DownloadByIndex(p, p->vel.get(), velocityDeltas);
+            });
+    }
+
+    int halhip_agents2d_set_motion(halhip_agents2d *p, double friction, double maxVel)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_agents2d_set_motion");
+                Require(std::isfinite(friction) && friction >= 0.0 && friction <= 1.0,
+                        "halhip_agents2d_set_motion: friction must lie in [0, 1]");
+                Require(std::isfinite(maxVel) && maxVel >= 0.0,
+                        "halhip_agents2d_set_motion: maxVel must be finite and non-negative");
+                const bool on = friction > 0.0 || maxVel > 0.0;
+                if (on && !p->motion)
+                {
+                    const size_t cap = static_cast<size_t>(std::max(1, p->capacity));
+                    for (int b = 0; b < 2; b++)
+                    {
+                        if (p->carry[b].get() == nullptr)
+                        {
+                            p->carry[b].Allocate(p->ctx, cap);
+                        }
+                    }
+                    // velocities start at rest
+                    Check(hipMemsetAsync(p->carry[p->cur].get(), 0, sizeof(float2) * cap, p->clock.stream),
+                          "hipMemsetAsync");
+                    Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                }
+                p->motion = on;
+                p->friction = static_cast<float>(friction);
+                p->maxVel = static_cast<float>(maxVel);
+            });
+    }
+
+    int halhip_agents2d_upload_motion(halhip_agents2d *p, const float *velocities, int32_t nAgents)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_agents2d_upload_motion");
+                Require(p->motion, "halhip_agents2d_upload_motion: no friction or speed cap set");
+                Require(nAgents == p->N, "halhip_agents2d_upload_motion: count differs from the uploaded count");
+                Require(nAgents == 0 || velocities != nullptr, "halhip_agents2d_upload_motion: null array");
+                if (nAgents == 0)
+                {
+                    return;
+                }
+                // agent-index order in; slot order on the device
+                Check(hipMemcpyAsync(p->outBuf.get(), velocities, sizeof(float2) * nAgents, hipMemcpyHostToDevice,
+                                     p->clock.stream),
+                      "hipMemcpyAsync");
+                hipLaunchKernelGGL(AgentGather, dim3(Blocks(nAgents, kThreads)), dim3(kThreads), 0, p->clock.stream,
+                                   nAgents, static_cast<const int32_t *>(p->id[p->cur].get()),
+                                   static_cast<const float2 *>(p->outBuf.get()), p->carry[p->cur].get());
+                Check(hipGetLastError(), "AgentGather launch");
+                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+            });
+    }
+
+    int halhip_agents2d_download_motion(halhip_agents2d *p, float *velocities, int32_t nAgents)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_agents2d_download_motion");
+                Require(p->motion, "halhip_agents2d_download_motion: no friction or speed cap set");
+                Require(nAgents == p->N, "halhip_agents2d_download_motion: count differs from the uploaded count");
+                Require(nAgents == 0 || velocities != nullptr, "halhip_agents2d_download_motion: null array");
+                if (nAgents == 0)
+                {
+                    return;
+                }
+                DownloadByIndex(p, p->carry[p->cur].get(), velocities);
             });
     }
 

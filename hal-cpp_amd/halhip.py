@@ -133,6 +133,9 @@ _SIGS = {
     "halhip_agents2d_download_positions": [_vp, _fp, _i32],
     "halhip_agents2d_download_velocities": [_vp, _fp, _i32],
     "halhip_agents2d_download_bins": [_vp, _ip, _ip],
+    "halhip_agents2d_set_motion": [_vp, _f64, _f64],
+    "halhip_agents2d_upload_motion": [_vp, _fp, _i32],
+    "halhip_agents2d_download_motion": [_vp, _fp, _i32],
     "halhip_agents2d_run_ticks": [_vp, _i32, _f64, _f64, _f64, _i32, _i32],
     "halhip_agents2d_run_ticks_async": [_vp, _i32, _f64, _f64, _f64, _i32, _i32],
     "halhip_agents2d_sync": [_vp, _fp],
@@ -472,6 +475,21 @@ class Agents2D(_Plan):
     def velocities(self):
         out = np.empty(2 * self.n, np.float32)
         _check(self._fn("download_velocities")(self.h, out.ctypes.data_as(_fp), self.n))
+        return out
+
+    def set_motion(self, friction, max_vel=0.0):
+        """Carry friction * velocity across ticks and cap its norm at max_vel
+        (0: no cap), as SphericalAgent2D::ApplyFriction / CapVelocity."""
+        _check(self._fn("set_motion")(self.h, friction, max_vel))
+
+    def upload_motion(self, velocities):
+        velocities = _f32(velocities)
+        assert velocities.size == 2 * self.n
+        _check(self._fn("upload_motion")(self.h, velocities.ctypes.data_as(_fp), self.n))
+
+    def motion(self):
+        out = np.empty(2 * self.n, np.float32)
+        _check(self._fn("download_motion")(self.h, out.ctypes.data_as(_fp), self.n))
         return out
 
     def bins(self):

@@ -275,6 +275,19 @@ int halhip_agents2d_download_velocities(halhip_agents2d *plan, float *velocityDe
  * ascending index within a bin). */
 int halhip_agents2d_download_bins(halhip_agents2d *plan, int32_t *offsets, int32_t *bucket);
 
+/* Velocity carried across ticks, as SphericalAgent2D keeps xVel/yVel
+ * (include/Hal/Agents/SphericalAgent2D.h:100-156): each tick the velocity
+ * is carry + the tick's force delta, its norm is capped at maxVel
+ * (CapVelocity, :139-150; maxVel == 0 means no cap), the move uses it
+ * (ForceMove), and friction * velocity is carried (ApplyFriction, :133-137).
+ * friction in [0, 1].  friction == 0 with no cap (the default) carries
+ * nothing: the move uses the force delta alone.  Enabling starts every
+ * agent at rest, and so does an upload.  upload/download_motion move the
+ * carried velocities (2*nAgents, agent-index order) for checkpoints. */
+int halhip_agents2d_set_motion(halhip_agents2d *plan, double friction, double maxVel);
+int halhip_agents2d_upload_motion(halhip_agents2d *plan, const float *velocities, int32_t nAgents);
+int halhip_agents2d_download_motion(halhip_agents2d *plan, float *velocities, int32_t nAgents);
+
 /* `ticks` ticks of bin -> scan -> stable scatter -> force -> move.  The move
  * is p += dt*v with AgentPT2D::MoveSafePT semantics
  * (include/Hal/Agents/AgentPT2D.h:157-186): an axis whose target leaves a

@@ -17,7 +17,10 @@
 //                           Crank-Nicolson eigenmode product; Dirichlet fixed
 //                           point and linear ramp; scipy banded solve
 //   oracle_agent_*          golden vectors; brute-force all-pairs sum; sorted
-//                           (cell, index) order for bins
+//                           (cell, index) order for bins; carried-velocity
+//                           ticks: the reference's friction/cap unit-test
+//                           values, geometric-series closed form, force +
+//                           carry sum, and "off == plain ticks"
 //   oracle_advect*          golden vectors (uniform and per-cell velocity,
 //                           zero flux, periodic, Dirichlet); mass invariance
 //                           of the flux form; uniform velocity fields give
@@ -839,6 +842,40 @@ extern "C"
             }
             pos[2 * static_cast<size_t>(i)] = nx;
             pos[2 * static_cast<size_t>(i) + 1] = ny;
+        }
+    }
+
+    // `ticks` ticks of forces + move with a carried velocity `carry` (2N,
+    // updated in place), as SphericalAgent2D keeps xVel/yVel
+    // (include/Hal/Agents/SphericalAgent2D.h:100-156): v = carry + force
+    // delta (SumForces accumulates into the velocity), |v| capped at maxVel
+    // when maxVel > 0 (CapVelocity, :139-150), move by dt*v (ForceMove with
+    // MoveSafePT), carry = friction*v (ApplyFriction, :133-137).
+    void oracle_agent_ticks_motion(double *pos, double *carry, const double *radii, int32_t N, int32_t ticks,
+                                   double R, double k, double dt, double friction, double maxVel, int32_t X,
+                                   int32_t Y, int32_t wrapX, int32_t wrapY)
+    {
+        std::vector<double> vel(2 * static_cast<size_t>(N > 0 ? N : 1));
+        for (int32_t t = 0; t < ticks; t++)
+        {
+            oracle_agent_forces(pos, radii, N, vel.data(), R, k, X, Y, wrapX, wrapY);
+            for (int32_t i = 0; i < N; i++)
+            {
+                double vx = vel[2 * static_cast<size_t>(i)] + carry[2 * static_cast<size_t>(i)];
+                double vy = vel[2 * static_cast<size_t>(i) + 1] + carry[2 * static_cast<size_t>(i) + 1];
+                const double normSq = vx * vx + vy * vy;
+                if (maxVel > 0.0 && normSq > maxVel * maxVel)
+                {
+                    const double f = maxVel / std::sqrt(normSq);
+                    vx *= f;
+                    vy *= f;
+                }
+                vel[2 * static_cast<size_t>(i)] = vx;
+                vel[2 * static_cast<size_t>(i) + 1] = vy;
+                carry[2 * static_cast<size_t>(i)] = friction * vx;
+                carry[2 * static_cast<size_t>(i) + 1] = friction * vy;
+            }
+            oracle_agent_move(pos, vel.data(), N, dt, X, Y, wrapX, wrapY);
         }
     }
 

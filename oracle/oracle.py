@@ -201,6 +201,17 @@ class Oracle(_Lib):
         f(_p(pos), _p(vel), pos.shape[0] // 2, dt, X, Y, int(wrapX), int(wrapY))
         return pos
 
+    def agent_ticks_motion(self, pos, carry, radii, ticks, R, k, dt, friction, max_vel, X, Y, wrapX, wrapY):
+        """Returns (positions, carried velocities) after `ticks` ticks."""
+        pos = _f(pos).copy()
+        carry = _f(carry).copy()
+        radii = _f(radii)
+        f = self.fn("agent_ticks_motion")
+        f.argtypes = [_dp, _dp, _dp, _i32, _i32, _f64, _f64, _f64, _f64, _f64, _i32, _i32, _i32, _i32]
+        f(_p(pos), _p(carry), _p(radii), radii.shape[0], ticks, R, k, dt, friction, max_vel, X, Y, int(wrapX),
+          int(wrapY))
+        return pos, carry
+
     def agent_ticks(self, pos, radii, ticks, R, k, dt, X, Y, wrapX, wrapY):
         pos = _f(pos).copy()
         radii = _f(radii)

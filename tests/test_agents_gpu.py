@@ -132,3 +132,87 @@ def test_workload_4m_first_tick(ctx, oracle):
     err = rel_l2(v, ref)
     print(f"4M agents forces: rel L2 vs FP64 = {err:.3e}")
     assert err < 1e-5
+
+
+# ---- carried velocity: friction and speed cap ------------------------------
+
+def _wrapped_diff(got, ref, X, Y, wrap):
+    d = got.astype(np.float64) - ref
+    if wrap[0]:
+        d[0::2] = (d[0::2] + X / 2) % X - X / 2
+    if wrap[1]:
+        d[1::2] = (d[1::2] + Y / 2) % Y - Y / 2
+    return d
+
+
+@pytest.mark.parametrize("wrap", [(0, 0), (1, 1)])
+def test_motion_ticks_vs_oracle(ctx, oracle, wrap):
+    # SphericalAgent2D-style ticks: carry + force -> cap -> move -> friction.
+    rng = np.random.default_rng(31)
+    X, Y, n = 64, 48, 2500
+    pos = (rng.random(2 * n) * np.tile([X, Y], n)).astype(np.float32)
+    rad = np.full(n, 0.5, np.float32)
+    plan = halhip.Agents2D(ctx, X, Y, n)
+    plan.upload(pos, rad)
+    plan.set_motion(0.7, 1.5)
+    plan.run_ticks(1, 1.0, 1.0, 0.1, *wrap)
+    p1, c1 = oracle.agent_ticks_motion(pos, np.zeros(2 * n), rad, 1, 1.0, 1.0, 0.1, 0.7, 1.5, X, Y, *wrap)
+    np.testing.assert_allclose(plan.positions(), p1, rtol=0, atol=2e-5)
+    np.testing.assert_allclose(plan.motion(), c1, rtol=0, atol=2e-5)
+    plan.run_ticks(19, 1.0, 1.0, 0.1, *wrap)    # a 10-tick graph chunk and single ticks
+    p, c = oracle.agent_ticks_motion(pos, np.zeros(2 * n), rad, 20, 1.0, 1.0, 0.1, 0.7, 1.5, X, Y, *wrap)
+    assert np.abs(_wrapped_diff(plan.positions(), p, X, Y, wrap)).max() < 1e-3
+    assert np.abs(plan.motion() - c).max() < 1e-3
+    # the cap holds: |carry| <= friction * maxVel
+    cm = plan.motion().reshape(-1, 2)
+    assert np.sqrt((cm.astype(np.float64) ** 2).sum(1)).max() <= 0.7 * 1.5 * (1 + 1e-6)
+
+
+def test_motion_closed_form_and_checkpoint(ctx, oracle):
+    # Lone agents: ApplyFriction(0.5) / CapVelocity(2) on (3, 4)
+    # (tests/SphericalAgentTests.cpp:88-102), one tick with dt = 1.
+    plan = halhip.Agents2D(ctx, 40, 40, 2)
+    plan.upload(np.array([10.0, 10.0, 30.0, 30.0], np.float32), np.array([0.5, 0.5], np.float32))
+    plan.set_motion(0.5, 2.0)
+    plan.upload_motion(np.array([3.0, 4.0, -0.3, 0.4], np.float32))
+    plan.run_ticks(1, 1.0, 1.0, 1.0, 0, 0)
+    np.testing.assert_allclose(plan.positions(), [11.2, 11.6, 29.7, 30.4], rtol=0, atol=1e-5)
+    np.testing.assert_allclose(plan.motion(), [0.6, 0.8, -0.15, 0.2], rtol=0, atol=1e-6)
+
+    # Checkpoint (positions, radii, carried velocities) and resume on a new
+    # plan: bins are stable by agent index, so the resumed run is bit-exact.
+    rng = np.random.default_rng(32)
+    X, Y, n = 40, 30, 900
+    pos = (rng.random(2 * n) * np.tile([X, Y], n)).astype(np.float32)
+    rad = np.full(n, 0.5, np.float32)
+    a = halhip.Agents2D(ctx, X, Y, n)
+    a.upload(pos, rad)
+    a.set_motion(0.8, 0.0)
+    a.run_ticks(7, 1.0, 1.0, 0.2, 0, 1)
+    ckpt_pos, ckpt_vel = a.positions(), a.motion()
+    a.run_ticks(13, 1.0, 1.0, 0.2, 0, 1)
+    b = halhip.Agents2D(ctx, X, Y, n)
+    b.upload(ckpt_pos, rad)
+    b.set_motion(0.8, 0.0)
+    b.upload_motion(ckpt_vel)
+    b.run_ticks(13, 1.0, 1.0, 0.2, 0, 1)
+    np.testing.assert_array_equal(b.positions(), a.positions())
+    np.testing.assert_array_equal(b.motion(), a.motion())
+
+
+def test_motion_misuse_and_reset(ctx):
+    plan = halhip.Agents2D(ctx, 8, 8, 4)
+    plan.upload(np.array([1.0, 1.0, 5.0, 5.0], np.float32), np.array([0.5, 0.5], np.float32))
+    with pytest.raises(halhip.InvalidArgument):
+        plan.upload_motion(np.zeros(4, np.float32))      # no friction or cap set
+    with pytest.raises(halhip.InvalidArgument):
+        plan.set_motion(1.5, 0.0)
+    with pytest.raises(halhip.InvalidArgument):
+        plan.set_motion(0.5, -1.0)
+    plan.set_motion(0.9, 0.0)
+    plan.upload_motion(np.array([0.5, 0.0, 0.0, 0.5], np.float32))
+    plan.upload(np.array([1.0, 1.0, 5.0, 5.0], np.float32), np.array([0.5, 0.5], np.float32))
+    assert np.all(plan.motion() == 0.0)                  # an upload starts agents at rest
+    plan.set_motion(0.0, 0.0)                            # off: plain ticks again
+    plan.run_ticks(2, 1.0, 1.0, 0.1, 0, 0)
+    np.testing.assert_array_equal(plan.positions(), np.array([1.0, 1.0, 5.0, 5.0], np.float32))

@@ -231,6 +231,49 @@ def test_agent_move_safe_semantics(oracle):
     np.testing.assert_allclose(got, [1.1, 1.1, 0.5, 2.1, 5.0, 9.95])
 
 
+def test_agent_motion_off_equals_plain_ticks(oracle):
+    rng = np.random.default_rng(21)
+    X, Y, n = 12, 10, 150
+    pos = rng.random(2 * n) * np.tile([X, Y], n)
+    rad = np.full(n, 0.5)
+    got, carry = oracle.agent_ticks_motion(pos, np.zeros(2 * n), rad, 5, 1.0, 1.0, 0.1, 0.0, 0.0, X, Y, 1, 0)
+    np.testing.assert_array_equal(got, oracle.agent_ticks(pos, rad, 5, 1.0, 1.0, 0.1, X, Y, 1, 0))
+    assert np.all(carry == 0.0)
+
+
+def test_agent_motion_friction_and_cap_closed_forms(oracle):
+    # tests/SphericalAgentTests.cpp:88-102: ApplyFriction(0.5) on (3, 4) gives
+    # (1.5, 2); CapVelocity(2) on (3, 4) gives norm 2.  A lone agent feels no
+    # force, so its tick is exactly carry -> cap -> move -> friction.
+    pos = np.array([10.0, 10.0])
+    rad = np.array([0.5])
+    p1, c1 = oracle.agent_ticks_motion(pos, [3.0, 4.0], rad, 1, 1.0, 1.0, 1.0, 0.5, 0.0, 40, 40, 0, 0)
+    np.testing.assert_allclose(c1, [1.5, 2.0], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(p1, [13.0, 14.0], rtol=0, atol=1e-13)
+    # cap before the move, friction after it
+    p2, c2 = oracle.agent_ticks_motion(pos, [3.0, 4.0], rad, 1, 1.0, 1.0, 1.0, 0.5, 2.0, 40, 40, 0, 0)
+    np.testing.assert_allclose(p2, [11.2, 11.6], rtol=0, atol=1e-13)
+    np.testing.assert_allclose(c2, [0.6, 0.8], rtol=0, atol=1e-15)
+    # geometric series over t ticks: moved dt*v0*(1-f^t)/(1-f), carry f^t*v0
+    f, dt, t = 0.8, 0.25, 7
+    p3, c3 = oracle.agent_ticks_motion(pos, [2.0, -1.0], rad, t, 1.0, 1.0, dt, f, 0.0, 40, 40, 0, 0)
+    s = (1 - f ** t) / (1 - f)
+    np.testing.assert_allclose(p3, [10.0 + dt * 2.0 * s, 10.0 - dt * 1.0 * s], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(c3, [2.0 * f ** t, -1.0 * f ** t], rtol=1e-13, atol=0)
+
+
+def test_agent_motion_adds_forces_to_carried_velocity(oracle):
+    # SumForces accumulates into the carried velocity (SphericalAgent2D.h:100-128).
+    pos = np.array([5.0, 5.0, 5.6, 5.1])
+    rad = np.array([0.5, 0.5])
+    v0 = np.array([0.3, -0.2, 0.0, 0.4])
+    F = oracle.agent_forces(pos, rad, 1.0, 1.0, 16, 16, 0, 0)
+    assert np.abs(F).max() > 0.1
+    p1, c1 = oracle.agent_ticks_motion(pos, v0, rad, 1, 1.0, 1.0, 0.5, 1.0, 0.0, 16, 16, 0, 0)
+    np.testing.assert_allclose(c1, F + v0, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(p1, pos + 0.5 * (F + v0), rtol=0, atol=1e-14)
+
+
 # ---- per-cell-diffusivity ADI and adjoint (no reference counterpart) ------
 
 def test_cell_adi_uniform_equals_reference_adi(oracle):
