@@ -401,23 +401,35 @@ extern "C"
 
     // Hal::Rng(seed).NextDouble() stream: splitmix64 seeding and xoshiro256**
     // (Blackman & Vigna), top 53 bits scaled by 2^-53; src/Rng.cpp:13-101.
-    int halhip_rng_doubles(uint64_t seed, int64_t n, double *out)
+    int halhip_rng_seed_state(uint64_t seed, uint64_t *state)
     {
         return Guard(
             [&]
             {
-                Require(n >= 0 && (n == 0 || out != nullptr), "halhip_rng_doubles: bad output");
+                Require(state != nullptr, "halhip_rng_seed_state: null state");
+                // splitmix64 over the seed fills the four state words
                 uint64_t st = seed;
-                uint64_t s[4];
-                for (auto &w : s)
+                for (int w = 0; w < 4; w++)
                 {
                     st += 0x9E3779B97F4A7C15ULL;
                     uint64_t z = st;
                     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
                     z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-                    w = z ^ (z >> 31);
+                    state[w] = z ^ (z >> 31);
                 }
+            });
+    }
+
+    int halhip_rng_next_doubles(uint64_t *state, int64_t n, double *out)
+    {
+        return Guard(
+            [&]
+            {
+                Require(state != nullptr, "halhip_rng_next_doubles: null state");
+                Require(n >= 0 && (n == 0 || out != nullptr), "halhip_rng_next_doubles: bad output");
+                uint64_t s[4] = {state[0], state[1], state[2], state[3]};
                 auto rotl = [](uint64_t x, int k) { return (x << k) | (x >> (64 - k)); };
+                // xoshiro256** step; the top 53 bits scaled to [0, 1)
                 for (int64_t i = 0; i < n; i++)
                 {
                     uint64_t r = rotl(s[1] * 5, 7) * 9;
@@ -430,6 +442,17 @@ extern "C"
                     s[3] = rotl(s[3], 45);
                     out[i] = static_cast<double>(r >> 11) * (1.0 / 9007199254740992.0);
                 }
+                for (int w = 0; w < 4; w++)
+                {
+                    state[w] = s[w];
+                }
             });
+    }
+
+    int halhip_rng_doubles(uint64_t seed, int64_t n, double *out)
+    {
+        uint64_t state[4];
+        const int rc = halhip_rng_seed_state(seed, state);
+        return rc != HALHIP_OK ? rc : halhip_rng_next_doubles(state, n, out);
     }
 }

@@ -66,6 +66,8 @@ _SIGS = {
     "halhip_context_synchronize": [_vp],
     "halhip_calibration_copy": [_vp, _i64, _i32, _fp],
     "halhip_rng_doubles": [ctypes.c_uint64, _i64, _dp],
+    "halhip_rng_seed_state": [ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)],
+    "halhip_rng_next_doubles": [ctypes.POINTER(ctypes.c_uint64), _i64, _dp],
     "halhip_diffusion2d_create": [_vp, _i32, _i32, ctypes.POINTER(_vp)],
     "halhip_diffusion2d_destroy": [_vp],
     "halhip_diffusion2d_diffuse_f64": [_vp, _dp, _dp, _i64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
@@ -192,6 +194,29 @@ def rng_doubles(seed, n):
     out = np.empty(n, np.float64)
     _check(load().halhip_rng_doubles(seed, n, out.ctypes.data_as(_dp)))
     return out
+
+
+class Rng:
+    """Hal::Rng as an explicit state: `state` (4 uint64) is byte for byte
+    what Hal::Rng::Save writes (src/Rng.cpp:122-130)."""
+
+    def __init__(self, seed=None, state=None):
+        self.state = np.zeros(4, np.uint64)
+        if state is not None:
+            st = np.frombuffer(bytes(state), np.uint64) if isinstance(state, (bytes, bytearray)) else state
+            self.state[:] = np.asarray(st, np.uint64)
+        else:
+            _check(load().halhip_rng_seed_state(seed, self.state.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+
+    def doubles(self, n):
+        out = np.empty(n, np.float64)
+        _check(load().halhip_rng_next_doubles(self.state.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), n,
+                                              out.ctypes.data_as(_dp)))
+        return out
+
+    def save_bytes(self):
+        """The 32 bytes of Hal::Rng::Save."""
+        return self.state.tobytes()
 
 
 def _f64(a):

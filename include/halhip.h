@@ -91,6 +91,15 @@ int halhip_context_fail_next_allocs(halhip_context *ctx, int32_t count);
 /* n draws of Rng(seed).NextDouble(). */
 int halhip_rng_doubles(uint64_t seed, int64_t n, double *out);
 
+/* The generator as an explicit state of four uint64 words, for checkpoints:
+ * the words in order are the 32 bytes Hal::Rng::Save writes and
+ * Hal::Rng::Load reads (src/Rng.cpp:122-130, raw native-endian
+ * BinaryWriter::WriteArray), so a state saved by HAL resumes here and back.
+ * seed_state is Hal::Rng(seed)'s state (src/Rng.cpp:37-45); next_doubles
+ * draws n NextDouble values (src/Rng.cpp:56-69, 91-95) and advances state. */
+int halhip_rng_seed_state(uint64_t seed, uint64_t *state);
+int halhip_rng_next_doubles(uint64_t *state, int64_t n, double *out);
+
 /* ---- explicit diffusion -------------------------------------------------- */
 
 /* Replaces Hal::Gpu::DiffusionKernel2D / GpuDiffusionBackend2D

@@ -61,6 +61,21 @@ class _Lib:
         f(seed, n, _p(out))
         return out
 
+    def rng_save(self, seed, skip, path):
+        """Hal::Rng(seed) after `skip` draws, written by Rng::Save."""
+        f = self.fn("rng_save")
+        f.restype = ctypes.c_int32
+        f.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_char_p]
+        assert f(seed, skip, path.encode()) == 0, path
+
+    def rng_load_doubles(self, path, n):
+        """n draws of a Hal::Rng restored by Rng::Load from `path`."""
+        out = np.empty(n, np.float64)
+        f = self.fn("rng_load_doubles")
+        f.argtypes = [ctypes.c_char_p, ctypes.c_int64, _dp]
+        f(path.encode(), n, _p(out))
+        return out
+
     def rng_u64(self, seed, n):
         out = np.empty(n, np.uint64)
         f = self.fn("rng_u64")

@@ -7,6 +7,8 @@
 // links it.  Each entry point calls the reference API it names.
 #include "Hal/Grids/PDEGridSerialization.h"
 #include "Hal/Agents/CpuAgentForceBackend2D.h"
+#include "Hal/IO/BinaryReader.h"
+#include "Hal/IO/BinaryWriter.h"
 #include "Hal/Grids/PDEGrid2D.h"
 #include "Hal/Pde/ADIequations.h"
 #include "Hal/Pde/CpuAdiDiffusionBackend2D.h"
@@ -65,6 +67,33 @@ extern "C"
         for (int64_t i = 0; i < n; i++)
         {
             out[i] = rng.NextUint64();
+        }
+    }
+
+    // Hal::Rng(seed) after `skip` NextDouble draws, saved with Rng::Save
+    // (src/Rng.cpp:122-125) to `path`.
+    int32_t ref_rng_save(uint64_t seed, int64_t skip, const char *path)
+    {
+        Hal::Rng rng(seed);
+        for (int64_t i = 0; i < skip; i++)
+        {
+            (void)rng.NextDouble();
+        }
+        Hal::BinaryWriter writer(path);
+        rng.Save(writer);
+        return writer.IsGood() ? 0 : 1;
+    }
+
+    // A Hal::Rng restored with Rng::Load (src/Rng.cpp:127-130) from `path`,
+    // then n NextDouble draws.
+    void ref_rng_load_doubles(const char *path, int64_t n, double *out)
+    {
+        Hal::Rng rng(0);
+        Hal::BinaryReader reader(path);
+        rng.Load(reader);
+        for (int64_t i = 0; i < n; i++)
+        {
+            out[i] = rng.NextDouble();
         }
     }
 

@@ -26,6 +26,12 @@ def main():
     out["rng42_doubles"] = ref.rng_doubles(42, 64)
     out["rng1234_u64"] = ref.rng_u64(1234, 16)
     out["rng7_doubles"] = ref.rng_doubles(7, 16)
+    # Hal::Rng::Save of Rng(42) after 10 draws (src/Rng.cpp:122-125): 32 raw bytes
+    state_path = os.path.join(HERE, "_rng_state.tmp")
+    ref.rng_save(42, 10, state_path)
+    with open(state_path, "rb") as fh:
+        out["rng42_state_after10"] = np.frombuffer(fh.read(), np.uint8).copy()
+    os.remove(state_path)
 
     rng = np.random.default_rng(20240601)
     cases = []
