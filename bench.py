@@ -71,6 +71,31 @@ def host_note():
     return f"host {cpu_model()}, nproc {os.cpu_count()}, one thread"
 
 
+def bench_adi_model(ctx, w, ticks=200):
+    """The model tick on the ADI workload's grid: DiffusionADI + MulAll(-1e-3)
+    + Add at the 64 blob centres + Update, as HAL model code runs it, on the
+    automatic path (decay and sources inside the LDS tick kernel) and on the
+    register row solve (reduce, scan, tick and source launches)."""
+    import halhip
+    n = w["X"]
+    u = halhip.rng_doubles(42, 3 * 64)
+    idx = [int(n * u[3 * b]) * n + int(n * u[3 * b + 1]) for b in range(64)]
+    out = {"workload": f"ADI {n}x{n}, D=5, Dirichlet 0, decay 1e-3, 64 point sources 0.01, {ticks} ticks"}
+    for name, rows in (("fused", halhip.ADI_AUTO), ("register", halhip.ADI_ROWS_WORKGROUP_SCAN)):
+        plan = halhip.Adi2D(ctx, n, n)
+        plan.set_methods(rows, halhip.ADI_AUTO)
+        plan.set_sources(idx, [0.01] * 64)
+        plan.upload(w["field"])
+        plan.run_steps(20, w["rate"], 1e-3, False, False, bc=w["bc"])
+        plan.sync()
+        plan.run_steps(ticks, w["rate"], 1e-3, False, False, bc=w["bc"], asynchronous=True)
+        ms = plan.sync()
+        out[name + "_us_per_tick"] = round(ms * 1e3 / ticks, 3)
+        out[name + "_Gcell_s"] = round(n * n * ticks / (ms * 1e-3) / 1e9, 2)
+        plan.close()
+    return out
+
+
 def bench_explicit(ctx, ticks=1000, cpu=True):
     """Explicit 256^2 (workloads.explicit_256): r = 0.2, decay 1e-3, centre
     source 1.0, periodic, 1000 ticks resident; 8 B/cell-tick algorithmic."""
@@ -247,6 +272,9 @@ def main():
     ap.add_argument("--adi-only", action="store_true", help="skip the batched and adjoint comparisons")
     ap.add_argument("--shape", default=None,
                     help="strip shape columns_per_thread,strip_rows,vector(0|1) instead of the automatic one")
+    ap.add_argument("--methods", default=None,
+                    help="ADI row solver and carries as ROWS,CARRIES with ROWS in auto|scan|lds and "
+                         "CARRIES in auto|scan|window, instead of the automatic choice")
     ap.add_argument("--companion", choices=["explicit", "agents", "batched", "adjoint"], default=None,
                     help="run only this companion workload and print its JSON (for profiling)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -285,6 +313,12 @@ def main():
     if args.shape:
         cpt, rows, vec = (int(v) for v in args.shape.split(","))
         plan.set_strip_shape(cpt, rows, bool(vec))
+    if args.methods:
+        rows_name, carries_name = args.methods.split(",")
+        plan.set_methods({"auto": halhip.ADI_AUTO, "scan": halhip.ADI_ROWS_WORKGROUP_SCAN,
+                          "lds": halhip.ADI_ROWS_LDS}[rows_name],
+                         {"auto": halhip.ADI_AUTO, "scan": halhip.ADI_CARRIES_SCAN,
+                          "window": halhip.ADI_CARRIES_WINDOW}[carries_name])
     plan.upload(w["field"])
 
     plan.run_steps(args.warmup, w["rate"], 0.0, False, False, bc=w["bc"])
@@ -294,6 +328,10 @@ def main():
     event_ms = plan.sync()
     ctx.synchronize()
     wall = time.perf_counter() - t0
+
+    r_used, c_used, k_used = plan.methods()
+    adi_methods = ({halhip.ADI_ROWS_WORKGROUP_SCAN: "scan", halhip.ADI_ROWS_LDS: "lds"}.get(r_used, r_used),
+                   {halhip.ADI_CARRIES_SCAN: "scan", halhip.ADI_CARRIES_WINDOW: "window"}.get(c_used, c_used), k_used)
 
     # Per-kernel device time (HIP events around each launch on the plan's
     # stream), in a separate profiled pass of the same ticks.
@@ -308,6 +346,8 @@ def main():
     traffic = traffic_from_profile()
 
     extras = {}
+    if not args.shape and not args.methods:
+        extras["adi_model_tick"] = bench_adi_model(ctx, w)
     if not args.adi_only:
         cpu = not args.no_cpu_baseline
         extras["explicit"] = bench_explicit(ctx, cpu=cpu)
@@ -335,7 +375,8 @@ def main():
         "data": "synthetic: 64 Gaussian blobs (sigma 20, amplitude U[0.5,1]) at positions from Hal::Rng(42)",
         "config": {"workload": "ADI 4096x4096, D=5, dt=1, Dirichlet 0, 200 ticks", "grid": [n, n],
                    "bc": "dirichlet-0", "rate": w["rate"], "parallelism": "single GPU",
-                   "strip_shape": list(plan.strip_shape())},
+                   "strip_shape": list(plan.strip_shape()),
+                   "methods": dict(zip(("rows", "carries", "window"), adi_methods))},
         "hbm_frac_wall": round(value * BYTES_PER_CELL_TICK / HBM_PEAK_GBS, 4),
         "event_ms_per_step": round(event_ms / args.steps, 5),
         "kernels_ms_per_launch": {k: round(v[0] / v[1], 5) for k, v in prof.items()},

@@ -145,42 +145,48 @@ namespace
 {
     using LdsKernel = void (*)(AdiArgs, float *, int32_t, RowSolve);
 
-    template <int CPT, int S, int E, int T>
+    template <int CPT, int S, int E, int T, bool DECAY>
     LdsKernel PickLdsWindow(int32_t K)
     {
         if (K == 0)
         {
-            return AdiStripLds<CPT, S, E, 0, T>;
+            return AdiStripLds<CPT, S, E, 0, T, DECAY>;
         }
-        return (K <= 4) ? AdiStripLds<CPT, S, E, 4, T> : AdiStripLds<CPT, S, E, kMaxWindow, T>;
+        return (K <= 4) ? AdiStripLds<CPT, S, E, 4, T, DECAY> : AdiStripLds<CPT, S, E, kMaxWindow, T, DECAY>;
     }
 
     template <int CPT, int S, int E>
-    LdsKernel PickLdsThreads(int threads, int32_t K)
+    LdsKernel PickLdsThreads(int threads, int32_t K, bool decay)
     {
-        return threads == 1024 ? PickLdsWindow<CPT, S, E, 1024>(K)
-                               : (threads == 512 ? PickLdsWindow<CPT, S, E, 512>(K) : nullptr);
+        if (decay)
+        {
+            return threads == 1024 ? PickLdsWindow<CPT, S, E, 1024, true>(K)
+                                   : (threads == 512 ? PickLdsWindow<CPT, S, E, 512, true>(K) : nullptr);
+        }
+        return threads == 1024 ? PickLdsWindow<CPT, S, E, 1024, false>(K)
+                               : (threads == 512 ? PickLdsWindow<CPT, S, E, 512, false>(K) : nullptr);
     }
 
     // Kernel of the LDS row solve for a strip shape, E and workgroup size
     // (512 or 1024) with windowed carries of size K (0: carries from the scan
-    // kernel), or null when that combination is not compiled.
-    LdsKernel PickLds(int cpt, int s, int e, int32_t window, int threads)
+    // kernel), plain or with decay and sources, or null when that
+    // combination is not compiled.
+    LdsKernel PickLds(int cpt, int s, int e, int32_t window, int threads, bool decay = false)
     {
         switch (cpt * 10000 + s * 100 + e)
         {
             case 40832:
-                return PickLdsThreads<4, 8, 32>(threads, window);
+                return PickLdsThreads<4, 8, 32>(threads, window, decay);
             case 80432:
-                return PickLdsThreads<8, 4, 32>(threads, window);
+                return PickLdsThreads<8, 4, 32>(threads, window, decay);
             case 20816:
-                return PickLdsThreads<2, 8, 16>(threads, window);
+                return PickLdsThreads<2, 8, 16>(threads, window, decay);
             case 40416:
-                return PickLdsThreads<4, 4, 16>(threads, window);
+                return PickLdsThreads<4, 4, 16>(threads, window, decay);
             case 10808:
-                return PickLdsThreads<1, 8, 8>(threads, window);
+                return PickLdsThreads<1, 8, 8>(threads, window, decay);
             case 20408:
-                return PickLdsThreads<2, 4, 8>(threads, window);
+                return PickLdsThreads<2, 4, 8>(threads, window, decay);
             default:
                 return nullptr;
         }
@@ -310,6 +316,15 @@ struct halhip_adi2d
     DeviceArray<int64_t> srcIdx;
     DeviceArray<float> srcVal;
     int32_t nSources = 0;
+    // The same sources sorted by strip for the LDS decay kernel (stable, so
+    // list order within a strip): offsets [P+1], (x, y) pairs and values,
+    // built for strip height srcStripS (0: not built).
+    std::vector<int64_t> srcHostIdx;
+    std::vector<float> srcHostVal;
+    DeviceArray<int32_t> srcStrip;
+    DeviceArray<int32_t> srcXY;
+    DeviceArray<float> srcStripVal;
+    int32_t srcStripS = 0;
     std::string tableKey;
     bool reducedValid = false; // red half redCur holds the reduction of the current field for tableKey
     int32_t redCur = 0;        // which half of red[] holds the current reductions
@@ -551,12 +566,12 @@ namespace
         return p->red.get() + static_cast<size_t>(half) * 2 * static_cast<size_t>(p->P) * static_cast<size_t>(p->Y);
     }
 
-    void LaunchLds(halhip_adi2d *p)
+    void LaunchLds(halhip_adi2d *p, bool decay)
     {
         const int32_t window = (p->useCarry == HALHIP_ADI_CARRIES_WINDOW) ? p->window : 0;
-        LdsKernel k = PickLds(p->shape.cpt, p->shape.s, p->ldsE, window, p->shape.threads);
+        LdsKernel k = PickLds(p->shape.cpt, p->shape.s, p->ldsE, window, p->shape.threads, decay);
         Require(k != nullptr, "adi2d: no LDS kernel for the strip shape");
-        p->prof.Begin(p->clock.stream, "adi_strip_tick");
+        p->prof.Begin(p->clock.stream, decay ? "adi_strip_tick_decay" : "adi_strip_tick");
         hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(p->P)), dim3(static_cast<unsigned>(p->shape.threads)), 0,
                            p->clock.stream, p->args, p->field.get(), p->window, p->rs);
         Check(hipGetLastError(), "AdiStripLds launch");
@@ -597,15 +612,16 @@ namespace
         p->prof.End(p->clock.stream);
     }
 
-    // One plain tick's launches (the reduction of the field is resident in
-    // red half redCur).  Windowed ticks read that half and write the other.
-    void PlainTick(halhip_adi2d *p)
+    // One tick's launches (the reduction of the field is resident in red
+    // half redCur).  Windowed ticks read that half and write the other.
+    // decay: the LDS kernel with decay and sources (needs the LDS row solve).
+    void PlainTick(halhip_adi2d *p, bool decay = false)
     {
         if (p->useCarry == HALHIP_ADI_CARRIES_WINDOW)
         {
             p->args.redIn = RedHalf(p, p->redCur);
             p->args.red = RedHalf(p, 1 - p->redCur);
-            LaunchLds(p);
+            LaunchLds(p, decay);
             p->redCur = 1 - p->redCur;
             return;
         }
@@ -613,11 +629,91 @@ namespace
         LaunchScan(p);
         if (p->useRow == HALHIP_ADI_ROWS_LDS)
         {
-            LaunchLds(p);
+            LaunchLds(p, decay);
         }
         else
         {
             LaunchStrip(p, 1, "adi_strip_tick");
+        }
+    }
+
+    // Strip-sorted copy of the point sources for strip height S.
+    void BuildStripSources(halhip_adi2d *p)
+    {
+        const int32_t S = p->shape.s;
+        if (p->srcStripS == S)
+        {
+            return;
+        }
+        const int32_t n = p->nSources;
+        std::vector<int32_t> offsets(static_cast<size_t>(p->P) + 1, 0);
+        for (int32_t k = 0; k < n; k++)
+        {
+            offsets[static_cast<size_t>(p->srcHostIdx[k] / p->Y / S) + 1]++;
+        }
+        for (int32_t q = 0; q < p->P; q++)
+        {
+            offsets[q + 1] += offsets[q];
+        }
+        std::vector<int32_t> cursor(offsets.begin(), offsets.end() - 1);
+        std::vector<int32_t> xy(2 * static_cast<size_t>(n));
+        std::vector<float> val(static_cast<size_t>(n));
+        for (int32_t k = 0; k < n; k++)
+        {
+            const int64_t i = p->srcHostIdx[k];
+            const int32_t at = cursor[static_cast<size_t>(i / p->Y / S)]++;
+            xy[2 * static_cast<size_t>(at)] = static_cast<int32_t>(i / p->Y);
+            xy[2 * static_cast<size_t>(at) + 1] = static_cast<int32_t>(i % p->Y);
+            val[static_cast<size_t>(at)] = p->srcHostVal[k];
+        }
+        p->srcStrip.Release();
+        p->srcXY.Release();
+        p->srcStripVal.Release();
+        p->srcStrip.Allocate(p->ctx, offsets.size());
+        p->srcXY.Allocate(p->ctx, xy.size());
+        p->srcStripVal.Allocate(p->ctx, val.size());
+        Check(hipMemcpy(p->srcStrip.get(), offsets.data(), sizeof(int32_t) * offsets.size(), hipMemcpyHostToDevice),
+              "hipMemcpy");
+        Check(hipMemcpy(p->srcXY.get(), xy.data(), sizeof(int32_t) * xy.size(), hipMemcpyHostToDevice), "hipMemcpy");
+        Check(hipMemcpy(p->srcStripVal.get(), val.data(), sizeof(float) * val.size(), hipMemcpyHostToDevice),
+              "hipMemcpy");
+        p->srcStripS = S;
+        p->graphs.Clear();
+    }
+
+    // `steps` ticks of the loop body `tick`, replayed as 20-tick graphs
+    // keyed by `key` (even: a windowed chunk leaves redCur as it found it).
+    template <class F>
+    void RunChunked(halhip_adi2d *p, int32_t steps, std::string key, F tick)
+    {
+        constexpr int32_t kChunk = 20;
+        int32_t done = 0;
+        if (!p->prof.On() && steps >= kChunk)
+        {
+            AppendKey(key, kChunk);
+            AppendKey(key, p->redCur);
+            hipGraphExec_t exec = p->graphs.Find(key);
+            if (exec == nullptr)
+            {
+                hipGraph_t graph;
+                Check(hipStreamBeginCapture(p->clock.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+                for (int32_t k = 0; k < kChunk; k++)
+                {
+                    tick();
+                }
+                Check(hipStreamEndCapture(p->clock.stream, &graph), "hipStreamEndCapture");
+                Check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+                (void)hipGraphDestroy(graph);
+                p->graphs.Insert(key, exec);
+            }
+            for (; done + kChunk <= steps; done += kChunk)
+            {
+                Check(hipGraphLaunch(exec, p->clock.stream), "hipGraphLaunch");
+            }
+        }
+        for (; done < steps; done++)
+        {
+            tick();
         }
     }
 
@@ -639,41 +735,35 @@ namespace
             {
                 LaunchStrip(p, 0, "adi_strip_reduce");
             }
-            constexpr int32_t kChunk = 20; // even: a windowed chunk leaves redCur as it found it
-            int32_t done = 0;
-            if (!p->prof.On() && steps >= kChunk)
-            {
-                std::string key = p->tableKey;
-                AppendKey(key, kChunk);
-                AppendKey(key, p->redCur);
-                hipGraphExec_t exec = p->graphs.Find(key);
-                if (exec == nullptr)
-                {
-                    hipGraph_t graph;
-                    Check(hipStreamBeginCapture(p->clock.stream, hipStreamCaptureModeThreadLocal),
-                          "hipStreamBeginCapture");
-                    for (int32_t k = 0; k < kChunk; k++)
-                    {
-                        PlainTick(p);
-                    }
-                    Check(hipStreamEndCapture(p->clock.stream, &graph), "hipStreamEndCapture");
-                    Check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-                    (void)hipGraphDestroy(graph);
-                    p->graphs.Insert(key, exec);
-                }
-                for (; done + kChunk <= steps; done += kChunk)
-                {
-                    Check(hipGraphLaunch(exec, p->clock.stream), "hipGraphLaunch");
-                }
-            }
-            for (; done < steps; done++)
-            {
-                PlainTick(p);
-            }
+            RunChunked(p, steps, p->tableKey, [p] { PlainTick(p); });
             p->reducedValid = true;
             return;
         }
         p->args.negDecay = static_cast<float>(-decay);
+        if (p->useRow == HALHIP_ADI_ROWS_LDS)
+        {
+            // Decay and sources inside the LDS tick kernel, which also leaves
+            // the reduction of u' for the next tick.
+            const bool src = withSources && p->nSources > 0;
+            if (src)
+            {
+                BuildStripSources(p);
+            }
+            p->args.srcStrip = src ? p->srcStrip.get() : nullptr;
+            p->args.srcXY = src ? p->srcXY.get() : nullptr;
+            p->args.srcVal = src ? p->srcStripVal.get() : nullptr;
+            if (!p->reducedValid)
+            {
+                LaunchStrip(p, 0, "adi_strip_reduce");
+            }
+            std::string key = p->tableKey;
+            AppendKey(key, -1); // decay ticks
+            AppendKey(key, p->args.negDecay);
+            AppendKey(key, src ? 1 : 0);
+            RunChunked(p, steps, key, [p] { PlainTick(p, true); });
+            p->reducedValid = true;
+            return;
+        }
         for (int32_t t = 0; t < steps; t++)
         {
             LaunchStrip(p, 0, "adi_strip_reduce");
@@ -848,6 +938,10 @@ extern "C"
                 p->srcIdx.Release();
                 p->srcVal.Release();
                 p->nSources = count;
+                p->srcHostIdx.assign(index, index + count);
+                p->srcHostVal.assign(value, value + count);
+                p->srcStripS = 0;
+                p->graphs.Clear();
                 if (count == 0)
                 {
                     return;

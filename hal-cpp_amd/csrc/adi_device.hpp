@@ -64,6 +64,9 @@ namespace
         const float *srcVal;
         int32_t srcBegin;
         int32_t srcEnd;
+        // single grid with decay/sources on the LDS path: per-strip offsets
+        // [P+1] into srcXY / srcVal (null: no sources)
+        const int32_t *srcStrip;
         int32_t window; // K of windowed carries (0: carries from the scan kernel)
     };
 
@@ -1207,8 +1210,12 @@ namespace
 
     // KW: 0 takes the carries from the scan kernel, else windowed carries
     // with K <= KW.  EDGE: the strip holds row 0 or row X-1 (or rows past X).
+    // DECAY: the tick is u' = ADI(u) - k*u + sources (a.negDecay; the strip's
+    // sources are entries [a.srcStrip[p], a.srcStrip[p+1]) of a.srcXY /
+    // a.srcVal, in list order, or none when a.srcStrip is null), and the
+    // reduction for the next tick is taken of that u'.
     // rows: blockDim.x * (E+1) floats (S rows x 64W chunks x (E+1)).
-    template <int CPT, int S, int E, int KW, bool EDGE>
+    template <int CPT, int S, int E, int KW, bool EDGE, bool DECAY = false>
     __device__ __forceinline__ void StripLdsBody(const AdiArgs &a, float *__restrict__ u, int p, int K,
                                                  const RowSolve &rs, float *rows, float *sTotF, float *sTotB)
     {
@@ -1392,17 +1399,92 @@ namespace
         __syncthreads();
 
         // Column phase: u' from LDS, store, and next tick's strip reduction.
+        if constexpr (DECAY)
+        {
+            // u' = ADI(u) - k*u from the rows still in HBM, then the strip's
+            // point sources (PDEGrid2D::MulAll + Add), as the reference sums
+            // its deltas
+            float old[S][CPT];
 #pragma unroll
-        for (int r = 0; r < S; r++)
+            for (int r = 0; r < S; r++)
+            {
+                if (s0 + r < a.X)
+                {
+                    LoadRow<CPT, true>(u + static_cast<size_t>(s0 + r) * a.Y, y0, a.Y, old[r]);
+                }
+                else
+                {
+#pragma unroll
+                    for (int c = 0; c < CPT; c++)
+                    {
+                        old[r][c] = 0.0f;
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    v[r][c] = (s0 + r < a.X) ? fmaf(old[r][c], a.negDecay, rows[r * rowStride + RowAddr<E>(y0 + c)])
+                                             : 0.0f;
+                }
+            }
+            if (a.srcStrip != nullptr)
+            {
+                const int kEnd = a.srcStrip[p + 1];
+                for (int k = a.srcStrip[p]; k < kEnd; k++)
+                {
+                    const int sx = a.srcXY[2 * k] - s0;
+                    const int sy = a.srcXY[2 * k + 1] - y0;
+                    const float sv = a.srcVal[k];
+#pragma unroll
+                    for (int r = 0; r < S; r++)
+                    {
+#pragma unroll
+                        for (int c = 0; c < CPT; c++)
+                        {
+                            v[r][c] += (sx == r && sy == c) ? sv : 0.0f;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+                if (s0 + r < a.X)
+                {
+                    StoreRow<CPT, true>(u + static_cast<size_t>(s0 + r) * a.Y, y0, a.Y, v[r]);
+                }
+            }
+            // RhsAFromLds reads the rows of u' from LDS; each thread rewrites
+            // only the columns it read itself
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    rows[r * rowStride + RowAddr<E>(y0 + c)] = v[r][c];
+                }
+            }
+            __syncthreads();
+        }
+        else
         {
 #pragma unroll
-            for (int c = 0; c < CPT; c++)
+            for (int r = 0; r < S; r++)
             {
-                v[r][c] = (s0 + r < a.X) ? rows[r * rowStride + RowAddr<E>(y0 + c)] : 0.0f;
-            }
-            if (s0 + r < a.X)
-            {
-                StoreRow<CPT, true>(u + static_cast<size_t>(s0 + r) * a.Y, y0, a.Y, v[r]);
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    v[r][c] = (s0 + r < a.X) ? rows[r * rowStride + RowAddr<E>(y0 + c)] : 0.0f;
+                }
+                if (s0 + r < a.X)
+                {
+                    StoreRow<CPT, true>(u + static_cast<size_t>(s0 + r) * a.Y, y0, a.Y, v[r]);
+                }
             }
         }
         RhsAFromLds<CPT, S, E, EDGE>(a, rows, rowStride, s0, y0, first, last, v);
@@ -1425,7 +1507,7 @@ namespace
 
     // T: threads per workgroup (= yDim / CPT); the LDS row buffer holds
     // T * (E+1) floats.
-    template <int CPT, int S, int E, int KW, int T>
+    template <int CPT, int S, int E, int KW, int T, bool DECAY = false>
     __global__ __launch_bounds__(T) void AdiStripLds(AdiArgs a, float *__restrict__ u, int32_t K, RowSolve rs)
     {
         static_assert(T * (E + 1) * 4 <= 140 * 1024, "LDS row buffer too large");
@@ -1435,11 +1517,11 @@ namespace
         const int p = StripOfBlock(blockIdx.x, a.P, KW > 0 && (a.P % 8 == 0));
         if (p == 0 || (p + 1) * S >= a.X)
         {
-            StripLdsBody<CPT, S, E, KW, true>(a, u, p, K, rs, rows, sTotF, sTotB);
+            StripLdsBody<CPT, S, E, KW, true, DECAY>(a, u, p, K, rs, rows, sTotF, sTotB);
         }
         else
         {
-            StripLdsBody<CPT, S, E, KW, false>(a, u, p, K, rs, rows, sTotF, sTotB);
+            StripLdsBody<CPT, S, E, KW, false, DECAY>(a, u, p, K, rs, rows, sTotF, sTotB);
         }
     }
 

@@ -265,6 +265,64 @@ def test_methods_match_oracle(ctx, oracle, X, Y, shape, rate, wx, bc):
         assert (halhip.ADI_ROWS_LDS, halhip.ADI_CARRIES_WINDOW) in used
 
 
+DECAY_CASES = [
+    # X, Y, (cpt, rows), rate, wrapX, bc
+    (300, 1024, (1, 8), 5.0, 0, 0.2),
+    (200, 4096, (4, 8), 5.0, 0, 0.0),
+    (61, 4096, (4, 4), 2.0, 0, 0.5),
+    (100, 4096, (8, 4), 5.0, 0, None),
+    (128, 2048, (4, 8), 5.0, 1, None),
+]
+
+
+@pytest.mark.parametrize("X,Y,shape,rate,wx,bc", DECAY_CASES)
+def test_decay_and_sources_every_method_match_oracle(ctx, oracle, X, Y, shape, rate, wx, bc):
+    # DiffusionADI + MulAll(-k) + Add + Update ticks: on the LDS row solve the
+    # decay and the strip's sources are applied inside the tick kernel.
+    # Sources: a repeated cell, the first and last cells, both sides of a
+    # strip boundary; runs of odd length and graph-sized chunks.
+    S = shape[1]
+    f = np.random.default_rng(X * 3 + Y).random(X * Y).astype(np.float32)
+    idx = [0, 5 * Y + 7, 5 * Y + 7, X * Y - 1, (S - 1) * Y + 100, S * Y + 100, (X // 2) * Y + Y - 1]
+    val = [1.0, 0.5, 0.25, 2.0, 0.75, -0.5, 0.125]
+    ref = adi_ref_ticks(oracle, f, 25, rate, X, Y, wx, 0, bc, 4e-3, idx, val)
+    plan = halhip.Adi2D(ctx, X, Y)
+    plan.set_strip_shape(shape[0], shape[1], True)
+    plan.set_sources(idx, val)
+    used = set()
+    for rows, carries in ((halhip.ADI_ROWS_WORKGROUP_SCAN, halhip.ADI_CARRIES_SCAN),
+                          (halhip.ADI_ROWS_LDS, halhip.ADI_CARRIES_SCAN),
+                          (halhip.ADI_ROWS_LDS, halhip.ADI_CARRIES_WINDOW)):
+        plan.set_methods(rows, carries)
+        plan.upload(f)
+        for n in (1, 21, 3):
+            plan.run_steps(n, rate, 4e-3, wx, 0, bc=bc)
+        m = plan.methods()
+        used.add(m[:2])
+        err = rel_l2(plan.download(), ref)
+        assert err < 1e-5, (rows, carries, m, err)
+    assert (halhip.ADI_ROWS_LDS, halhip.ADI_CARRIES_SCAN) in used
+
+
+def test_plain_and_decay_runs_chain(ctx, oracle):
+    # The decay ticks leave the reduction of their result for a following
+    # plain run, and the other way round.
+    X, Y = 96, 4096
+    f = np.random.default_rng(8).random(X * Y).astype(np.float32)
+    plan = halhip.Adi2D(ctx, X, Y)
+    plan.upload(f)
+    plan.run_steps(3, 5.0, 0.0, False, False, bc=0.0)
+    plan.set_sources([3 * Y + 9], [1.5])
+    plan.run_steps(22, 5.0, 1e-2, False, False, bc=0.0)
+    plan.set_sources([], [])
+    plan.run_steps(5, 5.0, 0.0, False, False, bc=0.0)
+    ref = adi_ref_ticks(oracle, f, 3, 5.0, X, Y, 0, 0, 0.0)
+    ref = adi_ref_ticks(oracle, ref, 22, 5.0, X, Y, 0, 0, 0.0, 1e-2, [3 * Y + 9], [1.5])
+    ref = adi_ref_ticks(oracle, ref, 5, 5.0, X, Y, 0, 0, 0.0)
+    assert plan.methods()[0] == halhip.ADI_ROWS_LDS
+    assert rel_l2(plan.download(), ref) < 1e-5
+
+
 def test_methods_odd_tick_counts_keep_reductions(ctx, oracle):
     # Windowed ticks alternate two reduction buffers; runs of odd length and
     # graph-sized chunks must chain correctly.
