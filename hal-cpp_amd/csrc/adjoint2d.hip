@@ -17,11 +17,12 @@
 //   dJ/dk   -= <lam, u>
 // with dx p = p(x+1,y) - p(x,y) (0 on the last row), dy likewise.
 //
-// Kernels: a line solve (one wavefront per line, lane-local recurrences plus a
-// wavefront scan of affine maps; the per-cell Thomas factors are computed once
-// per diffusivity field on the host in FP64), an LDS-tiled transpose so each
-// sweep reads its lines contiguously, and a fused gradient / adjoint-update
-// kernel.  Checkpoints every K ticks; the backward pass recomputes each
+// Kernels: a line solve (one workgroup of four wavefronts per line,
+// thread-local recurrences composed by wavefront scans of affine maps and
+// LDS-exchanged wavefront totals; the per-cell Thomas factors are computed
+// once per diffusivity field on the host in FP64), an LDS-tiled transpose so
+// each sweep reads its lines contiguously, and a fused gradient /
+// adjoint-update kernel.  Checkpoints every K ticks; the backward pass recomputes each
 // segment from its checkpoint, keeping u, s2, s4 of the segment.
 #include "common.hpp"
 
@@ -34,8 +35,6 @@ using namespace halhip;
 
 namespace
 {
-    constexpr int kLineThreads = 256; // four lines per block
-
     template <int CTRL, int ROW_MASK = 0xF>
     __device__ __forceinline__ float DppF(float old, float v)
     {
@@ -59,65 +58,139 @@ namespace
         float *stash;      // optional copy of x before decay
     };
 
-    // RHS 0: d = c + 1/2 (rate[l] (in[l+1] - c) + rate[l-1] (in[l-1] - c)) across lines.
-    // RHS 1: the same along the line (neighbours j+-1 of the same line).
-    // RHS 2: d = in.
-    template <int EPL, int RHS, bool DECAY, bool STASH>
-    __global__ __launch_bounds__(kLineThreads) void CellLineSolve(LineArgs a)
+    // Line solve: a workgroup of kBlockLine threads per line (four
+    // wavefronts, EPL adjacent elements per thread, L <= kBlockLine * EPL).
+    // With VEC (EPL and L multiples of 4) every array moves as 16-byte
+    // vectors.  Each recurrence composes per-thread folds, a wavefront scan
+    // of affine maps (DPP) and the wavefront totals exchanged through LDS.
+    constexpr int kBlockLine = 256;
+
+    template <int EPL, bool VEC>
+    __device__ __forceinline__ void LoadLine(const float *__restrict__ src, size_t base, int j0, int L, float pad,
+                                             float (&v)[EPL])
     {
-        const int lane = threadIdx.x & 63;
-        const int line = blockIdx.x * (kLineThreads / 64) + (threadIdx.x >> 6);
-        if (line >= a.nLines)
+        if constexpr (VEC)
         {
-            return;
-        }
-        const int L = a.L;
-        const size_t base = static_cast<size_t>(line) * L;
-        const int j0 = lane * EPL;
-        float d[EPL];
-        float fw[EPL];
+            // L is a multiple of 4: each group of 4 lies wholly inside or
+            // wholly past the line
 #pragma unroll
-        for (int e = 0; e < EPL; e++)
-        {
-            const int j = j0 + e;
-            const bool in = j < L;
-            const float c = in ? a.in[base + j] : 0.0f;
-            d[e] = c;
-            fw[e] = in ? a.fw[base + j] : 1.0f;
+            for (int q = 0; q < EPL / 4; q++)
+            {
+                float4 t = make_float4(pad, pad, pad, pad);
+                if (j0 + 4 * q < L)
+                {
+                    t = *reinterpret_cast<const float4 *>(src + base + j0 + 4 * q);
+                }
+                v[4 * q] = t.x;
+                v[4 * q + 1] = t.y;
+                v[4 * q + 2] = t.z;
+                v[4 * q + 3] = t.w;
+            }
         }
-        if constexpr (RHS == 0)
+        else
         {
 #pragma unroll
             for (int e = 0; e < EPL; e++)
             {
-                const int j = j0 + e;
-                if (j < L)
+                v[e] = (j0 + e < L) ? src[base + j0 + e] : pad;
+            }
+        }
+    }
+
+    template <int EPL, bool VEC>
+    __device__ __forceinline__ void StoreLine(float *__restrict__ dst, size_t base, int j0, int L, const float (&v)[EPL])
+    {
+        if constexpr (VEC)
+        {
+#pragma unroll
+            for (int q = 0; q < EPL / 4; q++)
+            {
+                if (j0 + 4 * q < L)
                 {
-                    const float c = d[e];
-                    float s = 0.0f;
-                    if (line + 1 < a.nLines)
-                    {
-                        s += a.rate[base + j] * (a.in[base + L + j] - c);
-                    }
-                    if (line > 0)
-                    {
-                        s += a.rate[base - L + j] * (a.in[base - L + j] - c);
-                    }
-                    d[e] = fmaf(0.5f, s, c);
+                    *reinterpret_cast<float4 *>(dst + base + j0 + 4 * q) =
+                        make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
                 }
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int e = 0; e < EPL; e++)
+            {
+                if (j0 + e < L)
+                {
+                    dst[base + j0 + e] = v[e];
+                }
+            }
+        }
+    }
+
+    // RHS 0: d = c + 1/2 (rate[l] (in[l+1] - c) + rate[l-1] (in[l-1] - c)) across lines.
+    // RHS 1: the same along the line (neighbours j+-1 of the same line).
+    // RHS 2: d = in.
+    // DECAY: out = x - k * prev; STASH: keeps x before the decay.
+    template <int EPL, int RHS, bool DECAY, bool STASH, bool VEC>
+    __global__ __launch_bounds__(kBlockLine) void CellLineSolveBlock(LineArgs a)
+    {
+        __shared__ float sA[kBlockLine / 64];
+        __shared__ float sB[kBlockLine / 64];
+        const int t = threadIdx.x;
+        const int lane = t & 63;
+        const int wave = t >> 6;
+        const int line = blockIdx.x;
+        const int L = a.L;
+        const size_t base = static_cast<size_t>(line) * L;
+        const int j0 = t * EPL;
+        float d[EPL];
+        float fw[EPL];
+        LoadLine<EPL, VEC>(a.in, base, j0, L, 0.0f, d);
+        LoadLine<EPL, VEC>(a.fw, base, j0, L, 1.0f, fw);
+        if constexpr (RHS == 0)
+        {
+            // explicit half-step across lines: neighbours in lines l +- 1
+            float r[EPL];
+            float nb[EPL];
+            float s[EPL];
+#pragma unroll
+            for (int e = 0; e < EPL; e++)
+            {
+                s[e] = 0.0f;
+            }
+            if (line + 1 < a.nLines)
+            {
+                LoadLine<EPL, VEC>(a.rate, base, j0, L, 0.0f, r);
+                LoadLine<EPL, VEC>(a.in, base + L, j0, L, 0.0f, nb);
+#pragma unroll
+                for (int e = 0; e < EPL; e++)
+                {
+                    s[e] += r[e] * (nb[e] - d[e]);
+                }
+            }
+            if (line > 0)
+            {
+                LoadLine<EPL, VEC>(a.rate, base - L, j0, L, 0.0f, r);
+                LoadLine<EPL, VEC>(a.in, base - L, j0, L, 0.0f, nb);
+#pragma unroll
+                for (int e = 0; e < EPL; e++)
+                {
+                    s[e] += r[e] * (nb[e] - d[e]);
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < EPL; e++)
+            {
+                d[e] = (j0 + e < L) ? fmaf(0.5f, s[e], d[e]) : 0.0f;
             }
         }
         else if constexpr (RHS == 1)
         {
+            // explicit half-step along the line; the outer neighbours of the
+            // thread's elements come from the cache
             float r[EPL];
-#pragma unroll
-            for (int e = 0; e < EPL; e++)
-            {
-                r[e] = (j0 + e < L) ? a.rate[base + j0 + e] : 0.0f;
-            }
-            const float cLeft = DppF<0x138>(0.0f, d[EPL - 1]); // previous lane's last value
-            const float rLeft = DppF<0x138>(0.0f, r[EPL - 1]); // and its face rate
-            const float cRight = DppF<0x130>(0.0f, d[0]);      // next lane's first value
+            LoadLine<EPL, VEC>(a.rate, base, j0, L, 0.0f, r);
+            const float cLeft = (j0 > 0 && j0 - 1 < L) ? a.in[base + j0 - 1] : 0.0f;
+            const float rLeft = (j0 > 0 && j0 - 1 < L) ? a.rate[base + j0 - 1] : 0.0f;
+            const float cRight = (j0 + EPL < L) ? a.in[base + j0 + EPL] : 0.0f;
             float out[EPL];
 #pragma unroll
             for (int e = 0; e < EPL; e++)
@@ -127,10 +200,10 @@ namespace
                 const float cp = (e + 1 < EPL) ? d[e + 1] : cRight;
                 const float cm = (e > 0) ? d[e - 1] : cLeft;
                 const float rm = (e > 0) ? r[e - 1] : rLeft;
-                float s = 0.0f;
-                s += (j + 1 < L) ? r[e] * (cp - c) : 0.0f;
-                s += (j > 0) ? rm * (cm - c) : 0.0f;
-                out[e] = (j < L) ? fmaf(0.5f, s, c) : 0.0f;
+                float sum = 0.0f;
+                sum += (j + 1 < L) ? r[e] * (cp - c) : 0.0f;
+                sum += (j > 0) ? rm * (cm - c) : 0.0f;
+                out[e] = (j < L) ? fmaf(0.5f, sum, c) : 0.0f;
             }
 #pragma unroll
             for (int e = 0; e < EPL; e++)
@@ -139,8 +212,7 @@ namespace
             }
         }
 
-        // Forward recurrence d'_j = fw_j d'_{j-1} + d_j: lane-local, then a
-        // wavefront scan of (A, B) with the carry into the line equal to 0.
+        // Forward recurrence d'_j = fw_j d'_{j-1} + d_j, carry into the line 0.
         float A = 1.0f;
         float B = 0.0f;
 #pragma unroll
@@ -150,26 +222,39 @@ namespace
             A *= fw[e];
         }
         {
-            float t;
-            t = DppF<0x111>(0.0f, B);
-            B = fmaf(A, t, B);
+            float tb;
+            tb = DppF<0x111>(0.0f, B);
+            B = fmaf(A, tb, B);
             A *= DppF<0x111>(1.0f, A);
-            t = DppF<0x112>(0.0f, B);
-            B = fmaf(A, t, B);
+            tb = DppF<0x112>(0.0f, B);
+            B = fmaf(A, tb, B);
             A *= DppF<0x112>(1.0f, A);
-            t = DppF<0x114>(0.0f, B);
-            B = fmaf(A, t, B);
+            tb = DppF<0x114>(0.0f, B);
+            B = fmaf(A, tb, B);
             A *= DppF<0x114>(1.0f, A);
-            t = DppF<0x118>(0.0f, B);
-            B = fmaf(A, t, B);
+            tb = DppF<0x118>(0.0f, B);
+            B = fmaf(A, tb, B);
             A *= DppF<0x118>(1.0f, A);
-            t = DppF<0x142, 0xA>(0.0f, B);
-            B = fmaf(A, t, B);
+            tb = DppF<0x142, 0xA>(0.0f, B);
+            B = fmaf(A, tb, B);
             A *= DppF<0x142, 0xA>(1.0f, A);
-            t = DppF<0x143, 0xC>(0.0f, B);
-            B = fmaf(A, t, B);
+            tb = DppF<0x143, 0xC>(0.0f, B);
+            B = fmaf(A, tb, B);
+            A *= DppF<0x143, 0xC>(1.0f, A);
         }
-        float acc = DppF<0x138>(0.0f, B); // carry into this lane
+        if (lane == 63)
+        {
+            sA[wave] = A;
+            sB[wave] = B;
+        }
+        __syncthreads();
+        float cw = 0.0f;
+        for (int g = 0; g < wave; g++)
+        {
+            cw = fmaf(sA[g], cw, sB[g]);
+        }
+        // this lane's carry: the preceding lanes' composition applied to cw
+        float acc = fmaf(DppF<0x138>(1.0f, A), cw, DppF<0x138>(0.0f, B));
 #pragma unroll
         for (int e = 0; e < EPL; e++)
         {
@@ -180,14 +265,8 @@ namespace
         // Backward recurrence x_j = bw_j x_{j+1} + ib_j d'_j from the right.
         float bw[EPL];
         float ib[EPL];
-#pragma unroll
-        for (int e = 0; e < EPL; e++)
-        {
-            const int j = j0 + e;
-            const bool in = j < L;
-            bw[e] = in ? a.bw[base + j] : 0.0f;
-            ib[e] = in ? a.ib[base + j] : 0.0f;
-        }
+        LoadLine<EPL, VEC>(a.bw, base, j0, L, 0.0f, bw);
+        LoadLine<EPL, VEC>(a.ib, base, j0, L, 0.0f, ib);
         A = 1.0f;
         B = 0.0f;
 #pragma unroll
@@ -208,30 +287,80 @@ namespace
                 A *= Ad;
             }
         }
-        float carry = __shfl_down(B, 1);
-        acc = (lane == 63) ? 0.0f : carry;
+        __syncthreads(); // the forward totals are read
+        if (lane == 0)
+        {
+            sA[wave] = A;
+            sB[wave] = B;
+        }
+        __syncthreads();
+        cw = 0.0f;
+        for (int g = kBlockLine / 64 - 1; g > wave; g--)
+        {
+            cw = fmaf(sA[g], cw, sB[g]);
+        }
+        const float An = __shfl_down(A, 1);
+        const float Bn = __shfl_down(B, 1);
+        acc = (lane == 63) ? cw : fmaf(An, cw, Bn);
 #pragma unroll
         for (int e = EPL - 1; e >= 0; e--)
         {
             acc = fmaf(bw[e], acc, ib[e] * d[e]);
             d[e] = acc;
         }
-#pragma unroll
-        for (int e = 0; e < EPL; e++)
+        if constexpr (STASH)
         {
-            const int j = j0 + e;
-            if (j < L)
+            StoreLine<EPL, VEC>(a.stash, base, j0, L, d);
+        }
+        if constexpr (DECAY)
+        {
+            float pv[EPL];
+            LoadLine<EPL, VEC>(a.prev, base, j0, L, 0.0f, pv);
+#pragma unroll
+            for (int e = 0; e < EPL; e++)
             {
-                if constexpr (STASH)
-                {
-                    a.stash[base + j] = d[e];
-                }
-                float v = d[e];
-                if constexpr (DECAY)
-                {
-                    v = fmaf(a.negK, a.prev[base + j], v);
-                }
-                a.out[base + j] = v;
+                d[e] = fmaf(a.negK, pv[e], d[e]);
+            }
+        }
+        StoreLine<EPL, VEC>(a.out, base, j0, L, d);
+    }
+
+    // out[c][r] = in[r][c] for an R x C matrix (R, C multiples of 4) through
+    // 64 x 64 LDS tiles, reading and writing 16-byte vectors.
+    __global__ __launch_bounds__(256) void Transpose4(const float *__restrict__ in, float *__restrict__ out, int32_t R,
+                                                      int32_t C)
+    {
+        __shared__ float tile[64][65];
+        const int c0 = blockIdx.x * 64;
+        const int r0 = blockIdx.y * 64;
+        const int q = threadIdx.x & 15;  // 4-column group
+        const int rr = threadIdx.x >> 4; // 0..15
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+        {
+            const int r = r0 + rr + 16 * k;
+            const int c = c0 + 4 * q;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (r < R && c < C)
+            {
+                v = *reinterpret_cast<const float4 *>(in + static_cast<size_t>(r) * C + c);
+            }
+            tile[rr + 16 * k][4 * q] = v.x;
+            tile[rr + 16 * k][4 * q + 1] = v.y;
+            tile[rr + 16 * k][4 * q + 2] = v.z;
+            tile[rr + 16 * k][4 * q + 3] = v.w;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+        {
+            const int c = c0 + rr + 16 * k; // output row
+            const int r = r0 + 4 * q;
+            if (c < C && r < R)
+            {
+                const int tc = rr + 16 * k;
+                *reinterpret_cast<float4 *>(out + static_cast<size_t>(c) * R + r) =
+                    make_float4(tile[4 * q][tc], tile[4 * q + 1][tc], tile[4 * q + 2][tc], tile[4 * q + 3][tc]);
             }
         }
     }
@@ -399,15 +528,6 @@ namespace
         }
     }
 
-    int Epl(int32_t L)
-    {
-        int e = 1;
-        while (64 * e < L)
-        {
-            e *= 2;
-        }
-        return e;
-    }
 }
 
 struct halhip_adjoint2d
@@ -445,40 +565,50 @@ namespace
         Bind(p->ctx);
     }
 
+    template <int EPL, int RHS, bool DECAY, bool STASH>
+    void LaunchBlockLine(halhip_adjoint2d *p, const LineArgs &a)
+    {
+        const bool vec = (EPL % 4 == 0) && (a.L % 4 == 0);
+        if (vec)
+        {
+            hipLaunchKernelGGL((CellLineSolveBlock<EPL, RHS, DECAY, STASH, (EPL % 4 == 0)>),
+                               dim3(static_cast<unsigned>(a.nLines)), dim3(kBlockLine), 0, p->clock.stream, a);
+        }
+        else
+        {
+            hipLaunchKernelGGL((CellLineSolveBlock<EPL, RHS, DECAY, STASH, false>),
+                               dim3(static_cast<unsigned>(a.nLines)), dim3(kBlockLine), 0, p->clock.stream, a);
+        }
+    }
+
+    // One line solve per line of the family: a workgroup of four
+    // wavefronts per line, EPL = the smallest power of two with
+    // 256 * EPL >= L (sides are at most 4096).
     template <int RHS, bool DECAY, bool STASH>
     void LaunchLine(halhip_adjoint2d *p, const LineArgs &a, const char *name)
     {
-        const unsigned blocks = static_cast<unsigned>((a.nLines + 3) / 4);
         p->prof.Begin(p->clock.stream, name);
-        switch (Epl(a.L))
+        int epl = 1;
+        while (kBlockLine * epl < a.L)
+        {
+            epl *= 2;
+        }
+        switch (epl)
         {
             case 1:
-                hipLaunchKernelGGL((CellLineSolve<1, RHS, DECAY, STASH>), dim3(blocks), dim3(kLineThreads), 0,
-                                   p->clock.stream, a);
+                LaunchBlockLine<1, RHS, DECAY, STASH>(p, a);
                 break;
             case 2:
-                hipLaunchKernelGGL((CellLineSolve<2, RHS, DECAY, STASH>), dim3(blocks), dim3(kLineThreads), 0,
-                                   p->clock.stream, a);
+                LaunchBlockLine<2, RHS, DECAY, STASH>(p, a);
                 break;
             case 4:
-                hipLaunchKernelGGL((CellLineSolve<4, RHS, DECAY, STASH>), dim3(blocks), dim3(kLineThreads), 0,
-                                   p->clock.stream, a);
+                LaunchBlockLine<4, RHS, DECAY, STASH>(p, a);
                 break;
             case 8:
-                hipLaunchKernelGGL((CellLineSolve<8, RHS, DECAY, STASH>), dim3(blocks), dim3(kLineThreads), 0,
-                                   p->clock.stream, a);
-                break;
-            case 16:
-                hipLaunchKernelGGL((CellLineSolve<16, RHS, DECAY, STASH>), dim3(blocks), dim3(kLineThreads), 0,
-                                   p->clock.stream, a);
-                break;
-            case 32:
-                hipLaunchKernelGGL((CellLineSolve<32, RHS, DECAY, STASH>), dim3(blocks), dim3(kLineThreads), 0,
-                                   p->clock.stream, a);
+                LaunchBlockLine<8, RHS, DECAY, STASH>(p, a);
                 break;
             default:
-                hipLaunchKernelGGL((CellLineSolve<64, RHS, DECAY, STASH>), dim3(blocks), dim3(kLineThreads), 0,
-                                   p->clock.stream, a);
+                LaunchBlockLine<16, RHS, DECAY, STASH>(p, a);
                 break;
         }
         Check(hipGetLastError(), name);
@@ -488,8 +618,15 @@ namespace
     void LaunchTranspose(halhip_adjoint2d *p, const float *in, float *out, int32_t R, int32_t C)
     {
         p->prof.Begin(p->clock.stream, "adjoint_transpose");
-        hipLaunchKernelGGL(Transpose, dim3(static_cast<unsigned>((C + 63) / 64), static_cast<unsigned>((R + 63) / 64)),
-                           dim3(256), 0, p->clock.stream, in, out, R, C);
+        const dim3 grid(static_cast<unsigned>((C + 63) / 64), static_cast<unsigned>((R + 63) / 64));
+        if (R % 4 == 0 && C % 4 == 0)
+        {
+            hipLaunchKernelGGL(Transpose4, grid, dim3(256), 0, p->clock.stream, in, out, R, C);
+        }
+        else
+        {
+            hipLaunchKernelGGL(Transpose, grid, dim3(256), 0, p->clock.stream, in, out, R, C);
+        }
         Check(hipGetLastError(), "Transpose launch");
         p->prof.End(p->clock.stream);
     }

@@ -18,7 +18,10 @@ def small_problem(X=24, Y=20, seed=0):
     return D, u0
 
 
-@pytest.mark.parametrize("X,Y", [(24, 20), (64, 200), (130, 70)])
+# Line lengths cover every per-thread width of the line solve (1, 2, 4, 8,
+# 16 elements) with and without 16-byte vectors, including partly filled
+# vector groups at the end of a line (2052 = 8 * 256 + 4).
+@pytest.mark.parametrize("X,Y", [(24, 20), (64, 200), (130, 70), (300, 520), (1030, 18), (2052, 8)])
 def test_forward_matches_oracle(ctx, oracle, X, Y):
     D, u0 = small_problem(X, Y, X)
     p = halhip.Adjoint2D(ctx, X, Y)
@@ -62,6 +65,21 @@ def test_gradient_matches_fp64_adjoint(ctx, oracle, every):
     J, gD, gk = oracle.adi_cell_gradient(u0.astype(np.float64), f32(D), float(np.float32(k)), ticks, X, Y, target)
     p = halhip.Adjoint2D(ctx, X, Y)
     p.set_problem(D, k, ticks, every)
+    Jg, gDg, gkg = p.gradient(u0, target.astype(np.float32))
+    assert abs(Jg - J) < 1e-4 * abs(J)
+    assert rel_l2(gDg, gD) < 1e-4
+    assert abs(gkg - gk) < 1e-4 * abs(gk)
+
+
+def test_gradient_long_lines_match_fp64_adjoint(ctx, oracle):
+    # x-lines of 516 cells: vector loads and four wavefronts per line in the
+    # forward, recompute and adjoint solves
+    X, Y, ticks, k = 516, 12, 10, 5e-3
+    D, u0 = small_problem(X, Y, 4)
+    target = make_target(oracle, u0, D, k, ticks, X, Y, 10)
+    J, gD, gk = oracle.adi_cell_gradient(u0.astype(np.float64), f32(D), float(np.float32(k)), ticks, X, Y, target)
+    p = halhip.Adjoint2D(ctx, X, Y)
+    p.set_problem(D, k, ticks, 3)
     Jg, gDg, gkg = p.gradient(u0, target.astype(np.float32))
     assert abs(Jg - J) < 1e-4 * abs(J)
     assert rel_l2(gDg, gD) < 1e-4
