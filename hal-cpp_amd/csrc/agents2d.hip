@@ -307,24 +307,35 @@ namespace
                                                     const float2 *__restrict__ spos, const float *__restrict__ srad,
                                                     float &vx, float &vy)
     {
-        for (int q = s; q < e; q++)
+        // one candidate: hardware square root and reciprocal (within 1 ulp);
+        // the FP32 sums stay within 1e-5 of the reference's FP64 ones
+        auto one = [&](int q, float2 o, float rq)
         {
-            if (q == k)
-            {
-                continue;
-            }
-            const float2 o = spos[q];
             const float dx = Disp(p.x, o.x, fX, wrapX);
             const float dy = Disp(p.y, o.y, fY, wrapY);
-            // hardware square root and reciprocal (within 1 ulp): the FP32
-            // sums stay within 1e-5 of the reference's FP64 ones
             const float dist = __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
-            if (dist < R && dist > 0.0f)
+            if (q != k && dist < R && dist > 0.0f)
             {
-                const float f = kSpring * ((ri + srad[q]) - dist) * __builtin_amdgcn_rcpf(dist);
+                const float f = kSpring * ((ri + rq) - dist) * __builtin_amdgcn_rcpf(dist);
                 vx -= dx * f;
                 vy -= dy * f;
             }
+        };
+        // candidates in pairs, both loads in flight together, summed in slot
+        // order
+        int q = s;
+        for (; q + 1 < e; q += 2)
+        {
+            const float2 o0 = spos[q];
+            const float2 o1 = spos[q + 1];
+            const float r0 = srad[q];
+            const float r1 = srad[q + 1];
+            one(q, o0, r0);
+            one(q + 1, o1, r1);
+        }
+        if (q < e)
+        {
+            one(q, spos[q], srad[q]);
         }
     }
 
@@ -354,6 +365,31 @@ namespace
         const int y1 = static_cast<int>(ceilf(p.y + R));
         float vx = 0.0f;
         float vy = 0.0f;
+        if (!WX && !WY && x1 - x0 <= 3)
+        {
+            // Up to three neighbour rows: all six slot bounds are requested
+            // together, then the rows are summed x-major as below.
+            const int ya = max(y0, 0);
+            const int yb = min(y1, Y);
+            int rs[3];
+            int re[3];
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+            {
+                const int xs = x0 + r;
+                const bool ok = xs < x1 && xs >= 0 && xs < X && ya < yb;
+                const int64_t rowBase = static_cast<int64_t>(ok ? xs : 0) * Y;
+                rs[r] = ok ? offsets[rowBase + ya] : 0;
+                re[r] = ok ? offsets[rowBase + yb] : 0;
+            }
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+            {
+                AccumulateRange(rs[r], re[r], k, p, ri, R, kSpring, fX, fY, false, false, spos, srad, vx, vy);
+            }
+            vel[k] = make_float2(vx, vy);
+            return;
+        }
         for (int xs = x0; xs < x1; xs++)
         {
             int cx = xs;
