@@ -339,8 +339,79 @@ namespace
         }
     }
 
+    // What the move does to the agent in slot i: p += dt*v with MoveSafePT
+    // (an off-grid target wraps a wrapped axis and keeps the old coordinate
+    // on a non-wrapped one).  With a carried velocity (friction or a speed
+    // cap set), the velocity is carry + this tick's force delta v, as
+    // SphericalAgent2D accumulates SumForces into xVel/yVel
+    // (include/Hal/Agents/SphericalAgent2D.h:100-128); its norm is capped at
+    // maxVel when maxVel > 0 (CapVelocity, :139-150), the move uses it
+    // (ForceMove, :153-156), and friction * velocity is carried to the next
+    // tick (ApplyFriction, :133-137).
+    struct MoveArgs
+    {
+        int32_t on;      // 0: forces only
+        float dt;
+        float2 *posNext; // moved positions, slot order
+        float2 *carry;   // carried velocities (null: none)
+        float friction;
+        float maxVel;
+    };
+
+    __device__ __forceinline__ float2 MovedPosition(const MoveArgs &m, int i, float2 o, float2 v, int32_t X,
+                                                    int32_t Y, bool wrapX, bool wrapY)
+    {
+        if (m.carry != nullptr)
+        {
+            const float2 c = m.carry[i];
+            v.x += c.x;
+            v.y += c.y;
+            if (m.maxVel > 0.0f)
+            {
+                const float normSq = v.x * v.x + v.y * v.y;
+                if (normSq > m.maxVel * m.maxVel)
+                {
+                    const float f = m.maxVel / sqrtf(normSq);
+                    v.x *= f;
+                    v.y *= f;
+                }
+            }
+            m.carry[i] = make_float2(m.friction * v.x, m.friction * v.y);
+        }
+        float nx = o.x + m.dt * v.x;
+        float ny = o.y + m.dt * v.y;
+        const float fX = static_cast<float>(X);
+        const float fY = static_cast<float>(Y);
+        const int gx = static_cast<int>(floorf(nx));
+        const int gy = static_cast<int>(floorf(ny));
+        const bool in = gx >= 0 && gx < X && gy >= 0 && gy < Y;
+        if (!in)
+        {
+            if (wrapX)
+            {
+                float md = fmodf(nx, fX);
+                nx = (md < 0.0f) ? (fX + md) : md;
+            }
+            else if (!(nx >= 0.0f && nx < fX))
+            {
+                nx = o.x;
+            }
+            if (wrapY)
+            {
+                float md = fmodf(ny, fY);
+                ny = (md < 0.0f) ? (fY + md) : md;
+            }
+            else if (!(ny >= 0.0f && ny < fY))
+            {
+                ny = o.y;
+            }
+        }
+        return make_float2(nx, ny);
+    }
+
     // One lane per slot k; writes the velocity delta of slot k's agent to
-    // vel[k] (slot order).  Slots are sorted by cell x*Y + y, so the cells
+    // vel[k] (slot order) and, with move.on, its moved position to
+    // move.posNext[k].  Slots are sorted by cell x*Y + y, so the cells
     // (cx, ya .. yb-1) of one neighbour row are the contiguous slot range
     // [offsets[cx*Y+ya], offsets[cx*Y+yb]); visiting the rows x-major and the
     // wrapped y segments in increasing ys keeps the reference's order.
@@ -348,7 +419,8 @@ namespace
     __global__ __launch_bounds__(kThreads) void AgentForces(int32_t N, int32_t X, int32_t Y, float R, float kSpring,
                                                             const int32_t *__restrict__ offsets,
                                                             const float2 *__restrict__ spos,
-                                                            const float *__restrict__ srad, float2 *__restrict__ vel)
+                                                            const float *__restrict__ srad, float2 *__restrict__ vel,
+                                                            MoveArgs move)
     {
         const int k = blockIdx.x * blockDim.x + threadIdx.x;
         if (k >= N)
@@ -388,6 +460,10 @@ namespace
                 AccumulateRange(rs[r], re[r], k, p, ri, R, kSpring, fX, fY, false, false, spos, srad, vx, vy);
             }
             vel[k] = make_float2(vx, vy);
+            if (move.on)
+            {
+                move.posNext[k] = MovedPosition(move, k, p, make_float2(vx, vy), X, Y, false, false);
+            }
             return;
         }
         for (int xs = x0; xs < x1; xs++)
@@ -423,74 +499,10 @@ namespace
             }
         }
         vel[k] = make_float2(vx, vy);
-    }
-
-    // p += dt*v with MoveSafePT: off-grid target -> wrap a wrapped axis, keep
-    // the old coordinate on a non-wrapped one.
-    // With a carried velocity (friction or a speed cap set), the agent's
-    // velocity is carry + this tick's force delta, as SphericalAgent2D
-    // accumulates SumForces into xVel/yVel (include/Hal/Agents/SphericalAgent2D.h:100-128);
-    // its norm is capped at maxVel when maxVel > 0 (CapVelocity, :139-150),
-    // the move uses it (ForceMove, :153-156), and friction * velocity is
-    // carried to the next tick (ApplyFriction, :133-137).
-    __global__ __launch_bounds__(kThreads) void AgentMove(int32_t N, int32_t X, int32_t Y, float dt, int32_t wrapX,
-                                                          int32_t wrapY, float2 *__restrict__ pos,
-                                                          const float2 *__restrict__ vel, float2 *__restrict__ carry,
-                                                          float friction, float maxVel)
-    {
-        const int i = blockIdx.x * blockDim.x + threadIdx.x;
-        if (i >= N)
+        if (move.on)
         {
-            return;
+            move.posNext[k] = MovedPosition(move, k, p, make_float2(vx, vy), X, Y, WX, WY);
         }
-        const float2 o = pos[i];
-        float2 v = vel[i];
-        if (carry != nullptr)
-        {
-            const float2 c = carry[i];
-            v.x += c.x;
-            v.y += c.y;
-            if (maxVel > 0.0f)
-            {
-                const float normSq = v.x * v.x + v.y * v.y;
-                if (normSq > maxVel * maxVel)
-                {
-                    const float f = maxVel / sqrtf(normSq);
-                    v.x *= f;
-                    v.y *= f;
-                }
-            }
-            carry[i] = make_float2(friction * v.x, friction * v.y);
-        }
-        float nx = o.x + dt * v.x;
-        float ny = o.y + dt * v.y;
-        const float fX = static_cast<float>(X);
-        const float fY = static_cast<float>(Y);
-        const int gx = static_cast<int>(floorf(nx));
-        const int gy = static_cast<int>(floorf(ny));
-        const bool in = gx >= 0 && gx < X && gy >= 0 && gy < Y;
-        if (!in)
-        {
-            if (wrapX)
-            {
-                float m = fmodf(nx, fX);
-                nx = (m < 0.0f) ? (fX + m) : m;
-            }
-            else if (!(nx >= 0.0f && nx < fX))
-            {
-                nx = o.x;
-            }
-            if (wrapY)
-            {
-                float m = fmodf(ny, fY);
-                ny = (m < 0.0f) ? (fY + m) : m;
-            }
-            else if (!(ny >= 0.0f && ny < fY))
-            {
-                ny = o.y;
-            }
-        }
-        pos[i] = make_float2(nx, ny);
     }
 
     unsigned Blocks(int64_t n, int threads)
@@ -515,6 +527,11 @@ struct halhip_agents2d
     DeviceArray<float> rad[2];
     int32_t cur = 0;
     DeviceArray<float2> vel;    // slot order of set `cur`
+    // Moved positions, slot order of set `cur`: after a tick the resident
+    // positions are here (moved == true) until the next binning scatters
+    // them into the other set.
+    DeviceArray<float2> posNext;
+    bool moved = false;
     // Carried velocity (slot order, per buffer set), kept while friction or
     // a speed cap is set; allocated on first use.
     DeviceArray<float2> carry[2];
@@ -551,6 +568,12 @@ namespace
         p->prof.End(p->clock.stream);
     }
 
+    // Resident positions in slot order of set `cur`.
+    float2 *CurPositions(halhip_agents2d *p)
+    {
+        return p->moved ? p->posNext.get() : p->pos[p->cur].get();
+    }
+
     // Bins of the resident positions: counts, offsets, and the agents moved
     // to the other buffer set in bucket order, which becomes the resident set.
     void EnqueueBins(halhip_agents2d *p)
@@ -558,10 +581,11 @@ namespace
         const int32_t N = p->N;
         const int c = p->cur;
         const int n = 1 - c;
+        const float2 *positions = CurPositions(p);
         if (N > 0)
         {
-            Launch(p, "agents_count", AgentCount, Blocks(N, kThreads), kThreads, N, p->X, p->Y,
-                   static_cast<const float2 *>(p->pos[c].get()), p->counts.get(), p->cellOf.get(), p->ticketOf.get());
+            Launch(p, "agents_count", AgentCount, Blocks(N, kThreads), kThreads, N, p->X, p->Y, positions,
+                   p->counts.get(), p->cellOf.get(), p->ticketOf.get());
         }
         int4 *counts4 = reinterpret_cast<int4 *>(p->counts.get());
         const int64_t cPad = static_cast<int64_t>(p->nTiles) * kScanTile;
@@ -576,7 +600,7 @@ namespace
             Launch(p, "agents_scatter", AgentScatter, Blocks(N, kThreads), kThreads, N,
                    static_cast<const int32_t *>(p->cellOf.get()), static_cast<const int32_t *>(p->ticketOf.get()),
                    static_cast<const int32_t *>(p->offsets.get()), static_cast<const int32_t *>(p->id[c].get()),
-                   static_cast<const float2 *>(p->pos[c].get()), static_cast<const float *>(p->rad[c].get()),
+                   positions, static_cast<const float *>(p->rad[c].get()),
                    static_cast<const float2 *>(p->motion ? p->carry[c].get() : nullptr), p->id[n].get(),
                    p->pos[n].get(), p->rad[n].get(), p->motion ? p->carry[n].get() : nullptr);
             Launch(p, "agents_order", AgentOrderCells, Blocks(p->C, kThreads), kThreads, p->C,
@@ -584,32 +608,37 @@ namespace
                    p->rad[n].get(), p->motion ? p->carry[n].get() : nullptr);
         }
         p->cur = n;
+        p->moved = false;
     }
 
-    void EnqueueForces(halhip_agents2d *p, float R, float k, int32_t wrapX, int32_t wrapY)
+    // Forces on the freshly binned set `cur`; with move, the moved positions
+    // go to posNext (the force pass reads every neighbour's old position).
+    void EnqueueForces(halhip_agents2d *p, float R, float k, int32_t wrapX, int32_t wrapY, bool move, float dt)
     {
         if (p->N == 0)
         {
             return;
         }
         const int c = p->cur;
+        MoveArgs m{};
+        m.on = move ? 1 : 0;
+        m.dt = dt;
+        m.posNext = p->posNext.get();
+        m.carry = p->motion ? p->carry[c].get() : nullptr;
+        m.friction = p->friction;
+        m.maxVel = p->maxVel;
         auto kernel = wrapX ? (wrapY ? AgentForces<true, true> : AgentForces<true, false>)
                             : (wrapY ? AgentForces<false, true> : AgentForces<false, false>);
         Launch(p, "agents_force", kernel, Blocks(p->N, kThreads), kThreads, p->N, p->X, p->Y, R, k,
                static_cast<const int32_t *>(p->offsets.get()), static_cast<const float2 *>(p->pos[c].get()),
-               static_cast<const float *>(p->rad[c].get()), p->vel.get());
+               static_cast<const float *>(p->rad[c].get()), p->vel.get(), m);
+        p->moved = move;
     }
 
     void EnqueueTick(halhip_agents2d *p, float R, float k, float dt, int32_t wrapX, int32_t wrapY)
     {
         EnqueueBins(p);
-        EnqueueForces(p, R, k, wrapX, wrapY);
-        if (p->N > 0)
-        {
-            Launch(p, "agents_move", AgentMove, Blocks(p->N, kThreads), kThreads, p->N, p->X, p->Y, dt,
-                   wrapX ? 1 : 0, wrapY ? 1 : 0, p->pos[p->cur].get(), static_cast<const float2 *>(p->vel.get()),
-                   p->motion ? p->carry[p->cur].get() : nullptr, p->friction, p->maxVel);
-        }
+        EnqueueForces(p, R, k, wrapX, wrapY, true, dt);
     }
 
     // Copies the resident agents in and resets slot k to agent k.
@@ -621,6 +650,7 @@ namespace
         }
         p->N = nAgents;
         p->binsValid = false;
+        p->moved = false;
         if (nAgents == 0)
         {
             return;
@@ -670,13 +700,17 @@ namespace
         if (ticks == 0)
         {
             EnqueueBins(p);
-            EnqueueForces(p, fR, fk, wrapX, wrapY);
+            EnqueueForces(p, fR, fk, wrapX, wrapY, false, 0.0f);
             p->binsValid = true;
             return;
         }
-        constexpr int32_t kChunk = 10; // even: a chunk leaves `cur` as it found it
+        // 10-tick graph chunks.  A chunk leaves `cur` as it found it (even
+        // length) and the positions moved; the first chunk after an upload or
+        // a forces-only pass starts from the binned set instead, so `moved`
+        // is part of the key and each chunk is looked up on its own.
+        constexpr int32_t kChunk = 10;
         int32_t done = 0;
-        if (!p->prof.On() && ticks >= kChunk)
+        while (!p->prof.On() && ticks - done >= kChunk)
         {
             std::string key;
             AppendKey(key, p->N);
@@ -689,9 +723,11 @@ namespace
             AppendKey(key, p->motion ? 1 : 0);
             AppendKey(key, p->friction);
             AppendKey(key, p->maxVel);
+            AppendKey(key, p->moved ? 1 : 0);
             hipGraphExec_t exec = p->graphs.Find(key);
             if (exec == nullptr)
             {
+                // capturing enqueues the ticks' host-side bookkeeping too
                 hipGraph_t graph;
                 Check(hipStreamBeginCapture(p->clock.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
                 for (int32_t t = 0; t < kChunk; t++)
@@ -703,10 +739,12 @@ namespace
                 (void)hipGraphDestroy(graph);
                 p->graphs.Insert(key, exec);
             }
-            for (; done + kChunk <= ticks; done += kChunk)
+            else
             {
-                Check(hipGraphLaunch(exec, p->clock.stream), "hipGraphLaunch");
+                p->moved = true;
             }
+            Check(hipGraphLaunch(exec, p->clock.stream), "hipGraphLaunch");
+            done += kChunk;
         }
         for (; done < ticks; done++)
         {
@@ -749,6 +787,7 @@ extern "C"
                     p->rad[b].Allocate(ctx, cap);
                 }
                 p->vel.Allocate(ctx, cap);
+                p->posNext.Allocate(ctx, cap);
                 p->outBuf.Allocate(ctx, cap);
                 p->cellOf.Allocate(ctx, cap);
                 p->ticketOf.Allocate(ctx, cap);
@@ -814,7 +853,7 @@ extern "C"
                 {
                     return;
                 }
-                DownloadByIndex(p, p->pos[p->cur].get(), positions);
+                DownloadByIndex(p, CurPositions(p), positions);
             });
     }
 
@@ -955,7 +994,8 @@ extern "C"
                 }
                 UploadAgents(p, p->hostBuf.data(), p->hostBuf.data() + 2 * n, nAgents);
                 EnqueueBins(p);
-                EnqueueForces(p, static_cast<float>(interactionRad), static_cast<float>(springConstant), wrapX, wrapY);
+                EnqueueForces(p, static_cast<float>(interactionRad), static_cast<float>(springConstant), wrapX, wrapY,
+                              false, 0.0f);
                 p->binsValid = true;
                 if (n > 0)
                 {

@@ -99,6 +99,30 @@ def test_ticks_vs_oracle_small(ctx, oracle, wrap):
     assert np.abs(d).max() < 1e-3, np.abs(d).max()
 
 
+def test_graph_chunks_from_fresh_upload_and_forces_only(ctx, oracle):
+    # The first graph chunk after an upload (or after a forces-only pass)
+    # starts from the binned positions, later chunks from the moved ones.
+    rng = np.random.default_rng(17)
+    X, Y, n = 48, 40, 1500
+    pos = (rng.random(2 * n) * np.tile([X, Y], n)).astype(np.float32)
+    rad = np.full(n, 0.5, np.float32)
+    plan = halhip.Agents2D(ctx, X, Y, n)
+    ref = oracle.agent_ticks(pos.astype(np.float64), rad.astype(np.float64), 25, 1.0, 1.0, 0.1, X, Y, 0, 1)
+    for _ in range(2):   # the second pass replays the cached graphs
+        plan.upload(pos, rad)
+        plan.run_ticks(25, 1.0, 1.0, 0.1, 0, 1)
+        d = plan.positions().astype(np.float64) - ref
+        d[1::2] = (d[1::2] + Y / 2) % Y - Y / 2
+        assert np.abs(d).max() < 1e-3
+    plan.upload(pos, rad)
+    plan.run_ticks(0, 1.0, 1.0, 0.0, 0, 1)   # bins and forces only
+    plan.run_ticks(20, 1.0, 1.0, 0.1, 0, 1)
+    ref20 = oracle.agent_ticks(pos.astype(np.float64), rad.astype(np.float64), 20, 1.0, 1.0, 0.1, X, Y, 0, 1)
+    d = plan.positions().astype(np.float64) - ref20
+    d[1::2] = (d[1::2] + Y / 2) % Y - Y / 2
+    assert np.abs(d).max() < 1e-3
+
+
 def test_bins_each_tick_match_oracle_on_gpu_positions(ctx, oracle):
     # Per tick the GPU bins are bit-exact against the reference binning of the
     # GPU's own FP32 positions at that tick.
