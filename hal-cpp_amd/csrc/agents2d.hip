@@ -356,6 +356,11 @@ namespace
         float2 *carry;   // carried velocities (null: none)
         float friction;
         float maxVel;
+        // the next binning's count of the moved positions (AgentCount's
+        // work): clamped cell and a ticket in it
+        int32_t *counts;
+        int32_t *cellOf;
+        int32_t *ticketOf;
     };
 
     __device__ __forceinline__ float2 MovedPosition(const MoveArgs &m, int i, float2 o, float2 v, int32_t X,
@@ -409,9 +414,20 @@ namespace
         return make_float2(nx, ny);
     }
 
+    // Moves slot k's agent and counts it into the next binning.
+    __device__ __forceinline__ void MoveAndCount(const MoveArgs &m, int k, float2 o, float2 v, int32_t X, int32_t Y,
+                                                 bool wrapX, bool wrapY)
+    {
+        const float2 q = MovedPosition(m, k, o, v, X, Y, wrapX, wrapY);
+        m.posNext[k] = q;
+        const int cell = ClampCell(q.x, X) * Y + ClampCell(q.y, Y);
+        m.cellOf[k] = cell;
+        m.ticketOf[k] = atomicAdd(&m.counts[cell], 1);
+    }
+
     // One lane per slot k; writes the velocity delta of slot k's agent to
     // vel[k] (slot order) and, with move.on, its moved position to
-    // move.posNext[k].  Slots are sorted by cell x*Y + y, so the cells
+    // move.posNext[k] and its count for the next binning.  Slots are sorted by cell x*Y + y, so the cells
     // (cx, ya .. yb-1) of one neighbour row are the contiguous slot range
     // [offsets[cx*Y+ya], offsets[cx*Y+yb]); visiting the rows x-major and the
     // wrapped y segments in increasing ys keeps the reference's order.
@@ -462,7 +478,7 @@ namespace
             vel[k] = make_float2(vx, vy);
             if (move.on)
             {
-                move.posNext[k] = MovedPosition(move, k, p, make_float2(vx, vy), X, Y, false, false);
+                MoveAndCount(move, k, p, make_float2(vx, vy), X, Y, false, false);
             }
             return;
         }
@@ -501,7 +517,7 @@ namespace
         vel[k] = make_float2(vx, vy);
         if (move.on)
         {
-            move.posNext[k] = MovedPosition(move, k, p, make_float2(vx, vy), X, Y, WX, WY);
+            MoveAndCount(move, k, p, make_float2(vx, vy), X, Y, WX, WY);
         }
     }
 
@@ -528,8 +544,9 @@ struct halhip_agents2d
     int32_t cur = 0;
     DeviceArray<float2> vel;    // slot order of set `cur`
     // Moved positions, slot order of set `cur`: after a tick the resident
-    // positions are here (moved == true) until the next binning scatters
-    // them into the other set.
+    // positions are here (moved == true), already counted into
+    // counts / cellOf / ticketOf, until the next binning scatters them into
+    // the other set.
     DeviceArray<float2> posNext;
     bool moved = false;
     // Carried velocity (slot order, per buffer set), kept while friction or
@@ -582,8 +599,9 @@ namespace
         const int c = p->cur;
         const int n = 1 - c;
         const float2 *positions = CurPositions(p);
-        if (N > 0)
+        if (N > 0 && !p->moved)
         {
+            // (moved positions were counted by the move itself)
             Launch(p, "agents_count", AgentCount, Blocks(N, kThreads), kThreads, N, p->X, p->Y, positions,
                    p->counts.get(), p->cellOf.get(), p->ticketOf.get());
         }
@@ -627,6 +645,9 @@ namespace
         m.carry = p->motion ? p->carry[c].get() : nullptr;
         m.friction = p->friction;
         m.maxVel = p->maxVel;
+        m.counts = p->counts.get();
+        m.cellOf = p->cellOf.get();
+        m.ticketOf = p->ticketOf.get();
         auto kernel = wrapX ? (wrapY ? AgentForces<true, true> : AgentForces<true, false>)
                             : (wrapY ? AgentForces<false, true> : AgentForces<false, false>);
         Launch(p, "agents_force", kernel, Blocks(p->N, kThreads), kThreads, p->N, p->X, p->Y, R, k,
@@ -647,6 +668,13 @@ namespace
         if (nAgents != p->N)
         {
             p->graphs.Clear();
+        }
+        if (p->moved)
+        {
+            // the last move counted its positions for a binning that will
+            // not happen: start the counts from zero again
+            const size_t cPad = static_cast<size_t>(p->nTiles) * kScanTile;
+            Check(hipMemsetAsync(p->counts.get(), 0, cPad * sizeof(int32_t), p->clock.stream), "hipMemsetAsync");
         }
         p->N = nAgents;
         p->binsValid = false;
