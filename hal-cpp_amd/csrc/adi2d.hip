@@ -585,8 +585,8 @@ namespace
         if (Qs <= 32)
         {
             auto kernel = (Qs <= 4) ? AdiStripScan<4> : ((Qs <= 16) ? AdiStripScan<16> : AdiStripScan<32>);
-            hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>((p->Y + kScanCols - 1) / kScanCols)),
-                               dim3(kScanCols * kScanSegs), 0, p->clock.stream, p->args, Qs);
+            hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>((p->Y + kScanColsWide - 1) / kScanColsWide)),
+                               dim3(kScanColsWide * kScanSegs), 0, p->clock.stream, p->args, Qs);
         }
         else
         {

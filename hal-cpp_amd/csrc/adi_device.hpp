@@ -1526,28 +1526,30 @@ namespace
     }
 
     // Scan of the strip reductions along each column (sweep A across strips).
-    // Block: 256 threads = 16 columns x 16 segments; lane group k of 16 lanes
-    // takes column y0 + (lane & 15) and segment g = 4*wave + (lane >> 4), which
-    // owns strips [g*Q, (g+1)*Q) with Q <= QMAX.  A thread first loads its
-    // segment's reduction values into registers (all loads in flight
-    // together), folds them, exchanges segment totals through LDS, then runs
-    // the recurrences from registers.  Writes the carry-in values of every
-    // strip and, per column, the periodic Sherman-Morrison factor and the
-    // wrapped rows of u*.
+    // Block: COLS columns x kScanSegs segments; thread t takes column
+    // y0 + (t % COLS) and segment g = t / COLS, which owns strips
+    // [g*Q, (g+1)*Q) with Q <= QMAX.  A thread first loads its segment's
+    // reduction values into registers (all loads in flight together), folds
+    // them, exchanges segment totals through LDS, then runs the recurrences
+    // from registers.  Writes the carry-in values of every strip and, per
+    // column, the periodic Sherman-Morrison factor and the wrapped rows of
+    // u*.  Wide grids use COLS = 32 (each row read as whole 128-byte lines);
+    // the batched wells use COLS = 16 (more blocks for narrow wells).
     constexpr int kScanCols = 16;
+    constexpr int kScanColsWide = 32;
     constexpr int kScanSegs = 16;
 
-    template <int QMAX>
+    template <int QMAX, int COLS = kScanCols>
     __device__ __forceinline__ void ScanBody(const AdiArgs &a, int yBase, int32_t Q)
     {
-        __shared__ float sB[kScanSegs][kScanCols];
+        __shared__ float sB[kScanSegs][COLS];
         __shared__ float sA[kScanSegs];
-        __shared__ float sB2[kScanSegs][kScanCols];
+        __shared__ float sB2[kScanSegs][COLS];
         __shared__ float sA2[kScanSegs];
-        __shared__ float sCf[kScanCols];
-        __shared__ float sX0[kScanCols];
-        const int col = threadIdx.x & (kScanCols - 1);
-        const int g = threadIdx.x / kScanCols;
+        __shared__ float sCf[COLS];
+        __shared__ float sX0[COLS];
+        const int col = threadIdx.x & (COLS - 1);
+        const int g = threadIdx.x / COLS;
         const int y = yBase + col;
         const bool active = y < a.Y;
         const int p0 = g * Q;
@@ -1653,9 +1655,9 @@ namespace
     }
 
     template <int QMAX>
-    __global__ __launch_bounds__(kScanCols *kScanSegs) void AdiStripScan(AdiArgs a, int32_t Q)
+    __global__ __launch_bounds__(kScanColsWide *kScanSegs) void AdiStripScan(AdiArgs a, int32_t Q)
     {
-        ScanBody<QMAX>(a, blockIdx.x * kScanCols, Q);
+        ScanBody<QMAX, kScanColsWide>(a, blockIdx.x * kScanColsWide, Q);
     }
 
     // Scan of the strip reductions for any strip count (reloads from memory).

@@ -152,15 +152,30 @@ namespace
 
     // Pass 3: offsets = tile prefix + in-tile exclusive prefix (offsets past
     // the last real cell read N), and the counts reset to 0 for the next
-    // binning.
+    // binning.  With fromSums the tile prefix is summed here from the
+    // unscanned tile totals (few tiles: pass 2 is skipped); otherwise
+    // tileOffsets holds pass 2's prefixes.
     __global__ __launch_bounds__(kScanThreads) void ScanApply(int4 *__restrict__ counts,
                                                               const int32_t *__restrict__ tileOffsets,
                                                               int4 *__restrict__ offsets, int32_t *__restrict__ offsetsTail,
-                                                              int32_t N)
+                                                              int32_t N, int32_t fromSums)
     {
         __shared__ int sWave[kScanThreads / 64];
         const int64_t tileBase = static_cast<int64_t>(blockIdx.x) * (kScanTile / 4);
-        int run = tileOffsets[blockIdx.x];
+        int run;
+        if (fromSums)
+        {
+            int part = 0;
+            for (int t = threadIdx.x; t < static_cast<int>(blockIdx.x); t += kScanThreads)
+            {
+                part += tileOffsets[t];
+            }
+            BlockExclusiveSum(part, sWave, run);
+        }
+        else
+        {
+            run = tileOffsets[blockIdx.x];
+        }
 #pragma unroll
         for (int r = 0; r < kScanRounds; r++)
         {
@@ -609,10 +624,16 @@ namespace
         const int64_t cPad = static_cast<int64_t>(p->nTiles) * kScanTile;
         Launch(p, "agents_scan_tiles", ScanTileSums, static_cast<unsigned>(p->nTiles), kScanThreads,
                static_cast<const int4 *>(counts4), p->tileSums.get());
-        Launch(p, "agents_scan_tile_offsets", ScanTileOffsets, 1u, 1024u, p->tileSums.get(), p->nTiles);
+        // up to kDirectTiles tiles, each block of pass 3 sums its own prefix
+        constexpr int32_t kDirectTiles = 4096;
+        const bool direct = p->nTiles <= kDirectTiles;
+        if (!direct)
+        {
+            Launch(p, "agents_scan_tile_offsets", ScanTileOffsets, 1u, 1024u, p->tileSums.get(), p->nTiles);
+        }
         Launch(p, "agents_scan_apply", ScanApply, static_cast<unsigned>(p->nTiles), kScanThreads, counts4,
                static_cast<const int32_t *>(p->tileSums.get()), reinterpret_cast<int4 *>(p->offsets.get()),
-               p->offsets.get() + cPad, N);
+               p->offsets.get() + cPad, N, direct ? 1 : 0);
         if (N > 0)
         {
             Launch(p, "agents_scatter", AgentScatter, Blocks(N, kThreads), kThreads, N,

@@ -99,6 +99,23 @@ def test_ticks_vs_oracle_small(ctx, oracle, wrap):
     assert np.abs(d).max() < 1e-3, np.abs(d).max()
 
 
+def test_bins_on_a_grid_of_more_than_4096_scan_tiles(ctx, oracle):
+    # Above 4096 scan tiles (16 M cells) the tile prefixes take a pass of
+    # their own; below it each block sums them itself.
+    rng = np.random.default_rng(23)
+    X, Y, n = 4200, 4100, 3000
+    pos = (rng.random(2 * n) * np.tile([X, Y], n)).astype(np.float32)
+    pos[:200] = np.tile(pos[:2], 100)            # 100 agents in one cell
+    rad = np.full(n, 0.5, np.float32)
+    plan = halhip.Agents2D(ctx, X, Y, n)
+    plan.upload(pos, rad)
+    plan.run_ticks(0, 1.0, 1.0, 0.0, 0, 0)
+    offsets, bucket = plan.bins()
+    ro, rb = oracle.agent_bins(pos.astype(np.float64), X, Y)
+    np.testing.assert_array_equal(offsets, ro)
+    np.testing.assert_array_equal(bucket, rb)
+
+
 def test_graph_chunks_from_fresh_upload_and_forces_only(ctx, oracle):
     # The first graph chunk after an upload (or after a forces-only pass)
     # starts from the binned positions, later chunks from the moved ones.
