@@ -232,6 +232,35 @@ namespace
         }
     }
 
+    // A thread's CPT columns as vectors with no column check: the caller
+    // guarantees y0 + CPT <= Y and y0 a multiple of CPT.
+    template <int CPT>
+    __device__ __forceinline__ void LoadRowFull(const float *__restrict__ row, int y0, float (&d)[CPT])
+    {
+        if constexpr (CPT % 4 == 0)
+        {
+#pragma unroll
+            for (int q = 0; q < CPT / 4; q++)
+            {
+                const float4 t = *reinterpret_cast<const float4 *>(row + y0 + 4 * q);
+                d[4 * q + 0] = t.x;
+                d[4 * q + 1] = t.y;
+                d[4 * q + 2] = t.z;
+                d[4 * q + 3] = t.w;
+            }
+        }
+        else if constexpr (CPT == 2)
+        {
+            const float2 t = *reinterpret_cast<const float2 *>(row + y0);
+            d[0] = t.x;
+            d[1] = t.y;
+        }
+        else
+        {
+            d[0] = row[y0];
+        }
+    }
+
     struct StripLds
     {
         float *left;  // [kMaxWaves][S] first column of each wavefront
@@ -866,6 +895,168 @@ namespace
         }
     }
 
+    // Windowed carries of a strip with K <= KW full strips on both sides
+    // (K <= p <= P-1-K): every load is issued before any is used
+    // (WindowLoads), then WindowFold composes them in the order
+    // WindowCarries does, so the results are the same bits.
+    // up: B rows p-K .. p-1; bd: B rows p .. p+K-1; wd: D rows p+1 .. p+K.
+    template <int CPT, bool VEC, int KW>
+    __device__ __forceinline__ void WindowLoads(const AdiArgs &a, int p, int y0, int K, float (&up)[KW][CPT],
+                                                float (&bd)[KW][CPT], float (&wd)[KW][CPT])
+    {
+        const size_t plane = static_cast<size_t>(a.P) * a.Y;
+#pragma unroll
+        for (int i = 0; i < KW; i++)
+        {
+            if (i < K)
+            {
+                LoadRow<CPT, VEC>(a.redIn + static_cast<size_t>(p - K + i) * a.Y, y0, a.Y, up[i]);
+                LoadRow<CPT, VEC>(a.redIn + static_cast<size_t>(p + i) * a.Y, y0, a.Y, bd[i]);
+                LoadRow<CPT, VEC>(a.redIn + plane + static_cast<size_t>(p + 1 + i) * a.Y, y0, a.Y, wd[i]);
+            }
+        }
+    }
+
+    template <int CPT, int KW>
+    __device__ __forceinline__ void WindowFold(const AdiArgs &a, int p, int K, const float (&up)[KW][CPT],
+                                               const float (&bd)[KW][CPT], const float (&wd)[KW][CPT],
+                                               float (&cf)[CPT], float (&xb)[CPT])
+    {
+        float F[CPT];
+        float X[CPT];
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            F[c] = 0.0f;
+            X[c] = 0.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < KW; i++)
+        {
+            if (i < K)
+            {
+                const float A = a.stripA[p - K + i];
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    F[c] = fmaf(A, F[c], up[i][c]);
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            cf[c] = F[c];
+        }
+        float coef = 1.0f;
+#pragma unroll
+        for (int i = 0; i < KW; i++)
+        {
+            if (i < K)
+            {
+                const int q = p + 1 + i;
+                const float A = a.stripA[q - 1];
+                const float Eq = a.stripE[q];
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    F[c] = fmaf(A, F[c], bd[i][c]);
+                    X[c] = fmaf(coef, fmaf(Eq, F[c], wd[i][c]), X[c]);
+                }
+                coef *= a.stripC[q];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            xb[c] = X[c];
+        }
+    }
+
+    // A full window: K (<= KW) strips on both sides of strip p.
+    template <int KW>
+    __device__ __forceinline__ bool FullWindow(const AdiArgs &a, int p, int K)
+    {
+        return KW > 0 && K <= KW && p >= K && p + K <= a.P - 1;
+    }
+
+    // Whether a full window's 3 KW rows of CPT values fit in registers
+    // beside the strip (larger windows fold strip by strip).
+    template <int CPT, int KW>
+    constexpr bool kWindowInRegisters = KW > 0 && 3 * KW * CPT <= 48;
+
+    // The strip's rows s0 .. s0+S-1 (rows past the grid read as 0).
+    template <int CPT, int S, bool VEC>
+    __device__ __forceinline__ void LoadStripRows(const AdiArgs &a, const float *__restrict__ u, int s0, int y0,
+                                                  float (&v)[S][CPT])
+    {
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            if (s0 + r < a.X)
+            {
+                LoadRow<CPT, VEC>(u + static_cast<size_t>(s0 + r) * a.Y, y0, a.Y, v[r]);
+            }
+            else
+            {
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    v[r][c] = 0.0f;
+                }
+            }
+        }
+    }
+
+    template <int CPT, int KW>
+    __device__ __forceinline__ void WindowFold(const AdiArgs &a, int p, const float (&wb)[2 * KW][CPT],
+                                               const float (&wd)[KW][CPT], float (&cf)[CPT], float (&xb)[CPT])
+    {
+        float F[CPT];
+        float X[CPT];
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            F[c] = 0.0f;
+            X[c] = 0.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < KW; i++)
+        {
+            const float A = a.stripA[p - KW + i];
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                F[c] = fmaf(A, F[c], wb[i][c]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            cf[c] = F[c];
+        }
+        float coef = 1.0f;
+#pragma unroll
+        for (int i = 0; i < KW; i++)
+        {
+            const int q = p + 1 + i;
+            const float A = a.stripA[q - 1];
+            const float Eq = a.stripE[q];
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                F[c] = fmaf(A, F[c], wb[KW + i][c]);
+                X[c] = fmaf(coef, fmaf(Eq, F[c], wd[i][c]), X[c]);
+            }
+            coef *= a.stripC[q];
+        }
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            xb[c] = X[c];
+        }
+    }
+
     // MODE 0: reduction of the current field only (first tick of a run).
     // MODE 1: plain tick in place, then the reduction of u' for the next tick.
     // MODE 2: tick with decay: u' = ADI(u) - k*u (no reduction; the next tick
@@ -895,30 +1086,36 @@ namespace
         const int y0 = tid * CPT;
 
         // Windowed carries first: their loads are in flight while the
-        // strip's rows load, and their arithmetic waits only for them.
+        // strip's rows load.  With a full window every load is issued before
+        // the fold waits for its own (loads complete in order); near the
+        // grid's ends the window is folded strip by strip.
         float cf[CPT];
         float dn[CPT];
-        if constexpr (KW > 0 && MODE != 0)
-        {
-            WindowCarries<CPT, VEC, KW>(a, p, y0, a.window, cf, dn);
-        }
-
         float v[S][CPT];
-#pragma unroll
-        for (int r = 0; r < S; r++)
+        if constexpr (KW > 0 && MODE != 0 && kWindowInRegisters<CPT, KW>)
         {
-            if (s0 + r < a.X)
+            if (FullWindow<KW>(a, p, a.window))
             {
-                LoadRow<CPT, VEC>(u + static_cast<size_t>(s0 + r) * a.Y, y0, a.Y, v[r]);
+                float wu[KW][CPT];
+                float wb[KW][CPT];
+                float wd[KW][CPT];
+                WindowLoads<CPT, VEC, KW>(a, p, y0, a.window, wu, wb, wd);
+                LoadStripRows<CPT, S, VEC>(a, u, s0, y0, v);
+                WindowFold<CPT, KW>(a, p, a.window, wu, wb, wd, cf, dn);
             }
             else
             {
-#pragma unroll
-                for (int c = 0; c < CPT; c++)
-                {
-                    v[r][c] = 0.0f;
-                }
+                WindowCarries<CPT, VEC, KW>(a, p, y0, a.window, cf, dn);
+                LoadStripRows<CPT, S, VEC>(a, u, s0, y0, v);
             }
+        }
+        else
+        {
+            if constexpr (KW > 0 && MODE != 0)
+            {
+                WindowCarries<CPT, VEC, KW>(a, p, y0, a.window, cf, dn);
+            }
+            LoadStripRows<CPT, S, VEC>(a, u, s0, y0, v);
         }
         float old[MODE == 3 ? S : 1][CPT];
         if constexpr (MODE == 3)
@@ -1230,31 +1427,57 @@ namespace
         const bool first = tid == 0;
         const bool last = tid == static_cast<int>(blockDim.x) - 1;
 
-        // The strip's rows are requested first so that their latency overlaps
-        // the carries' loads and arithmetic.
-        float v[S][CPT];
-#pragma unroll
-        for (int r = 0; r < S; r++)
-        {
-            if (s0 + r < a.X)
-            {
-                LoadRow<CPT, true>(u + static_cast<size_t>(s0 + r) * a.Y, y0, a.Y, v[r]);
-            }
-            else
-            {
-#pragma unroll
-                for (int c = 0; c < CPT; c++)
-                {
-                    v[r][c] = 0.0f;
-                }
-            }
-        }
+        // Interior strips with a full window request the carries' rows and
+        // then the strip's rows before waiting for any of them (loads
+        // complete in order, so the window fold waits only for its own).
+        // Rows are whole (yDim = threads * CPT); interior strips lie inside
+        // the grid.
         float cf[CPT];
         float up[CPT];
         float dn[CPT];
+        float v[S][CPT];
+        const bool fullWindow = kWindowInRegisters<CPT, KW> && FullWindow<KW>(a, p, K);
+        if constexpr (KW > 0 && !EDGE && kWindowInRegisters<CPT, KW>)
+        {
+            if (fullWindow)
+            {
+                float wu[KW][CPT];
+                float wb[KW][CPT];
+                float wd[KW][CPT];
+                WindowLoads<CPT, true, KW>(a, p, y0, K, wu, wb, wd);
+#pragma unroll
+                for (int r = 0; r < S; r++)
+                {
+                    LoadRowFull<CPT>(u + static_cast<size_t>(s0 + r) * a.Y, y0, v[r]);
+                }
+                WindowFold<CPT, KW>(a, p, K, wu, wb, wd, cf, dn);
+            }
+        }
+        if (!(KW > 0 && !EDGE && kWindowInRegisters<CPT, KW> && fullWindow))
+        {
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+                if (!EDGE || s0 + r < a.X)
+                {
+                    LoadRowFull<CPT>(u + static_cast<size_t>(s0 + r) * a.Y, y0, v[r]);
+                }
+                else
+                {
+#pragma unroll
+                    for (int c = 0; c < CPT; c++)
+                    {
+                        v[r][c] = 0.0f;
+                    }
+                }
+            }
+        }
         if constexpr (KW > 0)
         {
-            WindowCarries<CPT, true, KW>(a, p, y0, K, cf, dn);
+            if (EDGE || !fullWindow)
+            {
+                WindowCarries<CPT, true, KW>(a, p, y0, K, cf, dn);
+            }
         }
         else
         {
