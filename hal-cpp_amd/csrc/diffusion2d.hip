@@ -179,16 +179,6 @@ namespace
     constexpr int kMultiThreads = 1024;
     constexpr int kMultiRowsPerThread = 8; // rows of the held B + 2T one thread covers
 
-    __device__ __forceinline__ float EdgeDiffX(const ExplicitArgs &a, bool low, int y, float c)
-    {
-        if (a.dirichlet)
-        {
-            const float bc = low ? a.bcXlo[y] : a.bcXhi[y];
-            return (bc - c) * 2.0f;
-        }
-        return 0.0f;
-    }
-
     // Threads: thread `tid` works on column y = tid % Y of rows
     // tid / Y, tid / Y + R, ... (R = 1024 / Y rows per pass; Y <= 1024), so
     // its y neighbours and y-edge rules are fixed for the whole launch.

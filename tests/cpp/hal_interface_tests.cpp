@@ -363,6 +363,44 @@ namespace
         }
         Check(RelL2(walls.GetField(), ref) < 1e-5, "resident ADI / explicit ticks match the array adapters");
 
+        // The same ADI model ticks on a grid whose rows fill whole
+        // wavefronts (yDim a multiple of 64): decay and sources then run
+        // inside the LDS tick kernel.
+        {
+            const int wx = 96;
+            const int wy = 128;
+            std::vector<double> w0(static_cast<size_t>(wx * wy));
+            halhip_rng_doubles(11, wx * wy, w0.data());
+            Hal::Hip::HipPDEGrid2D wide(ctx, wx, wy, false, false);
+            wide.SetField(w0);
+            std::vector<double> wref = w0;
+            const int wsrc[3] = {0, 40 * wy + 77, wx * wy - 1};
+            for (int tick = 0; tick < 25; tick++)
+            {
+                wide.DiffusionADI(5.0, 0.0);
+                wide.MulAll(-2e-3);
+                for (int k = 0; k < 3; k++)
+                {
+                    wide.Add(wsrc[k], 0.5 * (k + 1));
+                }
+                wide.Update();
+                std::vector<double> d(wref.size(), 0.0);
+                Hal::Hip::DirichletBoundary zero;
+                zero.value = 0.0;
+                adi.DiffuseADI(wref, d, 5.0, wx, wy, false, false, zero);
+                for (size_t i = 0; i < wref.size(); i++)
+                {
+                    wref[i] += d[i] - 2e-3 * wref[i];
+                }
+                for (int k = 0; k < 3; k++)
+                {
+                    wref[static_cast<size_t>(wsrc[k])] += 0.5 * (k + 1);
+                }
+            }
+            Check(RelL2(wide.GetField(), wref) < 1e-5,
+                  "resident ADI model ticks on 64-multiple rows match the array adapter loop");
+        }
+
         // A tick with no calls commits nothing but counts.
         const std::vector<double> before = walls.GetField();
         walls.Update();
