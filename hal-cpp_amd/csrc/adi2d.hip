@@ -187,6 +187,8 @@ namespace
                 return PickLdsThreads<1, 8, 8>(threads, window, decay);
             case 20408:
                 return PickLdsThreads<2, 4, 8>(threads, window, decay);
+            case 10404:
+                return PickLdsThreads<1, 4, 4>(threads, window, decay);
             default:
                 return nullptr;
         }
