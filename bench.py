@@ -96,13 +96,15 @@ def bench_adi_model(ctx, w, ticks=200):
     return out
 
 
-def bench_explicit(ctx, ticks=1000, cpu=True):
+def bench_explicit(ctx, ticks=1000, cpu=True, shape=None):
     """Explicit 256^2 (workloads.explicit_256): r = 0.2, decay 1e-3, centre
     source 1.0, periodic, 1000 ticks resident; 8 B/cell-tick algorithmic."""
     import halhip
     import workloads
     w = workloads.explicit_256()
     p = halhip.Diffusion2D(ctx, w["X"], w["Y"])
+    if shape is not None:
+        p.set_multi_shape(*shape)
     p.set_sources(w["src_idx"], w["src_val"])
     p.upload(w["field"])
     p.run_steps(100, w["rate"], w["decay"], True, True)
@@ -113,6 +115,7 @@ def bench_explicit(ctx, ticks=1000, cpu=True):
     p.set_profiling(True)
     p.run_steps(100, w["rate"], w["decay"], True, True)
     prof = p.profile()
+    multi = p.multi_shape()
     p.close()
     name, (total_ms, launches) = max(prof.items(), key=lambda kv: kv[1][0])
     out = {"workload": "explicit 256^2, r=0.2, decay 1e-3, centre source 1.0, periodic, 1000 ticks",
@@ -120,6 +123,7 @@ def bench_explicit(ctx, ticks=1000, cpu=True):
            "Gcell_s": round(cells * ticks / (ms * 1e-3) / 1e9, 2), "kernel": name,
            "kernel_us_per_launch": round(total_ms / launches * 1e3, 3),
            "ticks_per_launch": round(100 / launches, 2),
+           "multi_shape": list(multi),
            "algorithmic_GBps": round(8 * cells * ticks / (ms * 1e-3) / 1e9, 1),
            "note": "256 KiB field stays in L2: several ticks per launch in LDS (temporal blocking); "
                    "8 B/cell-tick priced against HBM is not a bound here"}
@@ -304,6 +308,8 @@ def main():
     ap.add_argument("--methods", default=None,
                     help="ADI row solver and carries as ROWS,CARRIES with ROWS in auto|scan|lds and "
                          "CARRIES in auto|scan|window, instead of the automatic choice")
+    ap.add_argument("--explicit-shape", default=None,
+                    help="rows per workgroup and ticks per launch of the explicit companion as B,T")
     ap.add_argument("--companion", choices=["explicit", "agents", "batched", "adjoint"], default=None,
                     help="run only this companion workload and print its JSON (for profiling)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -330,7 +336,10 @@ def main():
         ctx = halhip.Context(0)
         run = {"explicit": bench_explicit, "agents": bench_agents, "batched": bench_batched,
                "adjoint": bench_adjoint}[args.companion]
-        print(json.dumps({args.companion: run(ctx, cpu=not args.no_cpu_baseline)}), flush=True)
+        kwargs = {}
+        if args.companion == "explicit" and args.explicit_shape:
+            kwargs["shape"] = tuple(int(v) for v in args.explicit_shape.split(","))
+        print(json.dumps({args.companion: run(ctx, cpu=not args.no_cpu_baseline, **kwargs)}), flush=True)
         ctx.close()
         return
 

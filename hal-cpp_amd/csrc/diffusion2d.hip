@@ -481,6 +481,8 @@ struct halhip_diffusion2d
     int32_t nSources = 0;
     double xVel = 0.0; // uniform advection of the resident ticks
     double yVel = 0.0;
+    int32_t multiB = 0; // pinned multi-tick shape (0: automatic)
+    int32_t multiT = 0;
     EdgeValues edges;
     std::string edgeKey;
     StreamClock clock;
@@ -560,25 +562,40 @@ namespace
 
     constexpr size_t kMultiLds = 64 * 1024; // dynamic LDS per workgroup
 
+    // Whether (B, T) fits the multi-tick kernel for rows of Y cells: the
+    // B + 2T held rows, twice, in kMultiLds bytes, at most
+    // kMultiRowsPerThread rows per thread and 64 held rows.
+    bool MultiFits(int32_t Y, const MultiShape &c)
+    {
+        if (Y > kMultiThreads || c.B < 1 || c.T < 1)
+        {
+            return false;
+        }
+        const int32_t rowsPerPass = kMultiThreads / Y;
+        const int32_t held = c.B + 2 * c.T;
+        const size_t bytes = 2 * static_cast<size_t>(held) * static_cast<size_t>(Y) * sizeof(float);
+        return held <= 64 && bytes <= kMultiLds && (held + rowsPerPass - 1) / rowsPerPass <= kMultiRowsPerThread;
+    }
+
     MultiShape ChooseMulti(const halhip_diffusion2d *p)
     {
-        MultiShape m;
-        if (p->Y > kMultiThreads)
+        if (p->multiB > 0)
         {
-            return m;
+            return MultiShape{p->multiB, p->multiT};
         }
-        // (B, T): 8 ticks over 4-row workgroups where the rows fit, else fewer
-        const int32_t rowsPerPass = kMultiThreads / p->Y;
-        for (const MultiShape c : {MultiShape{4, 8}, MultiShape{4, 4}, MultiShape{2, 2}})
+        // 8 ticks per launch over the tallest workgroups (B <= 4 rows) that
+        // still give one workgroup per CU: redundant halo work is (B + T) / B,
+        // and at 256^2 spreading over all 256 CUs wins (1 x 8: 1.85 us/tick,
+        // 4 x 8: 1.97); then fewer ticks where the rows do not fit.
+        const int32_t b = std::max(1, std::min(4, p->X / 256));
+        for (const MultiShape c : {MultiShape{b, 8}, MultiShape{4, 8}, MultiShape{4, 4}, MultiShape{2, 2}})
         {
-            const int32_t held = c.B + 2 * c.T;
-            const size_t bytes = 2 * static_cast<size_t>(held) * static_cast<size_t>(p->Y) * sizeof(float);
-            if (bytes <= kMultiLds && (held + rowsPerPass - 1) / rowsPerPass <= kMultiRowsPerThread)
+            if (MultiFits(p->Y, c))
             {
                 return c;
             }
         }
-        return m;
+        return MultiShape{};
     }
 
     void LaunchMulti(halhip_diffusion2d *p, const ExplicitArgs &a, const MultiShape &m, int32_t ticks,
@@ -964,6 +981,35 @@ extern "C"
                 {
                     deltas[k] += static_cast<double>(p->hostBuf[k]);
                 }
+            });
+    }
+
+    int halhip_diffusion2d_set_multi_shape(halhip_diffusion2d *p, int32_t rowsPerGroup, int32_t ticksPerLaunch)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_set_multi_shape");
+                const bool automatic = rowsPerGroup == 0 && ticksPerLaunch == 0;
+                Require(automatic || MultiFits(p->Y, MultiShape{rowsPerGroup, ticksPerLaunch}),
+                        "halhip_diffusion2d_set_multi_shape: the shape does not fit this grid's rows");
+                p->multiB = rowsPerGroup;
+                p->multiT = ticksPerLaunch;
+                p->graphs.Clear();
+            });
+    }
+
+    int halhip_diffusion2d_get_multi_shape(halhip_diffusion2d *p, int32_t *rowsPerGroup, int32_t *ticksPerLaunch)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_get_multi_shape");
+                Require(rowsPerGroup != nullptr && ticksPerLaunch != nullptr,
+                        "halhip_diffusion2d_get_multi_shape: null output");
+                const MultiShape m = ChooseMulti(p);
+                *rowsPerGroup = m.B;
+                *ticksPerLaunch = m.T;
             });
     }
 

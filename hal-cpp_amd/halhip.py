@@ -80,6 +80,8 @@ _SIGS = {
     "halhip_diffusion2d_load_grid": [_vp, ctypes.c_char_p, _ip, _ip, _ip],
     "halhip_adi2d_save_grid": [_vp, ctypes.c_char_p, _i32, _i32, _i32],
     "halhip_adi2d_load_grid": [_vp, ctypes.c_char_p, _ip, _ip, _ip],
+    "halhip_diffusion2d_set_multi_shape": [_vp, _i32, _i32],
+    "halhip_diffusion2d_get_multi_shape": [_vp, _ip, _ip],
     "halhip_diffusion2d_upload": [_vp, _fp, _i64],
     "halhip_diffusion2d_download": [_vp, _fp, _i64],
     "halhip_diffusion2d_set_sources": [_vp, _i32, _lp, _fp],
@@ -381,6 +383,15 @@ class Diffusion2D(_Plan):
                                             xVels.ctypes.data_as(_dp), yVels.ctypes.data_as(_dp), field.size,
                                             int(wrapX), int(wrapY), _bcp(s), _bcp(v)))
         return deltas
+
+    def set_multi_shape(self, rows_per_group=0, ticks_per_launch=0):
+        """Pin the multi-tick shape (0, 0: automatic)."""
+        _check(self._fn("set_multi_shape")(self.h, rows_per_group, ticks_per_launch))
+
+    def multi_shape(self):
+        b, t = ctypes.c_int32(), ctypes.c_int32()
+        _check(self._fn("get_multi_shape")(self.h, ctypes.byref(b), ctypes.byref(t)))
+        return b.value, t.value
 
     def set_advection(self, xVel, yVel):
         _check(self._fn("set_advection")(self.h, xVel, yVel))

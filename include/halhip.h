@@ -149,6 +149,15 @@ int halhip_diffusion2d_advect_cells_f64(halhip_diffusion2d *plan, const double *
  * PDEGrid2D::CheckAdvectionStable (include/Hal/Grids/PDEGrid2D.h:444-451). */
 int halhip_diffusion2d_set_advection(halhip_diffusion2d *plan, double xVel, double yVel);
 
+/* Resident ticks of grids with yDim <= 1024 run several ticks per launch
+ * (temporal blocking): each workgroup owns rowsPerGroup rows and advances
+ * ticksPerLaunch ticks in LDS on rowsPerGroup + 2 ticksPerLaunch rows.
+ * set_multi_shape pins (rowsPerGroup, ticksPerLaunch) for tests and sweeps
+ * (0, 0: automatic); results do not depend on it.  get_multi_shape reports
+ * the shape in use ((0, 0): one launch per tick). */
+int halhip_diffusion2d_set_multi_shape(halhip_diffusion2d *plan, int32_t rowsPerGroup, int32_t ticksPerLaunch);
+int halhip_diffusion2d_get_multi_shape(halhip_diffusion2d *plan, int32_t *rowsPerGroup, int32_t *ticksPerLaunch);
+
 int halhip_diffusion2d_upload(halhip_diffusion2d *plan, const float *field, int64_t n);
 int halhip_diffusion2d_download(halhip_diffusion2d *plan, float *field, int64_t n);
 /* The resident field saved as / loaded from a HAL PDEGrid2D file

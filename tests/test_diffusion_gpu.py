@@ -134,3 +134,32 @@ def test_multi_tick_launches_match_oracle(ctx, oracle, X, Y, wx, wy, bc, nsrc, t
     got = plan.download()
     ref = _explicit_ref(oracle, f, ticks, 0.2, 1e-3, X, Y, wx, wy, bc, idx, val)
     assert rel_l2(got, ref) < 1e-6, (X, Y, rel_l2(got, ref))
+
+
+def test_multi_tick_shapes_are_bit_identical(ctx, oracle):
+    # Every (rows per workgroup, ticks per launch) gives the same bits as one
+    # launch per tick, and matches the FP64 oracle.
+    X, Y, ticks = 40, 256, 37
+    f = np.random.default_rng(12).random(X * Y).astype(np.float32)
+    idx = np.array([5 * Y + 9, 39 * Y + 255], np.int64)
+    val = np.array([1.0, 0.5], np.float32)
+    outs = {}
+    for shape in ((0, 0), (1, 8), (2, 15), (8, 8), (4, 12), (1, 1)):
+        p = halhip.Diffusion2D(ctx, X, Y)
+        p.set_multi_shape(*shape)
+        if shape != (0, 0):
+            assert p.multi_shape() == shape
+        p.set_sources(idx, val)
+        p.upload(f)
+        p.run_steps(ticks, 0.2, 1e-3, 1, 0, bc=None)
+        outs[shape] = p.download()
+        p.close()
+    for shape, got in outs.items():
+        np.testing.assert_array_equal(got, outs[(0, 0)], err_msg=str(shape))
+    ref = oracle.explicit_ticks(f.astype(np.float64), ticks, 0.2, 1e-3, X, Y, 1, 0, src_idx=idx,
+                                src_val=val.astype(np.float64))
+    assert rel_l2(outs[(0, 0)], ref) < 1e-5
+    p = halhip.Diffusion2D(ctx, X, Y)
+    with pytest.raises(halhip.InvalidArgument):
+        p.set_multi_shape(4, 40)      # 84 rows held: more than fit
+    p.close()
