@@ -326,41 +326,37 @@ namespace
     }
 
     // out[c][r] = in[r][c] for an R x C matrix (R, C multiples of 4) through
-    // 64 x 64 LDS tiles, reading and writing 16-byte vectors.
+    // 32 x 32 LDS tiles, one 16-byte vector read and written per thread
+    // (many small tiles keep every CU busy on a 1024^2 field).
     __global__ __launch_bounds__(256) void Transpose4(const float *__restrict__ in, float *__restrict__ out, int32_t R,
                                                       int32_t C)
     {
-        __shared__ float tile[64][65];
-        const int c0 = blockIdx.x * 64;
-        const int r0 = blockIdx.y * 64;
-        const int q = threadIdx.x & 15;  // 4-column group
-        const int rr = threadIdx.x >> 4; // 0..15
-#pragma unroll
-        for (int k = 0; k < 4; k++)
+        __shared__ float tile[32][33];
+        const int c0 = blockIdx.x * 32;
+        const int r0 = blockIdx.y * 32;
+        const int q = threadIdx.x & 7;   // 4-column group
+        const int rr = threadIdx.x >> 3; // 0..31
         {
-            const int r = r0 + rr + 16 * k;
+            const int r = r0 + rr;
             const int c = c0 + 4 * q;
             float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (r < R && c < C)
             {
                 v = *reinterpret_cast<const float4 *>(in + static_cast<size_t>(r) * C + c);
             }
-            tile[rr + 16 * k][4 * q] = v.x;
-            tile[rr + 16 * k][4 * q + 1] = v.y;
-            tile[rr + 16 * k][4 * q + 2] = v.z;
-            tile[rr + 16 * k][4 * q + 3] = v.w;
+            tile[rr][4 * q] = v.x;
+            tile[rr][4 * q + 1] = v.y;
+            tile[rr][4 * q + 2] = v.z;
+            tile[rr][4 * q + 3] = v.w;
         }
         __syncthreads();
-#pragma unroll
-        for (int k = 0; k < 4; k++)
         {
-            const int c = c0 + rr + 16 * k; // output row
+            const int c = c0 + rr; // output row
             const int r = r0 + 4 * q;
             if (c < C && r < R)
             {
-                const int tc = rr + 16 * k;
                 *reinterpret_cast<float4 *>(out + static_cast<size_t>(c) * R + r) =
-                    make_float4(tile[4 * q][tc], tile[4 * q + 1][tc], tile[4 * q + 2][tc], tile[4 * q + 3][tc]);
+                    make_float4(tile[4 * q][rr], tile[4 * q + 1][rr], tile[4 * q + 2][rr], tile[4 * q + 3][rr]);
             }
         }
     }
@@ -618,13 +614,14 @@ namespace
     void LaunchTranspose(halhip_adjoint2d *p, const float *in, float *out, int32_t R, int32_t C)
     {
         p->prof.Begin(p->clock.stream, "adjoint_transpose");
-        const dim3 grid(static_cast<unsigned>((C + 63) / 64), static_cast<unsigned>((R + 63) / 64));
         if (R % 4 == 0 && C % 4 == 0)
         {
+            const dim3 grid(static_cast<unsigned>((C + 31) / 32), static_cast<unsigned>((R + 31) / 32));
             hipLaunchKernelGGL(Transpose4, grid, dim3(256), 0, p->clock.stream, in, out, R, C);
         }
         else
         {
+            const dim3 grid(static_cast<unsigned>((C + 63) / 64), static_cast<unsigned>((R + 63) / 64));
             hipLaunchKernelGGL(Transpose, grid, dim3(256), 0, p->clock.stream, in, out, R, C);
         }
         Check(hipGetLastError(), "Transpose launch");
