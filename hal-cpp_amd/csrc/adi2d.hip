@@ -583,12 +583,13 @@ namespace
     void LaunchScan(halhip_adi2d *p)
     {
         p->prof.Begin(p->clock.stream, "adi_strip_scan");
-        const int32_t Qs = (p->P + kScanSegs - 1) / kScanSegs;
-        if (Qs <= 32)
+        // up to 16 strips per thread in registers (1024 strips); more reload
+        const int32_t Qs = (p->P + kScanSegsWide - 1) / kScanSegsWide;
+        if (Qs <= 16)
         {
-            auto kernel = (Qs <= 4) ? AdiStripScan<4> : ((Qs <= 16) ? AdiStripScan<16> : AdiStripScan<32>);
-            hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>((p->Y + kScanColsWide - 1) / kScanColsWide)),
-                               dim3(kScanColsWide * kScanSegs), 0, p->clock.stream, p->args, Qs);
+            auto kernel = (Qs <= 4) ? AdiStripScan<4> : ((Qs <= 8) ? AdiStripScan<8> : AdiStripScan<16>);
+            hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>((p->Y + kScanCols - 1) / kScanCols)),
+                               dim3(kScanCols * kScanSegsWide), 0, p->clock.stream, p->args, Qs);
         }
         else
         {

@@ -1749,26 +1749,27 @@ namespace
     }
 
     // Scan of the strip reductions along each column (sweep A across strips).
-    // Block: COLS columns x kScanSegs segments; thread t takes column
+    // Block: COLS columns x SEGS segments; thread t takes column
     // y0 + (t % COLS) and segment g = t / COLS, which owns strips
     // [g*Q, (g+1)*Q) with Q <= QMAX.  A thread first loads its segment's
     // reduction values into registers (all loads in flight together), folds
     // them, exchanges segment totals through LDS, then runs the recurrences
     // from registers.  Writes the carry-in values of every strip and, per
     // column, the periodic Sherman-Morrison factor and the wrapped rows of
-    // u*.  Wide grids use COLS = 32 (each row read as whole 128-byte lines);
-    // the batched wells use COLS = 16 (more blocks for narrow wells).
+    // u*.
     constexpr int kScanCols = 16;
-    constexpr int kScanColsWide = 32;
     constexpr int kScanSegs = 16;
+    // the single grid's scan: 16 columns x 64 segments (1024 threads), so a
+    // 4096-wide grid fills 256 workgroups and each thread walks P/64 strips
+    constexpr int kScanSegsWide = 64;
 
-    template <int QMAX, int COLS = kScanCols>
+    template <int QMAX, int COLS = kScanCols, int SEGS = kScanSegs>
     __device__ __forceinline__ void ScanBody(const AdiArgs &a, int yBase, int32_t Q)
     {
-        __shared__ float sB[kScanSegs][COLS];
-        __shared__ float sA[kScanSegs];
-        __shared__ float sB2[kScanSegs][COLS];
-        __shared__ float sA2[kScanSegs];
+        __shared__ float sB[SEGS][COLS];
+        __shared__ float sA[SEGS];
+        __shared__ float sB2[SEGS][COLS];
+        __shared__ float sA2[SEGS];
         __shared__ float sCf[COLS];
         __shared__ float sX0[COLS];
         const int col = threadIdx.x & (COLS - 1);
@@ -1843,7 +1844,7 @@ namespace
         }
         __syncthreads();
         float xb = 0.0f;
-        for (int w = kScanSegs - 1; w > g; w--)
+        for (int w = SEGS - 1; w > g; w--)
         {
             xb = fmaf(sA2[w], xb, sB2[w][col]);
         }
@@ -1878,9 +1879,9 @@ namespace
     }
 
     template <int QMAX>
-    __global__ __launch_bounds__(kScanColsWide *kScanSegs) void AdiStripScan(AdiArgs a, int32_t Q)
+    __global__ __launch_bounds__(kScanCols *kScanSegsWide) void AdiStripScan(AdiArgs a, int32_t Q)
     {
-        ScanBody<QMAX, kScanColsWide>(a, blockIdx.x * kScanColsWide, Q);
+        ScanBody<QMAX, kScanCols, kScanSegsWide>(a, blockIdx.x * kScanCols, Q);
     }
 
     // Scan of the strip reductions for any strip count (reloads from memory).

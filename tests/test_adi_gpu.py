@@ -304,6 +304,20 @@ def test_decay_and_sources_every_method_match_oracle(ctx, oracle, X, Y, shape, r
     assert (halhip.ADI_ROWS_LDS, halhip.ADI_CARRIES_SCAN) in used
 
 
+@pytest.mark.parametrize("X,wx", [(9000, 1), (9000, 0), (3000, 1)])
+def test_tall_grids_scan_strip_counts(ctx, oracle, X, wx):
+    # More than 1024 strips: the strip scan reloads its values instead of
+    # keeping 16 per thread (9000 rows); 3000 rows keep them in registers.
+    Y = 64
+    f = np.random.default_rng(X + wx).random(X * Y).astype(np.float32)
+    plan = halhip.Adi2D(ctx, X, Y)
+    plan.set_methods(halhip.ADI_ROWS_WORKGROUP_SCAN, halhip.ADI_CARRIES_SCAN)
+    plan.upload(f)
+    plan.run_steps(3, 5.0, 0.0, wx, 1, bc=None)
+    ref = adi_ref_ticks(oracle, f, 3, 5.0, X, Y, wx, 1)
+    assert rel_l2(plan.download(), ref) < 1e-5
+
+
 def test_plain_and_decay_runs_chain(ctx, oracle):
     # The decay ticks leave the reduction of their result for a following
     # plain run, and the other way round.
