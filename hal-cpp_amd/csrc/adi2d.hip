@@ -227,7 +227,11 @@ namespace
         size_t oPw = 0, oLane = 0, oChunk = 0, oPowE = 0;
     };
 
-    RowTables BuildRowTables(int32_t n, int32_t E, int32_t W, double rate, bool dirichlet)
+    // periodic: the cyclic line (TDMAperiodic's matrix, include/Hal/Pde/TdmaSolver.h:68-120):
+    // diagonal 1 + r everywhere and corners -r/2, i.e. T + U C U^T with
+    // C = [[1 + r - beta, -r/2], [-r/2, 0]]; otherwise C is diagonal with the
+    // end coefficients of ADISolveRow.
+    RowTables BuildRowTables(int32_t n, int32_t E, int32_t W, double rate, bool dirichlet, bool periodic)
     {
         RowTables t;
         const double hr = rate / 2;
@@ -235,17 +239,26 @@ namespace
         const double phi = hr / beta;
         const double ib = 1 / beta;
         const double bEnd = dirichlet ? 1 + 1.5 * rate : 1 + 0.5 * rate;
-        const double d0 = bEnd - beta;
-        const double d1 = bEnd - (1 + rate);
+        double c00 = bEnd - beta;
+        double c01 = 0.0;
+        double c11 = bEnd - (1 + rate);
+        if (periodic)
+        {
+            c00 = (1 + rate) - beta;
+            c01 = -hr;
+            c11 = 0.0;
+        }
+        const double c10 = c01;
         auto pw = [&](double e) { return std::pow(phi, e); };
-        // G = U^T T^-1 U (symmetric), Kmat = (I + diag(d) G)^-1 diag(d)
+        // G = U^T T^-1 U (symmetric), Kmat = (I + C G)^-1 C
         const double g00 = ib * (1 - pw(2.0 * n)) / (1 - phi * phi);
         const double g01 = ib * pw(n - 1.0);
         const double g11 = ib;
-        const double m00 = 1 + d0 * g00, m01 = d0 * g01, m10 = d1 * g01, m11 = 1 + d1 * g11;
+        const double m00 = 1 + c00 * g00 + c01 * g01, m01 = c00 * g01 + c01 * g11;
+        const double m10 = c10 * g00 + c11 * g01, m11 = 1 + c10 * g01 + c11 * g11;
         const double det = m00 * m11 - m01 * m10;
-        const double k00 = m11 * d0 / det, k01 = -m01 * d1 / det;
-        const double k10 = -m10 * d0 / det, k11 = m00 * d1 / det;
+        const double k00 = (m11 * c00 - m01 * c10) / det, k01 = (m11 * c01 - m01 * c11) / det;
+        const double k10 = (m00 * c10 - m10 * c00) / det, k11 = (m00 * c11 - m10 * c01) / det;
         // (alpha, gamma) = Kmat ib (X_0, X_{n-1}); h = alpha / (1 - phi^2);
         // t = gamma - h phi^(n+1)
         const double q00 = k00 * ib / (1 - phi * phi), q01 = k01 * ib / (1 - phi * phi);
@@ -476,7 +489,7 @@ namespace
         size_t oXlo = putf(e.xlo), oXhi = putf(e.xhi), oYlo = putf(e.ylo), oYhi = putf(e.yhi);
 
         // Row solver and carries for plain ticks.
-        const int32_t ldsE = (wrapY || p->wantRow == HALHIP_ADI_ROWS_WORKGROUP_SCAN) ? 0 : LdsElements(p->shape, p->Y);
+        const int32_t ldsE = (p->wantRow == HALHIP_ADI_ROWS_WORKGROUP_SCAN) ? 0 : LdsElements(p->shape, p->Y);
         int32_t K = (ldsE == 0 || wrapX || p->wantCarry == HALHIP_ADI_CARRIES_SCAN) ? 0 : WindowSize(sA, sC);
         if (K > 0 && PickLds(p->shape.cpt, p->shape.s, ldsE, K, p->shape.threads) == nullptr)
         {
@@ -486,7 +499,7 @@ namespace
         RowTables rt;
         if (ldsE > 0)
         {
-            rt = BuildRowTables(p->Y, ldsE, p->shape.threads / 64 / p->shape.s, rate, dir);
+            rt = BuildRowTables(p->Y, ldsE, p->shape.threads / 64 / p->shape.s, rate, dir && !wrapY, wrapY != 0);
             oRow = put(rt.data);
         }
 

@@ -1267,10 +1267,14 @@ namespace
     __device__ __forceinline__ void RhsAFromLds(const AdiArgs &a, const float *rows, int rowStride, int s0, int y0,
                                                 bool first, bool last, float (&v)[S][CPT])
     {
-        const int yl = (y0 > 0) ? y0 - 1 : 0;
-        const int yr = (y0 + CPT < a.Y) ? y0 + CPT : a.Y - 1;
+        // y neighbours of the thread's columns; on a periodic y axis the
+        // row's far end (the whole row is in LDS)
+        const int yl = (y0 > 0) ? y0 - 1 : (a.wrapY ? a.Y - 1 : 0);
+        const int yr = (y0 + CPT < a.Y) ? y0 + CPT : (a.wrapY ? 0 : a.Y - 1);
         const int al = RowAddr<E>(yl);
         const int ar = RowAddr<E>(yr);
+        const bool ghostL = first && !a.wrapY;
+        const bool ghostR = last && !a.wrapY;
 #pragma unroll
         for (int r = 0; r < S; r++)
         {
@@ -1281,8 +1285,8 @@ namespace
             const float cl = v[r][CPT - 1];
             const float gl = a.dirichlet ? fmaf(2.0f, a.bcYlo[x], -c0) : c0;
             const float gr = a.dirichlet ? fmaf(2.0f, a.bcYhi[x], -cl) : cl;
-            L = first ? gl : L;
-            R = last ? gr : R;
+            L = ghostL ? gl : L;
+            R = ghostR ? gr : R;
             // (nxt - cur) + (prev - cur) = e_c - e_{c-1} with e_c = nxt - cur
             float eLeft = v[r][0] - L;
 #pragma unroll
@@ -1348,8 +1352,8 @@ namespace
         for (int r = 0; r < S; r++)
         {
             const int x = s0 + r;
-            const float srcLo = (a.dirichlet && first) ? a.bcYlo[x] * a.r : 0.0f;
-            const float srcHi = (a.dirichlet && last) ? a.bcYhi[x] * a.r : 0.0f;
+            const float srcLo = (a.dirichlet && !a.wrapY && first) ? a.bcYlo[x] * a.r : 0.0f;
+            const float srcHi = (a.dirichlet && !a.wrapY && last) ? a.bcYhi[x] * a.r : 0.0f;
 #pragma unroll
             for (int c = 0; c < CPT; c++)
             {

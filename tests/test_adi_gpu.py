@@ -265,6 +265,40 @@ def test_methods_match_oracle(ctx, oracle, X, Y, shape, rate, wx, bc):
         assert (halhip.ADI_ROWS_LDS, halhip.ADI_CARRIES_WINDOW) in used
 
 
+PERIODIC_Y_CASES = [
+    # X, Y, (cpt, rows), rate, wrapX, bc (Dirichlet applies to x only)
+    (300, 1024, (1, 8), 5.0, 0, 0.2),
+    (200, 4096, (4, 8), 5.0, 0, None),
+    (128, 4096, (4, 8), 5.0, 1, None),
+    (61, 4096, (4, 4), 50.0, 0, 0.5),
+    (100, 4096, (8, 4), 0.5, 1, None),
+    (96, 512, (1, 4), 5.0, 0, None),
+]
+
+
+@pytest.mark.parametrize("X,Y,shape,rate,wx,bc", PERIODIC_Y_CASES)
+def test_periodic_y_lds_rows_match_oracle(ctx, oracle, X, Y, shape, rate, wx, bc):
+    # A periodic y axis solves each row as the cyclic system of TDMAperiodic
+    # in LDS (a rank-2 correction with off-diagonal corners), against the
+    # register solve and the FP64 oracle.
+    f = np.random.default_rng(X * 7 + Y).random(X * Y).astype(np.float32)
+    ref = adi_ref_ticks(oracle, f, 6, rate, X, Y, wx, 1, bc)
+    plan = halhip.Adi2D(ctx, X, Y)
+    plan.set_strip_shape(shape[0], shape[1], True)
+    used = set()
+    for rows, carries in ((halhip.ADI_ROWS_WORKGROUP_SCAN, halhip.ADI_CARRIES_SCAN),
+                          (halhip.ADI_ROWS_LDS, halhip.ADI_CARRIES_SCAN),
+                          (halhip.ADI_ROWS_LDS, halhip.ADI_CARRIES_WINDOW)):
+        plan.set_methods(rows, carries)
+        plan.upload(f)
+        plan.run_steps(6, rate, 0.0, wx, 1, bc=bc)
+        m = plan.methods()
+        used.add(m[0])
+        err = rel_l2(plan.download(), ref)
+        assert err < 1e-5, (rows, carries, m, err)
+    assert halhip.ADI_ROWS_LDS in used
+
+
 DECAY_CASES = [
     # X, Y, (cpt, rows), rate, wrapX, bc
     (300, 1024, (1, 8), 5.0, 0, 0.2),
