@@ -240,12 +240,13 @@ int halhip_adi2d_sync(halhip_adi2d *plan, float *elapsedMs);
 int halhip_adi2d_set_strip_shape(halhip_adi2d *plan, int32_t columnsPerThread, int32_t stripRows, int32_t vector);
 int halhip_adi2d_get_strip_shape(halhip_adi2d *plan, int32_t *columnsPerThread, int32_t *stripRows, int32_t *vector);
 
-/* Methods of the plain ADI tick (no decay, no point sources), for testing
- * and benchmarking; both are HALHIP_ADI_AUTO by default.
+/* Methods of the resident ADI tick, for testing and benchmarking; both are
+ * HALHIP_ADI_AUTO by default.
  *   rowSolver: HALHIP_ADI_ROWS_WORKGROUP_SCAN solves each contiguous line as a
  *     scan across the workgroup's registers; HALHIP_ADI_ROWS_LDS stages the
  *     strip in LDS and solves each line with constant Thomas factors plus an
- *     exact rank-2 end correction (non-periodic yDim, supported shapes only).
+ *     exact rank-2 correction for its ends (or, on a periodic y axis, its
+ *     corners); supported shapes only.
  *   carries: HALHIP_ADI_CARRIES_SCAN runs a cross-strip scan kernel per tick;
  *     HALHIP_ADI_CARRIES_WINDOW composes each strip's carries from the K
  *     strips on either side when the line factors decay fast enough that the
