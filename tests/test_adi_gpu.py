@@ -352,6 +352,23 @@ def test_tall_grids_scan_strip_counts(ctx, oracle, X, wx):
     assert rel_l2(plan.download(), ref) < 1e-5
 
 
+@pytest.mark.parametrize("wx", [0, 1])
+def test_periodic_y_model_ticks(ctx, oracle, wx):
+    # Decay and point sources in the LDS tick on a periodic y axis.
+    X, Y = 120, 2048
+    f = np.random.default_rng(31 + wx).random(X * Y).astype(np.float32)
+    idx = [0, Y - 1, 7 * Y + 100, 7 * Y + 100, X * Y - 1]
+    val = [1.0, 0.5, 0.25, 0.25, 2.0]
+    plan = halhip.Adi2D(ctx, X, Y)
+    plan.set_sources(idx, val)
+    plan.upload(f)
+    for n in (1, 20, 4):
+        plan.run_steps(n, 5.0, 3e-3, wx, 1, bc=None)
+    assert plan.methods()[0] == halhip.ADI_ROWS_LDS
+    ref = adi_ref_ticks(oracle, f, 25, 5.0, X, Y, wx, 1, None, 3e-3, idx, val)
+    assert rel_l2(plan.download(), ref) < 1e-5
+
+
 def test_plain_and_decay_runs_chain(ctx, oracle):
     # The decay ticks leave the reduction of their result for a following
     # plain run, and the other way round.
