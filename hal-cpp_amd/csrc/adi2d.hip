@@ -146,49 +146,54 @@ namespace
     using LdsKernel = void (*)(AdiArgs, float *, int32_t, RowSolve);
 
     template <int CPT, int S, int E, int T, bool DECAY>
-    LdsKernel PickLdsWindow(int32_t K)
+    LdsKernel PickLdsWindow(int32_t K, bool px)
     {
         if (K == 0)
         {
             return AdiStripLds<CPT, S, E, 0, T, DECAY>;
         }
+        if (px)
+        {
+            return (K <= 4) ? AdiStripLds<CPT, S, E, 4, T, DECAY, true>
+                            : AdiStripLds<CPT, S, E, kMaxWindow, T, DECAY, true>;
+        }
         return (K <= 4) ? AdiStripLds<CPT, S, E, 4, T, DECAY> : AdiStripLds<CPT, S, E, kMaxWindow, T, DECAY>;
     }
 
     template <int CPT, int S, int E>
-    LdsKernel PickLdsThreads(int threads, int32_t K, bool decay)
+    LdsKernel PickLdsThreads(int threads, int32_t K, bool decay, bool px)
     {
         if (decay)
         {
-            return threads == 1024 ? PickLdsWindow<CPT, S, E, 1024, true>(K)
-                                   : (threads == 512 ? PickLdsWindow<CPT, S, E, 512, true>(K) : nullptr);
+            return threads == 1024 ? PickLdsWindow<CPT, S, E, 1024, true>(K, px)
+                                   : (threads == 512 ? PickLdsWindow<CPT, S, E, 512, true>(K, px) : nullptr);
         }
-        return threads == 1024 ? PickLdsWindow<CPT, S, E, 1024, false>(K)
-                               : (threads == 512 ? PickLdsWindow<CPT, S, E, 512, false>(K) : nullptr);
+        return threads == 1024 ? PickLdsWindow<CPT, S, E, 1024, false>(K, px)
+                               : (threads == 512 ? PickLdsWindow<CPT, S, E, 512, false>(K, px) : nullptr);
     }
 
     // Kernel of the LDS row solve for a strip shape, E and workgroup size
     // (512 or 1024) with windowed carries of size K (0: carries from the scan
-    // kernel), plain or with decay and sources, or null when that
-    // combination is not compiled.
-    LdsKernel PickLds(int cpt, int s, int e, int32_t window, int threads, bool decay = false)
+    // kernel), plain or with decay and sources, for a periodic x axis (px,
+    // windowed only) or not, or null when that combination is not compiled.
+    LdsKernel PickLds(int cpt, int s, int e, int32_t window, int threads, bool decay = false, bool px = false)
     {
         switch (cpt * 10000 + s * 100 + e)
         {
             case 40832:
-                return PickLdsThreads<4, 8, 32>(threads, window, decay);
+                return PickLdsThreads<4, 8, 32>(threads, window, decay, px);
             case 80432:
-                return PickLdsThreads<8, 4, 32>(threads, window, decay);
+                return PickLdsThreads<8, 4, 32>(threads, window, decay, px);
             case 20816:
-                return PickLdsThreads<2, 8, 16>(threads, window, decay);
+                return PickLdsThreads<2, 8, 16>(threads, window, decay, px);
             case 40416:
-                return PickLdsThreads<4, 4, 16>(threads, window, decay);
+                return PickLdsThreads<4, 4, 16>(threads, window, decay, px);
             case 10808:
-                return PickLdsThreads<1, 8, 8>(threads, window, decay);
+                return PickLdsThreads<1, 8, 8>(threads, window, decay, px);
             case 20408:
-                return PickLdsThreads<2, 4, 8>(threads, window, decay);
+                return PickLdsThreads<2, 4, 8>(threads, window, decay, px);
             case 10404:
-                return PickLdsThreads<1, 4, 4>(threads, window, decay);
+                return PickLdsThreads<1, 4, 4>(threads, window, decay, px);
             default:
                 return nullptr;
         }
@@ -490,7 +495,7 @@ namespace
 
         // Row solver and carries for plain ticks.
         const int32_t ldsE = (p->wantRow == HALHIP_ADI_ROWS_WORKGROUP_SCAN) ? 0 : LdsElements(p->shape, p->Y);
-        int32_t K = (ldsE == 0 || wrapX || p->wantCarry == HALHIP_ADI_CARRIES_SCAN) ? 0 : WindowSize(sA, sC);
+        int32_t K = (ldsE == 0 || p->wantCarry == HALHIP_ADI_CARRIES_SCAN) ? 0 : WindowSize(sA, sC);
         if (K > 0 && PickLds(p->shape.cpt, p->shape.s, ldsE, K, p->shape.threads) == nullptr)
         {
             K = 0;
@@ -549,6 +554,8 @@ namespace
         a.xibLast = static_cast<float>(fx.ib[static_cast<size_t>(p->X - 1)]);
         a.xz0 = static_cast<float>(fx.z[0]);
         a.xzLast = static_cast<float>(fx.z[static_cast<size_t>(p->X - 1)]);
+        a.window = K;
+        a.colWindow = PeriodicWindowSize(sA, sC);
         p->ldsE = ldsE;
         p->window = K;
         p->useRow = (ldsE > 0) ? HALHIP_ADI_ROWS_LDS : HALHIP_ADI_ROWS_WORKGROUP_SCAN;
@@ -584,7 +591,8 @@ namespace
     void LaunchLds(halhip_adi2d *p, bool decay)
     {
         const int32_t window = (p->useCarry == HALHIP_ADI_CARRIES_WINDOW) ? p->window : 0;
-        LdsKernel k = PickLds(p->shape.cpt, p->shape.s, p->ldsE, window, p->shape.threads, decay);
+        LdsKernel k = PickLds(p->shape.cpt, p->shape.s, p->ldsE, window, p->shape.threads, decay,
+                              window > 0 && p->args.wrapX != 0);
         Require(k != nullptr, "adi2d: no LDS kernel for the strip shape");
         p->prof.Begin(p->clock.stream, decay ? "adi_strip_tick_decay" : "adi_strip_tick");
         hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(p->P)), dim3(static_cast<unsigned>(p->shape.threads)), 0,

@@ -68,6 +68,9 @@ namespace
         // [P+1] into srcXY / srcVal (null: no sources)
         const int32_t *srcStrip;
         int32_t window; // K of windowed carries (0: carries from the scan kernel)
+        // periodic x with windowed carries: the first and last colWindow
+        // strips' reductions give the column values (PeriodicWindowSize)
+        int32_t colWindow;
     };
 
     // ---- wavefront primitives ------------------------------------------------------
@@ -450,13 +453,27 @@ namespace
         StoreRow<CPT, VEC>(a.red + plane + base, y0, a.Y, acc);
     }
 
+    // Per-column values of a periodic x axis (Sherman-Morrison on the cyclic
+    // sweep-A lines): the correction factor, u*(0, y) and u*(X-1, y).  The
+    // scan kernel writes them to colInfo; with windowed carries each strip
+    // computes them itself (WindowPeriodicCols).
+    template <int CPT>
+    struct PeriodicCols
+    {
+        float factor[CPT];
+        float top[CPT];
+        float bottom[CPT];
+    };
+
     // Finishes sweep A on the strip from its carries (v: RHS in, u* out): cf
     // is d' of row s0-1 (forward carry-in), dn on entry u* of row s0+S
     // (backward carry-in).  Returns u* of the rows just above (s0-1, in up)
-    // and below (s0+S, in dn) the strip.
-    template <int CPT, int S, bool VEC>
+    // and below (s0+S, in dn) the strip.  On a periodic x axis the factor
+    // comes from pc when PCREG, else from colInfo.
+    template <int CPT, int S, bool VEC, bool PCREG = false>
     __device__ __forceinline__ void FinishACore(const AdiArgs &a, int s0, int y0, float (&v)[S][CPT],
-                                                const float (&cf)[CPT], float (&up)[CPT], float (&dn)[CPT])
+                                                const float (&cf)[CPT], float (&up)[CPT], float (&dn)[CPT],
+                                                const PeriodicCols<CPT> *pc = nullptr)
     {
         float acc[CPT];
 #pragma unroll
@@ -513,7 +530,18 @@ namespace
         if (a.wrapX)
         {
             float factor[CPT];
-            LoadRow<CPT, VEC>(a.colInfo, y0, a.Y, factor);
+            if constexpr (PCREG)
+            {
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    factor[c] = pc->factor[c];
+                }
+            }
+            else
+            {
+                LoadRow<CPT, VEC>(a.colInfo, y0, a.Y, factor);
+            }
             const float zUp = (s0 > 0) ? a.xz[s0 - 1] : 0.0f;
             const float zDn = (s0 + S < a.X) ? a.xz[s0 + S] : 0.0f;
 #pragma unroll
@@ -552,9 +580,10 @@ namespace
     // plus the Dirichlet implicit term on columns 0 and Y-1, with the same
     // ghost-value treatment of off-grid neighbours as sweep A.
     // ExplicitDiffusionX2ADI (src/Pde/ADIequations.cpp:19-32, ADIequations.h:54-69).
-    template <int CPT, int S, bool VEC>
+    template <int CPT, int S, bool VEC, bool PCREG = false>
     __device__ __forceinline__ void SweepBRhs(const AdiArgs &a, int s0, int y0, int tid, float (&v)[S][CPT],
-                                              const float (&up)[CPT], const float (&dn)[CPT])
+                                              const float (&up)[CPT], const float (&dn)[CPT],
+                                              const PeriodicCols<CPT> *pc = nullptr)
     {
         const int lastT = (a.Y - 1) / CPT;
         const int lastC = (a.Y - 1) % CPT;
@@ -567,7 +596,18 @@ namespace
             float g[CPT];
             if (a.wrapX)
             {
-                LoadRow<CPT, VEC>(a.colInfo + 2 * static_cast<size_t>(a.Y), y0, a.Y, g);
+                if constexpr (PCREG)
+                {
+#pragma unroll
+                    for (int c = 0; c < CPT; c++)
+                    {
+                        g[c] = pc->bottom[c];
+                    }
+                }
+                else
+                {
+                    LoadRow<CPT, VEC>(a.colInfo + 2 * static_cast<size_t>(a.Y), y0, a.Y, g);
+                }
             }
             else if (dirX)
             {
@@ -609,7 +649,18 @@ namespace
                 float g[CPT];
                 if (a.wrapX)
                 {
-                    LoadRow<CPT, VEC>(a.colInfo + static_cast<size_t>(a.Y), y0, a.Y, g);
+                    if constexpr (PCREG)
+                    {
+#pragma unroll
+                        for (int c = 0; c < CPT; c++)
+                        {
+                            g[c] = pc->top[c];
+                        }
+                    }
+                    else
+                    {
+                        LoadRow<CPT, VEC>(a.colInfo + static_cast<size_t>(a.Y), y0, a.Y, g);
+                    }
                 }
                 else if (dirX)
                 {
@@ -970,6 +1021,95 @@ namespace
         for (int c = 0; c < CPT; c++)
         {
             xb[c] = X[c];
+        }
+    }
+
+    // The periodic-x column values from the first and last K' = a.colWindow
+    // strips' reductions, as the scan kernel's full folds truncated where
+    // the dropped terms are below 2^-28: the forward carry out of the last
+    // strip (strips P-K' .. P-1, from 0) and the backward value at the top
+    // (strips 0 .. K'-1, forward carries from 0 at the top).  KW only sizes
+    // the batches of rows in flight.
+    template <int CPT, bool VEC, int KW>
+    __device__ __forceinline__ void WindowPeriodicCols(const AdiArgs &a, int y0, PeriodicCols<CPT> &pc)
+    {
+        const size_t plane = static_cast<size_t>(a.P) * a.Y;
+        const float *B = a.redIn;
+        const float *D = a.redIn + plane;
+        const int kk = a.colWindow;
+        // rows are requested in batches of KR strips (at most 48 values)
+        // before they are folded
+        constexpr int KR = (3 * KW * CPT <= 48) ? KW : ((3 * 4 * CPT <= 48) ? 4 : 1);
+        float cfEnd[CPT];
+        float F[CPT];
+        float X[CPT];
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            cfEnd[c] = 0.0f;
+            F[c] = 0.0f;
+            X[c] = 0.0f;
+        }
+        float coef = 1.0f;
+        for (int i0 = 0; i0 < kk; i0 += KR)
+        {
+            float be[KR][CPT]; // B rows P-kk+i (the end)
+            float dt[KR][CPT]; // D rows i (the top)
+            float bt[KR][CPT]; // B rows i-1 (the top)
+#pragma unroll
+            for (int j = 0; j < KR; j++)
+            {
+                const int i = i0 + j;
+                if (i < kk)
+                {
+                    LoadRow<CPT, VEC>(B + static_cast<size_t>(a.P - kk + i) * a.Y, y0, a.Y, be[j]);
+                    LoadRow<CPT, VEC>(D + static_cast<size_t>(i) * a.Y, y0, a.Y, dt[j]);
+                    if (i > 0)
+                    {
+                        LoadRow<CPT, VEC>(B + static_cast<size_t>(i - 1) * a.Y, y0, a.Y, bt[j]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < KR; j++)
+            {
+                const int i = i0 + j;
+                if (i < kk)
+                {
+                    const float Ae = a.stripA[a.P - kk + i];
+#pragma unroll
+                    for (int c = 0; c < CPT; c++)
+                    {
+                        cfEnd[c] = fmaf(Ae, cfEnd[c], be[j][c]);
+                    }
+                    if (i > 0)
+                    {
+                        const float At = a.stripA[i - 1];
+#pragma unroll
+                        for (int c = 0; c < CPT; c++)
+                        {
+                            F[c] = fmaf(At, F[c], bt[j][c]);
+                        }
+                    }
+                    const float Eq = a.stripE[i];
+#pragma unroll
+                    for (int c = 0; c < CPT; c++)
+                    {
+                        X[c] = fmaf(coef, fmaf(Eq, F[c], dt[j][c]), X[c]);
+                    }
+                    coef *= a.stripC[i];
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+        {
+            const float x0 = X[c];
+            const float xLast = a.xibLast * cfEnd[c];
+            const float factor = (x0 + xLast * a.xvEnd) * a.xInv;
+            pc.factor[c] = factor;
+            pc.top[c] = x0 - factor * a.xz0;
+            pc.bottom[c] = xLast - factor * a.xzLast;
         }
     }
 
@@ -1411,12 +1551,14 @@ namespace
 
     // KW: 0 takes the carries from the scan kernel, else windowed carries
     // with K <= KW.  EDGE: the strip holds row 0 or row X-1 (or rows past X).
+    // PX (with KW > 0): periodic x axis; the strip forms the Sherman-Morrison
+    // column values from the first and last strips' reductions.
     // DECAY: the tick is u' = ADI(u) - k*u + sources (a.negDecay; the strip's
     // sources are entries [a.srcStrip[p], a.srcStrip[p+1]) of a.srcXY /
     // a.srcVal, in list order, or none when a.srcStrip is null), and the
     // reduction for the next tick is taken of that u'.
     // rows: blockDim.x * (E+1) floats (S rows x 64W chunks x (E+1)).
-    template <int CPT, int S, int E, int KW, bool EDGE, bool DECAY = false>
+    template <int CPT, int S, int E, int KW, bool EDGE, bool DECAY = false, bool PX = false>
     __device__ __forceinline__ void StripLdsBody(const AdiArgs &a, float *__restrict__ u, int p, int K,
                                                  const RowSolve &rs, float *rows, float *sTotF, float *sTotB)
     {
@@ -1430,6 +1572,14 @@ namespace
         const int rowStride = 64 * W * (E + 1);
         const bool first = tid == 0;
         const bool last = tid == static_cast<int>(blockDim.x) - 1;
+
+        // Periodic x with windowed carries: the column values first, while
+        // nothing else is held in registers.
+        PeriodicCols<CPT> pc;
+        if constexpr (PX)
+        {
+            WindowPeriodicCols<CPT, true, KW>(a, y0, pc);
+        }
 
         // Interior strips with a full window request the carries' rows and
         // then the strip's rows before waiting for any of them (loads
@@ -1501,14 +1651,31 @@ namespace
         }
         __syncthreads();
         RhsAFromLds<CPT, S, E, EDGE>(a, rows, rowStride, s0, y0, first, last, v);
-        FinishACore<CPT, S, true>(a, s0, y0, v, cf, up, dn);
-        if constexpr (EDGE)
+        if constexpr (PX)
         {
-            SweepBRhs<CPT, S, true>(a, s0, y0, tid, v, up, dn);
+            // windowed carries on a periodic x axis: no scan kernel wrote
+            // colInfo, so the column values come from pc
+            FinishACore<CPT, S, true, true>(a, s0, y0, v, cf, up, dn, &pc);
+            if constexpr (EDGE)
+            {
+                SweepBRhs<CPT, S, true, true>(a, s0, y0, tid, v, up, dn, &pc);
+            }
+            else
+            {
+                RhsBInterior<CPT, S>(a, s0, first, last, v, up, dn);
+            }
         }
         else
         {
-            RhsBInterior<CPT, S>(a, s0, first, last, v, up, dn);
+            FinishACore<CPT, S, true>(a, s0, y0, v, cf, up, dn);
+            if constexpr (EDGE)
+            {
+                SweepBRhs<CPT, S, true>(a, s0, y0, tid, v, up, dn);
+            }
+            else
+            {
+                RhsBInterior<CPT, S>(a, s0, first, last, v, up, dn);
+            }
         }
         __syncthreads(); // every read of the staged u is done
 
@@ -1734,7 +1901,7 @@ namespace
 
     // T: threads per workgroup (= yDim / CPT); the LDS row buffer holds
     // T * (E+1) floats.
-    template <int CPT, int S, int E, int KW, int T, bool DECAY = false>
+    template <int CPT, int S, int E, int KW, int T, bool DECAY = false, bool PX = false>
     __global__ __launch_bounds__(T) void AdiStripLds(AdiArgs a, float *__restrict__ u, int32_t K, RowSolve rs)
     {
         static_assert(T * (E + 1) * 4 <= 140 * 1024, "LDS row buffer too large");
@@ -1744,11 +1911,11 @@ namespace
         const int p = StripOfBlock(blockIdx.x, a.P, KW > 0 && (a.P % 8 == 0));
         if (p == 0 || (p + 1) * S >= a.X)
         {
-            StripLdsBody<CPT, S, E, KW, true, DECAY>(a, u, p, K, rs, rows, sTotF, sTotB);
+            StripLdsBody<CPT, S, E, KW, true, DECAY, PX>(a, u, p, K, rs, rows, sTotF, sTotB);
         }
         else
         {
-            StripLdsBody<CPT, S, E, KW, false, DECAY>(a, u, p, K, rs, rows, sTotF, sTotB);
+            StripLdsBody<CPT, S, E, KW, false, DECAY, PX>(a, u, p, K, rs, rows, sTotF, sTotB);
         }
     }
 
@@ -2058,6 +2225,30 @@ namespace
             }
         }
         return 0;
+    }
+
+    // Strips whose reductions a periodic x axis's column values need: the
+    // smallest k such that the terms dropped from u*(0, y) (weight
+    // C_0 .. C_{k-1}) and from the forward carry out of the last strip
+    // (weight A_{P-k} .. A_{P-1}) are at most 2^-28; P when there is none.
+    // Unlike WindowSize, these products start at the grid's ends, where the
+    // strip multipliers are not zero.
+    int32_t PeriodicWindowSize(const std::vector<double> &A, const std::vector<double> &C)
+    {
+        const double tau = std::ldexp(1.0, -28);
+        const int32_t P = static_cast<int32_t>(A.size());
+        double prodC = 1.0;
+        double prodA = 1.0;
+        for (int32_t k = 1; k < P; k++)
+        {
+            prodC *= std::fabs(C[static_cast<size_t>(k - 1)]);
+            prodA *= std::fabs(A[static_cast<size_t>(P - k)]);
+            if (prodC <= tau && prodA <= tau)
+            {
+                return k;
+            }
+        }
+        return P;
     }
 
     AxisFactors FactorLine(int32_t n, int32_t padded, double rate, bool wrap, bool dirichlet)

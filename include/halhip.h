@@ -250,7 +250,8 @@ int halhip_adi2d_get_strip_shape(halhip_adi2d *plan, int32_t *columnsPerThread, 
  *   carries: HALHIP_ADI_CARRIES_SCAN runs a cross-strip scan kernel per tick;
  *     HALHIP_ADI_CARRIES_WINDOW composes each strip's carries from the K
  *     strips on either side when the line factors decay fast enough that the
- *     dropped terms are below 2^-28 (non-periodic xDim, LDS rows only).
+ *     dropped terms are below 2^-28 (LDS rows only; a periodic xDim also
+ *     forms its column corrections from the first and last strips).
  * A method that does not apply falls back to the other one;
  * halhip_adi2d_get_methods reports what the last plain run used and the
  * window K (0 without windowed carries).  No counterpart in the reference. */

@@ -299,6 +299,30 @@ def test_periodic_y_lds_rows_match_oracle(ctx, oracle, X, Y, shape, rate, wx, bc
     assert halhip.ADI_ROWS_LDS in used
 
 
+@pytest.mark.parametrize("rate", [0.05, 0.5, 5.0])
+@pytest.mark.parametrize("wy", [0, 1])
+def test_periodic_x_windowed_carries_match_oracle(ctx, oracle, rate, wy):
+    # A periodic x axis with windowed carries: every strip forms the
+    # Sherman-Morrison column values from the first and last strips'
+    # reductions (no scan kernel).  Those windows start at the grid's ends,
+    # where the strip multipliers are not zero, so they can be longer than
+    # the carries' window; 1000 rows in 250 strips truncate both.
+    X, Y = 1000, 1024
+    f = np.random.default_rng(int(rate * 100) + wy).random(X * Y).astype(np.float32)
+    ref = adi_ref_ticks(oracle, f, 4, rate, X, Y, 1, wy, None)
+    plan = halhip.Adi2D(ctx, X, Y)
+    got = {}
+    for carries in (halhip.ADI_CARRIES_SCAN, halhip.ADI_CARRIES_WINDOW):
+        plan.set_methods(halhip.ADI_ROWS_LDS, carries)
+        plan.upload(f)
+        plan.run_steps(4, rate, 0.0, 1, wy)
+        got[plan.methods()[1]] = plan.download()
+    assert set(got) == {halhip.ADI_CARRIES_SCAN, halhip.ADI_CARRIES_WINDOW}
+    for carries, d in got.items():
+        assert rel_l2(d, ref) < 1e-5, (carries, rel_l2(d, ref))
+    assert rel_l2(got[halhip.ADI_CARRIES_WINDOW], got[halhip.ADI_CARRIES_SCAN]) < 2e-6
+
+
 DECAY_CASES = [
     # X, Y, (cpt, rows), rate, wrapX, bc
     (300, 1024, (1, 8), 5.0, 0, 0.2),
