@@ -308,6 +308,8 @@ def main():
     ap.add_argument("--methods", default=None,
                     help="ADI row solver and carries as ROWS,CARRIES with ROWS in auto|scan|lds and "
                          "CARRIES in auto|scan|window, instead of the automatic choice")
+    ap.add_argument("--wrap", default="0,0",
+                    help="periodic axes as WRAPX,WRAPY (0|1); a non-periodic axis keeps the Dirichlet 0 edges")
     ap.add_argument("--explicit-shape", default=None,
                     help="rows per workgroup and ticks per launch of the explicit companion as B,T")
     ap.add_argument("--companion", choices=["explicit", "agents", "batched", "adjoint"], default=None,
@@ -343,6 +345,8 @@ def main():
         ctx.close()
         return
 
+    wx, wy = (int(v) != 0 for v in args.wrap.split(","))
+    headline = not wx and not wy
     w = workloads.adi_blobs(n=args.size)
     n = w["X"]
     cells = n * n
@@ -359,10 +363,10 @@ def main():
                           "window": halhip.ADI_CARRIES_WINDOW}[carries_name])
     plan.upload(w["field"])
 
-    plan.run_steps(args.warmup, w["rate"], 0.0, False, False, bc=w["bc"])
+    plan.run_steps(args.warmup, w["rate"], 0.0, wx, wy, bc=w["bc"])
     ctx.synchronize()
     t0 = time.perf_counter()
-    plan.run_steps(args.steps, w["rate"], 0.0, False, False, bc=w["bc"], asynchronous=True)
+    plan.run_steps(args.steps, w["rate"], 0.0, wx, wy, bc=w["bc"], asynchronous=True)
     event_ms = plan.sync()
     ctx.synchronize()
     wall = time.perf_counter() - t0
@@ -375,16 +379,16 @@ def main():
     # stream), in a separate profiled pass of the same ticks.
     plan.set_profiling(True)
     prof_ticks = min(args.steps, 50)
-    plan.run_steps(prof_ticks, w["rate"], 0.0, False, False, bc=w["bc"])
+    plan.run_steps(prof_ticks, w["rate"], 0.0, wx, wy, bc=w["bc"])
     prof = plan.profile()
     plan.set_profiling(False)
     dom = "adi_strip_tick"
     dom_ms = prof[dom][0] / prof[dom][1]
     achieved = BYTES_PER_CELL_TICK * cells / (dom_ms * 1e-3) / 1e9
-    traffic = traffic_from_profile() if n == 4096 else None
+    traffic = traffic_from_profile() if n == 4096 and headline else None
 
     extras = {}
-    if not args.shape and not args.methods:
+    if not args.shape and not args.methods and headline:
         extras["adi_model_tick"] = bench_adi_model(ctx, w)
     if not args.adi_only:
         cpu = not args.no_cpu_baseline
@@ -394,10 +398,12 @@ def main():
         extras["adjoint"] = bench_adjoint(ctx, cpu=cpu)
 
     base = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and headline:
         base = cpu_baseline(w["field"], n, w["rate"], args.cpu_ticks)
 
     value = cells * args.steps / wall / 1e9
+    edges = {(False, False): "Dirichlet 0", (True, True): "periodic x and y",
+             (True, False): "periodic x, Dirichlet 0 in y", (False, True): "Dirichlet 0 in x, periodic y"}[(wx, wy)]
     line = {
         "metric": f"Gcell-updates/s (ADI {n}^2 FP32)",
         "value": round(value, 3),
@@ -411,8 +417,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: 64 Gaussian blobs (sigma 20, amplitude U[0.5,1]) at positions from Hal::Rng(42)",
-        "config": {"workload": f"ADI {n}x{n}, D=5, dt=1, Dirichlet 0, {args.steps} ticks", "grid": [n, n],
-                   "bc": "dirichlet-0", "rate": w["rate"], "parallelism": "single GPU",
+        "config": {"workload": f"ADI {n}x{n}, D=5, dt=1, {edges}, {args.steps} ticks", "grid": [n, n],
+                   "bc": edges, "rate": w["rate"], "parallelism": "single GPU",
                    "strip_shape": list(plan.strip_shape()),
                    "methods": dict(zip(("rows", "carries", "window"), adi_methods))},
         "hbm_frac_wall": round(value * BYTES_PER_CELL_TICK / HBM_PEAK_GBS, 4),

@@ -197,6 +197,20 @@ def test_4096_linearity(ctx):
     assert rel_l2(outs[2], 2.0 * outs[0] - 3.0 * outs[1]) < 1e-5
 
 
+@pytest.mark.parametrize("n,shape", [(512, (1, 4, True)), (1024, (2, 4, True)), (2048, (4, 8, True)),
+                                     (4096, (4, 8, True)), (1000, (1, 4, True))])
+def test_automatic_shape(ctx, n, shape):
+    # Up to yDim 2048 the automatic shape prefers 512-thread workgroups
+    # where the LDS row solve covers them; 1000 columns (not a multiple of
+    # 64) keep the register solve at one column per thread.
+    plan = halhip.Adi2D(ctx, n, n)
+    assert plan.strip_shape() == shape
+    plan.upload(np.zeros(n * n, np.float32))
+    plan.run_steps(1, 5.0, 0.0, False, False, bc=0.0)
+    want = halhip.ADI_ROWS_LDS if n % 64 == 0 else halhip.ADI_ROWS_WORKGROUP_SCAN
+    assert plan.methods()[0] == want
+
+
 @pytest.mark.parametrize("Y", [4096, 1000])
 def test_every_strip_shape_matches_oracle(ctx, oracle, Y):
     X = 150
