@@ -104,13 +104,17 @@ def test_gradient_vs_central_differences(ctx, oracle):
 
     h = 1e-6
     scale = np.abs(gD).max()
+    worst = 0.0
     for c in rng.choice(X * Y, 64, replace=False):
         Dp, Dm = D.copy(), D.copy()
         Dp[c] += h
         Dm[c] -= h
         fd = (loss(Dp, k) - loss(Dm, k)) / (2 * h)
+        worst = max(worst, abs(gD[c] - fd) / scale)
         assert abs(gD[c] - fd) < 1e-4 * scale, (c, gD[c], fd)
     fdk = (loss(D, k + h) - loss(D, k - h)) / (2 * h)
+    print(f"adjoint vs FP64 central differences: D cells max |g - fd| / max|g| = {worst:.3e}, "
+          f"decay |g - fd| / |fd| = {abs(gk - fdk) / abs(fdk):.3e}")
     assert abs(gk - fdk) < 1e-4 * abs(fdk)
 
 
@@ -120,7 +124,9 @@ def test_workload_1024_forward_first_ticks(ctx, oracle):
     p.set_problem(w["D"], w["decay"], 3, 1)
     got = p.forward(w["u0"])
     ref = oracle.adi_cell_forward(w["u0"].astype(np.float64), w["D"], w["decay"], 3, w["X"], w["Y"])
-    assert rel_l2(got, ref) < 1e-5
+    err = rel_l2(got, ref)
+    print(f"adjoint workload 1024^2 forward, 3 ticks: rel L2 vs FP64 = {err:.3e}")
+    assert err < 1e-5
 
 
 def test_torch_autograd_function(ctx, oracle):

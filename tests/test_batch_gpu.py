@@ -111,9 +111,13 @@ def test_workload_512_wells_first_ticks(ctx, oracle):
     got = b.download()
     # check a sample of wells (small, median, largest) against the oracle
     order = np.argsort(spec["xdims"])
+    worst = 0.0
     for k in (int(order[0]), int(order[len(order) // 2]), int(order[-1])):
         X = int(spec["xdims"][k])
         sel = spec["src_well"] == k
         ref = oracle.adi_ticks(np.zeros(X * X), 3, float(spec["rate"][k]), float(spec["decay"][k]), X, X, 0, 0,
                                src_idx=spec["src_cell"][sel], src_val=spec["src_val"][sel].astype(np.float64))
-        assert rel_l2(b.well(got, k), ref) < 1e-5, (k, X)
+        err = rel_l2(b.well(got, k), ref)
+        worst = max(worst, err)
+        assert err < 1e-5, (k, X)
+    print(f"512 wells, 3 ticks (smallest, median, largest well): worst rel L2 vs FP64 = {worst:.3e}")
