@@ -317,25 +317,31 @@ namespace
 
     // Interaction of the agent in slot k (position p, radius ri) with the
     // agents of slots [s, e), accumulated into (vx, vy) in slot order.
+    // One candidate q (position o, radius rq) acting on slot k: hardware
+    // square root and reciprocal (within 1 ulp); the FP32 sums stay within
+    // 1e-5 of the reference's FP64 ones.
+    __device__ __forceinline__ void AccumulateOne(int q, float2 o, float rq, int k, float2 p, float ri, float R,
+                                                  float kSpring, float fX, float fY, bool wrapX, bool wrapY,
+                                                  float &vx, float &vy)
+    {
+        const float dx = Disp(p.x, o.x, fX, wrapX);
+        const float dy = Disp(p.y, o.y, fY, wrapY);
+        const float dist = __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
+        if (q != k && dist < R && dist > 0.0f)
+        {
+            const float f = kSpring * ((ri + rq) - dist) * __builtin_amdgcn_rcpf(dist);
+            vx -= dx * f;
+            vy -= dy * f;
+        }
+    }
+
     __device__ __forceinline__ void AccumulateRange(int s, int e, int k, float2 p, float ri, float R, float kSpring,
                                                     float fX, float fY, bool wrapX, bool wrapY,
                                                     const float2 *__restrict__ spos, const float *__restrict__ srad,
                                                     float &vx, float &vy)
     {
-        // one candidate: hardware square root and reciprocal (within 1 ulp);
-        // the FP32 sums stay within 1e-5 of the reference's FP64 ones
         auto one = [&](int q, float2 o, float rq)
-        {
-            const float dx = Disp(p.x, o.x, fX, wrapX);
-            const float dy = Disp(p.y, o.y, fY, wrapY);
-            const float dist = __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
-            if (q != k && dist < R && dist > 0.0f)
-            {
-                const float f = kSpring * ((ri + rq) - dist) * __builtin_amdgcn_rcpf(dist);
-                vx -= dx * f;
-                vy -= dy * f;
-            }
-        };
+        { AccumulateOne(q, o, rq, k, p, ri, R, kSpring, fX, fY, wrapX, wrapY, vx, vy); };
         // candidates in pairs, both loads in flight together, summed in slot
         // order
         int q = s;
@@ -446,6 +452,10 @@ namespace
     // (cx, ya .. yb-1) of one neighbour row are the contiguous slot range
     // [offsets[cx*Y+ya], offsets[cx*Y+yb]); visiting the rows x-major and the
     // wrapped y segments in increasing ys keeps the reference's order.
+    // Candidates per neighbour row loaded up front by the force pass's
+    // common case (about 2.9 agents per 3-cell row at one agent per cell).
+    constexpr int kHead = 4;
+
     template <bool WX, bool WY>
     __global__ __launch_bounds__(kThreads) void AgentForces(int32_t N, int32_t X, int32_t Y, float R, float kSpring,
                                                             const int32_t *__restrict__ offsets,
@@ -485,10 +495,42 @@ namespace
                 rs[r] = ok ? offsets[rowBase + ya] : 0;
                 re[r] = ok ? offsets[rowBase + yb] : 0;
             }
+            // The first kHead candidates of every row are requested before
+            // any is used (one memory round trip for most agents instead of
+            // one per pair); the rest of a row follows in pairs.  The sum
+            // keeps slot order within each row and rows x-major.
+            float2 o[3][kHead];
+            float rq[3][kHead];
 #pragma unroll
             for (int r = 0; r < 3; r++)
             {
-                AccumulateRange(rs[r], re[r], k, p, ri, R, kSpring, fX, fY, false, false, spos, srad, vx, vy);
+#pragma unroll
+                for (int j = 0; j < kHead; j++)
+                {
+                    const int q = rs[r] + j;
+                    o[r][j] = make_float2(0.0f, 0.0f);
+                    rq[r][j] = 0.0f;
+                    if (q < re[r])
+                    {
+                        o[r][j] = spos[q];
+                        rq[r][j] = srad[q];
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+            {
+#pragma unroll
+                for (int j = 0; j < kHead; j++)
+                {
+                    if (rs[r] + j < re[r])
+                    {
+                        AccumulateOne(rs[r] + j, o[r][j], rq[r][j], k, p, ri, R, kSpring, fX, fY, false, false, vx,
+                                      vy);
+                    }
+                }
+                AccumulateRange(rs[r] + kHead, re[r], k, p, ri, R, kSpring, fX, fY, false, false, spos, srad, vx,
+                                vy);
             }
             vel[k] = make_float2(vx, vy);
             if (move.on)
