@@ -2227,30 +2227,6 @@ namespace
         return 0;
     }
 
-    // Strips whose reductions a periodic x axis's column values need: the
-    // smallest k such that the terms dropped from u*(0, y) (weight
-    // C_0 .. C_{k-1}) and from the forward carry out of the last strip
-    // (weight A_{P-k} .. A_{P-1}) are at most 2^-28; P when there is none.
-    // Unlike WindowSize, these products start at the grid's ends, where the
-    // strip multipliers are not zero.
-    int32_t PeriodicWindowSize(const std::vector<double> &A, const std::vector<double> &C)
-    {
-        const double tau = std::ldexp(1.0, -28);
-        const int32_t P = static_cast<int32_t>(A.size());
-        double prodC = 1.0;
-        double prodA = 1.0;
-        for (int32_t k = 1; k < P; k++)
-        {
-            prodC *= std::fabs(C[static_cast<size_t>(k - 1)]);
-            prodA *= std::fabs(A[static_cast<size_t>(P - k)]);
-            if (prodC <= tau && prodA <= tau)
-            {
-                return k;
-            }
-        }
-        return P;
-    }
-
     AxisFactors FactorLine(int32_t n, int32_t padded, double rate, bool wrap, bool dirichlet)
     {
         AxisFactors f;
