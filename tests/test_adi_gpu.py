@@ -43,6 +43,25 @@ def test_eigenmode_through_interface(ctx):
         np.testing.assert_allclose(got, lam * f, rtol=0, atol=2e-6)
 
 
+def test_resident_eigenmode_powers(ctx):
+    # tests/AdiKernel2DTests.cpp:18-68: the periodic eigenmode after N = 50
+    # resident ticks at r = 0.8 is (lam_x lam_y)^N f, then ^2N (1e-3 there).
+    # The mode decays 5000-fold over 2N ticks while the FP32 rounding of the
+    # first ticks does not, so the error relative to the decayed mode grows
+    # to ~3e-5; relative to the initial field it stays ~1e-8.
+    nx, ny, r, n = 24, 32, 0.8, 50
+    f = eigen_field(nx, ny)
+    lam = lambda_cn(r, nx) * lambda_cn(r, ny)
+    plan = halhip.Adi2D(ctx, nx, ny)
+    plan.upload(f)
+    for k, bar in ((1, 1e-5), (2, 1e-4)):
+        plan.run_steps(n, r, 0.0, True, True)
+        got = plan.download()
+        want = lam ** (k * n) * f
+        assert rel_l2(got, want) < bar
+        assert np.linalg.norm(got - want) < 1e-7 * np.linalg.norm(f)
+
+
 def test_zero_flux_mass_and_cell_for_cell(ctx, oracle):
     # tests/AdiKernel2DTests.cpp:70-141: 20x14, 5 steps, r = 0.6
     nx, ny = 20, 14

@@ -71,6 +71,30 @@ def test_coincident_and_misuse(ctx):
     assert empty.size == 0
 
 
+def test_hand_placed_pairs_closed_form(ctx):
+    # tests/AgentForceKernel2DTests.cpp:49-123 and SphericalAgentTests.cpp:
+    # linear overlap force k((ri + rj) - d) pushing each agent of a pair
+    # apart along the pair's axis; none at d >= R; pairs across a periodic
+    # edge by the minimum image.  Distances are exact in binary.
+    k, R = 1.0, 1.0
+    pos = [2.0, 3.0, 2.75, 3.0,      # pair along x, d = 0.75, overlap 0.25
+           6.0, 6.0, 6.0, 6.5,       # pair along y, d = 0.5, radii 0.5 and 0.25: overlap 0.25
+           10.0, 10.0, 11.5, 10.0,   # d = 1.5 >= R: no force
+           0.25, 12.0, 15.5, 12.0]   # across the x edge of a 16-wide grid, d = 0.75
+    rad = [0.5, 0.5, 0.5, 0.25, 0.5, 0.5, 0.5, 0.5]
+    plan = halhip.Agents2D(ctx, 16, 16, 8)
+    want = np.array([-0.25, 0.0, 0.25, 0.0,
+                     0.0, -0.25, 0.0, 0.25,
+                     0.0, 0.0, 0.0, 0.0,
+                     0.25, 0.0, -0.25, 0.0])
+    got = plan.accumulate_forces(pos, rad, R, k, 1, 0)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-6)
+    # without wrap the last pair is 15.25 apart: no force
+    want[12:] = 0.0
+    got = plan.accumulate_forces(pos, rad, R, k, 0, 0)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-6)
+
+
 @pytest.mark.parametrize("wrap", [(0, 0), (1, 1), (1, 0)])
 def test_ticks_vs_oracle_small(ctx, oracle, wrap):
     rng = np.random.default_rng(3)

@@ -33,6 +33,46 @@ def test_eigenmode_through_interface(ctx):
     np.testing.assert_allclose(got, lam * f, rtol=0, atol=1e-6)
 
 
+def test_resident_eigenmode_powers(ctx):
+    # tests/DiffusionKernel2DTests.cpp:16-107: after N and 2N resident steps
+    # the periodic eigenmode is lam^N f and lam^2N f (1e-3 there).
+    nx, ny, r, n = 16, 20, 0.05, 50
+    f = eigen_field(nx, ny)
+    lam = 1 - 4 * r * (np.sin(np.pi / nx) ** 2 + np.sin(np.pi / ny) ** 2)
+    plan = halhip.Diffusion2D(ctx, nx, ny)
+    plan.upload(f)
+    for k in (1, 2):
+        plan.run_steps(n, r, 0.0, True, True)
+        np.testing.assert_allclose(plan.download(), lam ** (k * n) * f, rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("wrap", [True, False])
+def test_resident_mass_conservation(ctx, wrap):
+    # tests/PDEGridTests.cpp:72-104: periodic and zero-flux diffusion keep
+    # the sum of the field.
+    X, Y = 48, 40
+    f = np.random.default_rng(11).random(X * Y).astype(np.float32)
+    plan = halhip.Diffusion2D(ctx, X, Y)
+    plan.upload(f)
+    plan.run_steps(200, 0.2, 0.0, wrap, wrap)
+    got = plan.download().astype(np.float64).sum()
+    m0 = f.astype(np.float64).sum()
+    assert abs(got - m0) < 1e-5 * m0
+
+
+def test_uniform_per_cell_rates_equal_uniform_stencil(ctx):
+    # tests/PDEGridTests.cpp:291-330: per-cell rates that are all equal give
+    # the constant-rate stencil.
+    X, Y, r = 13, 17, 0.15
+    f = np.random.default_rng(12).random(X * Y)
+    plan = halhip.Diffusion2D(ctx, X, Y)
+    for wx in (0, 1):
+        for wy in (0, 1):
+            want = plan.diffuse(f, np.zeros(X * Y), r, wx, wy)
+            got = plan.diffuse_rates(f, np.zeros(X * Y), np.full(X * Y, r), np.full(X * Y, r), wx, wy)
+            np.testing.assert_allclose(got, want, rtol=0, atol=1e-7)
+
+
 def test_rates_interface(ctx, golden, oracle):
     f, rx, ry = golden["rates_field"], golden["rates_x"], golden["rates_y"]
     plan = halhip.Diffusion2D(ctx, 9, 11)
