@@ -543,6 +543,14 @@ namespace
         {
             K = 0;
         }
+        // A periodic x axis also folds the first and last colWindow strips in
+        // every strip.  With strip multipliers of magnitude <= 1 that window
+        // is at most K + 1; anything longer takes the scan kernel instead.
+        const int32_t colWindow = wrapX ? PeriodicWindowSize(sA, sC) : 0;
+        if (K > 0 && colWindow > K + 1)
+        {
+            K = 0;
+        }
         size_t oRow = 0;
         RowTables rt;
         if (ldsE > 0)
@@ -598,7 +606,7 @@ namespace
         a.xz0 = static_cast<float>(fx.z[0]);
         a.xzLast = static_cast<float>(fx.z[static_cast<size_t>(p->X - 1)]);
         a.window = K;
-        a.colWindow = PeriodicWindowSize(sA, sC);
+        a.colWindow = colWindow;
         p->ldsE = ldsE;
         p->window = K;
         p->useRow = (ldsE > 0) ? HALHIP_ADI_ROWS_LDS : HALHIP_ADI_ROWS_WORKGROUP_SCAN;
