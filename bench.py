@@ -228,7 +228,42 @@ def bench_batched(ctx, ticks=50, cpu=True):
                                          f"{secs:.2f} s: PDEGrid2D DiffusionADI + MulAll + Add + Update; "
                                          f"{host_note()}"}
         out["speedup_vs_cpu"] = round(out["batched_Gcell_s"] / out["cpu_baseline"]["value"], 1)
+        out["cpu_baseline_threads"] = cpu_wells_threaded(ref, spec, out["cells"])
     return out
+
+
+def cpu_wells_threaded(ref, spec, cells):
+    """The reference's own parallel shape for a batch of wells:
+    MultiWellExperiment::StepMultiThread runs one well per task on
+    hardware_concurrency() threads, pulling wells from a shared counter
+    (include/Hal/Tools/MultiWellExperiment.h:95-103,
+    include/Hal/Concurrency/MultiThread.h:19-77).  The compiled reference's
+    tick runs in a pool of the host's threads (ctypes releases the GIL); a
+    step takes the busiest thread's summed tick time (grid set-up, which
+    happens once per well in a model, is not counted)."""
+    import collections
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+    import numpy as np
+    threads = max(1, min(16, os.cpu_count() or 1))
+    busy = collections.defaultdict(float)
+    lock = threading.Lock()
+
+    def one(wi):
+        side = int(spec["xdims"][wi])
+        m = spec["src_well"] == wi
+        s, _ = ref.grid_ticks_timed(np.zeros(side * side), 0, 1, True, spec["rate"][wi], spec["decay"][wi], side,
+                                    side, False, False, None, spec["src_cell"][m],
+                                    spec["src_val"][m].astype(np.float64))
+        with lock:
+            busy[threading.get_ident()] += s
+
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        list(pool.map(one, range(len(spec["xdims"]))))
+    step = max(busy.values())
+    return {"value": round(cells / step / 1e9, 5), "unit": "Gcell-updates/s", "cores": threads, "kind": "reference",
+            "sample": f"one tick of every well, wells pulled by {threads} threads (StepMultiThread shape); "
+                      f"busiest thread {step:.3f} s; host {cpu_model()}, nproc {os.cpu_count()}"}
 
 
 def bench_adjoint(ctx, cpu=True):
