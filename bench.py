@@ -266,6 +266,48 @@ def cpu_wells_threaded(ref, spec, cells):
                       f"busiest thread {step:.3f} s; host {cpu_model()}, nproc {os.cpu_count()}"}
 
 
+def bench_per_call(ctx, cpu=True):
+    """The per-call interfaces on host FP64 arrays, the reference's drop-in
+    shape: IDiffusionBackend2D::Diffuse and IAdiDiffusionBackend2D::DiffuseADI
+    (include/Hal/Pde/IDiffusionBackend2D.h:26-32,
+    include/Hal/Pde/IAdiDiffusionBackend2D.h:25-31).  Every call converts to
+    FP32, uploads, runs one step, downloads and accumulates into deltas; it is
+    timed through the ctypes binding.  The CPU side is the reference's
+    CpuDiffusionBackend2D / CpuAdiDiffusionBackend2D on the same arrays."""
+    import numpy as np
+    import halhip
+    ref = reference_lib() if cpu else None
+    cases = []
+    for n in (256, 1024):
+        f = np.random.default_rng(n).random(n * n)
+        for adi in (False, True):
+            plan = halhip.Adi2D(ctx, n, n) if adi else halhip.Diffusion2D(ctx, n, n)
+            rate = 0.8 if adi else 0.2
+            d = np.zeros(n * n)
+            calls = 100 if n <= 256 else 20
+            # as many untimed calls first: the first calls of a process pay
+            # one-time host costs
+            for _ in range(calls):
+                plan.diffuse(f, d, rate, True, True)
+            t0 = time.perf_counter()
+            for _ in range(calls):
+                plan.diffuse(f, d, rate, True, True)
+            gpu_ms = (time.perf_counter() - t0) / calls * 1e3
+            plan.close()
+            case = {"interface": "DiffuseADI" if adi else "Diffuse", "grid": n, "wrap": "periodic",
+                    "gpu_ms_per_call": round(gpu_ms, 4)}
+            if ref is not None:
+                cpu_calls = 20 if n <= 256 else 4
+                cpu_ms = ref.diffuse_calls_timed(f, rate, n, n, True, True, adi, cpu_calls) / cpu_calls * 1e3
+                case["cpu_ms_per_call"] = round(cpu_ms, 4)
+                case["speedup_vs_cpu"] = round(cpu_ms / gpu_ms, 2)
+            cases.append(case)
+    return {"workload": "per-call interfaces on FP64 host arrays, periodic, r = 0.2 (explicit) and 0.8 (ADI)",
+            "cases": cases,
+            "note": "each call includes the FP64->FP32 conversion, upload, download and FP64 accumulation; "
+                    "CPU: the reference backends on one thread"}
+
+
 def bench_adjoint(ctx, cpu=True):
     """1024^2 per-cell-diffusivity ADI (workloads.adjoint_problem): forward
     200 ticks vs forward + checkpointed adjoint gradient."""
@@ -347,7 +389,7 @@ def main():
                     help="periodic axes as WRAPX,WRAPY (0|1); a non-periodic axis keeps the Dirichlet 0 edges")
     ap.add_argument("--explicit-shape", default=None,
                     help="rows per workgroup and ticks per launch of the explicit companion as B,T")
-    ap.add_argument("--companion", choices=["explicit", "agents", "batched", "adjoint"], default=None,
+    ap.add_argument("--companion", choices=["explicit", "agents", "batched", "adjoint", "per_call"], default=None,
                     help="run only this companion workload and print its JSON (for profiling)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run the copy kernel (read + write this many MiB, 10 launches) for counter calibration")
@@ -372,7 +414,7 @@ def main():
     if args.companion:
         ctx = halhip.Context(0)
         run = {"explicit": bench_explicit, "agents": bench_agents, "batched": bench_batched,
-               "adjoint": bench_adjoint}[args.companion]
+               "adjoint": bench_adjoint, "per_call": bench_per_call}[args.companion]
         kwargs = {}
         if args.companion == "explicit" and args.explicit_shape:
             kwargs["shape"] = tuple(int(v) for v in args.explicit_shape.split(","))
@@ -431,6 +473,7 @@ def main():
         extras["agents"] = bench_agents(ctx, cpu=cpu)
         extras["batched"] = bench_batched(ctx, cpu=cpu)
         extras["adjoint"] = bench_adjoint(ctx, cpu=cpu)
+        extras["per_call"] = bench_per_call(ctx, cpu=cpu)
 
     base = None
     if not args.no_cpu_baseline and headline:

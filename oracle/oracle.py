@@ -340,6 +340,15 @@ class Reference(_Lib):
         f.argtypes = [_dp, _dp, _i32, _i32, _f64, _f64, _i32, _i32, _i32, _i32]
         return f(_p(pos), _p(radii), radii.size, calls, R, k, X, Y, int(wrapX), int(wrapY))
 
+    def diffuse_calls_timed(self, field, rate, X, Y, wrapX, wrapY, adi, calls):
+        """Seconds for `calls` CpuDiffusionBackend2D::Diffuse (or
+        CpuAdiDiffusionBackend2D::DiffuseADI) calls after one untimed call."""
+        field = _f(field)
+        f = self.fn("diffuse_calls_timed")
+        f.restype = ctypes.c_double
+        f.argtypes = [_dp, _f64, _i32, _i32, _i32, _i32, _i32, _i32]
+        return f(_p(field), rate, X, Y, int(wrapX), int(wrapY), int(adi), calls)
+
     def adi_ticks_timed(self, field, warmup, ticks, rate, X, Y, bc_value):
         """Seconds for `ticks` Dirichlet ADI ticks of the compiled reference."""
         field = _f(field).copy()

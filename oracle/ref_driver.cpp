@@ -223,6 +223,40 @@ extern "C"
         return std::chrono::duration<double>(t1 - t0).count();
     }
 
+    // CPU baseline of the per-call interfaces: one untimed then `calls` timed
+    // CpuDiffusionBackend2D::Diffuse (adi == 0) or
+    // CpuAdiDiffusionBackend2D::DiffuseADI calls on the same caller-owned
+    // vectors, as examples/BackendBenchmark.cpp:60-140 times a backend.
+    // Returns the seconds of the timed calls.
+    double ref_diffuse_calls_timed(const double *field, double rate, int32_t X, int32_t Y, int32_t wrapX,
+                                   int32_t wrapY, int32_t adi, int32_t calls)
+    {
+        size_t n = static_cast<size_t>(X) * Y;
+        std::vector<double> f(field, field + n);
+        std::vector<double> d(n, 0.0);
+        Hal::CpuDiffusionBackend2D explicitBackend;
+        Hal::CpuAdiDiffusionBackend2D adiBackend;
+        auto call = [&]()
+        {
+            if (adi != 0)
+            {
+                adiBackend.DiffuseADI(f, d, rate, X, Y, wrapX != 0, wrapY != 0);
+            }
+            else
+            {
+                explicitBackend.Diffuse(f, d, rate, X, Y, wrapX != 0, wrapY != 0);
+            }
+        };
+        call();
+        auto t0 = std::chrono::steady_clock::now();
+        for (int32_t c = 0; c < calls; c++)
+        {
+            call();
+        }
+        auto t1 = std::chrono::steady_clock::now();
+        return std::chrono::duration<double>(t1 - t0).count();
+    }
+
     // Hal::Internal::Advection2 with a uniform velocity (zero flux / periodic,
     // or Dirichlet edge values when xlo is non-null).
     void ref_advect(const double *field, double *deltas, double xVel, double yVel, int32_t X, int32_t Y,
