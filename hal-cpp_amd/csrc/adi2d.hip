@@ -378,7 +378,7 @@ struct halhip_adi2d
     StreamClock clock;
     Profiler prof;
     GraphCache graphs;
-    std::vector<float> hostBuf;
+    HostStaging<float> hostBuf; // pinned staging of the per-call FP64 paths
 };
 
 namespace

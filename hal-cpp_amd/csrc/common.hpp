@@ -93,6 +93,48 @@ namespace halhip
         Check(hipSetDevice(ctx->device), "hipSetDevice");
     }
 
+    // Pinned (page-locked) host staging for the per-call FP64 interfaces:
+    // copies between it and the device run at full link speed, which
+    // pageable memory does not.  Grows only; the contents are not kept
+    // across a resize (every user writes before it reads).
+    template <typename T>
+    class HostStaging
+    {
+    public:
+        HostStaging() = default;
+        HostStaging(const HostStaging &) = delete;
+        HostStaging &operator=(const HostStaging &) = delete;
+        ~HostStaging() { Release(); }
+
+        void resize(size_t count)
+        {
+            if (count <= m_count)
+            {
+                return;
+            }
+            Release();
+            Check(hipHostMalloc(reinterpret_cast<void **>(&m_ptr), count * sizeof(T), hipHostMallocDefault),
+                  "hipHostMalloc");
+            m_count = count;
+        }
+        void Release()
+        {
+            if (m_ptr != nullptr)
+            {
+                (void)hipHostFree(m_ptr);
+                m_ptr = nullptr;
+                m_count = 0;
+            }
+        }
+        T *data() { return m_ptr; }
+        T &operator[](size_t k) { return m_ptr[k]; }
+        size_t size() const { return m_count; }
+
+    private:
+        T *m_ptr = nullptr;
+        size_t m_count = 0;
+    };
+
     // Device allocation owned by a plan; freed on destruction.
     template <class T>
     class DeviceArray

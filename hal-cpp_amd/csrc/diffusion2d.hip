@@ -488,7 +488,7 @@ struct halhip_diffusion2d
     StreamClock clock;
     Profiler prof;
     GraphCache graphs;
-    std::vector<float> hostBuf;
+    HostStaging<float> hostBuf; // pinned staging of the per-call FP64 paths
 };
 
 namespace
