@@ -1640,13 +1640,15 @@ namespace
             LoadRow<CPT, true>(a.carry + base, y0, a.Y, cf);
             LoadRow<CPT, true>(a.carry + plane + base, y0, a.Y, dn);
         }
+        // RhsAFromLds reads only the neighbours' edge columns (and, on a
+        // periodic y axis, the row's far ends, which are edge columns too)
 #pragma unroll
         for (int r = 0; r < S; r++)
         {
-#pragma unroll
-            for (int c = 0; c < CPT; c++)
+            rows[r * rowStride + RowAddr<E>(y0)] = v[r][0];
+            if constexpr (CPT > 1)
             {
-                rows[r * rowStride + RowAddr<E>(y0 + c)] = v[r][c];
+                rows[r * rowStride + RowAddr<E>(y0 + CPT - 1)] = v[r][CPT - 1];
             }
         }
         __syncthreads();
@@ -1852,15 +1854,16 @@ namespace
                     StoreRow<CPT, true>(u + static_cast<size_t>(s0 + r) * a.Y, y0, a.Y, v[r]);
                 }
             }
-            // RhsAFromLds reads the rows of u' from LDS; each thread rewrites
-            // only the columns it read itself
+            // RhsAFromLds reads the neighbours' edge columns of u' from LDS;
+            // each thread rewrites its own edge columns (the rest of the row
+            // in LDS is not read again)
 #pragma unroll
             for (int r = 0; r < S; r++)
             {
-#pragma unroll
-                for (int c = 0; c < CPT; c++)
+                rows[r * rowStride + RowAddr<E>(y0)] = v[r][0];
+                if constexpr (CPT > 1)
                 {
-                    rows[r * rowStride + RowAddr<E>(y0 + c)] = v[r][c];
+                    rows[r * rowStride + RowAddr<E>(y0 + CPT - 1)] = v[r][CPT - 1];
                 }
             }
             __syncthreads();
