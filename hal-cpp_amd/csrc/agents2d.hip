@@ -219,20 +219,12 @@ namespace
         }
     }
 
-    // Sorts every cell's slice of slots by agent index, moving position,
+    // Sorts one cell's slice [s, e) of slots by agent index, moving position,
     // radius and (when kept) carried velocity along (insertion sort: slices
     // hold a handful of agents).
-    __global__ __launch_bounds__(kThreads) void AgentOrderCells(int64_t C, const int32_t *__restrict__ offsets,
-                                                                int32_t *__restrict__ id, float2 *__restrict__ pos,
-                                                                float *__restrict__ rad, float2 *__restrict__ carry)
+    __device__ __forceinline__ void OrderSlice(int s, int e, int32_t *__restrict__ id, float2 *__restrict__ pos,
+                                               float *__restrict__ rad, float2 *__restrict__ carry)
     {
-        const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-        if (c >= C)
-        {
-            return;
-        }
-        const int s = offsets[c];
-        const int e = offsets[c + 1];
         for (int a = s + 1; a < e; a++)
         {
             const int key = id[a];
@@ -261,6 +253,31 @@ namespace
             if (carry != nullptr)
             {
                 carry[b + 1] = kc;
+            }
+        }
+    }
+
+    // Sorts every cell's slice, four consecutive cells per thread: their five
+    // offsets are one int4 and one int (offsets holds whole scan tiles plus
+    // 4 entries, and c0 is a multiple of 4, so both loads are in bounds).
+    __global__ __launch_bounds__(kThreads) void AgentOrderCells(int64_t C, const int32_t *__restrict__ offsets,
+                                                                int32_t *__restrict__ id, float2 *__restrict__ pos,
+                                                                float *__restrict__ rad, float2 *__restrict__ carry)
+    {
+        const int64_t c0 = 4 * (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
+        if (c0 >= C)
+        {
+            return;
+        }
+        const int4 o = *reinterpret_cast<const int4 *>(offsets + c0);
+        const int o4 = offsets[c0 + 4];
+        const int b[5] = {o.x, o.y, o.z, o.w, o4};
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+        {
+            if (c0 + j < C && b[j + 1] - b[j] > 1)
+            {
+                OrderSlice(b[j], b[j + 1], id, pos, rad, carry);
             }
         }
     }
@@ -684,7 +701,7 @@ namespace
                    positions, static_cast<const float *>(p->rad[c].get()),
                    static_cast<const float2 *>(p->motion ? p->carry[c].get() : nullptr), p->id[n].get(),
                    p->pos[n].get(), p->rad[n].get(), p->motion ? p->carry[n].get() : nullptr);
-            Launch(p, "agents_order", AgentOrderCells, Blocks(p->C, kThreads), kThreads, p->C,
+            Launch(p, "agents_order", AgentOrderCells, Blocks((p->C + 3) / 4, kThreads), kThreads, p->C,
                    static_cast<const int32_t *>(p->offsets.get()), p->id[n].get(), p->pos[n].get(),
                    p->rad[n].get(), p->motion ? p->carry[n].get() : nullptr);
         }
