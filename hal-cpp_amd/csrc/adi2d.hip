@@ -379,6 +379,7 @@ struct halhip_adi2d
     Profiler prof;
     GraphCache graphs;
     HostStaging<float> hostBuf; // pinned staging of the per-call FP64 paths
+    ChunkedTransfer chunks;
 };
 
 namespace
@@ -1039,24 +1040,11 @@ extern "C"
                 Require(field != nullptr && deltas != nullptr, "halhip_adi2d_diffuse_f64: null array");
                 const size_t cells = Cells(p);
                 p->hostBuf.resize(cells);
-                for (size_t k = 0; k < cells; k++)
-                {
-                    p->hostBuf[k] = static_cast<float>(field[k]);
-                }
-                Check(hipMemcpyAsync(p->field.get(), p->hostBuf.data(), cells * sizeof(float), hipMemcpyHostToDevice,
-                                     p->clock.stream),
-                      "hipMemcpyAsync");
+                p->chunks.Upload(field, cells, p->hostBuf.data(), p->field.get(), p->clock.stream);
                 p->reducedValid = false;
                 EnqueueTicks(p, 1, diffRate, 0.0, wrapX, wrapY, bc, false);
-                Check(hipMemcpyAsync(p->hostBuf.data(), p->field.get(), cells * sizeof(float), hipMemcpyDeviceToHost,
-                                     p->clock.stream),
-                      "hipMemcpyAsync");
-                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                p->chunks.DownloadAdd(p->field.get(), cells, p->hostBuf.data(), deltas, field, p->clock.stream);
                 p->prof.Collect();
-                for (size_t k = 0; k < cells; k++)
-                {
-                    deltas[k] += static_cast<double>(p->hostBuf[k]) - field[k];
-                }
             });
     }
 

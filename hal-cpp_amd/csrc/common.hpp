@@ -135,6 +135,116 @@ namespace halhip
         size_t m_count = 0;
     };
 
+    // The per-call FP64 paths' copies in pieces of at least kMinPiece values
+    // (at most kChunks) through pinned staging: the host converts piece c+1
+    // while piece c is copied up, and adds piece c into the caller's array
+    // while piece c+1 comes down.  Smaller arrays move in one piece, where a
+    // copy's fixed cost would outweigh the overlap.
+    class ChunkedTransfer
+    {
+    public:
+        static constexpr int kChunks = 8;
+        static constexpr size_t kMinPiece = 256 * 1024;
+        ChunkedTransfer() = default;
+        ChunkedTransfer(const ChunkedTransfer &) = delete;
+        ChunkedTransfer &operator=(const ChunkedTransfer &) = delete;
+        ~ChunkedTransfer()
+        {
+            for (hipEvent_t e : m_ev)
+            {
+                if (e != nullptr)
+                {
+                    (void)hipEventDestroy(e);
+                }
+            }
+        }
+
+        // dev[0, n) = float(src[0, n)), enqueued on s.
+        void Upload(const double *src, size_t n, float *staging, float *dev, hipStream_t s)
+        {
+            const int pieces = Pieces(n);
+            for (int c = 0; c < pieces; c++)
+            {
+                const size_t b = Begin(n, c, pieces);
+                const size_t e = Begin(n, c + 1, pieces);
+                for (size_t k = b; k < e; k++)
+                {
+                    staging[k] = static_cast<float>(src[k]);
+                }
+                if (e > b)
+                {
+                    Check(hipMemcpyAsync(dev + b, staging + b, (e - b) * sizeof(float), hipMemcpyHostToDevice, s),
+                          "hipMemcpyAsync");
+                }
+            }
+        }
+
+        // dst[k] += double(dev[k]) (minus base[k] when base is given) once
+        // the work queued on s before this call has finished; returns when
+        // every piece has been added.
+        void DownloadAdd(const float *dev, size_t n, float *staging, double *dst, const double *base, hipStream_t s)
+        {
+            if (m_ev[0] == nullptr)
+            {
+                for (hipEvent_t &e : m_ev)
+                {
+                    Check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreateWithFlags");
+                }
+            }
+            const int pieces = Pieces(n);
+            for (int c = 0; c < pieces; c++)
+            {
+                const size_t b = Begin(n, c, pieces);
+                const size_t e = Begin(n, c + 1, pieces);
+                if (e > b)
+                {
+                    Check(hipMemcpyAsync(staging + b, dev + b, (e - b) * sizeof(float), hipMemcpyDeviceToHost, s),
+                          "hipMemcpyAsync");
+                }
+                if (pieces > 1)
+                {
+                    Check(hipEventRecord(m_ev[c], s), "hipEventRecord");
+                }
+            }
+            for (int c = 0; c < pieces; c++)
+            {
+                if (pieces > 1)
+                {
+                    Check(hipEventSynchronize(m_ev[c]), "hipEventSynchronize");
+                }
+                else
+                {
+                    Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+                }
+                const size_t b = Begin(n, c, pieces);
+                const size_t e = Begin(n, c + 1, pieces);
+                if (base != nullptr)
+                {
+                    for (size_t k = b; k < e; k++)
+                    {
+                        dst[k] += static_cast<double>(staging[k]) - base[k];
+                    }
+                }
+                else
+                {
+                    for (size_t k = b; k < e; k++)
+                    {
+                        dst[k] += static_cast<double>(staging[k]);
+                    }
+                }
+            }
+        }
+
+    private:
+        static int Pieces(size_t n)
+        {
+            const size_t p = n / kMinPiece;
+            return static_cast<int>(p < 1 ? 1 : (p > static_cast<size_t>(kChunks) ? kChunks : p));
+        }
+        static size_t Begin(size_t n, int c, int pieces) { return n * static_cast<size_t>(c) / pieces; }
+        hipEvent_t m_ev[kChunks] = {};
+    };
+
     // Device allocation owned by a plan; freed on destruction.
     template <class T>
     class DeviceArray

@@ -489,6 +489,7 @@ struct halhip_diffusion2d
     Profiler prof;
     GraphCache graphs;
     HostStaging<float> hostBuf; // pinned staging of the per-call FP64 paths
+    ChunkedTransfer chunks;
 };
 
 namespace
@@ -774,31 +775,18 @@ extern "C"
                 Require(field != nullptr && deltas != nullptr, "halhip_diffusion2d_diffuse_f64: null array");
                 size_t cells = Cells(p);
                 p->hostBuf.resize(cells);
-                for (size_t k = 0; k < cells; k++)
-                {
-                    p->hostBuf[k] = static_cast<float>(field[k]);
-                }
-                Check(hipMemcpyAsync(p->aux[0].get(), p->hostBuf.data(), cells * sizeof(float),
-                                     hipMemcpyHostToDevice, p->clock.stream),
-                      "hipMemcpyAsync");
                 ExplicitArgs a = MakeArgs(p, diffRate, 0.0, wrapX, wrapY, bc);
                 if (p->aux[1].get() == nullptr)
                 {
                     p->aux[1].Allocate(p->ctx, cells);
                 }
+                p->chunks.Upload(field, cells, p->hostBuf.data(), p->aux[0].get(), p->clock.stream);
                 p->prof.Begin(p->clock.stream, "explicit_delta");
                 hipLaunchKernelGGL(ExplicitStep<false>, Grid(p), dim3(kBlockY, kBlockX), 0, p->clock.stream, a,
                                    p->aux[0].get(), p->aux[1].get());
                 Check(hipGetLastError(), "ExplicitStep launch");
                 p->prof.End(p->clock.stream);
-                Check(hipMemcpyAsync(p->hostBuf.data(), p->aux[1].get(), cells * sizeof(float),
-                                     hipMemcpyDeviceToHost, p->clock.stream),
-                      "hipMemcpyAsync");
-                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
-                for (size_t k = 0; k < cells; k++)
-                {
-                    deltas[k] += static_cast<double>(p->hostBuf[k]);
-                }
+                p->chunks.DownloadAdd(p->aux[1].get(), cells, p->hostBuf.data(), deltas, nullptr, p->clock.stream);
             });
     }
 
