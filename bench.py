@@ -276,6 +276,7 @@ def bench_per_call(ctx, cpu=True):
     CpuDiffusionBackend2D / CpuAdiDiffusionBackend2D on the same arrays."""
     import numpy as np
     import halhip
+    import workloads
     ref = reference_lib() if cpu else None
     cases = []
     for n in (256, 1024):
@@ -302,9 +303,31 @@ def bench_per_call(ctx, cpu=True):
                 case["cpu_ms_per_call"] = round(cpu_ms, 4)
                 case["speedup_vs_cpu"] = round(cpu_ms / gpu_ms, 2)
             cases.append(case)
-    return {"workload": "per-call interfaces on FP64 host arrays, periodic, r = 0.2 (explicit) and 0.8 (ADI)",
+    # IAgentForceBackend2D::AccumulateForces (include/Hal/Agents/IAgentForceBackend2D.h:28-36):
+    # upload, bin, forces and the download of the velocity deltas in every call
+    aw = workloads.agents_uniform(n_agents=1_000_000, dim=1024)
+    pos = aw["pos"].astype(np.float64)
+    rad = aw["rad"].astype(np.float64)
+    n_ag = rad.size
+    plan = halhip.Agents2D(ctx, aw["X"], aw["Y"], n_ag)
+    for _ in range(3):
+        plan.accumulate_forces(pos, rad, aw["R"], aw["k"], 0, 0)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        plan.accumulate_forces(pos, rad, aw["R"], aw["k"], 0, 0)
+    gpu_ms = (time.perf_counter() - t0) / 10 * 1e3
+    plan.close()
+    case = {"interface": "AccumulateForces", "agents": n_ag, "bins": [aw["X"], aw["Y"]], "gpu_ms_per_call": round(gpu_ms, 4)}
+    if ref is not None:
+        cpu_ms = ref.agent_forces_timed(pos, rad, 1, aw["R"], aw["k"], aw["X"], aw["Y"], 0, 0) * 1e3
+        case["cpu_ms_per_call"] = round(cpu_ms, 4)
+        case["speedup_vs_cpu"] = round(cpu_ms / gpu_ms, 2)
+    cases.append(case)
+    return {"workload": "per-call interfaces on FP64 host arrays: Diffuse / DiffuseADI periodic, r = 0.2 (explicit) "
+                        "and 0.8 (ADI); AccumulateForces on 1M agents over 1024^2 bins (r = 0.5, R = 1, k = 1)",
             "cases": cases,
-            "note": "each call includes the FP64->FP32 conversion, upload, download and FP64 accumulation; "
+            "note": "each call includes the FP64->FP32 conversion, upload, download and the FP64 result "
+                    "(accumulated into deltas for the diffusion interfaces, written for the forces); "
                     "CPU: the reference backends on one thread"}
 
 
