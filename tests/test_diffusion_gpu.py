@@ -73,6 +73,23 @@ def test_uniform_per_cell_rates_equal_uniform_stencil(ctx):
             np.testing.assert_allclose(got, want, rtol=0, atol=1e-7)
 
 
+@pytest.mark.parametrize("X,Y", [(1000, 704), (1536, 1088)])
+def test_per_call_in_pieces_matches_oracle(ctx, oracle, X, Y):
+    # Per-call arrays of 256K values and more are copied in several pieces,
+    # the host converting one piece while another is copied: 704,000 and
+    # 1,671,168 cells make 2 and 6 pieces of uneven size.  The deltas start
+    # non-zero, so the accumulation is checked too.
+    f = np.random.default_rng(X + Y).random(X * Y)
+    d0 = np.random.default_rng(1).random(X * Y)
+    for adi in (False, True):
+        plan = halhip.Adi2D(ctx, X, Y) if adi else halhip.Diffusion2D(ctx, X, Y)
+        rate = 0.8 if adi else 0.2
+        got = plan.diffuse(f, d0.copy(), rate, True, False)
+        want = (oracle.adi if adi else oracle.diffuse)(f, d0.copy(), rate, X, Y, 1, 0)
+        np.testing.assert_allclose(got, want, rtol=0, atol=3e-6, err_msg=f"adi={adi}")
+        plan.close()
+
+
 def test_rates_interface(ctx, golden, oracle):
     f, rx, ry = golden["rates_field"], golden["rates_x"], golden["rates_y"]
     plan = halhip.Diffusion2D(ctx, 9, 11)
