@@ -44,16 +44,18 @@ def test_bins_bit_exact_and_forces(ctx, golden, oracle):
     np.testing.assert_array_equal(bucket, rb)
 
 
-def test_crowded_cells_order(ctx, oracle):
-    # Many agents per cell exercise the per-cell ordering pass.
+@pytest.mark.parametrize("X,Y", [(5, 4), (7, 9), (3, 1)])
+def test_crowded_cells_order(ctx, oracle, X, Y):
+    # Many agents per cell exercise the per-cell ordering pass, which takes
+    # four cells per thread: 63 and 3 cells leave a partial last group.
     rng = np.random.default_rng(7)
     n = 20000
-    pos = (rng.random(2 * n) * 6.0 - 1.0).astype(np.float32)       # 6x6 grid, some clamped
-    plan = halhip.Agents2D(ctx, 5, 4, n)
+    pos = (rng.random(2 * n) * np.tile([X + 2.0, Y + 2.0], n) - 1.0).astype(np.float32)   # some clamped
+    plan = halhip.Agents2D(ctx, X, Y, n)
     plan.upload(pos, np.full(n, 0.1, np.float32))
     plan.run_ticks(0, 0.05, 1.0, 0.0, 0, 0)
     offsets, bucket = plan.bins()
-    ro, rb = oracle.agent_bins(pos.astype(np.float64), 5, 4)
+    ro, rb = oracle.agent_bins(pos.astype(np.float64), X, Y)
     np.testing.assert_array_equal(offsets, ro)
     np.testing.assert_array_equal(bucket, rb)
 
