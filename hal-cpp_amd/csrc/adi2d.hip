@@ -55,6 +55,126 @@ namespace
         bool vec;
     };
 
+    // Scan of the strip reductions for any strip count (reloads from memory).
+    // Block: 64 columns (lanes) x G wavefronts; wavefront g owns strips
+    // [g*Q, (g+1)*Q).  Writes the carry-in values of every strip and, per
+    // column, the periodic Sherman-Morrison factor and the wrapped rows of u*.
+    __global__ __launch_bounds__(1024) void AdiStripScanGeneric(AdiArgs a, int32_t Q)
+    {
+        __shared__ float sB[kMaxWaves][64];
+        __shared__ float sA[kMaxWaves];
+        __shared__ float sB2[kMaxWaves][64];
+        __shared__ float sA2[kMaxWaves];
+        __shared__ float sCf[64];
+        __shared__ float sX0[64];
+        const int lane = threadIdx.x & 63;
+        const int g = threadIdx.x >> 6;
+        const int G = blockDim.x >> 6;
+        const int y = blockIdx.x * 64 + lane;
+        const bool active = y < a.Y;
+        const int p0 = g * Q;
+        const int p1 = min(a.P, p0 + Q);
+        const size_t plane = static_cast<size_t>(a.P) * a.Y;
+        const float *B = a.red;
+        const float *D = a.red + plane;
+        float *cfIn = a.carry;
+        float *xbIn = a.carry + plane;
+
+        float acc = 0.0f;
+        float aSeg = 1.0f;
+        for (int p = p0; p < p1; p++)
+        {
+            const float A = a.stripA[p];
+            acc = A * acc + (active ? B[static_cast<size_t>(p) * a.Y + y] : 0.0f);
+            aSeg *= A;
+        }
+        sB[g][lane] = acc;
+        if (lane == 0)
+        {
+            sA[g] = aSeg;
+        }
+        __syncthreads();
+        float cf = 0.0f;
+        for (int w = 0; w < g; w++)
+        {
+            cf = sA[w] * cf + sB[w][lane];
+        }
+        for (int p = p0; p < p1; p++)
+        {
+            const size_t k = static_cast<size_t>(p) * a.Y + y;
+            const float bv = active ? B[k] : 0.0f;
+            if (active)
+            {
+                cfIn[k] = cf;
+            }
+            cf = a.stripA[p] * cf + bv;
+        }
+        if (p0 < p1 && p1 == a.P)
+        {
+            sCf[lane] = cf;
+        }
+
+        acc = 0.0f;
+        aSeg = 1.0f;
+        for (int p = p1 - 1; p >= p0; p--)
+        {
+            const size_t k = static_cast<size_t>(p) * a.Y + y;
+            const float C = a.stripC[p];
+            const float off = active ? (D[k] + a.stripE[p] * cfIn[k]) : 0.0f;
+            acc = C * acc + off;
+            aSeg *= C;
+        }
+        sB2[g][lane] = acc;
+        if (lane == 0)
+        {
+            sA2[g] = aSeg;
+        }
+        __syncthreads();
+        float xb = 0.0f;
+        for (int w = G - 1; w > g; w--)
+        {
+            xb = sA2[w] * xb + sB2[w][lane];
+        }
+        for (int p = p1 - 1; p >= p0; p--)
+        {
+            const size_t k = static_cast<size_t>(p) * a.Y + y;
+            float off = 0.0f;
+            if (active)
+            {
+                xbIn[k] = xb;
+                off = D[k] + a.stripE[p] * cfIn[k];
+            }
+            xb = a.stripC[p] * xb + off;
+        }
+        if (g == 0)
+        {
+            sX0[lane] = xb;
+        }
+        __syncthreads();
+        if (g == 0 && active)
+        {
+            const float x0 = sX0[lane];
+            const float xLast = a.xibLast * sCf[lane];
+            const float factor = (x0 + xLast * a.xvEnd) * a.xInv;
+            a.colInfo[y] = factor;
+            a.colInfo[a.Y + y] = x0 - factor * a.xz0;
+            a.colInfo[2 * static_cast<size_t>(a.Y) + y] = xLast - factor * a.xzLast;
+        }
+    }
+
+    // Point sources, applied serially so repeated indices add in list order.
+    __global__ void AdiAddSources(float *__restrict__ u, int32_t n, const int64_t *__restrict__ idx,
+                                  const float *__restrict__ val)
+    {
+        if (threadIdx.x == 0 && blockIdx.x == 0)
+        {
+            for (int k = 0; k < n; k++)
+            {
+                u[idx[k]] += val[k];
+            }
+        }
+    }
+
     bool ShapeCompiled(int32_t cpt, int32_t s)
     {
         return (cpt == 1 || cpt == 2 || cpt == 4) ? (s == 4 || s == 8 || s == 16) : (cpt == 8 && (s == 4 || s == 8));
