@@ -498,6 +498,7 @@ struct halhip_adi2d
     StreamClock clock;
     Profiler prof;
     GraphCache graphs;
+    bool prepareOnly = false;   // inside halhip_adi2d_prepare_steps
     HostStaging<float> hostBuf; // pinned staging of the per-call FP64 paths
     ChunkedTransfer chunks;
 };
@@ -878,7 +879,11 @@ namespace
     }
 
     // `steps` ticks of the loop body `tick`, replayed as 20-tick graphs
-    // keyed by `key` (even: a windowed chunk leaves redCur as it found it).
+    // keyed by `key` and the reduction half the chunk starts from (even: a
+    // windowed chunk leaves redCur as it found it).  In a prepare call the
+    // chunk is captured for both halves, so a later run of this length
+    // replays a graph whatever ticks ran in between, and nothing is
+    // enqueued.
     template <class F>
     void RunChunked(halhip_adi2d *p, int32_t steps, std::string key, F tick)
     {
@@ -887,25 +892,41 @@ namespace
         if (!p->prof.On() && steps >= kChunk)
         {
             AppendKey(key, kChunk);
-            AppendKey(key, p->redCur);
-            hipGraphExec_t exec = p->graphs.Find(key);
-            if (exec == nullptr)
+            auto chunk = [&](int32_t half)
             {
-                hipGraph_t graph;
-                Check(hipStreamBeginCapture(p->clock.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-                for (int32_t k = 0; k < kChunk; k++)
+                std::string k = key;
+                AppendKey(k, half);
+                const int32_t saved = p->redCur;
+                p->redCur = half;
+                hipGraphExec_t exec = FindOrCapture(p->graphs, k, p->clock.stream,
+                                                    [&]
+                                                    {
+                                                        for (int32_t t = 0; t < kChunk; t++)
+                                                        {
+                                                            tick();
+                                                        }
+                                                    });
+                p->redCur = saved;
+                return exec;
+            };
+            if (p->prepareOnly)
+            {
+                chunk(p->redCur);
+                if (p->useCarry == HALHIP_ADI_CARRIES_WINDOW)
                 {
-                    tick();
+                    chunk(1 - p->redCur);
                 }
-                Check(hipStreamEndCapture(p->clock.stream, &graph), "hipStreamEndCapture");
-                Check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-                (void)hipGraphDestroy(graph);
-                p->graphs.Insert(key, exec);
+                return;
             }
+            hipGraphExec_t exec = chunk(p->redCur);
             for (; done + kChunk <= steps; done += kChunk)
             {
                 Check(hipGraphLaunch(exec, p->clock.stream), "hipGraphLaunch");
             }
+        }
+        if (p->prepareOnly)
+        {
+            return;
         }
         for (; done < steps; done++)
         {
@@ -927,12 +948,12 @@ namespace
         p->args.red = RedHalf(p, p->redCur);
         if (plain)
         {
-            if (!p->reducedValid)
+            if (!p->reducedValid && !p->prepareOnly)
             {
                 LaunchStrip(p, 0, "adi_strip_reduce");
             }
             RunChunked(p, steps, p->tableKey, [p] { PlainTick(p); });
-            p->reducedValid = true;
+            p->reducedValid = p->reducedValid || !p->prepareOnly;
             return;
         }
         p->args.negDecay = static_cast<float>(-decay);
@@ -948,7 +969,7 @@ namespace
             p->args.srcStrip = src ? p->srcStrip.get() : nullptr;
             p->args.srcXY = src ? p->srcXY.get() : nullptr;
             p->args.srcVal = src ? p->srcStripVal.get() : nullptr;
-            if (!p->reducedValid)
+            if (!p->reducedValid && !p->prepareOnly)
             {
                 LaunchStrip(p, 0, "adi_strip_reduce");
             }
@@ -957,8 +978,12 @@ namespace
             AppendKey(key, p->args.negDecay);
             AppendKey(key, src ? 1 : 0);
             RunChunked(p, steps, key, [p] { PlainTick(p, true); });
-            p->reducedValid = true;
+            p->reducedValid = p->reducedValid || !p->prepareOnly;
             return;
+        }
+        if (p->prepareOnly)
+        {
+            return; // the register path's model ticks launch directly
         }
         for (int32_t t = 0; t < steps; t++)
         {
@@ -1188,6 +1213,18 @@ extern "C"
                 RequirePlan(p, "halhip_adi2d_sync");
                 p->clock.Sync(elapsedMs);
                 p->prof.Collect();
+            });
+    }
+
+    int halhip_adi2d_prepare_steps(halhip_adi2d *p, int32_t steps, double diffRate, double decay, int32_t wrapX,
+                                   int32_t wrapY, const halhip_bc *bc)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_adi2d_prepare_steps");
+                PrepareScope scope(p->prepareOnly);
+                EnqueueTicks(p, steps, diffRate, decay, wrapX, wrapY, bc, true);
             });
     }
 

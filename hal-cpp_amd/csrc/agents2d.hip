@@ -639,6 +639,7 @@ struct halhip_agents2d
     StreamClock clock;
     Profiler prof;
     GraphCache graphs;
+    bool prepareOnly = false;   // inside halhip_agents2d_prepare_ticks
     HostStaging<float> hostBuf; // pinned staging of the per-call FP64 paths
 };
 
@@ -807,6 +808,10 @@ namespace
         const float fdt = static_cast<float>(dt);
         if (ticks == 0)
         {
+            if (p->prepareOnly)
+            {
+                return;
+            }
             EnqueueBins(p);
             EnqueueForces(p, fR, fk, wrapX, wrapY, false, 0.0f);
             p->binsValid = true;
@@ -815,14 +820,15 @@ namespace
         // 10-tick graph chunks.  A chunk leaves `cur` as it found it (even
         // length) and the positions moved; the first chunk after an upload or
         // a forces-only pass starts from the binned set instead, so `moved`
-        // is part of the key and each chunk is looked up on its own.
+        // is part of the key and each chunk is looked up on its own.  A
+        // prepare call captures the chunk from both buffer sets, moved and
+        // (if it is not) as it is now, and enqueues nothing.
         constexpr int32_t kChunk = 10;
-        int32_t done = 0;
-        while (!p->prof.On() && ticks - done >= kChunk)
+        auto chunkGraph = [&](int32_t c0, bool moved0)
         {
             std::string key;
             AppendKey(key, p->N);
-            AppendKey(key, p->cur);
+            AppendKey(key, c0);
             AppendKey(key, fR);
             AppendKey(key, fk);
             AppendKey(key, fdt);
@@ -831,27 +837,46 @@ namespace
             AppendKey(key, p->motion ? 1 : 0);
             AppendKey(key, p->friction);
             AppendKey(key, p->maxVel);
-            AppendKey(key, p->moved ? 1 : 0);
-            hipGraphExec_t exec = p->graphs.Find(key);
-            if (exec == nullptr)
+            AppendKey(key, moved0 ? 1 : 0);
+            const int32_t savedCur = p->cur;
+            const bool savedMoved = p->moved;
+            p->cur = c0;
+            p->moved = moved0;
+            // capturing runs the ticks' host-side bookkeeping too, which
+            // leaves cur as it was and moved set
+            hipGraphExec_t exec = FindOrCapture(p->graphs, key, p->clock.stream,
+                                                [&]
+                                                {
+                                                    for (int32_t t = 0; t < kChunk; t++)
+                                                    {
+                                                        EnqueueTick(p, fR, fk, fdt, wrapX, wrapY);
+                                                    }
+                                                });
+            p->cur = savedCur;
+            p->moved = savedMoved;
+            return exec;
+        };
+        int32_t done = 0;
+        if (p->prepareOnly)
+        {
+            if (!p->prof.On() && ticks >= kChunk)
             {
-                // capturing enqueues the ticks' host-side bookkeeping too
-                hipGraph_t graph;
-                Check(hipStreamBeginCapture(p->clock.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-                for (int32_t t = 0; t < kChunk; t++)
+                for (int32_t c0 = 0; c0 < 2; c0++)
                 {
-                    EnqueueTick(p, fR, fk, fdt, wrapX, wrapY);
+                    chunkGraph(c0, true);
+                    if (!p->moved)
+                    {
+                        chunkGraph(c0, false);
+                    }
                 }
-                Check(hipStreamEndCapture(p->clock.stream, &graph), "hipStreamEndCapture");
-                Check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-                (void)hipGraphDestroy(graph);
-                p->graphs.Insert(key, exec);
             }
-            else
-            {
-                p->moved = true;
-            }
+            return;
+        }
+        while (!p->prof.On() && ticks - done >= kChunk)
+        {
+            hipGraphExec_t exec = chunkGraph(p->cur, p->moved);
             Check(hipGraphLaunch(exec, p->clock.stream), "hipGraphLaunch");
+            p->moved = true;
             done += kChunk;
         }
         for (; done < ticks; done++)
@@ -1138,6 +1163,18 @@ extern "C"
                 RequirePlan(p, "halhip_agents2d_sync");
                 p->clock.Sync(elapsedMs);
                 p->prof.Collect();
+            });
+    }
+
+    int halhip_agents2d_prepare_ticks(halhip_agents2d *p, int32_t ticks, double interactionRad,
+                                      double springConstant, double dt, int32_t wrapX, int32_t wrapY)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_agents2d_prepare_ticks");
+                PrepareScope scope(p->prepareOnly);
+                EnqueueTicks(p, ticks, interactionRad, springConstant, dt, wrapX, wrapY);
             });
     }
 

@@ -170,6 +170,7 @@ struct halhip_batch
     StreamClock clock;
     Profiler prof;
     GraphCache graphs;
+    bool prepareOnly = false; // inside halhip_batch_prepare_ticks
 };
 
 namespace
@@ -497,35 +498,53 @@ namespace
         {
             return;
         }
-        if (!b->reducedValid)
+        if (!b->reducedValid && !b->prepareOnly)
         {
             LaunchBatchStrip(b, 0, 0, b->nStripItems, "batch_strip_reduce", true);
         }
-        constexpr int32_t kChunk = 10; // even: a windowed chunk leaves `cur` as it found it
-        int32_t done = 0;
-        if (!b->prof.On() && ticks >= kChunk)
+        // 10-tick graph chunks keyed by the reduction half they start from
+        // (even: a windowed chunk leaves `cur` as it found it); a prepare
+        // call captures both halves and enqueues nothing.
+        constexpr int32_t kChunk = 10;
+        auto chunkGraph = [&](int32_t half)
         {
             std::string key = b->builtKey;
             AppendKey(key, kChunk);
-            AppendKey(key, b->cur);
-            hipGraphExec_t exec = b->graphs.Find(key);
-            if (exec == nullptr)
+            AppendKey(key, half);
+            const int32_t saved = b->cur;
+            b->cur = half;
+            hipGraphExec_t exec = FindOrCapture(b->graphs, key, b->clock.stream,
+                                                [&]
+                                                {
+                                                    for (int32_t k = 0; k < kChunk; k++)
+                                                    {
+                                                        BatchTick(b);
+                                                    }
+                                                });
+            b->cur = saved;
+            return exec;
+        };
+        int32_t done = 0;
+        if (!b->prof.On() && ticks >= kChunk)
+        {
+            if (b->prepareOnly)
             {
-                hipGraph_t graph;
-                Check(hipStreamBeginCapture(b->clock.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-                for (int32_t k = 0; k < kChunk; k++)
+                chunkGraph(b->cur);
+                if (b->window)
                 {
-                    BatchTick(b);
+                    chunkGraph(1 - b->cur);
                 }
-                Check(hipStreamEndCapture(b->clock.stream, &graph), "hipStreamEndCapture");
-                Check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-                (void)hipGraphDestroy(graph);
-                b->graphs.Insert(key, exec);
+                return;
             }
+            hipGraphExec_t exec = chunkGraph(b->cur);
             for (; done + kChunk <= ticks; done += kChunk)
             {
                 Check(hipGraphLaunch(exec, b->clock.stream), "hipGraphLaunch");
             }
+        }
+        if (b->prepareOnly)
+        {
+            return;
         }
         for (; done < ticks; done++)
         {
@@ -751,6 +770,17 @@ extern "C"
                 RequireBatch(b, "halhip_batch_sync");
                 b->clock.Sync(elapsedMs);
                 b->prof.Collect();
+            });
+    }
+
+    int halhip_batch_prepare_ticks(halhip_batch *b, int32_t ticks, int32_t wrapX, int32_t wrapY, const halhip_bc *bc)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_prepare_ticks");
+                PrepareScope scope(b->prepareOnly);
+                EnqueueBatchTicks(b, ticks, wrapX, wrapY, bc);
             });
     }
 

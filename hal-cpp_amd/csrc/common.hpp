@@ -353,6 +353,54 @@ namespace halhip
         key.append(reinterpret_cast<const char *>(&v), sizeof(T));
     }
 
+    // The cached graph for `key`, or a new one holding the launches body()
+    // enqueues on s (captured, instantiated and cached).  Host-side state
+    // that body() changes while it is captured changes as if it had run.
+    template <class F>
+    hipGraphExec_t FindOrCapture(GraphCache &cache, const std::string &key, hipStream_t s, F body)
+    {
+        hipGraphExec_t exec = cache.Find(key);
+        if (exec != nullptr)
+        {
+            return exec;
+        }
+        hipGraph_t graph;
+        Check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try
+        {
+            body();
+        }
+        catch (...)
+        {
+            if (hipStreamEndCapture(s, &graph) == hipSuccess)
+            {
+                (void)hipGraphDestroy(graph);
+            }
+            throw;
+        }
+        Check(hipStreamEndCapture(s, &graph), "hipStreamEndCapture");
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        Check(e, "hipGraphInstantiate");
+        cache.Insert(key, exec);
+        return exec;
+    }
+
+    // Sets a plan's "build graphs only" flag for one prepare call: the
+    // run path then captures every graph a run of the same arguments would
+    // replay and enqueues nothing.
+    class PrepareScope
+    {
+    public:
+        explicit PrepareScope(bool &flag) : m_flag(flag) { m_flag = true; }
+        ~PrepareScope() { m_flag = false; }
+        PrepareScope(const PrepareScope &) = delete;
+        PrepareScope &operator=(const PrepareScope &) = delete;
+
+    private:
+        bool &m_flag;
+    };
+
     // Host copy of a Dirichlet description expanded to four edge arrays.
     struct EdgeValues
     {

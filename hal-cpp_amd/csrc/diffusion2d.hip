@@ -488,6 +488,7 @@ struct halhip_diffusion2d
     StreamClock clock;
     Profiler prof;
     GraphCache graphs;
+    bool prepareOnly = false; // inside halhip_diffusion2d_prepare_steps
     HostStaging<float> hostBuf; // pinned staging of the per-call FP64 paths
     ChunkedTransfer chunks;
 };
@@ -622,91 +623,77 @@ namespace
         p->prof.End(p->clock.stream);
     }
 
-    // Enqueues `steps` ticks; full chunks of kChunk ticks replay a cached
-    // graph (an even chunk keeps the ping-pong parity), the rest launch
-    // directly.  With profiling on, every tick is launched directly so each
-    // gets its own event pair.
+    // Enqueues `steps` ticks; full chunks of ticks replay a cached graph
+    // (an even launch count keeps the ping-pong parity, which is part of
+    // the key), the rest launch directly.  With profiling on, every tick is
+    // launched directly so each gets its own event pair.  In a prepare call
+    // the chunk is captured for both parities and nothing is enqueued.
     void EnqueueTicks(halhip_diffusion2d *p, int32_t steps, double rate, double decay, int32_t wrapX, int32_t wrapY,
                       const halhip_bc *bc)
     {
         Require(steps >= 0, "halhip_diffusion2d_run_steps: negative steps");
         ExplicitArgs a = MakeArgs(p, rate, decay, wrapX, wrapY, bc, p->xVel, p->yVel);
         const MultiShape m = ChooseMulti(p);
-        if (m.T > 0)
+        // Graph chunks of six multi-tick launches of T ticks each, or of 50
+        // single-tick launches.
+        const int32_t launches = m.T > 0 ? 6 : 50;
+        const int32_t chunk = m.T > 0 ? launches * m.T : launches;
+        auto launchAll = [&](int c0, int32_t count)
         {
-            // Graph chunks of six launches (an even count keeps the ping-pong
-            // parity) of T ticks each; the rest launch directly.
-            const int32_t chunk = 6 * m.T;
-            int32_t done = 0;
-            if (!p->prof.On() && steps >= chunk)
+            int c = c0;
+            for (int32_t k = 0; k < count; k++)
             {
-                std::string key;
-                AppendKey(key, a);
-                AppendKey(key, m);
-                AppendKey(key, p->cur);
-                hipGraphExec_t exec = p->graphs.Find(key);
-                if (exec == nullptr)
+                if (m.T > 0)
                 {
-                    hipGraph_t graph;
-                    Check(hipStreamBeginCapture(p->clock.stream, hipStreamCaptureModeThreadLocal),
-                          "hipStreamBeginCapture");
-                    int c = p->cur;
-                    for (int32_t k = 0; k < 6; k++)
-                    {
-                        LaunchMulti(p, a, m, m.T, p->field[c].get(), p->field[1 - c].get());
-                        c = 1 - c;
-                    }
-                    Check(hipStreamEndCapture(p->clock.stream, &graph), "hipStreamEndCapture");
-                    Check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-                    (void)hipGraphDestroy(graph);
-                    p->graphs.Insert(key, exec);
+                    LaunchMulti(p, a, m, m.T, p->field[c].get(), p->field[1 - c].get());
                 }
-                for (; done + chunk <= steps; done += chunk)
+                else
                 {
-                    Check(hipGraphLaunch(exec, p->clock.stream), "hipGraphLaunch");
+                    LaunchTick(p, a, p->field[c].get(), p->field[1 - c].get());
                 }
+                c = 1 - c;
             }
-            while (done < steps)
-            {
-                const int32_t t = std::min(m.T, steps - done);
-                LaunchMulti(p, a, m, t, p->field[p->cur].get(), p->field[1 - p->cur].get());
-                p->cur = 1 - p->cur;
-                done += t;
-            }
-            return;
-        }
-        constexpr int32_t kChunk = 50;
-        int32_t done = 0;
-        if (!p->prof.On() && steps >= kChunk)
+        };
+        auto chunkGraph = [&](int c0)
         {
             std::string key;
             AppendKey(key, a);
-            AppendKey(key, p->cur);
-            hipGraphExec_t exec = p->graphs.Find(key);
-            if (exec == nullptr)
+            AppendKey(key, m);
+            AppendKey(key, c0);
+            return FindOrCapture(p->graphs, key, p->clock.stream, [&] { launchAll(c0, launches); });
+        };
+        int32_t done = 0;
+        if (!p->prof.On() && steps >= chunk)
+        {
+            if (p->prepareOnly)
             {
-                hipGraph_t graph;
-                Check(hipStreamBeginCapture(p->clock.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-                int c = p->cur;
-                for (int32_t k = 0; k < kChunk; k++)
-                {
-                    LaunchTick(p, a, p->field[c].get(), p->field[1 - c].get());
-                    c = 1 - c;
-                }
-                Check(hipStreamEndCapture(p->clock.stream, &graph), "hipStreamEndCapture");
-                Check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-                (void)hipGraphDestroy(graph);
-                p->graphs.Insert(key, exec);
+                chunkGraph(p->cur);
+                chunkGraph(1 - p->cur);
+                return;
             }
-            for (; done + kChunk <= steps; done += kChunk)
+            hipGraphExec_t exec = chunkGraph(p->cur);
+            for (; done + chunk <= steps; done += chunk)
             {
                 Check(hipGraphLaunch(exec, p->clock.stream), "hipGraphLaunch");
             }
         }
-        for (; done < steps; done++)
+        if (p->prepareOnly)
         {
-            LaunchTick(p, a, p->field[p->cur].get(), p->field[1 - p->cur].get());
+            return;
+        }
+        while (done < steps)
+        {
+            const int32_t t = m.T > 0 ? std::min(m.T, steps - done) : 1;
+            if (m.T > 0)
+            {
+                LaunchMulti(p, a, m, t, p->field[p->cur].get(), p->field[1 - p->cur].get());
+            }
+            else
+            {
+                LaunchTick(p, a, p->field[p->cur].get(), p->field[1 - p->cur].get());
+            }
             p->cur = 1 - p->cur;
+            done += t;
         }
     }
 }
@@ -1165,6 +1152,18 @@ extern "C"
                 RequirePlan(p, "halhip_diffusion2d_sync");
                 p->clock.Sync(elapsedMs);
                 p->prof.Collect();
+            });
+    }
+
+    int halhip_diffusion2d_prepare_steps(halhip_diffusion2d *p, int32_t steps, double diffRate, double decay,
+                                         int32_t wrapX, int32_t wrapY, const halhip_bc *bc)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_prepare_steps");
+                PrepareScope scope(p->prepareOnly);
+                EnqueueTicks(p, steps, diffRate, decay, wrapX, wrapY, bc);
             });
     }
 

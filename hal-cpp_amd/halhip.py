@@ -88,6 +88,7 @@ _SIGS = {
     "halhip_diffusion2d_run_steps": [_vp, _i32, _f64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
     "halhip_diffusion2d_run_steps_async": [_vp, _i32, _f64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
     "halhip_diffusion2d_sync": [_vp, _fp],
+    "halhip_diffusion2d_prepare_steps": [_vp, _i32, _f64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
     "halhip_diffusion2d_set_profiling": [_vp, _i32],
     "halhip_diffusion2d_profile": [_vp, _i32, _ip, ctypes.c_char_p, _i32, _dp, _lp],
     "halhip_adi2d_create": [_vp, _i32, _i32, ctypes.POINTER(_vp)],
@@ -99,6 +100,7 @@ _SIGS = {
     "halhip_adi2d_run_steps": [_vp, _i32, _f64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
     "halhip_adi2d_run_steps_async": [_vp, _i32, _f64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
     "halhip_adi2d_sync": [_vp, _fp],
+    "halhip_adi2d_prepare_steps": [_vp, _i32, _f64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
     "halhip_adi2d_set_strip_shape": [_vp, _i32, _i32, _i32],
     "halhip_adi2d_get_strip_shape": [_vp, _ip, _ip, _ip],
     "halhip_adi2d_set_methods": [_vp, _i32, _i32],
@@ -116,6 +118,7 @@ _SIGS = {
     "halhip_batch_run_ticks_per_instance": [_vp, _i32, _i32, _i32, ctypes.POINTER(HalBC)],
     "halhip_batch_run_ticks_per_instance_async": [_vp, _i32, _i32, _i32, ctypes.POINTER(HalBC)],
     "halhip_batch_sync": [_vp, _fp],
+    "halhip_batch_prepare_ticks": [_vp, _i32, _i32, _i32, ctypes.POINTER(HalBC)],
     "halhip_batch_set_strip_rows": [_vp, _i32],
     "halhip_batch_get_strip_rows": [_vp, _ip],
     "halhip_batch_set_carries": [_vp, _i32],
@@ -143,6 +146,7 @@ _SIGS = {
     "halhip_agents2d_run_ticks": [_vp, _i32, _f64, _f64, _f64, _i32, _i32],
     "halhip_agents2d_run_ticks_async": [_vp, _i32, _f64, _f64, _f64, _i32, _i32],
     "halhip_agents2d_sync": [_vp, _fp],
+    "halhip_agents2d_prepare_ticks": [_vp, _i32, _f64, _f64, _f64, _i32, _i32],
     "halhip_agents2d_set_profiling": [_vp, _i32],
     "halhip_agents2d_profile": [_vp, _i32, _ip, ctypes.c_char_p, _i32, _dp, _lp],
 }
@@ -415,6 +419,11 @@ class Diffusion2D(_Plan):
         name = "run_steps_async" if asynchronous else "run_steps"
         _check(self._fn(name)(self.h, steps, rate, decay, int(wrapX), int(wrapY), _bcp(s)))
 
+    def prepare_steps(self, steps, rate, decay, wrapX, wrapY, bc=None):
+        """Build the graphs run_steps(steps, ...) replays; runs no tick."""
+        s, keep = make_bc(bc, self.X, self.Y)
+        _check(self._fn("prepare_steps")(self.h, steps, rate, decay, int(wrapX), int(wrapY), _bcp(s)))
+
 
 class Adi2D(_Plan):
     """ADI plan (IAdiDiffusionBackend2D + resident RunSteps)."""
@@ -471,6 +480,11 @@ class Adi2D(_Plan):
         name = "run_steps_async" if asynchronous else "run_steps"
         _check(self._fn(name)(self.h, steps, rate, decay, int(wrapX), int(wrapY), _bcp(s)))
 
+    def prepare_steps(self, steps, rate, decay, wrapX, wrapY, bc=None):
+        """Build the graphs run_steps(steps, ...) replays; runs no tick."""
+        s, keep = make_bc(bc, self.X, self.Y)
+        _check(self._fn("prepare_steps")(self.h, steps, rate, decay, int(wrapX), int(wrapY), _bcp(s)))
+
 
 class Agents2D(_Plan):
     """Agent-force plan (IAgentForceBackend2D + resident ticks)."""
@@ -502,6 +516,10 @@ class Agents2D(_Plan):
     def run_ticks(self, ticks, R, k, dt, wrapX, wrapY, asynchronous=False):
         name = "run_ticks_async" if asynchronous else "run_ticks"
         _check(self._fn(name)(self.h, ticks, R, k, dt, int(wrapX), int(wrapY)))
+
+    def prepare_ticks(self, ticks, R, k, dt, wrapX, wrapY):
+        """Build the graphs run_ticks(ticks, ...) replays; runs no tick."""
+        _check(self._fn("prepare_ticks")(self.h, ticks, R, k, dt, int(wrapX), int(wrapY)))
 
     def positions(self):
         out = np.empty(2 * self.n, np.float32)
@@ -581,6 +599,11 @@ class Batch(_Plan):
         if asynchronous:
             name += "_async"
         _check(self._fn(name)(self.h, ticks, int(wrapX), int(wrapY), _bcp(s)))
+
+    def prepare_ticks(self, ticks, wrapX=False, wrapY=False, bc=None):
+        """Build the graphs run_ticks(ticks, ...) replays; runs no tick."""
+        s, keep = make_bc(bc, 1, 1) if (bc is None or np.isscalar(bc)) else (None, ())
+        _check(self._fn("prepare_ticks")(self.h, ticks, int(wrapX), int(wrapY), _bcp(s)))
 
     def set_strip_rows(self, rows):
         _check(self._fn("set_strip_rows")(self.h, rows))

@@ -189,6 +189,12 @@ int halhip_diffusion2d_run_steps(halhip_diffusion2d *plan, int32_t steps, double
 int halhip_diffusion2d_run_steps_async(halhip_diffusion2d *plan, int32_t steps, double diffRate, double decay,
                                        int32_t wrapX, int32_t wrapY, const halhip_bc *bc);
 int halhip_diffusion2d_sync(halhip_diffusion2d *plan, float *elapsedMs);
+/* Builds every graph a later run_steps with the same arguments would replay
+ * (for either ping-pong buffer) and runs no tick; resident state is
+ * unchanged.  No counterpart in the reference (DiffusionKernel2D::RunSteps
+ * records its command buffer per call, src/Gpu/DiffusionKernel2D.cpp:244-304). */
+int halhip_diffusion2d_prepare_steps(halhip_diffusion2d *plan, int32_t steps, double diffRate, double decay,
+                                     int32_t wrapX, int32_t wrapY, const halhip_bc *bc);
 int halhip_diffusion2d_set_profiling(halhip_diffusion2d *plan, int32_t on);
 int halhip_diffusion2d_profile(halhip_diffusion2d *plan, int32_t maxKernels, int32_t *nKernels, char *names,
                                int32_t namesLen, double *totalMs, int64_t *launches);
@@ -231,6 +237,14 @@ int halhip_adi2d_run_steps(halhip_adi2d *plan, int32_t steps, double diffRate, d
 int halhip_adi2d_run_steps_async(halhip_adi2d *plan, int32_t steps, double diffRate, double decay, int32_t wrapX,
                                  int32_t wrapY, const halhip_bc *bc);
 int halhip_adi2d_sync(halhip_adi2d *plan, float *elapsedMs);
+/* Builds (captures and instantiates) every graph a later run_steps with the
+ * same arguments would replay, from either reduction half, and runs no tick:
+ * a serving loop or a timed run then pays no capture cost.  Resident state is
+ * unchanged.  Runs of fewer than 20 steps replay no graph.  No counterpart in
+ * the reference (its Vulkan RunSteps records its command buffer per call,
+ * src/Gpu/AdiKernel2D.cpp:296-368). */
+int halhip_adi2d_prepare_steps(halhip_adi2d *plan, int32_t steps, double diffRate, double decay, int32_t wrapX,
+                               int32_t wrapY, const halhip_bc *bc);
 
 /* Strip decomposition of the plan: each workgroup holds `stripRows` full rows
  * with `columnsPerThread` adjacent columns per thread (1, 2 or 4 with 4, 8 or
@@ -318,6 +332,13 @@ int halhip_agents2d_run_ticks(halhip_agents2d *plan, int32_t ticks, double inter
 int halhip_agents2d_run_ticks_async(halhip_agents2d *plan, int32_t ticks, double interactionRad,
                                     double springConstant, double dt, int32_t wrapX, int32_t wrapY);
 int halhip_agents2d_sync(halhip_agents2d *plan, float *elapsedMs);
+/* Builds every graph a later run_ticks with the same arguments and agent
+ * count would replay (from either buffer set) and runs no tick; resident
+ * state is unchanged.  Runs of fewer than 10 ticks replay no graph.  No
+ * counterpart in the reference (AgentForceKernel2D::Dispatch records its
+ * command buffer per call, src/Gpu/AgentForceKernel2D.cpp:189-290). */
+int halhip_agents2d_prepare_ticks(halhip_agents2d *plan, int32_t ticks, double interactionRad,
+                                  double springConstant, double dt, int32_t wrapX, int32_t wrapY);
 int halhip_agents2d_set_profiling(halhip_agents2d *plan, int32_t on);
 int halhip_agents2d_profile(halhip_agents2d *plan, int32_t maxKernels, int32_t *nKernels, char *names,
                             int32_t namesLen, double *totalMs, int64_t *launches);
@@ -355,6 +376,12 @@ int halhip_batch_run_ticks_per_instance(halhip_batch *batch, int32_t ticks, int3
 int halhip_batch_run_ticks_per_instance_async(halhip_batch *batch, int32_t ticks, int32_t wrapX, int32_t wrapY,
                                               const halhip_bc *bc);
 int halhip_batch_sync(halhip_batch *batch, float *elapsedMs);
+/* Builds every graph a later run_ticks with the same arguments would replay
+ * (from either reduction half) and runs no tick; the wells are unchanged.
+ * Runs of fewer than 10 ticks replay no graph.  No counterpart in the
+ * reference (MultiWellExperiment steps wells on the host). */
+int halhip_batch_prepare_ticks(halhip_batch *batch, int32_t ticks, int32_t wrapX, int32_t wrapY,
+                               const halhip_bc *bc);
 /* Strip height of the batched kernels (4 or 8 rows). */
 int halhip_batch_set_strip_rows(halhip_batch *batch, int32_t rows);
 int halhip_batch_get_strip_rows(halhip_batch *batch, int32_t *rows);
