@@ -1,6 +1,9 @@
 // HipBackends.cpp -- Hal::Hip adapters of HAL-CPP's strategy interfaces over
-// the C ABI, plus the interface base classes, CheckFieldSizes and Logger.
-// Host code only: compiled with g++ and linked against libhalhip.so.
+// the C ABI.  Host code only: compiled with g++ and linked against
+// libhalhip.so.  This file defines only Hal::Hip symbols, so it compiles into
+// HAL's own library next to HAL's Logger, interface bases and
+// Internal::CheckFieldSizes; a build without HAL links
+// standalone/HalInterfaces.cpp for those instead.
 #include "Hal/Hip/HipBackends.h"
 
 #include <cmath>
@@ -13,76 +16,6 @@
 
 namespace Hal
 {
-    // ---- Logger (src/Log/Logger.cpp:16-37 behaviour) ------------------------------------
-
-    Logger::Logger() : m_minSeverity(LogSeverity::Info) {}
-
-    Logger::~Logger() {}
-
-    Logger &Logger::Instance()
-    {
-        static Logger instance;
-        return instance;
-    }
-
-    void Logger::SetMinSeverity(LogSeverity severity)
-    {
-        m_minSeverity = severity;
-    }
-
-    void Logger::Log(LogSeverity severity, const std::string &message)
-    {
-        if (severity < m_minSeverity)
-        {
-            return;
-        }
-        std::ostream &out = (severity == LogSeverity::Error) ? std::cerr : std::cout;
-        out << "[" << ToString(severity) << "] " << message << std::endl;
-    }
-
-    const char *ToString(LogSeverity severity)
-    {
-        switch (severity)
-        {
-            case LogSeverity::Debug:
-                return "DEBUG";
-            case LogSeverity::Info:
-                return "INFO";
-            case LogSeverity::Warning:
-                return "WARNING";
-            case LogSeverity::Error:
-                return "ERROR";
-        }
-        return "UNKNOWN";
-    }
-
-    // ---- interface bases -------------------------------------------------------------
-
-    IDiffusionBackend2D::IDiffusionBackend2D() {}
-    IDiffusionBackend2D::~IDiffusionBackend2D() {}
-    IAdiDiffusionBackend2D::IAdiDiffusionBackend2D() {}
-    IAdiDiffusionBackend2D::~IAdiDiffusionBackend2D() {}
-    IAgentForceBackend2D::IAgentForceBackend2D() {}
-    IAgentForceBackend2D::~IAgentForceBackend2D() {}
-
-    namespace Internal
-    {
-        void CheckFieldSizes(const std::vector<double> &field,
-                             const std::vector<double> &deltas,
-                             int32_t                    xDim,
-                             int32_t                    yDim,
-                             const char                *pCaller)
-        {
-            const size_t n = static_cast<size_t>(xDim) * static_cast<size_t>(yDim);
-            if (xDim <= 0 || yDim <= 0 || field.size() != n || deltas.size() != n)
-            {
-                const std::string message = std::string(pCaller) + ": field and deltas must each hold xDim*yDim values";
-                Logger::Instance().Log(LogSeverity::Error, message);
-                throw std::invalid_argument(message);
-            }
-        }
-    }
-
     namespace Hip
     {
         namespace
