@@ -35,6 +35,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 
 using namespace halhip;
@@ -598,6 +599,142 @@ namespace
     unsigned Blocks(int64_t n, int threads)
     {
         return static_cast<unsigned>(std::max<int64_t>(1, (n + threads - 1) / threads));
+    }
+}
+
+// ---- bins without allocation (halhip_agents2d_bins_noalloc) ----------------
+//
+// For callers that cannot allocate device memory (code under graph capture,
+// serving loops): the binning's working memory is one file-scope device
+// array per GPU, `scratch`, of fixed size.  Host code reaches it only through
+// the address hipGetSymbolAddress returns on that device (kept per device by
+// InitBinsScratch, called when a context is created); the array's name in
+// host code is not a device address.  Layout in int32 words:
+//   counts  [kNaMaxBins]         per-bin counts (whole scan tiles)
+//   offsets [kNaMaxBins + 64]    exclusive prefix, padded like the plan's
+//   ticket  [kNaMaxAgents]       per-agent ticket from the count atomics
+//   tiles   [kNaMaxBins / tile]  per-tile totals
+namespace
+{
+    constexpr int64_t kNaMaxBins = int64_t(1) << 20;
+    constexpr int64_t kNaMaxAgents = int64_t(1) << 20;
+    constexpr int64_t kNaCounts = 0;
+    constexpr int64_t kNaOffsets = kNaCounts + kNaMaxBins;
+    constexpr int64_t kNaTicket = kNaOffsets + kNaMaxBins + 64;
+    constexpr int64_t kNaTiles = kNaTicket + kNaMaxAgents;
+    constexpr int64_t kNaWords = kNaTiles + kNaMaxBins / kScanTile;
+    static_assert(kNaMaxBins % kScanTile == 0, "bins must fill whole scan tiles");
+    static_assert(kNaMaxBins / kScanTile <= 4096, "the scan's tile prefix must fit one pass");
+
+    static __device__ __align__(256) int32_t scratch[kNaWords];
+
+    constexpr int kNaMaxDevices = 64;
+    std::mutex g_naMutex;
+    int32_t *g_naScratch[kNaMaxDevices] = {};
+    hipError_t g_naLookupError[kNaMaxDevices] = {};
+
+    __global__ __launch_bounds__(kThreads) void FillZero4(int4 *__restrict__ p, int64_t n4)
+    {
+        const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+        if (i < n4)
+        {
+            p[i] = make_int4(0, 0, 0, 0);
+        }
+    }
+
+    // Per agent: ticket = atomicAdd(counts[cell], 1).
+    __global__ __launch_bounds__(kThreads) void AgentTickets(int32_t N, int32_t X, int32_t Y,
+                                                             const float2 *__restrict__ pos,
+                                                             int32_t *__restrict__ counts, int32_t *__restrict__ ticket)
+    {
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= N)
+        {
+            return;
+        }
+        const float2 p = pos[i];
+        ticket[i] = atomicAdd(&counts[ClampCell(p.x, X) * Y + ClampCell(p.y, Y)], 1);
+    }
+
+    // bucket[offsets[cell] + ticket] = agent index (a permutation of 0..N-1).
+    __global__ __launch_bounds__(kThreads) void AgentScatterIndex(int32_t N, int32_t X, int32_t Y,
+                                                                  const float2 *__restrict__ pos,
+                                                                  const int32_t *__restrict__ ticket,
+                                                                  const int32_t *__restrict__ offsets,
+                                                                  int32_t *__restrict__ bucket)
+    {
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= N)
+        {
+            return;
+        }
+        const float2 p = pos[i];
+        bucket[offsets[ClampCell(p.x, X) * Y + ClampCell(p.y, Y)] + ticket[i]] = i;
+    }
+
+    // Each bin's slice of bucket sorted ascending (insertion sort: slices
+    // hold a handful of agents), one bin per thread.
+    __global__ __launch_bounds__(kThreads) void AgentOrderIndex(int64_t C, const int32_t *__restrict__ offsets,
+                                                                int32_t *__restrict__ bucket)
+    {
+        const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+        if (c >= C)
+        {
+            return;
+        }
+        const int s = offsets[c];
+        const int e = offsets[c + 1];
+        for (int a = s + 1; a < e; a++)
+        {
+            const int key = bucket[a];
+            int b = a - 1;
+            while (b >= s && bucket[b] > key)
+            {
+                bucket[b + 1] = bucket[b];
+                b--;
+            }
+            bucket[b + 1] = key;
+        }
+    }
+
+    // The device address of `scratch` on ctx's device (set at context creation).
+    int32_t *BinsScratch(const halhip_context *ctx)
+    {
+        std::lock_guard<std::mutex> lock(g_naMutex);
+        int32_t *ptr = (ctx->device >= 0 && ctx->device < kNaMaxDevices) ? g_naScratch[ctx->device] : nullptr;
+        if (ptr == nullptr)
+        {
+            const hipError_t e = (ctx->device >= 0 && ctx->device < kNaMaxDevices) ? g_naLookupError[ctx->device]
+                                                                                    : hipSuccess;
+            Fail(HALHIP_ERR_HIP, std::string("halhip_agents2d_bins_noalloc: the device address of the scratch array "
+                                             "is not known: ") +
+                                     (e != hipSuccess ? hipGetErrorString(e) : "no context was created on this device"));
+        }
+        return ptr;
+    }
+}
+
+namespace halhip
+{
+    void InitBinsScratch(int32_t device)
+    {
+        std::lock_guard<std::mutex> lock(g_naMutex);
+        if (device < 0 || device >= kNaMaxDevices || g_naScratch[device] != nullptr)
+        {
+            return;
+        }
+        void *ptr = nullptr;
+        // The device is current (halhip_context_create bound it).  A failed
+        // lookup leaves the address unknown, which only the no-allocation
+        // entry point reports; every other entry point stays usable.
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+        {
+            (void)hipGetLastError();
+            g_naLookupError[device] = e;
+            return;
+        }
+        g_naScratch[device] = static_cast<int32_t *>(ptr);
     }
 }
 
@@ -1208,6 +1345,69 @@ extern "C"
             {
                 RequirePlan(p, "halhip_agents2d_profile");
                 p->prof.Export(maxKernels, nKernels, names, namesLen, totalMs, launches);
+            });
+    }
+
+    int halhip_agents2d_bins_noalloc(halhip_context *ctx, const float *positions, int32_t nAgents, int32_t xDim,
+                                     int32_t yDim, int32_t *offsets, int32_t *bucket, void *stream)
+    {
+        return Guard(
+            [&]
+            {
+                Require(ctx != nullptr, "halhip_agents2d_bins_noalloc: null context");
+                Require(xDim > 0 && yDim > 0, "halhip_agents2d_bins_noalloc: dimensions must be positive");
+                Require(nAgents >= 0, "halhip_agents2d_bins_noalloc: negative agent count");
+                const int64_t C = static_cast<int64_t>(xDim) * yDim;
+                Require(C <= kNaMaxBins, "halhip_agents2d_bins_noalloc: xDim*yDim exceeds "
+                                         "HALHIP_AGENTS2D_NOALLOC_MAX_BINS (1048576)");
+                Require(nAgents <= kNaMaxAgents, "halhip_agents2d_bins_noalloc: nAgents exceeds "
+                                                 "HALHIP_AGENTS2D_NOALLOC_MAX_AGENTS (1048576)");
+                Require(offsets != nullptr, "halhip_agents2d_bins_noalloc: null offsets");
+                Require(nAgents == 0 || (positions != nullptr && bucket != nullptr),
+                        "halhip_agents2d_bins_noalloc: null positions or bucket");
+                Bind(ctx);
+                int32_t *base = BinsScratch(ctx);
+                hipStream_t s = static_cast<hipStream_t>(stream);
+                int32_t *counts = base + kNaCounts;
+                int32_t *offs = base + kNaOffsets;
+                int32_t *ticket = base + kNaTicket;
+                int32_t *tiles = base + kNaTiles;
+                const int32_t nTiles = static_cast<int32_t>((C + kScanTile - 1) / kScanTile);
+                const int64_t cPad = static_cast<int64_t>(nTiles) * kScanTile;
+                // The previous call's counts may not have been reset (a call
+                // can stop part-way), so they are cleared here, by a kernel:
+                // a memset node would not replay reliably inside a graph.
+                const int64_t n4 = cPad / 4;
+                hipLaunchKernelGGL(FillZero4, dim3(Blocks(n4, kThreads)), dim3(kThreads), 0, s,
+                                   reinterpret_cast<int4 *>(counts), n4);
+                Check(hipGetLastError(), "FillZero4 launch");
+                const float2 *pos = reinterpret_cast<const float2 *>(positions);
+                if (nAgents > 0)
+                {
+                    hipLaunchKernelGGL(AgentTickets, dim3(Blocks(nAgents, kThreads)), dim3(kThreads), 0, s, nAgents,
+                                       xDim, yDim, pos, counts, ticket);
+                    Check(hipGetLastError(), "AgentTickets launch");
+                }
+                hipLaunchKernelGGL(ScanTileSums, dim3(static_cast<unsigned>(nTiles)), dim3(kScanThreads), 0, s,
+                                   reinterpret_cast<const int4 *>(counts), tiles);
+                Check(hipGetLastError(), "ScanTileSums launch");
+                hipLaunchKernelGGL(ScanApply, dim3(static_cast<unsigned>(nTiles)), dim3(kScanThreads), 0, s,
+                                   reinterpret_cast<int4 *>(counts), static_cast<const int32_t *>(tiles),
+                                   reinterpret_cast<int4 *>(offs), offs + cPad, nAgents, 1);
+                Check(hipGetLastError(), "ScanApply launch");
+                if (nAgents > 0)
+                {
+                    hipLaunchKernelGGL(AgentScatterIndex, dim3(Blocks(nAgents, kThreads)), dim3(kThreads), 0, s,
+                                       nAgents, xDim, yDim, pos, static_cast<const int32_t *>(ticket),
+                                       static_cast<const int32_t *>(offs), bucket);
+                    Check(hipGetLastError(), "AgentScatterIndex launch");
+                    hipLaunchKernelGGL(AgentOrderIndex, dim3(Blocks(C, kThreads)), dim3(kThreads), 0, s, C,
+                                       static_cast<const int32_t *>(offs), bucket);
+                    Check(hipGetLastError(), "AgentOrderIndex launch");
+                }
+                Check(hipMemcpyAsync(offsets, offs, sizeof(int32_t) * static_cast<size_t>(C + 1),
+                                     hipMemcpyDeviceToDevice, s),
+                      "hipMemcpyAsync");
             });
     }
 }

@@ -87,6 +87,10 @@ namespace halhip
         }
     }
 
+    // Looks up, once per device and with that device current, the device
+    // address of the no-allocation binning's scratch array (agents2d.hip).
+    void InitBinsScratch(int32_t device);
+
     // Makes ctx's device current for the calling thread.
     inline void Bind(const halhip_context *ctx)
     {

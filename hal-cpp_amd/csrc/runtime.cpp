@@ -366,6 +366,7 @@ extern "C"
                     Fail(HALHIP_ERR_NO_DEVICE, std::string("halhip_context_create: device is ") + prop.gcnArchName +
                                                    ", this library is built for gfx950 only");
                 }
+                InitBinsScratch(device);
                 auto *ctx = new halhip_context();
                 ctx->device = device;
                 *out = ctx;

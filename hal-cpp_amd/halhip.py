@@ -147,6 +147,7 @@ _SIGS = {
     "halhip_agents2d_run_ticks_async": [_vp, _i32, _f64, _f64, _f64, _i32, _i32],
     "halhip_agents2d_sync": [_vp, _fp],
     "halhip_agents2d_prepare_ticks": [_vp, _i32, _f64, _f64, _f64, _i32, _i32],
+    "halhip_agents2d_bins_noalloc": [_vp, _vp, _i32, _i32, _i32, _vp, _vp, _vp],
     "halhip_agents2d_set_profiling": [_vp, _i32],
     "halhip_agents2d_profile": [_vp, _i32, _ip, ctypes.c_char_p, _i32, _dp, _lp],
 }
@@ -194,6 +195,14 @@ def device_count():
     n = ctypes.c_int32(0)
     _check(load().halhip_device_count(ctypes.byref(n)))
     return n.value
+
+
+def agents_bins_noalloc(ctx, positions_dev, n_agents, X, Y, offsets_dev, bucket_dev, stream=None):
+    """halhip_agents2d_bins_noalloc on caller-owned DEVICE buffers (integer
+    addresses): enqueues the binning on `stream` (None: the null stream) and
+    returns without waiting."""
+    _check(load().halhip_agents2d_bins_noalloc(ctx.h, _vp(positions_dev), n_agents, X, Y, _vp(offsets_dev),
+                                               _vp(bucket_dev), _vp(stream or 0)))
 
 
 def rng_doubles(seed, n):

@@ -343,6 +343,27 @@ int halhip_agents2d_set_profiling(halhip_agents2d *plan, int32_t on);
 int halhip_agents2d_profile(halhip_agents2d *plan, int32_t maxKernels, int32_t *nKernels, char *names,
                             int32_t namesLen, double *totalMs, int64_t *launches);
 
+/* Bins without allocation.  The same bins as download_bins after a binning
+ * of these positions (offsets = exclusive prefix of per-bin counts, xDim*yDim
+ * + 1 values; bucket = agent indices in bin order, ascending within a bin),
+ * for callers that cannot allocate device memory: code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc.  No counterpart in
+ * the reference.  positions (2*nAgents floats, x then y per agent), offsets
+ * and bucket are DEVICE pointers owned by the caller.  The work is enqueued
+ * on `stream` (a hipStream_t; null is the null stream) and the call returns
+ * without waiting; the results are ready when the stream reaches that point.
+ * Nothing is allocated or freed: the working memory is one fixed array per
+ * GPU (12 MiB, held until the process ends), so nAgents is at most
+ * HALHIP_AGENTS2D_NOALLOC_MAX_AGENTS and xDim*yDim at most
+ * HALHIP_AGENTS2D_NOALLOC_MAX_BINS (larger inputs are rejected with
+ * HALHIP_ERR_INVALID_ARGUMENT).  Every call on one GPU shares that array:
+ * calls that overlap in time on one GPU (other streams, other threads) race
+ * and must be serialised by the caller. */
+#define HALHIP_AGENTS2D_NOALLOC_MAX_AGENTS 1048576
+#define HALHIP_AGENTS2D_NOALLOC_MAX_BINS 1048576
+int halhip_agents2d_bins_noalloc(halhip_context *ctx, const float *positions, int32_t nAgents, int32_t xDim,
+                                 int32_t yDim, int32_t *offsets, int32_t *bucket, void *stream);
+
 /* ---- batched wells --------------------------------------------------------- */
 
 /* Many independent ADI grids ("wells") of uneven size ticked together, each
