@@ -6,7 +6,7 @@
 // stencil (src/Pde/PDEequations.cpp:214-219; the face between cells i and i+1
 // carries D_i), zero-flux boundaries, and appends MulAll(-k) + Update.
 //
-// Tick (N = X*Y cells, u in HAL's x*Y+y layout, uT its transpose):
+// Tick (N = X*Y cells, u in HAL's x*Y+y layout):
 //   s1 = (I + L_y/2) u;  s2 = (I - L_x/2)^{-1} s1;  s3 = (I + L_x/2) s2;
 //   s4 = (I - L_y/2)^{-1} s3;  u' = s4 - k u.
 // Every factor is symmetric, so the adjoint tick applies them transposed in
@@ -17,13 +17,21 @@
 //   dJ/dk   -= <lam, u>
 // with dx p = p(x+1,y) - p(x,y) (0 on the last row), dy likewise.
 //
-// Kernels: a line solve (one workgroup of four wavefronts per line,
-// thread-local recurrences composed by wavefront scans of affine maps and
-// LDS-exchanged wavefront totals; the per-cell Thomas factors are computed
-// once per diffusivity field on the host in FP64), an LDS-tiled transpose so
-// each sweep reads its lines contiguously, and a fused gradient /
-// adjoint-update kernel.  Checkpoints every K ticks; the backward pass recomputes each
-// segment from its checkpoint, keeping u, s2, s4 of the segment.
+// Kernels.  The Thomas factors of a zero-flux line with face rates D are
+// fw_j = (D_{j-1}/2) ib_{j-1} and bw_j = (D_j/2) ib_j, with ib = 1/b' the
+// reciprocal pivots, so only ib is tabulated (once per diffusivity field, on
+// the host in FP64, one table per line direction in the field's own layout)
+// and the multipliers are formed from D, which the explicit half-steps read
+// anyway.  Lines along y (contiguous rows) are solved by one workgroup of
+// four wavefronts per row: thread-local recurrences composed by wavefront
+// scans of affine maps and LDS-exchanged wavefront totals.  Lines along x
+// (stride Y) are solved without a transpose: a workgroup stages C adjacent
+// columns over all X rows in LDS with row-major loads, one wavefront solves
+// each column with E elements per lane, and the result goes back row-major.
+// A fused kernel forms the gradient contributions and the adjoint update.
+// A forward tick is two launches and an adjoint tick three.  Checkpoints
+// every K ticks; the backward pass recomputes each segment from its
+// checkpoint, keeping u, s2, s4 of the segment.
 #include "common.hpp"
 
 #include <algorithm>
@@ -48,10 +56,8 @@ namespace
         int32_t nLines;
         int32_t L;
         const float *in;   // [nLines][L]
-        const float *rate; // [nLines][L] face rates of this line family
-        const float *fw;   // [nLines][L] Thomas factors
-        const float *ib;
-        const float *bw;
+        const float *rate; // [nLines][L] face rates (face j..j+1 of a line, and line l..l+1)
+        const float *ib;   // [nLines][L] reciprocal Thomas pivots of the lines
         float *out;        // [nLines][L]
         const float *prev; // decay: out = x - k * prev
         float negK;
@@ -64,6 +70,7 @@ namespace
     // vectors.  Each recurrence composes per-thread folds, a wavefront scan
     // of affine maps (DPP) and the wavefront totals exchanged through LDS.
     constexpr int kBlockLine = 256;
+    constexpr unsigned kLossBlocks = 1024; // blocks (and partial sums) of the loss seed
 
     template <int EPL, bool VEC>
     __device__ __forceinline__ void LoadLine(const float *__restrict__ src, size_t base, int j0, int L, float pad,
@@ -126,7 +133,6 @@ namespace
     }
 
     // RHS 0: d = c + 1/2 (rate[l] (in[l+1] - c) + rate[l-1] (in[l-1] - c)) across lines.
-    // RHS 1: the same along the line (neighbours j+-1 of the same line).
     // RHS 2: d = in.
     // DECAY: out = x - k * prev; STASH: keeps x before the decay.
     template <int EPL, int RHS, bool DECAY, bool STASH, bool VEC>
@@ -142,13 +148,14 @@ namespace
         const size_t base = static_cast<size_t>(line) * L;
         const int j0 = t * EPL;
         float d[EPL];
-        float fw[EPL];
+        float dl[EPL]; // this line's face rates
+        float ib[EPL];
         LoadLine<EPL, VEC>(a.in, base, j0, L, 0.0f, d);
-        LoadLine<EPL, VEC>(a.fw, base, j0, L, 1.0f, fw);
+        LoadLine<EPL, VEC>(a.rate, base, j0, L, 0.0f, dl);
+        LoadLine<EPL, VEC>(a.ib, base, j0, L, 0.0f, ib);
         if constexpr (RHS == 0)
         {
             // explicit half-step across lines: neighbours in lines l +- 1
-            float r[EPL];
             float nb[EPL];
             float s[EPL];
 #pragma unroll
@@ -158,16 +165,16 @@ namespace
             }
             if (line + 1 < a.nLines)
             {
-                LoadLine<EPL, VEC>(a.rate, base, j0, L, 0.0f, r);
                 LoadLine<EPL, VEC>(a.in, base + L, j0, L, 0.0f, nb);
 #pragma unroll
                 for (int e = 0; e < EPL; e++)
                 {
-                    s[e] += r[e] * (nb[e] - d[e]);
+                    s[e] += dl[e] * (nb[e] - d[e]);
                 }
             }
             if (line > 0)
             {
+                float r[EPL];
                 LoadLine<EPL, VEC>(a.rate, base - L, j0, L, 0.0f, r);
                 LoadLine<EPL, VEC>(a.in, base - L, j0, L, 0.0f, nb);
 #pragma unroll
@@ -182,34 +189,20 @@ namespace
                 d[e] = (j0 + e < L) ? fmaf(0.5f, s[e], d[e]) : 0.0f;
             }
         }
-        else if constexpr (RHS == 1)
+        // h_j = (D_j / 2) ib_j: bw_j = h_j (0 at the line's end), fw_j = h_{j-1}
+        // (0 at its start; the first element's comes from the previous thread)
+        float h[EPL];
+#pragma unroll
+        for (int e = 0; e < EPL; e++)
         {
-            // explicit half-step along the line; the outer neighbours of the
-            // thread's elements come from the cache
-            float r[EPL];
-            LoadLine<EPL, VEC>(a.rate, base, j0, L, 0.0f, r);
-            const float cLeft = (j0 > 0 && j0 - 1 < L) ? a.in[base + j0 - 1] : 0.0f;
-            const float rLeft = (j0 > 0 && j0 - 1 < L) ? a.rate[base + j0 - 1] : 0.0f;
-            const float cRight = (j0 + EPL < L) ? a.in[base + j0 + EPL] : 0.0f;
-            float out[EPL];
+            h[e] = 0.5f * dl[e] * ib[e];
+        }
+        const float hLeft = (j0 > 0 && j0 - 1 < L) ? 0.5f * a.rate[base + j0 - 1] * a.ib[base + j0 - 1] : 0.0f;
+        float fw[EPL];
 #pragma unroll
-            for (int e = 0; e < EPL; e++)
-            {
-                const int j = j0 + e;
-                const float c = d[e];
-                const float cp = (e + 1 < EPL) ? d[e + 1] : cRight;
-                const float cm = (e > 0) ? d[e - 1] : cLeft;
-                const float rm = (e > 0) ? r[e - 1] : rLeft;
-                float sum = 0.0f;
-                sum += (j + 1 < L) ? r[e] * (cp - c) : 0.0f;
-                sum += (j > 0) ? rm * (cm - c) : 0.0f;
-                out[e] = (j < L) ? fmaf(0.5f, sum, c) : 0.0f;
-            }
-#pragma unroll
-            for (int e = 0; e < EPL; e++)
-            {
-                d[e] = out[e];
-            }
+        for (int e = 0; e < EPL; e++)
+        {
+            fw[e] = (e == 0) ? hLeft : h[e - 1];
         }
 
         // Forward recurrence d'_j = fw_j d'_{j-1} + d_j, carry into the line 0.
@@ -264,9 +257,11 @@ namespace
 
         // Backward recurrence x_j = bw_j x_{j+1} + ib_j d'_j from the right.
         float bw[EPL];
-        float ib[EPL];
-        LoadLine<EPL, VEC>(a.bw, base, j0, L, 0.0f, bw);
-        LoadLine<EPL, VEC>(a.ib, base, j0, L, 0.0f, ib);
+#pragma unroll
+        for (int e = 0; e < EPL; e++)
+        {
+            bw[e] = (j0 + e < L - 1) ? h[e] : 0.0f;
+        }
         A = 1.0f;
         B = 0.0f;
 #pragma unroll
@@ -325,65 +320,313 @@ namespace
         StoreLine<EPL, VEC>(a.out, base, j0, L, d);
     }
 
-    // out[c][r] = in[r][c] for an R x C matrix (R, C multiples of 4) through
-    // 32 x 32 LDS tiles, one 16-byte vector read and written per thread
-    // (many small tiles keep every CU busy on a 1024^2 field).
-    __global__ __launch_bounds__(256) void Transpose4(const float *__restrict__ in, float *__restrict__ out, int32_t R,
-                                                      int32_t C)
+    struct ColArgs
     {
-        __shared__ float tile[32][33];
-        const int c0 = blockIdx.x * 32;
-        const int r0 = blockIdx.y * 32;
-        const int q = threadIdx.x & 7;   // 4-column group
-        const int rr = threadIdx.x >> 3; // 0..31
+        int32_t X;
+        int32_t Y;
+        int32_t blocks;    // column blocks, ceil(Y / C)
+        const float *in;   // [X][Y]
+        const float *D;    // [X][Y] face rates
+        const float *ib;   // [X][Y] reciprocal pivots of the x-lines
+        float *out;        // [X][Y]
+    };
+
+    // C consecutive floats of row x from column y0 (those past Y read 0).
+    template <int C, bool VEC>
+    __device__ __forceinline__ void LoadRow(const float *__restrict__ src, size_t at, int y0, int Y, float (&v)[C])
+    {
+        if constexpr (VEC && C == 4)
         {
-            const int r = r0 + rr;
-            const int c = c0 + 4 * q;
-            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (r < R && c < C)
-            {
-                v = *reinterpret_cast<const float4 *>(in + static_cast<size_t>(r) * C + c);
-            }
-            tile[rr][4 * q] = v.x;
-            tile[rr][4 * q + 1] = v.y;
-            tile[rr][4 * q + 2] = v.z;
-            tile[rr][4 * q + 3] = v.w;
+            const float4 t = *reinterpret_cast<const float4 *>(src + at);
+            v[0] = t.x;
+            v[1] = t.y;
+            v[2] = t.z;
+            v[3] = t.w;
         }
-        __syncthreads();
+        else if constexpr (VEC && C == 2)
         {
-            const int c = c0 + rr; // output row
-            const int r = r0 + 4 * q;
-            if (c < C && r < R)
+            const float2 t = *reinterpret_cast<const float2 *>(src + at);
+            v[0] = t.x;
+            v[1] = t.y;
+        }
+        else
+        {
+#pragma unroll
+            for (int c = 0; c < C; c++)
             {
-                *reinterpret_cast<float4 *>(out + static_cast<size_t>(c) * R + r) =
-                    make_float4(tile[4 * q][rr], tile[4 * q + 1][rr], tile[4 * q + 2][rr], tile[4 * q + 3][rr]);
+                v[c] = (y0 + c < Y) ? src[at + c] : 0.0f;
             }
         }
     }
 
-    // out[c][r] = in[r][c] for an R x C matrix, through 64 x 64 LDS tiles.
-    __global__ __launch_bounds__(256) void Transpose(const float *__restrict__ in, float *__restrict__ out, int32_t R,
-                                                     int32_t C)
+    // Lines along x (one per column, stride Y) solved without a transpose.  A
+    // workgroup of C wavefronts holds C adjacent columns over all X rows in
+    // LDS: rows are loaded and stored row-major (C consecutive floats each),
+    // wavefront c solves column c with lane l holding rows [lE, lE + E)
+    // (LDS pitch E + 1 per lane, so the lanes' element reads hit distinct
+    // banks).  Rows past X are identity padding (d = D = ib = 0).
+    //   MODE 0: d = (I + L_y/2) in, the explicit half-step across columns
+    //           (reads columns y0 - 1 and y0 + C too).
+    //   MODE 1: d = (I + L_x/2) in, the explicit half-step along the line.
+    // Column blocks are dealt so that each XCD (blocks b, b + 8, ...) takes
+    // a contiguous range: blocks that share cache lines share an L2.
+    template <int C, int E, int MODE, bool VEC>
+    __global__ __launch_bounds__(64 * C) void CellColumnSolve(ColArgs a)
     {
-        __shared__ float tile[64][65];
-        const int c0 = blockIdx.x * 64;
-        const int r0 = blockIdx.y * 64;
-        const int tx = threadIdx.x & 63;
-        const int ty = threadIdx.x >> 6;
-        for (int k = ty; k < 64; k += 4)
+        constexpr int kPitch = 64 * (E + 1); // LDS floats per column
+        constexpr int kRows = 64 * E;
+        __shared__ float sD[C * kPitch];
+        __shared__ float sR[C * kPitch];
+        __shared__ float sI[C * kPitch];
+        const int X = a.X;
+        const int Y = a.Y;
+        int b = static_cast<int>(blockIdx.x);
+        if (a.blocks % 8 == 0)
         {
-            const int r = r0 + k;
-            const int c = c0 + tx;
-            tile[k][tx] = (r < R && c < C) ? in[static_cast<size_t>(r) * C + c] : 0.0f;
+            b = (b % 8) * (a.blocks / 8) + b / 8;
+        }
+        const int y0 = b * C;
+        const int t = threadIdx.x;
+        // Every thread holds kMine rows: all their loads are issued before
+        // the first LDS store, so one memory latency covers the tile.
+        constexpr int kMine = (kRows + 64 * C - 1) / (64 * C);
+        float u[kMine][C];
+        float r[kMine][C];
+        float q[kMine][C];
+        float uL[kMine];
+        float rL[kMine];
+        float uR[kMine];
+#pragma unroll
+        for (int i = 0; i < kMine; i++)
+        {
+            const int x = t + i * 64 * C;
+            const size_t row = static_cast<size_t>(x) * Y + y0;
+            uL[i] = 0.0f;
+            rL[i] = 0.0f;
+            uR[i] = 0.0f;
+            if (x < X)
+            {
+                LoadRow<C, VEC>(a.in, row, y0, Y, u[i]);
+                LoadRow<C, VEC>(a.D, row, y0, Y, r[i]);
+                LoadRow<C, VEC>(a.ib, row, y0, Y, q[i]);
+                if constexpr (MODE == 0)
+                {
+                    uL[i] = (y0 > 0) ? a.in[row - 1] : 0.0f;
+                    rL[i] = (y0 > 0) ? a.D[row - 1] : 0.0f;
+                    uR[i] = (y0 + C < Y) ? a.in[row + C] : 0.0f;
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int c = 0; c < C; c++)
+                {
+                    u[i][c] = 0.0f;
+                    r[i][c] = 0.0f;
+                    q[i][c] = 0.0f;
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kMine; i++)
+        {
+            const int x = t + i * 64 * C;
+            const int at = x + x / E;
+            if constexpr (MODE == 0)
+            {
+                float dv[C];
+#pragma unroll
+                for (int c = 0; c < C; c++)
+                {
+                    const int y = y0 + c;
+                    const float up = (c + 1 < C) ? u[i][c + 1] : uR[i];
+                    const float um = (c > 0) ? u[i][c - 1] : uL[i];
+                    const float rm = (c > 0) ? r[i][c - 1] : rL[i];
+                    float sum = 0.0f;
+                    sum += (y + 1 < Y) ? r[i][c] * (up - u[i][c]) : 0.0f;
+                    sum += (y > 0) ? rm * (um - u[i][c]) : 0.0f;
+                    dv[c] = (y < Y && x < X) ? fmaf(0.5f, sum, u[i][c]) : 0.0f;
+                }
+#pragma unroll
+                for (int c = 0; c < C; c++)
+                {
+                    u[i][c] = dv[c];
+                }
+            }
+            if (x < kRows)
+            {
+#pragma unroll
+                for (int c = 0; c < C; c++)
+                {
+                    sD[c * kPitch + at] = u[i][c];
+                    sR[c * kPitch + at] = r[i][c];
+                    sI[c * kPitch + at] = q[i][c];
+                }
+            }
         }
         __syncthreads();
-        for (int k = ty; k < 64; k += 4)
+
+        const int lane = t & 63;
+        const int c = t >> 6;
+        const int x0 = lane * E;
+        const int col = c * kPitch;
+        const int me = lane * (E + 1);
+        float d[E];
+        float h[E];
+        float ib[E];
+#pragma unroll
+        for (int e = 0; e < E; e++)
         {
-            const int c = c0 + k;
-            const int r = r0 + tx;
-            if (c < C && r < R)
+            d[e] = sD[col + me + e];
+            ib[e] = sI[col + me + e];
+        }
+        if constexpr (MODE == 1)
+        {
+            // (I + L_x/2) along the column: neighbours x +- 1 from LDS
+            // (rows past X hold D = 0, so no face reaches them)
+            float r[E];
+#pragma unroll
+            for (int e = 0; e < E; e++)
             {
-                out[static_cast<size_t>(c) * R + r] = tile[tx][k];
+                r[e] = sR[col + me + e];
+            }
+            const int xl = x0 - 1;
+            const int xr = x0 + E;
+            const float dL = (x0 > 0) ? sD[col + xl + xl / E] : 0.0f;
+            const float rL = (x0 > 0) ? sR[col + xl + xl / E] : 0.0f;
+            const float dR = (xr < kRows) ? sD[col + xr + xr / E] : 0.0f;
+            float out[E];
+#pragma unroll
+            for (int e = 0; e < E; e++)
+            {
+                const int x = x0 + e;
+                const float cv = d[e];
+                const float dp = (e + 1 < E) ? d[e + 1] : dR;
+                const float dm = (e > 0) ? d[e - 1] : dL;
+                const float rm = (e > 0) ? r[e - 1] : rL;
+                float s = 0.0f;
+                s += (x + 1 < X) ? r[e] * (dp - cv) : 0.0f;
+                s += (x > 0) ? rm * (dm - cv) : 0.0f;
+                out[e] = (x < X) ? fmaf(0.5f, s, cv) : 0.0f;
+            }
+#pragma unroll
+            for (int e = 0; e < E; e++)
+            {
+                d[e] = out[e];
+                h[e] = 0.5f * r[e] * ib[e];
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int e = 0; e < E; e++)
+            {
+                h[e] = 0.5f * sR[col + me + e] * ib[e];
+            }
+        }
+        const float hLeft = (x0 > 0) ? 0.5f * sR[col + me - 2] * sI[col + me - 2] : 0.0f;
+
+        // forward d'_x = fw_x d'_{x-1} + d_x with fw_x = h_{x-1}
+        float A = 1.0f;
+        float B = 0.0f;
+#pragma unroll
+        for (int e = 0; e < E; e++)
+        {
+            const float fw = (e == 0) ? hLeft : h[e - 1];
+            B = fmaf(fw, B, d[e]);
+            A *= fw;
+        }
+        {
+            float tb;
+            tb = DppF<0x111>(0.0f, B);
+            B = fmaf(A, tb, B);
+            A *= DppF<0x111>(1.0f, A);
+            tb = DppF<0x112>(0.0f, B);
+            B = fmaf(A, tb, B);
+            A *= DppF<0x112>(1.0f, A);
+            tb = DppF<0x114>(0.0f, B);
+            B = fmaf(A, tb, B);
+            A *= DppF<0x114>(1.0f, A);
+            tb = DppF<0x118>(0.0f, B);
+            B = fmaf(A, tb, B);
+            A *= DppF<0x118>(1.0f, A);
+            tb = DppF<0x142, 0xA>(0.0f, B);
+            B = fmaf(A, tb, B);
+            A *= DppF<0x142, 0xA>(1.0f, A);
+            tb = DppF<0x143, 0xC>(0.0f, B);
+            B = fmaf(A, tb, B);
+            A *= DppF<0x143, 0xC>(1.0f, A);
+        }
+        // carry into this lane: the preceding lanes' inclusive value
+        float acc = DppF<0x138>(0.0f, B);
+#pragma unroll
+        for (int e = 0; e < E; e++)
+        {
+            const float fw = (e == 0) ? hLeft : h[e - 1];
+            acc = fmaf(fw, acc, d[e]);
+            d[e] = acc;
+        }
+
+        // backward x_x = bw_x x_{x+1} + ib_x d'_x with bw_x = h_x (0 at X - 1)
+        A = 1.0f;
+        B = 0.0f;
+#pragma unroll
+        for (int e = E - 1; e >= 0; e--)
+        {
+            const float bw = (x0 + e < X - 1) ? h[e] : 0.0f;
+            B = fmaf(bw, B, ib[e] * d[e]);
+            A *= bw;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; k++)
+        {
+            const int o = 1 << k;
+            const float Bd = __shfl_down(B, o);
+            const float Ad = __shfl_down(A, o);
+            if (lane + o < 64)
+            {
+                B = fmaf(A, Bd, B);
+                A *= Ad;
+            }
+        }
+        const float Bn = __shfl_down(B, 1);
+        acc = (lane == 63) ? 0.0f : Bn;
+#pragma unroll
+        for (int e = E - 1; e >= 0; e--)
+        {
+            const float bw = (x0 + e < X - 1) ? h[e] : 0.0f;
+            acc = fmaf(bw, acc, ib[e] * d[e]);
+            d[e] = acc;
+        }
+#pragma unroll
+        for (int e = 0; e < E; e++)
+        {
+            sD[col + me + e] = d[e];
+        }
+        __syncthreads();
+        for (int x = t; x < X; x += 64 * C)
+        {
+            const int at = x + x / E;
+            const size_t row = static_cast<size_t>(x) * Y + y0;
+            if constexpr (VEC && C == 4)
+            {
+                *reinterpret_cast<float4 *>(a.out + row) =
+                    make_float4(sD[at], sD[kPitch + at], sD[2 * kPitch + at], sD[3 * kPitch + at]);
+            }
+            else if constexpr (VEC && C == 2)
+            {
+                *reinterpret_cast<float2 *>(a.out + row) = make_float2(sD[at], sD[kPitch + at]);
+            }
+            else
+            {
+#pragma unroll
+                for (int cc = 0; cc < C; cc++)
+                {
+                    if (y0 + cc < Y)
+                    {
+                        a.out[row + cc] = sD[cc * kPitch + at];
+                    }
+                }
             }
         }
     }
@@ -489,36 +732,43 @@ namespace
         }
     }
 
-    // Thomas factors of the line family: lines of length L with face rates
-    // rate[line][j] (face j..j+1), zero flux.
-    void FactorFamily(int32_t nLines, int32_t L, const std::vector<double> &rate, std::vector<float> &fw,
-                      std::vector<float> &ib, std::vector<float> &bw)
+    // dst[0, n) = 0 (a kernel, not a memset node, so graph replays clear it).
+    __global__ __launch_bounds__(256) void FillZero(float *__restrict__ dst, int64_t n)
     {
-        const size_t n = static_cast<size_t>(nLines) * L;
-        fw.assign(n, 0.0f);
-        ib.assign(n, 0.0f);
-        bw.assign(n, 0.0f);
+        for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+             i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        {
+            dst[i] = 0.0f;
+        }
+    }
+
+    // Reciprocal Thomas pivots ib = 1/b' of the zero-flux lines with face
+    // rates D: b_j = 1 + (D_{j-1} + D_j)/2 (absent faces dropped), b'_0 = b_0,
+    // b'_j = b_j - (D_{j-1}/2)^2 / b'_{j-1}.  Lines along x (alongX: one per
+    // column, element j at j*Y + y) or along y (one per row); ib is written in
+    // the [X][Y] layout either way.
+    void LinePivots(int32_t X, int32_t Y, const std::vector<double> &D, bool alongX, std::vector<float> &ib)
+    {
+        ib.assign(static_cast<size_t>(X) * Y, 0.0f);
+        const int32_t nLines = alongX ? Y : X;
+        const int32_t L = alongX ? X : Y;
         for (int32_t l = 0; l < nLines; l++)
         {
-            const size_t base = static_cast<size_t>(l) * L;
+            auto at = [&](int32_t j) -> size_t
+            {
+                return alongX ? static_cast<size_t>(j) * Y + l : static_cast<size_t>(l) * Y + j;
+            };
             double bpPrev = 0.0;
             for (int32_t j = 0; j < L; j++)
             {
-                const double dm = (j > 0) ? rate[base + j - 1] : 0.0;
-                const double dp = (j < L - 1) ? rate[base + j] : 0.0;
-                const double aj = -0.5 * dm;
-                const double cj = -0.5 * dp;
-                const double cPrev = (j > 0) ? -0.5 * rate[base + j - 1] : 0.0;
+                const double dm = (j > 0) ? D[at(j - 1)] : 0.0;
+                const double dp = (j < L - 1) ? D[at(j)] : 0.0;
                 double bp = 1.0 + 0.5 * (dm + dp);
-                double w = 0.0;
                 if (j > 0)
                 {
-                    w = aj / bpPrev;
-                    bp -= w * cPrev;
+                    bp -= (0.5 * dm) * (0.5 * dm) / bpPrev;
                 }
-                fw[base + j] = static_cast<float>(-w);
-                ib[base + j] = static_cast<float>(1.0 / bp);
-                bw[base + j] = static_cast<float>((j < L - 1) ? -cj / bp : 0.0);
+                ib[at(j)] = static_cast<float>(1.0 / bp);
                 bpPrev = bp;
             }
         }
@@ -535,17 +785,17 @@ struct halhip_adjoint2d
     int32_t every = 1;
     float decay = 0.0f;
     bool ready = false;
-    DeviceArray<float> D, DT;              // rates in both layouts
-    DeviceArray<float> fA, iA, bA;         // x-line factors, [Y][X]
-    DeviceArray<float> fB, iB, bB;         // y-line factors, [X][Y]
-    DeviceArray<float> u, uT, s2T, work;   // field work buffers
+    DeviceArray<float> D;                  // face rates, [X][Y]
+    DeviceArray<float> ibX, ibY;           // reciprocal pivots of the x- and y-lines, [X][Y]
+    DeviceArray<float> u, s2, work;        // field work buffers
     DeviceArray<float> ckpt;               // (ticks/every + 1) checkpoints
     DeviceArray<float> stashU, stashS2, stashS4; // one segment
-    DeviceArray<float> lam, lamNext, m1, m1T, m2T, m2, grad;
+    DeviceArray<float> lam, lamNext, m1, m2, grad;
     DeviceArray<double> partialK;
     DeviceArray<double> partialJ;
     StreamClock clock;
     Profiler prof;
+    GraphCache graphs; // the forward and gradient passes, per problem
 };
 
 namespace
@@ -611,61 +861,86 @@ namespace
         p->prof.End(p->clock.stream);
     }
 
-    void LaunchTranspose(halhip_adjoint2d *p, const float *in, float *out, int32_t R, int32_t C)
+    template <int C, int E, int MODE>
+    void LaunchColumnE(halhip_adjoint2d *p, const ColArgs &a)
     {
-        p->prof.Begin(p->clock.stream, "adjoint_transpose");
-        if (R % 4 == 0 && C % 4 == 0)
+        const dim3 grid(static_cast<unsigned>(a.blocks));
+        if (a.Y % C == 0)
         {
-            const dim3 grid(static_cast<unsigned>((C + 31) / 32), static_cast<unsigned>((R + 31) / 32));
-            hipLaunchKernelGGL(Transpose4, grid, dim3(256), 0, p->clock.stream, in, out, R, C);
+            hipLaunchKernelGGL((CellColumnSolve<C, E, MODE, true>), grid, dim3(64 * C), 0, p->clock.stream, a);
         }
         else
         {
-            const dim3 grid(static_cast<unsigned>((C + 63) / 64), static_cast<unsigned>((R + 63) / 64));
-            hipLaunchKernelGGL(Transpose, grid, dim3(256), 0, p->clock.stream, in, out, R, C);
+            hipLaunchKernelGGL((CellColumnSolve<C, E, MODE, false>), grid, dim3(64 * C), 0, p->clock.stream, a);
         }
-        Check(hipGetLastError(), "Transpose launch");
+    }
+
+    // One x-line solve per column: E = the smallest power of two with
+    // 64 E >= X, four columns per workgroup (two at E = 64, for LDS).
+    template <int MODE>
+    void LaunchColumn(halhip_adjoint2d *p, const float *in, float *out, const char *name)
+    {
+        ColArgs a{};
+        a.X = p->X;
+        a.Y = p->Y;
+        a.in = in;
+        a.D = p->D.get();
+        a.ib = p->ibX.get();
+        a.out = out;
+        int e = 1;
+        while (64 * e < a.X)
+        {
+            e *= 2;
+        }
+        const int C = (e == 64) ? 2 : 4;
+        a.blocks = (a.Y + C - 1) / C;
+        p->prof.Begin(p->clock.stream, name);
+        switch (e)
+        {
+            case 1:
+                LaunchColumnE<4, 1, MODE>(p, a);
+                break;
+            case 2:
+                LaunchColumnE<4, 2, MODE>(p, a);
+                break;
+            case 4:
+                LaunchColumnE<4, 4, MODE>(p, a);
+                break;
+            case 8:
+                LaunchColumnE<4, 8, MODE>(p, a);
+                break;
+            case 16:
+                LaunchColumnE<4, 16, MODE>(p, a);
+                break;
+            case 32:
+                LaunchColumnE<4, 32, MODE>(p, a);
+                break;
+            default:
+                LaunchColumnE<2, 64, MODE>(p, a);
+                break;
+        }
+        Check(hipGetLastError(), name);
         p->prof.End(p->clock.stream);
     }
 
-    LineArgs FamilyA(halhip_adjoint2d *p)
-    {
-        LineArgs a{};
-        a.nLines = p->Y;
-        a.L = p->X;
-        a.rate = p->DT.get();
-        a.fw = p->fA.get();
-        a.ib = p->iA.get();
-        a.bw = p->bA.get();
-        return a;
-    }
-
-    LineArgs FamilyB(halhip_adjoint2d *p)
+    // Lines along y: one per row of the [X][Y] field.
+    LineArgs Rows(halhip_adjoint2d *p)
     {
         LineArgs a{};
         a.nLines = p->X;
         a.L = p->Y;
         a.rate = p->D.get();
-        a.fw = p->fB.get();
-        a.ib = p->iB.get();
-        a.bw = p->bB.get();
+        a.ib = p->ibY.get();
         return a;
     }
 
     // One forward tick from `in` to `out` (both [X][Y]); with stash pointers,
-    // keeps s2 and s4 ([X][Y]).
+    // keeps s2 and s4.
     void ForwardTick(halhip_adjoint2d *p, const float *in, float *out, float *s2, float *s4)
     {
-        const int32_t X = p->X;
-        const int32_t Y = p->Y;
-        LaunchTranspose(p, in, p->uT.get(), X, Y);
-        LineArgs a = FamilyA(p);
-        a.in = p->uT.get();
-        a.out = p->s2T.get();
-        LaunchLine<0, false, false>(p, a, "adjoint_fwd_solve_x");
-        float *s2dst = (s2 != nullptr) ? s2 : p->work.get();
-        LaunchTranspose(p, p->s2T.get(), s2dst, Y, X);
-        LineArgs b = FamilyB(p);
+        float *s2dst = (s2 != nullptr) ? s2 : p->s2.get();
+        LaunchColumn<0>(p, in, s2dst, "adjoint_fwd_solve_x");
+        LineArgs b = Rows(p);
         b.in = s2dst;
         b.out = out;
         b.prev = in;
@@ -681,24 +956,116 @@ namespace
         }
     }
 
-    // Runs the forward model from u0 (device) storing checkpoints; result in p->u.
-    void Forward(halhip_adjoint2d *p)
+    // Enqueues the forward model from u0 in p->u, storing the checkpoints;
+    // returns the buffer holding u_T (p->u or p->lamNext: tick t reads one
+    // and writes the other, so no pointer changes hands and the pass can be
+    // replayed as a graph).
+    float *ForwardResult(halhip_adjoint2d *p)
+    {
+        return (p->ticks % 2 == 0) ? p->u.get() : p->lamNext.get();
+    }
+
+    float *EnqueueForward(halhip_adjoint2d *p)
     {
         const size_t n = Cells(p);
-        Check(hipMemcpyAsync(p->ckpt.get(), p->u.get(), n * sizeof(float), hipMemcpyDeviceToDevice, p->clock.stream),
+        float *buf[2] = {p->u.get(), p->lamNext.get()};
+        Check(hipMemcpyAsync(p->ckpt.get(), buf[0], n * sizeof(float), hipMemcpyDeviceToDevice, p->clock.stream),
               "hipMemcpyAsync");
         for (int32_t t = 0; t < p->ticks; t++)
         {
-            ForwardTick(p, p->u.get(), p->lamNext.get(), nullptr, nullptr);
-            p->u.SwapWith(p->lamNext);
+            ForwardTick(p, buf[t % 2], buf[(t + 1) % 2], nullptr, nullptr);
             if ((t + 1) % p->every == 0 && t + 1 < p->ticks)
             {
                 const size_t slot = static_cast<size_t>((t + 1) / p->every);
-                Check(hipMemcpyAsync(p->ckpt.get() + slot * n, p->u.get(), n * sizeof(float), hipMemcpyDeviceToDevice,
-                                     p->clock.stream),
+                Check(hipMemcpyAsync(p->ckpt.get() + slot * n, buf[(t + 1) % 2], n * sizeof(float),
+                                     hipMemcpyDeviceToDevice, p->clock.stream),
                       "hipMemcpyAsync");
             }
         }
+        return ForwardResult(p);
+    }
+
+    // Enqueues the whole gradient pass from u0 in p->u and the target in
+    // p->work: the forward with checkpoints, the loss seed, then per segment
+    // (last first) the recompute with stashes and the adjoint ticks.  Loss
+    // partials go to partialJ, decay-gradient partials to partialK.
+    void EnqueueGradient(halhip_adjoint2d *p)
+    {
+        const int32_t X = p->X;
+        const int32_t Y = p->Y;
+        const size_t n = Cells(p);
+        hipStream_t s = p->clock.stream;
+        float *uT = EnqueueForward(p);
+        hipLaunchKernelGGL(LossSeed, dim3(kLossBlocks), dim3(256), 0, s, uT, p->work.get(), p->lam.get(),
+                           static_cast<int64_t>(n), p->partialJ.get());
+        Check(hipGetLastError(), "LossSeed launch");
+        hipLaunchKernelGGL(FillZero, dim3(1024), dim3(256), 0, s, p->grad.get(), static_cast<int64_t>(n));
+        Check(hipGetLastError(), "FillZero launch");
+        const dim3 gGrid(static_cast<unsigned>((Y + 63) / 64), static_cast<unsigned>((X + 3) / 4));
+        const size_t gBlocks = static_cast<size_t>(gGrid.x) * gGrid.y;
+        float *lam[2] = {p->lam.get(), p->lamNext.get()};
+        int32_t j = 0; // adjoint ticks done: lam[j % 2] holds the current one
+        const int32_t nSeg = (p->ticks + p->every - 1) / p->every;
+        for (int32_t seg = nSeg - 1; seg >= 0; seg--)
+        {
+            const int32_t t0 = seg * p->every;
+            const int32_t t1 = std::min(p->ticks, t0 + p->every);
+            // Recompute the segment from its checkpoint, keeping u, s2, s4.
+            Check(hipMemcpyAsync(p->stashU.get(), p->ckpt.get() + static_cast<size_t>(seg) * n, n * sizeof(float),
+                                 hipMemcpyDeviceToDevice, s),
+                  "hipMemcpyAsync");
+            for (int32_t t = t0; t < t1; t++)
+            {
+                const size_t k = static_cast<size_t>(t - t0);
+                float *next = (t + 1 < t1) ? p->stashU.get() + (k + 1) * n : p->s2.get();
+                ForwardTick(p, p->stashU.get() + k * n, next, p->stashS2.get() + k * n, p->stashS4.get() + k * n);
+            }
+            // Adjoint ticks backwards through the segment.
+            for (int32_t t = t1 - 1; t >= t0; t--, j++)
+            {
+                const size_t k = static_cast<size_t>(t - t0);
+                LineArgs b = Rows(p);
+                b.in = lam[j % 2];
+                b.out = p->m1.get();
+                LaunchLine<2, false, false>(p, b, "adjoint_solve_y");
+                LaunchColumn<1>(p, p->m1.get(), p->m2.get(), "adjoint_solve_x");
+                GradArgs g{};
+                g.X = X;
+                g.Y = Y;
+                g.D = p->D.get();
+                g.m1 = p->m1.get();
+                g.m2 = p->m2.get();
+                g.s2 = p->stashS2.get() + k * n;
+                g.s4 = p->stashS4.get() + k * n;
+                g.u = p->stashU.get() + k * n;
+                g.lam = lam[j % 2];
+                g.lamNext = lam[(j + 1) % 2];
+                g.gradD = p->grad.get();
+                g.partialK = p->partialK.get() + static_cast<size_t>(t) * gBlocks;
+                g.negK = -p->decay;
+                p->prof.Begin(s, "adjoint_gradient");
+                hipLaunchKernelGGL(CellGradient, gGrid, dim3(256), 0, s, g);
+                Check(hipGetLastError(), "CellGradient launch");
+                p->prof.End(s);
+            }
+        }
+    }
+
+    // Runs `body` (a whole pass) on the plan's stream: replayed from a graph
+    // captured on first use, or launched directly while profiling (so each
+    // kernel gets its own events).
+    template <class F>
+    void RunPass(halhip_adjoint2d *p, int32_t pass, F body)
+    {
+        if (p->prof.On())
+        {
+            body();
+            return;
+        }
+        std::string key;
+        AppendKey(key, pass);
+        hipGraphExec_t exec = FindOrCapture(p->graphs, key, p->clock.stream, body);
+        Check(hipGraphLaunch(exec, p->clock.stream), "hipGraphLaunch");
     }
 }
 
@@ -724,9 +1091,8 @@ extern "C"
                 p->X = xDim;
                 p->Y = yDim;
                 const size_t n = Cells(p);
-                for (DeviceArray<float> *b : {&p->D, &p->DT, &p->fA, &p->iA, &p->bA, &p->fB, &p->iB, &p->bB, &p->u,
-                                              &p->uT, &p->s2T, &p->work, &p->lam, &p->lamNext, &p->m1, &p->m1T,
-                                              &p->m2T, &p->m2, &p->grad})
+                for (DeviceArray<float> *b : {&p->D, &p->ibX, &p->ibY, &p->u, &p->s2, &p->work, &p->lam, &p->lamNext,
+                                              &p->m1, &p->m2, &p->grad})
                 {
                     b->Allocate(ctx, n);
                 }
@@ -771,34 +1137,27 @@ extern "C"
                 const int32_t Y = p->Y;
                 const size_t n = Cells(p);
                 std::vector<double> Dxy(diffusivity, diffusivity + n);
-                std::vector<double> Dyx(n);
-                for (int32_t x = 0; x < X; x++)
+                for (double v : Dxy)
                 {
-                    for (int32_t y = 0; y < Y; y++)
-                    {
-                        const double v = Dxy[static_cast<size_t>(x) * Y + y];
-                        Require(std::isfinite(v) && v >= 0.0, "halhip_adjoint2d_set_problem: diffusivity must be finite "
-                                                               "and non-negative");
-                        Dyx[static_cast<size_t>(y) * X + x] = v;
-                    }
+                    Require(std::isfinite(v) && v >= 0.0, "halhip_adjoint2d_set_problem: diffusivity must be finite "
+                                                           "and non-negative");
                 }
-                std::vector<float> f, i, b;
+                // The device computes with D in FP32; the pivots are formed
+                // in FP64 from the same FP32 values.
+                std::vector<float> Df(Dxy.begin(), Dxy.end());
+                std::vector<double> Dr(Df.begin(), Df.end());
+                std::vector<float> ibx, iby;
+                LinePivots(X, Y, Dr, true, ibx);
+                LinePivots(X, Y, Dr, false, iby);
                 auto up = [&](DeviceArray<float> &dst, const std::vector<float> &src)
                 {
                     Check(hipMemcpy(dst.get(), src.data(), src.size() * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
                 };
-                std::vector<float> Df(Dxy.begin(), Dxy.end()), DTf(Dyx.begin(), Dyx.end());
                 Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
                 up(p->D, Df);
-                up(p->DT, DTf);
-                FactorFamily(Y, X, Dyx, f, i, b);
-                up(p->fA, f);
-                up(p->iA, i);
-                up(p->bA, b);
-                FactorFamily(X, Y, Dxy, f, i, b);
-                up(p->fB, f);
-                up(p->iB, i);
-                up(p->bB, b);
+                up(p->ibX, ibx);
+                up(p->ibY, iby);
+                p->graphs.Clear(); // they hold the old checkpoint and stash buffers
                 p->decay = static_cast<float>(decay);
                 p->ticks = ticks;
                 p->every = checkpointEvery;
@@ -825,8 +1184,8 @@ extern "C"
                 const size_t n = Cells(p);
                 Check(hipMemcpyAsync(p->u.get(), u0, n * sizeof(float), hipMemcpyHostToDevice, p->clock.stream),
                       "hipMemcpyAsync");
-                Forward(p);
-                Check(hipMemcpyAsync(uT, p->u.get(), n * sizeof(float), hipMemcpyDeviceToHost, p->clock.stream),
+                RunPass(p, 0, [&] { EnqueueForward(p); });
+                Check(hipMemcpyAsync(uT, ForwardResult(p), n * sizeof(float), hipMemcpyDeviceToHost, p->clock.stream),
                       "hipMemcpyAsync");
                 Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
                 p->prof.Collect();
@@ -849,71 +1208,15 @@ extern "C"
                 const size_t n = Cells(p);
                 hipStream_t s = p->clock.stream;
                 Check(hipMemcpyAsync(p->u.get(), u0, n * sizeof(float), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
-                Forward(p);
-                // Seed: lam = u_T - target, J = 1/2 |lam|^2.  The target goes
-                // through the work buffer.
+                // The target goes through the work buffer.
                 Check(hipMemcpyAsync(p->work.get(), target, n * sizeof(float), hipMemcpyHostToDevice, s),
                       "hipMemcpyAsync");
-                const unsigned lossBlocks = 1024;
-                hipLaunchKernelGGL(LossSeed, dim3(lossBlocks), dim3(256), 0, s, p->u.get(), p->work.get(), p->lam.get(),
-                                   static_cast<int64_t>(n), p->partialJ.get());
-                Check(hipGetLastError(), "LossSeed launch");
-                Check(hipMemsetAsync(p->grad.get(), 0, n * sizeof(float), s), "hipMemsetAsync");
+                RunPass(p, 1, [&] { EnqueueGradient(p); });
                 const dim3 gGrid(static_cast<unsigned>((Y + 63) / 64), static_cast<unsigned>((X + 3) / 4));
                 const size_t gBlocks = static_cast<size_t>(gGrid.x) * gGrid.y;
                 std::vector<double> hostK(gBlocks * static_cast<size_t>(p->ticks));
                 double gk = 0.0;
-                const int32_t nSeg = (p->ticks + p->every - 1) / p->every;
-                for (int32_t seg = nSeg - 1; seg >= 0; seg--)
-                {
-                    const int32_t t0 = seg * p->every;
-                    const int32_t t1 = std::min(p->ticks, t0 + p->every);
-                    // Recompute the segment from its checkpoint, keeping u, s2, s4.
-                    Check(hipMemcpyAsync(p->stashU.get(), p->ckpt.get() + static_cast<size_t>(seg) * n,
-                                         n * sizeof(float), hipMemcpyDeviceToDevice, s),
-                          "hipMemcpyAsync");
-                    for (int32_t t = t0; t < t1; t++)
-                    {
-                        const size_t k = static_cast<size_t>(t - t0);
-                        float *next = (t + 1 < t1) ? p->stashU.get() + (k + 1) * n : p->work.get();
-                        ForwardTick(p, p->stashU.get() + k * n, next, p->stashS2.get() + k * n,
-                                    p->stashS4.get() + k * n);
-                    }
-                    // Adjoint ticks backwards through the segment.
-                    for (int32_t t = t1 - 1; t >= t0; t--)
-                    {
-                        const size_t k = static_cast<size_t>(t - t0);
-                        LineArgs b = FamilyB(p);
-                        b.in = p->lam.get();
-                        b.out = p->m1.get();
-                        LaunchLine<2, false, false>(p, b, "adjoint_solve_y");
-                        LaunchTranspose(p, p->m1.get(), p->m1T.get(), X, Y);
-                        LineArgs a = FamilyA(p);
-                        a.in = p->m1T.get();
-                        a.out = p->m2T.get();
-                        LaunchLine<1, false, false>(p, a, "adjoint_solve_x");
-                        LaunchTranspose(p, p->m2T.get(), p->m2.get(), Y, X);
-                        GradArgs g{};
-                        g.X = X;
-                        g.Y = Y;
-                        g.D = p->D.get();
-                        g.m1 = p->m1.get();
-                        g.m2 = p->m2.get();
-                        g.s2 = p->stashS2.get() + k * n;
-                        g.s4 = p->stashS4.get() + k * n;
-                        g.u = p->stashU.get() + k * n;
-                        g.lam = p->lam.get();
-                        g.lamNext = p->lamNext.get();
-                        g.gradD = p->grad.get();
-                        g.partialK = p->partialK.get() + static_cast<size_t>(t) * gBlocks;
-                        g.negK = -p->decay;
-                        p->prof.Begin(s, "adjoint_gradient");
-                        hipLaunchKernelGGL(CellGradient, gGrid, dim3(256), 0, s, g);
-                        Check(hipGetLastError(), "CellGradient launch");
-                        p->prof.End(s);
-                        p->lam.SwapWith(p->lamNext);
-                    }
-                }
+                const unsigned lossBlocks = kLossBlocks;
                 std::vector<double> hostJ(lossBlocks);
                 Check(hipMemcpyAsync(hostJ.data(), p->partialJ.get(), lossBlocks * sizeof(double),
                                      hipMemcpyDeviceToHost, s),

@@ -31,8 +31,10 @@
 //                           oracle_adi", the dot-product test <Ax,y>=<x,A^T y>
 //                           and FP64 central finite differences.
 #include <cmath>
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace
@@ -190,114 +192,196 @@ namespace
         }
     }
 
-    // Coefficients and solve of one ADI line.  off = -rate/2; periodic lines
-    // use b = 1 - 2*off; otherwise each in-line neighbour weighs 1 and past an
+    // Worker threads for the line loops of the ADI sweeps and the per-agent
+    // force sums (1 by default; oracle_set_threads).  Lines and agents are
+    // independent, so the result does not depend on the count.
+    int g_threads = 1;
+
+    // Runs body(begin, end) over [0, n) split into g_threads contiguous
+    // ranges, one thread each.
+    template <class F>
+    void ParallelRanges(int32_t n, F body)
+    {
+        const int32_t T = std::max(1, std::min<int32_t>(g_threads, n));
+        if (T == 1)
+        {
+            body(0, n);
+            return;
+        }
+        std::vector<std::thread> pool;
+        for (int32_t t = 0; t < T; t++)
+        {
+            const int32_t b = static_cast<int32_t>(static_cast<int64_t>(n) * t / T);
+            const int32_t e = static_cast<int32_t>(static_cast<int64_t>(n) * (t + 1) / T);
+            pool.emplace_back([=] { body(b, e); });
+        }
+        for (auto &th : pool)
+        {
+            th.join();
+        }
+    }
+
+    // Coefficients of one ADI line.  off = -rate/2; periodic lines use
+    // b = 1 - 2*off; otherwise each in-line neighbour weighs 1 and past an
     // end a Dirichlet ghost weighs 2, a zero-flux edge 0.
     // include/Hal/Pde/ADIequations.h:225-270.
+    struct LineCoefs
+    {
+        int32_t n = 0;
+        bool wrap = false;
+        std::vector<double> a, b, c;
+        LineCoefs(int32_t len, double rate, bool wrapLine, bool hasBc) : n(len), wrap(wrapLine)
+        {
+            double off = -rate / 2;
+            a.assign(static_cast<size_t>(n), off);
+            c.assign(static_cast<size_t>(n), off);
+            b.resize(static_cast<size_t>(n));
+            if (wrap)
+            {
+                for (int32_t i = 0; i < n; i++)
+                {
+                    b[i] = 1 - 2 * off;
+                }
+                return;
+            }
+            double endAdjust = hasBc ? 1.0 : -1.0;
+            for (int32_t i = 0; i < n; i++)
+            {
+                double weight = 2.0;
+                if (i == 0)
+                {
+                    weight += endAdjust;
+                }
+                if (i == n - 1)
+                {
+                    weight += endAdjust;
+                }
+                b[i] = 1 - weight * off;
+            }
+        }
+        // TDMA / TDMAperiodic (include/Hal/Pde/TdmaSolver.h:27-120).
+        void Solve(const double *d, double *x, std::vector<double> &work) const
+        {
+            if (wrap)
+            {
+                ThomasPeriodic(n, a.data(), b.data(), c.data(), d, x, work);
+            }
+            else
+            {
+                Thomas(n, a.data(), b.data(), c.data(), d, x, work);
+            }
+        }
+    };
+
     void SolveLine(int32_t n, double rate, bool wrap, bool hasBc, const double *d, double *x,
                    std::vector<double> &work)
     {
-        double off = -rate / 2;
-        std::vector<double> a(static_cast<size_t>(n), off);
-        std::vector<double> c(static_cast<size_t>(n), off);
-        std::vector<double> b(static_cast<size_t>(n));
-        if (wrap)
-        {
-            for (int32_t i = 0; i < n; i++)
-            {
-                b[i] = 1 - 2 * off;
-            }
-            ThomasPeriodic(n, a.data(), b.data(), c.data(), d, x, work);
-            return;
-        }
-        double endAdjust = hasBc ? 1.0 : -1.0;
-        for (int32_t i = 0; i < n; i++)
-        {
-            double weight = 2.0;
-            if (i == 0)
-            {
-                weight += endAdjust;
-            }
-            if (i == n - 1)
-            {
-                weight += endAdjust;
-            }
-            b[i] = 1 - weight * off;
-        }
-        Thomas(n, a.data(), b.data(), c.data(), d, x, work);
+        LineCoefs(n, rate, wrap, hasBc).Solve(d, x, work);
     }
 
     // Peaceman-Rachford ADI: x-implicit sweep (explicit in y) into scratch,
     // then y-implicit sweep (explicit in x) into deltas += v - field.
     // include/Hal/Pde/ADIequations.h:306-378; src/Pde/ADIequations.cpp:19-47;
     // ImplicitBoundarySource include/Hal/Pde/ADIequations.h:198-220.
+    // The x-sweep handles lines in groups of kGroup adjacent columns, so its
+    // reads and writes run along rows; every line's arithmetic is unchanged.
     void Adi2D(const double *field, double *deltas, double rate, int32_t X, int32_t Y, bool wrapX, bool wrapY,
                const double *bcXlo, const double *bcXhi, const double *bcYlo, const double *bcYhi)
     {
+        constexpr int32_t kGroup = 16;
         bool hasBc = (bcXlo != nullptr);
         std::vector<double> scratch(static_cast<size_t>(X) * Y);
-        std::vector<double> work;
-        std::vector<double> d(static_cast<size_t>(X > Y ? X : Y));
-        std::vector<double> v(d.size());
-        for (int32_t y = 0; y < Y; y++)
+        const LineCoefs lineX(X, rate, wrapX, hasBc);
+        const LineCoefs lineY(Y, rate, wrapY, hasBc);
+        const int32_t groups = (Y + kGroup - 1) / kGroup;
+        ParallelRanges(groups, [&](int32_t g0, int32_t g1)
         {
-            for (int32_t i = 0; i < X; i++)
+            std::vector<double> work;
+            std::vector<double> d(static_cast<size_t>(X) * kGroup);
+            std::vector<double> v(static_cast<size_t>(X));
+            std::vector<double> out(static_cast<size_t>(X) * kGroup);
+            for (int32_t g = g0; g < g1; g++)
             {
-                double c = field[static_cast<size_t>(i) * Y + y];
-                double s = DeltaY(field, c, i, y + 1, X, Y, wrapY, bcYlo, bcYhi);
-                s += DeltaY(field, c, i, y - 1, X, Y, wrapY, bcYlo, bcYhi);
-                double rhs = c + (s * rate) / 2;
-                if (hasBc && !wrapX)
+                const int32_t y0 = g * kGroup;
+                const int32_t ny = std::min(kGroup, Y - y0);
+                for (int32_t i = 0; i < X; i++)
                 {
-                    double src = 0.0;
-                    if (i == 0)
+                    for (int32_t j = 0; j < ny; j++)
                     {
-                        src += bcXlo[y];
+                        const int32_t y = y0 + j;
+                        double c = field[static_cast<size_t>(i) * Y + y];
+                        double s = DeltaY(field, c, i, y + 1, X, Y, wrapY, bcYlo, bcYhi);
+                        s += DeltaY(field, c, i, y - 1, X, Y, wrapY, bcYlo, bcYhi);
+                        double rhs = c + (s * rate) / 2;
+                        if (hasBc && !wrapX)
+                        {
+                            double src = 0.0;
+                            if (i == 0)
+                            {
+                                src += bcXlo[y];
+                            }
+                            if (i == X - 1)
+                            {
+                                src += bcXhi[y];
+                            }
+                            rhs += src * 2.0 * rate / 2;
+                        }
+                        d[static_cast<size_t>(j) * X + i] = rhs;
                     }
-                    if (i == X - 1)
-                    {
-                        src += bcXhi[y];
-                    }
-                    rhs += src * 2.0 * rate / 2;
                 }
-                d[i] = rhs;
+                for (int32_t j = 0; j < ny; j++)
+                {
+                    lineX.Solve(d.data() + static_cast<size_t>(j) * X, v.data(), work);
+                    for (int32_t i = 0; i < X; i++)
+                    {
+                        out[static_cast<size_t>(i) * kGroup + j] = v[i];
+                    }
+                }
+                for (int32_t i = 0; i < X; i++)
+                {
+                    for (int32_t j = 0; j < ny; j++)
+                    {
+                        scratch[static_cast<size_t>(i) * Y + y0 + j] = out[static_cast<size_t>(i) * kGroup + j];
+                    }
+                }
             }
-            SolveLine(X, rate, wrapX, hasBc, d.data(), v.data(), work);
-            for (int32_t i = 0; i < X; i++)
-            {
-                scratch[static_cast<size_t>(i) * Y + y] = v[i];
-            }
-        }
-        for (int32_t x = 0; x < X; x++)
+        });
+        ParallelRanges(X, [&](int32_t x0, int32_t x1)
         {
-            for (int32_t i = 0; i < Y; i++)
+            std::vector<double> work;
+            std::vector<double> d(static_cast<size_t>(Y));
+            std::vector<double> v(static_cast<size_t>(Y));
+            for (int32_t x = x0; x < x1; x++)
             {
-                double c = scratch[static_cast<size_t>(x) * Y + i];
-                double s = DeltaX(scratch.data(), c, x + 1, i, X, Y, wrapX, bcXlo, bcXhi);
-                s += DeltaX(scratch.data(), c, x - 1, i, X, Y, wrapX, bcXlo, bcXhi);
-                double rhs = c + (s * rate) / 2;
-                if (hasBc && !wrapY)
+                for (int32_t i = 0; i < Y; i++)
                 {
-                    double src = 0.0;
-                    if (i == 0)
+                    double c = scratch[static_cast<size_t>(x) * Y + i];
+                    double s = DeltaX(scratch.data(), c, x + 1, i, X, Y, wrapX, bcXlo, bcXhi);
+                    s += DeltaX(scratch.data(), c, x - 1, i, X, Y, wrapX, bcXlo, bcXhi);
+                    double rhs = c + (s * rate) / 2;
+                    if (hasBc && !wrapY)
                     {
-                        src += bcYlo[x];
+                        double src = 0.0;
+                        if (i == 0)
+                        {
+                            src += bcYlo[x];
+                        }
+                        if (i == Y - 1)
+                        {
+                            src += bcYhi[x];
+                        }
+                        rhs += src * 2.0 * rate / 2;
                     }
-                    if (i == Y - 1)
-                    {
-                        src += bcYhi[x];
-                    }
-                    rhs += src * 2.0 * rate / 2;
+                    d[i] = rhs;
                 }
-                d[i] = rhs;
+                lineY.Solve(d.data(), v.data(), work);
+                for (int32_t i = 0; i < Y; i++)
+                {
+                    size_t k = static_cast<size_t>(x) * Y + i;
+                    deltas[k] += v[i] - field[k];
+                }
             }
-            SolveLine(Y, rate, wrapY, hasBc, d.data(), v.data(), work);
-            for (int32_t i = 0; i < Y; i++)
-            {
-                size_t k = static_cast<size_t>(x) * Y + i;
-                deltas[k] += v[i] - field[k];
-            }
-        }
+        });
     }
 
     // xoshiro256** seeded through splitmix64; src/Rng.cpp:13-20, 37-45, 56-69,
@@ -657,6 +741,13 @@ extern "C"
     }
 
     // One ADI line solve with the reference's coefficients (ADISolveRow).
+    // Worker threads of the ADI line loops and the per-agent force sums
+    // (results do not depend on the count).
+    void oracle_set_threads(int32_t threads)
+    {
+        g_threads = threads < 1 ? 1 : threads;
+    }
+
     void oracle_adi_line(int32_t n, double rate, int32_t wrap, int32_t hasBc, const double *d, double *x)
     {
         std::vector<double> work;
@@ -743,7 +834,8 @@ extern "C"
         std::vector<int32_t> offsets(static_cast<size_t>(C + 1));
         std::vector<int32_t> bucket(static_cast<size_t>(N > 0 ? N : 1));
         oracle_agent_bins(pos, N, X, Y, offsets.data(), bucket.data());
-        for (int32_t i = 0; i < N; i++)
+        // each agent's sum is independent: split over the worker threads
+        ParallelRanges(N, [&](int32_t i0, int32_t i1) { for (int32_t i = i0; i < i1; i++)
         {
             double px = pos[2 * static_cast<size_t>(i)];
             double py = pos[2 * static_cast<size_t>(i) + 1];
@@ -800,7 +892,7 @@ extern "C"
             }
             vel[2 * static_cast<size_t>(i)] = vx;
             vel[2 * static_cast<size_t>(i) + 1] = vy;
-        }
+        } });
     }
 
     // Move with AgentPT2D::MoveSafePT semantics (include/Hal/Agents/AgentPT2D.h:157-186):
@@ -939,7 +1031,7 @@ namespace
     // out = (I + sign*L/2) in along x (axis 0) or y (axis 1).
     void CellExplicit(int32_t X, int32_t Y, const double *D, const double *in, double *out, int axis, double sign)
     {
-        for (int32_t x = 0; x < X; x++)
+        ParallelRanges(X, [&](int32_t x0, int32_t x1) { for (int32_t x = x0; x < x1; x++)
         {
             for (int32_t y = 0; y < Y; y++)
             {
@@ -970,26 +1062,33 @@ namespace
                 }
                 out[i] = c + sign * 0.5 * s;
             }
-        }
+        } });
     }
 
     // x-implicit (axis 0: lines along x, one per y) or y-implicit solve.
     void CellSolve(int32_t X, int32_t Y, const double *D, const double *d, double *x, int axis)
     {
-        std::vector<double> work;
         if (axis == 0)
         {
-            for (int32_t y = 0; y < Y; y++)
+            ParallelRanges(Y, [&](int32_t y0, int32_t y1)
             {
-                CellLineSolve(X, D, static_cast<size_t>(y), static_cast<size_t>(Y), d, x, work);
-            }
+                std::vector<double> work;
+                for (int32_t y = y0; y < y1; y++)
+                {
+                    CellLineSolve(X, D, static_cast<size_t>(y), static_cast<size_t>(Y), d, x, work);
+                }
+            });
         }
         else
         {
-            for (int32_t xx = 0; xx < X; xx++)
+            ParallelRanges(X, [&](int32_t x0, int32_t x1)
             {
-                CellLineSolve(Y, D, static_cast<size_t>(xx) * Y, 1, d, x, work);
-            }
+                std::vector<double> work;
+                for (int32_t xx = x0; xx < x1; xx++)
+                {
+                    CellLineSolve(Y, D, static_cast<size_t>(xx) * Y, 1, d, x, work);
+                }
+            });
         }
     }
 
@@ -1075,6 +1174,7 @@ extern "C"
         }
         std::fill(gradD, gradD + n, 0.0);
         double gk = 0.0;
+        std::vector<double> gkRow(static_cast<size_t>(X));
         auto dx = [&](const std::vector<double> &p, size_t i, int32_t x) { return (x + 1 < X) ? p[i + Y] - p[i] : 0.0; };
         auto dy = [&](const std::vector<double> &p, size_t i, int32_t y) { return (y + 1 < Y) ? p[i + 1] - p[i] : 0.0; };
         for (int32_t t = ticks - 1; t >= 0; t--)
@@ -1082,16 +1182,27 @@ extern "C"
             CellSolve(X, Y, D, lam.data(), m1.data(), 1);
             CellExplicit(X, Y, D, m1.data(), w.data(), 0, 1.0);
             CellSolve(X, Y, D, w.data(), m2.data(), 0);
+            // per-row partial sums of dJ/dk, added in row order: the same
+            // bits for any thread count
+            ParallelRanges(X, [&](int32_t x0, int32_t x1)
+            {
+                for (int32_t x = x0; x < x1; x++)
+                {
+                    double row = 0.0;
+                    for (int32_t y = 0; y < Y; y++)
+                    {
+                        const size_t i = static_cast<size_t>(x) * Y + y;
+                        const double g = dy(m1, i, y) * dy(s4[t], i, y) + dx(m1, i, x) * dx(s2[t], i, x) +
+                                         dx(m2, i, x) * dx(s2[t], i, x) + dy(m2, i, y) * dy(u[t], i, y);
+                        gradD[i] -= 0.5 * g;
+                        row -= lam[i] * u[t][i];
+                    }
+                    gkRow[static_cast<size_t>(x)] = row;
+                }
+            });
             for (int32_t x = 0; x < X; x++)
             {
-                for (int32_t y = 0; y < Y; y++)
-                {
-                    const size_t i = static_cast<size_t>(x) * Y + y;
-                    const double g = dy(m1, i, y) * dy(s4[t], i, y) + dx(m1, i, x) * dx(s2[t], i, x) +
-                                     dx(m2, i, x) * dx(s2[t], i, x) + dy(m2, i, y) * dy(u[t], i, y);
-                    gradD[i] -= 0.5 * g;
-                    gk -= lam[i] * u[t][i];
-                }
+                gk += gkRow[static_cast<size_t>(x)];
             }
             CellExplicit(X, Y, D, m2.data(), next.data(), 1, 1.0);
             for (size_t i = 0; i < n; i++)

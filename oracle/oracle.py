@@ -149,6 +149,13 @@ class Oracle(_Lib):
     def __init__(self, path=ORACLE_SO):
         super().__init__(path, "oracle_")
 
+    def set_threads(self, n):
+        """Worker threads of the line loops (ADI and the per-cell ADI and its
+        adjoint); the results do not depend on the count."""
+        f = self.fn("set_threads")
+        f.argtypes = [_i32]
+        f(int(n))
+
     def model_ticks(self, field, ticks, rate, xVel, yVel, decay, X, Y, wrapX, wrapY, bc=None, src_idx=(),
                     src_val=()):
         """Diffusion + Advection + MulAll + Add + Update ticks (FP64)."""

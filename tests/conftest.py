@@ -16,10 +16,17 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built libhalhip.so")
 
 
+# Worker threads for the oracle's line loops: this host's CPU share, at most
+# 16 (results do not depend on the count).
+ORACLE_THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
 @pytest.fixture(scope="session")
 def oracle():
     from oracle import Oracle
-    return Oracle()
+    o = Oracle()
+    o.set_threads(ORACLE_THREADS)
+    return o
 
 
 @pytest.fixture(scope="session")

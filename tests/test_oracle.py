@@ -319,6 +319,35 @@ def test_cell_adi_gradient_vs_central_differences(oracle):
     assert abs(fdk - gk) < 1e-6 * max(1.0, abs(fdk))
 
 
+def test_cell_adi_gradient_at_workload_size_vs_fixture(oracle):
+    """The FP64 adjoint on the 1024^2, 200-tick adjoint workload against the
+    committed FP64 central differences (tests/golden/make_adjoint_fd.py)."""
+    import os
+    import workloads
+    fd = np.load(os.path.join(os.path.dirname(__file__), "golden", "adjoint_fd_1024.npz"), allow_pickle=False)
+    w = workloads.adjoint_problem()
+    X, Y, ticks = w["X"], w["Y"], w["ticks"]
+    f32 = lambda a: np.asarray(a, np.float64).astype(np.float32).astype(np.float64)
+    k = float(np.float32(w["decay"]))
+    u0 = w["u0"].astype(np.float64)
+    target = f32(oracle.adi_cell_forward(u0, f32(w["D_target"]), k, ticks, X, Y))
+    J, gD, gk = oracle.adi_cell_gradient(u0, f32(w["D"]), k, ticks, X, Y, target)
+    assert abs(J - fd["loss"][0]) < 1e-12 * J
+    scale = np.abs(gD).max()
+    g = gD[fd["cells"]]
+    # central differences at h = 1e-6 carry about eps J / h (~1.3e-12) of
+    # rounding in the loss difference; the bound allows 8 times that
+    atol = 8 * np.finfo(np.float64).eps * J / fd["h"][0]
+    np.testing.assert_allclose(g, fd["fd_gradD"], rtol=1e-5, atol=atol)
+    assert np.abs(fd["fd_gradD"]).max() > 1e4 * atol  # some cells carry a real gradient
+    assert scale >= np.abs(g).max()
+    # dJ/dk: the fourth-order difference; the two-point one is off by its
+    # h^2 J''' / 6 truncation (~1e-4 relative here: u_T is a degree-200
+    # polynomial in k)
+    assert abs(gk - fd["fd4_gradK"][0]) < 1e-7 * abs(gk)
+    assert abs(gk - fd["fd_gradK"][0]) < 2e-4 * abs(gk)
+
+
 # ---- upwind advection ----------------------------------------------------------
 
 def test_advection_matches_golden_bit_for_bit(oracle, golden):

@@ -1,0 +1,196 @@
+"""The headline workloads at their full size and configured length, on the
+GPU, against the FP64 oracle (whose line loops run on this host's threads;
+the results do not depend on the count).
+
+  ADI      4096^2 blobs, D = 5, Dirichlet 0, 200 ticks
+           (Diffusion2DADI<BCFn>, include/Hal/Pde/ADIequations.h:339-378)
+  agents   4M agents on 2048^2 bins, 100 ticks of bin -> scan -> stable
+           scatter -> forces -> move (CpuAgentForceBackend2D.cpp:85-176,
+           AgentPT2D.h:157-186): bins, offsets and order bit-exact every tick
+  wells    512 uneven wells: every well for 10 ticks, the smallest, median
+           and largest for the full 500 (MultiWellExperiment.h:82-103)
+  adjoint  1024^2, 200 ticks, checkpoint every 20: gradient vs the FP64 CPU
+           adjoint and vs FP64 central differences (tests/golden/adjoint_fd_1024.npz)
+
+Tolerances: 1e-5 relative L2 for FP32 fields against FP64 (per 200 or 500
+ticks, tighter than the 1e-5 per 1000 ticks the reference CPU path is held
+to); bins bit-exact; gradients within 1e-4.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import halhip
+import workloads
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def f32(a):
+    return np.asarray(a, np.float64).astype(np.float32).astype(np.float64)
+
+
+def test_adi_4096_dirichlet_200_ticks(ctx, oracle):
+    w = workloads.adi_blobs()
+    n = w["X"]
+    plan = halhip.Adi2D(ctx, n, n)
+    plan.upload(w["field"])
+    print("ADI 4096^2 Dirichlet 0, 200 ticks: GPU", flush=True)
+    plan.prepare_steps(200, w["rate"], 0.0, False, False, bc=w["bc"])
+    plan.run_steps(200, w["rate"], 0.0, False, False, bc=w["bc"])
+    got = plan.download()
+    rows, carries, window = plan.methods()
+    plan.close()
+    ref = oracle.adi_ticks(w["field"].astype(np.float64), 200, w["rate"], 0.0, n, n, 0, 0, bc=w["bc"])
+    err = rel_l2(got, ref)
+    print(f"ADI 4096^2 Dirichlet 0, 200 ticks (rows {rows}, carries {carries}, K {window}): "
+          f"rel L2 vs FP64 oracle = {err:.3e}, max abs = {np.abs(got - ref).max():.3e}", flush=True)
+    assert rows == halhip.ADI_ROWS_LDS and carries == halhip.ADI_CARRIES_WINDOW
+    assert err < 1e-5
+
+
+def test_agents_4m_100_ticks(ctx, oracle):
+    w = workloads.agents_uniform()
+    n = w["rad"].size
+    X, Y = w["X"], w["Y"]
+    args = (w["R"], w["k"], w["dt"], False, False)
+    rad64 = w["rad"].astype(np.float64)
+    plan = halhip.Agents2D(ctx, X, Y, n)
+    plan.upload(w["pos"], w["rad"])
+    before = plan.positions()
+    for t in range(100):
+        plan.run_ticks(1, *args)
+        offsets, bucket = plan.bins()
+        ro, rb = oracle.agent_bins(before.astype(np.float64), X, Y)
+        assert np.array_equal(offsets, ro), t
+        assert np.array_equal(bucket, rb), t
+        after = plan.positions()
+        if t in (0, 49, 99):
+            # one tick from the same FP32 positions against the FP64 tick:
+            # the FP32 rounding of p + dt v (half an ulp, 1.2e-4 below 2048)
+            # plus FP32 force sums.  An agent whose target sits within
+            # rounding of a wall may be kept by one side and moved by the
+            # other (MoveSafePT), so agents within 0.1 of a wall get dt |v|.
+            ref = oracle.agent_ticks(before.astype(np.float64), rad64, 1, *args[:3], X, Y, 0, 0)
+            d = np.abs(after.astype(np.float64) - ref).reshape(-1, 2)
+            p = ref.reshape(-1, 2)
+            wall = ((p < 0.1) | (p > np.array([X, Y]) - 0.1)).any(1)
+            print(f"agents 4M: tick {t} bins bit-exact; one-tick positions vs FP64 max abs = "
+                  f"{d[~wall].max():.3e} (agents near a wall: {d[wall].max() if wall.any() else 0:.3e})", flush=True)
+            assert d[~wall].max() <= 1.3e-4
+            assert (d[wall].max() if wall.any() else 0.0) <= 0.1
+        before = after
+    stepped = before
+    # the same 100 ticks in one call (replayed 10-tick graphs) give the same bits
+    plan.upload(w["pos"], w["rad"])
+    plan.prepare_ticks(100, *args)
+    plan.run_ticks(100, *args)
+    graphed = plan.positions()
+    plan.close()
+    assert np.array_equal(graphed, stepped)
+    # The 100-tick trajectories: agents pressed together at a tiny separation
+    # turn along disp / |disp|, which FP32 and FP64 round differently, so a
+    # few trajectories part; the bulk stays at FP32 rounding.
+    ref = oracle.agent_ticks(w["pos"].astype(np.float64), rad64, 100, *args[:3], X, Y, 0, 0)
+    d = np.abs(graphed.astype(np.float64) - ref)
+    moved = np.abs(ref - w["pos"].astype(np.float64))
+    rel = np.linalg.norm(graphed - ref) / np.linalg.norm(moved)
+    print(f"agents 4M, 100 ticks: positions vs FP64 trajectory mean abs = {d.mean():.3e}, 99.9th percentile = "
+          f"{np.quantile(d, 0.999):.3e}, max = {d.max():.3e}; rel L2 of the displacement = {rel:.3e}", flush=True)
+    assert d.mean() < 1e-3
+    assert np.quantile(d, 0.999) < 1e-2
+    assert rel < 5e-3
+
+
+def _batch(ctx, spec, carries):
+    b = halhip.Batch(ctx, spec["xdims"], spec["ydims"])
+    b.set_wells(spec["rate"], spec["decay"])
+    b.set_sources(spec["src_well"], spec["src_cell"], spec["src_val"])
+    b.set_carries(carries)
+    b.upload(np.zeros(b.cells, np.float32))
+    return b
+
+
+def _oracle_well(oracle, spec, k, ticks):
+    X = int(spec["xdims"][k])
+    sel = spec["src_well"] == k
+    return oracle.adi_ticks(np.zeros(X * X), ticks, float(spec["rate"][k]), float(spec["decay"][k]), X, X, 0, 0,
+                            src_idx=spec["src_cell"][sel], src_val=spec["src_val"][sel].astype(np.float64))
+
+
+@pytest.mark.parametrize("carries", [halhip.ADI_CARRIES_SCAN, halhip.ADI_CARRIES_WINDOW])
+def test_wells_512_all_wells_10_ticks(ctx, oracle, carries):
+    spec = workloads.wells_batch()
+    b = _batch(ctx, spec, carries)
+    b.run_ticks(10)
+    got = b.download()
+    used = b.carries()
+    b.close()
+    worst = 0.0
+    for k in range(len(spec["xdims"])):
+        err = rel_l2(b.well(got, k), _oracle_well(oracle, spec, k, 10))
+        worst = max(worst, err)
+        assert err < 1e-5, (k, int(spec["xdims"][k]), err)
+    print(f"512 wells, 10 ticks, carries {used}: worst per-well rel L2 vs FP64 = {worst:.3e}", flush=True)
+    assert used == carries
+
+
+def test_wells_512_full_500_ticks(ctx, oracle):
+    spec = workloads.wells_batch()
+    b = halhip.Batch(ctx, spec["xdims"], spec["ydims"])
+    b.set_wells(spec["rate"], spec["decay"])
+    b.set_sources(spec["src_well"], spec["src_cell"], spec["src_val"])
+    b.upload(np.zeros(b.cells, np.float32))
+    b.autotune(10)  # the configuration the benchmark runs
+    b.upload(np.zeros(b.cells, np.float32))
+    b.prepare_ticks(500)
+    b.run_ticks(500)
+    got = b.download()
+    rows, used = b.strip_rows(), b.carries()
+    b.close()
+    order = np.argsort(spec["xdims"], kind="stable")
+    for k in (int(order[0]), int(order[len(order) // 2]), int(order[-1])):
+        err = rel_l2(b.well(got, k), _oracle_well(oracle, spec, k, 500))
+        print(f"512 wells, 500 ticks (strip rows {rows}, carries {used}): well {k} side "
+              f"{int(spec['xdims'][k])} rel L2 vs FP64 = {err:.3e}", flush=True)
+        assert err < 1e-5, (k, err)
+
+
+def test_adjoint_1024_gradient_200_ticks_checkpoint_20(ctx, oracle):
+    fd = np.load(os.path.join(GOLDEN, "adjoint_fd_1024.npz"), allow_pickle=False)
+    w = workloads.adjoint_problem()
+    X, Y, ticks, every = (int(v) for v in fd["meta"])
+    assert (X, Y, ticks, every) == (w["X"], w["Y"], w["ticks"], w["every"])
+    k = float(np.float32(w["decay"]))
+    u64 = w["u0"].astype(np.float64)
+    # the target of the fixture: FP64 forward of the perturbed diffusivity, in FP32
+    target = f32(oracle.adi_cell_forward(u64, f32(w["D_target"]), k, ticks, X, Y))
+    p = halhip.Adjoint2D(ctx, X, Y)
+    p.set_problem(w["D"], w["decay"], ticks, every)
+    print("adjoint 1024^2, 200 ticks, checkpoint 20: GPU gradient", flush=True)
+    J, gD, gk = p.gradient(w["u0"], target.astype(np.float32))
+    p.close()
+    # FP64 central differences at 64 seeded cells and for the decay rate
+    cells = fd["cells"]
+    scale = np.abs(fd["fd_gradD"]).max()
+    cell_err = np.abs(gD[cells] - fd["fd_gradD"]) / scale
+    # dJ/dk against the fourth-order difference (the two-point one carries a
+    # ~1e-4 truncation error in k; tests/golden/make_adjoint_fd.py)
+    k_err = abs(gk - fd["fd4_gradK"][0]) / abs(fd["fd4_gradK"][0])
+    print(f"adjoint 1024^2 vs FP64 central differences (h = {fd['h'][0]:g}): J rel = "
+          f"{abs(J - fd['loss'][0]) / fd['loss'][0]:.3e}, cells max |g - fd| / max|g| = {cell_err.max():.3e}, "
+          f"dJ/dk rel = {k_err:.3e}", flush=True)
+    assert abs(J - fd["loss"][0]) < 1e-4 * fd["loss"][0]
+    assert cell_err.max() < 1e-4
+    assert k_err < 1e-4
+    # the whole gradient against the FP64 CPU adjoint
+    J64, gD64, gk64 = oracle.adi_cell_gradient(u64, f32(w["D"]), k, ticks, X, Y, target)
+    err = rel_l2(gD, gD64)
+    print(f"adjoint 1024^2 vs FP64 CPU adjoint: dJ/dD rel L2 = {err:.3e}, dJ/dk rel = "
+          f"{abs(gk - gk64) / abs(gk64):.3e}", flush=True)
+    assert err < 1e-4
+    assert abs(gk - gk64) < 1e-4 * abs(gk64)
