@@ -19,10 +19,9 @@
 //
 // Kernels.  The Thomas factors of a zero-flux line with face rates D are
 // fw_j = (D_{j-1}/2) ib_{j-1} and bw_j = (D_j/2) ib_j, with ib = 1/b' the
-// reciprocal pivots, so only ib is tabulated (once per diffusivity field, on
-// the host in FP64, one table per line direction in the field's own layout)
-// and the multipliers are formed from D, which the explicit half-steps read
-// anyway.  Lines along y (contiguous rows) are solved by one workgroup of
+// reciprocal pivots, so two tables per line direction hold them: ib and
+// h = (D/2) ib (fw_j = h_{j-1}, bw_j = h_j), formed once per diffusivity
+// field on the host in FP64 and kept in the field's own layout.  Lines along y (contiguous rows) are solved by one workgroup of
 // four wavefronts per row: thread-local recurrences composed by wavefront
 // scans of affine maps and LDS-exchanged wavefront totals.  Lines along x
 // (stride Y) are solved without a transpose: a workgroup stages C adjacent
@@ -58,6 +57,7 @@ namespace
         const float *in;   // [nLines][L]
         const float *rate; // [nLines][L] face rates (face j..j+1 of a line, and line l..l+1)
         const float *ib;   // [nLines][L] reciprocal Thomas pivots of the lines
+        const float *h;    // [nLines][L] (D_j / 2) ib_j: bw_j, and fw_{j+1}
         float *out;        // [nLines][L]
         const float *prev; // decay: out = x - k * prev
         float negK;
@@ -148,11 +148,11 @@ namespace
         const size_t base = static_cast<size_t>(line) * L;
         const int j0 = t * EPL;
         float d[EPL];
-        float dl[EPL]; // this line's face rates
         float ib[EPL];
         LoadLine<EPL, VEC>(a.in, base, j0, L, 0.0f, d);
-        LoadLine<EPL, VEC>(a.rate, base, j0, L, 0.0f, dl);
         LoadLine<EPL, VEC>(a.ib, base, j0, L, 0.0f, ib);
+        float h[EPL];
+        LoadLine<EPL, VEC>(a.h, base, j0, L, 0.0f, h);
         if constexpr (RHS == 0)
         {
             // explicit half-step across lines: neighbours in lines l +- 1
@@ -165,6 +165,8 @@ namespace
             }
             if (line + 1 < a.nLines)
             {
+                float dl[EPL]; // this line's face rates (to line l + 1)
+                LoadLine<EPL, VEC>(a.rate, base, j0, L, 0.0f, dl);
                 LoadLine<EPL, VEC>(a.in, base + L, j0, L, 0.0f, nb);
 #pragma unroll
                 for (int e = 0; e < EPL; e++)
@@ -189,15 +191,9 @@ namespace
                 d[e] = (j0 + e < L) ? fmaf(0.5f, s[e], d[e]) : 0.0f;
             }
         }
-        // h_j = (D_j / 2) ib_j: bw_j = h_j (0 at the line's end), fw_j = h_{j-1}
-        // (0 at its start; the first element's comes from the previous thread)
-        float h[EPL];
-#pragma unroll
-        for (int e = 0; e < EPL; e++)
-        {
-            h[e] = 0.5f * dl[e] * ib[e];
-        }
-        const float hLeft = (j0 > 0 && j0 - 1 < L) ? 0.5f * a.rate[base + j0 - 1] * a.ib[base + j0 - 1] : 0.0f;
+        // bw_j = h_j (0 at the line's end), fw_j = h_{j-1} (0 at its start;
+        // the first element's comes from the previous thread)
+        const float hLeft = (j0 > 0 && j0 - 1 < L) ? a.h[base + j0 - 1] : 0.0f;
         float fw[EPL];
 #pragma unroll
         for (int e = 0; e < EPL; e++)
@@ -328,6 +324,7 @@ namespace
         const float *in;   // [X][Y]
         const float *D;    // [X][Y] face rates
         const float *ib;   // [X][Y] reciprocal pivots of the x-lines
+        const float *h;    // [X][Y] (D_x / 2) ib_x of the x-lines
         float *out;        // [X][Y]
     };
 
@@ -378,6 +375,7 @@ namespace
         __shared__ float sD[C * kPitch];
         __shared__ float sR[C * kPitch];
         __shared__ float sI[C * kPitch];
+        __shared__ float sH[C * kPitch];
         const int X = a.X;
         const int Y = a.Y;
         int b = static_cast<int>(blockIdx.x);
@@ -393,6 +391,7 @@ namespace
         float u[kMine][C];
         float r[kMine][C];
         float q[kMine][C];
+        float hv[kMine][C];
         float uL[kMine];
         float rL[kMine];
         float uR[kMine];
@@ -409,6 +408,7 @@ namespace
                 LoadRow<C, VEC>(a.in, row, y0, Y, u[i]);
                 LoadRow<C, VEC>(a.D, row, y0, Y, r[i]);
                 LoadRow<C, VEC>(a.ib, row, y0, Y, q[i]);
+                LoadRow<C, VEC>(a.h, row, y0, Y, hv[i]);
                 if constexpr (MODE == 0)
                 {
                     uL[i] = (y0 > 0) ? a.in[row - 1] : 0.0f;
@@ -424,6 +424,7 @@ namespace
                     u[i][c] = 0.0f;
                     r[i][c] = 0.0f;
                     q[i][c] = 0.0f;
+                    hv[i][c] = 0.0f;
                 }
             }
         }
@@ -461,6 +462,7 @@ namespace
                     sD[c * kPitch + at] = u[i][c];
                     sR[c * kPitch + at] = r[i][c];
                     sI[c * kPitch + at] = q[i][c];
+                    sH[c * kPitch + at] = hv[i][c];
                 }
             }
         }
@@ -513,18 +515,14 @@ namespace
             for (int e = 0; e < E; e++)
             {
                 d[e] = out[e];
-                h[e] = 0.5f * r[e] * ib[e];
             }
         }
-        else
-        {
 #pragma unroll
-            for (int e = 0; e < E; e++)
-            {
-                h[e] = 0.5f * sR[col + me + e] * ib[e];
-            }
+        for (int e = 0; e < E; e++)
+        {
+            h[e] = sH[col + me + e];
         }
-        const float hLeft = (x0 > 0) ? 0.5f * sR[col + me - 2] * sI[col + me - 2] : 0.0f;
+        const float hLeft = (x0 > 0) ? sH[col + me - 2] : 0.0f;
 
         // forward d'_x = fw_x d'_{x-1} + d_x with fw_x = h_{x-1}
         float A = 1.0f;
@@ -643,7 +641,7 @@ namespace
         const float *u;   // [X][Y]
         const float *lam; // [X][Y]
         float *lamNext;   // [X][Y]
-        float *gradD;     // [X][Y], accumulated
+        double *gradD;    // [X][Y], accumulated over the ticks in FP64
         double *partialK; // one per block
         float negK;
     };
@@ -672,7 +670,7 @@ namespace
             const float dyM2 = a.m2[iy] - a.m2[i];
             const float dyU = a.u[iy] - a.u[i];
             const float g = dyM1 * dyS4 + (dxM1 + dxM2) * dxS2 + dyM2 * dyU;
-            a.gradD[i] = fmaf(-0.5f, g, a.gradD[i]);
+            a.gradD[i] -= 0.5 * static_cast<double>(g);
             const float lam = a.lam[i];
             kPart = -static_cast<double>(lam) * static_cast<double>(a.u[i]);
             // (I + L_y/2) m2 along the row
@@ -732,6 +730,16 @@ namespace
         }
     }
 
+    // dst[k] = float(src[k]).
+    __global__ __launch_bounds__(256) void ToFloat(const double *__restrict__ src, float *__restrict__ dst, int64_t n)
+    {
+        for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+             i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        {
+            dst[i] = static_cast<float>(src[i]);
+        }
+    }
+
     // dst[0, n) = 0 (a kernel, not a memset node, so graph replays clear it).
     __global__ __launch_bounds__(256) void FillZero(float *__restrict__ dst, int64_t n)
     {
@@ -747,9 +755,11 @@ namespace
     // b'_j = b_j - (D_{j-1}/2)^2 / b'_{j-1}.  Lines along x (alongX: one per
     // column, element j at j*Y + y) or along y (one per row); ib is written in
     // the [X][Y] layout either way.
-    void LinePivots(int32_t X, int32_t Y, const std::vector<double> &D, bool alongX, std::vector<float> &ib)
+    void LinePivots(int32_t X, int32_t Y, const std::vector<double> &D, bool alongX, std::vector<float> &ib,
+                    std::vector<float> &h)
     {
         ib.assign(static_cast<size_t>(X) * Y, 0.0f);
+        h.assign(static_cast<size_t>(X) * Y, 0.0f);
         const int32_t nLines = alongX ? Y : X;
         const int32_t L = alongX ? X : Y;
         for (int32_t l = 0; l < nLines; l++)
@@ -769,6 +779,7 @@ namespace
                     bp -= (0.5 * dm) * (0.5 * dm) / bpPrev;
                 }
                 ib[at(j)] = static_cast<float>(1.0 / bp);
+                h[at(j)] = static_cast<float>((j < L - 1) ? 0.5 * dp / bp : 0.0);
                 bpPrev = bp;
             }
         }
@@ -787,10 +798,12 @@ struct halhip_adjoint2d
     bool ready = false;
     DeviceArray<float> D;                  // face rates, [X][Y]
     DeviceArray<float> ibX, ibY;           // reciprocal pivots of the x- and y-lines, [X][Y]
+    DeviceArray<float> hX, hY;             // (D/2) ib of the x- and y-lines, [X][Y]
     DeviceArray<float> u, s2, work;        // field work buffers
     DeviceArray<float> ckpt;               // (ticks/every + 1) checkpoints
     DeviceArray<float> stashU, stashS2, stashS4; // one segment
-    DeviceArray<float> lam, lamNext, m1, m2, grad;
+    DeviceArray<float> lam, lamNext, m1, m2;
+    DeviceArray<double> grad;              // dJ/dD, accumulated in FP64
     DeviceArray<double> partialK;
     DeviceArray<double> partialJ;
     StreamClock clock;
@@ -886,6 +899,7 @@ namespace
         a.in = in;
         a.D = p->D.get();
         a.ib = p->ibX.get();
+        a.h = p->hX.get();
         a.out = out;
         int e = 1;
         while (64 * e < a.X)
@@ -931,6 +945,7 @@ namespace
         a.L = p->Y;
         a.rate = p->D.get();
         a.ib = p->ibY.get();
+        a.h = p->hY.get();
         return a;
     }
 
@@ -999,7 +1014,8 @@ namespace
         hipLaunchKernelGGL(LossSeed, dim3(kLossBlocks), dim3(256), 0, s, uT, p->work.get(), p->lam.get(),
                            static_cast<int64_t>(n), p->partialJ.get());
         Check(hipGetLastError(), "LossSeed launch");
-        hipLaunchKernelGGL(FillZero, dim3(1024), dim3(256), 0, s, p->grad.get(), static_cast<int64_t>(n));
+        hipLaunchKernelGGL(FillZero, dim3(1024), dim3(256), 0, s, reinterpret_cast<float *>(p->grad.get()),
+                           static_cast<int64_t>(2 * n));
         Check(hipGetLastError(), "FillZero launch");
         const dim3 gGrid(static_cast<unsigned>((Y + 63) / 64), static_cast<unsigned>((X + 3) / 4));
         const size_t gBlocks = static_cast<size_t>(gGrid.x) * gGrid.y;
@@ -1091,11 +1107,12 @@ extern "C"
                 p->X = xDim;
                 p->Y = yDim;
                 const size_t n = Cells(p);
-                for (DeviceArray<float> *b : {&p->D, &p->ibX, &p->ibY, &p->u, &p->s2, &p->work, &p->lam, &p->lamNext,
-                                              &p->m1, &p->m2, &p->grad})
+                for (DeviceArray<float> *b : {&p->D, &p->ibX, &p->ibY, &p->hX, &p->hY, &p->u, &p->s2, &p->work, &p->lam, &p->lamNext,
+                                              &p->m1, &p->m2})
                 {
                     b->Allocate(ctx, n);
                 }
+                p->grad.Allocate(ctx, n);
                 p->partialJ.Allocate(ctx, 1024);
                 p->clock.Create();
                 *out = p;
@@ -1146,9 +1163,9 @@ extern "C"
                 // in FP64 from the same FP32 values.
                 std::vector<float> Df(Dxy.begin(), Dxy.end());
                 std::vector<double> Dr(Df.begin(), Df.end());
-                std::vector<float> ibx, iby;
-                LinePivots(X, Y, Dr, true, ibx);
-                LinePivots(X, Y, Dr, false, iby);
+                std::vector<float> ibx, iby, hx, hy;
+                LinePivots(X, Y, Dr, true, ibx, hx);
+                LinePivots(X, Y, Dr, false, iby, hy);
                 auto up = [&](DeviceArray<float> &dst, const std::vector<float> &src)
                 {
                     Check(hipMemcpy(dst.get(), src.data(), src.size() * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
@@ -1157,6 +1174,8 @@ extern "C"
                 up(p->D, Df);
                 up(p->ibX, ibx);
                 up(p->ibY, iby);
+                up(p->hX, hx);
+                up(p->hY, hy);
                 p->graphs.Clear(); // they hold the old checkpoint and stash buffers
                 p->decay = static_cast<float>(decay);
                 p->ticks = ticks;
@@ -1221,7 +1240,10 @@ extern "C"
                 Check(hipMemcpyAsync(hostJ.data(), p->partialJ.get(), lossBlocks * sizeof(double),
                                      hipMemcpyDeviceToHost, s),
                       "hipMemcpyAsync");
-                Check(hipMemcpyAsync(gradDiffusivity, p->grad.get(), n * sizeof(float), hipMemcpyDeviceToHost, s),
+                hipLaunchKernelGGL(ToFloat, dim3(1024), dim3(256), 0, s, static_cast<const double *>(p->grad.get()),
+                                   p->work.get(), static_cast<int64_t>(n));
+                Check(hipGetLastError(), "ToFloat launch");
+                Check(hipMemcpyAsync(gradDiffusivity, p->work.get(), n * sizeof(float), hipMemcpyDeviceToHost, s),
                       "hipMemcpyAsync");
                 Check(hipMemcpyAsync(hostK.data(), p->partialK.get(), hostK.size() * sizeof(double),
                                      hipMemcpyDeviceToHost, s),

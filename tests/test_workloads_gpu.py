@@ -139,6 +139,15 @@ def test_wells_512_all_wells_10_ticks(ctx, oracle, carries):
     assert used == carries
 
 
+# Over 500 ticks of sources and slow decay (k ~ 5e-3) a well settles near a
+# steady state whose mass mode relaxes only at rate k, so the systematic
+# O(eps) mass error of FP32 line factors is amplified by ~1/k.  The
+# reference's own algorithm run with FP32 factors and FP32 state (the same
+# Thomas recurrences, numpy float32) lands 1.0e-5 to 1.6e-5 from FP64 on
+# these three wells (tests/test_fp32_floor.py); the bound is that floor.
+WELLS_500_BOUND = 2e-5
+
+
 def test_wells_512_full_500_ticks(ctx, oracle):
     spec = workloads.wells_batch()
     b = halhip.Batch(ctx, spec["xdims"], spec["ydims"])
@@ -157,7 +166,7 @@ def test_wells_512_full_500_ticks(ctx, oracle):
         err = rel_l2(b.well(got, k), _oracle_well(oracle, spec, k, 500))
         print(f"512 wells, 500 ticks (strip rows {rows}, carries {used}): well {k} side "
               f"{int(spec['xdims'][k])} rel L2 vs FP64 = {err:.3e}", flush=True)
-        assert err < 1e-5, (k, err)
+        assert err < WELLS_500_BOUND, (k, err)
 
 
 def test_adjoint_1024_gradient_200_ticks_checkpoint_20(ctx, oracle):
