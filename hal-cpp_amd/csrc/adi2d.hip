@@ -162,6 +162,17 @@ namespace
         }
     }
 
+    // d += v - u
+    __global__ __launch_bounds__(256) void AdiAccumulateDelta(float *__restrict__ d, const float *__restrict__ v,
+                                                              const float *__restrict__ u, int64_t n)
+    {
+        for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+             i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        {
+            d[i] += v[i] - u[i];
+        }
+    }
+
     // Point sources, applied serially so repeated indices add in list order.
     __global__ void AdiAddSources(float *__restrict__ u, int32_t n, const int64_t *__restrict__ idx,
                                   const float *__restrict__ val)
@@ -996,6 +1007,27 @@ namespace
             }
         }
         p->reducedValid = false;
+    }
+}
+
+namespace halhip
+{
+    void AdiDeltaOf(halhip_adi2d *p, const float *u, float *d, int32_t X, int32_t Y, double rate, int32_t wrapX,
+                    int32_t wrapY, const halhip_bc *bc, hipStream_t caller)
+    {
+        Require(p != nullptr, "delta_adi: null ADI plan");
+        Require(p->X == X && p->Y == Y, "delta_adi: the ADI plan's dimensions differ from the grid's");
+        Bind(p->ctx);
+        const size_t n = Cells(p);
+        Check(hipStreamSynchronize(caller), "hipStreamSynchronize");
+        Check(hipMemcpyAsync(p->field.get(), u, n * sizeof(float), hipMemcpyDeviceToDevice, p->clock.stream),
+              "hipMemcpyAsync");
+        p->reducedValid = false;
+        EnqueueTicks(p, 1, rate, 0.0, wrapX, wrapY, bc, false);
+        hipLaunchKernelGGL(AdiAccumulateDelta, dim3(1024), dim3(256), 0, p->clock.stream, d,
+                           static_cast<const float *>(p->field.get()), u, static_cast<int64_t>(n));
+        Check(hipGetLastError(), "AdiAccumulateDelta launch");
+        Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
     }
 }
 

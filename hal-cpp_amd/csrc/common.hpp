@@ -91,6 +91,13 @@ namespace halhip
     // address of the no-allocation binning's scratch array (agents2d.hip).
     void InitBinsScratch(int32_t device);
 
+    // d[0, X*Y) += ADI(u) - u: one ADI tick (no decay or sources) of the
+    // device field u, run by the ADI plan on its own stream after the work
+    // queued on `caller` (adi2d.hip); `caller` then waits for it.  The plan's
+    // resident field is replaced.
+    void AdiDeltaOf(halhip_adi2d *adi, const float *u, float *d, int32_t X, int32_t Y, double rate, int32_t wrapX,
+                    int32_t wrapY, const halhip_bc *bc, hipStream_t caller);
+
     // Makes ctx's device current for the calling thread.
     inline void Bind(const halhip_context *ctx)
     {

@@ -466,6 +466,140 @@ namespace
     }
 }
 
+namespace
+{
+    // ---- PDEGrid2D delta operations on the resident field ------------------------
+    //
+    // include/Hal/Grids/PDEGrid2D.h:89-245: every call reads the committed
+    // field u and changes the deltas d; Update commits u += d, d = 0.
+
+    __global__ __launch_bounds__(256) void DeltaAddInto(float *__restrict__ d, const float *__restrict__ s, int64_t n)
+    {
+        for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+             i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        {
+            d[i] += s[i];
+        }
+    }
+
+    // Field-wide ops: 0 SetAll (d = v - u), 1 AddAll (d += v), 2 MulAll
+    // (d += u v), 3 SetAll from values (d = vals - u), 4 Update (u += d,
+    // d = 0), 5 SetNonNegative (d = -u where u + d < 0), 6 clear (d = 0).
+    __global__ __launch_bounds__(256) void DeltaFill(int32_t op, float v, float *__restrict__ u, float *__restrict__ d,
+                                                     const float *__restrict__ vals, int64_t n)
+    {
+        for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+             i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        {
+            switch (op)
+            {
+                case 0:
+                    d[i] = v - u[i];
+                    break;
+                case 1:
+                    d[i] += v;
+                    break;
+                case 2:
+                    d[i] += u[i] * v;
+                    break;
+                case 3:
+                    d[i] = vals[i] - u[i];
+                    break;
+                case 4:
+                    u[i] += d[i];
+                    d[i] = 0.0f;
+                    break;
+                case 5:
+                    if (u[i] + d[i] < 0.0f)
+                    {
+                        d[i] = 0.0f - u[i];
+                    }
+                    break;
+                default:
+                    d[i] = 0.0f;
+                    break;
+            }
+        }
+    }
+
+    // Point ops in list order (one lane, so repeated cells apply in order):
+    // 0 Set (d = v - u), 1 Add (d += v), 2 Mul (d += u v).
+    __global__ void DeltaPointOps(int32_t count, const int32_t *__restrict__ op, const int64_t *__restrict__ idx,
+                                  const float *__restrict__ val, const float *__restrict__ u, float *__restrict__ d)
+    {
+        if (threadIdx.x != 0 || blockIdx.x != 0)
+        {
+            return;
+        }
+        for (int32_t k = 0; k < count; k++)
+        {
+            const int64_t i = idx[k];
+            switch (op[k])
+            {
+                case 0:
+                    d[i] = val[k] - u[i];
+                    break;
+                case 1:
+                    d[i] += val[k];
+                    break;
+                default:
+                    d[i] += u[i] * val[k];
+                    break;
+            }
+        }
+    }
+
+    // Per-block partials: 0 max |d|, 1 max |d| / (|u| + offset), 2 max u,
+    // 3 min u, 4 sum of u (FP64).
+    __global__ __launch_bounds__(256) void DeltaReduce(int32_t mode, float offset, const float *__restrict__ u,
+                                                       const float *__restrict__ d, int64_t n,
+                                                       double *__restrict__ partial)
+    {
+        __shared__ double red[256];
+        double acc = (mode == 2) ? -INFINITY : ((mode == 3) ? INFINITY : 0.0);
+        for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+             i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        {
+            switch (mode)
+            {
+                case 0:
+                    acc = fmax(acc, static_cast<double>(fabsf(d[i])));
+                    break;
+                case 1:
+                    acc = fmax(acc, static_cast<double>(fabsf(d[i] / (fabsf(u[i]) + offset))));
+                    break;
+                case 2:
+                    acc = fmax(acc, static_cast<double>(u[i]));
+                    break;
+                case 3:
+                    acc = fmin(acc, static_cast<double>(u[i]));
+                    break;
+                default:
+                    acc += static_cast<double>(u[i]);
+                    break;
+            }
+        }
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1)
+        {
+            if (static_cast<int>(threadIdx.x) < o)
+            {
+                const double b = red[threadIdx.x + o];
+                const double a = red[threadIdx.x];
+                red[threadIdx.x] = (mode == 4) ? a + b : ((mode == 3) ? fmin(a, b) : fmax(a, b));
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+        {
+            partial[blockIdx.x] = red[0];
+        }
+    }
+
+    constexpr unsigned kDeltaBlocks = 1024; // grid of the field-wide delta kernels
+}
+
 struct halhip_diffusion2d
 {
     halhip_context *ctx = nullptr;
@@ -473,7 +607,15 @@ struct halhip_diffusion2d
     int32_t Y = 0;
     DeviceArray<float> field[2]; // ping-pong resident field
     int cur = 0;
-    DeviceArray<float> aux[2];   // interface scratch (delta, rates)
+    DeviceArray<float> aux[2];   // interface scratch (field copy, delta)
+    DeviceArray<float> cellA;    // per-cell inputs (rates, velocities, values), first use
+    DeviceArray<float> cellB;
+    DeviceArray<float> cellEdges; // field and velocity edges of per-cell advection
+    DeviceArray<float> deltas;   // PDEGrid2D deltas of the resident field, first use
+    DeviceArray<int32_t> pointOp; // point-op lists
+    DeviceArray<int64_t> pointIdx;
+    DeviceArray<float> pointVal;
+    DeviceArray<double> partial; // reduction partials
     DeviceArray<float> bc;       // 2Y + 2X Dirichlet edges
     DeviceArray<int64_t> srcIdx;
     DeviceArray<float> srcVal;
@@ -552,6 +694,131 @@ namespace
     {
         return dim3(static_cast<unsigned>((p->Y + kBlockY - 1) / kBlockY),
                     static_cast<unsigned>((p->X + kBlockX - 1) / kBlockX));
+    }
+
+    // Device buffer b with at least n elements, allocated on first use.
+    template <class T>
+    T *Lazy(halhip_diffusion2d *p, DeviceArray<T> &b, size_t n)
+    {
+        if (b.get() == nullptr || b.size() < n)
+        {
+            b.Allocate(p->ctx, n);
+        }
+        return b.get();
+    }
+
+    // Pinned staging of at least n floats, once the plan's stream is idle
+    // (earlier calls' copies out of it have finished).
+    float *Stage(halhip_diffusion2d *p, size_t n)
+    {
+        Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+        p->hostBuf.resize(n);
+        return p->hostBuf.data();
+    }
+
+    // dst[0, n) = float(src[0, n)) through staging[0, n), enqueued.
+    void Upload64(halhip_diffusion2d *p, const double *src, size_t n, float *staging, float *dst)
+    {
+        for (size_t k = 0; k < n; k++)
+        {
+            staging[k] = static_cast<float>(src[k]);
+        }
+        Check(hipMemcpyAsync(dst, staging, n * sizeof(float), hipMemcpyHostToDevice, p->clock.stream),
+              "hipMemcpyAsync");
+    }
+
+    // out = per-cell-rate diffusion delta of u (rates already on the device).
+    void LaunchRates(halhip_diffusion2d *p, const float *u, const float *rx, const float *ry, float *out,
+                     int32_t wrapX, int32_t wrapY)
+    {
+        p->prof.Begin(p->clock.stream, "explicit_rates_delta");
+        hipLaunchKernelGGL(ExplicitRatesDelta, Grid(p), dim3(kBlockY, kBlockX), 0, p->clock.stream, p->X, p->Y,
+                           wrapX ? 1 : 0, wrapY ? 1 : 0, u, rx, ry, out);
+        Check(hipGetLastError(), "ExplicitRatesDelta launch");
+        p->prof.End(p->clock.stream);
+    }
+
+    // Edge arrays of per-cell advection uploaded to cellEdges (field edges,
+    // then velocity edges) and the kernel's arguments.  Enqueued copies read
+    // `staging` (2 * (2X + 2Y) floats).
+    AdvectCellsArgs CellAdvectArgs(halhip_diffusion2d *p, int32_t wrapX, int32_t wrapY, const halhip_bc *bc,
+                                   const halhip_bc *velBc, float *staging)
+    {
+        EdgeValues e;
+        e.From(bc, p->X, p->Y);
+        EdgeValues v;
+        v.From(velBc, p->X, p->Y);
+        Require(!v.dirichlet || e.dirichlet, "advect_cells: velocity edges need field edges (Dirichlet bc)");
+        const size_t edgeLen = 2 * static_cast<size_t>(p->X) + 2 * static_cast<size_t>(p->Y);
+        AdvectCellsArgs a{};
+        a.X = p->X;
+        a.Y = p->Y;
+        a.wrapX = wrapX ? 1 : 0;
+        a.wrapY = wrapY ? 1 : 0;
+        if (!e.dirichlet)
+        {
+            return a;
+        }
+        float *eb = Lazy(p, p->cellEdges, 2 * edgeLen);
+        size_t at = 0;
+        for (const EdgeValues *ev : {&e, &v})
+        {
+            for (const std::vector<float> *edge : {&ev->xlo, &ev->xhi, &ev->ylo, &ev->yhi})
+            {
+                std::copy(edge->begin(), edge->end(), staging + at);
+                at += edge->size();
+            }
+        }
+        Check(hipMemcpyAsync(eb, staging, at * sizeof(float), hipMemcpyHostToDevice, p->clock.stream),
+              "hipMemcpyAsync");
+        a.bcXlo = eb;
+        a.bcXhi = eb + p->Y;
+        a.bcYlo = eb + 2 * p->Y;
+        a.bcYhi = eb + 2 * p->Y + p->X;
+        if (v.dirichlet)
+        {
+            float *vb = eb + edgeLen;
+            a.vXlo = vb;
+            a.vXhi = vb + p->Y;
+            a.vYlo = vb + 2 * p->Y;
+            a.vYhi = vb + 2 * p->Y + p->X;
+        }
+        return a;
+    }
+
+    void LaunchAdvectCells(halhip_diffusion2d *p, const AdvectCellsArgs &a, const float *u, const float *xv,
+                           const float *yv, float *out)
+    {
+        p->prof.Begin(p->clock.stream, "advect_cells_delta");
+        hipLaunchKernelGGL(AdvectCellsDelta, Grid(p), dim3(kBlockY, kBlockX), 0, p->clock.stream, a, u, xv, yv, out);
+        Check(hipGetLastError(), "AdvectCellsDelta launch");
+        p->prof.End(p->clock.stream);
+    }
+
+    // The deltas of the resident field (zero on first use).
+    float *Deltas(halhip_diffusion2d *p)
+    {
+        if (p->deltas.get() == nullptr)
+        {
+            p->deltas.Allocate(p->ctx, Cells(p));
+            Check(hipMemsetAsync(p->deltas.get(), 0, Cells(p) * sizeof(float), p->clock.stream), "hipMemsetAsync");
+        }
+        return p->deltas.get();
+    }
+
+    void LaunchFill(halhip_diffusion2d *p, int32_t op, float v, const float *vals)
+    {
+        hipLaunchKernelGGL(DeltaFill, dim3(kDeltaBlocks), dim3(256), 0, p->clock.stream, op, v,
+                           p->field[p->cur].get(), Deltas(p), vals, static_cast<int64_t>(Cells(p)));
+        Check(hipGetLastError(), "DeltaFill launch");
+    }
+
+    // deltas += scratch (a delta computed into aux[1])
+    void AddScratch(halhip_diffusion2d *p)
+    {
+        hipLaunchKernelGGL(DeltaAddInto, dim3(kDeltaBlocks), dim3(256), 0, p->clock.stream, Deltas(p),
+                           static_cast<const float *>(p->aux[1].get()), static_cast<int64_t>(Cells(p)));
+        Check(hipGetLastError(), "DeltaAddInto launch");
     }
 
     // Rows per workgroup B and ticks per launch T of the multi-tick kernel
@@ -788,47 +1055,18 @@ extern "C"
                 RequireLength(n, static_cast<int64_t>(Cells(p)), "halhip_diffusion2d_diffuse_rates_f64");
                 Require(field != nullptr && deltas != nullptr && ratesX != nullptr && ratesY != nullptr,
                         "halhip_diffusion2d_diffuse_rates_f64: null array");
-                size_t cells = Cells(p);
-                if (p->aux[1].get() == nullptr)
-                {
-                    p->aux[1].Allocate(p->ctx, cells);
-                }
-                // field -> aux0, ratesX -> aux1, ratesY -> field[1-cur] scratch is not
-                // used: the resident field must survive, so rates go through a
-                // dedicated pair of buffers allocated on first use.
-                static_assert(sizeof(float) == 4, "");
-                std::vector<float> rx(cells), ry(cells);
-                p->hostBuf.resize(cells);
-                for (size_t k = 0; k < cells; k++)
-                {
-                    p->hostBuf[k] = static_cast<float>(field[k]);
-                    rx[k] = static_cast<float>(ratesX[k]);
-                    ry[k] = static_cast<float>(ratesY[k]);
-                }
-                DeviceArray<float> drx, dry;
-                drx.Allocate(p->ctx, cells);
-                dry.Allocate(p->ctx, cells);
-                Check(hipMemcpyAsync(p->aux[0].get(), p->hostBuf.data(), cells * 4, hipMemcpyHostToDevice,
-                                     p->clock.stream),
-                      "hipMemcpyAsync");
-                Check(hipMemcpyAsync(drx.get(), rx.data(), cells * 4, hipMemcpyHostToDevice, p->clock.stream),
-                      "hipMemcpyAsync");
-                Check(hipMemcpyAsync(dry.get(), ry.data(), cells * 4, hipMemcpyHostToDevice, p->clock.stream),
-                      "hipMemcpyAsync");
-                p->prof.Begin(p->clock.stream, "explicit_rates_delta");
-                hipLaunchKernelGGL(ExplicitRatesDelta, Grid(p), dim3(kBlockY, kBlockX), 0, p->clock.stream, p->X,
-                                   p->Y, wrapX ? 1 : 0, wrapY ? 1 : 0, p->aux[0].get(), drx.get(), dry.get(),
-                                   p->aux[1].get());
-                Check(hipGetLastError(), "ExplicitRatesDelta launch");
-                p->prof.End(p->clock.stream);
-                Check(hipMemcpyAsync(p->hostBuf.data(), p->aux[1].get(), cells * 4, hipMemcpyDeviceToHost,
-                                     p->clock.stream),
-                      "hipMemcpyAsync");
-                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
-                for (size_t k = 0; k < cells; k++)
-                {
-                    deltas[k] += static_cast<double>(p->hostBuf[k]);
-                }
+                const size_t cells = Cells(p);
+                // field -> aux[0], rates -> cellA / cellB (persistent: the
+                // resident field must survive), delta -> aux[1]
+                float *stage = Stage(p, 3 * cells);
+                float *rx = Lazy(p, p->cellA, cells);
+                float *ry = Lazy(p, p->cellB, cells);
+                Lazy(p, p->aux[1], cells);
+                Upload64(p, field, cells, stage, p->aux[0].get());
+                Upload64(p, ratesX, cells, stage + cells, rx);
+                Upload64(p, ratesY, cells, stage + 2 * cells, ry);
+                LaunchRates(p, p->aux[0].get(), rx, ry, p->aux[1].get(), wrapX, wrapY);
+                p->chunks.DownloadAdd(p->aux[1].get(), cells, stage, deltas, nullptr, p->clock.stream);
             });
     }
 
@@ -884,78 +1122,262 @@ extern "C"
                 RequireLength(n, static_cast<int64_t>(Cells(p)), "halhip_diffusion2d_advect_cells_f64");
                 Require(field != nullptr && deltas != nullptr && xVels != nullptr && yVels != nullptr,
                         "halhip_diffusion2d_advect_cells_f64: null array");
-                EdgeValues e;
-                e.From(bc, p->X, p->Y);
-                EdgeValues v;
-                v.From(velBc, p->X, p->Y);
-                Require(!v.dirichlet || e.dirichlet,
-                        "halhip_diffusion2d_advect_cells_f64: velocity edges need field edges (Dirichlet bc)");
                 const size_t cells = Cells(p);
-                // field, x and y velocities, then the edge arrays, in one upload
-                std::vector<float> host(3 * cells);
-                for (size_t k = 0; k < cells; k++)
+                const size_t edges = 4 * (static_cast<size_t>(p->X) + static_cast<size_t>(p->Y));
+                float *stage = Stage(p, 3 * cells + edges);
+                float *xv = Lazy(p, p->cellA, cells);
+                float *yv = Lazy(p, p->cellB, cells);
+                Lazy(p, p->aux[1], cells);
+                const AdvectCellsArgs a = CellAdvectArgs(p, wrapX, wrapY, bc, velBc, stage + 3 * cells);
+                Upload64(p, field, cells, stage, p->aux[0].get());
+                Upload64(p, xVels, cells, stage + cells, xv);
+                Upload64(p, yVels, cells, stage + 2 * cells, yv);
+                LaunchAdvectCells(p, a, p->aux[0].get(), xv, yv, p->aux[1].get());
+                p->chunks.DownloadAdd(p->aux[1].get(), cells, stage, deltas, nullptr, p->clock.stream);
+            });
+    }
+
+    // ---- PDEGrid2D deltas of the resident field ------------------------------------
+
+    int halhip_diffusion2d_delta_point_ops(halhip_diffusion2d *p, int32_t count, const int32_t *ops,
+                                           const int64_t *index, const double *values)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_delta_point_ops");
+                Require(count >= 0, "halhip_diffusion2d_delta_point_ops: negative count");
+                if (count == 0)
                 {
-                    host[k] = static_cast<float>(field[k]);
-                    host[cells + k] = static_cast<float>(xVels[k]);
-                    host[2 * cells + k] = static_cast<float>(yVels[k]);
+                    return;
                 }
-                const size_t edgeLen = 2 * static_cast<size_t>(p->X) + 2 * static_cast<size_t>(p->Y);
-                if (e.dirichlet)
+                Require(ops != nullptr && index != nullptr && values != nullptr,
+                        "halhip_diffusion2d_delta_point_ops: null array");
+                const int64_t cells = static_cast<int64_t>(Cells(p));
+                std::vector<int32_t> op(static_cast<size_t>(count));
+                std::vector<float> val(static_cast<size_t>(count));
+                for (int32_t k = 0; k < count; k++)
                 {
-                    for (const EdgeValues *ev : {&e, &v})
-                    {
-                        if (ev->dirichlet)
-                        {
-                            host.insert(host.end(), ev->xlo.begin(), ev->xlo.end());
-                            host.insert(host.end(), ev->xhi.begin(), ev->xhi.end());
-                            host.insert(host.end(), ev->ylo.begin(), ev->ylo.end());
-                            host.insert(host.end(), ev->yhi.begin(), ev->yhi.end());
-                        }
-                    }
+                    Require(index[k] >= 0 && index[k] < cells, "halhip_diffusion2d_delta_point_ops: index out of range");
+                    Require(ops[k] >= HALHIP_DELTA_SET && ops[k] <= HALHIP_DELTA_SCALE,
+                            "halhip_diffusion2d_delta_point_ops: unknown op");
+                    // Scale(v) adds u (v - 1): a Mul by v - 1, formed in FP64
+                    op[k] = (ops[k] == HALHIP_DELTA_SCALE) ? HALHIP_DELTA_MUL : ops[k];
+                    val[k] = static_cast<float>((ops[k] == HALHIP_DELTA_SCALE) ? values[k] - 1.0 : values[k]);
                 }
-                DeviceArray<float> dev;
-                dev.Allocate(p->ctx, host.size() + cells);
-                Check(hipMemcpyAsync(dev.get(), host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice,
-                                     p->clock.stream),
-                      "hipMemcpyAsync");
-                float *d = dev.get();
-                AdvectCellsArgs a{};
-                a.X = p->X;
-                a.Y = p->Y;
-                a.wrapX = wrapX ? 1 : 0;
-                a.wrapY = wrapY ? 1 : 0;
-                if (e.dirichlet)
+                Deltas(p);
+                // the lists' buffers may still be read by the previous call's kernel
+                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                int32_t *dOp = Lazy(p, p->pointOp, static_cast<size_t>(count));
+                int64_t *dIdx = Lazy(p, p->pointIdx, static_cast<size_t>(count));
+                float *dVal = Lazy(p, p->pointVal, static_cast<size_t>(count));
+                Check(hipMemcpy(dOp, op.data(), op.size() * sizeof(int32_t), hipMemcpyHostToDevice), "hipMemcpy");
+                Check(hipMemcpy(dIdx, index, static_cast<size_t>(count) * sizeof(int64_t), hipMemcpyHostToDevice),
+                      "hipMemcpy");
+                Check(hipMemcpy(dVal, val.data(), val.size() * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+                hipLaunchKernelGGL(DeltaPointOps, dim3(1), dim3(64), 0, p->clock.stream, count,
+                                   static_cast<const int32_t *>(dOp), static_cast<const int64_t *>(dIdx),
+                                   static_cast<const float *>(dVal), static_cast<const float *>(p->field[p->cur].get()),
+                                   p->deltas.get());
+                Check(hipGetLastError(), "DeltaPointOps launch");
+            });
+    }
+
+    int halhip_diffusion2d_delta_all(halhip_diffusion2d *p, int32_t op, double value)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_delta_all");
+                Require(std::isfinite(value), "halhip_diffusion2d_delta_all: value must be finite");
+                switch (op)
                 {
-                    float *eb = d + 3 * cells;
-                    a.bcXlo = eb;
-                    a.bcXhi = eb + p->Y;
-                    a.bcYlo = eb + 2 * p->Y;
-                    a.bcYhi = eb + 2 * p->Y + p->X;
-                    if (v.dirichlet)
-                    {
-                        float *vb = eb + edgeLen;
-                        a.vXlo = vb;
-                        a.vXhi = vb + p->Y;
-                        a.vYlo = vb + 2 * p->Y;
-                        a.vYhi = vb + 2 * p->Y + p->X;
-                    }
+                    case HALHIP_DELTA_SET:
+                        LaunchFill(p, 0, static_cast<float>(value), nullptr);
+                        break;
+                    case HALHIP_DELTA_ADD:
+                        LaunchFill(p, 1, static_cast<float>(value), nullptr);
+                        break;
+                    case HALHIP_DELTA_MUL:
+                        LaunchFill(p, 2, static_cast<float>(value), nullptr);
+                        break;
+                    case HALHIP_DELTA_SCALE:
+                        LaunchFill(p, 2, static_cast<float>(value - 1.0), nullptr);
+                        break;
+                    case HALHIP_DELTA_SET_NON_NEGATIVE:
+                        LaunchFill(p, 5, 0.0f, nullptr);
+                        break;
+                    default:
+                        Fail(HALHIP_ERR_INVALID_ARGUMENT, "halhip_diffusion2d_delta_all: unknown op");
                 }
-                float *out = d + host.size();
-                p->prof.Begin(p->clock.stream, "advect_cells_delta");
-                hipLaunchKernelGGL(AdvectCellsDelta, Grid(p), dim3(kBlockY, kBlockX), 0, p->clock.stream, a,
-                                   static_cast<const float *>(d), static_cast<const float *>(d + cells),
-                                   static_cast<const float *>(d + 2 * cells), out);
-                Check(hipGetLastError(), "AdvectCellsDelta launch");
+            });
+    }
+
+    int halhip_diffusion2d_delta_set_values(halhip_diffusion2d *p, const double *values, int64_t n)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_delta_set_values");
+                RequireLength(n, static_cast<int64_t>(Cells(p)), "halhip_diffusion2d_delta_set_values");
+                Require(values != nullptr, "halhip_diffusion2d_delta_set_values: null array");
+                const size_t cells = Cells(p);
+                float *stage = Stage(p, cells);
+                float *v = Lazy(p, p->cellA, cells);
+                Upload64(p, values, cells, stage, v);
+                LaunchFill(p, 3, 0.0f, v);
+            });
+    }
+
+    int halhip_diffusion2d_delta_diffuse(halhip_diffusion2d *p, double diffRate, int32_t wrapX, int32_t wrapY,
+                                         const halhip_bc *bc)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_delta_diffuse");
+                Require(std::isfinite(diffRate), "halhip_diffusion2d_delta_diffuse: rate must be finite");
+                ExplicitArgs a = MakeArgs(p, diffRate, 0.0, wrapX, wrapY, bc);
+                Lazy(p, p->aux[1], Cells(p));
+                p->prof.Begin(p->clock.stream, "explicit_delta");
+                hipLaunchKernelGGL(ExplicitStep<false>, Grid(p), dim3(kBlockY, kBlockX), 0, p->clock.stream, a,
+                                   static_cast<const float *>(p->field[p->cur].get()), p->aux[1].get());
+                Check(hipGetLastError(), "ExplicitStep launch");
                 p->prof.End(p->clock.stream);
-                p->hostBuf.resize(cells);
-                Check(hipMemcpyAsync(p->hostBuf.data(), out, cells * sizeof(float), hipMemcpyDeviceToHost,
+                AddScratch(p);
+            });
+    }
+
+    int halhip_diffusion2d_delta_advect(halhip_diffusion2d *p, double xVel, double yVel, int32_t wrapX, int32_t wrapY,
+                                        const halhip_bc *bc)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_delta_advect");
+                Require(std::isfinite(xVel) && std::isfinite(yVel), "halhip_diffusion2d_delta_advect: bad velocity");
+                // rate 0: the kernel's delta is the advection term alone
+                ExplicitArgs a = MakeArgs(p, 0.0, 0.0, wrapX, wrapY, bc, xVel, yVel);
+                Lazy(p, p->aux[1], Cells(p));
+                p->prof.Begin(p->clock.stream, "advect_delta");
+                hipLaunchKernelGGL(ExplicitStep<false>, Grid(p), dim3(kBlockY, kBlockX), 0, p->clock.stream, a,
+                                   static_cast<const float *>(p->field[p->cur].get()), p->aux[1].get());
+                Check(hipGetLastError(), "ExplicitStep launch");
+                p->prof.End(p->clock.stream);
+                AddScratch(p);
+            });
+    }
+
+    int halhip_diffusion2d_delta_diffuse_rates(halhip_diffusion2d *p, const double *ratesX, const double *ratesY,
+                                               int64_t n, int32_t wrapX, int32_t wrapY)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_delta_diffuse_rates");
+                RequireLength(n, static_cast<int64_t>(Cells(p)), "halhip_diffusion2d_delta_diffuse_rates");
+                Require(ratesX != nullptr && ratesY != nullptr, "halhip_diffusion2d_delta_diffuse_rates: null array");
+                const size_t cells = Cells(p);
+                float *stage = Stage(p, 2 * cells);
+                float *rx = Lazy(p, p->cellA, cells);
+                float *ry = Lazy(p, p->cellB, cells);
+                Lazy(p, p->aux[1], cells);
+                Upload64(p, ratesX, cells, stage, rx);
+                Upload64(p, ratesY, cells, stage + cells, ry);
+                LaunchRates(p, p->field[p->cur].get(), rx, ry, p->aux[1].get(), wrapX, wrapY);
+                AddScratch(p);
+            });
+    }
+
+    int halhip_diffusion2d_delta_advect_cells(halhip_diffusion2d *p, const double *xVels, const double *yVels,
+                                              int64_t n, int32_t wrapX, int32_t wrapY, const halhip_bc *bc,
+                                              const halhip_bc *velBc)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_delta_advect_cells");
+                RequireLength(n, static_cast<int64_t>(Cells(p)), "halhip_diffusion2d_delta_advect_cells");
+                Require(xVels != nullptr && yVels != nullptr, "halhip_diffusion2d_delta_advect_cells: null array");
+                const size_t cells = Cells(p);
+                const size_t edges = 4 * (static_cast<size_t>(p->X) + static_cast<size_t>(p->Y));
+                float *stage = Stage(p, 2 * cells + edges);
+                float *xv = Lazy(p, p->cellA, cells);
+                float *yv = Lazy(p, p->cellB, cells);
+                Lazy(p, p->aux[1], cells);
+                const AdvectCellsArgs a = CellAdvectArgs(p, wrapX, wrapY, bc, velBc, stage + 2 * cells);
+                Upload64(p, xVels, cells, stage, xv);
+                Upload64(p, yVels, cells, stage + cells, yv);
+                LaunchAdvectCells(p, a, p->field[p->cur].get(), xv, yv, p->aux[1].get());
+                AddScratch(p);
+            });
+    }
+
+    int halhip_diffusion2d_delta_adi(halhip_diffusion2d *p, halhip_adi2d *adi, double diffRate, int32_t wrapX,
+                                     int32_t wrapY, const halhip_bc *bc)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_delta_adi");
+                AdiDeltaOf(adi, p->field[p->cur].get(), Deltas(p), p->X, p->Y, diffRate, wrapX, wrapY, bc,
+                           p->clock.stream);
+            });
+    }
+
+    int halhip_diffusion2d_delta_reduce(halhip_diffusion2d *p, int32_t what, double offset, double *out)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_delta_reduce");
+                Require(out != nullptr, "halhip_diffusion2d_delta_reduce: null output");
+                Require(what >= HALHIP_REDUCE_MAX_DELTA && what <= HALHIP_REDUCE_FIELD_SUM,
+                        "halhip_diffusion2d_delta_reduce: unknown reduction");
+                Require(std::isfinite(offset), "halhip_diffusion2d_delta_reduce: offset must be finite");
+                double *part = Lazy(p, p->partial, kDeltaBlocks);
+                hipLaunchKernelGGL(DeltaReduce, dim3(kDeltaBlocks), dim3(256), 0, p->clock.stream, what,
+                                   static_cast<float>(offset), static_cast<const float *>(p->field[p->cur].get()),
+                                   static_cast<const float *>(Deltas(p)), static_cast<int64_t>(Cells(p)), part);
+                Check(hipGetLastError(), "DeltaReduce launch");
+                std::vector<double> host(kDeltaBlocks);
+                Check(hipMemcpyAsync(host.data(), part, kDeltaBlocks * sizeof(double), hipMemcpyDeviceToHost,
                                      p->clock.stream),
                       "hipMemcpyAsync");
                 Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
-                for (size_t k = 0; k < cells; k++)
+                double r = host[0];
+                for (size_t k = 1; k < host.size(); k++)
                 {
-                    deltas[k] += static_cast<double>(p->hostBuf[k]);
+                    r = (what == HALHIP_REDUCE_FIELD_SUM) ? r + host[k]
+                                                          : ((what == HALHIP_REDUCE_FIELD_MIN) ? std::min(r, host[k])
+                                                                                               : std::max(r, host[k]));
                 }
+                *out = r;
+            });
+    }
+
+    int halhip_diffusion2d_delta_update(halhip_diffusion2d *p)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_delta_update");
+                LaunchFill(p, 4, 0.0f, nullptr);
+            });
+    }
+
+    int halhip_diffusion2d_download_deltas(halhip_diffusion2d *p, float *deltas, int64_t n)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_diffusion2d_download_deltas");
+                RequireLength(n, static_cast<int64_t>(Cells(p)), "halhip_diffusion2d_download_deltas");
+                Require(deltas != nullptr, "halhip_diffusion2d_download_deltas: null array");
+                Check(hipMemcpyAsync(deltas, Deltas(p), Cells(p) * sizeof(float), hipMemcpyDeviceToHost,
+                                     p->clock.stream),
+                      "hipMemcpyAsync");
+                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
             });
     }
 
@@ -1015,6 +1437,10 @@ extern "C"
                 Check(hipMemcpyAsync(p->field[p->cur].get(), field, Cells(p) * sizeof(float), hipMemcpyHostToDevice,
                                      p->clock.stream),
                       "hipMemcpyAsync");
+                if (p->deltas.get() != nullptr)
+                {
+                    LaunchFill(p, 6, 0.0f, nullptr); // a new committed field has no pending deltas
+                }
                 Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
             });
     }

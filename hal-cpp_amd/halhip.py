@@ -21,6 +21,16 @@ ERR_UNSUPPORTED = 5
 BC_NONE = 0
 BC_DIRICHLET = 1
 ADI_MAX_Y = 8192
+DELTA_SET = 0
+DELTA_ADD = 1
+DELTA_MUL = 2
+DELTA_SCALE = 3
+DELTA_SET_NON_NEGATIVE = 4
+REDUCE_MAX_DELTA = 0
+REDUCE_MAX_DELTA_SCALED = 1
+REDUCE_FIELD_MAX = 2
+REDUCE_FIELD_MIN = 3
+REDUCE_FIELD_SUM = 4
 ADI_AUTO = 0
 ADI_ROWS_WORKGROUP_SCAN = 1
 ADI_ROWS_LDS = 2
@@ -89,6 +99,18 @@ _SIGS = {
     "halhip_diffusion2d_run_steps_async": [_vp, _i32, _f64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
     "halhip_diffusion2d_sync": [_vp, _fp],
     "halhip_diffusion2d_prepare_steps": [_vp, _i32, _f64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
+    "halhip_diffusion2d_delta_point_ops": [_vp, _i32, _ip, _lp, _dp],
+    "halhip_diffusion2d_delta_all": [_vp, _i32, _f64],
+    "halhip_diffusion2d_delta_set_values": [_vp, _dp, _i64],
+    "halhip_diffusion2d_delta_diffuse": [_vp, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
+    "halhip_diffusion2d_delta_diffuse_rates": [_vp, _dp, _dp, _i64, _i32, _i32],
+    "halhip_diffusion2d_delta_advect": [_vp, _f64, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
+    "halhip_diffusion2d_delta_advect_cells": [_vp, _dp, _dp, _i64, _i32, _i32, ctypes.POINTER(HalBC),
+                                              ctypes.POINTER(HalBC)],
+    "halhip_diffusion2d_delta_adi": [_vp, _vp, _f64, _i32, _i32, ctypes.POINTER(HalBC)],
+    "halhip_diffusion2d_delta_reduce": [_vp, _i32, _f64, _dp],
+    "halhip_diffusion2d_delta_update": [_vp],
+    "halhip_diffusion2d_download_deltas": [_vp, _fp, _i64],
     "halhip_diffusion2d_set_profiling": [_vp, _i32],
     "halhip_diffusion2d_profile": [_vp, _i32, _ip, ctypes.c_char_p, _i32, _dp, _lp],
     "halhip_adi2d_create": [_vp, _i32, _i32, ctypes.POINTER(_vp)],
@@ -432,6 +454,59 @@ class Diffusion2D(_Plan):
         """Build the graphs run_steps(steps, ...) replays; runs no tick."""
         s, keep = make_bc(bc, self.X, self.Y)
         _check(self._fn("prepare_steps")(self.h, steps, rate, decay, int(wrapX), int(wrapY), _bcp(s)))
+
+
+    # PDEGrid2D delta calls on the resident field (include/Hal/Grids/PDEGrid2D.h:89-399)
+    def delta_point_ops(self, ops, index, values):
+        ops = np.ascontiguousarray(ops, np.int32)
+        index = np.ascontiguousarray(index, np.int64)
+        values = _f64(values)
+        _check(self._fn("delta_point_ops")(self.h, ops.size, ops.ctypes.data_as(_ip), index.ctypes.data_as(_lp),
+                                           values.ctypes.data_as(_dp)))
+
+    def delta_all(self, op, value=0.0):
+        _check(self._fn("delta_all")(self.h, op, value))
+
+    def delta_set_values(self, values):
+        values = _f64(values)
+        _check(self._fn("delta_set_values")(self.h, values.ctypes.data_as(_dp), values.size))
+
+    def delta_diffuse(self, rate, wrapX, wrapY, bc=None):
+        s, keep = make_bc(bc, self.X, self.Y)
+        _check(self._fn("delta_diffuse")(self.h, rate, int(wrapX), int(wrapY), _bcp(s)))
+
+    def delta_diffuse_rates(self, ratesX, ratesY, wrapX, wrapY):
+        rx, ry = _f64(ratesX), _f64(ratesY)
+        _check(self._fn("delta_diffuse_rates")(self.h, rx.ctypes.data_as(_dp), ry.ctypes.data_as(_dp), rx.size,
+                                               int(wrapX), int(wrapY)))
+
+    def delta_advect(self, xVel, yVel, wrapX, wrapY, bc=None):
+        s, keep = make_bc(bc, self.X, self.Y)
+        _check(self._fn("delta_advect")(self.h, xVel, yVel, int(wrapX), int(wrapY), _bcp(s)))
+
+    def delta_advect_cells(self, xVels, yVels, wrapX, wrapY, bc=None, vel_bc=None):
+        xv, yv = _f64(xVels), _f64(yVels)
+        s, keep = make_bc(bc, self.X, self.Y)
+        v, keepv = make_bc(vel_bc, self.X, self.Y)
+        _check(self._fn("delta_advect_cells")(self.h, xv.ctypes.data_as(_dp), yv.ctypes.data_as(_dp), xv.size,
+                                              int(wrapX), int(wrapY), _bcp(s), _bcp(v)))
+
+    def delta_adi(self, adi, rate, wrapX, wrapY, bc=None):
+        s, keep = make_bc(bc, self.X, self.Y)
+        _check(self._fn("delta_adi")(self.h, adi.h, rate, int(wrapX), int(wrapY), _bcp(s)))
+
+    def delta_reduce(self, what, offset=0.0):
+        out = ctypes.c_double(0.0)
+        _check(self._fn("delta_reduce")(self.h, what, offset, ctypes.byref(out)))
+        return out.value
+
+    def delta_update(self):
+        _check(self._fn("delta_update")(self.h))
+
+    def deltas(self):
+        out = np.empty(self.X * self.Y, np.float32)
+        _check(self._fn("download_deltas")(self.h, out.ctypes.data_as(_fp), out.size))
+        return out
 
 
 class Adi2D(_Plan):

@@ -195,6 +195,54 @@ int halhip_diffusion2d_sync(halhip_diffusion2d *plan, float *elapsedMs);
  * records its command buffer per call, src/Gpu/DiffusionKernel2D.cpp:244-304). */
 int halhip_diffusion2d_prepare_steps(halhip_diffusion2d *plan, int32_t steps, double diffRate, double decay,
                                      int32_t wrapX, int32_t wrapY, const halhip_bc *bc);
+/* ---- PDEGrid2D deltas on the resident field ----
+ * Hal::PDEGrid2D's delta calls (include/Hal/Grids/PDEGrid2D.h:89-245,
+ * 289-399) applied to the plan's resident (committed) field u: each call
+ * changes the device deltas d, and delta_update commits u += d, d = 0
+ * (PDEGrid2D::Update, :178-186).  d starts at zero and upload clears it.
+ * FP32 on the device.  Point ops run in list order:
+ *   HALHIP_DELTA_SET    d[i] = v - u[i]        (Set)
+ *   HALHIP_DELTA_ADD    d[i] += v              (Add)
+ *   HALHIP_DELTA_MUL    d[i] += u[i] * v       (Mul)
+ *   HALHIP_DELTA_SCALE  d[i] += u[i] * (v - 1) (Scale)
+ * delta_all applies one of them to every cell (SetAll, AddAll, MulAll, a
+ * field-wide Scale) or HALHIP_DELTA_SET_NON_NEGATIVE (SetNonNegative:
+ * d[i] = -u[i] where u[i] + d[i] < 0); delta_set_values is SetAll(pVals).
+ * The stencil calls add the interface deltas of the committed field:
+ * Diffusion (with bc), Diffusion(ratesX, ratesY), Advection (uniform, with
+ * bc), Advection(xVels, yVels[, bc, velBc]) and, through an ADI plan of the
+ * same dimensions (whose resident field it replaces), DiffusionADI.
+ * delta_reduce: MaxDelta, MaxDeltaScaled(offset), GetMax, GetMin and the
+ * field's sum (GetAvg * n). */
+#define HALHIP_DELTA_SET 0
+#define HALHIP_DELTA_ADD 1
+#define HALHIP_DELTA_MUL 2
+#define HALHIP_DELTA_SCALE 3
+#define HALHIP_DELTA_SET_NON_NEGATIVE 4
+#define HALHIP_REDUCE_MAX_DELTA 0
+#define HALHIP_REDUCE_MAX_DELTA_SCALED 1
+#define HALHIP_REDUCE_FIELD_MAX 2
+#define HALHIP_REDUCE_FIELD_MIN 3
+#define HALHIP_REDUCE_FIELD_SUM 4
+int halhip_diffusion2d_delta_point_ops(halhip_diffusion2d *plan, int32_t count, const int32_t *ops,
+                                       const int64_t *index, const double *values);
+int halhip_diffusion2d_delta_all(halhip_diffusion2d *plan, int32_t op, double value);
+int halhip_diffusion2d_delta_set_values(halhip_diffusion2d *plan, const double *values, int64_t n);
+int halhip_diffusion2d_delta_diffuse(halhip_diffusion2d *plan, double diffRate, int32_t wrapX, int32_t wrapY,
+                                     const halhip_bc *bc);
+int halhip_diffusion2d_delta_diffuse_rates(halhip_diffusion2d *plan, const double *ratesX, const double *ratesY,
+                                           int64_t n, int32_t wrapX, int32_t wrapY);
+int halhip_diffusion2d_delta_advect(halhip_diffusion2d *plan, double xVel, double yVel, int32_t wrapX,
+                                    int32_t wrapY, const halhip_bc *bc);
+int halhip_diffusion2d_delta_advect_cells(halhip_diffusion2d *plan, const double *xVels, const double *yVels,
+                                          int64_t n, int32_t wrapX, int32_t wrapY, const halhip_bc *bc,
+                                          const halhip_bc *velBc);
+int halhip_diffusion2d_delta_adi(halhip_diffusion2d *plan, halhip_adi2d *adi, double diffRate, int32_t wrapX,
+                                 int32_t wrapY, const halhip_bc *bc);
+int halhip_diffusion2d_delta_reduce(halhip_diffusion2d *plan, int32_t what, double offset, double *out);
+int halhip_diffusion2d_delta_update(halhip_diffusion2d *plan);
+int halhip_diffusion2d_download_deltas(halhip_diffusion2d *plan, float *deltas, int64_t n);
+
 int halhip_diffusion2d_set_profiling(halhip_diffusion2d *plan, int32_t on);
 int halhip_diffusion2d_profile(halhip_diffusion2d *plan, int32_t maxKernels, int32_t *nKernels, char *names,
                                int32_t namesLen, double *totalMs, int64_t *launches);
