@@ -276,11 +276,57 @@ namespace Hal
 
         // ---- device-resident PDEGrid2D ----------------------------------------------------
 
+        namespace
+        {
+            // halhip_bc over a recorded boundary (constant or edge arrays).
+            struct BcView
+            {
+                halhip_bc bc{};
+                const halhip_bc *Ptr() const { return bc.kind == HALHIP_BC_DIRICHLET ? &bc : nullptr; }
+            };
+
+            BcView ViewOf(bool has, bool constant, double value, const DirichletBoundary *edges)
+            {
+                BcView v;
+                if (!has)
+                {
+                    return v;
+                }
+                v.bc.kind = HALHIP_BC_DIRICHLET;
+                v.bc.value = value;
+                if (!constant && edges != nullptr)
+                {
+                    v.bc.xlo = edges->xlo.data();
+                    v.bc.xhi = edges->xhi.data();
+                    v.bc.ylo = edges->ylo.data();
+                    v.bc.yhi = edges->yhi.data();
+                }
+                return v;
+            }
+
+            // PDEGrid2D::CheckDiffusionStable / CheckAdvectionStable
+            // (include/Hal/Grids/PDEGrid2D.h:436-451)
+            void CheckDiffusionRate(double diffCoef, const char *caller)
+            {
+                if (diffCoef > 0.25)
+                {
+                    Throw(true, std::string(caller) + ": Diffusion rate above stable maximum value of 0.25");
+                }
+            }
+
+            void CheckAdvectionRate(double xVel, double yVel, const char *caller)
+            {
+                if (std::fabs(xVel) + std::fabs(yVel) > 0.5)
+                {
+                    Throw(true, std::string(caller) + ": Advection rate component sum above stable maximum value of 0.5");
+                }
+            }
+        }
+
         HipPDEGrid2D::HipPDEGrid2D(HipContext &context, int32_t xDim, int32_t yDim, bool wrapX, bool wrapY)
             : m_context(context), m_xDim(xDim), m_yDim(yDim), m_wrapX(wrapX), m_wrapY(wrapY), m_updateCt(0),
-              m_explicit(nullptr), m_adi(nullptr), m_resident(Stencil::Explicit), m_stencil(Stencil::None),
-              m_rate(0.0), m_advect(false), m_xVel(0.0), m_yVel(0.0), m_hasBc(false), m_bc(0.0), m_bcSet(false),
-              m_mulAll(0.0), m_explicitVel{0.0, 0.0}
+              m_explicit(nullptr), m_adi(nullptr), m_resident(Stencil::Explicit), m_general(false),
+              m_explicitVel{0.0, 0.0}
         {
             ThrowIfFailed(halhip_diffusion2d_create(context.Handle(), xDim, yDim, &m_explicit), "HipPDEGrid2D");
         }
@@ -291,16 +337,22 @@ namespace Hal
             halhip_diffusion2d_destroy(m_explicit);
         }
 
+        halhip_adi2d *HipPDEGrid2D::Adi()
+        {
+            if (m_adi == nullptr)
+            {
+                ThrowIfFailed(halhip_adi2d_create(m_context.Handle(), m_xDim, m_yDim, &m_adi), "HipPDEGrid2D::DiffusionADI");
+            }
+            return m_adi;
+        }
+
         void HipPDEGrid2D::MoveFieldTo(Stencil s)
         {
             if (s == m_resident)
             {
                 return;
             }
-            if (s == Stencil::Adi && m_adi == nullptr)
-            {
-                ThrowIfFailed(halhip_adi2d_create(m_context.Handle(), m_xDim, m_yDim, &m_adi), "HipPDEGrid2D::DiffusionADI");
-            }
+            Adi();
             const int64_t n = static_cast<int64_t>(m_xDim) * m_yDim;
             std::vector<float> host(static_cast<size_t>(n));
             if (m_resident == Stencil::Adi)
@@ -314,6 +366,15 @@ namespace Hal
                 ThrowIfFailed(halhip_adi2d_upload(m_adi, host.data(), n), "HipPDEGrid2D");
             }
             m_resident = s;
+        }
+
+        void HipPDEGrid2D::ClearTick()
+        {
+            m_ops.clear();
+            m_pointOps.clear();
+            m_pointIdx.clear();
+            m_pointVal.clear();
+            m_general = false;
         }
 
         void HipPDEGrid2D::SetField(const std::vector<double> &field)
@@ -331,8 +392,9 @@ namespace Hal
             }
             else
             {
-                ThrowIfFailed(halhip_diffusion2d_upload(m_explicit, host.data(), n), caller);
+                ThrowIfFailed(halhip_diffusion2d_upload(m_explicit, host.data(), n), caller); // clears the deltas
             }
+            ClearTick();
         }
 
         std::vector<double> HipPDEGrid2D::GetField()
@@ -353,188 +415,525 @@ namespace Hal
 
         double HipPDEGrid2D::Get(int32_t i)
         {
-            if (i < 0 || static_cast<int64_t>(i) >= static_cast<int64_t>(m_xDim) * m_yDim)
-            {
-                Throw(true, "HipPDEGrid2D::Get: index out of range");
-            }
+            CheckIndex(i, "HipPDEGrid2D::Get");
             return GetField()[static_cast<size_t>(i)];
         }
 
-        void HipPDEGrid2D::SetBoundary(bool has, double value, const char *caller)
+        void HipPDEGrid2D::CheckIndex(int32_t i, const char *caller) const
         {
-            if (m_bcSet && (has != m_hasBc || (has && value != m_bc)))
+            if (i < 0 || static_cast<int64_t>(i) >= static_cast<int64_t>(m_xDim) * m_yDim)
             {
-                Throw(true, std::string(caller) + ": the calls of one tick must share their boundary value");
+                Throw(true, std::string(caller) + ": index out of range");
             }
-            m_bcSet = true;
-            m_hasBc = has;
-            m_bc = value;
         }
 
-        void HipPDEGrid2D::SetStencil(Stencil s, double rate, const char *caller)
+        // The committed field's reductions and the pending deltas' need both
+        // on the explicit plan; the deltas are materialised first.
+        double HipPDEGrid2D::Reduce(int32_t what, double offset, const char *caller)
         {
-            if (m_stencil != Stencil::None)
+            if (what == HALHIP_REDUCE_MAX_DELTA || what == HALHIP_REDUCE_MAX_DELTA_SCALED)
             {
-                Throw(true, std::string(caller) + ": one diffusion call per tick on the device grid");
+                Materialize();
+                FlushPoints();
             }
-            if (s == Stencil::Adi && m_advect)
+            else if (m_resident != Stencil::Explicit)
             {
-                Throw(true, std::string(caller) + ": advection is fused with explicit diffusion only");
+                // the pending calls were recorded against the committed field,
+                // which is the same wherever it lives
+                MoveFieldTo(Stencil::Explicit);
             }
-            m_stencil = s;
-            m_rate = rate;
+            double out = 0.0;
+            ThrowIfFailed(halhip_diffusion2d_delta_reduce(m_explicit, what, offset, &out), caller);
+            return out;
         }
 
-        void HipPDEGrid2D::Diffusion(double diffCoef)
+        double HipPDEGrid2D::GetMax()
         {
-            // PDEGrid2D::CheckDiffusionStable (include/Hal/Grids/PDEGrid2D.h:436-442)
-            if (diffCoef > 0.25)
-            {
-                Throw(true, "HipPDEGrid2D::Diffusion: Diffusion rate above stable maximum value of 0.25");
-            }
-            SetBoundary(false, 0.0, "HipPDEGrid2D::Diffusion");
-            SetStencil(Stencil::Explicit, diffCoef, "HipPDEGrid2D::Diffusion");
+            return Reduce(HALHIP_REDUCE_FIELD_MAX, 0.0, "HipPDEGrid2D::GetMax");
         }
 
-        void HipPDEGrid2D::Diffusion(double diffCoef, double boundaryValue)
+        double HipPDEGrid2D::GetMin()
         {
-            if (diffCoef > 0.25)
-            {
-                Throw(true, "HipPDEGrid2D::Diffusion: Diffusion rate above stable maximum value of 0.25");
-            }
-            SetBoundary(true, boundaryValue, "HipPDEGrid2D::Diffusion");
-            SetStencil(Stencil::Explicit, diffCoef, "HipPDEGrid2D::Diffusion");
+            return Reduce(HALHIP_REDUCE_FIELD_MIN, 0.0, "HipPDEGrid2D::GetMin");
         }
 
-        void HipPDEGrid2D::DiffusionADI(double diffRate)
+        double HipPDEGrid2D::GetAvg()
         {
-            SetBoundary(false, 0.0, "HipPDEGrid2D::DiffusionADI");
-            SetStencil(Stencil::Adi, diffRate, "HipPDEGrid2D::DiffusionADI");
+            return Reduce(HALHIP_REDUCE_FIELD_SUM, 0.0, "HipPDEGrid2D::GetAvg") / Length();
         }
 
-        void HipPDEGrid2D::DiffusionADI(double diffRate, double boundaryValue)
+        double HipPDEGrid2D::MaxDelta()
         {
-            SetBoundary(true, boundaryValue, "HipPDEGrid2D::DiffusionADI");
-            SetStencil(Stencil::Adi, diffRate, "HipPDEGrid2D::DiffusionADI");
+            return Reduce(HALHIP_REDUCE_MAX_DELTA, 0.0, "HipPDEGrid2D::MaxDelta");
         }
 
-        void HipPDEGrid2D::Advection(double xVel, double yVel)
+        double HipPDEGrid2D::MaxDeltaScaled(double denomOffset)
         {
-            const char *caller = "HipPDEGrid2D::Advection";
-            if (std::fabs(xVel) + std::fabs(yVel) > 0.5)
-            {
-                Throw(true, std::string(caller) + ": Advection rate component sum above stable maximum value of 0.5");
-            }
-            if (m_stencil == Stencil::Adi || m_advect)
-            {
-                Throw(true, std::string(caller) + ": one advection call per tick, fused with explicit diffusion only");
-            }
-            SetBoundary(false, 0.0, caller);
-            m_advect = true;
-            m_xVel = xVel;
-            m_yVel = yVel;
+            return Reduce(HALHIP_REDUCE_MAX_DELTA_SCALED, denomOffset, "HipPDEGrid2D::MaxDeltaScaled");
         }
 
-        void HipPDEGrid2D::Advection(double xVel, double yVel, double boundaryValue)
+        void HipPDEGrid2D::Record(Op op)
         {
-            const char *caller = "HipPDEGrid2D::Advection";
-            if (std::fabs(xVel) + std::fabs(yVel) > 0.5)
+            if (m_general)
             {
-                Throw(true, std::string(caller) + ": Advection rate component sum above stable maximum value of 0.5");
+                Issue(op);
+                return;
             }
-            if (m_stencil == Stencil::Adi || m_advect)
-            {
-                Throw(true, std::string(caller) + ": one advection call per tick, fused with explicit diffusion only");
-            }
-            SetBoundary(true, boundaryValue, caller);
-            m_advect = true;
-            m_xVel = xVel;
-            m_yVel = yVel;
+            m_ops.push_back(std::move(op));
         }
 
-        void HipPDEGrid2D::MulAll(double val)
+        void HipPDEGrid2D::Set(int32_t i, double val)
         {
-            m_mulAll += val;
+            CheckIndex(i, "HipPDEGrid2D::Set");
+            Op op(Op::Kind::Point);
+            op.code = HALHIP_DELTA_SET;
+            op.index = i;
+            op.a = val;
+            Record(std::move(op));
         }
 
         void HipPDEGrid2D::Add(int32_t i, double val)
         {
-            if (i < 0 || static_cast<int64_t>(i) >= static_cast<int64_t>(m_xDim) * m_yDim)
+            CheckIndex(i, "HipPDEGrid2D::Add");
+            Op op(Op::Kind::Point);
+            op.code = HALHIP_DELTA_ADD;
+            op.index = i;
+            op.a = val;
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::Mul(int32_t i, double val)
+        {
+            CheckIndex(i, "HipPDEGrid2D::Mul");
+            Op op(Op::Kind::Point);
+            op.code = HALHIP_DELTA_MUL;
+            op.index = i;
+            op.a = val;
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::Scale(int32_t i, double val)
+        {
+            CheckIndex(i, "HipPDEGrid2D::Scale");
+            Op op(Op::Kind::Point);
+            op.code = HALHIP_DELTA_SCALE;
+            op.index = i;
+            op.a = val;
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::SetAll(double val)
+        {
+            Op op(Op::Kind::All);
+            op.code = HALHIP_DELTA_SET;
+            op.a = val;
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::SetAll(const double *pVals)
+        {
+            if (pVals == nullptr)
             {
-                Throw(true, "HipPDEGrid2D::Add: index out of range");
+                Throw(true, "HipPDEGrid2D::SetAll: null values");
             }
-            m_adds.emplace_back(static_cast<int64_t>(i), static_cast<float>(val));
+            Op op(Op::Kind::SetValues);
+            op.u = std::make_shared<const std::vector<double>>(pVals, pVals + Length());
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::AddAll(double val)
+        {
+            Op op(Op::Kind::All);
+            op.code = HALHIP_DELTA_ADD;
+            op.a = val;
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::MulAll(double val)
+        {
+            Op op(Op::Kind::All);
+            op.code = HALHIP_DELTA_MUL;
+            op.a = val;
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::SetNonNegative()
+        {
+            Op op(Op::Kind::All);
+            op.code = HALHIP_DELTA_SET_NON_NEGATIVE;
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::Diffusion(double diffCoef)
+        {
+            CheckDiffusionRate(diffCoef, "HipPDEGrid2D::Diffusion");
+            Op op(Op::Kind::Diffusion);
+            op.a = diffCoef;
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::Diffusion(double diffCoef, double boundaryValue)
+        {
+            CheckDiffusionRate(diffCoef, "HipPDEGrid2D::Diffusion");
+            Op op(Op::Kind::Diffusion);
+            op.a = diffCoef;
+            op.hasBc = true;
+            op.constBc = true;
+            op.bcValue = boundaryValue;
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::DiffusionEdges(double diffCoef, const DirichletBoundary &bc)
+        {
+            CheckDiffusionRate(diffCoef, "HipPDEGrid2D::Diffusion");
+            Op op(Op::Kind::Diffusion);
+            op.a = diffCoef;
+            op.hasBc = true;
+            op.bc = std::make_shared<const DirichletBoundary>(bc);
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::Diffusion(const std::vector<double> &diffRatesX, const std::vector<double> &diffRatesY)
+        {
+            if (static_cast<int64_t>(diffRatesX.size()) != Length() || static_cast<int64_t>(diffRatesY.size()) != Length())
+            {
+                Throw(true, "HipPDEGrid2D::Diffusion: rates must hold xDim*yDim values");
+            }
+            Op op(Op::Kind::DiffusionRates);
+            op.u = std::make_shared<const std::vector<double>>(diffRatesX);
+            op.v = std::make_shared<const std::vector<double>>(diffRatesY);
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::DiffusionADI(double diffRate)
+        {
+            Op op(Op::Kind::Adi);
+            op.a = diffRate;
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::DiffusionADI(double diffRate, double boundaryValue)
+        {
+            Op op(Op::Kind::Adi);
+            op.a = diffRate;
+            op.hasBc = true;
+            op.constBc = true;
+            op.bcValue = boundaryValue;
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::DiffusionADIEdges(double diffRate, const DirichletBoundary &bc)
+        {
+            Op op(Op::Kind::Adi);
+            op.a = diffRate;
+            op.hasBc = true;
+            op.bc = std::make_shared<const DirichletBoundary>(bc);
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::Advection(double xVel, double yVel)
+        {
+            CheckAdvectionRate(xVel, yVel, "HipPDEGrid2D::Advection");
+            Op op(Op::Kind::Advection);
+            op.a = xVel;
+            op.b = yVel;
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::Advection(double xVel, double yVel, double boundaryValue)
+        {
+            CheckAdvectionRate(xVel, yVel, "HipPDEGrid2D::Advection");
+            Op op(Op::Kind::Advection);
+            op.a = xVel;
+            op.b = yVel;
+            op.hasBc = true;
+            op.constBc = true;
+            op.bcValue = boundaryValue;
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::AdvectionEdges(double xVel, double yVel, const DirichletBoundary &bc)
+        {
+            CheckAdvectionRate(xVel, yVel, "HipPDEGrid2D::Advection");
+            Op op(Op::Kind::Advection);
+            op.a = xVel;
+            op.b = yVel;
+            op.hasBc = true;
+            op.bc = std::make_shared<const DirichletBoundary>(bc);
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::Advection(const std::vector<double> &xVels, const std::vector<double> &yVels)
+        {
+            if (static_cast<int64_t>(xVels.size()) != Length() || static_cast<int64_t>(yVels.size()) != Length())
+            {
+                Throw(true, "HipPDEGrid2D::Advection: velocities must hold xDim*yDim values");
+            }
+            Op op(Op::Kind::AdvectionCells);
+            op.u = std::make_shared<const std::vector<double>>(xVels);
+            op.v = std::make_shared<const std::vector<double>>(yVels);
+            Record(std::move(op));
+        }
+
+        void HipPDEGrid2D::AdvectionCellsEdges(const std::vector<double> &xVels, const std::vector<double> &yVels,
+                                               const DirichletBoundary &bc, const DirichletBoundary &velBc)
+        {
+            if (static_cast<int64_t>(xVels.size()) != Length() || static_cast<int64_t>(yVels.size()) != Length())
+            {
+                Throw(true, "HipPDEGrid2D::Advection: velocities must hold xDim*yDim values");
+            }
+            Op op(Op::Kind::AdvectionCells);
+            op.u = std::make_shared<const std::vector<double>>(xVels);
+            op.v = std::make_shared<const std::vector<double>>(yVels);
+            op.hasBc = true;
+            op.bc = std::make_shared<const DirichletBoundary>(bc);
+            op.velBc = std::make_shared<const DirichletBoundary>(velBc);
+            Record(std::move(op));
+        }
+
+        // The fused tick takes one uniform stencil (explicit or ADI), at most
+        // one uniform advection (explicit ticks only), MulAll and Add, with
+        // no boundary or one shared constant boundary value.
+        bool HipPDEGrid2D::Fusable() const
+        {
+            int stencils = 0;
+            int advections = 0;
+            bool adi = false;
+            bool bcSeen = false;
+            bool hasBc = false;
+            double bcValue = 0.0;
+            for (const Op &op : m_ops)
+            {
+                switch (op.kind)
+                {
+                    case Op::Kind::Diffusion:
+                    case Op::Kind::Adi:
+                    case Op::Kind::Advection:
+                        if (op.hasBc && !op.constBc)
+                        {
+                            return false;
+                        }
+                        if (bcSeen && (op.hasBc != hasBc || (op.hasBc && op.bcValue != bcValue)))
+                        {
+                            return false;
+                        }
+                        bcSeen = true;
+                        hasBc = op.hasBc;
+                        bcValue = op.bcValue;
+                        if (op.kind == Op::Kind::Advection)
+                        {
+                            advections++;
+                        }
+                        else
+                        {
+                            stencils++;
+                            adi = adi || op.kind == Op::Kind::Adi;
+                        }
+                        break;
+                    case Op::Kind::Point:
+                        if (op.code != HALHIP_DELTA_ADD)
+                        {
+                            return false;
+                        }
+                        break;
+                    case Op::Kind::All:
+                        if (op.code != HALHIP_DELTA_MUL)
+                        {
+                            return false;
+                        }
+                        break;
+                    default:
+                        return false;
+                }
+            }
+            return stencils <= 1 && advections <= 1 && !(adi && advections > 0);
+        }
+
+        void HipPDEGrid2D::RunFused()
+        {
+            const char *caller = "HipPDEGrid2D::Update";
+            Stencil stencil = Stencil::None;
+            double rate = 0.0;
+            double vx = 0.0;
+            double vy = 0.0;
+            bool hasBc = false;
+            double bcValue = 0.0;
+            double mulAll = 0.0;
+            std::vector<std::pair<int64_t, float>> adds;
+            for (const Op &op : m_ops)
+            {
+                switch (op.kind)
+                {
+                    case Op::Kind::Diffusion:
+                    case Op::Kind::Adi:
+                        stencil = (op.kind == Op::Kind::Adi) ? Stencil::Adi : Stencil::Explicit;
+                        rate = op.a;
+                        hasBc = hasBc || op.hasBc;
+                        bcValue = op.hasBc ? op.bcValue : bcValue;
+                        break;
+                    case Op::Kind::Advection:
+                        vx = op.a;
+                        vy = op.b;
+                        hasBc = hasBc || op.hasBc;
+                        bcValue = op.hasBc ? op.bcValue : bcValue;
+                        break;
+                    case Op::Kind::All:
+                        mulAll += op.a;
+                        break;
+                    default:
+                        adds.emplace_back(op.index, static_cast<float>(op.a));
+                        break;
+                }
+            }
+            halhip_bc bc{};
+            bc.kind = HALHIP_BC_DIRICHLET;
+            bc.value = bcValue;
+            const halhip_bc *pbc = hasBc ? &bc : nullptr;
+            std::vector<int64_t> idx;
+            std::vector<float> val;
+            for (const auto &a : adds)
+            {
+                idx.push_back(a.first);
+                val.push_back(a.second);
+            }
+            // MulAll(v) adds v*u to the deltas: the tick's decay is -sum(v).
+            const double decay = -mulAll;
+            if (stencil == Stencil::Adi)
+            {
+                MoveFieldTo(Stencil::Adi);
+                if (adds != m_adiSources)
+                {
+                    ThrowIfFailed(halhip_adi2d_set_sources(m_adi, static_cast<int32_t>(idx.size()), idx.data(), val.data()),
+                                  caller);
+                    m_adiSources = adds;
+                }
+                ThrowIfFailed(halhip_adi2d_run_steps_async(m_adi, 1, rate, decay, m_wrapX, m_wrapY, pbc), caller);
+                return;
+            }
+            // explicit diffusion (rate 0 when the tick has none)
+            MoveFieldTo(Stencil::Explicit);
+            if (adds != m_explicitSources)
+            {
+                ThrowIfFailed(halhip_diffusion2d_set_sources(m_explicit, static_cast<int32_t>(idx.size()), idx.data(),
+                                                             val.data()),
+                              caller);
+                m_explicitSources = adds;
+            }
+            if (vx != m_explicitVel[0] || vy != m_explicitVel[1])
+            {
+                ThrowIfFailed(halhip_diffusion2d_set_advection(m_explicit, vx, vy), caller);
+                m_explicitVel[0] = vx;
+                m_explicitVel[1] = vy;
+            }
+            const double r = (stencil == Stencil::Explicit) ? rate : 0.0;
+            ThrowIfFailed(halhip_diffusion2d_run_steps_async(m_explicit, 1, r, decay, m_wrapX, m_wrapY, pbc), caller);
+        }
+
+        // Switches the pending tick to device deltas on the explicit plan:
+        // the committed field moves there and the recorded calls run in order.
+        void HipPDEGrid2D::Materialize()
+        {
+            if (m_general)
+            {
+                return;
+            }
+            MoveFieldTo(Stencil::Explicit);
+            m_general = true;
+            std::vector<Op> ops;
+            ops.swap(m_ops);
+            for (const Op &op : ops)
+            {
+                Issue(op);
+            }
+        }
+
+        void HipPDEGrid2D::FlushPoints()
+        {
+            if (m_pointOps.empty())
+            {
+                return;
+            }
+            ThrowIfFailed(halhip_diffusion2d_delta_point_ops(m_explicit, static_cast<int32_t>(m_pointOps.size()),
+                                                             m_pointOps.data(), m_pointIdx.data(), m_pointVal.data()),
+                          "HipPDEGrid2D");
+            m_pointOps.clear();
+            m_pointIdx.clear();
+            m_pointVal.clear();
+        }
+
+        void HipPDEGrid2D::Issue(const Op &op)
+        {
+            const char *caller = "HipPDEGrid2D";
+            if (op.kind == Op::Kind::Point)
+            {
+                m_pointOps.push_back(op.code);
+                m_pointIdx.push_back(op.index);
+                m_pointVal.push_back(op.a);
+                return;
+            }
+            FlushPoints();
+            const BcView bc = ViewOf(op.hasBc, op.constBc, op.bcValue, op.bc.get());
+            const int64_t n = Length();
+            switch (op.kind)
+            {
+                case Op::Kind::Diffusion:
+                    ThrowIfFailed(halhip_diffusion2d_delta_diffuse(m_explicit, op.a, m_wrapX, m_wrapY, bc.Ptr()), caller);
+                    break;
+                case Op::Kind::DiffusionRates:
+                    ThrowIfFailed(halhip_diffusion2d_delta_diffuse_rates(m_explicit, op.u->data(), op.v->data(), n,
+                                                                         m_wrapX, m_wrapY),
+                                  caller);
+                    break;
+                case Op::Kind::Adi:
+                    ThrowIfFailed(halhip_diffusion2d_delta_adi(m_explicit, Adi(), op.a, m_wrapX, m_wrapY, bc.Ptr()), caller);
+                    break;
+                case Op::Kind::Advection:
+                    ThrowIfFailed(halhip_diffusion2d_delta_advect(m_explicit, op.a, op.b, m_wrapX, m_wrapY, bc.Ptr()),
+                                  caller);
+                    break;
+                case Op::Kind::AdvectionCells:
+                {
+                    const BcView vel = ViewOf(op.velBc != nullptr, false, 0.0, op.velBc.get());
+                    ThrowIfFailed(halhip_diffusion2d_delta_advect_cells(m_explicit, op.u->data(), op.v->data(), n,
+                                                                        m_wrapX, m_wrapY, bc.Ptr(), vel.Ptr()),
+                                  caller);
+                    break;
+                }
+                case Op::Kind::All:
+                    ThrowIfFailed(halhip_diffusion2d_delta_all(m_explicit, op.code, op.a), caller);
+                    break;
+                case Op::Kind::SetValues:
+                    ThrowIfFailed(halhip_diffusion2d_delta_set_values(m_explicit, op.u->data(), n), caller);
+                    break;
+                default:
+                    break;
+            }
         }
 
         void HipPDEGrid2D::Update()
         {
             const char *caller = "HipPDEGrid2D::Update";
-            const bool pending = m_stencil != Stencil::None || m_advect || m_mulAll != 0.0 || !m_adds.empty();
-            if (pending)
+            if (!m_general && Fusable())
             {
-                halhip_bc bc{};
-                bc.kind = HALHIP_BC_DIRICHLET;
-                bc.value = m_bc;
-                const halhip_bc *pbc = m_hasBc ? &bc : nullptr;
-                std::vector<int64_t> idx;
-                std::vector<float> val;
-                for (const auto &a : m_adds)
+                if (!m_ops.empty())
                 {
-                    idx.push_back(a.first);
-                    val.push_back(a.second);
-                }
-                // MulAll(v) adds v*u to the deltas: the tick's decay is -sum(v).
-                const double decay = -m_mulAll;
-                if (m_stencil == Stencil::Adi)
-                {
-                    MoveFieldTo(Stencil::Adi);
-                    if (m_adds != m_adiSources)
-                    {
-                        ThrowIfFailed(halhip_adi2d_set_sources(m_adi, static_cast<int32_t>(idx.size()), idx.data(),
-                                                               val.data()),
-                                      caller);
-                        m_adiSources = m_adds;
-                    }
-                    ThrowIfFailed(halhip_adi2d_run_steps_async(m_adi, 1, m_rate, decay, m_wrapX, m_wrapY, pbc), caller);
-                }
-                else
-                {
-                    // explicit diffusion (rate 0 when the tick has none)
-                    MoveFieldTo(Stencil::Explicit);
-                    if (m_adds != m_explicitSources)
-                    {
-                        ThrowIfFailed(halhip_diffusion2d_set_sources(m_explicit, static_cast<int32_t>(idx.size()),
-                                                                     idx.data(), val.data()),
-                                      caller);
-                        m_explicitSources = m_adds;
-                    }
-                    const double vx = m_advect ? m_xVel : 0.0;
-                    const double vy = m_advect ? m_yVel : 0.0;
-                    if (vx != m_explicitVel[0] || vy != m_explicitVel[1])
-                    {
-                        ThrowIfFailed(halhip_diffusion2d_set_advection(m_explicit, vx, vy), caller);
-                        m_explicitVel[0] = vx;
-                        m_explicitVel[1] = vy;
-                    }
-                    const double rate = (m_stencil == Stencil::Explicit) ? m_rate : 0.0;
-                    ThrowIfFailed(halhip_diffusion2d_run_steps_async(m_explicit, 1, rate, decay, m_wrapX, m_wrapY, pbc),
-                                  caller);
+                    RunFused();
                 }
             }
-            m_stencil = Stencil::None;
-            m_rate = 0.0;
-            m_advect = false;
-            m_hasBc = false;
-            m_bcSet = false;
-            m_mulAll = 0.0;
-            m_adds.clear();
+            else
+            {
+                Materialize();
+                FlushPoints();
+                ThrowIfFailed(halhip_diffusion2d_delta_update(m_explicit), caller);
+            }
+            ClearTick();
             m_updateCt++;
         }
 
         void HipPDEGrid2D::Save(const std::string &path)
         {
             const char *caller = "HipPDEGrid2D::Save";
-            if (m_stencil != Stencil::None || m_advect || m_mulAll != 0.0 || !m_adds.empty())
+            if (!m_ops.empty() || m_general)
             {
                 Throw(true, std::string(caller) + ": the grid has pending calls; Update before saving");
             }
@@ -555,7 +954,7 @@ namespace Hal
             int32_t wx = 0;
             int32_t wy = 0;
             int32_t ct = 0;
-            if (m_stencil != Stencil::None || m_advect || m_mulAll != 0.0 || !m_adds.empty())
+            if (!m_ops.empty() || m_general)
             {
                 Throw(true, std::string(caller) + ": the grid has pending calls; Update before loading");
             }

@@ -202,15 +202,18 @@ namespace Hal
 
         // A PDEGrid2D whose field stays on the GPU, for model code written
         // against Hal::PDEGrid2D's tick calls (include/Hal/Grids/PDEGrid2D.h):
-        // Diffusion / DiffusionADI / Advection / MulAll / Add accumulate a
-        // pending tick, and Update runs it as one fused device tick and
-        // commits, like PDEGrid2D::Update (include/Hal/Grids/PDEGrid2D.h:178-186).
-        // Ticks are queued asynchronously; GetField, Get and Save wait for them.
-        // A tick holds at most one diffusion call (explicit or ADI); Advection
-        // combines with explicit diffusion (or stands alone), and the
-        // boundary value, when given, must be the same for every call of a
-        // tick.  Other combinations throw std::invalid_argument.  Get(i)
-        // downloads the whole field.  One thread per grid object.
+        // every call adds to the pending tick's deltas of the committed field
+        // and Update commits, like PDEGrid2D::Update (:178-186).  Calls are
+        // recorded; a tick of one uniform-rate diffusion (explicit or ADI,
+        // optionally with a constant boundary value), uniform advection
+        // (explicit ticks), MulAll and Add runs on Update as one fused device
+        // tick.  Any other tick -- per-cell rates or velocities, boundary
+        // callables, Set / Mul / Scale / SetAll / AddAll, SetNonNegative, or
+        // several diffusion calls -- and any query of the pending deltas
+        // (MaxDelta) runs its calls in order on device-resident deltas.
+        // Ticks are queued asynchronously; GetField, Get and Save wait for
+        // them.  Get(i) downloads the whole field.  One thread per grid
+        // object.
         class HipPDEGrid2D
         {
         public:
@@ -222,29 +225,90 @@ namespace Hal
 
             int32_t Xdim() const { return m_xDim; }
             int32_t Ydim() const { return m_yDim; }
+            int32_t Length() const { return m_xDim * m_yDim; }
             bool IsWrapX() const { return m_wrapX; }
             bool IsWrapY() const { return m_wrapY; }
             int32_t UpdateCt() const { return m_updateCt; }
 
-            // Committed field (PDEGrid2D::SetState with zero deltas / GetField).
+            // Committed field (PDEGrid2D::SetState with zero deltas / GetField);
+            // SetField drops the pending tick.
             void SetField(const std::vector<double> &field);
             std::vector<double> GetField();
             double Get(int32_t i);
             double Get(int32_t x, int32_t y) { return Get(x * m_yDim + y); }
+            // PDEGrid2D::GetMax / GetMin / GetAvg of the committed field (:133-162)
+            double GetMax();
+            double GetMin();
+            double GetAvg();
 
-            // PDEGrid2D::Diffusion (include/Hal/Grids/PDEGrid2D.h:289-310; rate <= 0.25)
+            // PDEGrid2D::Set / Add / Mul / Scale (include/Hal/Grids/PDEGrid2D.h:89-145)
+            void Set(int32_t i, double val);
+            void Set(int32_t x, int32_t y, double val) { Set(x * m_yDim + y, val); }
+            void Add(int32_t i, double val);
+            void Add(int32_t x, int32_t y, double val) { Add(x * m_yDim + y, val); }
+            void Mul(int32_t i, double val);
+            void Mul(int32_t x, int32_t y, double val) { Mul(x * m_yDim + y, val); }
+            void Scale(int32_t i, double val);
+            void Scale(int32_t x, int32_t y, double val) { Scale(x * m_yDim + y, val); }
+            // PDEGrid2D::SetAll / AddAll / MulAll (:177-218); SetAll(pVals) reads Length() values
+            void SetAll(double val);
+            void SetAll(const double *pVals);
+            void AddAll(double val);
+            void MulAll(double val);
+            // PDEGrid2D::MaxDelta / MaxDeltaScaled / SetNonNegative (:220-259)
+            double MaxDelta();
+            double MaxDeltaScaled(double denomOffset);
+            void SetNonNegative();
+
+            // PDEGrid2D::Diffusion (include/Hal/Grids/PDEGrid2D.h:289-327; uniform rate <= 0.25)
             void Diffusion(double diffCoef);
             void Diffusion(double diffCoef, double boundaryValue);
+            template <typename BCFn>
+            void Diffusion(double diffCoef, BCFn bc)
+            {
+                DiffusionEdges(diffCoef, EdgesOf(bc));
+            }
+            void Diffusion(const std::vector<double> &diffRatesX, const std::vector<double> &diffRatesY);
             // PDEGrid2D::DiffusionADI (include/Hal/Grids/PDEGrid2D.h:328-347)
             void DiffusionADI(double diffRate);
             void DiffusionADI(double diffRate, double boundaryValue);
-            // PDEGrid2D::Advection (include/Hal/Grids/PDEGrid2D.h:349-375; |xVel|+|yVel| <= 0.5)
+            template <typename BCFn>
+            void DiffusionADI(double diffRate, BCFn bc)
+            {
+                DiffusionADIEdges(diffRate, EdgesOf(bc));
+            }
+            // PDEGrid2D::Advection (include/Hal/Grids/PDEGrid2D.h:349-399; uniform |xVel|+|yVel| <= 0.5)
             void Advection(double xVel, double yVel);
             void Advection(double xVel, double yVel, double boundaryValue);
-            // PDEGrid2D::MulAll / Add (include/Hal/Grids/PDEGrid2D.h:103-130, 212-218)
-            void MulAll(double val);
-            void Add(int32_t i, double val);
-            void Add(int32_t x, int32_t y, double val) { Add(x * m_yDim + y, val); }
+            template <typename BCFn>
+            void Advection(double xVel, double yVel, BCFn bc)
+            {
+                AdvectionEdges(xVel, yVel, EdgesOf(bc));
+            }
+            void Advection(const std::vector<double> &xVels, const std::vector<double> &yVels);
+            // Off-grid field values from bc; x velocities at x = -1 / xDim from
+            // xVelBC, y velocities at y = -1 / yDim from yVelBC.
+            template <typename BCFn, typename XVelBCFn, typename YVelBCFn>
+            void Advection(const std::vector<double> &xVels, const std::vector<double> &yVels, BCFn bc,
+                           XVelBCFn xVelBC, YVelBCFn yVelBC)
+            {
+                DirichletBoundary vel;
+                vel.xlo.resize(static_cast<size_t>(m_yDim));
+                vel.xhi.resize(static_cast<size_t>(m_yDim));
+                vel.ylo.resize(static_cast<size_t>(m_xDim));
+                vel.yhi.resize(static_cast<size_t>(m_xDim));
+                for (int32_t y = 0; y < m_yDim; y++)
+                {
+                    vel.xlo[static_cast<size_t>(y)] = xVelBC(-1, y);
+                    vel.xhi[static_cast<size_t>(y)] = xVelBC(m_xDim, y);
+                }
+                for (int32_t x = 0; x < m_xDim; x++)
+                {
+                    vel.ylo[static_cast<size_t>(x)] = yVelBC(x, -1);
+                    vel.yhi[static_cast<size_t>(x)] = yVelBC(x, m_yDim);
+                }
+                AdvectionCellsEdges(xVels, yVels, EdgesOf(bc), vel);
+            }
             // PDEGrid2D::Update: runs and commits the pending tick.
             void Update();
 
@@ -259,9 +323,73 @@ namespace Hal
                 Explicit,
                 Adi
             };
-            void SetBoundary(bool has, double value, const char *caller);
-            void SetStencil(Stencil s, double rate, const char *caller);
+            // One recorded call of the pending tick.
+            struct Op
+            {
+                enum class Kind
+                {
+                    Diffusion,
+                    DiffusionRates,
+                    Adi,
+                    Advection,
+                    AdvectionCells,
+                    Point,
+                    All,
+                    SetValues
+                };
+                explicit Op(Kind k) : kind(k) {}
+                Kind kind;
+                double a = 0.0; // rate, x velocity or value
+                double b = 0.0; // y velocity
+                int32_t code = 0; // point / field-wide op (HALHIP_DELTA_*)
+                int64_t index = 0;
+                bool hasBc = false;
+                bool constBc = false; // a constant boundary value (bcValue) rather than edge arrays
+                double bcValue = 0.0;
+                std::shared_ptr<const DirichletBoundary> bc;    // edge arrays (constBc false)
+                std::shared_ptr<const DirichletBoundary> velBc; // per-cell advection velocity edges
+                std::shared_ptr<const std::vector<double>> u;  // rates / velocities / values
+                std::shared_ptr<const std::vector<double>> v;
+            };
+
+            // Ghost-cell values of a boundary callable bc(x, y).
+            template <typename BCFn>
+            DirichletBoundary EdgesOf(BCFn bc) const
+            {
+                DirichletBoundary e;
+                e.xlo.resize(static_cast<size_t>(m_yDim));
+                e.xhi.resize(static_cast<size_t>(m_yDim));
+                e.ylo.resize(static_cast<size_t>(m_xDim));
+                e.yhi.resize(static_cast<size_t>(m_xDim));
+                for (int32_t y = 0; y < m_yDim; y++)
+                {
+                    e.xlo[static_cast<size_t>(y)] = bc(-1, y);
+                    e.xhi[static_cast<size_t>(y)] = bc(m_xDim, y);
+                }
+                for (int32_t x = 0; x < m_xDim; x++)
+                {
+                    e.ylo[static_cast<size_t>(x)] = bc(x, -1);
+                    e.yhi[static_cast<size_t>(x)] = bc(x, m_yDim);
+                }
+                return e;
+            }
+            void DiffusionEdges(double diffCoef, const DirichletBoundary &bc);
+            void DiffusionADIEdges(double diffRate, const DirichletBoundary &bc);
+            void AdvectionEdges(double xVel, double yVel, const DirichletBoundary &bc);
+            void AdvectionCellsEdges(const std::vector<double> &xVels, const std::vector<double> &yVels,
+                                     const DirichletBoundary &bc, const DirichletBoundary &velBc);
+
+            void Record(Op op);
+            void CheckIndex(int32_t i, const char *caller) const;
+            bool Fusable() const;
+            void RunFused();
+            void Materialize();
+            void Issue(const Op &op);
+            void FlushPoints();
+            void ClearTick();
+            double Reduce(int32_t what, double offset, const char *caller);
             void MoveFieldTo(Stencil s);
+            halhip_adi2d *Adi();
 
             HipContext &m_context;
             int32_t m_xDim;
@@ -272,18 +400,15 @@ namespace Hal
             halhip_diffusion2d *m_explicit;
             halhip_adi2d *m_adi;
             Stencil m_resident; // plan holding the committed field
-            // pending tick
-            Stencil m_stencil;
-            double m_rate;
-            bool m_advect;
-            double m_xVel;
-            double m_yVel;
-            bool m_hasBc;
-            double m_bc;
-            bool m_bcSet;
-            double m_mulAll;
-            std::vector<std::pair<int64_t, float>> m_adds;
-            // sources last handed to each plan
+            // the pending tick: recorded calls, or (m_general) calls already
+            // applied to the explicit plan's device deltas, with point ops
+            // waiting in m_points to go in one batch
+            std::vector<Op> m_ops;
+            bool m_general;
+            std::vector<int32_t> m_pointOps;
+            std::vector<int64_t> m_pointIdx;
+            std::vector<double> m_pointVal;
+            // sources and velocity last handed to each plan's fused tick
             std::vector<std::pair<int64_t, float>> m_explicitSources;
             std::vector<std::pair<int64_t, float>> m_adiSources;
             double m_explicitVel[2];

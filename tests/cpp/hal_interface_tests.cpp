@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <memory>
+#include <cstdint>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -408,15 +409,33 @@ namespace
 
         Check(Throws([&] { walls.Diffusion(0.3); }), "unstable explicit rate throws");
         Check(Throws([&] { walls.Advection(0.4, 0.2); }), "unstable advection velocity throws");
-        walls.Diffusion(0.1);
-        Check(Throws([&] { walls.DiffusionADI(1.0); }), "two diffusion calls in one tick throw");
-        Check(Throws([&] { walls.Advection(0.1, 0.1, 2.0); }), "differing boundary values in one tick throw");
-        Check(Throws([&] { walls.Save(scratchDir + "/pending.bin"); }), "saving with pending calls throws");
-        walls.Update();
-        walls.DiffusionADI(1.0);
-        Check(Throws([&] { walls.Advection(0.1, 0.1); }), "advection with ADI in one tick throws");
-        walls.Update();
+        // Ticks the fused kernels do not cover run call by call on device
+        // deltas: two diffusion calls, differing boundary values, advection
+        // with ADI.  Each matches the same calls through the array adapters.
+        {
+            std::vector<double> r0 = walls.GetField();
+            std::vector<double> d(r0.size(), 0.0);
+            walls.Diffusion(0.1);
+            walls.DiffusionADI(1.0);
+            walls.Advection(0.1, 0.1, 2.0);
+            Check(Throws([&] { walls.Save(scratchDir + "/pending.bin"); }), "saving with pending calls throws");
+            walls.Update();
+            expl.Diffuse(r0, d, 0.1, nx, ny, false, false);
+            adi.DiffuseADI(r0, d, 1.0, nx, ny, false, false);
+            Hal::Hip::DirichletBoundary two;
+            two.value = 2.0;
+            expl.Advect(r0, d, 0.1, 0.1, nx, ny, false, false, two);
+            for (size_t i = 0; i < r0.size(); i++)
+            {
+                r0[i] += d[i];
+            }
+            Check(RelL2(walls.GetField(), r0) < 1e-5,
+                  "a tick of explicit + ADI diffusion + advection with its own boundary matches the adapters");
+        }
         Check(Throws([&] { walls.Add(nx * ny, 1.0); }), "out-of-range Add throws");
+        Check(Throws([&] { walls.Set(-1, 1.0); }), "out-of-range Set throws");
+        Check(Throws([&] { walls.Diffusion(std::vector<double>(3, 0.1), std::vector<double>(3, 0.1)); }),
+              "per-cell rates of the wrong length throw");
 
         const std::string path = scratchDir + "/resident_grid.bin";
         walls.Save(path);
@@ -428,6 +447,123 @@ namespace
         Check(Throws([&] { wrapped.Load(path); }), "loading a file with other wrap flags throws");
         Hal::Hip::HipPDEGrid2D other(ctx, nx, ny + 1, false, false);
         Check(Throws([&] { other.Load(path); }), "loading a file with other dimensions throws");
+    }
+}
+
+namespace
+{
+    void WriteDoubles(const std::string &path, const std::vector<double> &v)
+    {
+        FILE *f = std::fopen(path.c_str(), "wb");
+        Check(f != nullptr, "open " + path);
+        if (f != nullptr)
+        {
+            Check(std::fwrite(v.data(), sizeof(double), v.size(), f) == v.size(), "write " + path);
+            std::fclose(f);
+        }
+    }
+
+    std::vector<double> Draws(uint64_t seed, int n, double scale, double shift)
+    {
+        std::vector<double> v(static_cast<size_t>(n));
+        halhip_rng_doubles(seed, n, v.data());
+        for (double &x : v)
+        {
+            x = scale * (x + shift);
+        }
+        return v;
+    }
+
+    // Model code over the whole PDEGrid2D tick surface on the device grid.
+    // The results go to files that the Python side compares with the FP64
+    // oracle running the same calls (tests/test_cpp_interfaces.py).
+    void ResidentGridScenarios(Hal::Hip::HipContext &ctx, const std::string &dir)
+    {
+        // 1: fused ADI model ticks (DiffusionADI with a boundary value,
+        // MulAll, Add) on rows that fill whole wavefronts
+        {
+            const int X = 96;
+            const int Y = 128;
+            Hal::Hip::HipPDEGrid2D g(ctx, X, Y, false, false);
+            g.SetField(Draws(11, X * Y, 1.0, 0.0));
+            const int src[3] = {0, 40 * Y + 77, X * Y - 1};
+            for (int t = 0; t < 25; t++)
+            {
+                g.DiffusionADI(5.0, 0.0);
+                g.MulAll(-2e-3);
+                for (int k = 0; k < 3; k++)
+                {
+                    g.Add(src[k], 0.5 * (k + 1));
+                }
+                g.Update();
+            }
+            WriteDoubles(dir + "/facade_adi.bin", g.GetField());
+        }
+        // 2: a tick of every kind of call: boundary callables, per-cell
+        // velocities with velocity callables, a second (ADI) diffusion,
+        // Set / Scale / Mul / Add, AddAll / MulAll, SetNonNegative, MaxDelta
+        {
+            const int X = 40;
+            const int Y = 36;
+            Hal::Hip::HipPDEGrid2D g(ctx, X, Y, false, false);
+            g.SetField(Draws(21, X * Y, 1.0, -0.2));
+            const std::vector<double> xv = Draws(22, X * Y, 0.1, -0.5);
+            const std::vector<double> yv = Draws(23, X * Y, 0.1, -0.5);
+            auto bc = [](int32_t x, int32_t y) { return 0.5 + 0.01 * x - 0.02 * y; };
+            auto xVelBC = [](int32_t, int32_t) { return 0.05; };
+            auto yVelBC = [](int32_t, int32_t) { return -0.03; };
+            std::vector<double> stats;
+            for (int t = 0; t < 8; t++)
+            {
+                g.Diffusion(0.1, bc);
+                g.Advection(xv, yv, bc, xVelBC, yVelBC);
+                g.DiffusionADI(2.0);
+                g.Set(7, 5, 1.5);
+                g.Scale(100, 1.2);
+                g.Mul(200, -0.1);
+                g.Add(300, 0.25);
+                g.AddAll(1e-3);
+                g.MulAll(-5e-3);
+                if (t % 2 == 1)
+                {
+                    g.SetNonNegative();
+                }
+                stats.push_back(g.MaxDelta());
+                stats.push_back(g.MaxDeltaScaled(0.1));
+                g.Update();
+            }
+            stats.push_back(g.GetMax());
+            stats.push_back(g.GetMin());
+            stats.push_back(g.GetAvg());
+            WriteDoubles(dir + "/facade_general.bin", g.GetField());
+            WriteDoubles(dir + "/facade_general_stats.bin", stats);
+        }
+        // 3: per-cell rates with uniform advection, on a grid periodic in x;
+        // then SetAll from values and a fused tick from the same state
+        {
+            const int X = 32;
+            const int Y = 40;
+            Hal::Hip::HipPDEGrid2D g(ctx, X, Y, true, false);
+            g.SetField(Draws(31, X * Y, 1.0, 0.0));
+            const std::vector<double> rx = Draws(32, X * Y, 0.2, 0.0);
+            const std::vector<double> ry = Draws(33, X * Y, 0.2, 0.0);
+            for (int t = 0; t < 12; t++)
+            {
+                g.Diffusion(rx, ry);
+                g.Advection(0.1, 0.05);
+                g.MulAll(-1e-3);
+                g.Add(5, 0.1);
+                g.Update();
+            }
+            const std::vector<double> vals = Draws(34, X * Y, 1.0, 0.0);
+            g.SetAll(vals.data());
+            g.Diffusion(0.2);
+            g.Update();
+            g.Diffusion(0.2);
+            g.MulAll(-1e-3);
+            g.Update();
+            WriteDoubles(dir + "/facade_rates.bin", g.GetField());
+        }
     }
 }
 
@@ -501,6 +637,7 @@ int main(int argc, char **argv)
               "coincident agents give zero force");
     }
     ResidentGrid(*ctx, scratchDir);
+    ResidentGridScenarios(*ctx, scratchDir);
     std::printf("%d checks, %d failures\n", g_checks, g_failures);
     return g_failures == 0 ? 0 : 1;
 }

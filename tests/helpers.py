@@ -48,3 +48,41 @@ def eigen_field(nx, ny):
 def lambda_cn(rate, n, k=1):
     s = np.sin(np.pi * k / n) ** 2
     return (1 - 2 * rate * s) / (1 + 2 * rate * s)
+
+
+class RefGrid:
+    """The reference's PDEGrid2D delta semantics in FP64 (host), with the
+    oracle supplying the stencils' interface deltas."""
+
+    def __init__(self, oracle, field, X, Y, wrap):
+        self.o, self.u, self.X, self.Y, self.wrap = oracle, field.astype(np.float64), X, Y, wrap
+        self.d = np.zeros(X * Y)
+
+    def diffuse(self, rate, bc=None):
+        self.d = self.o.diffuse(self.u, self.d, rate, self.X, self.Y, *self.wrap, bc=bc)
+
+    def diffuse_rates(self, rx, ry):
+        self.d = self.o.diffuse_rates(self.u, self.d, rx, ry, self.X, self.Y, *self.wrap)
+
+    def advect(self, vx, vy, bc=None):
+        self.d = self.o.advect(self.u, self.d, vx, vy, self.X, self.Y, *self.wrap, bc=bc)
+
+    def advect_cells(self, xv, yv, bc=None, vbc=None):
+        self.d = self.o.advect_cells(self.u, self.d, xv, yv, self.X, self.Y, *self.wrap, bc=bc, vbc=vbc)
+
+    def adi(self, rate, bc=None):
+        self.d = self.o.adi(self.u, self.d, rate, self.X, self.Y, *self.wrap, bc=bc)
+
+    def point(self, op, i, v):
+        if op == 0:  # Set
+            self.d[i] = v - self.u[i]
+        elif op == 1:  # Add
+            self.d[i] += v
+        elif op == 2:  # Mul
+            self.d[i] += self.u[i] * v
+        else:  # Scale
+            self.d[i] += self.u[i] * (v - 1)
+
+    def update(self):
+        self.u = self.u + self.d
+        self.d = np.zeros_like(self.d)
