@@ -143,7 +143,14 @@ namespace
         const int t = threadIdx.x;
         const int lane = t & 63;
         const int wave = t >> 6;
-        const int line = blockIdx.x;
+        // Lines are dealt so that each XCD (blocks b, b + 8, ...) takes a
+        // contiguous range: the neighbour lines the explicit half-step reads
+        // are then in the same L2.
+        int line = static_cast<int>(blockIdx.x);
+        if (a.nLines % 8 == 0)
+        {
+            line = (line % 8) * (a.nLines / 8) + line / 8;
+        }
         const int L = a.L;
         const size_t base = static_cast<size_t>(line) * L;
         const int j0 = t * EPL;
