@@ -186,6 +186,338 @@ namespace
         }
     }
 
+    // ---- rows longer than one workgroup --------------------------------------------
+    //
+    // A row of yDim > HALHIP_ADI_MAX_Y columns is cut into G segments of W =
+    // 4 x threads columns, and strip p's segment g is one workgroup.  Sweep A
+    // is unchanged (its lines are columns).  Sweep B is linear in its two
+    // carries, so each segment solves with zero carries (x0) and the segments
+    // are joined exactly afterwards:
+    //     d'_j = d'0_j + P_j cf,   x_j = x0_j + Q_j xb + R_j cf   (yS <= j < yE)
+    // with cf = d' of column yS-1, xb = x of column yE, and the host tables
+    //     P_j = prod_{yS<=k<=j} fw_k,  Q_j = prod_{j<=k<yE} bw_k,
+    //     R_j = bw_j R_{j+1} + ib_j P_j  (R_yE = 0).
+    // The carries follow from two numbers per row and segment: x0 at the
+    // first column and x0 at the last (d'0 there is x0 * b').  A periodic y
+    // axis applies its Sherman-Morrison closure to the joined row.
+    // A tick: AdiSegReduce (sweep A strip reductions), the cross-strip scan,
+    // AdiSegSolve (sweep A finish, sweep B right-hand side and zero-carry
+    // solve into x0), AdiSegScan (carries per row) and AdiSegFix (x0 plus the
+    // carry responses, with decay, written over the field).
+    struct SegArgs
+    {
+        int32_t G;         // segments per row
+        int32_t W;         // columns per segment
+        float *x0;         // [X][Y] zero-carry sweep-B solution
+        float *first;      // [G][X] x0 at each segment's first column
+        float *last;       // [G][X] x0 at each segment's last column
+        float *cf;         // [G][X] forward carry into each segment
+        float *xb;         // [G][X] backward carry into each segment
+        float *factor;     // [X] Sherman-Morrison factor of a periodic y axis
+        const float *q;    // [G*W] Q_j
+        const float *rr;   // [G*W] R_j
+        const float *pEnd; // [G] P at each segment's last column
+        const float *bEnd; // [G] b' (1/ib) at each segment's last column
+        float rLast;       // R at column Y-1
+    };
+
+    constexpr int kSegCpt = 4;
+
+    // Sweep A right-hand side of the segment [yS, yE) of every strip row, as
+    // SweepARhs forms it for a whole row.  The neighbours of the segment's
+    // first and last columns are read from the field (the adjacent segments)
+    // or, at the grid's ends, are the y boundary's ghost values.
+    template <int CPT, int S, bool VEC>
+    __device__ __forceinline__ void SegSweepARhs(const AdiArgs &a, const float *__restrict__ u, int s0, int yS,
+                                                 int yE, int tid, float (&v)[S][CPT], const StripLds &L)
+    {
+        const int lane = tid & 63;
+        const int wave = tid >> 6;
+        const int nWaves = blockDim.x >> 6;
+        const int y0 = yS + tid * CPT;
+        const int lastT = (yE - 1 - yS) / CPT;
+        const int lastC = (yE - 1 - yS) % CPT;
+        if (lane == 0)
+        {
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+                L.left[wave * S + r] = v[r][0];
+            }
+        }
+        if (lane == 63)
+        {
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+                L.right[wave * S + r] = v[r][CPT - 1];
+            }
+        }
+        __syncthreads();
+        const bool dirY = a.dirichlet && !a.wrapY;
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            const int x = s0 + r;
+            float left = Dpp<0x138>(0.0f, v[r][CPT - 1]);
+            float right = Dpp<0x130>(0.0f, v[r][0]);
+            if (x >= a.X)
+            {
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    v[r][c] = 0.0f;
+                }
+                continue;
+            }
+            if (lane == 0 && wave > 0)
+            {
+                left = L.right[(wave - 1) * S + r];
+            }
+            if (lane == 63 && wave + 1 < nWaves)
+            {
+                right = L.left[(wave + 1) * S + r];
+            }
+            const float *row = u + static_cast<size_t>(x) * a.Y;
+            if (tid == 0)
+            {
+                const float c0 = v[r][0];
+                left = (yS > 0) ? row[yS - 1] : (a.wrapY ? row[a.Y - 1] : (dirY ? 2.0f * a.bcYlo[x] - c0 : c0));
+            }
+            float ghostHi = 0.0f;
+            if (tid == lastT)
+            {
+                float cl = 0.0f;
+#pragma unroll
+                for (int c = 0; c < CPT; c++)
+                {
+                    cl = (c == lastC) ? v[r][c] : cl;
+                }
+                ghostHi = (yE < a.Y) ? row[yE] : (a.wrapY ? row[0] : (dirY ? 2.0f * a.bcYhi[x] - cl : cl));
+            }
+            const bool srcRow = a.dirichlet && !a.wrapX && (x == 0 || x == a.X - 1);
+            float prev = left;
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+            {
+                const float cur = v[r][c];
+                float nxt = (c + 1 < CPT) ? v[r][c + 1] : right;
+                if (tid == lastT && c == lastC)
+                {
+                    nxt = ghostHi;
+                }
+                float rhs = fmaf((nxt - cur) + (prev - cur), a.hr, cur);
+                if (srcRow)
+                {
+                    float src = 0.0f;
+                    if (x == 0)
+                    {
+                        src += a.bcXlo[min(y0 + c, a.Y - 1)];
+                    }
+                    if (x == a.X - 1)
+                    {
+                        src += a.bcXhi[min(y0 + c, a.Y - 1)];
+                    }
+                    rhs = fmaf(src, a.r, rhs);
+                }
+                prev = cur;
+                v[r][c] = (y0 + c < a.Y) ? rhs : 0.0f;
+            }
+        }
+    }
+
+    // Workgroup b runs strip b / G, segment b % G (neighbouring workgroups
+    // share rows).
+    struct SegTile
+    {
+        int p, g, s0, yS, yE, y0;
+    };
+
+    template <int S>
+    __device__ __forceinline__ SegTile SegTileOf(const AdiArgs &a, const SegArgs &sg)
+    {
+        SegTile t;
+        t.p = static_cast<int>(blockIdx.x) / sg.G;
+        t.g = static_cast<int>(blockIdx.x) - t.p * sg.G;
+        t.s0 = t.p * S;
+        t.yS = t.g * sg.W;
+        t.yE = min(a.Y, t.yS + sg.W);
+        t.y0 = t.yS + static_cast<int>(threadIdx.x) * kSegCpt;
+        return t;
+    }
+
+    // Sweep A reductions of the current field (the strip kernel's MODE 0).
+    template <int S, bool VEC>
+    __global__ __launch_bounds__(1024) void AdiSegReduce(AdiArgs a, SegArgs sg, const float *__restrict__ u)
+    {
+        __shared__ float sLeft[kMaxWaves * S];
+        __shared__ float sRight[kMaxWaves * S];
+        const StripLds L{sLeft, sRight, nullptr, nullptr, nullptr, nullptr, nullptr};
+        const SegTile t = SegTileOf<S>(a, sg);
+        float v[S][kSegCpt];
+        LoadStripRows<kSegCpt, S, VEC>(a, u, t.s0, t.y0, v);
+        SegSweepARhs<kSegCpt, S, VEC>(a, u, t.s0, t.yS, t.yE, static_cast<int>(threadIdx.x), v, L);
+        ReduceA<kSegCpt, S, VEC>(a, t.p, t.s0, t.y0, v);
+    }
+
+    // Sweep A finished from the scan's carries, sweep B's right-hand side and
+    // its zero-carry solve on the segment, written to x0 with the segment's
+    // first and last values.
+    template <int S, bool VEC>
+    __global__ __launch_bounds__(1024) void AdiSegSolve(AdiArgs a, SegArgs sg, const float *__restrict__ u)
+    {
+        __shared__ float sLeft[kMaxWaves * S];
+        __shared__ float sRight[kMaxWaves * S];
+        __shared__ float sTot[kMaxWaves * S];
+        __shared__ float sAw[kMaxWaves];
+        const StripLds L{sLeft, sRight, nullptr, nullptr, sTot, sAw, nullptr};
+        const int tid = static_cast<int>(threadIdx.x);
+        const SegTile t = SegTileOf<S>(a, sg);
+        float v[S][kSegCpt];
+        LoadStripRows<kSegCpt, S, VEC>(a, u, t.s0, t.y0, v);
+        SegSweepARhs<kSegCpt, S, VEC>(a, u, t.s0, t.yS, t.yE, tid, v, L);
+        float up[kSegCpt];
+        float dn[kSegCpt];
+        FinishA<kSegCpt, S, VEC>(a, t.p, t.s0, t.y0, v, up, dn);
+        // SweepBRhs takes the grid-wide thread index of the columns (its
+        // Dirichlet terms sit on the grid's first and last columns).
+        SweepBRhs<kSegCpt, S, VEC>(a, t.s0, t.y0, t.y0 / kSegCpt, v, up, dn);
+        SolveB<kSegCpt, S, true>(a, t.y0, tid, v, L);
+        const int lastT = (t.yE - 1 - t.yS) / kSegCpt;
+        const int lastC = (t.yE - 1 - t.yS) % kSegCpt;
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            const int x = t.s0 + r;
+            if (x >= a.X)
+            {
+                break;
+            }
+            StoreRow<kSegCpt, VEC>(sg.x0 + static_cast<size_t>(x) * a.Y, t.y0, a.Y, v[r]);
+            const size_t k = static_cast<size_t>(t.g) * a.X + x;
+            if (tid == 0)
+            {
+                sg.first[k] = v[r][0];
+            }
+            if (tid == lastT)
+            {
+                float cl = 0.0f;
+#pragma unroll
+                for (int c = 0; c < kSegCpt; c++)
+                {
+                    cl = (c == lastC) ? v[r][c] : cl;
+                }
+                sg.last[k] = cl;
+            }
+        }
+    }
+
+    // Carries of every segment of row x: forward from the left (d' of the
+    // column before each segment), backward from the right (x of the column
+    // after it), and the periodic closure's factor from the row's end values.
+    __global__ __launch_bounds__(256) void AdiSegScan(AdiArgs a, SegArgs sg)
+    {
+        const int x = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+        if (x >= a.X)
+        {
+            return;
+        }
+        float cf = 0.0f;
+        float cfLast = 0.0f;
+        for (int g = 0; g < sg.G; g++)
+        {
+            const size_t k = static_cast<size_t>(g) * a.X + x;
+            sg.cf[k] = cf;
+            cfLast = cf;
+            cf = fmaf(sg.pEnd[g], cf, sg.last[k] * sg.bEnd[g]);
+        }
+        float xb = 0.0f;
+        for (int g = sg.G - 1; g >= 0; g--)
+        {
+            const size_t k = static_cast<size_t>(g) * a.X + x;
+            const size_t j = static_cast<size_t>(g) * sg.W;
+            sg.xb[k] = xb;
+            xb = fmaf(sg.q[j], xb, fmaf(sg.rr[j], sg.cf[k], sg.first[k]));
+        }
+        if (a.wrapY)
+        {
+            // xb is now x of column 0; column Y-1 has no backward carry.
+            const float xLast = fmaf(sg.rLast, cfLast, sg.last[static_cast<size_t>(sg.G - 1) * a.X + x]);
+            sg.factor[x] = (xb + xLast * a.yvEnd) * a.yInv;
+        }
+    }
+
+    // u' = x0 + Q xb + R cf (- factor z on a periodic y axis) (- k u with
+    // decay), in place over the field.  V cells per thread: 4 when yDim is a
+    // multiple of 4 (a group then never straddles a row or a segment).
+    template <int V, bool DECAY>
+    __global__ __launch_bounds__(256) void AdiSegFix(AdiArgs a, SegArgs sg, float *__restrict__ u)
+    {
+        const int64_t n = static_cast<int64_t>(a.X) * a.Y / V;
+        for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+             t += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        {
+            const int64_t i = t * V;
+            const int x = static_cast<int>(i / a.Y);
+            const int y = static_cast<int>(i - static_cast<int64_t>(x) * a.Y);
+            const size_t k = static_cast<size_t>(y / sg.W) * a.X + x;
+            const float xb = sg.xb[k];
+            const float cf = sg.cf[k];
+            const float fac = a.wrapY ? sg.factor[x] : 0.0f;
+            float v[V];
+            float q[V];
+            float rr[V];
+            float z[V];
+            float old[V];
+            // The field and x0 move as vectors; the tables (at any float
+            // offset of the plan's table buffer) load per element.
+            if constexpr (V == 4)
+            {
+                const float4 tv = *reinterpret_cast<const float4 *>(sg.x0 + i);
+                v[0] = tv.x, v[1] = tv.y, v[2] = tv.z, v[3] = tv.w;
+                if constexpr (DECAY)
+                {
+                    const float4 to = *reinterpret_cast<const float4 *>(u + i);
+                    old[0] = to.x, old[1] = to.y, old[2] = to.z, old[3] = to.w;
+                }
+            }
+            else
+            {
+                v[0] = sg.x0[i];
+                if constexpr (DECAY)
+                {
+                    old[0] = u[i];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < V; c++)
+            {
+                q[c] = sg.q[y + c];
+                rr[c] = sg.rr[y + c];
+                z[c] = a.yz[y + c];
+            }
+#pragma unroll
+            for (int c = 0; c < V; c++)
+            {
+                float r = fmaf(q[c], xb, fmaf(rr[c], cf, v[c]));
+                r = fmaf(-fac, z[c], r);
+                if constexpr (DECAY)
+                {
+                    r = fmaf(old[c], a.negDecay, r);
+                }
+                v[c] = r;
+            }
+            if constexpr (V == 4)
+            {
+                *reinterpret_cast<float4 *>(u + i) = make_float4(v[0], v[1], v[2], v[3]);
+            }
+            else
+            {
+                u[i] = v[0];
+            }
+        }
+    }
+
     bool ShapeCompiled(int32_t cpt, int32_t s)
     {
         return (cpt == 1 || cpt == 2 || cpt == 4) ? (s == 4 || s == 8 || s == 16) : (cpt == 8 && (s == 4 || s == 8));
@@ -510,6 +842,12 @@ struct halhip_adi2d
     Profiler prof;
     GraphCache graphs;
     bool prepareOnly = false;   // inside halhip_adi2d_prepare_steps
+    // Rows cut into segments across workgroups (segG > 0): yDim above
+    // HALHIP_ADI_MAX_Y, or forced with HALHIP_ADI_ROWS_SEGMENTED.
+    int32_t segG = 0;
+    DeviceArray<float> segX0;
+    DeviceArray<float> segBuf;
+    SegArgs seg{};
     HostStaging<float> hostBuf; // pinned staging of the per-call FP64 paths
     ChunkedTransfer chunks;
 };
@@ -564,6 +902,48 @@ namespace
         p->graphs.Clear();
     }
 
+    // Cuts the rows into segments of 4 x T columns (T = 1024 threads above
+    // HALHIP_ADI_MAX_Y; T = 64 when forced on shorter rows, so that they
+    // still span several segments), or restores whole-row strips (on =
+    // false).  The caller reallocates the strip buffers.
+    void ConfigureSegments(halhip_adi2d *p, bool on)
+    {
+        if (!on)
+        {
+            p->segG = 0;
+            p->segX0.Release();
+            p->segBuf.Release();
+            p->seg = SegArgs{};
+            p->shape = AutoShape(p->X, p->Y);
+            p->useRow = HALHIP_ADI_ROWS_WORKGROUP_SCAN;
+            return;
+        }
+        const int32_t T = (p->Y > HALHIP_ADI_MAX_Y) ? 1024 : 64;
+        const int32_t W = T * kSegCpt;
+        const int32_t G = (p->Y + W - 1) / W;
+        p->segG = G;
+        p->shape.cpt = kSegCpt;
+        p->shape.threads = T;
+        p->shape.vec = (p->Y % kSegCpt) == 0;
+        p->shape.s = (static_cast<int64_t>((p->X + 7) / 8) * G >= 512) ? 8 : 4;
+        const size_t X = static_cast<size_t>(p->X);
+        p->segX0.Release();
+        p->segBuf.Release();
+        p->segX0.Allocate(p->ctx, Cells(p));
+        p->segBuf.Allocate(p->ctx, 4 * static_cast<size_t>(G) * X + X);
+        SegArgs &g = p->seg;
+        g.G = G;
+        g.W = W;
+        g.x0 = p->segX0.get();
+        g.first = p->segBuf.get();
+        g.last = g.first + static_cast<size_t>(G) * X;
+        g.cf = g.last + static_cast<size_t>(G) * X;
+        g.xb = g.cf + static_cast<size_t>(G) * X;
+        g.factor = g.xb + static_cast<size_t>(G) * X;
+        p->useRow = HALHIP_ADI_ROWS_SEGMENTED;
+        p->useCarry = HALHIP_ADI_CARRIES_SCAN;
+    }
+
     // Builds and uploads the line factors and strip constants for a rate and
     // boundary set, when they differ from the resident ones.
     void PrepareTables(halhip_adi2d *p, double rate, int32_t wrapX, int32_t wrapY, const halhip_bc *bc)
@@ -586,7 +966,7 @@ namespace
         const int32_t cpt = p->shape.cpt;
         const int32_t T = p->shape.threads;
         const int32_t xPad = p->P * S;
-        const int32_t yPad = T * cpt;
+        const int32_t yPad = (p->segG > 0) ? p->segG * T * cpt : T * cpt;
         const bool dir = e.dirichlet;
         AxisFactors fx = FactorLine(p->X, xPad, rate, wrapX != 0, dir);
         AxisFactors fy = FactorLine(p->Y, yPad, rate, wrapY != 0, dir);
@@ -669,8 +1049,48 @@ namespace
         size_t oA = put(sA), oC = put(sC), oE = put(sE), oW = put(redW);
         size_t oXlo = putf(e.xlo), oXhi = putf(e.xhi), oYlo = putf(e.ylo), oYhi = putf(e.yhi);
 
+        // Segment responses to the carries (SegArgs), in FP64 from the
+        // sweep-B factors.
+        size_t oSq = 0, oSr = 0, oSp = 0, oSb = 0;
+        double segRLast = 0.0;
+        if (p->segG > 0)
+        {
+            const int32_t W = T * cpt;
+            std::vector<double> q(static_cast<size_t>(yPad), 0.0), rr(static_cast<size_t>(yPad), 0.0);
+            std::vector<double> pEnd(static_cast<size_t>(p->segG)), bEnd(static_cast<size_t>(p->segG));
+            std::vector<double> prod(static_cast<size_t>(W));
+            for (int32_t g = 0; g < p->segG; g++)
+            {
+                const int32_t yS = g * W;
+                const int32_t yE = std::min(p->Y, yS + W);
+                double acc = 1.0;
+                for (int32_t j = yS; j < yE; j++)
+                {
+                    acc *= fy.fw[static_cast<size_t>(j)];
+                    prod[static_cast<size_t>(j - yS)] = acc;
+                }
+                pEnd[static_cast<size_t>(g)] = acc;
+                bEnd[static_cast<size_t>(g)] = 1.0 / fy.ib[static_cast<size_t>(yE - 1)];
+                double qn = 1.0;
+                double rn = 0.0;
+                for (int32_t j = yE - 1; j >= yS; j--)
+                {
+                    const size_t k = static_cast<size_t>(j);
+                    q[k] = fy.bw[k] * qn;
+                    rr[k] = fy.bw[k] * rn + fy.ib[k] * prod[static_cast<size_t>(j - yS)];
+                    qn = q[k];
+                    rn = rr[k];
+                }
+            }
+            segRLast = rr[static_cast<size_t>(p->Y - 1)];
+            oSq = put(q);
+            oSr = put(rr);
+            oSp = put(pEnd);
+            oSb = put(bEnd);
+        }
+
         // Row solver and carries for plain ticks.
-        const int32_t ldsE = (p->wantRow == HALHIP_ADI_ROWS_WORKGROUP_SCAN) ? 0 : LdsElements(p->shape, p->Y);
+        const int32_t ldsE = (p->segG > 0 || p->wantRow == HALHIP_ADI_ROWS_WORKGROUP_SCAN) ? 0 : LdsElements(p->shape, p->Y);
         int32_t K = (ldsE == 0 || p->wantCarry == HALHIP_ADI_CARRIES_SCAN) ? 0 : WindowSize(sA, sC);
         if (K > 0 && PickLds(p->shape.cpt, p->shape.s, ldsE, K, p->shape.threads) == nullptr)
         {
@@ -742,8 +1162,17 @@ namespace
         a.colWindow = colWindow;
         p->ldsE = ldsE;
         p->window = K;
-        p->useRow = (ldsE > 0) ? HALHIP_ADI_ROWS_LDS : HALHIP_ADI_ROWS_WORKGROUP_SCAN;
+        p->useRow = (p->segG > 0) ? HALHIP_ADI_ROWS_SEGMENTED
+                                  : ((ldsE > 0) ? HALHIP_ADI_ROWS_LDS : HALHIP_ADI_ROWS_WORKGROUP_SCAN);
         p->useCarry = (K > 0) ? HALHIP_ADI_CARRIES_WINDOW : HALHIP_ADI_CARRIES_SCAN;
+        if (p->segG > 0)
+        {
+            p->seg.q = t + oSq;
+            p->seg.rr = t + oSr;
+            p->seg.pEnd = t + oSp;
+            p->seg.bEnd = t + oSb;
+            p->seg.rLast = static_cast<float>(segRLast);
+        }
         if (ldsE > 0)
         {
             p->rs = rt.solve;
@@ -842,6 +1271,58 @@ namespace
         else
         {
             LaunchStrip(p, 1, "adi_strip_tick");
+        }
+    }
+
+    using SegKernel = void (*)(AdiArgs, SegArgs, const float *);
+
+    template <int S>
+    SegKernel PickSeg(bool solve, bool vec)
+    {
+        if (solve)
+        {
+            return vec ? AdiSegSolve<S, true> : AdiSegSolve<S, false>;
+        }
+        return vec ? AdiSegReduce<S, true> : AdiSegReduce<S, false>;
+    }
+
+    void LaunchSeg(halhip_adi2d *p, bool solve)
+    {
+        const SegKernel k = (p->shape.s == 8) ? PickSeg<8>(solve, p->shape.vec) : PickSeg<4>(solve, p->shape.vec);
+        p->prof.Begin(p->clock.stream, solve ? "adi_seg_solve" : "adi_seg_reduce");
+        hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(p->P) * static_cast<unsigned>(p->segG)),
+                           dim3(static_cast<unsigned>(p->shape.threads)), 0, p->clock.stream, p->args, p->seg,
+                           static_cast<const float *>(p->field.get()));
+        Check(hipGetLastError(), "AdiSeg launch");
+        p->prof.End(p->clock.stream);
+    }
+
+    // One tick of a segmented plan (see SegArgs): every tick starts from the
+    // reductions of the current field, so point sources added after the
+    // previous tick need nothing more.
+    void SegTick(halhip_adi2d *p, bool decay, bool sources)
+    {
+        p->args.red = RedHalf(p, p->redCur);
+        LaunchSeg(p, false);
+        LaunchScan(p);
+        LaunchSeg(p, true);
+        p->prof.Begin(p->clock.stream, "adi_seg_scan");
+        hipLaunchKernelGGL(AdiSegScan, dim3(static_cast<unsigned>((p->X + 255) / 256)), dim3(256), 0,
+                           p->clock.stream, p->args, p->seg);
+        Check(hipGetLastError(), "AdiSegScan launch");
+        p->prof.End(p->clock.stream);
+        const int64_t groups = static_cast<int64_t>(Cells(p)) / (p->shape.vec ? 4 : 1);
+        const unsigned blocks = static_cast<unsigned>(std::min<int64_t>((groups + 255) / 256, 16384));
+        using FixKernel = void (*)(AdiArgs, SegArgs, float *);
+        const FixKernel fix = p->shape.vec ? (decay ? AdiSegFix<4, true> : AdiSegFix<4, false>)
+                                           : (decay ? AdiSegFix<1, true> : AdiSegFix<1, false>);
+        p->prof.Begin(p->clock.stream, "adi_seg_fix");
+        hipLaunchKernelGGL(fix, dim3(blocks), dim3(256), 0, p->clock.stream, p->args, p->seg, p->field.get());
+        Check(hipGetLastError(), "AdiSegFix launch");
+        p->prof.End(p->clock.stream);
+        if (sources)
+        {
+            LaunchSources(p);
         }
     }
 
@@ -957,6 +1438,19 @@ namespace
         }
         const bool plain = (decay == 0.0) && (!withSources || p->nSources == 0);
         p->args.red = RedHalf(p, p->redCur);
+        if (p->segG > 0)
+        {
+            const bool src = withSources && p->nSources > 0;
+            const bool dec = decay != 0.0;
+            p->args.negDecay = static_cast<float>(-decay);
+            std::string key = p->tableKey;
+            AppendKey(key, -2); // segmented ticks
+            AppendKey(key, p->args.negDecay);
+            AppendKey(key, src ? 1 : 0);
+            RunChunked(p, steps, key, [p, dec, src] { SegTick(p, dec, src); });
+            p->reducedValid = false;
+            return;
+        }
         if (plain)
         {
             if (!p->reducedValid && !p->prepareOnly)
@@ -1043,10 +1537,6 @@ extern "C"
                 *out = nullptr;
                 Require(ctx != nullptr, "halhip_adi2d_create: null context");
                 Require(xDim > 0 && yDim > 0, "halhip_adi2d_create: dimensions must be positive");
-                if (yDim > HALHIP_ADI_MAX_Y)
-                {
-                    Fail(HALHIP_ERR_UNSUPPORTED, "halhip_adi2d_create: yDim exceeds HALHIP_ADI_MAX_Y (8192)");
-                }
                 Require(static_cast<int64_t>(xDim) * yDim <= (int64_t(1) << 31),
                         "halhip_adi2d_create: grid larger than 2^31 cells");
                 Bind(ctx);
@@ -1054,9 +1544,16 @@ extern "C"
                 p->ctx = ctx;
                 p->X = xDim;
                 p->Y = yDim;
-                p->shape = AutoShape(xDim, yDim);
                 const size_t n = Cells(p);
                 p->field.Allocate(ctx, n);
+                if (yDim > HALHIP_ADI_MAX_Y)
+                {
+                    ConfigureSegments(p, true);
+                }
+                else
+                {
+                    p->shape = AutoShape(xDim, yDim);
+                }
                 AllocateStrips(p);
                 p->colInfo.Allocate(ctx, 3 * static_cast<size_t>(yDim));
                 p->clock.Create();
@@ -1277,6 +1774,19 @@ extern "C"
             [&]
             {
                 RequirePlan(p, "halhip_adi2d_set_strip_shape");
+                if (p->segG > 0)
+                {
+                    // Segmented rows: 4 columns per thread; the strip height is free.
+                    Require(columnsPerThread == kSegCpt && (stripRows == 4 || stripRows == 8),
+                            "halhip_adi2d_set_strip_shape: segmented rows take (4, 4) or (4, 8)");
+                    Require(!vector || p->Y % kSegCpt == 0,
+                            "halhip_adi2d_set_strip_shape: vector rows need yDim to be a multiple of 4");
+                    Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                    p->shape.s = stripRows;
+                    p->shape.vec = vector != 0;
+                    AllocateStrips(p);
+                    return;
+                }
                 Require(ShapeCompiled(columnsPerThread, stripRows),
                         "halhip_adi2d_set_strip_shape: (columnsPerThread, stripRows) must be one of "
                         "{1,2,4}x{4,8,16} or 8x{4,8}");
@@ -1299,11 +1809,27 @@ extern "C"
             [&]
             {
                 RequirePlan(p, "halhip_adi2d_set_methods");
-                Require(rowSolver >= HALHIP_ADI_AUTO && rowSolver <= HALHIP_ADI_ROWS_LDS,
+                Require(rowSolver >= HALHIP_ADI_AUTO && rowSolver <= HALHIP_ADI_ROWS_SEGMENTED,
                         "halhip_adi2d_set_methods: unknown row solver");
                 Require(carries >= HALHIP_ADI_AUTO && carries <= HALHIP_ADI_CARRIES_WINDOW,
                         "halhip_adi2d_set_methods: unknown carry method");
+                const bool longRows = p->Y > HALHIP_ADI_MAX_Y;
+                if (longRows && rowSolver != HALHIP_ADI_AUTO && rowSolver != HALHIP_ADI_ROWS_SEGMENTED)
+                {
+                    Fail(HALHIP_ERR_UNSUPPORTED,
+                         "halhip_adi2d_set_methods: rows longer than HALHIP_ADI_MAX_Y are always segmented");
+                }
+                if (longRows && carries == HALHIP_ADI_CARRIES_WINDOW)
+                {
+                    Fail(HALHIP_ERR_UNSUPPORTED, "halhip_adi2d_set_methods: segmented rows take scan carries");
+                }
                 Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                const bool seg = longRows || rowSolver == HALHIP_ADI_ROWS_SEGMENTED;
+                if (seg != (p->segG > 0))
+                {
+                    ConfigureSegments(p, seg);
+                    AllocateStrips(p);
+                }
                 p->wantRow = rowSolver;
                 p->wantCarry = carries;
                 p->tableKey.clear();

@@ -695,8 +695,11 @@ namespace
         }
     }
 
-    // Sweep B solve of every row of the strip across the workgroup.
-    template <int CPT, int S>
+    // Sweep B solve of every row of the strip across the workgroup.  SEG:
+    // the workgroup holds one segment of longer rows, solved with zero
+    // carries at both of its ends and without the periodic closure (the
+    // segments are joined afterwards).
+    template <int CPT, int S, bool SEG = false>
     __device__ __forceinline__ void SolveB(const AdiArgs &a, int y0, int tid, float (&v)[S][CPT], const StripLds &L)
     {
         const int lane = tid & 63;
@@ -821,7 +824,7 @@ namespace
                 acc = fmaf(bw[c], acc, ib[c] * v[r][c]);
                 v[r][c] = acc;
             }
-            if (a.wrapY)
+            if (!SEG && a.wrapY)
             {
                 if (tid == 0)
                 {
@@ -840,7 +843,7 @@ namespace
                 }
             }
         }
-        if (a.wrapY)
+        if (!SEG && a.wrapY)
         {
             // Sherman-Morrison combination (TdmaSolver.h:112-119).
             __syncthreads();

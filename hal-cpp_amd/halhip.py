@@ -34,6 +34,7 @@ REDUCE_FIELD_SUM = 4
 ADI_AUTO = 0
 ADI_ROWS_WORKGROUP_SCAN = 1
 ADI_ROWS_LDS = 2
+ADI_ROWS_SEGMENTED = 3
 ADI_CARRIES_SCAN = 1
 ADI_CARRIES_WINDOW = 2
 

@@ -251,8 +251,12 @@ int halhip_diffusion2d_profile(halhip_diffusion2d *plan, int32_t maxKernels, int
 
 /* Replaces Hal::Gpu::AdiKernel2D / GpuAdiDiffusionBackend2D
  * (include/Hal/Gpu/AdiKernel2D.h:33-41, src/Gpu/GpuAdiDiffusionBackend2D.cpp:24-50).
- * Limits: yDim <= HALHIP_ADI_MAX_Y (the contiguous axis is solved inside one
- * workgroup); xDim unbounded.  The reference caps both axes at 1024. */
+ * Limits: xDim * yDim <= 2^31; no cap on either axis (the reference's GPU
+ * path caps both at 1024, its CPU path has none,
+ * include/Hal/Pde/TdmaSolver.h:123-130).  Rows of up to HALHIP_ADI_MAX_Y
+ * columns are solved inside one workgroup; longer rows are cut into
+ * 4096-column segments across workgroups and joined exactly
+ * (HALHIP_ADI_ROWS_SEGMENTED), at about twice the traffic per tick. */
 #define HALHIP_ADI_MAX_Y 8192
 int halhip_adi2d_create(halhip_context *ctx, int32_t xDim, int32_t yDim, halhip_adi2d **out);
 int halhip_adi2d_destroy(halhip_adi2d *plan);
@@ -297,8 +301,9 @@ int halhip_adi2d_prepare_steps(halhip_adi2d *plan, int32_t steps, double diffRat
 /* Strip decomposition of the plan: each workgroup holds `stripRows` full rows
  * with `columnsPerThread` adjacent columns per thread (1, 2 or 4 with 4, 8 or
  * 16 rows; 8 with 4 or 8 rows); `vector` moves rows as 16-byte vectors and
- * needs yDim % columnsPerThread == 0.  Results do not depend on the shape
- * beyond FP32 rounding.  The default is chosen from the grid size. */
+ * needs yDim % columnsPerThread == 0.  Segmented rows take 4 columns per
+ * thread with 4 or 8 rows.  Results do not depend on the shape beyond FP32
+ * rounding.  The default is chosen from the grid size. */
 int halhip_adi2d_set_strip_shape(halhip_adi2d *plan, int32_t columnsPerThread, int32_t stripRows, int32_t vector);
 int halhip_adi2d_get_strip_shape(halhip_adi2d *plan, int32_t *columnsPerThread, int32_t *stripRows, int32_t *vector);
 
@@ -308,7 +313,13 @@ int halhip_adi2d_get_strip_shape(halhip_adi2d *plan, int32_t *columnsPerThread, 
  *     scan across the workgroup's registers; HALHIP_ADI_ROWS_LDS stages the
  *     strip in LDS and solves each line with constant Thomas factors plus an
  *     exact rank-2 correction for its ends (or, on a periodic y axis, its
- *     corners); supported shapes only.
+ *     corners); supported shapes only.  HALHIP_ADI_ROWS_SEGMENTED cuts the
+ *     rows into segments across workgroups, each solved with zero carries,
+ *     then joins them with the segments' exact responses to their carries
+ *     (4096-column segments; forced on rows of up to HALHIP_ADI_MAX_Y it
+ *     uses 256-column segments, for testing).  Rows longer than
+ *     HALHIP_ADI_MAX_Y are always segmented, with scan carries; asking them
+ *     for another row solver or windowed carries is HALHIP_ERR_UNSUPPORTED.
  *   carries: HALHIP_ADI_CARRIES_SCAN runs a cross-strip scan kernel per tick;
  *     HALHIP_ADI_CARRIES_WINDOW composes each strip's carries from the K
  *     strips on either side when the line factors decay fast enough that the
@@ -320,6 +331,7 @@ int halhip_adi2d_get_strip_shape(halhip_adi2d *plan, int32_t *columnsPerThread, 
 #define HALHIP_ADI_AUTO 0
 #define HALHIP_ADI_ROWS_WORKGROUP_SCAN 1
 #define HALHIP_ADI_ROWS_LDS 2
+#define HALHIP_ADI_ROWS_SEGMENTED 3
 #define HALHIP_ADI_CARRIES_SCAN 1
 #define HALHIP_ADI_CARRIES_WINDOW 2
 int halhip_adi2d_set_methods(halhip_adi2d *plan, int32_t rowSolver, int32_t carries);
