@@ -97,6 +97,41 @@ def bench_adi_model(ctx, w, ticks=200):
     return out
 
 
+def bench_long_rows(ctx, ticks=40, cpu=True, X=4096, Y=16384):
+    """ADI on rows longer than one workgroup: the blob workload on a 4096 x
+    16384 grid, Dirichlet 0, r = 5.  Rows are cut into 4096-column segments
+    (reduce, cross-strip scan, segment solve, per-row segment scan and
+    fix-up launches per tick).  Priced at the same 16 B/cell-tick as the
+    whole-row tick."""
+    import halhip
+    import workloads
+    w = workloads.adi_blobs(n=X, ny=Y)
+    plan = halhip.Adi2D(ctx, X, Y)
+    plan.upload(w["field"])
+    plan.run_steps(5, w["rate"], 0.0, False, False, bc=w["bc"])
+    plan.prepare_steps(ticks, w["rate"], 0.0, False, False, bc=w["bc"])
+    plan.sync()
+    plan.run_steps(ticks, w["rate"], 0.0, False, False, bc=w["bc"], asynchronous=True)
+    ms = plan.sync()
+    plan.set_profiling(True)
+    plan.run_steps(10, w["rate"], 0.0, False, False, bc=w["bc"])
+    prof = plan.profile()
+    plan.set_profiling(False)
+    cells = X * Y
+    kernel_ms = sum(v[0] for v in prof.values()) / 10
+    out = {"workload": f"ADI {X}x{Y} (rows of {Y} columns, 4096-column segments), D=5, Dirichlet 0, "
+                       f"{ticks} ticks",
+           "ms_per_tick": round(ms / ticks, 5), "Gcell_s": round(cells * ticks / (ms * 1e-3) / 1e9, 2),
+           "kernels_us_per_launch": {k: round(v[0] / v[1] * 1e3, 2) for k, v in prof.items()},
+           "kernel_ms_per_tick": round(kernel_ms, 5),
+           "roofline": {"bound": "hbm", "achieved": round(BYTES_PER_CELL_TICK * cells / (kernel_ms * 1e-3) / 1e9, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(BYTES_PER_CELL_TICK * cells / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+           "methods": list(plan.methods()), "strip_shape": list(plan.strip_shape())}
+    plan.close()
+    return out
+
+
 def bench_explicit(ctx, ticks=1000, cpu=True, shape=None):
     """Explicit 256^2 (workloads.explicit_256): r = 0.2, decay 1e-3, centre
     source 1.0, periodic, 1000 ticks resident; 8 B/cell-tick algorithmic."""
@@ -455,7 +490,8 @@ def main():
                     help="periodic axes as WRAPX,WRAPY (0|1); a non-periodic axis keeps the Dirichlet 0 edges")
     ap.add_argument("--explicit-shape", default=None,
                     help="rows per workgroup and ticks per launch of the explicit companion as B,T")
-    ap.add_argument("--companion", choices=["explicit", "agents", "batched", "adjoint", "per_call"], default=None,
+    ap.add_argument("--companion", choices=["explicit", "agents", "batched", "adjoint", "per_call", "long_rows"],
+                    default=None,
                     help="run only this companion workload and print its JSON (for profiling)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run the copy kernel (read + write this many MiB, 10 launches) for counter calibration")
@@ -480,7 +516,7 @@ def main():
     if args.companion:
         ctx = halhip.Context(0)
         run = {"explicit": bench_explicit, "agents": bench_agents, "batched": bench_batched,
-               "adjoint": bench_adjoint, "per_call": bench_per_call}[args.companion]
+               "adjoint": bench_adjoint, "per_call": bench_per_call, "long_rows": bench_long_rows}[args.companion]
         kwargs = {}
         if args.companion == "explicit" and args.explicit_shape:
             kwargs["shape"] = tuple(int(v) for v in args.explicit_shape.split(","))
@@ -543,6 +579,7 @@ def main():
         extras["batched"] = bench_batched(ctx, cpu=cpu)
         extras["adjoint"] = bench_adjoint(ctx, cpu=cpu)
         extras["per_call"] = bench_per_call(ctx, cpu=cpu)
+        extras["long_rows"] = bench_long_rows(ctx, cpu=cpu)
 
     base = None
     if not args.no_cpu_baseline and headline:

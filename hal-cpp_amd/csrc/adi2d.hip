@@ -448,26 +448,24 @@ namespace
     }
 
     // u' = x0 + Q xb + R cf (- factor z on a periodic y axis) (- k u with
-    // decay), in place over the field.  V cells per thread: 4 when yDim is a
-    // multiple of 4 (a group then never straddles a row or a segment).
+    // decay), in place over the field.  Block row blockIdx.x is grid row x;
+    // each thread takes V consecutive cells (V = 4 when yDim is a multiple
+    // of 4: a group then never straddles a segment), blocks of 256 threads
+    // stride along the row.
     template <int V, bool DECAY>
     __global__ __launch_bounds__(256) void AdiSegFix(AdiArgs a, SegArgs sg, float *__restrict__ u)
     {
-        const int64_t n = static_cast<int64_t>(a.X) * a.Y / V;
-        for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
-             t += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        const int x = static_cast<int>(blockIdx.x);
+        const size_t rowBase = static_cast<size_t>(x) * a.Y;
+        const float fac = a.wrapY ? sg.factor[x] : 0.0f;
+        for (int y = (static_cast<int>(blockIdx.y) * 256 + static_cast<int>(threadIdx.x)) * V; y < a.Y;
+             y += static_cast<int>(gridDim.y) * 256 * V)
         {
-            const int64_t i = t * V;
-            const int x = static_cast<int>(i / a.Y);
-            const int y = static_cast<int>(i - static_cast<int64_t>(x) * a.Y);
             const size_t k = static_cast<size_t>(y / sg.W) * a.X + x;
             const float xb = sg.xb[k];
             const float cf = sg.cf[k];
-            const float fac = a.wrapY ? sg.factor[x] : 0.0f;
+            const size_t i = rowBase + static_cast<size_t>(y);
             float v[V];
-            float q[V];
-            float rr[V];
-            float z[V];
             float old[V];
             // The field and x0 move as vectors; the tables (at any float
             // offset of the plan's table buffer) load per element.
@@ -492,15 +490,8 @@ namespace
 #pragma unroll
             for (int c = 0; c < V; c++)
             {
-                q[c] = sg.q[y + c];
-                rr[c] = sg.rr[y + c];
-                z[c] = a.yz[y + c];
-            }
-#pragma unroll
-            for (int c = 0; c < V; c++)
-            {
-                float r = fmaf(q[c], xb, fmaf(rr[c], cf, v[c]));
-                r = fmaf(-fac, z[c], r);
+                float r = fmaf(sg.q[y + c], xb, fmaf(sg.rr[y + c], cf, v[c]));
+                r = fmaf(-fac, a.yz[y + c], r);
                 if constexpr (DECAY)
                 {
                     r = fmaf(old[c], a.negDecay, r);
@@ -1311,13 +1302,14 @@ namespace
                            p->clock.stream, p->args, p->seg);
         Check(hipGetLastError(), "AdiSegScan launch");
         p->prof.End(p->clock.stream);
-        const int64_t groups = static_cast<int64_t>(Cells(p)) / (p->shape.vec ? 4 : 1);
-        const unsigned blocks = static_cast<unsigned>(std::min<int64_t>((groups + 255) / 256, 16384));
+        const int32_t perBlock = 256 * (p->shape.vec ? 4 : 1);
+        const unsigned chunks = static_cast<unsigned>(std::min<int32_t>((p->Y + perBlock - 1) / perBlock, 65535));
         using FixKernel = void (*)(AdiArgs, SegArgs, float *);
         const FixKernel fix = p->shape.vec ? (decay ? AdiSegFix<4, true> : AdiSegFix<4, false>)
                                            : (decay ? AdiSegFix<1, true> : AdiSegFix<1, false>);
         p->prof.Begin(p->clock.stream, "adi_seg_fix");
-        hipLaunchKernelGGL(fix, dim3(blocks), dim3(256), 0, p->clock.stream, p->args, p->seg, p->field.get());
+        hipLaunchKernelGGL(fix, dim3(static_cast<unsigned>(p->X), chunks), dim3(256), 0, p->clock.stream, p->args,
+                           p->seg, p->field.get());
         Check(hipGetLastError(), "AdiSegFix launch");
         p->prof.End(p->clock.stream);
         if (sources)

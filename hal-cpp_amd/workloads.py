@@ -19,22 +19,24 @@ def explicit_256(seed=42, n=256):
                 src_idx=np.array([c], np.int64), src_val=np.array([1.0], np.float32))
 
 
-def adi_blobs(n=4096, blobs=64, sigma=20.0, seed=42):
-    """ADI workload: sum of `blobs` Gaussian blobs.  Per blob, in order from
-    Rng(seed): cx = n*U, cy = n*U, A = 0.5 + 0.5*U; value
-    A*exp(-((x-cx)^2 + (y-cy)^2) / (2 sigma^2)) at integer cells, truncated at
-    |dx|, |dy| <= 8 sigma.  Dirichlet boundary value 0, rate 5, 200 ticks."""
+def adi_blobs(n=4096, blobs=64, sigma=20.0, seed=42, ny=None):
+    """ADI workload: sum of `blobs` Gaussian blobs on an n x ny grid (ny =
+    n by default).  Per blob, in order from Rng(seed): cx = n*U, cy = ny*U,
+    A = 0.5 + 0.5*U; value A*exp(-((x-cx)^2 + (y-cy)^2) / (2 sigma^2)) at
+    integer cells, truncated at |dx|, |dy| <= 8 sigma.  Dirichlet boundary
+    value 0, rate 5, 200 ticks."""
+    ny = n if ny is None else ny
     u = halhip.rng_doubles(seed, 3 * blobs)
-    field = np.zeros((n, n), np.float64)
+    field = np.zeros((n, ny), np.float64)
     half = int(round(8 * sigma))
     for b in range(blobs):
-        cx, cy, amp = n * u[3 * b], n * u[3 * b + 1], 0.5 + 0.5 * u[3 * b + 2]
+        cx, cy, amp = n * u[3 * b], ny * u[3 * b + 1], 0.5 + 0.5 * u[3 * b + 2]
         x0, x1 = max(0, int(np.floor(cx)) - half), min(n, int(np.floor(cx)) + half + 1)
-        y0, y1 = max(0, int(np.floor(cy)) - half), min(n, int(np.floor(cy)) + half + 1)
+        y0, y1 = max(0, int(np.floor(cy)) - half), min(ny, int(np.floor(cy)) + half + 1)
         xs = np.arange(x0, x1, dtype=np.float64)[:, None]
         ys = np.arange(y0, y1, dtype=np.float64)[None, :]
         field[x0:x1, y0:y1] += amp * np.exp(-((xs - cx) ** 2 + (ys - cy) ** 2) / (2 * sigma * sigma))
-    return dict(X=n, Y=n, field=field.astype(np.float32).ravel(), rate=5.0, bc=0.0, wrapX=False, wrapY=False)
+    return dict(X=n, Y=ny, field=field.astype(np.float32).ravel(), rate=5.0, bc=0.0, wrapX=False, wrapY=False)
 
 
 def agents_uniform(n_agents=4_000_000, dim=2048, seed=42):
