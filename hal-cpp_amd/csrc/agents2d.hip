@@ -205,7 +205,7 @@ namespace
                                                              int32_t *__restrict__ nId, float2 *__restrict__ nPos,
                                                              float *__restrict__ nRad, float2 *__restrict__ nCarry)
     {
-        const int k = blockIdx.x * blockDim.x + threadIdx.x;
+        const int k = XcdContiguousBlock(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
         if (k >= N)
         {
             return;
@@ -265,7 +265,8 @@ namespace
                                                                 int32_t *__restrict__ id, float2 *__restrict__ pos,
                                                                 float *__restrict__ rad, float2 *__restrict__ carry)
     {
-        const int64_t c0 = 4 * (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
+        const int64_t c0 =
+            4 * (static_cast<int64_t>(XcdContiguousBlock(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x);
         if (c0 >= C)
         {
             return;
@@ -481,7 +482,9 @@ namespace
                                                             const float *__restrict__ srad, float2 *__restrict__ vel,
                                                             MoveArgs move)
     {
-        const int k = blockIdx.x * blockDim.x + threadIdx.x;
+        // Each XCD takes a contiguous range of slots: the neighbour rows a
+        // slot reads (about Y slots away) then sit in the same L2.
+        const int k = XcdContiguousBlock(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
         if (k >= N)
         {
             return;

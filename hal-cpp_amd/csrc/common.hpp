@@ -24,6 +24,19 @@ struct halhip_context
 
 namespace halhip
 {
+    // Workgroup b of a launch of n goes to XCD b % 8 (the dispatcher deals
+    // workgroups to the 8 XCDs in turn).  This maps b to a block index such
+    // that each XCD runs one contiguous range of [0, n) (a bijection for any
+    // n), so work items that share data (neighbouring slots, rows or cells)
+    // meet in one XCD's L2.
+    __device__ __forceinline__ int XcdContiguousBlock(int b, int n)
+    {
+        const int j = b & 7;
+        const int q = n >> 3;
+        const int rem = n & 7;
+        return j * q + (j < rem ? j : rem) + (b >> 3);
+    }
+
     // Error carried from the implementation to the C ABI, where guard()
     // turns it into a status code plus the thread's last-error string.
     struct Error : std::exception
