@@ -841,6 +841,9 @@ struct halhip_adi2d
     SegArgs seg{};
     HostStaging<float> hostBuf; // pinned staging of the per-call FP64 paths
     ChunkedTransfer chunks;
+    // The per-call DiffuseADI's field (allocated on its first call): the
+    // call swaps it in for the tick, so the resident field survives.
+    DeviceArray<float> callField;
 };
 
 namespace
@@ -1706,8 +1709,26 @@ extern "C"
                 Require(field != nullptr && deltas != nullptr, "halhip_adi2d_diffuse_f64: null array");
                 const size_t cells = Cells(p);
                 p->hostBuf.resize(cells);
-                p->chunks.Upload(field, cells, p->hostBuf.data(), p->field.get(), p->clock.stream);
+                if (p->callField.get() == nullptr)
+                {
+                    p->callField.Allocate(p->ctx, cells);
+                }
+                // One ungraphed tick on the call's own field; the swap back
+                // runs on every exit, and the reductions left behind belong
+                // to the call's field, so the resident field's are redone.
+                struct SwapBack
+                {
+                    halhip_adi2d *plan;
+                    ~SwapBack()
+                    {
+                        plan->field.SwapWith(plan->callField);
+                        plan->reducedValid = false;
+                    }
+                };
+                p->field.SwapWith(p->callField);
                 p->reducedValid = false;
+                SwapBack restore{p};
+                p->chunks.Upload(field, cells, p->hostBuf.data(), p->field.get(), p->clock.stream);
                 EnqueueTicks(p, 1, diffRate, 0.0, wrapX, wrapY, bc, false);
                 p->chunks.DownloadAdd(p->field.get(), cells, p->hostBuf.data(), deltas, field, p->clock.stream);
                 p->prof.Collect();

@@ -264,7 +264,9 @@ int halhip_adi2d_destroy(halhip_adi2d *plan);
 /* IAdiDiffusionBackend2D::DiffuseADI (include/Hal/Pde/IAdiDiffusionBackend2D.h:25-31):
  * deltas[i] += ADI(field)[i] - field[i].  bc may be null (zero flux /
  * periodic); a Dirichlet bc gives PDEGrid2D::DiffusionADI(rate, bc)
- * (include/Hal/Grids/PDEGrid2D.h:328-347). */
+ * (include/Hal/Grids/PDEGrid2D.h:328-347).  The call works on a field
+ * buffer of its own (allocated on the first call): the plan's resident field
+ * is left as it was. */
 int halhip_adi2d_diffuse_f64(halhip_adi2d *plan, const double *field, double *deltas, int64_t n, double diffRate,
                              int32_t wrapX, int32_t wrapY, const halhip_bc *bc);
 

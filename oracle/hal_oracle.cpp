@@ -27,9 +27,13 @@
 //                           the uniform-velocity form in the interior
 //   oracle_model_ticks      golden tick compositions
 //   oracle_adi_cell_*       per-cell-diffusivity ADI and its adjoint: no
-//                           reference counterpart.  Pinned by "uniform D ==
-//                           oracle_adi", the dot-product test <Ax,y>=<x,A^T y>
-//                           and FP64 central finite differences.
+//                           reference counterpart.  Pinned, for every boundary
+//                           kind, by "uniform D == oracle_adi", the explicit
+//                           half-step against the per-cell-rate stencil
+//                           oracle_diffuse_rates, the dot-product test
+//                           <Ax,y>=<x,A^T y> and FP64 central finite
+//                           differences; at full size by the committed
+//                           central-difference fixture.
 #include <cmath>
 #include <algorithm>
 #include <cstdint>
@@ -986,107 +990,210 @@ extern "C"
 
 // ---- per-cell-diffusivity ADI and its adjoint (no reference counterpart) ----
 //
-// Forward model, generalising Diffusion2DADI (include/Hal/Pde/ADIequations.h:306-337)
+// Forward model, generalising Diffusion2DADI (include/Hal/Pde/ADIequations.h:306-378)
 // to a per-cell rate D with the face convention of the per-cell explicit
-// stencil (src/Pde/PDEequations.cpp:214-219): the face between cells i and
-// i+1 along an axis carries D_i.  Zero-flux boundaries.  With
-//   (L_x u)_i = D_i (u_{i+1} - u_i) + D_{i-1} (u_{i-1} - u_i)   (absent faces dropped)
-// a tick is
-//   s1 = (I + L_y/2) u ;  s2 = (I - L_x/2)^{-1} s1 ;
-//   s3 = (I + L_x/2) s2 ; s4 = (I - L_y/2)^{-1} s3 ;  u' = s4 - k u,
-// which for uniform D = r is exactly the reference's zero-flux ADI tick plus
-// MulAll(-k).  The loss is J = 1/2 |u_T - target|^2.
+// stencil (src/Pde/PDEequations.cpp:199-228): the face between cells i and
+// i+1 along an axis carries D_i, and on a wrapped axis the face between the
+// last and the first cell carries D_{n-1}.  Boundaries per axis, as in the
+// reference's ADI: periodic on a wrapped axis; otherwise zero flux, or
+// Dirichlet with ghost values bc (the reference's DeltaX2D ghost rule
+// (bc - c)*2, include/Hal/Pde/PDEequations.h:50-69), whose face carries the
+// boundary cell's own D.  With
+//   (L_x u)_i = D_i (u_{i+1} - u_i) + D_{i-1} (u_{i-1} - u_i)
+//               + [Dirichlet end] 2 D_i (bc - u_i)        (absent faces dropped)
+// (affine: L u = A u + g) a tick is
+//   s1 = (I + L_y/2) u ;  s2 = (I - A_x/2)^{-1} (s1 + g_x/2) ;
+//   s3 = (I + L_x/2) s2 ; s4 = (I - A_y/2)^{-1} (s3 + g_y/2) ;  u' = s4 - k u,
+// which for uniform D = r is exactly the reference's ADI tick (diagonal
+// 1 + 3r/2 and right-hand side r*bc at implicit Dirichlet ends, the cyclic
+// line with corners -r/2 on a wrapped axis) plus MulAll(-k).  The loss is
+// J = 1/2 |u_T - target|^2.
 namespace
 {
-    // Line solve (I - L/2) x = d along one line of a strided field:
-    // element j of the line is at base + j*stride; the line's face rates are
-    // Dline[j] (face j..j+1) at the same addressing.
-    void CellLineSolve(int32_t n, const double *D, size_t base, size_t stride, const double *d, double *x,
-                       std::vector<double> &work)
+    // Boundary description of the per-cell model: wrap flags and, for a
+    // Dirichlet grid, the ghost values as four edge arrays
+    // [xlo(Y) | xhi(Y) | ylo(X) | yhi(X)] (bc(-1,y), bc(X,y), bc(x,-1), bc(x,Y));
+    // edges == nullptr is zero flux on the non-wrapped axes.
+    struct CellBounds
     {
-        work.resize(3 * static_cast<size_t>(n));
-        double *a = work.data();
-        double *b = a + n;
-        double *c = b + n;
+        bool wrapX = false;
+        bool wrapY = false;
+        const double *edges = nullptr;
+    };
+
+    // Ghost value beyond cell (x, y) on the low (hi = false) or high end of axis.
+    inline double Ghost(const CellBounds &b, int32_t X, int32_t Y, int axis, bool hi, int32_t x, int32_t y)
+    {
+        if (axis == 0)
+        {
+            return b.edges[(hi ? Y : 0) + y];
+        }
+        return b.edges[2 * Y + (hi ? X : 0) + x];
+    }
+
+    // Line solve (I - A/2) x = d along one line of a strided field: element j
+    // of the line is at base + j*stride; the line's face rates are D at the
+    // same addressing.  A periodic line is cyclic (corners -D_{n-1}/2,
+    // include/Hal/Pde/TdmaSolver.h:68-120); a Dirichlet line adds D_j to the
+    // diagonal of each end (its ghost face 2 D_j, halved).
+    void CellLineSolve(int32_t n, const double *D, size_t base, size_t stride, const double *d, double *x,
+                       bool wrap, bool dirichlet, std::vector<double> &work)
+    {
+        if (wrap && n == 1)
+        {
+            // the wrap face of a one-cell line joins the cell to itself
+            x[base] = d[base];
+            return;
+        }
+        std::vector<double> a(static_cast<size_t>(n)), b(static_cast<size_t>(n)), c(static_cast<size_t>(n));
         for (int32_t j = 0; j < n; j++)
         {
-            const double dm = (j > 0) ? D[base + (j - 1) * stride] : 0.0;
-            const double dp = (j < n - 1) ? D[base + j * stride] : 0.0;
+            double dm = (j > 0) ? D[base + (j - 1) * stride] : 0.0;
+            double dp = (j < n - 1) ? D[base + j * stride] : 0.0;
+            double diag = 0.5 * (dm + dp);
+            if (wrap)
+            {
+                // the wrap face (n-1 .. 0) carries D_{n-1}
+                const double dw = D[base + (n - 1) * stride];
+                if (j == 0)
+                {
+                    diag += 0.5 * dw;
+                    dm = dw;
+                }
+                if (j == n - 1)
+                {
+                    diag += 0.5 * dw;
+                    dp = dw;
+                }
+            }
+            else if (dirichlet)
+            {
+                const double own = D[base + j * stride];
+                diag += (j == 0 ? own : 0.0) + (j == n - 1 ? own : 0.0);
+            }
             a[j] = -0.5 * dm;
             c[j] = -0.5 * dp;
-            b[j] = 1.0 + 0.5 * (dm + dp);
+            b[j] = 1.0 + diag;
         }
-        std::vector<double> dd(static_cast<size_t>(n)), xx(static_cast<size_t>(n)), w2;
+        std::vector<double> dd(static_cast<size_t>(n)), xx(static_cast<size_t>(n));
         for (int32_t j = 0; j < n; j++)
         {
             dd[j] = d[base + j * stride];
         }
-        Thomas(n, a, b, c, dd.data(), xx.data(), w2);
+        if (wrap)
+        {
+            ThomasPeriodic(n, a.data(), b.data(), c.data(), dd.data(), xx.data(), work);
+        }
+        else
+        {
+            Thomas(n, a.data(), b.data(), c.data(), dd.data(), xx.data(), work);
+        }
         for (int32_t j = 0; j < n; j++)
         {
             x[base + j * stride] = xx[j];
         }
     }
 
-    // out = (I + sign*L/2) in along x (axis 0) or y (axis 1).
-    void CellExplicit(int32_t X, int32_t Y, const double *D, const double *in, double *out, int axis, double sign)
+    // out = in + 1/2 L in along x (axis 0) or y (axis 1); withBc = false
+    // applies the linear part A only (ghost values 0).
+    void CellExplicit(int32_t X, int32_t Y, const double *D, const double *in, double *out, int axis,
+                      const CellBounds &bd, bool withBc)
     {
+        const bool wrap = (axis == 0) ? bd.wrapX : bd.wrapY;
+        const bool dir = !wrap && bd.edges != nullptr;
+        const int32_t n = (axis == 0) ? X : Y;
         ParallelRanges(X, [&](int32_t x0, int32_t x1) { for (int32_t x = x0; x < x1; x++)
         {
             for (int32_t y = 0; y < Y; y++)
             {
                 const size_t i = static_cast<size_t>(x) * Y + y;
+                const int32_t j = (axis == 0) ? x : y;
+                const size_t step = (axis == 0) ? static_cast<size_t>(Y) : 1;
                 const double c = in[i];
                 double s = 0.0;
-                if (axis == 0)
+                if (j + 1 < n)
                 {
-                    if (x + 1 < X)
+                    s += D[i] * (in[i + step] - c);
+                }
+                else if (wrap && n > 1)
+                {
+                    s += D[i] * (in[i - (n - 1) * step] - c);
+                }
+                if (j > 0)
+                {
+                    s += D[i - step] * (in[i - step] - c);
+                }
+                else if (wrap && n > 1)
+                {
+                    s += D[i + (n - 1) * step] * (in[i + (n - 1) * step] - c);
+                }
+                if (dir)
+                {
+                    if (j == 0)
                     {
-                        s += D[i] * (in[i + Y] - c);
+                        s += 2.0 * D[i] * ((withBc ? Ghost(bd, X, Y, axis, false, x, y) : 0.0) - c);
                     }
-                    if (x > 0)
+                    if (j == n - 1)
                     {
-                        s += D[i - Y] * (in[i - Y] - c);
+                        s += 2.0 * D[i] * ((withBc ? Ghost(bd, X, Y, axis, true, x, y) : 0.0) - c);
                     }
                 }
-                else
-                {
-                    if (y + 1 < Y)
-                    {
-                        s += D[i] * (in[i + 1] - c);
-                    }
-                    if (y > 0)
-                    {
-                        s += D[i - 1] * (in[i - 1] - c);
-                    }
-                }
-                out[i] = c + sign * 0.5 * s;
+                out[i] = c + 0.5 * s;
             }
         } });
     }
 
+    // d += g/2 along the axis: D_i * bc at each Dirichlet end cell.
+    void CellImplicitSource(int32_t X, int32_t Y, const double *D, double *d, int axis, const CellBounds &bd)
+    {
+        const bool wrap = (axis == 0) ? bd.wrapX : bd.wrapY;
+        if (wrap || bd.edges == nullptr)
+        {
+            return;
+        }
+        for (int32_t x = 0; x < X; x++)
+        {
+            for (int32_t y = 0; y < Y; y++)
+            {
+                const int32_t j = (axis == 0) ? x : y;
+                const int32_t n = (axis == 0) ? X : Y;
+                const size_t i = static_cast<size_t>(x) * Y + y;
+                if (j == 0)
+                {
+                    d[i] += D[i] * Ghost(bd, X, Y, axis, false, x, y);
+                }
+                if (j == n - 1)
+                {
+                    d[i] += D[i] * Ghost(bd, X, Y, axis, true, x, y);
+                }
+            }
+        }
+    }
+
     // x-implicit (axis 0: lines along x, one per y) or y-implicit solve.
-    void CellSolve(int32_t X, int32_t Y, const double *D, const double *d, double *x, int axis)
+    void CellSolve(int32_t X, int32_t Y, const double *D, const double *d, double *x, int axis, const CellBounds &bd)
     {
         if (axis == 0)
         {
+            const bool dir = !bd.wrapX && bd.edges != nullptr;
             ParallelRanges(Y, [&](int32_t y0, int32_t y1)
             {
                 std::vector<double> work;
                 for (int32_t y = y0; y < y1; y++)
                 {
-                    CellLineSolve(X, D, static_cast<size_t>(y), static_cast<size_t>(Y), d, x, work);
+                    CellLineSolve(X, D, static_cast<size_t>(y), static_cast<size_t>(Y), d, x, bd.wrapX, dir, work);
                 }
             });
         }
         else
         {
+            const bool dir = !bd.wrapY && bd.edges != nullptr;
             ParallelRanges(X, [&](int32_t x0, int32_t x1)
             {
                 std::vector<double> work;
                 for (int32_t xx = x0; xx < x1; xx++)
                 {
-                    CellLineSolve(Y, D, static_cast<size_t>(xx) * Y, 1, d, x, work);
+                    CellLineSolve(Y, D, static_cast<size_t>(xx) * Y, 1, d, x, bd.wrapY, dir, work);
                 }
             });
         }
@@ -1094,14 +1201,16 @@ namespace
 
     // One forward tick; s2 and s4 receive the intermediate states if non-null.
     void CellTick(int32_t X, int32_t Y, const double *D, double k, const double *u, double *uNext, double *s2,
-                  double *s4)
+                  double *s4, const CellBounds &bd)
     {
         const size_t n = static_cast<size_t>(X) * Y;
         std::vector<double> t1(n), t2(n), t3(n), t4(n);
-        CellExplicit(X, Y, D, u, t1.data(), 1, 1.0);
-        CellSolve(X, Y, D, t1.data(), t2.data(), 0);
-        CellExplicit(X, Y, D, t2.data(), t3.data(), 0, 1.0);
-        CellSolve(X, Y, D, t3.data(), t4.data(), 1);
+        CellExplicit(X, Y, D, u, t1.data(), 1, bd, true);
+        CellImplicitSource(X, Y, D, t1.data(), 0, bd);
+        CellSolve(X, Y, D, t1.data(), t2.data(), 0, bd);
+        CellExplicit(X, Y, D, t2.data(), t3.data(), 0, bd, true);
+        CellImplicitSource(X, Y, D, t3.data(), 1, bd);
+        CellSolve(X, Y, D, t3.data(), t4.data(), 1, bd);
         for (size_t i = 0; i < n; i++)
         {
             uNext[i] = t4[i] - k * u[i];
@@ -1115,45 +1224,56 @@ namespace
             std::copy(t4.begin(), t4.end(), s4);
         }
     }
-}
 
-extern "C"
-{
-    // `ticks` forward ticks of the per-cell-diffusivity model; u updated in place.
-    void oracle_adi_cell_forward(double *u, const double *D, double k, int32_t ticks, int32_t X, int32_t Y)
+    CellBounds Bounds(int32_t wrapX, int32_t wrapY, const double *edges)
+    {
+        CellBounds b;
+        b.wrapX = wrapX != 0;
+        b.wrapY = wrapY != 0;
+        b.edges = edges;
+        return b;
+    }
+
+    void CellForward(double *u, const double *D, double k, int32_t ticks, int32_t X, int32_t Y, const CellBounds &bd)
     {
         const size_t n = static_cast<size_t>(X) * Y;
         std::vector<double> next(n);
         for (int32_t t = 0; t < ticks; t++)
         {
-            CellTick(X, Y, D, k, u, next.data(), nullptr, nullptr);
+            CellTick(X, Y, D, k, u, next.data(), nullptr, nullptr, bd);
             std::copy(next.begin(), next.end(), u);
         }
     }
 
-    // Transposed tick: lambdaOut = M^T lambdaIn with M the tick's linear map
-    // at fixed D (the factors are symmetric, so M^T applies them in reverse
-    // order: (I - L_y/2)^{-1}, (I + L_x/2), (I - L_x/2)^{-1}, (I + L_y/2), - k).
-    void oracle_adi_cell_tick_transpose(const double *lambdaIn, double *lambdaOut, const double *D, double k,
-                                        int32_t X, int32_t Y)
+    // Transposed tick: lambdaOut = M^T lambdaIn with M the linear part of the
+    // tick at fixed D (every factor is symmetric, so M^T applies them in
+    // reverse order: (I - A_y/2)^{-1}, (I + A_x/2), (I - A_x/2)^{-1},
+    // (I + A_y/2), - k).
+    void CellTickTranspose(const double *lambdaIn, double *lambdaOut, const double *D, double k, int32_t X,
+                           int32_t Y, const CellBounds &bd)
     {
         const size_t n = static_cast<size_t>(X) * Y;
         std::vector<double> m1(n), w(n), m2(n);
-        CellSolve(X, Y, D, lambdaIn, m1.data(), 1);
-        CellExplicit(X, Y, D, m1.data(), w.data(), 0, 1.0);
-        CellSolve(X, Y, D, w.data(), m2.data(), 0);
-        CellExplicit(X, Y, D, m2.data(), lambdaOut, 1, 1.0);
+        CellSolve(X, Y, D, lambdaIn, m1.data(), 1, bd);
+        CellExplicit(X, Y, D, m1.data(), w.data(), 0, bd, false);
+        CellSolve(X, Y, D, w.data(), m2.data(), 0, bd);
+        CellExplicit(X, Y, D, m2.data(), lambdaOut, 1, bd, false);
         for (size_t i = 0; i < n; i++)
         {
             lambdaOut[i] -= k * lambdaIn[i];
         }
     }
 
-    // Loss J = 1/2 |u_T - target|^2 after `ticks` ticks from u0, and its
-    // gradient with respect to every cell's D (gradD, X*Y) and to k (gradK),
-    // by the discrete adjoint with every intermediate stored.  Returns J.
-    double oracle_adi_cell_gradient(const double *u0, const double *D, double k, int32_t ticks, int32_t X,
-                                    int32_t Y, const double *target, double *gradD, double *gradK)
+    // J and its gradients by the discrete adjoint with every intermediate
+    // stored.  Per tick (q, p pairs (m1, s4), (m2, u) on y faces and
+    // (m1, s2), (m2, s2) on x faces, m1 = (I - A_y/2)^{-1} lam,
+    // m2 = (I - A_x/2)^{-1} (I + A_x/2) m1):
+    //   dJ/dD_c += 1/2 sum q^T (dL/dD_c) p
+    // which is -1/2 dq dp on the interior or wrap face c owns
+    // (dp = p(next) - p(c)) and q_c (bc - p_c) on each of c's Dirichlet faces;
+    //   dJ/dk  -= <lam_{t+1}, u_t>.
+    double CellGradient(const double *u0, const double *D, double k, int32_t ticks, int32_t X, int32_t Y,
+                        const double *target, double *gradD, double *gradK, const CellBounds &bd)
     {
         const size_t n = static_cast<size_t>(X) * Y;
         std::vector<std::vector<double>> u(static_cast<size_t>(ticks) + 1, std::vector<double>(n));
@@ -1162,7 +1282,7 @@ extern "C"
         std::copy(u0, u0 + n, u[0].begin());
         for (int32_t t = 0; t < ticks; t++)
         {
-            CellTick(X, Y, D, k, u[t].data(), u[t + 1].data(), s2[t].data(), s4[t].data());
+            CellTick(X, Y, D, k, u[t].data(), u[t + 1].data(), s2[t].data(), s4[t].data(), bd);
         }
         double J = 0.0;
         std::vector<double> lam(n), m1(n), w(n), m2(n), next(n);
@@ -1175,13 +1295,30 @@ extern "C"
         std::fill(gradD, gradD + n, 0.0);
         double gk = 0.0;
         std::vector<double> gkRow(static_cast<size_t>(X));
-        auto dx = [&](const std::vector<double> &p, size_t i, int32_t x) { return (x + 1 < X) ? p[i + Y] - p[i] : 0.0; };
-        auto dy = [&](const std::vector<double> &p, size_t i, int32_t y) { return (y + 1 < Y) ? p[i + 1] - p[i] : 0.0; };
+        const bool dirX = !bd.wrapX && bd.edges != nullptr;
+        const bool dirY = !bd.wrapY && bd.edges != nullptr;
+        // forward differences over the face a cell owns (0 where it owns none)
+        auto dx = [&](const std::vector<double> &p, size_t i, int32_t x)
+        {
+            if (x + 1 < X)
+            {
+                return p[i + Y] - p[i];
+            }
+            return (bd.wrapX && X > 1) ? p[i - static_cast<size_t>(X - 1) * Y] - p[i] : 0.0;
+        };
+        auto dy = [&](const std::vector<double> &p, size_t i, int32_t y)
+        {
+            if (y + 1 < Y)
+            {
+                return p[i + 1] - p[i];
+            }
+            return (bd.wrapY && Y > 1) ? p[i - static_cast<size_t>(Y - 1)] - p[i] : 0.0;
+        };
         for (int32_t t = ticks - 1; t >= 0; t--)
         {
-            CellSolve(X, Y, D, lam.data(), m1.data(), 1);
-            CellExplicit(X, Y, D, m1.data(), w.data(), 0, 1.0);
-            CellSolve(X, Y, D, w.data(), m2.data(), 0);
+            CellSolve(X, Y, D, lam.data(), m1.data(), 1, bd);
+            CellExplicit(X, Y, D, m1.data(), w.data(), 0, bd, false);
+            CellSolve(X, Y, D, w.data(), m2.data(), 0, bd);
             // per-row partial sums of dJ/dk, added in row order: the same
             // bits for any thread count
             ParallelRanges(X, [&](int32_t x0, int32_t x1)
@@ -1194,7 +1331,21 @@ extern "C"
                         const size_t i = static_cast<size_t>(x) * Y + y;
                         const double g = dy(m1, i, y) * dy(s4[t], i, y) + dx(m1, i, x) * dx(s2[t], i, x) +
                                          dx(m2, i, x) * dx(s2[t], i, x) + dy(m2, i, y) * dy(u[t], i, y);
-                        gradD[i] -= 0.5 * g;
+                        double gb = 0.0;
+                        for (int hi = 0; hi < 2; hi++)
+                        {
+                            if (dirY && y == (hi ? Y - 1 : 0))
+                            {
+                                const double bc = Ghost(bd, X, Y, 1, hi != 0, x, y);
+                                gb += m1[i] * (bc - s4[t][i]) + m2[i] * (bc - u[t][i]);
+                            }
+                            if (dirX && x == (hi ? X - 1 : 0))
+                            {
+                                const double bc = Ghost(bd, X, Y, 0, hi != 0, x, y);
+                                gb += (m1[i] + m2[i]) * (bc - s2[t][i]);
+                            }
+                        }
+                        gradD[i] += gb - 0.5 * g;
                         row -= lam[i] * u[t][i];
                     }
                     gkRow[static_cast<size_t>(x)] = row;
@@ -1204,7 +1355,7 @@ extern "C"
             {
                 gk += gkRow[static_cast<size_t>(x)];
             }
-            CellExplicit(X, Y, D, m2.data(), next.data(), 1, 1.0);
+            CellExplicit(X, Y, D, m2.data(), next.data(), 1, bd, false);
             for (size_t i = 0; i < n; i++)
             {
                 next[i] -= k * lam[i];
@@ -1213,5 +1364,59 @@ extern "C"
         }
         *gradK = gk;
         return J;
+    }
+}
+
+extern "C"
+{
+    // `ticks` forward ticks of the per-cell-diffusivity model with zero-flux
+    // boundaries; u updated in place.
+    void oracle_adi_cell_forward(double *u, const double *D, double k, int32_t ticks, int32_t X, int32_t Y)
+    {
+        CellForward(u, D, k, ticks, X, Y, CellBounds{});
+    }
+
+    // The same with wrap flags and Dirichlet ghost values (edges: null for
+    // zero flux, else [xlo(Y) | xhi(Y) | ylo(X) | yhi(X)]).
+    void oracle_adi_cell_forward_bc(double *u, const double *D, double k, int32_t ticks, int32_t X, int32_t Y,
+                                    int32_t wrapX, int32_t wrapY, const double *edges)
+    {
+        CellForward(u, D, k, ticks, X, Y, Bounds(wrapX, wrapY, edges));
+    }
+
+    // out = in + 1/2 L in along one axis (0: x, 1: y) with the ghost values
+    // (withBc = 0: the linear part only).
+    void oracle_adi_cell_explicit(const double *in, double *out, const double *D, int32_t X, int32_t Y, int32_t axis,
+                                  int32_t wrapX, int32_t wrapY, const double *edges, int32_t withBc)
+    {
+        CellExplicit(X, Y, D, in, out, axis, Bounds(wrapX, wrapY, edges), withBc != 0);
+    }
+
+    void oracle_adi_cell_tick_transpose(const double *lambdaIn, double *lambdaOut, const double *D, double k,
+                                        int32_t X, int32_t Y)
+    {
+        CellTickTranspose(lambdaIn, lambdaOut, D, k, X, Y, CellBounds{});
+    }
+
+    void oracle_adi_cell_tick_transpose_bc(const double *lambdaIn, double *lambdaOut, const double *D, double k,
+                                           int32_t X, int32_t Y, int32_t wrapX, int32_t wrapY, const double *edges)
+    {
+        CellTickTranspose(lambdaIn, lambdaOut, D, k, X, Y, Bounds(wrapX, wrapY, edges));
+    }
+
+    // Loss J = 1/2 |u_T - target|^2 after `ticks` ticks from u0, and its
+    // gradient with respect to every cell's D (gradD, X*Y) and to k (gradK).
+    // Returns J.
+    double oracle_adi_cell_gradient(const double *u0, const double *D, double k, int32_t ticks, int32_t X,
+                                    int32_t Y, const double *target, double *gradD, double *gradK)
+    {
+        return CellGradient(u0, D, k, ticks, X, Y, target, gradD, gradK, CellBounds{});
+    }
+
+    double oracle_adi_cell_gradient_bc(const double *u0, const double *D, double k, int32_t ticks, int32_t X,
+                                       int32_t Y, const double *target, double *gradD, double *gradK, int32_t wrapX,
+                                       int32_t wrapY, const double *edges)
+    {
+        return CellGradient(u0, D, k, ticks, X, Y, target, gradD, gradK, Bounds(wrapX, wrapY, edges));
     }
 }

@@ -243,32 +243,60 @@ class Oracle(_Lib):
         return pos
 
 
-    def adi_cell_forward(self, u0, D, k, ticks, X, Y):
+    @staticmethod
+    def cell_edges(bc, X, Y):
+        """Ghost values of the per-cell model as one [xlo(Y) | xhi(Y) | ylo(X) |
+        yhi(X)] array: None (zero flux), a constant, or a 4-tuple of edge arrays."""
+        if bc is None:
+            return None
+        if np.isscalar(bc):
+            return np.full(2 * X + 2 * Y, float(bc))
+        e = np.concatenate([_f(a) for a in bc])
+        assert e.size == 2 * X + 2 * Y
+        return e
+
+    def adi_cell_forward(self, u0, D, k, ticks, X, Y, wrapX=False, wrapY=False, bc=None):
         u = _f(u0).copy()
         D = _f(D)
-        f = self.fn("adi_cell_forward")
-        f.argtypes = [_dp, _dp, _f64, _i32, _i32, _i32]
-        f(_p(u), _p(D), k, ticks, X, Y)
+        e = self.cell_edges(bc, X, Y)
+        f = self.fn("adi_cell_forward_bc")
+        f.argtypes = [_dp, _dp, _f64, _i32, _i32, _i32, _i32, _i32, _dp]
+        f(_p(u), _p(D), k, ticks, X, Y, int(wrapX), int(wrapY), None if e is None else _p(e))
         return u
 
-    def adi_cell_tick_transpose(self, lam, D, k, X, Y):
+    def adi_cell_explicit(self, field, D, X, Y, axis, wrapX=False, wrapY=False, bc=None, with_bc=True):
+        """field + 1/2 L field along one axis (0: x, 1: y)."""
+        field, D = _f(field), _f(D)
+        out = np.empty_like(field)
+        e = self.cell_edges(bc, X, Y)
+        f = self.fn("adi_cell_explicit")
+        f.argtypes = [_dp, _dp, _dp, _i32, _i32, _i32, _i32, _i32, _dp, _i32]
+        f(_p(field), _p(out), _p(D), X, Y, axis, int(wrapX), int(wrapY), None if e is None else _p(e),
+          int(with_bc))
+        return out
+
+    def adi_cell_tick_transpose(self, lam, D, k, X, Y, wrapX=False, wrapY=False, bc=None):
         lam = _f(lam)
         D = _f(D)
         out = np.empty_like(lam)
-        f = self.fn("adi_cell_tick_transpose")
-        f.argtypes = [_dp, _dp, _dp, _f64, _i32, _i32]
-        f(_p(lam), _p(out), _p(D), k, X, Y)
+        e = self.cell_edges(bc, X, Y)
+        f = self.fn("adi_cell_tick_transpose_bc")
+        f.argtypes = [_dp, _dp, _dp, _f64, _i32, _i32, _i32, _i32, _dp]
+        f(_p(lam), _p(out), _p(D), k, X, Y, int(wrapX), int(wrapY), None if e is None else _p(e))
         return out
 
-    def adi_cell_gradient(self, u0, D, k, ticks, X, Y, target):
+    def adi_cell_gradient(self, u0, D, k, ticks, X, Y, target, wrapX=False, wrapY=False, bc=None):
         """(J, dJ/dD (X*Y), dJ/dk) for J = 1/2 |u_T - target|^2."""
         u0, D, target = _f(u0), _f(D), _f(target)
         gD = np.zeros(X * Y)
         gk = ctypes.c_double(0.0)
-        f = self.fn("adi_cell_gradient")
+        e = self.cell_edges(bc, X, Y)
+        f = self.fn("adi_cell_gradient_bc")
         f.restype = ctypes.c_double
-        f.argtypes = [_dp, _dp, _f64, _i32, _i32, _i32, _dp, _dp, ctypes.POINTER(ctypes.c_double)]
-        J = f(_p(u0), _p(D), k, ticks, X, Y, _p(target), _p(gD), ctypes.byref(gk))
+        f.argtypes = [_dp, _dp, _f64, _i32, _i32, _i32, _dp, _dp, ctypes.POINTER(ctypes.c_double), _i32, _i32,
+                      _dp]
+        J = f(_p(u0), _p(D), k, ticks, X, Y, _p(target), _p(gD), ctypes.byref(gk), int(wrapX), int(wrapY),
+              None if e is None else _p(e))
         return J, gD, gk.value
 
 

@@ -43,6 +43,27 @@ def test_eigenmode_through_interface(ctx):
         np.testing.assert_allclose(got, lam * f, rtol=0, atol=2e-6)
 
 
+def test_per_call_leaves_resident_field(ctx, oracle):
+    # A per-call DiffuseADI between resident runs changes neither the resident
+    # field nor the ticks that follow it: 3 + 4 resident ticks equal 7 oracle
+    # ticks, and the call's own deltas match the oracle's one tick.
+    X, Y, r = 40, 512, 2.0
+    rng = np.random.default_rng(11)
+    f = rng.random(X * Y).astype(np.float32)
+    g = rng.random(X * Y)
+    plan = halhip.Adi2D(ctx, X, Y)
+    plan.upload(f)
+    plan.run_steps(3, r, 0.0, False, False, bc=0.0)
+    d = plan.diffuse(g, np.zeros_like(g), r, False, False, 0.0)
+    plan.run_steps(4, r, 0.0, False, False, bc=0.0)
+    got = plan.download()
+    ref = adi_ref_ticks(oracle, f, 7, r, X, Y, 0, 0, bc=0.0)
+    assert rel_l2(got, ref) < 1e-5
+    want = adi_ref_ticks(oracle, g, 1, r, X, Y, 0, 0, bc=0.0) - g
+    np.testing.assert_allclose(d, want, rtol=0, atol=3e-6)
+    plan.close()
+
+
 def test_resident_eigenmode_powers(ctx):
     # tests/AdiKernel2DTests.cpp:18-68: the periodic eigenmode after N = 50
     # resident ticks at r = 0.8 is (lam_x lam_y)^N f, then ^2N (1e-3 there).

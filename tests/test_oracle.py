@@ -319,6 +319,103 @@ def test_cell_adi_gradient_vs_central_differences(oracle):
     assert abs(fdk - gk) < 1e-6 * max(1.0, abs(fdk))
 
 
+# Boundary kinds of the per-cell model: (wrapX, wrapY, bc) with bc None (zero
+# flux), a constant Dirichlet value, or "edges" (seeded edge arrays).
+CELL_BOUNDS = [(1, 1, None), (1, 0, None), (0, 1, None), (0, 0, 0.7), (1, 0, 0.4), (0, 1, -0.3),
+               (0, 0, "edges"), (1, 0, "edges")]
+
+
+def cell_bc(bc, X, Y, seed=5):
+    if isinstance(bc, str):
+        rng = np.random.default_rng(seed)
+        return (rng.random(Y), rng.random(Y), rng.random(X), rng.random(X))
+    return bc
+
+
+@pytest.mark.parametrize("wx,wy,bc", CELL_BOUNDS)
+def test_cell_adi_uniform_equals_reference_adi_every_boundary(oracle, wx, wy, bc):
+    # Uniform D reduces to the reference's ADI tick (cyclic lines, Dirichlet
+    # diagonal 1 + 3r/2 and right-hand side r*bc) + MulAll(-k).
+    rng = np.random.default_rng(21)
+    X, Y = 13, 9
+    b = cell_bc(bc, X, Y)
+    u0 = rng.random(X * Y)
+    got = oracle.adi_cell_forward(u0, np.full(X * Y, 2.5), 3e-3, 4, X, Y, wx, wy, b)
+    ref = oracle.adi_ticks(u0, 4, 2.5, 3e-3, X, Y, wx, wy, bc=b)
+    np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("wx,wy", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_cell_explicit_half_step_is_the_per_cell_rate_stencil(oracle, wx, wy):
+    # (I + L/2) p - p along one axis is half the reference's per-cell-rate
+    # stencil (src/Pde/PDEequations.cpp:199-228) with that axis's rates D and
+    # the other axis's 0: the same face convention, wrap face included.
+    rng = np.random.default_rng(22)
+    X, Y = 7, 6
+    D = 0.3 + rng.random(X * Y)
+    p = rng.random(X * Y)
+    zero = np.zeros(X * Y)
+    for axis in (0, 1):
+        got = oracle.adi_cell_explicit(p, D, X, Y, axis, wx, wy) - p
+        rx, ry = (D, zero) if axis == 0 else (zero, D)
+        want = 0.5 * oracle.diffuse_rates(p, zero, rx, ry, X, Y, wx, wy)
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("wx,wy,bc", CELL_BOUNDS)
+def test_cell_adi_tick_transpose_dot_product_every_boundary(oracle, wx, wy, bc):
+    # <M x, y> = <x, M^T y> for the linear part M x = F(x) - F(0) of the tick.
+    rng = np.random.default_rng(23)
+    X, Y = 11, 7
+    b = cell_bc(bc, X, Y)
+    D = 0.5 + rng.random(X * Y)
+    x, y = rng.random(X * Y), rng.random(X * Y)
+    F0 = oracle.adi_cell_forward(np.zeros(X * Y), D, 0.02, 1, X, Y, wx, wy, b)
+    Mx = oracle.adi_cell_forward(x, D, 0.02, 1, X, Y, wx, wy, b) - F0
+    MTy = oracle.adi_cell_tick_transpose(y, D, 0.02, X, Y, wx, wy, b)
+    assert abs(Mx @ y - x @ MTy) < 1e-12 * abs(Mx @ y)
+
+
+@pytest.mark.parametrize("wx,wy,bc", CELL_BOUNDS)
+def test_cell_adi_gradient_vs_central_differences_every_boundary(oracle, wx, wy, bc):
+    rng = np.random.default_rng(24)
+    X, Y, ticks, k = 9, 8, 10, 5e-3
+    b = cell_bc(bc, X, Y)
+    D = 0.5 + rng.random(X * Y)
+    u0 = rng.random(X * Y)
+    target = oracle.adi_cell_forward(u0, D * (1 + 0.1 * rng.random(X * Y)), k, ticks, X, Y, wx, wy, b)
+    J, gD, gk = oracle.adi_cell_gradient(u0, D, k, ticks, X, Y, target, wx, wy, b)
+    h = 1e-6
+
+    def loss(Dv, kv):
+        uT = oracle.adi_cell_forward(u0, Dv, kv, ticks, X, Y, wx, wy, b)
+        return 0.5 * np.sum((uT - target) ** 2)
+
+    assert abs(J - loss(D, k)) < 1e-14
+    # every corner, a cell of every edge, and interior cells
+    cells = [0, Y - 1, (X - 1) * Y, X * Y - 1, 3, 3 * Y, 4 * Y + Y - 1, (X - 1) * Y + 2]
+    cells += list(rng.choice(X * Y, 6, replace=False))
+    for c in cells:
+        Dp, Dm = D.copy(), D.copy()
+        Dp[c] += h
+        Dm[c] -= h
+        fd = (loss(Dp, k) - loss(Dm, k)) / (2 * h)
+        assert abs(fd - gD[c]) < 1e-6 * max(1.0, abs(fd)), (c, fd, gD[c])
+    fdk = (loss(D, k + h) - loss(D, k - h)) / (2 * h)
+    assert abs(fdk - gk) < 1e-6 * max(1.0, abs(fdk))
+
+
+def test_cell_adi_periodic_two_cell_lines(oracle):
+    # A wrapped axis of two cells has two faces between the same cells; the
+    # tick must still match the reference's cyclic solve for uniform D.
+    rng = np.random.default_rng(25)
+    X, Y = 2, 5
+    u0 = rng.random(X * Y)
+    got = oracle.adi_cell_forward(u0, np.full(X * Y, 1.5), 0.0, 3, X, Y, 1, 1)
+    ref = oracle.adi_ticks(u0, 3, 1.5, 0.0, X, Y, 1, 1)
+    np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-14)
+
+
 def test_cell_adi_gradient_at_workload_size_vs_fixture(oracle):
     """The FP64 adjoint on the 1024^2, 200-tick adjoint workload against the
     committed FP64 central differences (tests/golden/make_adjoint_fd.py)."""
