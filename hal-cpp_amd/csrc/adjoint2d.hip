@@ -50,6 +50,22 @@ namespace
                                                                       0xF, false));
     }
 
+    // One step of the wavefront scan of affine maps (A, B) and, with TWO, a
+    // second offset B2 under the same multipliers: the element CTRL brings in
+    // composes before this one.
+    template <int CTRL, int ROW_MASK, bool TWO>
+    __device__ __forceinline__ void ScanStepUp(float &A, float &B, float &B2)
+    {
+        const float tb = DppF<CTRL, ROW_MASK>(0.0f, B);
+        if constexpr (TWO)
+        {
+            const float tb2 = DppF<CTRL, ROW_MASK>(0.0f, B2);
+            B2 = fmaf(A, tb2, B2);
+        }
+        B = fmaf(A, tb, B);
+        A *= DppF<CTRL, ROW_MASK>(1.0f, A);
+    }
+
     struct LineArgs
     {
         int32_t nLines;
@@ -62,6 +78,14 @@ namespace
         const float *prev; // decay: out = x - k * prev
         float negK;
         float *stash;      // optional copy of x before decay
+        const float4 *lineK; // periodic lines: (gamma, alpha, vEnd, 0) per line
+        // Dirichlet ghost values, [lo across (L) | hi across (L) | lo along
+        // (nLines) | hi along (nLines)]: for rows, bc(-1, y), bc(X, y),
+        // bc(x, -1), bc(x, Y).
+        const float *edges;
+        int32_t wrapAcross; // RHS 0: the neighbour lines wrap
+        int32_t dirAcross;  // RHS 0: Dirichlet ghosts beyond lines 0 and nLines - 1
+        int32_t dirAlong;   // RHS 0: the implicit Dirichlet source at the line's ends
     };
 
     // Line solve: a workgroup of kBlockLine threads per line (four
@@ -132,14 +156,23 @@ namespace
         }
     }
 
-    // RHS 0: d = c + 1/2 (rate[l] (in[l+1] - c) + rate[l-1] (in[l-1] - c)) across lines.
+    // RHS 0: d = c + 1/2 (rate[l] (in[l+1] - c) + rate[l-1] (in[l-1] - c)) across
+    //        lines (wrapped, or with Dirichlet ghosts 2 rate[l] (bc - c) beyond
+    //        the first and last line), plus rate * bc at a Dirichlet line's ends.
     // RHS 2: d = in.
     // DECAY: out = x - k * prev; STASH: keeps x before the decay.
-    template <int EPL, int RHS, bool DECAY, bool STASH, bool VEC>
+    // PER: the line is cyclic.  The tables then hold the Thomas factors of
+    // the corner-adjusted matrix and the solve runs a second right-hand side
+    // (gamma at the first element, alpha at the last) through the same
+    // recurrences; the two combine by Sherman-Morrison
+    // (include/Hal/Pde/TdmaSolver.h:68-120).
+    template <int EPL, int RHS, bool DECAY, bool STASH, bool VEC, bool PER>
     __global__ __launch_bounds__(kBlockLine) void CellLineSolveBlock(LineArgs a)
     {
         __shared__ float sA[kBlockLine / 64];
         __shared__ float sB[kBlockLine / 64];
+        __shared__ float sB2[kBlockLine / 64];
+        __shared__ float sEnd[4];
         const int t = threadIdx.x;
         const int lane = t & 63;
         const int wave = t >> 6;
@@ -170,32 +203,84 @@ namespace
             {
                 s[e] = 0.0f;
             }
-            if (line + 1 < a.nLines)
+            const bool hasNext = line + 1 < a.nLines;
+            const bool hasPrev = line > 0;
+            float dl[EPL]; // this line's face rates (to line l + 1): every cell's own D
+            LoadLine<EPL, VEC>(a.rate, base, j0, L, 0.0f, dl);
+            if (hasNext || a.wrapAcross)
             {
-                float dl[EPL]; // this line's face rates (to line l + 1)
-                LoadLine<EPL, VEC>(a.rate, base, j0, L, 0.0f, dl);
-                LoadLine<EPL, VEC>(a.in, base + L, j0, L, 0.0f, nb);
+                const size_t nbase = hasNext ? base + L : 0; // wrapped: line 0
+                LoadLine<EPL, VEC>(a.in, nbase, j0, L, 0.0f, nb);
 #pragma unroll
                 for (int e = 0; e < EPL; e++)
                 {
                     s[e] += dl[e] * (nb[e] - d[e]);
                 }
             }
-            if (line > 0)
+            if (hasPrev || a.wrapAcross)
             {
+                const size_t pbase = hasPrev ? base - L : static_cast<size_t>(a.nLines - 1) * L;
                 float r[EPL];
-                LoadLine<EPL, VEC>(a.rate, base - L, j0, L, 0.0f, r);
-                LoadLine<EPL, VEC>(a.in, base - L, j0, L, 0.0f, nb);
+                LoadLine<EPL, VEC>(a.rate, pbase, j0, L, 0.0f, r);
+                LoadLine<EPL, VEC>(a.in, pbase, j0, L, 0.0f, nb);
 #pragma unroll
                 for (int e = 0; e < EPL; e++)
                 {
                     s[e] += r[e] * (nb[e] - d[e]);
                 }
             }
+            if (a.dirAcross)
+            {
+#pragma unroll
+                for (int e = 0; e < EPL; e++)
+                {
+                    const int j = j0 + e;
+                    if (j < L)
+                    {
+                        if (line == 0)
+                        {
+                            s[e] += 2.0f * dl[e] * (a.edges[j] - d[e]);
+                        }
+                        if (line == a.nLines - 1)
+                        {
+                            s[e] += 2.0f * dl[e] * (a.edges[L + j] - d[e]);
+                        }
+                    }
+                }
+            }
 #pragma unroll
             for (int e = 0; e < EPL; e++)
             {
                 d[e] = (j0 + e < L) ? fmaf(0.5f, s[e], d[e]) : 0.0f;
+            }
+            if (a.dirAlong)
+            {
+                const float lo = a.edges[2 * L + line];
+                const float hi = a.edges[2 * L + a.nLines + line];
+#pragma unroll
+                for (int e = 0; e < EPL; e++)
+                {
+                    if (j0 + e == 0)
+                    {
+                        d[e] = fmaf(dl[e], lo, d[e]);
+                    }
+                    if (j0 + e == L - 1)
+                    {
+                        d[e] = fmaf(dl[e], hi, d[e]);
+                    }
+                }
+            }
+        }
+        // PER: the second right-hand side and its solution
+        float z[PER ? EPL : 1];
+        float4 K = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if constexpr (PER)
+        {
+            K = a.lineK[line];
+#pragma unroll
+            for (int e = 0; e < EPL; e++)
+            {
+                z[e] = (j0 + e == 0) ? K.x : ((j0 + e == L - 1) ? K.y : 0.0f);
             }
         }
         // bw_j = h_j (0 at the line's end), fw_j = h_{j-1} (0 at its start;
@@ -211,51 +296,58 @@ namespace
         // Forward recurrence d'_j = fw_j d'_{j-1} + d_j, carry into the line 0.
         float A = 1.0f;
         float B = 0.0f;
+        float B2 = 0.0f;
 #pragma unroll
         for (int e = 0; e < EPL; e++)
         {
             B = fmaf(fw[e], B, d[e]);
+            if constexpr (PER)
+            {
+                B2 = fmaf(fw[e], B2, z[e]);
+            }
             A *= fw[e];
         }
-        {
-            float tb;
-            tb = DppF<0x111>(0.0f, B);
-            B = fmaf(A, tb, B);
-            A *= DppF<0x111>(1.0f, A);
-            tb = DppF<0x112>(0.0f, B);
-            B = fmaf(A, tb, B);
-            A *= DppF<0x112>(1.0f, A);
-            tb = DppF<0x114>(0.0f, B);
-            B = fmaf(A, tb, B);
-            A *= DppF<0x114>(1.0f, A);
-            tb = DppF<0x118>(0.0f, B);
-            B = fmaf(A, tb, B);
-            A *= DppF<0x118>(1.0f, A);
-            tb = DppF<0x142, 0xA>(0.0f, B);
-            B = fmaf(A, tb, B);
-            A *= DppF<0x142, 0xA>(1.0f, A);
-            tb = DppF<0x143, 0xC>(0.0f, B);
-            B = fmaf(A, tb, B);
-            A *= DppF<0x143, 0xC>(1.0f, A);
-        }
+        ScanStepUp<0x111, 0xF, PER>(A, B, B2);
+        ScanStepUp<0x112, 0xF, PER>(A, B, B2);
+        ScanStepUp<0x114, 0xF, PER>(A, B, B2);
+        ScanStepUp<0x118, 0xF, PER>(A, B, B2);
+        ScanStepUp<0x142, 0xA, PER>(A, B, B2);
+        ScanStepUp<0x143, 0xC, PER>(A, B, B2);
         if (lane == 63)
         {
             sA[wave] = A;
             sB[wave] = B;
+            sB2[wave] = B2;
         }
         __syncthreads();
         float cw = 0.0f;
+        float cw2 = 0.0f;
         for (int g = 0; g < wave; g++)
         {
             cw = fmaf(sA[g], cw, sB[g]);
+            if constexpr (PER)
+            {
+                cw2 = fmaf(sA[g], cw2, sB2[g]);
+            }
         }
         // this lane's carry: the preceding lanes' composition applied to cw
-        float acc = fmaf(DppF<0x138>(1.0f, A), cw, DppF<0x138>(0.0f, B));
+        const float Aprev = DppF<0x138>(1.0f, A);
+        float acc = fmaf(Aprev, cw, DppF<0x138>(0.0f, B));
+        float acc2 = 0.0f;
+        if constexpr (PER)
+        {
+            acc2 = fmaf(Aprev, cw2, DppF<0x138>(0.0f, B2));
+        }
 #pragma unroll
         for (int e = 0; e < EPL; e++)
         {
             acc = fmaf(fw[e], acc, d[e]);
             d[e] = acc;
+            if constexpr (PER)
+            {
+                acc2 = fmaf(fw[e], acc2, z[e]);
+                z[e] = acc2;
+            }
         }
 
         // Backward recurrence x_j = bw_j x_{j+1} + ib_j d'_j from the right.
@@ -267,10 +359,15 @@ namespace
         }
         A = 1.0f;
         B = 0.0f;
+        B2 = 0.0f;
 #pragma unroll
         for (int e = EPL - 1; e >= 0; e--)
         {
             B = fmaf(bw[e], B, ib[e] * d[e]);
+            if constexpr (PER)
+            {
+                B2 = fmaf(bw[e], B2, ib[e] * z[e]);
+            }
             A *= bw[e];
         }
 #pragma unroll
@@ -279,9 +376,18 @@ namespace
             const int o = 1 << k;
             const float Bd = __shfl_down(B, o);
             const float Ad = __shfl_down(A, o);
+            float Bd2 = 0.0f;
+            if constexpr (PER)
+            {
+                Bd2 = __shfl_down(B2, o);
+            }
             if (lane + o < 64)
             {
                 B = fmaf(A, Bd, B);
+                if constexpr (PER)
+                {
+                    B2 = fmaf(A, Bd2, B2);
+                }
                 A *= Ad;
             }
         }
@@ -290,21 +396,71 @@ namespace
         {
             sA[wave] = A;
             sB[wave] = B;
+            sB2[wave] = B2;
         }
         __syncthreads();
         cw = 0.0f;
+        cw2 = 0.0f;
         for (int g = kBlockLine / 64 - 1; g > wave; g--)
         {
             cw = fmaf(sA[g], cw, sB[g]);
+            if constexpr (PER)
+            {
+                cw2 = fmaf(sA[g], cw2, sB2[g]);
+            }
         }
         const float An = __shfl_down(A, 1);
         const float Bn = __shfl_down(B, 1);
         acc = (lane == 63) ? cw : fmaf(An, cw, Bn);
+        if constexpr (PER)
+        {
+            const float Bn2 = __shfl_down(B2, 1);
+            acc2 = (lane == 63) ? cw2 : fmaf(An, cw2, Bn2);
+        }
 #pragma unroll
         for (int e = EPL - 1; e >= 0; e--)
         {
             acc = fmaf(bw[e], acc, ib[e] * d[e]);
             d[e] = acc;
+            if constexpr (PER)
+            {
+                acc2 = fmaf(bw[e], acc2, ib[e] * z[e]);
+                z[e] = acc2;
+            }
+        }
+        if constexpr (PER)
+        {
+            // x = y - f z with f = (y_0 + vEnd y_{L-1}) / (1 + z_0 + vEnd z_{L-1})
+            const int tLast = (L - 1) / EPL;
+            const int eLast = (L - 1) - tLast * EPL;
+            if (t == 0)
+            {
+                sEnd[0] = d[0];
+                sEnd[1] = z[0];
+            }
+            if (t == tLast)
+            {
+                float yl = 0.0f;
+                float zl = 0.0f;
+#pragma unroll
+                for (int e = 0; e < EPL; e++)
+                {
+                    if (e == eLast)
+                    {
+                        yl = d[e];
+                        zl = z[e];
+                    }
+                }
+                sEnd[2] = yl;
+                sEnd[3] = zl;
+            }
+            __syncthreads();
+            const float f = fmaf(K.z, sEnd[2], sEnd[0]) / (1.0f + fmaf(K.z, sEnd[3], sEnd[1]));
+#pragma unroll
+            for (int e = 0; e < EPL; e++)
+            {
+                d[e] = fmaf(-f, z[e], d[e]);
+            }
         }
         if constexpr (STASH)
         {
@@ -333,6 +489,12 @@ namespace
         const float *ib;   // [X][Y] reciprocal pivots of the x-lines
         const float *h;    // [X][Y] (D_x / 2) ib_x of the x-lines
         float *out;        // [X][Y]
+        const float4 *lineK; // periodic x-lines: (gamma, alpha, vEnd, 0) per column
+        const float *edges;  // Dirichlet ghosts [xlo(Y) | xhi(Y) | ylo(X) | yhi(X)]
+        int32_t wrapX;       // MODE 1: the column wraps
+        int32_t wrapY;       // MODE 0: the neighbour columns wrap
+        int32_t dirX;        // Dirichlet x ends (MODE 0: implicit source; MODE 1: ghost faces at 0)
+        int32_t dirY;        // MODE 0: Dirichlet ghosts beyond columns 0 and Y - 1
     };
 
     // C consecutive floats of row x from column y0 (those past Y read 0).
@@ -370,11 +532,16 @@ namespace
     // (LDS pitch E + 1 per lane, so the lanes' element reads hit distinct
     // banks).  Rows past X are identity padding (d = D = ib = 0).
     //   MODE 0: d = (I + L_y/2) in, the explicit half-step across columns
-    //           (reads columns y0 - 1 and y0 + C too).
-    //   MODE 1: d = (I + L_x/2) in, the explicit half-step along the line.
+    //           (reads columns y0 - 1 and y0 + C too; wrapped, or with
+    //           Dirichlet ghosts beyond the first and last column), plus
+    //           rate * bc at a Dirichlet column's ends.
+    //   MODE 1: d = (I + A_x/2) in, the linear explicit half-step along the
+    //           line (ghost values 0).
+    // PER: the column is cyclic (second right-hand side and Sherman-Morrison,
+    // as in CellLineSolveBlock).
     // Column blocks are dealt so that each XCD (blocks b, b + 8, ...) takes
     // a contiguous range: blocks that share cache lines share an L2.
-    template <int C, int E, int MODE, bool VEC>
+    template <int C, int E, int MODE, bool VEC, bool PER>
     __global__ __launch_bounds__(64 * C) void CellColumnSolve(ColArgs a)
     {
         constexpr int kPitch = 64 * (E + 1); // LDS floats per column
@@ -418,9 +585,11 @@ namespace
                 LoadRow<C, VEC>(a.h, row, y0, Y, hv[i]);
                 if constexpr (MODE == 0)
                 {
-                    uL[i] = (y0 > 0) ? a.in[row - 1] : 0.0f;
-                    rL[i] = (y0 > 0) ? a.D[row - 1] : 0.0f;
-                    uR[i] = (y0 + C < Y) ? a.in[row + C] : 0.0f;
+                    // wrapped: column Y - 1 before column 0, column 0 after Y - 1
+                    const size_t rowStart = static_cast<size_t>(x) * Y;
+                    uL[i] = (y0 > 0) ? a.in[row - 1] : (a.wrapY ? a.in[rowStart + Y - 1] : 0.0f);
+                    rL[i] = (y0 > 0) ? a.D[row - 1] : (a.wrapY ? a.D[rowStart + Y - 1] : 0.0f);
+                    uR[i] = (y0 + C < Y) ? a.in[row + C] : (a.wrapY ? a.in[rowStart] : 0.0f);
                 }
             }
             else
@@ -447,13 +616,36 @@ namespace
                 for (int c = 0; c < C; c++)
                 {
                     const int y = y0 + c;
-                    const float up = (c + 1 < C) ? u[i][c + 1] : uR[i];
+                    // past column Y - 1 (wrapped) the neighbour is uR, column 0
+                    const float up = (c + 1 < C && y + 1 < Y) ? u[i][c + 1] : uR[i];
                     const float um = (c > 0) ? u[i][c - 1] : uL[i];
                     const float rm = (c > 0) ? r[i][c - 1] : rL[i];
                     float sum = 0.0f;
-                    sum += (y + 1 < Y) ? r[i][c] * (up - u[i][c]) : 0.0f;
-                    sum += (y > 0) ? rm * (um - u[i][c]) : 0.0f;
+                    sum += (y + 1 < Y || a.wrapY) ? r[i][c] * (up - u[i][c]) : 0.0f;
+                    sum += (y > 0 || a.wrapY) ? rm * (um - u[i][c]) : 0.0f;
+                    if (a.dirY && x < X && y < Y)
+                    {
+                        if (y == 0)
+                        {
+                            sum += 2.0f * r[i][c] * (a.edges[2 * Y + x] - u[i][c]);
+                        }
+                        if (y == Y - 1)
+                        {
+                            sum += 2.0f * r[i][c] * (a.edges[2 * Y + X + x] - u[i][c]);
+                        }
+                    }
                     dv[c] = (y < Y && x < X) ? fmaf(0.5f, sum, u[i][c]) : 0.0f;
+                    if (a.dirX && y < Y)
+                    {
+                        if (x == 0)
+                        {
+                            dv[c] = fmaf(r[i][c], a.edges[y], dv[c]);
+                        }
+                        if (x == X - 1)
+                        {
+                            dv[c] = fmaf(r[i][c], a.edges[Y + y], dv[c]);
+                        }
+                    }
                 }
 #pragma unroll
                 for (int c = 0; c < C; c++)
@@ -499,23 +691,31 @@ namespace
             {
                 r[e] = sR[col + me + e];
             }
-            const int xl = x0 - 1;
+            const int xl = (x0 > 0) ? x0 - 1 : X - 1; // wrapped: row X - 1 before row 0
             const int xr = x0 + E;
-            const float dL = (x0 > 0) ? sD[col + xl + xl / E] : 0.0f;
-            const float rL = (x0 > 0) ? sR[col + xl + xl / E] : 0.0f;
+            const bool left = x0 > 0 || a.wrapX;
+            const float dL = left ? sD[col + xl + xl / E] : 0.0f;
+            const float rL = left ? sR[col + xl + xl / E] : 0.0f;
             const float dR = (xr < kRows) ? sD[col + xr + xr / E] : 0.0f;
+            const float dW = sD[col]; // row 0, after row X - 1 when wrapped
             float out[E];
 #pragma unroll
             for (int e = 0; e < E; e++)
             {
                 const int x = x0 + e;
                 const float cv = d[e];
-                const float dp = (e + 1 < E) ? d[e + 1] : dR;
+                const float dp = (x + 1 < X) ? ((e + 1 < E) ? d[e + 1] : dR) : dW;
                 const float dm = (e > 0) ? d[e - 1] : dL;
                 const float rm = (e > 0) ? r[e - 1] : rL;
                 float s = 0.0f;
-                s += (x + 1 < X) ? r[e] * (dp - cv) : 0.0f;
-                s += (x > 0) ? rm * (dm - cv) : 0.0f;
+                s += (x + 1 < X || a.wrapX) ? r[e] * (dp - cv) : 0.0f;
+                s += (x > 0 || a.wrapX) ? rm * (dm - cv) : 0.0f;
+                if (a.dirX)
+                {
+                    // ghost faces 2 D (0 - c) at both ends
+                    s -= (x == 0) ? 2.0f * r[e] * cv : 0.0f;
+                    s -= (x == X - 1) ? 2.0f * r[e] * cv : 0.0f;
+                }
                 out[e] = (x < X) ? fmaf(0.5f, s, cv) : 0.0f;
             }
 #pragma unroll
@@ -531,55 +731,73 @@ namespace
         }
         const float hLeft = (x0 > 0) ? sH[col + me - 2] : 0.0f;
 
+        // PER: the second right-hand side (gamma at row 0, alpha at X - 1)
+        float z[PER ? E : 1];
+        float4 K = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if constexpr (PER)
+        {
+            K = a.lineK[min(y0 + c, Y - 1)];
+#pragma unroll
+            for (int e = 0; e < E; e++)
+            {
+                z[e] = (x0 + e == 0) ? K.x : ((x0 + e == X - 1) ? K.y : 0.0f);
+            }
+        }
+
         // forward d'_x = fw_x d'_{x-1} + d_x with fw_x = h_{x-1}
         float A = 1.0f;
         float B = 0.0f;
+        float B2 = 0.0f;
 #pragma unroll
         for (int e = 0; e < E; e++)
         {
             const float fw = (e == 0) ? hLeft : h[e - 1];
             B = fmaf(fw, B, d[e]);
+            if constexpr (PER)
+            {
+                B2 = fmaf(fw, B2, z[e]);
+            }
             A *= fw;
         }
-        {
-            float tb;
-            tb = DppF<0x111>(0.0f, B);
-            B = fmaf(A, tb, B);
-            A *= DppF<0x111>(1.0f, A);
-            tb = DppF<0x112>(0.0f, B);
-            B = fmaf(A, tb, B);
-            A *= DppF<0x112>(1.0f, A);
-            tb = DppF<0x114>(0.0f, B);
-            B = fmaf(A, tb, B);
-            A *= DppF<0x114>(1.0f, A);
-            tb = DppF<0x118>(0.0f, B);
-            B = fmaf(A, tb, B);
-            A *= DppF<0x118>(1.0f, A);
-            tb = DppF<0x142, 0xA>(0.0f, B);
-            B = fmaf(A, tb, B);
-            A *= DppF<0x142, 0xA>(1.0f, A);
-            tb = DppF<0x143, 0xC>(0.0f, B);
-            B = fmaf(A, tb, B);
-            A *= DppF<0x143, 0xC>(1.0f, A);
-        }
+        ScanStepUp<0x111, 0xF, PER>(A, B, B2);
+        ScanStepUp<0x112, 0xF, PER>(A, B, B2);
+        ScanStepUp<0x114, 0xF, PER>(A, B, B2);
+        ScanStepUp<0x118, 0xF, PER>(A, B, B2);
+        ScanStepUp<0x142, 0xA, PER>(A, B, B2);
+        ScanStepUp<0x143, 0xC, PER>(A, B, B2);
         // carry into this lane: the preceding lanes' inclusive value
         float acc = DppF<0x138>(0.0f, B);
+        float acc2 = 0.0f;
+        if constexpr (PER)
+        {
+            acc2 = DppF<0x138>(0.0f, B2);
+        }
 #pragma unroll
         for (int e = 0; e < E; e++)
         {
             const float fw = (e == 0) ? hLeft : h[e - 1];
             acc = fmaf(fw, acc, d[e]);
             d[e] = acc;
+            if constexpr (PER)
+            {
+                acc2 = fmaf(fw, acc2, z[e]);
+                z[e] = acc2;
+            }
         }
 
         // backward x_x = bw_x x_{x+1} + ib_x d'_x with bw_x = h_x (0 at X - 1)
         A = 1.0f;
         B = 0.0f;
+        B2 = 0.0f;
 #pragma unroll
         for (int e = E - 1; e >= 0; e--)
         {
             const float bw = (x0 + e < X - 1) ? h[e] : 0.0f;
             B = fmaf(bw, B, ib[e] * d[e]);
+            if constexpr (PER)
+            {
+                B2 = fmaf(bw, B2, ib[e] * z[e]);
+            }
             A *= bw;
         }
 #pragma unroll
@@ -588,20 +806,67 @@ namespace
             const int o = 1 << k;
             const float Bd = __shfl_down(B, o);
             const float Ad = __shfl_down(A, o);
+            float Bd2 = 0.0f;
+            if constexpr (PER)
+            {
+                Bd2 = __shfl_down(B2, o);
+            }
             if (lane + o < 64)
             {
                 B = fmaf(A, Bd, B);
+                if constexpr (PER)
+                {
+                    B2 = fmaf(A, Bd2, B2);
+                }
                 A *= Ad;
             }
         }
         const float Bn = __shfl_down(B, 1);
         acc = (lane == 63) ? 0.0f : Bn;
+        if constexpr (PER)
+        {
+            const float Bn2 = __shfl_down(B2, 1);
+            acc2 = (lane == 63) ? 0.0f : Bn2;
+        }
 #pragma unroll
         for (int e = E - 1; e >= 0; e--)
         {
             const float bw = (x0 + e < X - 1) ? h[e] : 0.0f;
             acc = fmaf(bw, acc, ib[e] * d[e]);
             d[e] = acc;
+            if constexpr (PER)
+            {
+                acc2 = fmaf(bw, acc2, ib[e] * z[e]);
+                z[e] = acc2;
+            }
+        }
+        if constexpr (PER)
+        {
+            // x = y - f z with f = (y_0 + vEnd y_{X-1}) / (1 + z_0 + vEnd z_{X-1}):
+            // row 0 is lane 0's first element, row X - 1 lane (X-1)/E's
+            const int lLast = (X - 1) / E;
+            const int eLast = (X - 1) - lLast * E;
+            float yl = 0.0f;
+            float zl = 0.0f;
+#pragma unroll
+            for (int e = 0; e < E; e++)
+            {
+                if (e == eLast)
+                {
+                    yl = d[e];
+                    zl = z[e];
+                }
+            }
+            const float y0v = __shfl(d[0], 0);
+            const float z0v = __shfl(z[0], 0);
+            yl = __shfl(yl, lLast);
+            zl = __shfl(zl, lLast);
+            const float f = fmaf(K.z, yl, y0v) / (1.0f + fmaf(K.z, zl, z0v));
+#pragma unroll
+            for (int e = 0; e < E; e++)
+            {
+                d[e] = fmaf(-f, z[e], d[e]);
+            }
         }
 #pragma unroll
         for (int e = 0; e < E; e++)
@@ -651,24 +916,34 @@ namespace
         double *gradD;    // [X][Y], accumulated over the ticks in FP64
         double *partialK; // one per block
         float negK;
+        const float *edges; // Dirichlet ghosts [xlo(Y) | xhi(Y) | ylo(X) | yhi(X)]
+        int32_t wrapX;
+        int32_t wrapY;
+        int32_t dirX;
+        int32_t dirY;
     };
 
-    // Per-tick gradient contributions and lam' = (I + L_y/2) m2 - k lam.  The
+    // Per-tick gradient contributions and lam' = (I + A_y/2) m2 - k lam.  The
     // decay gradient -<lam, u> cancels heavily (lam has near-zero mean), so
-    // its products and sums are formed in FP64.
+    // its products and sums are formed in FP64.  A cell's interior or wrap
+    // face adds -1/2 dq dp per (q, p) pair; each of its Dirichlet faces adds
+    // q (bc - p).
     __global__ __launch_bounds__(256) void CellGradient(GradArgs a)
     {
         __shared__ double red[4];
         const int y = blockIdx.x * 64 + (threadIdx.x & 63);
         const int x = blockIdx.y * 4 + (threadIdx.x >> 6);
+        const int X = a.X;
+        const int Y = a.Y;
         double kPart = 0.0;
-        if (x < a.X && y < a.Y)
+        if (x < X && y < Y)
         {
-            const size_t i = static_cast<size_t>(x) * a.Y + y;
-            const bool hx = x + 1 < a.X;
-            const bool hy = y + 1 < a.Y;
-            const size_t ix = hx ? i + a.Y : i;
-            const size_t iy = hy ? i + 1 : i;
+            const size_t i = static_cast<size_t>(x) * Y + y;
+            const bool hx = x + 1 < X;
+            const bool hy = y + 1 < Y;
+            // the cell's own face leads to the next cell, or wraps to the first
+            const size_t ix = hx ? i + Y : (a.wrapX ? i - static_cast<size_t>(X - 1) * Y : i);
+            const size_t iy = hy ? i + 1 : (a.wrapY ? i - static_cast<size_t>(Y - 1) : i);
             const float dyM1 = a.m1[iy] - a.m1[i];
             const float dyS4 = a.s4[iy] - a.s4[i];
             const float dxM1 = a.m1[ix] - a.m1[i];
@@ -677,19 +952,58 @@ namespace
             const float dyM2 = a.m2[iy] - a.m2[i];
             const float dyU = a.u[iy] - a.u[i];
             const float g = dyM1 * dyS4 + (dxM1 + dxM2) * dxS2 + dyM2 * dyU;
-            a.gradD[i] -= 0.5 * static_cast<double>(g);
+            const float c = a.m2[i];
+            float gb = 0.0f;
+            if (a.dirY && (y == 0 || y == Y - 1))
+            {
+                const float m1 = a.m1[i];
+                const float s4 = a.s4[i];
+                const float u = a.u[i];
+                if (y == 0)
+                {
+                    const float bc = a.edges[2 * Y + x];
+                    gb += m1 * (bc - s4) + c * (bc - u);
+                }
+                if (y == Y - 1)
+                {
+                    const float bc = a.edges[2 * Y + X + x];
+                    gb += m1 * (bc - s4) + c * (bc - u);
+                }
+            }
+            if (a.dirX && (x == 0 || x == X - 1))
+            {
+                const float m12 = a.m1[i] + c;
+                const float s2 = a.s2[i];
+                if (x == 0)
+                {
+                    gb += m12 * (a.edges[y] - s2);
+                }
+                if (x == X - 1)
+                {
+                    gb += m12 * (a.edges[Y + y] - s2);
+                }
+            }
+            a.gradD[i] += static_cast<double>(gb) - 0.5 * static_cast<double>(g);
             const float lam = a.lam[i];
             kPart = -static_cast<double>(lam) * static_cast<double>(a.u[i]);
-            // (I + L_y/2) m2 along the row
-            const float c = a.m2[i];
+            // (I + A_y/2) m2 along the row
             float s = 0.0f;
-            if (hy)
+            if (hy || a.wrapY)
             {
-                s += a.D[i] * (a.m2[i + 1] - c);
+                s += a.D[i] * (a.m2[iy] - c);
             }
             if (y > 0)
             {
                 s += a.D[i - 1] * (a.m2[i - 1] - c);
+            }
+            else if (a.wrapY)
+            {
+                s += a.D[i + Y - 1] * (a.m2[i + Y - 1] - c);
+            }
+            if (a.dirY)
+            {
+                s -= (y == 0) ? 2.0f * a.D[i] * c : 0.0f;
+                s -= (y == Y - 1) ? 2.0f * a.D[i] * c : 0.0f;
             }
             a.lamNext[i] = fmaf(a.negK, lam, fmaf(0.5f, s, c));
         }
@@ -757,36 +1071,68 @@ namespace
         }
     }
 
-    // Reciprocal Thomas pivots ib = 1/b' of the zero-flux lines with face
-    // rates D: b_j = 1 + (D_{j-1} + D_j)/2 (absent faces dropped), b'_0 = b_0,
-    // b'_j = b_j - (D_{j-1}/2)^2 / b'_{j-1}.  Lines along x (alongX: one per
-    // column, element j at j*Y + y) or along y (one per row); ib is written in
-    // the [X][Y] layout either way.
-    void LinePivots(int32_t X, int32_t Y, const std::vector<double> &D, bool alongX, std::vector<float> &ib,
-                    std::vector<float> &h)
+    // Reciprocal Thomas pivots ib = 1/b' of the lines with face rates D:
+    // b_j = 1 + (D_{j-1} + D_j)/2 (absent faces dropped), b'_0 = b_0,
+    // b'_j = b_j - (D_{j-1}/2)^2 / b'_{j-1}, and h = (D/2) ib.  A Dirichlet
+    // line adds D_j to each end's diagonal (its ghost face 2 D_j, halved).  A
+    // periodic line (L > 1) adds the wrap face D_{L-1} to both ends and takes
+    // the corner-adjusted diagonal of include/Hal/Pde/TdmaSolver.h:68-120
+    // (gamma = -b_0, alpha = beta = -D_{L-1}/2: b_0 - gamma, b_{L-1} -
+    // alpha beta / gamma); lineK gets (gamma, alpha, beta / gamma, 0) per line.
+    // Lines along x (alongX: one per column, element j at j*Y + y) or along y
+    // (one per row); ib and h are written in the [X][Y] layout either way.
+    void LinePivots(int32_t X, int32_t Y, const std::vector<double> &D, bool alongX, bool wrap, bool dirichlet,
+                    std::vector<float> &ib, std::vector<float> &h, std::vector<float> &lineK)
     {
         ib.assign(static_cast<size_t>(X) * Y, 0.0f);
         h.assign(static_cast<size_t>(X) * Y, 0.0f);
         const int32_t nLines = alongX ? Y : X;
         const int32_t L = alongX ? X : Y;
+        const bool per = wrap && L > 1;
+        lineK.assign(4 * static_cast<size_t>(nLines), 0.0f);
+        std::vector<double> b(static_cast<size_t>(L));
         for (int32_t l = 0; l < nLines; l++)
         {
             auto at = [&](int32_t j) -> size_t
             {
                 return alongX ? static_cast<size_t>(j) * Y + l : static_cast<size_t>(l) * Y + j;
             };
-            double bpPrev = 0.0;
             for (int32_t j = 0; j < L; j++)
             {
                 const double dm = (j > 0) ? D[at(j - 1)] : 0.0;
                 const double dp = (j < L - 1) ? D[at(j)] : 0.0;
-                double bp = 1.0 + 0.5 * (dm + dp);
+                b[j] = 1.0 + 0.5 * (dm + dp);
+            }
+            if (per)
+            {
+                const double dw = D[at(L - 1)];
+                b[0] += 0.5 * dw;
+                b[L - 1] += 0.5 * dw;
+                const double gamma = -b[0];
+                const double alpha = -0.5 * dw;
+                b[0] -= gamma;
+                b[L - 1] -= alpha * alpha / gamma;
+                float *k = lineK.data() + 4 * static_cast<size_t>(l);
+                k[0] = static_cast<float>(gamma);
+                k[1] = static_cast<float>(alpha);
+                k[2] = static_cast<float>(alpha / gamma);
+            }
+            else if (dirichlet)
+            {
+                b[0] += D[at(0)];
+                b[L - 1] += D[at(L - 1)];
+            }
+            double bpPrev = 0.0;
+            for (int32_t j = 0; j < L; j++)
+            {
+                double bp = b[j];
                 if (j > 0)
                 {
-                    bp -= (0.5 * dm) * (0.5 * dm) / bpPrev;
+                    const double dm = 0.5 * D[at(j - 1)];
+                    bp -= dm * dm / bpPrev;
                 }
                 ib[at(j)] = static_cast<float>(1.0 / bp);
-                h[at(j)] = static_cast<float>((j < L - 1) ? 0.5 * dp / bp : 0.0);
+                h[at(j)] = static_cast<float>((j < L - 1) ? 0.5 * D[at(j)] / bp : 0.0);
                 bpPrev = bp;
             }
         }
@@ -806,6 +1152,15 @@ struct halhip_adjoint2d
     DeviceArray<float> D;                  // face rates, [X][Y]
     DeviceArray<float> ibX, ibY;           // reciprocal pivots of the x- and y-lines, [X][Y]
     DeviceArray<float> hX, hY;             // (D/2) ib of the x- and y-lines, [X][Y]
+    // Boundaries: wrap flags, Dirichlet ghosts on the non-wrapped axes
+    // (edges [xlo(Y) | xhi(Y) | ylo(X) | yhi(X)]), and the periodic lines'
+    // Sherman-Morrison constants (lineKX per column, lineKY per row).
+    bool wrapX = false;
+    bool wrapY = false;
+    bool dirX = false;
+    bool dirY = false;
+    DeviceArray<float> edges;
+    DeviceArray<float> lineKX, lineKY;
     DeviceArray<float> u, s2, work;        // field work buffers
     DeviceArray<float> ckpt;               // (ticks/every + 1) checkpoints
     DeviceArray<float> stashU, stashS2, stashS4; // one segment
@@ -831,19 +1186,44 @@ namespace
         Bind(p->ctx);
     }
 
-    template <int EPL, int RHS, bool DECAY, bool STASH>
-    void LaunchBlockLine(halhip_adjoint2d *p, const LineArgs &a)
+    // Periodic lines along y (rows) / along x (columns): a wrapped axis of
+    // more than one cell.
+    bool PeriodicRows(const halhip_adjoint2d *p)
+    {
+        return p->wrapY && p->Y > 1;
+    }
+
+    bool PeriodicColumns(const halhip_adjoint2d *p)
+    {
+        return p->wrapX && p->X > 1;
+    }
+
+    template <int EPL, int RHS, bool DECAY, bool STASH, bool PER>
+    void LaunchBlockLinePer(halhip_adjoint2d *p, const LineArgs &a)
     {
         const bool vec = (EPL % 4 == 0) && (a.L % 4 == 0);
         if (vec)
         {
-            hipLaunchKernelGGL((CellLineSolveBlock<EPL, RHS, DECAY, STASH, (EPL % 4 == 0)>),
+            hipLaunchKernelGGL((CellLineSolveBlock<EPL, RHS, DECAY, STASH, (EPL % 4 == 0), PER>),
                                dim3(static_cast<unsigned>(a.nLines)), dim3(kBlockLine), 0, p->clock.stream, a);
         }
         else
         {
-            hipLaunchKernelGGL((CellLineSolveBlock<EPL, RHS, DECAY, STASH, false>),
+            hipLaunchKernelGGL((CellLineSolveBlock<EPL, RHS, DECAY, STASH, false, PER>),
                                dim3(static_cast<unsigned>(a.nLines)), dim3(kBlockLine), 0, p->clock.stream, a);
+        }
+    }
+
+    template <int EPL, int RHS, bool DECAY, bool STASH>
+    void LaunchBlockLine(halhip_adjoint2d *p, const LineArgs &a)
+    {
+        if (PeriodicRows(p))
+        {
+            LaunchBlockLinePer<EPL, RHS, DECAY, STASH, true>(p, a);
+        }
+        else
+        {
+            LaunchBlockLinePer<EPL, RHS, DECAY, STASH, false>(p, a);
         }
     }
 
@@ -881,17 +1261,31 @@ namespace
         p->prof.End(p->clock.stream);
     }
 
-    template <int C, int E, int MODE>
-    void LaunchColumnE(halhip_adjoint2d *p, const ColArgs &a)
+    template <int C, int E, int MODE, bool PER>
+    void LaunchColumnPer(halhip_adjoint2d *p, const ColArgs &a)
     {
         const dim3 grid(static_cast<unsigned>(a.blocks));
         if (a.Y % C == 0)
         {
-            hipLaunchKernelGGL((CellColumnSolve<C, E, MODE, true>), grid, dim3(64 * C), 0, p->clock.stream, a);
+            hipLaunchKernelGGL((CellColumnSolve<C, E, MODE, true, PER>), grid, dim3(64 * C), 0, p->clock.stream, a);
         }
         else
         {
-            hipLaunchKernelGGL((CellColumnSolve<C, E, MODE, false>), grid, dim3(64 * C), 0, p->clock.stream, a);
+            hipLaunchKernelGGL((CellColumnSolve<C, E, MODE, false, PER>), grid, dim3(64 * C), 0, p->clock.stream,
+                               a);
+        }
+    }
+
+    template <int C, int E, int MODE>
+    void LaunchColumnE(halhip_adjoint2d *p, const ColArgs &a)
+    {
+        if (PeriodicColumns(p))
+        {
+            LaunchColumnPer<C, E, MODE, true>(p, a);
+        }
+        else
+        {
+            LaunchColumnPer<C, E, MODE, false>(p, a);
         }
     }
 
@@ -908,6 +1302,12 @@ namespace
         a.ib = p->ibX.get();
         a.h = p->hX.get();
         a.out = out;
+        a.lineK = reinterpret_cast<const float4 *>(p->lineKX.get());
+        a.edges = p->edges.get();
+        a.wrapX = p->wrapX ? 1 : 0;
+        a.wrapY = p->wrapY ? 1 : 0;
+        a.dirX = p->dirX ? 1 : 0;
+        a.dirY = p->dirY ? 1 : 0;
         int e = 1;
         while (64 * e < a.X)
         {
@@ -953,6 +1353,11 @@ namespace
         a.rate = p->D.get();
         a.ib = p->ibY.get();
         a.h = p->hY.get();
+        a.lineK = reinterpret_cast<const float4 *>(p->lineKY.get());
+        a.edges = p->edges.get();
+        a.wrapAcross = p->wrapX ? 1 : 0;
+        a.dirAcross = p->dirX ? 1 : 0;
+        a.dirAlong = p->dirY ? 1 : 0;
         return a;
     }
 
@@ -1066,6 +1471,11 @@ namespace
                 g.gradD = p->grad.get();
                 g.partialK = p->partialK.get() + static_cast<size_t>(t) * gBlocks;
                 g.negK = -p->decay;
+                g.edges = p->edges.get();
+                g.wrapX = p->wrapX ? 1 : 0;
+                g.wrapY = p->wrapY ? 1 : 0;
+                g.dirX = p->dirX ? 1 : 0;
+                g.dirY = p->dirY ? 1 : 0;
                 p->prof.Begin(s, "adjoint_gradient");
                 hipLaunchKernelGGL(CellGradient, gGrid, dim3(256), 0, s, g);
                 Check(hipGetLastError(), "CellGradient launch");
@@ -1121,6 +1531,9 @@ extern "C"
                 }
                 p->grad.Allocate(ctx, n);
                 p->partialJ.Allocate(ctx, 1024);
+                p->edges.Allocate(ctx, 2 * static_cast<size_t>(xDim) + 2 * static_cast<size_t>(yDim));
+                p->lineKX.Allocate(ctx, 4 * static_cast<size_t>(yDim));
+                p->lineKY.Allocate(ctx, 4 * static_cast<size_t>(xDim));
                 p->clock.Create();
                 *out = p;
             });
@@ -1147,8 +1560,8 @@ extern "C"
             });
     }
 
-    int halhip_adjoint2d_set_problem(halhip_adjoint2d *p, const double *diffusivity, double decay, int32_t ticks,
-                                     int32_t checkpointEvery)
+    int halhip_adjoint2d_set_problem_bc(halhip_adjoint2d *p, const double *diffusivity, double decay, int32_t ticks,
+                                        int32_t checkpointEvery, int32_t wrapX, int32_t wrapY, const halhip_bc *bc)
     {
         return Guard(
             [&]
@@ -1160,19 +1573,29 @@ extern "C"
                 const int32_t X = p->X;
                 const int32_t Y = p->Y;
                 const size_t n = Cells(p);
+                EdgeValues ev;
+                ev.From(bc, X, Y);
                 std::vector<double> Dxy(diffusivity, diffusivity + n);
                 for (double v : Dxy)
                 {
                     Require(std::isfinite(v) && v >= 0.0, "halhip_adjoint2d_set_problem: diffusivity must be finite "
                                                            "and non-negative");
                 }
+                const bool wx = wrapX != 0;
+                const bool wy = wrapY != 0;
                 // The device computes with D in FP32; the pivots are formed
                 // in FP64 from the same FP32 values.
                 std::vector<float> Df(Dxy.begin(), Dxy.end());
                 std::vector<double> Dr(Df.begin(), Df.end());
-                std::vector<float> ibx, iby, hx, hy;
-                LinePivots(X, Y, Dr, true, ibx, hx);
-                LinePivots(X, Y, Dr, false, iby, hy);
+                std::vector<float> ibx, iby, hx, hy, kx, ky;
+                LinePivots(X, Y, Dr, true, wx, ev.dirichlet && !wx, ibx, hx, kx);
+                LinePivots(X, Y, Dr, false, wy, ev.dirichlet && !wy, iby, hy, ky);
+                std::vector<float> edges;
+                edges.reserve(2 * static_cast<size_t>(X) + 2 * static_cast<size_t>(Y));
+                for (const std::vector<float> *e : {&ev.xlo, &ev.xhi, &ev.ylo, &ev.yhi})
+                {
+                    edges.insert(edges.end(), e->begin(), e->end());
+                }
                 auto up = [&](DeviceArray<float> &dst, const std::vector<float> &src)
                 {
                     Check(hipMemcpy(dst.get(), src.data(), src.size() * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
@@ -1183,6 +1606,13 @@ extern "C"
                 up(p->ibY, iby);
                 up(p->hX, hx);
                 up(p->hY, hy);
+                up(p->lineKX, kx);
+                up(p->lineKY, ky);
+                up(p->edges, edges);
+                p->wrapX = wx;
+                p->wrapY = wy;
+                p->dirX = ev.dirichlet && !wx;
+                p->dirY = ev.dirichlet && !wy;
                 p->graphs.Clear(); // they hold the old checkpoint and stash buffers
                 p->decay = static_cast<float>(decay);
                 p->ticks = ticks;
@@ -1197,6 +1627,12 @@ extern "C"
                 p->partialK.Allocate(p->ctx, gblocks * static_cast<size_t>(ticks));
                 p->ready = true;
             });
+    }
+
+    int halhip_adjoint2d_set_problem(halhip_adjoint2d *p, const double *diffusivity, double decay, int32_t ticks,
+                                     int32_t checkpointEvery)
+    {
+        return halhip_adjoint2d_set_problem_bc(p, diffusivity, decay, ticks, checkpointEvery, 0, 0, nullptr);
     }
 
     int halhip_adjoint2d_forward(halhip_adjoint2d *p, const float *u0, float *uT)

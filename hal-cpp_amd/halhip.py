@@ -152,6 +152,7 @@ _SIGS = {
     "halhip_adjoint2d_create": [_vp, _i32, _i32, ctypes.POINTER(_vp)],
     "halhip_adjoint2d_destroy": [_vp],
     "halhip_adjoint2d_set_problem": [_vp, _dp, _f64, _i32, _i32],
+    "halhip_adjoint2d_set_problem_bc": [_vp, _dp, _f64, _i32, _i32, _i32, _i32, ctypes.POINTER(HalBC)],
     "halhip_adjoint2d_forward": [_vp, _fp, _fp],
     "halhip_adjoint2d_gradient": [_vp, _fp, _fp, _dp, _fp, _dp],
     "halhip_adjoint2d_set_profiling": [_vp, _i32],
@@ -715,7 +716,8 @@ class Batch(_Plan):
 
 
 class Adjoint2D(_Plan):
-    """Per-cell-diffusivity ADI with adjoint gradients (zero flux)."""
+    """Per-cell-diffusivity ADI with adjoint gradients (zero flux, periodic
+    or Dirichlet boundaries)."""
     _prefix = "adjoint2d"
 
     def __init__(self, ctx, X, Y):
@@ -723,10 +725,17 @@ class Adjoint2D(_Plan):
         self.h = _vp()
         _check(load().halhip_adjoint2d_create(ctx.h, X, Y, ctypes.byref(self.h)))
 
-    def set_problem(self, diffusivity, decay, ticks, checkpoint_every):
+    def set_problem(self, diffusivity, decay, ticks, checkpoint_every, wrapX=False, wrapY=False, bc=None):
+        """bc: None (zero flux), a constant Dirichlet value or edge arrays,
+        on the non-wrapped axes."""
         D = _f64(diffusivity)
         assert D.size == self.X * self.Y
-        _check(self._fn("set_problem")(self.h, D.ctypes.data_as(_dp), decay, ticks, checkpoint_every))
+        if not wrapX and not wrapY and bc is None:
+            _check(self._fn("set_problem")(self.h, D.ctypes.data_as(_dp), decay, ticks, checkpoint_every))
+            return
+        s, keep = make_bc(bc, self.X, self.Y)
+        _check(self._fn("set_problem_bc")(self.h, D.ctypes.data_as(_dp), decay, ticks, checkpoint_every,
+                                          int(wrapX), int(wrapY), _bcp(s)))
 
     def forward(self, u0):
         u0 = _f32(u0)

@@ -489,12 +489,13 @@ int halhip_batch_profile(halhip_batch *batch, int32_t maxKernels, int32_t *nKern
 
 /* ---- adjoint ADI --------------------------------------------------------------- */
 
-/* Per-cell-diffusivity ADI on an xDim x yDim grid with zero-flux boundaries,
- * its discrete adjoint and gradients.  No reference counterpart: the forward
- * tick generalises PDEGrid2D::DiffusionADI (include/Hal/Pde/ADIequations.h:306-337)
- * to a per-cell rate D with the face convention of the per-cell explicit
- * stencil (src/Pde/PDEequations.cpp:214-219) and appends MulAll(-decay) +
- * Update.  Sides <= 4096. */
+/* Per-cell-diffusivity ADI on an xDim x yDim grid, its discrete adjoint and
+ * gradients.  No reference counterpart: the forward tick generalises
+ * PDEGrid2D::DiffusionADI (include/Hal/Pde/ADIequations.h:306-378) to a
+ * per-cell rate D with the face convention of the per-cell explicit stencil
+ * (src/Pde/PDEequations.cpp:199-228: the face between cells i and i+1 carries
+ * D_i, the wrap face D_{n-1}) and appends MulAll(-decay) + Update.  Sides
+ * <= 4096. */
 int halhip_adjoint2d_create(halhip_context *ctx, int32_t xDim, int32_t yDim, halhip_adjoint2d **out);
 int halhip_adjoint2d_destroy(halhip_adjoint2d *plan);
 /* Diffusivity field (xDim*yDim, HAL layout), decay rate, number of ticks
@@ -502,6 +503,13 @@ int halhip_adjoint2d_destroy(halhip_adjoint2d *plan);
  * its checkpoint). */
 int halhip_adjoint2d_set_problem(halhip_adjoint2d *plan, const double *diffusivity, double decay, int32_t ticks,
                                  int32_t checkpointEvery);
+/* The same with boundaries, as the reference's ADI takes them
+ * (include/Hal/Pde/ADIequations.h:306-378): a wrapped axis is periodic; a
+ * Dirichlet bc (constant or edge arrays) applies on the non-wrapped axes,
+ * its ghost face carrying the boundary cell's own D; otherwise zero flux.
+ * set_problem is this call with no wrap and a null bc. */
+int halhip_adjoint2d_set_problem_bc(halhip_adjoint2d *plan, const double *diffusivity, double decay, int32_t ticks,
+                                    int32_t checkpointEvery, int32_t wrapX, int32_t wrapY, const halhip_bc *bc);
 /* u_T from u0 (host FP32 arrays of xDim*yDim). */
 int halhip_adjoint2d_forward(halhip_adjoint2d *plan, const float *u0, float *uT);
 /* J = 1/2 |u_T - target|^2 and its gradients with respect to every cell's
