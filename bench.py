@@ -228,7 +228,7 @@ def bench_batched(ctx, ticks=500, cpu=True):
     b.set_sources(spec["src_well"], spec["src_cell"], spec["src_val"])
     zeros = np.zeros(b.cells, np.float32)
     b.upload(zeros)
-    b.autotune(20)  # strip rows x carry method, timed on this batch
+    b.autotune(20)  # strip rows x carry method x each width class's workgroup shape, timed on this batch
     b.run_ticks(5)
     b.prepare_ticks(ticks)
     b.run_ticks(ticks, asynchronous=True)
@@ -242,6 +242,7 @@ def bench_batched(ctx, ticks=500, cpu=True):
     out = {"workload": "512 wells, sides log-uniform 64-1024, D~U[0.1,5], decay~U[0,1e-2], 32 sources/well, "
                        "zero flux", "cells": int(b.cells), "strip_rows": b.strip_rows(),
            "carries": "windowed" if b.carries() == halhip.ADI_CARRIES_WINDOW else "scan",
+           "workgroup_shapes": {f"width<={w}": f"{t}x{c}" for w, (_, t, c) in zip((256, 512, 1024), b.shapes())},
            "batched_ms_per_tick": round(batched_ms, 4), "per_instance_ms_per_tick": round(loop_ms, 4),
            "batched_Gcell_s": round(b.cells / (batched_ms * 1e-3) / 1e9, 2),
            "per_instance_Gcell_s": round(b.cells / (loop_ms * 1e-3) / 1e9, 2),

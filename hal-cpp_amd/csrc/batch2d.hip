@@ -8,14 +8,17 @@
 // batch is two launches over a work list covering every well:
 //   BatchScan   one block per (well, 16-column group): the cross-strip scan
 //   BatchStrip  one block per (well, strip): sweep A finish, sweep B, decay,
-//               point sources and the next tick's strip reduction, with the
-//               strip body chosen per well (columns per thread 1/2/4, vector
-//               or scalar rows) by a block-uniform switch.
-// Both are captured in one hipGraph per chunk of ticks.  The strip height is
-// picked by an autotuner (timed candidates, cached per batch signature).  The
-// per-instance launch loop (halhip_batch_run_ticks_per_instance) runs the same
-// kernels one well at a time and is the baseline the batch is measured
-// against.
+//               point sources and the next tick's strip reduction.  Wells are
+//               grouped by width (<= 256, <= 512, <= 1024 columns) and each
+//               width class has a launch of its own with its own workgroup
+//               shape (threads x columns per thread), so a class's register
+//               budget and occupancy are its own; vector or scalar rows are a
+//               block-uniform switch.
+// All are captured in one hipGraph per chunk of ticks.  The shape of every
+// width class, the strip height and the carry method are picked by an
+// autotuner (timed candidates, cached per batch signature).  The per-instance
+// launch loop (halhip_batch_run_ticks_per_instance) runs the same kernels one
+// well at a time and is the baseline the batch is measured against.
 #include "adi_device.hpp"
 
 #include <algorithm>
@@ -32,14 +35,34 @@ using namespace halhip;
 
 namespace
 {
-    constexpr int kBatchThreads = 256;
     constexpr int kBatchMaxSide = 1024;
+    constexpr int kClasses = 3;       // well widths <= 256, <= 512, <= 1024
+    constexpr int kShapesPerClass = 3;
+
+    // Workgroup shapes (threads, columns per thread) of each width class;
+    // threads * columns covers the class's widest well.  Option 0 is the
+    // default.
+    struct BlockShape
+    {
+        int32_t threads;
+        int32_t cpt;
+    };
+    constexpr BlockShape kShapes[kClasses][kShapesPerClass] = {
+        {{256, 1}, {128, 2}, {64, 4}},
+        {{256, 2}, {512, 1}, {128, 4}},
+        {{256, 4}, {512, 2}, {1024, 1}},
+    };
+
+    int32_t WidthClass(int32_t Y)
+    {
+        return (Y <= 256) ? 0 : ((Y <= 512) ? 1 : 2);
+    }
 
     struct BatchWell
     {
         AdiArgs args;
         int64_t fieldOffset;
-        int32_t cls; // 0: 1 col/thread; 1: 2 vec; 2: 2 scalar; 3: 4 vec; 4: 4 scalar
+        int32_t vec; // rows move as vectors (yDim a multiple of the columns per thread)
         int32_t pad;
         float *redHalf[2]; // windowed carries: the two halves of this well's reductions
     };
@@ -58,37 +81,15 @@ namespace
         int32_t yBase;
     };
 
-    template <int S, int MODE, int KW>
-    __device__ __forceinline__ void StripByClass(int cls, const AdiArgs &a, float *u, int p)
-    {
-        switch (cls)
-        {
-            case 0:
-                StripBody<1, S, MODE, true, KW>(a, u, p);
-                break;
-            case 1:
-                StripBody<2, S, MODE, true, KW>(a, u, p);
-                break;
-            case 2:
-                StripBody<2, S, MODE, false, KW>(a, u, p);
-                break;
-            case 3:
-                StripBody<4, S, MODE, true, KW>(a, u, p);
-                break;
-            default:
-                StripBody<4, S, MODE, false, KW>(a, u, p);
-                break;
-        }
-    }
-
     // MODE 0: strip reductions of the current fields; MODE 3: full tick.
     // KW > 0 (windowed carries): the current reductions are in redHalf[cur];
     // a tick reads them and writes the next ones to redHalf[1 - cur], a
-    // MODE 0 pass writes redHalf[cur].
-    template <int S, int MODE, int KW>
-    __global__ __launch_bounds__(kBatchThreads) void BatchStrip(const BatchWell *__restrict__ wells,
-                                                                const StripItem *__restrict__ items,
-                                                                float *__restrict__ field, int32_t cur)
+    // MODE 0 pass writes redHalf[cur].  Every item's well has the launch's
+    // shape: T threads of CPT columns.
+    template <int T, int CPT, int S, int MODE, int KW>
+    __global__ __launch_bounds__(T) void BatchStrip(const BatchWell *__restrict__ wells,
+                                                    const StripItem *__restrict__ items, float *__restrict__ field,
+                                                    int32_t cur)
     {
         const StripItem it = items[blockIdx.x];
         if (it.well < 0)
@@ -104,7 +105,21 @@ namespace
             a.redIn = w->redHalf[cur];
             a.red = (MODE == 0) ? w->redHalf[cur] : w->redHalf[1 - cur];
         }
-        StripByClass<S, MODE, KW>(w->cls, a, field + w->fieldOffset, it.strip);
+        if constexpr (CPT == 1)
+        {
+            StripBody<1, S, MODE, true, KW>(a, field + w->fieldOffset, it.strip);
+        }
+        else
+        {
+            if (w->vec)
+            {
+                StripBody<CPT, S, MODE, true, KW>(a, field + w->fieldOffset, it.strip);
+            }
+            else
+            {
+                StripBody<CPT, S, MODE, false, KW>(a, field + w->fieldOffset, it.strip);
+            }
+        }
     }
 
     __global__ __launch_bounds__(kScanCols *kScanSegs) void BatchScan(const BatchWell *__restrict__ wells,
@@ -117,19 +132,45 @@ namespace
 
     using BatchStripKernel = void (*)(const BatchWell *, const StripItem *, float *, int32_t);
 
-    template <int S>
+    template <int T, int CPT, int S>
     BatchStripKernel PickBatchStripRows(int mode, bool window)
     {
         if (window)
         {
-            return mode == 0 ? BatchStrip<S, 0, kMaxWindow> : BatchStrip<S, 3, kMaxWindow>;
+            return mode == 0 ? BatchStrip<T, CPT, S, 0, kMaxWindow> : BatchStrip<T, CPT, S, 3, kMaxWindow>;
         }
-        return mode == 0 ? BatchStrip<S, 0, 0> : BatchStrip<S, 3, 0>;
+        return mode == 0 ? BatchStrip<T, CPT, S, 0, 0> : BatchStrip<T, CPT, S, 3, 0>;
     }
 
-    BatchStripKernel PickBatchStrip(int rows, int mode, bool window)
+    template <int T, int CPT>
+    BatchStripKernel PickBatchStripShape(int rows, int mode, bool window)
     {
-        return rows == 4 ? PickBatchStripRows<4>(mode, window) : PickBatchStripRows<8>(mode, window);
+        return rows == 4 ? PickBatchStripRows<T, CPT, 4>(mode, window) : PickBatchStripRows<T, CPT, 8>(mode, window);
+    }
+
+    BatchStripKernel PickBatchStrip(BlockShape sh, int rows, int mode, bool window)
+    {
+        switch (sh.threads * 8 + sh.cpt)
+        {
+            case 64 * 8 + 4:
+                return PickBatchStripShape<64, 4>(rows, mode, window);
+            case 128 * 8 + 2:
+                return PickBatchStripShape<128, 2>(rows, mode, window);
+            case 128 * 8 + 4:
+                return PickBatchStripShape<128, 4>(rows, mode, window);
+            case 256 * 8 + 1:
+                return PickBatchStripShape<256, 1>(rows, mode, window);
+            case 256 * 8 + 2:
+                return PickBatchStripShape<256, 2>(rows, mode, window);
+            case 256 * 8 + 4:
+                return PickBatchStripShape<256, 4>(rows, mode, window);
+            case 512 * 8 + 1:
+                return PickBatchStripShape<512, 1>(rows, mode, window);
+            case 512 * 8 + 2:
+                return PickBatchStripShape<512, 2>(rows, mode, window);
+            default:
+                return PickBatchStripShape<1024, 1>(rows, mode, window);
+        }
     }
 }
 
@@ -153,14 +194,17 @@ struct halhip_batch
     DeviceArray<float> colInfo;
     DeviceArray<BatchWell> wells;
     DeviceArray<StripItem> stripItems;    // natural order (per-instance ranges)
-    DeviceArray<StripItem> stripItemsXcd; // XCD-major order for batched launches
+    DeviceArray<StripItem> stripItemsXcd; // per width class, XCD-major, for batched launches
+    int32_t shapeIdx[kClasses] = {0, 0, 0};     // kShapes option of each width class
+    int32_t classItemBegin[kClasses] = {0, 0, 0}; // ranges of stripItemsXcd
+    int32_t classItemCount[kClasses] = {0, 0, 0};
+    std::vector<int32_t> wellClass;
     DeviceArray<ScanItem> scanItems;
     DeviceArray<int32_t> srcXY;
     DeviceArray<float> srcValDev;
     std::vector<int32_t> wellStripBegin, wellStripCount; // per-instance launches: item ranges
     std::vector<int32_t> wellScanBegin, wellScanCount;
     int32_t nStripItems = 0;
-    int32_t nStripItemsXcd = 0;
     int32_t nScanItems = 0;
     std::string builtKey;
     bool reducedValid = false;
@@ -181,9 +225,9 @@ namespace
         Bind(b->ctx);
     }
 
-    int32_t ColsPerThread(int32_t Y)
+    BlockShape ShapeOf(const halhip_batch *b, int32_t cls)
     {
-        return (Y <= 256) ? 1 : ((Y <= 512) ? 2 : 4);
+        return kShapes[cls][b->shapeIdx[cls]];
     }
 
     // Builds per-well factor tables, strip constants, work lists and source
@@ -199,6 +243,10 @@ namespace
         std::string key;
         AppendKey(key, b->rows);
         AppendKey(key, b->wantWindow);
+        for (int32_t c = 0; c < kClasses; c++)
+        {
+            AppendKey(key, b->shapeIdx[c]);
+        }
         AppendKey(key, wrapX);
         AppendKey(key, wrapY);
         AppendKey(key, dir);
@@ -258,14 +306,18 @@ namespace
         b->wellStripCount.assign(b->nWells, 0);
         b->wellScanBegin.assign(b->nWells, 0);
         b->wellScanCount.assign(b->nWells, 0);
+        b->wellClass.assign(b->nWells, 0);
         for (int32_t w = 0; w < b->nWells; w++)
         {
             const int32_t X = b->X[w];
             const int32_t Yw = b->Y[w];
-            const int32_t cpt = ColsPerThread(Yw);
+            const int32_t cls = WidthClass(Yw);
+            b->wellClass[w] = cls;
+            const BlockShape sh = ShapeOf(b, cls);
+            const int32_t cpt = sh.cpt;
             const int32_t P = (X + S - 1) / S;
             const int32_t xPad = P * S;
-            const int32_t yPad = kBatchThreads * cpt;
+            const int32_t yPad = sh.threads * cpt;
             AxisFactors fx = FactorLine(X, xPad, b->rate[w], wrapX != 0, dir);
             AxisFactors fy = FactorLine(Yw, yPad, b->rate[w], wrapY != 0, dir);
             auto put = [&](const std::vector<double> &v)
@@ -344,7 +396,7 @@ namespace
             a.xz0 = static_cast<float>(fx.z[0]);
             a.xzLast = static_cast<float>(fx.z[static_cast<size_t>(X - 1)]);
             bw.fieldOffset = b->offset[w];
-            bw.cls = (cpt == 1) ? 0 : (cpt == 2 ? (a.vecOk ? 1 : 2) : (a.vecOk ? 3 : 4));
+            bw.vec = a.vecOk;
 
             b->wellScanBegin[w] = static_cast<int32_t>(scans.size());
             for (int32_t yb = 0; yb < Yw; yb += kScanCols)
@@ -419,26 +471,40 @@ namespace
         b->stripItems.Allocate(b->ctx, strips.size());
         Check(hipMemcpy(b->stripItems.get(), strips.data(), strips.size() * sizeof(StripItem), hipMemcpyHostToDevice),
               "hipMemcpy");
-        // Workgroups are dealt to the 8 XCDs in turn; give XCD x the
-        // contiguous run [x*per, (x+1)*per) of the work list, so that a
-        // strip's neighbours (whose reductions the windowed carries read) ran
-        // on the same XCD and are likely still in its L2.
+        // One work list per width class.  Workgroups are dealt to the 8 XCDs
+        // in turn; give XCD x the contiguous run [x*per, (x+1)*per) of a
+        // class's list, so that a strip's neighbours (whose reductions the
+        // windowed carries read) ran on the same XCD and are likely still in
+        // its L2.
         {
-            const size_t n = strips.size();
-            const size_t per = (n + 7) / 8;
-            std::vector<StripItem> xcd(8 * per, StripItem{-1, 0, 0, 0});
-            for (size_t j = 0; j < xcd.size(); j++)
+            std::vector<StripItem> xcd;
+            for (int32_t c = 0; c < kClasses; c++)
             {
-                const size_t src = (j % 8) * per + j / 8;
-                if (src < n)
+                std::vector<StripItem> mine;
+                for (const StripItem &it : strips)
                 {
-                    xcd[j] = strips[src];
+                    if (b->wellClass[it.well] == c)
+                    {
+                        mine.push_back(it);
+                    }
+                }
+                const size_t n = mine.size();
+                const size_t per = (n + 7) / 8;
+                b->classItemBegin[c] = static_cast<int32_t>(xcd.size());
+                b->classItemCount[c] = static_cast<int32_t>(8 * per);
+                for (size_t j = 0; j < 8 * per; j++)
+                {
+                    const size_t src = (j % 8) * per + j / 8;
+                    xcd.push_back(src < n ? mine[src] : StripItem{-1, 0, 0, 0});
                 }
             }
-            b->stripItemsXcd.Allocate(b->ctx, xcd.size());
-            Check(hipMemcpy(b->stripItemsXcd.get(), xcd.data(), xcd.size() * sizeof(StripItem), hipMemcpyHostToDevice),
-                  "hipMemcpy");
-            b->nStripItemsXcd = static_cast<int32_t>(xcd.size());
+            b->stripItemsXcd.Allocate(b->ctx, std::max<size_t>(1, xcd.size()));
+            if (!xcd.empty())
+            {
+                Check(hipMemcpy(b->stripItemsXcd.get(), xcd.data(), xcd.size() * sizeof(StripItem),
+                                hipMemcpyHostToDevice),
+                      "hipMemcpy");
+            }
         }
         b->scanItems.Allocate(b->ctx, scans.size());
         Check(hipMemcpy(b->scanItems.get(), scans.data(), scans.size() * sizeof(ScanItem), hipMemcpyHostToDevice),
@@ -450,20 +516,44 @@ namespace
         b->graphs.Clear();
     }
 
-    // Strip kernel over items [itemBegin, itemBegin + itemCount) of the
-    // natural work list, or (all == true) over the whole XCD-major list.
-    void LaunchBatchStrip(halhip_batch *b, int mode, int32_t itemBegin, int32_t itemCount, const char *name,
-                          bool all = false)
+    // Strip kernel of width class cls over `count` items from `items`.
+    void LaunchBatchStripItems(halhip_batch *b, int mode, int32_t cls, const StripItem *items, int32_t count,
+                               const char *name)
     {
-        const StripItem *items = all ? b->stripItemsXcd.get() : b->stripItems.get() + itemBegin;
-        const int32_t count = all ? b->nStripItemsXcd : itemCount;
+        if (count <= 0)
+        {
+            return;
+        }
+        const BlockShape sh = ShapeOf(b, cls);
         b->prof.Begin(b->clock.stream, name);
-        hipLaunchKernelGGL(PickBatchStrip(b->rows, mode, b->window), dim3(static_cast<unsigned>(count)),
-                           dim3(kBatchThreads), 0, b->clock.stream, static_cast<const BatchWell *>(b->wells.get()),
-                           items, b->field.get(), b->cur);
+        hipLaunchKernelGGL(PickBatchStrip(sh, b->rows, mode, b->window), dim3(static_cast<unsigned>(count)),
+                           dim3(static_cast<unsigned>(sh.threads)), 0, b->clock.stream,
+                           static_cast<const BatchWell *>(b->wells.get()), items, b->field.get(), b->cur);
         Check(hipGetLastError(), "BatchStrip launch");
         b->prof.End(b->clock.stream);
     }
+
+    // Strip kernels of every width class over their XCD-major work lists.
+    void LaunchBatchStripAll(halhip_batch *b, int mode, const char *const (&names)[kClasses])
+    {
+        for (int32_t c = 0; c < kClasses; c++)
+        {
+            LaunchBatchStripItems(b, mode, c, b->stripItemsXcd.get() + b->classItemBegin[c], b->classItemCount[c],
+                                  names[c]);
+        }
+    }
+
+    // One well's strips (its range of the natural work list).
+    void LaunchBatchStripWell(halhip_batch *b, int mode, int32_t w, const char *name)
+    {
+        LaunchBatchStripItems(b, mode, b->wellClass[w], b->stripItems.get() + b->wellStripBegin[w],
+                              b->wellStripCount[w], name);
+    }
+
+    constexpr const char *kTickNames[kClasses] = {"batch_strip_tick_w256", "batch_strip_tick_w512",
+                                                  "batch_strip_tick_w1024"};
+    constexpr const char *kReduceNames[kClasses] = {"batch_strip_reduce", "batch_strip_reduce",
+                                                    "batch_strip_reduce"};
 
     void LaunchBatchScan(halhip_batch *b, int32_t itemBegin, int32_t itemCount, const char *name)
     {
@@ -483,7 +573,7 @@ namespace
         {
             LaunchBatchScan(b, 0, b->nScanItems, "batch_scan");
         }
-        LaunchBatchStrip(b, 3, 0, b->nStripItems, "batch_strip_tick", true);
+        LaunchBatchStripAll(b, 3, kTickNames);
         if (b->window)
         {
             b->cur = 1 - b->cur;
@@ -500,7 +590,7 @@ namespace
         }
         if (!b->reducedValid && !b->prepareOnly)
         {
-            LaunchBatchStrip(b, 0, 0, b->nStripItems, "batch_strip_reduce", true);
+            LaunchBatchStripAll(b, 0, kReduceNames);
         }
         // 10-tick graph chunks keyed by the reduction half they start from
         // (even: a windowed chunk leaves `cur` as it found it); a prepare
@@ -568,7 +658,7 @@ namespace
         {
             if (!b->reducedValid)
             {
-                LaunchBatchStrip(b, 0, b->wellStripBegin[w], b->wellStripCount[w], "instance_strip_reduce");
+                LaunchBatchStripWell(b, 0, w, "instance_strip_reduce");
             }
         }
         for (int32_t t = 0; t < ticks; t++)
@@ -579,7 +669,7 @@ namespace
                 {
                     LaunchBatchScan(b, b->wellScanBegin[w], b->wellScanCount[w], "instance_scan");
                 }
-                LaunchBatchStrip(b, 3, b->wellStripBegin[w], b->wellStripCount[w], "instance_strip_tick");
+                LaunchBatchStripWell(b, 3, w, "instance_strip_tick");
             }
             if (b->window)
             {
@@ -845,46 +935,130 @@ extern "C"
                         cache[k] = v;
                     }
                 }
-                // Candidates: strip rows 4 / 8, each with the scan kernel and
-                // with windowed carries; cached as rows + 100 * windowed.
+                // A choice is cached as rows + 100 * windowed + 1000 *
+                // (shape of widths <= 256) + 10000 * (<= 512) + 100000 * (<= 1024).
+                auto decode = [&](int32_t code)
+                {
+                    b->rows = code % 100;
+                    b->wantWindow = (code / 100) % 10 == 1;
+                    int32_t rest = code / 1000;
+                    for (int32_t c = 0; c < kClasses; c++)
+                    {
+                        b->shapeIdx[c] = rest % 10;
+                        rest /= 10;
+                    }
+                };
+                auto valid = [](int32_t code)
+                {
+                    if ((code % 100 != 4 && code % 100 != 8) || (code / 100) % 10 > 1 || code < 0)
+                    {
+                        return false;
+                    }
+                    int32_t rest = code / 1000;
+                    for (int32_t c = 0; c < kClasses; c++)
+                    {
+                        if (rest % 10 >= kShapesPerClass)
+                        {
+                            return false;
+                        }
+                        rest /= 10;
+                    }
+                    return rest == 0;
+                };
                 auto it = cache.find(sig);
-                int32_t best = 0;
-                if (it != cache.end() && (it->second % 100 == 4 || it->second % 100 == 8) && it->second / 100 <= 1)
+                int32_t best = -1;
+                if (it != cache.end() && valid(it->second))
                 {
                     best = it->second;
                 }
                 else
                 {
-                    // Time each candidate on a scratch copy of the fields so the
-                    // batch's state is unchanged.
+                    // Candidates: strip rows {4, 8} x carries {scan, windowed}
+                    // x the workgroup shapes of each width class.  The width
+                    // classes run as launches of their own, so each class's
+                    // shape is timed on its own kernel events and chosen
+                    // independently; a (rows, carries) pair costs the sum of
+                    // its classes' best times plus its scan launches.  Timing
+                    // runs on a scratch copy of the fields, so the batch's
+                    // state is unchanged.
                     DeviceArray<float> saved;
                     saved.Allocate(b->ctx, static_cast<size_t>(b->cells));
                     Check(hipMemcpyAsync(saved.get(), b->field.get(), static_cast<size_t>(b->cells) * sizeof(float),
                                          hipMemcpyDeviceToDevice, b->clock.stream),
                           "hipMemcpyAsync");
-                    float bestMs = 0.0f;
-                    for (int32_t code : {4, 8, 104, 108})
+                    const bool wasOn = b->prof.On();
+                    double bestMs = 0.0;
+                    for (int32_t rows : {4, 8})
                     {
-                        const int32_t rows = code % 100;
-                        b->rows = rows;
-                        b->wantWindow = code >= 100;
-                        b->builtKey.clear();
-                        b->reducedValid = false;
-                        EnqueueBatchTicks(b, 2, wrapX, wrapY, bc);
-                        Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
-                        b->clock.Mark();
-                        EnqueueBatchTicks(b, ticks, wrapX, wrapY, bc);
-                        float ms = 0.0f;
-                        b->clock.Sync(&ms);
-                        if (best == 0 || ms < bestMs)
+                        for (int32_t win = 0; win < 2; win++)
                         {
-                            best = code;
-                            bestMs = ms;
+                            double classMs[kClasses];
+                            int32_t classBest[kClasses] = {0, 0, 0};
+                            double scanMs = 0.0;
+                            bool usable = true;
+                            for (int32_t i = 0; i < kShapesPerClass; i++)
+                            {
+                                b->rows = rows;
+                                b->wantWindow = win == 1;
+                                for (int32_t c = 0; c < kClasses; c++)
+                                {
+                                    b->shapeIdx[c] = i;
+                                }
+                                b->builtKey.clear();
+                                b->reducedValid = false;
+                                b->prof.SetOn(false);
+                                EnqueueBatchTicks(b, 2, wrapX, wrapY, bc);
+                                Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
+                                if (win == 1 && !b->window)
+                                {
+                                    usable = false; // some well's window is too wide
+                                    break;
+                                }
+                                b->prof.Reset();
+                                b->prof.SetOn(true);
+                                EnqueueBatchTicks(b, ticks, wrapX, wrapY, bc);
+                                b->prof.Collect();
+                                for (int32_t c = 0; c < kClasses; c++)
+                                {
+                                    const double ms = b->prof.TotalMs(kTickNames[c]);
+                                    if (i == 0 || ms < classMs[c])
+                                    {
+                                        classMs[c] = ms;
+                                        classBest[c] = i;
+                                    }
+                                }
+                                const double sm = b->prof.TotalMs("batch_scan");
+                                scanMs = (i == 0) ? sm : std::min(scanMs, sm);
+                                Check(hipMemcpyAsync(b->field.get(), saved.get(),
+                                                     static_cast<size_t>(b->cells) * sizeof(float),
+                                                     hipMemcpyDeviceToDevice, b->clock.stream),
+                                      "hipMemcpyAsync");
+                            }
+                            if (!usable)
+                            {
+                                continue;
+                            }
+                            double total = scanMs;
+                            int32_t code = rows + 100 * win;
+                            int32_t scale = 1000;
+                            for (int32_t c = 0; c < kClasses; c++)
+                            {
+                                total += classMs[c];
+                                code += scale * classBest[c];
+                                scale *= 10;
+                            }
+                            if (best < 0 || total < bestMs)
+                            {
+                                best = code;
+                                bestMs = total;
+                            }
                         }
-                        Check(hipMemcpyAsync(b->field.get(), saved.get(), static_cast<size_t>(b->cells) * sizeof(float),
-                                             hipMemcpyDeviceToDevice, b->clock.stream),
-                              "hipMemcpyAsync");
                     }
+                    b->prof.Reset();
+                    b->prof.SetOn(wasOn);
+                    Check(hipMemcpyAsync(b->field.get(), saved.get(), static_cast<size_t>(b->cells) * sizeof(float),
+                                         hipMemcpyDeviceToDevice, b->clock.stream),
+                          "hipMemcpyAsync");
                     Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
                     if (cachePath != nullptr)
                     {
@@ -896,13 +1070,59 @@ extern "C"
                         }
                     }
                 }
-                b->rows = best % 100;
-                b->wantWindow = best >= 100;
+                decode(best);
                 b->builtKey.clear();
                 b->reducedValid = false;
                 if (chosenRows != nullptr)
                 {
                     *chosenRows = b->rows;
+                }
+            });
+    }
+
+    int halhip_batch_set_shapes(halhip_batch *b, const int32_t *shapes)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_set_shapes");
+                Require(shapes != nullptr, "halhip_batch_set_shapes: null shapes");
+                for (int32_t c = 0; c < kClasses; c++)
+                {
+                    Require(shapes[c] >= 0 && shapes[c] < kShapesPerClass,
+                            "halhip_batch_set_shapes: shape index out of range");
+                }
+                Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
+                for (int32_t c = 0; c < kClasses; c++)
+                {
+                    b->shapeIdx[c] = shapes[c];
+                }
+                b->builtKey.clear();
+                b->reducedValid = false;
+            });
+    }
+
+    int halhip_batch_get_shapes(halhip_batch *b, int32_t *shapes, int32_t *threads, int32_t *columnsPerThread)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_get_shapes");
+                for (int32_t c = 0; c < kClasses; c++)
+                {
+                    const BlockShape sh = ShapeOf(b, c);
+                    if (shapes != nullptr)
+                    {
+                        shapes[c] = b->shapeIdx[c];
+                    }
+                    if (threads != nullptr)
+                    {
+                        threads[c] = sh.threads;
+                    }
+                    if (columnsPerThread != nullptr)
+                    {
+                        columnsPerThread[c] = sh.cpt;
+                    }
                 }
             });
     }

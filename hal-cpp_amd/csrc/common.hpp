@@ -329,6 +329,18 @@ namespace halhip
         void Reset();
         int Export(int32_t maxKernels, int32_t *nKernels, char *names, int32_t namesLen, double *totalMs,
                    int64_t *launches);
+        // Total milliseconds collected under `name` (0 if none).
+        double TotalMs(const std::string &name) const
+        {
+            for (const auto &t : m_totals)
+            {
+                if (t.first == name)
+                {
+                    return t.second.first;
+                }
+            }
+            return 0.0;
+        }
 
     private:
         struct Pending

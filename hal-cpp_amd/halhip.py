@@ -149,6 +149,8 @@ _SIGS = {
     "halhip_batch_autotune": [_vp, _i32, _i32, _i32, ctypes.POINTER(HalBC), ctypes.c_char_p, _ip],
     "halhip_batch_set_profiling": [_vp, _i32],
     "halhip_batch_profile": [_vp, _i32, _ip, ctypes.c_char_p, _i32, _dp, _lp],
+    "halhip_batch_set_shapes": [_vp, _ip],
+    "halhip_batch_get_shapes": [_vp, _ip, _ip, _ip],
     "halhip_adjoint2d_create": [_vp, _i32, _i32, ctypes.POINTER(_vp)],
     "halhip_adjoint2d_destroy": [_vp],
     "halhip_adjoint2d_set_problem": [_vp, _dp, _f64, _i32, _i32],
@@ -701,6 +703,20 @@ class Batch(_Plan):
 
     def set_carries(self, method):
         _check(self._fn("set_carries")(self.h, method))
+
+    def set_shapes(self, shapes):
+        """Workgroup shape option (0..2) of each width class (<= 256, <= 512,
+        <= 1024 columns)."""
+        a = np.ascontiguousarray(shapes, np.int32)
+        assert a.size == 3
+        _check(self._fn("set_shapes")(self.h, a.ctypes.data_as(_ip)))
+
+    def shapes(self):
+        """(option, threads, columns per thread) of each width class."""
+        idx, thr, cpt = (np.zeros(3, np.int32) for _ in range(3))
+        _check(self._fn("get_shapes")(self.h, idx.ctypes.data_as(_ip), thr.ctypes.data_as(_ip),
+                                      cpt.ctypes.data_as(_ip)))
+        return [(int(i), int(t), int(c)) for i, t, c in zip(idx, thr, cpt)]
 
     def carries(self):
         m = ctypes.c_int32()

@@ -468,10 +468,24 @@ int halhip_batch_prepare_ticks(halhip_batch *batch, int32_t ticks, int32_t wrapX
 /* Strip height of the batched kernels (4 or 8 rows). */
 int halhip_batch_set_strip_rows(halhip_batch *batch, int32_t rows);
 int halhip_batch_get_strip_rows(halhip_batch *batch, int32_t *rows);
-/* Times every strip height, each with the scan kernel and with windowed
- * carries, on `ticks` ticks (state is restored afterwards) and keeps the
- * fastest; with a cache file, results are stored per batch signature (well
- * shapes, gfx target) as strip rows + 100 when windowed, and reused. */
+/* Wells are grouped by width (<= 256, <= 512, <= 1024 columns) and each
+ * width class runs as a launch of its own, with a workgroup shape picked from
+ * three options (threads x columns per thread):
+ *   <= 256:  0: 256 x 1   1: 128 x 2   2:   64 x 4
+ *   <= 512:  0: 256 x 2   1: 512 x 1   2:  128 x 4
+ *   <= 1024: 0: 256 x 4   1: 512 x 2   2: 1024 x 1
+ * set_shapes takes one option index per class (0 is the default);
+ * get_shapes reports them with their threads and columns per thread (any
+ * output may be null).  No counterpart in the reference. */
+int halhip_batch_set_shapes(halhip_batch *batch, const int32_t *shapes);
+int halhip_batch_get_shapes(halhip_batch *batch, int32_t *shapes, int32_t *threads, int32_t *columnsPerThread);
+/* Times strip heights {4, 8} x carries {scan, windowed} x every width class's
+ * workgroup shapes on `ticks` ticks (state is restored afterwards; each
+ * class's shape is timed on its own launches and chosen independently) and
+ * keeps the fastest; with a cache file, results are stored per batch
+ * signature (well shapes, gfx target) as rows + 100 * windowed + 1000 *
+ * shape(<= 256) + 10000 * shape(<= 512) + 100000 * shape(<= 1024), and
+ * reused. */
 int halhip_batch_autotune(halhip_batch *batch, int32_t ticks, int32_t wrapX, int32_t wrapY, const halhip_bc *bc,
                           const char *cachePath, int32_t *chosenRows);
 /* Carry method of the batch's sweep A across strips: HALHIP_ADI_CARRIES_SCAN

@@ -75,6 +75,32 @@ def test_wide_wells_and_boundaries(ctx, oracle, carries):
             assert rel_l2(b.well(got, k), ref[k]) < 1e-5, (k, wrap, bc)
 
 
+@pytest.mark.parametrize("shape", [0, 1, 2])
+@pytest.mark.parametrize("carries", CARRIES)
+def test_every_workgroup_shape_vs_oracle(ctx, oracle, carries, shape):
+    # every width class's workgroup shape option (threads x columns per
+    # thread), with widths at the class edges and ragged vector groups
+    xd = np.array([9, 20, 13, 33, 11, 7, 5], np.int32)
+    yd = np.array([64, 255, 257, 510, 513, 1022, 1024], np.int32)
+    spec = dict(xdims=xd, ydims=yd, rate=np.array([0.3, 5.0, 2.0, 1.0, 4.0, 0.7, 2.5]),
+                decay=np.array([0.0, 1e-3, 5e-3, 0.0, 1e-2, 2e-3, 0.0]),
+                src_well=np.array([0, 1, 2, 3, 4, 5, 6], np.int32),
+                src_cell=np.array([5, 254, 257 * 12 + 3, 509, 600, 1021, 4 * 1024 + 1023], np.int64),
+                src_val=np.array([1.0, 0.5, 0.25, 2.0, 1.5, 0.75, 1.25], np.float32))
+    b = make(ctx, spec, 8, carries)
+    b.set_shapes([shape] * 3)
+    f0 = np.random.default_rng(12).random(b.cells).astype(np.float32)
+    for wrap, bc in (((0, 0), None), ((0, 1), 0.3)):
+        b.upload(f0)
+        b.run_ticks(6, *wrap, bc=bc)
+        got = b.download()
+        ref = oracle_wells(oracle, spec, f0, b, 6, wrap, bc)
+        for k in range(len(xd)):
+            assert rel_l2(b.well(got, k), ref[k]) < 1e-5, (k, shape, wrap, bc)
+    assert [s[0] for s in b.shapes()] == [shape] * 3
+    b.close()
+
+
 @pytest.mark.parametrize("carries", CARRIES)
 def test_per_instance_loop_matches_batched(ctx, carries):
     spec = workloads.wells_batch(n_wells=40, seed=3, lo=16, hi=512)
@@ -98,10 +124,12 @@ def test_autotune_keeps_state_and_caches(ctx, tmp_path):
     assert rows in (4, 8)
     np.testing.assert_array_equal(b.download(), f0)
     code = int(open(cache).read().split()[-1])
-    assert code % 100 == rows and code // 100 in (0, 1)
+    shapes = [(code // 1000) % 10, (code // 10000) % 10, (code // 100000) % 10]
+    assert code % 100 == rows and (code // 100) % 10 in (0, 1) and max(shapes) <= 2
+    assert [s[0] for s in b.shapes()] == shapes
     assert b.autotune(5, cache_path=cache) == rows
     b.run_ticks(1)
-    assert b.carries() == (halhip.ADI_CARRIES_WINDOW if code >= 100 else halhip.ADI_CARRIES_SCAN)
+    assert b.carries() == (halhip.ADI_CARRIES_WINDOW if (code // 100) % 10 else halhip.ADI_CARRIES_SCAN)
 
 
 def test_workload_512_wells_first_ticks(ctx, oracle):
