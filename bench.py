@@ -242,7 +242,8 @@ def bench_batched(ctx, ticks=500, cpu=True):
     out = {"workload": "512 wells, sides log-uniform 64-1024, D~U[0.1,5], decay~U[0,1e-2], 32 sources/well, "
                        "zero flux", "cells": int(b.cells), "strip_rows": b.strip_rows(),
            "carries": "windowed" if b.carries() == halhip.ADI_CARRIES_WINDOW else "scan",
-           "workgroup_shapes": {f"width<={w}": f"{t}x{c}" for w, (_, t, c) in zip((256, 512, 1024), b.shapes())},
+           "workgroup_shapes": {f"width<={w}": f"{t} threads x {c} columns, {r}-row strips"
+                                for w, (_, t, c), r in zip((256, 512, 1024), b.shapes(), b.class_rows())},
            "batched_ms_per_tick": round(batched_ms, 4), "per_instance_ms_per_tick": round(loop_ms, 4),
            "batched_Gcell_s": round(b.cells / (batched_ms * 1e-3) / 1e9, 2),
            "per_instance_Gcell_s": round(b.cells / (loop_ms * 1e-3) / 1e9, 2),

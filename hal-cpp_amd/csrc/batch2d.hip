@@ -142,10 +142,25 @@ namespace
         return mode == 0 ? BatchStrip<T, CPT, S, 0, 0> : BatchStrip<T, CPT, S, 3, 0>;
     }
 
+    // Strip heights 4 and 8 for every shape; 16 where a thread holds at
+    // most 2 columns (the strip's rows stay in registers: rows x columns
+    // <= 32 values).
     template <int T, int CPT>
     BatchStripKernel PickBatchStripShape(int rows, int mode, bool window)
     {
+        if constexpr (CPT <= 2)
+        {
+            if (rows == 16)
+            {
+                return PickBatchStripRows<T, CPT, 16>(mode, window);
+            }
+        }
         return rows == 4 ? PickBatchStripRows<T, CPT, 4>(mode, window) : PickBatchStripRows<T, CPT, 8>(mode, window);
+    }
+
+    bool RowsAllowed(int32_t rows, int32_t cpt)
+    {
+        return rows == 4 || rows == 8 || (rows == 16 && cpt <= 2);
     }
 
     BatchStripKernel PickBatchStrip(BlockShape sh, int rows, int mode, bool window)
@@ -186,7 +201,7 @@ struct halhip_batch
     std::vector<int32_t> srcWell;
     std::vector<int64_t> srcCell;
     std::vector<float> srcVal;
-    int32_t rows = 8;
+    int32_t classRows[kClasses] = {8, 8, 8}; // strip height of each width class
     DeviceArray<float> field;
     DeviceArray<float> tables;
     DeviceArray<float> red;
@@ -241,7 +256,12 @@ namespace
         const bool dir = bc != nullptr && bc->kind == HALHIP_BC_DIRICHLET;
         const double bcValue = dir ? bc->value : 0.0;
         std::string key;
-        AppendKey(key, b->rows);
+        for (int32_t c = 0; c < kClasses; c++)
+        {
+            Require(RowsAllowed(b->classRows[c], ShapeOf(b, c).cpt),
+                    "halhip_batch: 16-row strips need a workgroup shape of at most 2 columns per thread");
+            AppendKey(key, b->classRows[c]);
+        }
         AppendKey(key, b->wantWindow);
         for (int32_t c = 0; c < kClasses; c++)
         {
@@ -262,7 +282,6 @@ namespace
             return;
         }
         Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
-        const int32_t S = b->rows;
         // Sources grouped by well then row (stable), as (x, y) pairs.
         std::vector<int32_t> order(b->srcWell.size());
         std::iota(order.begin(), order.end(), 0);
@@ -314,6 +333,7 @@ namespace
             const int32_t cls = WidthClass(Yw);
             b->wellClass[w] = cls;
             const BlockShape sh = ShapeOf(b, cls);
+            const int32_t S = b->classRows[cls];
             const int32_t cpt = sh.cpt;
             const int32_t P = (X + S - 1) / S;
             const int32_t xPad = P * S;
@@ -526,7 +546,7 @@ namespace
         }
         const BlockShape sh = ShapeOf(b, cls);
         b->prof.Begin(b->clock.stream, name);
-        hipLaunchKernelGGL(PickBatchStrip(sh, b->rows, mode, b->window), dim3(static_cast<unsigned>(count)),
+        hipLaunchKernelGGL(PickBatchStrip(sh, b->classRows[cls], mode, b->window), dim3(static_cast<unsigned>(count)),
                            dim3(static_cast<unsigned>(sh.threads)), 0, b->clock.stream,
                            static_cast<const BatchWell *>(b->wells.get()), items, b->field.get(), b->cur);
         Check(hipGetLastError(), "BatchStrip launch");
@@ -889,19 +909,8 @@ extern "C"
 
     int halhip_batch_set_strip_rows(halhip_batch *b, int32_t rows)
     {
-        return Guard(
-            [&]
-            {
-                RequireBatch(b, "halhip_batch_set_strip_rows");
-                Require(rows == 4 || rows == 8, "halhip_batch_set_strip_rows: rows must be 4 or 8");
-                if (rows != b->rows)
-                {
-                    Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
-                    b->rows = rows;
-                    b->builtKey.clear();
-                    b->reducedValid = false;
-                }
-            });
+        const int32_t all[kClasses] = {rows, rows, rows};
+        return halhip_batch_set_class_rows(b, all);
     }
 
     int halhip_batch_get_strip_rows(halhip_batch *b, int32_t *rows)
@@ -911,7 +920,46 @@ extern "C"
             {
                 RequireBatch(b, "halhip_batch_get_strip_rows");
                 Require(rows != nullptr, "halhip_batch_get_strip_rows: null output");
-                *rows = b->rows;
+                *rows = b->classRows[kClasses - 1];
+            });
+    }
+
+    int halhip_batch_set_class_rows(halhip_batch *b, const int32_t *rows)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_set_strip_rows");
+                Require(rows != nullptr, "halhip_batch_set_strip_rows: null rows");
+                for (int32_t c = 0; c < kClasses; c++)
+                {
+                    Require(rows[c] == 4 || rows[c] == 8 || rows[c] == 16,
+                            "halhip_batch_set_strip_rows: rows must be 4, 8 or 16");
+                }
+                Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
+                for (int32_t c = 0; c < kClasses; c++)
+                {
+                    if (rows[c] != b->classRows[c])
+                    {
+                        b->classRows[c] = rows[c];
+                        b->builtKey.clear();
+                        b->reducedValid = false;
+                    }
+                }
+            });
+    }
+
+    int halhip_batch_get_class_rows(halhip_batch *b, int32_t *rows)
+    {
+        return Guard(
+            [&]
+            {
+                RequireBatch(b, "halhip_batch_get_class_rows");
+                Require(rows != nullptr, "halhip_batch_get_class_rows: null output");
+                for (int32_t c = 0; c < kClasses; c++)
+                {
+                    rows[c] = b->classRows[c];
+                }
             });
     }
 
@@ -924,63 +972,107 @@ extern "C"
                 RequireBatch(b, "halhip_batch_autotune");
                 Require(ticks > 0, "halhip_batch_autotune: ticks must be positive");
                 const std::string sig = Signature(b);
-                std::map<std::string, int32_t> cache;
+                std::map<std::string, std::string> cache;
                 if (cachePath != nullptr)
                 {
                     std::ifstream in(cachePath);
                     std::string k;
-                    int32_t v;
+                    std::string v;
                     while (in >> k >> v)
                     {
                         cache[k] = v;
                     }
                 }
-                // A choice is cached as rows + 100 * windowed + 1000 *
-                // (shape of widths <= 256) + 10000 * (<= 512) + 100000 * (<= 1024).
-                auto decode = [&](int32_t code)
+                // A choice: windowed (0/1), then per width class its shape
+                // option and strip height, cached as "w:s0.r0,s1.r1,s2.r2".
+                struct Choice
                 {
-                    b->rows = code % 100;
-                    b->wantWindow = (code / 100) % 10 == 1;
-                    int32_t rest = code / 1000;
-                    for (int32_t c = 0; c < kClasses; c++)
-                    {
-                        b->shapeIdx[c] = rest % 10;
-                        rest /= 10;
-                    }
+                    int32_t window = 0;
+                    int32_t shape[kClasses] = {0, 0, 0};
+                    int32_t rows[kClasses] = {8, 8, 8};
                 };
-                auto valid = [](int32_t code)
+                auto encode = [](const Choice &c)
                 {
-                    if ((code % 100 != 4 && code % 100 != 8) || (code / 100) % 10 > 1 || code < 0)
+                    std::ostringstream o;
+                    o << c.window << ":";
+                    for (int32_t k = 0; k < kClasses; k++)
+                    {
+                        o << c.shape[k] << "." << c.rows[k] << (k + 1 < kClasses ? "," : "");
+                    }
+                    return o.str();
+                };
+                auto decode = [](const std::string &text, Choice &c)
+                {
+                    char sep[kClasses * 2 + 1];
+                    std::istringstream in(text);
+                    in >> c.window >> sep[0];
+                    for (int32_t k = 0; k < kClasses; k++)
+                    {
+                        in >> c.shape[k] >> sep[2 * k + 1] >> c.rows[k];
+                        if (k + 1 < kClasses)
+                        {
+                            in >> sep[2 * k + 2];
+                        }
+                    }
+                    if (!in || (c.window != 0 && c.window != 1))
                     {
                         return false;
                     }
-                    int32_t rest = code / 1000;
-                    for (int32_t c = 0; c < kClasses; c++)
+                    for (int32_t k = 0; k < kClasses; k++)
                     {
-                        if (rest % 10 >= kShapesPerClass)
+                        if (c.shape[k] < 0 || c.shape[k] >= kShapesPerClass ||
+                            !RowsAllowed(c.rows[k], kShapes[k][c.shape[k]].cpt))
                         {
                             return false;
                         }
-                        rest /= 10;
                     }
-                    return rest == 0;
+                    return true;
                 };
+                auto apply = [&](const Choice &c)
+                {
+                    b->wantWindow = c.window == 1;
+                    for (int32_t k = 0; k < kClasses; k++)
+                    {
+                        b->shapeIdx[k] = c.shape[k];
+                        b->classRows[k] = c.rows[k];
+                    }
+                    b->builtKey.clear();
+                    b->reducedValid = false;
+                };
+                Choice best;
+                bool found = false;
                 auto it = cache.find(sig);
-                int32_t best = -1;
-                if (it != cache.end() && valid(it->second))
+                if (it != cache.end())
                 {
-                    best = it->second;
+                    found = decode(it->second, best);
                 }
-                else
+                if (!found)
                 {
-                    // Candidates: strip rows {4, 8} x carries {scan, windowed}
-                    // x the workgroup shapes of each width class.  The width
-                    // classes run as launches of their own, so each class's
-                    // shape is timed on its own kernel events and chosen
-                    // independently; a (rows, carries) pair costs the sum of
-                    // its classes' best times plus its scan launches.  Timing
-                    // runs on a scratch copy of the fields, so the batch's
-                    // state is unchanged.
+                    // Candidates: carries {scan, windowed} x, per width class,
+                    // every (workgroup shape, strip height) its registers
+                    // allow.  The width classes run as launches of their own,
+                    // so each class's option is timed on its own kernel
+                    // events and chosen independently; a carry method costs
+                    // the sum of its classes' best times plus its scan
+                    // launches.  Candidate j runs option j of every class.
+                    // Timing runs on a scratch copy of the fields, so the
+                    // batch's state is unchanged.
+                    std::vector<std::pair<int32_t, int32_t>> opts[kClasses];
+                    size_t nCand = 0;
+                    for (int32_t k = 0; k < kClasses; k++)
+                    {
+                        for (int32_t sh = 0; sh < kShapesPerClass; sh++)
+                        {
+                            for (int32_t rows : {8, 4, 16})
+                            {
+                                if (RowsAllowed(rows, kShapes[k][sh].cpt))
+                                {
+                                    opts[k].push_back({sh, rows});
+                                }
+                            }
+                        }
+                        nCand = std::max(nCand, opts[k].size());
+                    }
                     DeviceArray<float> saved;
                     saved.Allocate(b->ctx, static_cast<size_t>(b->cells));
                     Check(hipMemcpyAsync(saved.get(), b->field.get(), static_cast<size_t>(b->cells) * sizeof(float),
@@ -988,81 +1080,94 @@ extern "C"
                           "hipMemcpyAsync");
                     const bool wasOn = b->prof.On();
                     double bestMs = 0.0;
-                    for (int32_t rows : {4, 8})
+                    for (int32_t win = 0; win < 2; win++)
                     {
-                        for (int32_t win = 0; win < 2; win++)
+                        Choice cand;
+                        cand.window = win;
+                        double classMs[kClasses] = {0.0, 0.0, 0.0};
+                        bool have[kClasses] = {false, false, false};
+                        double scanMs = 0.0;
+                        bool anyRun = false;
+                        for (size_t j = 0; j < nCand; j++)
                         {
-                            double classMs[kClasses];
-                            int32_t classBest[kClasses] = {0, 0, 0};
-                            double scanMs = 0.0;
-                            bool usable = true;
-                            for (int32_t i = 0; i < kShapesPerClass; i++)
+                            Choice c = cand;
+                            for (int32_t k = 0; k < kClasses; k++)
                             {
-                                b->rows = rows;
-                                b->wantWindow = win == 1;
-                                for (int32_t c = 0; c < kClasses; c++)
-                                {
-                                    b->shapeIdx[c] = i;
-                                }
-                                b->builtKey.clear();
-                                b->reducedValid = false;
-                                b->prof.SetOn(false);
-                                EnqueueBatchTicks(b, 2, wrapX, wrapY, bc);
-                                Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
-                                if (win == 1 && !b->window)
-                                {
-                                    usable = false; // some well's window is too wide
-                                    break;
-                                }
-                                b->prof.Reset();
-                                b->prof.SetOn(true);
-                                EnqueueBatchTicks(b, ticks, wrapX, wrapY, bc);
-                                b->prof.Collect();
-                                for (int32_t c = 0; c < kClasses; c++)
-                                {
-                                    const double ms = b->prof.TotalMs(kTickNames[c]);
-                                    if (i == 0 || ms < classMs[c])
-                                    {
-                                        classMs[c] = ms;
-                                        classBest[c] = i;
-                                    }
-                                }
-                                const double sm = b->prof.TotalMs("batch_scan");
-                                scanMs = (i == 0) ? sm : std::min(scanMs, sm);
+                                const auto &o = opts[k][std::min(j, opts[k].size() - 1)];
+                                c.shape[k] = o.first;
+                                c.rows[k] = o.second;
+                            }
+                            apply(c);
+                            b->prof.SetOn(false);
+                            EnqueueBatchTicks(b, 2, wrapX, wrapY, bc);
+                            Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
+                            if (win == 1 && !b->window)
+                            {
+                                // some well's window is too wide at these heights
                                 Check(hipMemcpyAsync(b->field.get(), saved.get(),
                                                      static_cast<size_t>(b->cells) * sizeof(float),
                                                      hipMemcpyDeviceToDevice, b->clock.stream),
                                       "hipMemcpyAsync");
-                            }
-                            if (!usable)
-                            {
                                 continue;
                             }
-                            double total = scanMs;
-                            int32_t code = rows + 100 * win;
-                            int32_t scale = 1000;
-                            for (int32_t c = 0; c < kClasses; c++)
+                            b->prof.Reset();
+                            b->prof.SetOn(true);
+                            EnqueueBatchTicks(b, ticks, wrapX, wrapY, bc);
+                            b->prof.Collect();
+                            for (int32_t k = 0; k < kClasses; k++)
                             {
-                                total += classMs[c];
-                                code += scale * classBest[c];
-                                scale *= 10;
+                                if (j >= opts[k].size())
+                                {
+                                    continue; // repeats this class's last option
+                                }
+                                const double ms = b->prof.TotalMs(kTickNames[k]);
+                                if (!have[k] || ms < classMs[k])
+                                {
+                                    classMs[k] = ms;
+                                    have[k] = true;
+                                    cand.shape[k] = c.shape[k];
+                                    cand.rows[k] = c.rows[k];
+                                }
                             }
-                            if (best < 0 || total < bestMs)
-                            {
-                                best = code;
-                                bestMs = total;
-                            }
+                            const double sm = b->prof.TotalMs("batch_scan");
+                            scanMs = anyRun ? std::min(scanMs, sm) : sm;
+                            anyRun = true;
+                            Check(hipMemcpyAsync(b->field.get(), saved.get(),
+                                                 static_cast<size_t>(b->cells) * sizeof(float),
+                                                 hipMemcpyDeviceToDevice, b->clock.stream),
+                                  "hipMemcpyAsync");
+                        }
+                        if (!anyRun || !have[0] || !have[1] || !have[2])
+                        {
+                            continue;
+                        }
+                        // the chosen per-class options must also allow the
+                        // carry method together
+                        apply(cand);
+                        EnqueueBatchTicks(b, 1, wrapX, wrapY, bc);
+                        Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
+                        Check(hipMemcpyAsync(b->field.get(), saved.get(), static_cast<size_t>(b->cells) * sizeof(float),
+                                             hipMemcpyDeviceToDevice, b->clock.stream),
+                              "hipMemcpyAsync");
+                        if (win == 1 && !b->window)
+                        {
+                            continue;
+                        }
+                        const double total = scanMs + classMs[0] + classMs[1] + classMs[2];
+                        if (!found || total < bestMs)
+                        {
+                            best = cand;
+                            bestMs = total;
+                            found = true;
                         }
                     }
                     b->prof.Reset();
                     b->prof.SetOn(wasOn);
-                    Check(hipMemcpyAsync(b->field.get(), saved.get(), static_cast<size_t>(b->cells) * sizeof(float),
-                                         hipMemcpyDeviceToDevice, b->clock.stream),
-                          "hipMemcpyAsync");
                     Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
+                    Require(found, "halhip_batch_autotune: no candidate ran");
                     if (cachePath != nullptr)
                     {
-                        cache[sig] = best;
+                        cache[sig] = encode(best);
                         std::ofstream outFile(cachePath, std::ios::trunc);
                         for (auto &kv : cache)
                         {
@@ -1070,12 +1175,10 @@ extern "C"
                         }
                     }
                 }
-                decode(best);
-                b->builtKey.clear();
-                b->reducedValid = false;
+                apply(best);
                 if (chosenRows != nullptr)
                 {
-                    *chosenRows = b->rows;
+                    *chosenRows = b->classRows[kClasses - 1];
                 }
             });
     }

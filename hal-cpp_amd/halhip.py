@@ -149,6 +149,8 @@ _SIGS = {
     "halhip_batch_autotune": [_vp, _i32, _i32, _i32, ctypes.POINTER(HalBC), ctypes.c_char_p, _ip],
     "halhip_batch_set_profiling": [_vp, _i32],
     "halhip_batch_profile": [_vp, _i32, _ip, ctypes.c_char_p, _i32, _dp, _lp],
+    "halhip_batch_set_class_rows": [_vp, _ip],
+    "halhip_batch_get_class_rows": [_vp, _ip],
     "halhip_batch_set_shapes": [_vp, _ip],
     "halhip_batch_get_shapes": [_vp, _ip, _ip, _ip],
     "halhip_adjoint2d_create": [_vp, _i32, _i32, ctypes.POINTER(_vp)],
@@ -710,6 +712,17 @@ class Batch(_Plan):
         a = np.ascontiguousarray(shapes, np.int32)
         assert a.size == 3
         _check(self._fn("set_shapes")(self.h, a.ctypes.data_as(_ip)))
+
+    def set_class_rows(self, rows):
+        """Strip height (4, 8 or 16) of each width class."""
+        a = np.ascontiguousarray(rows, np.int32)
+        assert a.size == 3
+        _check(self._fn("set_class_rows")(self.h, a.ctypes.data_as(_ip)))
+
+    def class_rows(self):
+        a = np.zeros(3, np.int32)
+        _check(self._fn("get_class_rows")(self.h, a.ctypes.data_as(_ip)))
+        return [int(v) for v in a]
 
     def shapes(self):
         """(option, threads, columns per thread) of each width class."""

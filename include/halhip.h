@@ -465,9 +465,14 @@ int halhip_batch_sync(halhip_batch *batch, float *elapsedMs);
  * reference (MultiWellExperiment steps wells on the host). */
 int halhip_batch_prepare_ticks(halhip_batch *batch, int32_t ticks, int32_t wrapX, int32_t wrapY,
                                const halhip_bc *bc);
-/* Strip height of the batched kernels (4 or 8 rows). */
+/* Strip height of the batched kernels: 4, 8 or 16 rows (16 only with a
+ * workgroup shape of at most 2 columns per thread).  set_strip_rows sets
+ * every width class (see halhip_batch_set_shapes); set_class_rows sets one
+ * height per class; get_strip_rows reports the widest class's. */
 int halhip_batch_set_strip_rows(halhip_batch *batch, int32_t rows);
 int halhip_batch_get_strip_rows(halhip_batch *batch, int32_t *rows);
+int halhip_batch_set_class_rows(halhip_batch *batch, const int32_t *rows);
+int halhip_batch_get_class_rows(halhip_batch *batch, int32_t *rows);
 /* Wells are grouped by width (<= 256, <= 512, <= 1024 columns) and each
  * width class runs as a launch of its own, with a workgroup shape picked from
  * three options (threads x columns per thread):
@@ -479,13 +484,13 @@ int halhip_batch_get_strip_rows(halhip_batch *batch, int32_t *rows);
  * output may be null).  No counterpart in the reference. */
 int halhip_batch_set_shapes(halhip_batch *batch, const int32_t *shapes);
 int halhip_batch_get_shapes(halhip_batch *batch, int32_t *shapes, int32_t *threads, int32_t *columnsPerThread);
-/* Times strip heights {4, 8} x carries {scan, windowed} x every width class's
- * workgroup shapes on `ticks` ticks (state is restored afterwards; each
- * class's shape is timed on its own launches and chosen independently) and
- * keeps the fastest; with a cache file, results are stored per batch
- * signature (well shapes, gfx target) as rows + 100 * windowed + 1000 *
- * shape(<= 256) + 10000 * shape(<= 512) + 100000 * shape(<= 1024), and
- * reused. */
+/* Times carries {scan, windowed} x, per width class, every workgroup shape
+ * with every strip height its registers allow, on `ticks` ticks (state is
+ * restored afterwards; each class's option is timed on its own launches and
+ * chosen independently), and keeps the fastest.  With a cache file, results
+ * are stored per batch signature (well shapes, gfx target) as
+ * "windowed:shape.rows,shape.rows,shape.rows" and reused.  chosenRows gets
+ * the widest class's strip height. */
 int halhip_batch_autotune(halhip_batch *batch, int32_t ticks, int32_t wrapX, int32_t wrapY, const halhip_bc *bc,
                           const char *cachePath, int32_t *chosenRows);
 /* Carry method of the batch's sweep A across strips: HALHIP_ADI_CARRIES_SCAN

@@ -75,9 +75,10 @@ def test_wide_wells_and_boundaries(ctx, oracle, carries):
             assert rel_l2(b.well(got, k), ref[k]) < 1e-5, (k, wrap, bc)
 
 
+@pytest.mark.parametrize("rows", [8, 16, 4])
 @pytest.mark.parametrize("shape", [0, 1, 2])
 @pytest.mark.parametrize("carries", CARRIES)
-def test_every_workgroup_shape_vs_oracle(ctx, oracle, carries, shape):
+def test_every_workgroup_shape_vs_oracle(ctx, oracle, carries, shape, rows):
     # every width class's workgroup shape option (threads x columns per
     # thread), with widths at the class edges and ragged vector groups
     xd = np.array([9, 20, 13, 33, 11, 7, 5], np.int32)
@@ -89,6 +90,10 @@ def test_every_workgroup_shape_vs_oracle(ctx, oracle, carries, shape):
                 src_val=np.array([1.0, 0.5, 0.25, 2.0, 1.5, 0.75, 1.25], np.float32))
     b = make(ctx, spec, 8, carries)
     b.set_shapes([shape] * 3)
+    # 16-row strips need at most 2 columns per thread: the widest class's
+    # default shape (4 columns) keeps 8
+    cls_rows = [rows if (rows != 16 or cpt <= 2) else 8 for (_, _, cpt) in b.shapes()]
+    b.set_class_rows(cls_rows)
     f0 = np.random.default_rng(12).random(b.cells).astype(np.float32)
     for wrap, bc in (((0, 0), None), ((0, 1), 0.3)):
         b.upload(f0)
@@ -96,8 +101,9 @@ def test_every_workgroup_shape_vs_oracle(ctx, oracle, carries, shape):
         got = b.download()
         ref = oracle_wells(oracle, spec, f0, b, 6, wrap, bc)
         for k in range(len(xd)):
-            assert rel_l2(b.well(got, k), ref[k]) < 1e-5, (k, shape, wrap, bc)
+            assert rel_l2(b.well(got, k), ref[k]) < 1e-5, (k, shape, cls_rows, wrap, bc)
     assert [s[0] for s in b.shapes()] == [shape] * 3
+    assert b.class_rows() == cls_rows
     b.close()
 
 
@@ -121,15 +127,23 @@ def test_autotune_keeps_state_and_caches(ctx, tmp_path):
     b.upload(f0)
     cache = str(tmp_path / "tune.txt")
     rows = b.autotune(5, cache_path=cache)
-    assert rows in (4, 8)
+    assert rows in (4, 8, 16)
     np.testing.assert_array_equal(b.download(), f0)
-    code = int(open(cache).read().split()[-1])
-    shapes = [(code // 1000) % 10, (code // 10000) % 10, (code // 100000) % 10]
-    assert code % 100 == rows and (code // 100) % 10 in (0, 1) and max(shapes) <= 2
-    assert [s[0] for s in b.shapes()] == shapes
+    # "windowed:shape.rows,shape.rows,shape.rows"
+    win, per = open(cache).read().split()[-1].split(":")
+    pairs = [tuple(int(v) for v in p.split(".")) for p in per.split(",")]
+    assert win in ("0", "1") and len(pairs) == 3
+    assert [s[0] for s in b.shapes()] == [p[0] for p in pairs]
+    assert b.class_rows() == [p[1] for p in pairs] and rows == pairs[2][1]
     assert b.autotune(5, cache_path=cache) == rows
     b.run_ticks(1)
-    assert b.carries() == (halhip.ADI_CARRIES_WINDOW if (code // 100) % 10 else halhip.ADI_CARRIES_SCAN)
+    assert b.carries() == (halhip.ADI_CARRIES_WINDOW if win == "1" else halhip.ADI_CARRIES_SCAN)
+    got = b.download()
+    # the tuned configuration against the default one, tick for tick
+    d = make(ctx, spec, 8)
+    d.upload(f0)
+    d.run_ticks(1)
+    assert rel_l2(got, d.download()) < 1e-6
 
 
 def test_workload_512_wells_first_ticks(ctx, oracle):
