@@ -13,12 +13,12 @@
 // the previous binning, with id[k] the agent index held in slot k (after an
 // upload, slot k holds agent k).  Agents move little per tick, so slot order
 // stays close to spatial order and every pass below touches memory almost in
-// order.  A tick is seven launches, captured in one hipGraph for multi-tick
-// runs:
+// order.  A tick is four launches (the count is part of the previous tick's
+// force pass after the first), captured in hipGraphs for multi-tick runs:
 //   count    per slot: cell = clamped floor cell; ticket = atomicAdd(count[cell], 1)
-//   scan     offsets = exclusive prefix of counts (three passes: per-block
-//            sums, scan of the block sums, block-local scan + block prefix;
-//            wavefront shuffles + LDS inside a block); the last pass also
+//   scan     offsets = exclusive prefix of counts in one pass: tiles claimed
+//            in order, each publishing its total and summing its
+//            predecessors' published totals (a bounded wait); the pass also
 //            resets the counts to 0 for the next binning
 //   scatter  (id, position, radius) of each slot to offsets[cell] + ticket
 //            in the other buffer set (a permutation)
@@ -132,50 +132,24 @@ namespace
         }
     }
 
-    // Pass 2: exclusive scan of the tile totals in place (one block, any count).
-    __global__ __launch_bounds__(1024) void ScanTileOffsets(int32_t *__restrict__ tileSums, int32_t nTiles)
-    {
-        __shared__ int sWave[16];
-        int carry = 0;
-        for (int base = 0; base < nTiles; base += blockDim.x)
-        {
-            const int i = base + threadIdx.x;
-            const int v = (i < nTiles) ? tileSums[i] : 0;
-            int total;
-            const int ex = BlockExclusiveSum(v, sWave, total);
-            if (i < nTiles)
-            {
-                tileSums[i] = carry + ex;
-            }
-            carry += total;
-        }
-    }
-
-    // Pass 3: offsets = tile prefix + in-tile exclusive prefix (offsets past
-    // the last real cell read N), and the counts reset to 0 for the next
-    // binning.  With fromSums the tile prefix is summed here from the
-    // unscanned tile totals (few tiles: pass 2 is skipped); otherwise
-    // tileOffsets holds pass 2's prefixes.
+    // Pass 2: offsets = tile prefix (summed here from the tile totals) +
+    // in-tile exclusive prefix (offsets past the last real cell read N), and
+    // the counts reset to 0 for the next binning.
     __global__ __launch_bounds__(kScanThreads) void ScanApply(int4 *__restrict__ counts,
-                                                              const int32_t *__restrict__ tileOffsets,
+                                                              const int32_t *__restrict__ tileSums,
                                                               int4 *__restrict__ offsets, int32_t *__restrict__ offsetsTail,
-                                                              int32_t N, int32_t fromSums)
+                                                              int32_t N)
     {
         __shared__ int sWave[kScanThreads / 64];
         const int64_t tileBase = static_cast<int64_t>(blockIdx.x) * (kScanTile / 4);
         int run;
-        if (fromSums)
         {
             int part = 0;
             for (int t = threadIdx.x; t < static_cast<int>(blockIdx.x); t += kScanThreads)
             {
-                part += tileOffsets[t];
+                part += tileSums[t];
             }
             BlockExclusiveSum(part, sWave, run);
-        }
-        else
-        {
-            run = tileOffsets[blockIdx.x];
         }
 #pragma unroll
         for (int r = 0; r < kScanRounds; r++)
@@ -194,6 +168,121 @@ namespace
         }
     }
 
+    // One-pass scan.  Workgroup t takes tile t.  Each tile publishes its
+    // total as soon as its counts are summed, then sums the published totals
+    // of all its predecessors (one round trip: every thread polls a few of
+    // them) for its prefix.  A published word is (epoch << 32) | total, so a
+    // word from an earlier scan is never taken for this one and the words
+    // need no clearing.  The last tile has seen every other tile's word, so
+    // every workgroup has read the epoch by then: it advances the epoch.
+    // Waits are on lower tiles only; workgroups are dispatched in index
+    // order, so the lowest unfinished tile never waits on one not running.
+    // The wait is bounded all the same: a thread that has polled spinLimit
+    // times without seeing a total stops waiting and raises *fault (sticky);
+    // the workgroup finishes, the kernels that read the offsets do nothing
+    // while the fault is up, and the host reports it and clears the scan
+    // state.  No counter is shared by all workgroups (such atomics serialize
+    // at the coherence point).  A published word carries its epoch and value
+    // together and no other data is read on its strength, so relaxed
+    // agent-scope atomics suffice: they reach the coherence point shared by
+    // the XCDs without the whole-L2 write-back and invalidation that release
+    // and acquire would add.
+    struct ScanState
+    {
+        unsigned long long *tiles; // [nTiles] published totals
+        int32_t *words;            // [2] epoch, [3] fault ([0], [1] unused)
+        int32_t spinLimit;
+    };
+
+    __global__ __launch_bounds__(kScanThreads) void ScanOnePass(int4 *__restrict__ counts,
+                                                                int4 *__restrict__ offsets,
+                                                                int32_t *__restrict__ offsetsTail, int32_t N,
+                                                                int32_t nTiles, ScanState st)
+    {
+        __shared__ int sWave[kScanThreads / 64];
+        __shared__ uint32_t sEpoch;
+        if (threadIdx.x == 0)
+        {
+            sEpoch = static_cast<uint32_t>(__hip_atomic_load(&st.words[2], __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT));
+        }
+        __syncthreads();
+        const int tile = static_cast<int>(blockIdx.x);
+        const uint32_t epoch = sEpoch;
+        const int64_t tileBase = static_cast<int64_t>(tile) * (kScanTile / 4);
+        int4 v[kScanRounds];
+#pragma unroll
+        for (int r = 0; r < kScanRounds; r++)
+        {
+            v[r] = counts[tileBase + r * kScanThreads + threadIdx.x];
+        }
+        int ex[kScanRounds];
+        int tileTotal = 0;
+#pragma unroll
+        for (int r = 0; r < kScanRounds; r++)
+        {
+            int total;
+            ex[r] = tileTotal + BlockExclusiveSum((v[r].x + v[r].y) + (v[r].z + v[r].w), sWave, total);
+            tileTotal += total;
+        }
+        if (threadIdx.x == 0)
+        {
+            __hip_atomic_store(&st.tiles[tile],
+                               (static_cast<unsigned long long>(epoch) << 32) | static_cast<uint32_t>(tileTotal),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // the prefix: every predecessor's published total
+        int part = 0;
+        int spins = 0;
+        bool gaveUp = false;
+        for (int j = threadIdx.x; j < tile && !gaveUp; j += kScanThreads)
+        {
+            for (;;)
+            {
+                const unsigned long long w =
+                    __hip_atomic_load(&st.tiles[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (static_cast<uint32_t>(w >> 32) == epoch)
+                {
+                    part += static_cast<int>(static_cast<uint32_t>(w));
+                    break;
+                }
+                if (++spins > st.spinLimit)
+                {
+                    gaveUp = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        if (gaveUp)
+        {
+            atomicExch(&st.words[3], 1);
+        }
+        int run;
+        BlockExclusiveSum(part, sWave, run); // (its barriers: every poll above is done)
+        if (tile == nTiles - 1 && threadIdx.x == 0)
+        {
+            // every other tile published, so every workgroup has read the
+            // epoch: the next scan's words are told apart from these
+            // (epoch 0 is the cleared words' own, never used)
+            const uint32_t next = (epoch + 1u == 0u) ? 1u : epoch + 1u;
+            __hip_atomic_store(&st.words[2], static_cast<int32_t>(next), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+#pragma unroll
+        for (int r = 0; r < kScanRounds; r++)
+        {
+            const int64_t i = tileBase + r * kScanThreads + threadIdx.x;
+            const int o = run + ex[r];
+            offsets[i] = make_int4(o, o + v[r].x, o + v[r].x + v[r].y, o + v[r].x + v[r].y + v[r].z);
+            counts[i] = make_int4(0, 0, 0, 0);
+        }
+        if (tile == 0 && threadIdx.x == 0)
+        {
+            *offsetsTail = N;
+        }
+    }
+
     // Moves slot k's (id, position, radius) to slot offsets[cell] + ticket.
     __global__ __launch_bounds__(kThreads) void AgentScatter(int32_t N, const int32_t *__restrict__ cellOf,
                                                              const int32_t *__restrict__ ticketOf,
@@ -203,12 +292,13 @@ namespace
                                                              const float *__restrict__ rad,
                                                              const float2 *__restrict__ carry,
                                                              int32_t *__restrict__ nId, float2 *__restrict__ nPos,
-                                                             float *__restrict__ nRad, float2 *__restrict__ nCarry)
+                                                             float *__restrict__ nRad, float2 *__restrict__ nCarry,
+                                                             const int32_t *__restrict__ scanFault)
     {
         const int k = XcdContiguousBlock(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-        if (k >= N)
+        if (k >= N || *scanFault != 0)
         {
-            return;
+            return; // (a scan that gave up left offsets that must not be used)
         }
         const int dst = offsets[cellOf[k]] + ticketOf[k];
         nId[dst] = id[k];
@@ -263,11 +353,14 @@ namespace
     // 4 entries, and c0 is a multiple of 4, so both loads are in bounds).
     __global__ __launch_bounds__(kThreads) void AgentOrderCells(int64_t C, const int32_t *__restrict__ offsets,
                                                                 int32_t *__restrict__ id, float2 *__restrict__ pos,
-                                                                float *__restrict__ rad, float2 *__restrict__ carry)
+                                                                float *__restrict__ rad, float2 *__restrict__ carry,
+                                                                const int32_t *__restrict__ scanFault)
     {
         const int64_t c0 =
             4 * (static_cast<int64_t>(XcdContiguousBlock(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x);
-        if (c0 >= C)
+        // Offsets from a scan that gave up are not monotonic: one cell's
+        // range could span most slots, which no insertion sort may walk.
+        if (c0 >= C || *scanFault != 0)
         {
             return;
         }
@@ -480,14 +573,14 @@ namespace
                                                             const int32_t *__restrict__ offsets,
                                                             const float2 *__restrict__ spos,
                                                             const float *__restrict__ srad, float2 *__restrict__ vel,
-                                                            MoveArgs move)
+                                                            MoveArgs move, const int32_t *__restrict__ scanFault)
     {
         // Each XCD takes a contiguous range of slots: the neighbour rows a
         // slot reads (about Y slots away) then sit in the same L2.
         const int k = XcdContiguousBlock(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-        if (k >= N)
+        if (k >= N || *scanFault != 0)
         {
-            return;
+            return; // (no neighbour ranges from a scan that gave up)
         }
         const float2 p = spos[k];
         const float ri = srad[k];
@@ -773,6 +866,10 @@ struct halhip_agents2d
     DeviceArray<int32_t> counts;
     DeviceArray<int32_t> offsets;
     DeviceArray<int32_t> tileSums;
+    // one-pass scan: published tile words and [ticket, finished, epoch, fault]
+    DeviceArray<unsigned long long> scanTiles;
+    DeviceArray<int32_t> scanWords;
+    int32_t scanSpinLimit = 1 << 20;
     DeviceArray<int32_t> cellOf;
     DeviceArray<int32_t> ticketOf;
     bool binsValid = false;
@@ -800,6 +897,38 @@ namespace
         p->prof.End(p->clock.stream);
     }
 
+    // Published words cleared, epoch 1, no fault (synchronous).
+    void ResetScanState(halhip_agents2d *p)
+    {
+        Check(hipMemsetAsync(p->scanTiles.get(), 0, static_cast<size_t>(p->nTiles) * sizeof(unsigned long long),
+                             p->clock.stream),
+              "hipMemsetAsync");
+        const int32_t words[4] = {0, 0, 1, 0};
+        Check(hipMemcpyAsync(p->scanWords.get(), words, sizeof(words), hipMemcpyHostToDevice, p->clock.stream),
+              "hipMemcpyAsync");
+        Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+    }
+
+    // Reports (and clears) a bin scan whose wait gave up since the last
+    // check; the bins it produced are not to be trusted.
+    void CheckScanFault(halhip_agents2d *p, const char *caller)
+    {
+        int32_t fault = 0;
+        Check(hipMemcpyAsync(&fault, p->scanWords.get() + 3, sizeof(fault), hipMemcpyDeviceToHost, p->clock.stream),
+              "hipMemcpyAsync");
+        Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+        if (fault != 0)
+        {
+            // a scan that gave up may have left words a later scan could
+            // take for its own: clear them all and restart the epochs
+            ResetScanState(p);
+            p->binsValid = false;
+            Fail(HALHIP_ERR_HIP, std::string(caller) + ": the bin scan's look-back wait gave up "
+                                                       "(spin limit reached); bins and positions since the last "
+                                                       "check are invalid");
+        }
+    }
+
     // Resident positions in slot order of set `cur`.
     float2 *CurPositions(halhip_agents2d *p)
     {
@@ -822,18 +951,9 @@ namespace
         }
         int4 *counts4 = reinterpret_cast<int4 *>(p->counts.get());
         const int64_t cPad = static_cast<int64_t>(p->nTiles) * kScanTile;
-        Launch(p, "agents_scan_tiles", ScanTileSums, static_cast<unsigned>(p->nTiles), kScanThreads,
-               static_cast<const int4 *>(counts4), p->tileSums.get());
-        // up to kDirectTiles tiles, each block of pass 3 sums its own prefix
-        constexpr int32_t kDirectTiles = 4096;
-        const bool direct = p->nTiles <= kDirectTiles;
-        if (!direct)
-        {
-            Launch(p, "agents_scan_tile_offsets", ScanTileOffsets, 1u, 1024u, p->tileSums.get(), p->nTiles);
-        }
-        Launch(p, "agents_scan_apply", ScanApply, static_cast<unsigned>(p->nTiles), kScanThreads, counts4,
-               static_cast<const int32_t *>(p->tileSums.get()), reinterpret_cast<int4 *>(p->offsets.get()),
-               p->offsets.get() + cPad, N, direct ? 1 : 0);
+        ScanState st{p->scanTiles.get(), p->scanWords.get(), p->scanSpinLimit};
+        Launch(p, "agents_scan", ScanOnePass, static_cast<unsigned>(p->nTiles), kScanThreads, counts4,
+               reinterpret_cast<int4 *>(p->offsets.get()), p->offsets.get() + cPad, N, p->nTiles, st);
         if (N > 0)
         {
             Launch(p, "agents_scatter", AgentScatter, Blocks(N, kThreads), kThreads, N,
@@ -841,10 +961,12 @@ namespace
                    static_cast<const int32_t *>(p->offsets.get()), static_cast<const int32_t *>(p->id[c].get()),
                    positions, static_cast<const float *>(p->rad[c].get()),
                    static_cast<const float2 *>(p->motion ? p->carry[c].get() : nullptr), p->id[n].get(),
-                   p->pos[n].get(), p->rad[n].get(), p->motion ? p->carry[n].get() : nullptr);
+                   p->pos[n].get(), p->rad[n].get(), p->motion ? p->carry[n].get() : nullptr,
+                   static_cast<const int32_t *>(p->scanWords.get() + 3));
             Launch(p, "agents_order", AgentOrderCells, Blocks((p->C + 3) / 4, kThreads), kThreads, p->C,
                    static_cast<const int32_t *>(p->offsets.get()), p->id[n].get(), p->pos[n].get(),
-                   p->rad[n].get(), p->motion ? p->carry[n].get() : nullptr);
+                   p->rad[n].get(), p->motion ? p->carry[n].get() : nullptr,
+                   static_cast<const int32_t *>(p->scanWords.get() + 3));
         }
         p->cur = n;
         p->moved = false;
@@ -873,7 +995,8 @@ namespace
                             : (wrapY ? AgentForces<false, true> : AgentForces<false, false>);
         Launch(p, "agents_force", kernel, Blocks(p->N, kThreads), kThreads, p->N, p->X, p->Y, R, k,
                static_cast<const int32_t *>(p->offsets.get()), static_cast<const float2 *>(p->pos[c].get()),
-               static_cast<const float *>(p->rad[c].get()), p->vel.get(), m);
+               static_cast<const float *>(p->rad[c].get()), p->vel.get(), m,
+               static_cast<const int32_t *>(p->scanWords.get() + 3));
         p->moved = move;
     }
 
@@ -1070,8 +1193,12 @@ extern "C"
                 p->counts.Allocate(ctx, cPad);
                 p->offsets.Allocate(ctx, cPad + 4);
                 p->tileSums.Allocate(ctx, static_cast<size_t>(p->nTiles));
+                p->scanTiles.Allocate(ctx, static_cast<size_t>(p->nTiles));
+                p->scanWords.Allocate(ctx, 4);
                 p->clock.Create();
                 Check(hipMemsetAsync(p->counts.get(), 0, cPad * sizeof(int32_t), p->clock.stream), "hipMemsetAsync");
+                // published words of epoch 0 never match the first scan's epoch 1
+                ResetScanState(p);
                 Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
                 *out = p;
             });
@@ -1127,6 +1254,7 @@ extern "C"
                     return;
                 }
                 DownloadByIndex(p, CurPositions(p), positions);
+                CheckScanFault(p, "halhip_agents2d_download_positions");
             });
     }
 
@@ -1144,6 +1272,7 @@ extern "C"
                     return;
                 }
                 DownloadByIndex(p, p->vel.get(), velocityDeltas);
+                CheckScanFault(p, "halhip_agents2d_download_velocities");
             });
     }
 
@@ -1239,6 +1368,20 @@ extern "C"
                           "hipMemcpyAsync");
                 }
                 Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                CheckScanFault(p, "halhip_agents2d_download_bins");
+            });
+    }
+
+    int halhip_agents2d_set_scan_spin_limit(halhip_agents2d *p, int32_t limit)
+    {
+        return Guard(
+            [&]
+            {
+                RequirePlan(p, "halhip_agents2d_set_scan_spin_limit");
+                Require(limit >= 0, "halhip_agents2d_set_scan_spin_limit: negative limit");
+                Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                p->scanSpinLimit = limit;
+                p->graphs.Clear(); // they hold the old limit
             });
     }
 
@@ -1276,6 +1419,7 @@ extern "C"
                 }
                 Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
                 p->prof.Collect();
+                CheckScanFault(p, "halhip_agents2d_forces_f64");
                 for (size_t i = 0; i < 2 * n; i++)
                 {
                     velocityDeltas[i] = static_cast<double>(p->hostBuf[i]);
@@ -1396,7 +1540,7 @@ extern "C"
                 Check(hipGetLastError(), "ScanTileSums launch");
                 hipLaunchKernelGGL(ScanApply, dim3(static_cast<unsigned>(nTiles)), dim3(kScanThreads), 0, s,
                                    reinterpret_cast<int4 *>(counts), static_cast<const int32_t *>(tiles),
-                                   reinterpret_cast<int4 *>(offs), offs + cPad, nAgents, 1);
+                                   reinterpret_cast<int4 *>(offs), offs + cPad, nAgents);
                 Check(hipGetLastError(), "ScanApply launch");
                 if (nAgents > 0)
                 {

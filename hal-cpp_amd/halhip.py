@@ -151,6 +151,7 @@ _SIGS = {
     "halhip_batch_profile": [_vp, _i32, _ip, ctypes.c_char_p, _i32, _dp, _lp],
     "halhip_batch_set_class_rows": [_vp, _ip],
     "halhip_batch_get_class_rows": [_vp, _ip],
+    "halhip_agents2d_set_scan_spin_limit": [_vp, _i32],
     "halhip_batch_set_shapes": [_vp, _ip],
     "halhip_batch_get_shapes": [_vp, _ip, _ip, _ip],
     "halhip_adjoint2d_create": [_vp, _i32, _i32, ctypes.POINTER(_vp)],

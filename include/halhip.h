@@ -370,6 +370,13 @@ int halhip_agents2d_download_velocities(halhip_agents2d *plan, float *velocityDe
  * per-bin counts) and bucket (nAgents int32, agent indices in bin order,
  * ascending index within a bin). */
 int halhip_agents2d_download_bins(halhip_agents2d *plan, int32_t *offsets, int32_t *bucket);
+/* Bound on the polls the one-pass bin scan makes while it waits for an
+ * earlier tile's published total (default 2^20).  A scan that reaches it
+ * stops waiting and finishes; the next download or per-call forces call then
+ * fails with HALHIP_ERR_HIP and the bins must be recomputed (upload again).
+ * A limit of 0 lets tests force that path.  No counterpart in the
+ * reference. */
+int halhip_agents2d_set_scan_spin_limit(halhip_agents2d *plan, int32_t limit);
 
 /* Velocity carried across ticks, as SphericalAgent2D keeps xVel/yVel
  * (include/Hal/Agents/SphericalAgent2D.h:100-156): each tick the velocity

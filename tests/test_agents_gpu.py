@@ -126,8 +126,8 @@ def test_ticks_vs_oracle_small(ctx, oracle, wrap):
 
 
 def test_bins_on_a_grid_of_more_than_4096_scan_tiles(ctx, oracle):
-    # Above 4096 scan tiles (16 M cells) the tile prefixes take a pass of
-    # their own; below it each block sums them itself.
+    # 4305 scan tiles: the one-pass scan's look-back crosses many windows of
+    # 64 predecessors.
     rng = np.random.default_rng(23)
     X, Y, n = 4200, 4100, 3000
     pos = (rng.random(2 * n) * np.tile([X, Y], n)).astype(np.float32)
@@ -140,6 +140,40 @@ def test_bins_on_a_grid_of_more_than_4096_scan_tiles(ctx, oracle):
     ro, rb = oracle.agent_bins(pos.astype(np.float64), X, Y)
     np.testing.assert_array_equal(offsets, ro)
     np.testing.assert_array_equal(bucket, rb)
+
+
+def test_scan_wait_limit_fails_loudly_never_silently(ctx, oracle):
+    # With a spin limit of 0 a scan whose predecessor tile has not yet
+    # published gives up at once: the bins are then reported as failed, never
+    # returned wrong.  Restoring the limit and binning again gives exact bins.
+    rng = np.random.default_rng(29)
+    X, Y, n = 2048, 2048, 200000
+    pos = (rng.random(2 * n) * np.tile([X, Y], n)).astype(np.float32)
+    rad = np.full(n, 0.5, np.float32)
+    ro, rb = oracle.agent_bins(pos.astype(np.float64), X, Y)
+    plan = halhip.Agents2D(ctx, X, Y, n)
+    plan._fn("set_scan_spin_limit")(plan.h, 0)
+    outcomes = []
+    for _ in range(6):
+        plan.upload(pos, rad)
+        plan.run_ticks(0, 1.0, 1.0, 0.0, 0, 0)
+        try:
+            offsets, bucket = plan.bins()
+        except halhip.HalHipError as e:
+            assert "look-back" in str(e)
+            outcomes.append("failed")
+            continue
+        np.testing.assert_array_equal(offsets, ro)
+        np.testing.assert_array_equal(bucket, rb)
+        outcomes.append("exact")
+    print("spin limit 0:", outcomes)
+    plan._fn("set_scan_spin_limit")(plan.h, 1 << 20)
+    plan.upload(pos, rad)
+    plan.run_ticks(0, 1.0, 1.0, 0.0, 0, 0)
+    offsets, bucket = plan.bins()
+    np.testing.assert_array_equal(offsets, ro)
+    np.testing.assert_array_equal(bucket, rb)
+    plan.close()
 
 
 def test_graph_chunks_from_fresh_upload_and_forces_only(ctx, oracle):
