@@ -486,8 +486,9 @@ namespace
         int32_t blocks;    // column blocks, ceil(Y / C)
         const float *in;   // [X][Y]
         const float *D;    // [X][Y] face rates
-        const float *ib;   // [X][Y] reciprocal pivots of the x-lines
-        const float *h;    // [X][Y] (D_x / 2) ib_x of the x-lines
+        const float *Dt;   // [Y][X] the same, column-major
+        const float *ib;   // [Y][X] reciprocal pivots of the x-lines, column-major
+        const float *h;    // [Y][X] (D_x / 2) ib_x of the x-lines, column-major
         float *out;        // [X][Y]
         const float4 *lineK; // periodic x-lines: (gamma, alpha, vEnd, 0) per column
         const float *edges;  // Dirichlet ghosts [xlo(Y) | xhi(Y) | ylo(X) | yhi(X)]
@@ -526,11 +527,17 @@ namespace
     }
 
     // Lines along x (one per column, stride Y) solved without a transpose.  A
-    // workgroup of C wavefronts holds C adjacent columns over all X rows in
-    // LDS: rows are loaded and stored row-major (C consecutive floats each),
-    // wavefront c solves column c with lane l holding rows [lE, lE + E)
-    // (LDS pitch E + 1 per lane, so the lanes' element reads hit distinct
-    // banks).  Rows past X are identity padding (d = D = ib = 0).
+    // workgroup holds C adjacent columns of the field over all X rows in LDS:
+    // rows are loaded and stored row-major (C consecutive floats each).  The
+    // line tables (ib, h and, for MODE 1, D) are fixed for a run and kept
+    // column-major, so each lane reads its own rows of them directly, as
+    // contiguous vectors: staging them too would pull every 128-byte line of
+    // them into several CUs for 16 bytes each.  W wavefronts
+    // share a column: lane l of wavefront h holds rows [kE, kE + E) with
+    // k = 64h + l (LDS pitch E + 1 per chunk, so the lanes' element reads hit
+    // distinct banks); each recurrence composes per-lane folds, a wavefront
+    // scan of affine maps and the W wavefronts' totals exchanged through LDS.
+    // Rows past X are identity padding (d = D = ib = 0).
     //   MODE 0: d = (I + L_y/2) in, the explicit half-step across columns
     //           (reads columns y0 - 1 and y0 + C too; wrapped, or with
     //           Dirichlet ghosts beyond the first and last column), plus
@@ -541,15 +548,23 @@ namespace
     // as in CellLineSolveBlock).
     // Column blocks are dealt so that each XCD (blocks b, b + 8, ...) takes
     // a contiguous range: blocks that share cache lines share an L2.
-    template <int C, int E, int MODE, bool VEC, bool PER>
-    __global__ __launch_bounds__(64 * C) void CellColumnSolve(ColArgs a)
+    template <int C, int W, int E, int MODE, bool VEC, bool PER>
+    __global__ __launch_bounds__(64 * C * W) void CellColumnSolve(ColArgs a)
     {
-        constexpr int kPitch = 64 * (E + 1); // LDS floats per column
-        constexpr int kRows = 64 * E;
+        constexpr int kThreads = 64 * C * W;
+        constexpr int kChunks = 64 * W;             // chunks of E rows per column
+        constexpr int kPitch = kChunks * (E + 1);   // LDS floats per column
+        constexpr int kRows = kChunks * E;
         __shared__ float sD[C * kPitch];
-        __shared__ float sR[C * kPitch];
-        __shared__ float sI[C * kPitch];
-        __shared__ float sH[C * kPitch];
+        // wavefront totals of the forward and backward scans, and (PER) the
+        // end values of each column's two solutions
+        __shared__ float sFA[C * W];
+        __shared__ float sFB[C * W];
+        __shared__ float sFB2[C * W];
+        __shared__ float sBA[C * W];
+        __shared__ float sBB[C * W];
+        __shared__ float sBB2[C * W];
+        __shared__ float sEnd[C * 4];
         const int X = a.X;
         const int Y = a.Y;
         int b = static_cast<int>(blockIdx.x);
@@ -559,20 +574,54 @@ namespace
         }
         const int y0 = b * C;
         const int t = threadIdx.x;
+        const int lane = t & 63;
+        const int wave = t >> 6;
+        const int c = wave / W;     // the column this thread solves
+        const int hw = wave - c * W; // its part
+        const int k = hw * 64 + lane;
+        const int x0 = k * E;
+        const int col = c * kPitch;
+        const int me = k * (E + 1);
+        const int tot = c * W;       // this column's slots of the totals
+        // This lane's rows of the column's tables (a column past Y reads the
+        // last one; its results are not stored), requested first: they do
+        // not wait for the staging.
+        const size_t colBase = static_cast<size_t>(min(y0 + c, Y - 1)) * X;
+        const bool vecT = (E % 4 == 0) && (X % 4 == 0);
+        float h[E];
+        float ib[E];
+        float rr[MODE == 1 ? E : 1];
+        if (vecT)
+        {
+            LoadLine<E, (E % 4 == 0)>(a.ib, colBase, x0, X, 0.0f, ib);
+            LoadLine<E, (E % 4 == 0)>(a.h, colBase, x0, X, 0.0f, h);
+            if constexpr (MODE == 1)
+            {
+                LoadLine<E, (E % 4 == 0)>(a.Dt, colBase, x0, X, 0.0f, rr);
+            }
+        }
+        else
+        {
+            LoadLine<E, false>(a.ib, colBase, x0, X, 0.0f, ib);
+            LoadLine<E, false>(a.h, colBase, x0, X, 0.0f, h);
+            if constexpr (MODE == 1)
+            {
+                LoadLine<E, false>(a.Dt, colBase, x0, X, 0.0f, rr);
+            }
+        }
+        const float hLeft = (x0 > 0 && x0 - 1 < X) ? a.h[colBase + x0 - 1] : 0.0f;
         // Every thread holds kMine rows: all their loads are issued before
         // the first LDS store, so one memory latency covers the tile.
-        constexpr int kMine = (kRows + 64 * C - 1) / (64 * C);
+        constexpr int kMine = (kRows + kThreads - 1) / kThreads;
         float u[kMine][C];
         float r[kMine][C];
-        float q[kMine][C];
-        float hv[kMine][C];
         float uL[kMine];
         float rL[kMine];
         float uR[kMine];
 #pragma unroll
         for (int i = 0; i < kMine; i++)
         {
-            const int x = t + i * 64 * C;
+            const int x = t + i * kThreads;
             const size_t row = static_cast<size_t>(x) * Y + y0;
             uL[i] = 0.0f;
             rL[i] = 0.0f;
@@ -580,11 +629,9 @@ namespace
             if (x < X)
             {
                 LoadRow<C, VEC>(a.in, row, y0, Y, u[i]);
-                LoadRow<C, VEC>(a.D, row, y0, Y, r[i]);
-                LoadRow<C, VEC>(a.ib, row, y0, Y, q[i]);
-                LoadRow<C, VEC>(a.h, row, y0, Y, hv[i]);
                 if constexpr (MODE == 0)
                 {
+                    LoadRow<C, VEC>(a.D, row, y0, Y, r[i]);
                     // wrapped: column Y - 1 before column 0, column 0 after Y - 1
                     const size_t rowStart = static_cast<size_t>(x) * Y;
                     uL[i] = (y0 > 0) ? a.in[row - 1] : (a.wrapY ? a.in[rowStart + Y - 1] : 0.0f);
@@ -599,15 +646,13 @@ namespace
                 {
                     u[i][c] = 0.0f;
                     r[i][c] = 0.0f;
-                    q[i][c] = 0.0f;
-                    hv[i][c] = 0.0f;
                 }
             }
         }
 #pragma unroll
         for (int i = 0; i < kMine; i++)
         {
-            const int x = t + i * 64 * C;
+            const int x = t + i * kThreads;
             const int at = x + x / E;
             if constexpr (MODE == 0)
             {
@@ -659,43 +704,26 @@ namespace
                 for (int c = 0; c < C; c++)
                 {
                     sD[c * kPitch + at] = u[i][c];
-                    sR[c * kPitch + at] = r[i][c];
-                    sI[c * kPitch + at] = q[i][c];
-                    sH[c * kPitch + at] = hv[i][c];
                 }
             }
         }
         __syncthreads();
 
-        const int lane = t & 63;
-        const int c = t >> 6;
-        const int x0 = lane * E;
-        const int col = c * kPitch;
-        const int me = lane * (E + 1);
         float d[E];
-        float h[E];
-        float ib[E];
 #pragma unroll
         for (int e = 0; e < E; e++)
         {
             d[e] = sD[col + me + e];
-            ib[e] = sI[col + me + e];
         }
         if constexpr (MODE == 1)
         {
             // (I + L_x/2) along the column: neighbours x +- 1 from LDS
-            // (rows past X hold D = 0, so no face reaches them)
-            float r[E];
-#pragma unroll
-            for (int e = 0; e < E; e++)
-            {
-                r[e] = sR[col + me + e];
-            }
+            // (rows past X have D = 0, so no face reaches them)
             const int xl = (x0 > 0) ? x0 - 1 : X - 1; // wrapped: row X - 1 before row 0
             const int xr = x0 + E;
-            const bool left = x0 > 0 || a.wrapX;
+            const bool left = (x0 > 0 && x0 - 1 < X) || (x0 == 0 && a.wrapX);
             const float dL = left ? sD[col + xl + xl / E] : 0.0f;
-            const float rL = left ? sR[col + xl + xl / E] : 0.0f;
+            const float rLv = left ? a.Dt[colBase + xl] : 0.0f;
             const float dR = (xr < kRows) ? sD[col + xr + xr / E] : 0.0f;
             const float dW = sD[col]; // row 0, after row X - 1 when wrapped
             float out[E];
@@ -706,17 +734,17 @@ namespace
                 const float cv = d[e];
                 const float dp = (x + 1 < X) ? ((e + 1 < E) ? d[e + 1] : dR) : dW;
                 const float dm = (e > 0) ? d[e - 1] : dL;
-                const float rm = (e > 0) ? r[e - 1] : rL;
-                float s = 0.0f;
-                s += (x + 1 < X || a.wrapX) ? r[e] * (dp - cv) : 0.0f;
-                s += (x > 0 || a.wrapX) ? rm * (dm - cv) : 0.0f;
+                const float rm = (e > 0) ? rr[e - 1] : rLv;
+                float sum = 0.0f;
+                sum += (x + 1 < X || a.wrapX) ? rr[e] * (dp - cv) : 0.0f;
+                sum += (x > 0 || a.wrapX) ? rm * (dm - cv) : 0.0f;
                 if (a.dirX)
                 {
                     // ghost faces 2 D (0 - c) at both ends
-                    s -= (x == 0) ? 2.0f * r[e] * cv : 0.0f;
-                    s -= (x == X - 1) ? 2.0f * r[e] * cv : 0.0f;
+                    sum -= (x == 0) ? 2.0f * rr[e] * cv : 0.0f;
+                    sum -= (x == X - 1) ? 2.0f * rr[e] * cv : 0.0f;
                 }
-                out[e] = (x < X) ? fmaf(0.5f, s, cv) : 0.0f;
+                out[e] = (x < X) ? fmaf(0.5f, sum, cv) : 0.0f;
             }
 #pragma unroll
             for (int e = 0; e < E; e++)
@@ -724,12 +752,6 @@ namespace
                 d[e] = out[e];
             }
         }
-#pragma unroll
-        for (int e = 0; e < E; e++)
-        {
-            h[e] = sH[col + me + e];
-        }
-        const float hLeft = (x0 > 0) ? sH[col + me - 2] : 0.0f;
 
         // PER: the second right-hand side (gamma at row 0, alpha at X - 1)
         float z[PER ? E : 1];
@@ -765,12 +787,34 @@ namespace
         ScanStepUp<0x118, 0xF, PER>(A, B, B2);
         ScanStepUp<0x142, 0xA, PER>(A, B, B2);
         ScanStepUp<0x143, 0xC, PER>(A, B, B2);
-        // carry into this lane: the preceding lanes' inclusive value
-        float acc = DppF<0x138>(0.0f, B);
+        float cw = 0.0f;
+        float cw2 = 0.0f;
+        if constexpr (W > 1)
+        {
+            if (lane == 63)
+            {
+                sFA[tot + hw] = A;
+                sFB[tot + hw] = B;
+                sFB2[tot + hw] = B2;
+            }
+            __syncthreads();
+            for (int g = 0; g < hw; g++)
+            {
+                cw = fmaf(sFA[tot + g], cw, sFB[tot + g]);
+                if constexpr (PER)
+                {
+                    cw2 = fmaf(sFA[tot + g], cw2, sFB2[tot + g]);
+                }
+            }
+        }
+        // carry into this lane: the preceding lanes' composition applied to
+        // the preceding wavefronts' carry
+        const float Aprev = DppF<0x138>(1.0f, A);
+        float acc = fmaf(Aprev, cw, DppF<0x138>(0.0f, B));
         float acc2 = 0.0f;
         if constexpr (PER)
         {
-            acc2 = DppF<0x138>(0.0f, B2);
+            acc2 = fmaf(Aprev, cw2, DppF<0x138>(0.0f, B2));
         }
 #pragma unroll
         for (int e = 0; e < E; e++)
@@ -801,9 +845,9 @@ namespace
             A *= bw;
         }
 #pragma unroll
-        for (int k = 0; k < 6; k++)
+        for (int kk = 0; kk < 6; kk++)
         {
-            const int o = 1 << k;
+            const int o = 1 << kk;
             const float Bd = __shfl_down(B, o);
             const float Ad = __shfl_down(A, o);
             float Bd2 = 0.0f;
@@ -821,12 +865,33 @@ namespace
                 A *= Ad;
             }
         }
+        cw = 0.0f;
+        cw2 = 0.0f;
+        if constexpr (W > 1)
+        {
+            if (lane == 0)
+            {
+                sBA[tot + hw] = A;
+                sBB[tot + hw] = B;
+                sBB2[tot + hw] = B2;
+            }
+            __syncthreads();
+            for (int g = W - 1; g > hw; g--)
+            {
+                cw = fmaf(sBA[tot + g], cw, sBB[tot + g]);
+                if constexpr (PER)
+                {
+                    cw2 = fmaf(sBA[tot + g], cw2, sBB2[tot + g]);
+                }
+            }
+        }
+        const float An = __shfl_down(A, 1);
         const float Bn = __shfl_down(B, 1);
-        acc = (lane == 63) ? 0.0f : Bn;
+        acc = (lane == 63) ? cw : fmaf(An, cw, Bn);
         if constexpr (PER)
         {
             const float Bn2 = __shfl_down(B2, 1);
-            acc2 = (lane == 63) ? 0.0f : Bn2;
+            acc2 = (lane == 63) ? cw2 : fmaf(An, cw2, Bn2);
         }
 #pragma unroll
         for (int e = E - 1; e >= 0; e--)
@@ -843,25 +908,33 @@ namespace
         if constexpr (PER)
         {
             // x = y - f z with f = (y_0 + vEnd y_{X-1}) / (1 + z_0 + vEnd z_{X-1}):
-            // row 0 is lane 0's first element, row X - 1 lane (X-1)/E's
-            const int lLast = (X - 1) / E;
-            const int eLast = (X - 1) - lLast * E;
-            float yl = 0.0f;
-            float zl = 0.0f;
-#pragma unroll
-            for (int e = 0; e < E; e++)
+            // row 0 is chunk 0's first element, row X - 1 chunk (X-1)/E's
+            const int kLast = (X - 1) / E;
+            const int eLast = (X - 1) - kLast * E;
+            if (k == 0)
             {
-                if (e == eLast)
-                {
-                    yl = d[e];
-                    zl = z[e];
-                }
+                sEnd[4 * c] = d[0];
+                sEnd[4 * c + 1] = z[0];
             }
-            const float y0v = __shfl(d[0], 0);
-            const float z0v = __shfl(z[0], 0);
-            yl = __shfl(yl, lLast);
-            zl = __shfl(zl, lLast);
-            const float f = fmaf(K.z, yl, y0v) / (1.0f + fmaf(K.z, zl, z0v));
+            if (k == kLast)
+            {
+                float yl = 0.0f;
+                float zl = 0.0f;
+#pragma unroll
+                for (int e = 0; e < E; e++)
+                {
+                    if (e == eLast)
+                    {
+                        yl = d[e];
+                        zl = z[e];
+                    }
+                }
+                sEnd[4 * c + 2] = yl;
+                sEnd[4 * c + 3] = zl;
+            }
+            __syncthreads();
+            const float f = fmaf(K.z, sEnd[4 * c + 2], sEnd[4 * c]) /
+                            (1.0f + fmaf(K.z, sEnd[4 * c + 3], sEnd[4 * c + 1]));
 #pragma unroll
             for (int e = 0; e < E; e++)
             {
@@ -874,7 +947,7 @@ namespace
             sD[col + me + e] = d[e];
         }
         __syncthreads();
-        for (int x = t; x < X; x += 64 * C)
+        for (int x = t; x < X; x += kThreads)
         {
             const int at = x + x / E;
             const size_t row = static_cast<size_t>(x) * Y + y0;
@@ -1150,8 +1223,9 @@ struct halhip_adjoint2d
     float decay = 0.0f;
     bool ready = false;
     DeviceArray<float> D;                  // face rates, [X][Y]
-    DeviceArray<float> ibX, ibY;           // reciprocal pivots of the x- and y-lines, [X][Y]
-    DeviceArray<float> hX, hY;             // (D/2) ib of the x- and y-lines, [X][Y]
+    DeviceArray<float> Dt;                 // face rates, column-major [Y][X]
+    DeviceArray<float> ibXt, hXt;          // x-line pivots and (D/2) ib, column-major [Y][X]
+    DeviceArray<float> ibY, hY;            // y-line pivots and (D/2) ib, [X][Y]
     // Boundaries: wrap flags, Dirichlet ghosts on the non-wrapped axes
     // (edges [xlo(Y) | xhi(Y) | ylo(X) | yhi(X)]), and the periodic lines'
     // Sherman-Morrison constants (lineKX per column, lineKY per row).
@@ -1261,36 +1335,39 @@ namespace
         p->prof.End(p->clock.stream);
     }
 
-    template <int C, int E, int MODE, bool PER>
+    template <int C, int W, int E, int MODE, bool PER>
     void LaunchColumnPer(halhip_adjoint2d *p, const ColArgs &a)
     {
         const dim3 grid(static_cast<unsigned>(a.blocks));
+        const dim3 block(static_cast<unsigned>(64 * C * W));
         if (a.Y % C == 0)
         {
-            hipLaunchKernelGGL((CellColumnSolve<C, E, MODE, true, PER>), grid, dim3(64 * C), 0, p->clock.stream, a);
+            hipLaunchKernelGGL((CellColumnSolve<C, W, E, MODE, true, PER>), grid, block, 0, p->clock.stream, a);
         }
         else
         {
-            hipLaunchKernelGGL((CellColumnSolve<C, E, MODE, false, PER>), grid, dim3(64 * C), 0, p->clock.stream,
-                               a);
+            hipLaunchKernelGGL((CellColumnSolve<C, W, E, MODE, false, PER>), grid, block, 0, p->clock.stream, a);
         }
     }
 
-    template <int C, int E, int MODE>
+    template <int C, int W, int E, int MODE>
     void LaunchColumnE(halhip_adjoint2d *p, const ColArgs &a)
     {
         if (PeriodicColumns(p))
         {
-            LaunchColumnPer<C, E, MODE, true>(p, a);
+            LaunchColumnPer<C, W, E, MODE, true>(p, a);
         }
         else
         {
-            LaunchColumnPer<C, E, MODE, false>(p, a);
+            LaunchColumnPer<C, W, E, MODE, false>(p, a);
         }
     }
 
-    // One x-line solve per column: E = the smallest power of two with
-    // 64 E >= X, four columns per workgroup (two at E = 64, for LDS).
+    // One x-line solve per column.  Up to 256 rows one wavefront holds a
+    // column (E = the smallest power of two with 64 E >= X); longer columns
+    // are shared by four wavefronts (E = 2 .. 16 rows per lane), so a
+    // workgroup of four columns runs 16 wavefronts; two columns per
+    // workgroup at E = 16, for LDS.
     template <int MODE>
     void LaunchColumn(halhip_adjoint2d *p, const float *in, float *out, const char *name)
     {
@@ -1299,8 +1376,9 @@ namespace
         a.Y = p->Y;
         a.in = in;
         a.D = p->D.get();
-        a.ib = p->ibX.get();
-        a.h = p->hX.get();
+        a.Dt = p->Dt.get();
+        a.ib = p->ibXt.get();
+        a.h = p->hXt.get();
         a.out = out;
         a.lineK = reinterpret_cast<const float4 *>(p->lineKX.get());
         a.edges = p->edges.get();
@@ -1308,37 +1386,37 @@ namespace
         a.wrapY = p->wrapY ? 1 : 0;
         a.dirX = p->dirX ? 1 : 0;
         a.dirY = p->dirY ? 1 : 0;
-        int e = 1;
-        while (64 * e < a.X)
-        {
-            e *= 2;
-        }
-        const int C = (e == 64) ? 2 : 4;
+        const int X = a.X;
+        const int C = (X > 2048) ? 2 : 4;
         a.blocks = (a.Y + C - 1) / C;
         p->prof.Begin(p->clock.stream, name);
-        switch (e)
+        if (X <= 64)
         {
-            case 1:
-                LaunchColumnE<4, 1, MODE>(p, a);
-                break;
-            case 2:
-                LaunchColumnE<4, 2, MODE>(p, a);
-                break;
-            case 4:
-                LaunchColumnE<4, 4, MODE>(p, a);
-                break;
-            case 8:
-                LaunchColumnE<4, 8, MODE>(p, a);
-                break;
-            case 16:
-                LaunchColumnE<4, 16, MODE>(p, a);
-                break;
-            case 32:
-                LaunchColumnE<4, 32, MODE>(p, a);
-                break;
-            default:
-                LaunchColumnE<2, 64, MODE>(p, a);
-                break;
+            LaunchColumnE<4, 1, 1, MODE>(p, a);
+        }
+        else if (X <= 128)
+        {
+            LaunchColumnE<4, 1, 2, MODE>(p, a);
+        }
+        else if (X <= 256)
+        {
+            LaunchColumnE<4, 1, 4, MODE>(p, a);
+        }
+        else if (X <= 512)
+        {
+            LaunchColumnE<4, 4, 2, MODE>(p, a);
+        }
+        else if (X <= 1024)
+        {
+            LaunchColumnE<4, 4, 4, MODE>(p, a);
+        }
+        else if (X <= 2048)
+        {
+            LaunchColumnE<4, 4, 8, MODE>(p, a);
+        }
+        else
+        {
+            LaunchColumnE<2, 4, 16, MODE>(p, a);
         }
         Check(hipGetLastError(), name);
         p->prof.End(p->clock.stream);
@@ -1524,7 +1602,8 @@ extern "C"
                 p->X = xDim;
                 p->Y = yDim;
                 const size_t n = Cells(p);
-                for (DeviceArray<float> *b : {&p->D, &p->ibX, &p->ibY, &p->hX, &p->hY, &p->u, &p->s2, &p->work, &p->lam, &p->lamNext,
+                for (DeviceArray<float> *b : {&p->D, &p->Dt, &p->ibXt, &p->ibY, &p->hXt, &p->hY, &p->u, &p->s2, &p->work,
+                                              &p->lam, &p->lamNext,
                                               &p->m1, &p->m2})
                 {
                     b->Allocate(ctx, n);
@@ -1601,10 +1680,25 @@ extern "C"
                     Check(hipMemcpy(dst.get(), src.data(), src.size() * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
                 };
                 Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
+                // the x-line tables and D again, column-major, for the
+                // column solve
+                auto columnMajor = [&](const std::vector<float> &src)
+                {
+                    std::vector<float> t(src.size());
+                    for (int32_t x = 0; x < X; x++)
+                    {
+                        for (int32_t y = 0; y < Y; y++)
+                        {
+                            t[static_cast<size_t>(y) * X + x] = src[static_cast<size_t>(x) * Y + y];
+                        }
+                    }
+                    return t;
+                };
                 up(p->D, Df);
-                up(p->ibX, ibx);
+                up(p->Dt, columnMajor(Df));
+                up(p->ibXt, columnMajor(ibx));
                 up(p->ibY, iby);
-                up(p->hX, hx);
+                up(p->hXt, columnMajor(hx));
                 up(p->hY, hy);
                 up(p->lineKX, kx);
                 up(p->lineKY, ky);
