@@ -532,12 +532,13 @@ namespace
     // line tables (ib, h and, for MODE 1, D) are fixed for a run and kept
     // column-major, so each lane reads its own rows of them directly, as
     // contiguous vectors: staging them too would pull every 128-byte line of
-    // them into several CUs for 16 bytes each.  W wavefronts
-    // share a column: lane l of wavefront h holds rows [kE, kE + E) with
-    // k = 64h + l (LDS pitch E + 1 per chunk, so the lanes' element reads hit
-    // distinct banks); each recurrence composes per-lane folds, a wavefront
-    // scan of affine maps and the W wavefronts' totals exchanged through LDS.
-    // Rows past X are identity padding (d = D = ib = 0).
+    // them into several CUs for 16 bytes each.  Wavefront c solves column c
+    // with lane l holding rows [lE, lE + E) (LDS pitch E + 1 per lane, so the
+    // lanes' element reads hit distinct banks): per-lane folds and a
+    // wavefront scan of affine maps.  (Sharing a column among four
+    // wavefronts measured no faster and moved the heavily cancelling decay
+    // gradient of the 1024^2 workload by 2e-4.)  Rows past X are identity
+    // padding (d = D = ib = 0).
     //   MODE 0: d = (I + L_y/2) in, the explicit half-step across columns
     //           (reads columns y0 - 1 and y0 + C too; wrapped, or with
     //           Dirichlet ghosts beyond the first and last column), plus
@@ -548,23 +549,13 @@ namespace
     // as in CellLineSolveBlock).
     // Column blocks are dealt so that each XCD (blocks b, b + 8, ...) takes
     // a contiguous range: blocks that share cache lines share an L2.
-    template <int C, int W, int E, int MODE, bool VEC, bool PER>
-    __global__ __launch_bounds__(64 * C * W) void CellColumnSolve(ColArgs a)
+    template <int C, int E, int MODE, bool VEC, bool PER>
+    __global__ __launch_bounds__(64 * C) void CellColumnSolve(ColArgs a)
     {
-        constexpr int kThreads = 64 * C * W;
-        constexpr int kChunks = 64 * W;             // chunks of E rows per column
-        constexpr int kPitch = kChunks * (E + 1);   // LDS floats per column
-        constexpr int kRows = kChunks * E;
+        constexpr int kThreads = 64 * C;
+        constexpr int kPitch = 64 * (E + 1); // LDS floats per column
+        constexpr int kRows = 64 * E;
         __shared__ float sD[C * kPitch];
-        // wavefront totals of the forward and backward scans, and (PER) the
-        // end values of each column's two solutions
-        __shared__ float sFA[C * W];
-        __shared__ float sFB[C * W];
-        __shared__ float sFB2[C * W];
-        __shared__ float sBA[C * W];
-        __shared__ float sBB[C * W];
-        __shared__ float sBB2[C * W];
-        __shared__ float sEnd[C * 4];
         const int X = a.X;
         const int Y = a.Y;
         int b = static_cast<int>(blockIdx.x);
@@ -575,14 +566,10 @@ namespace
         const int y0 = b * C;
         const int t = threadIdx.x;
         const int lane = t & 63;
-        const int wave = t >> 6;
-        const int c = wave / W;     // the column this thread solves
-        const int hw = wave - c * W; // its part
-        const int k = hw * 64 + lane;
-        const int x0 = k * E;
+        const int c = t >> 6; // the column this wavefront solves
+        const int x0 = lane * E;
         const int col = c * kPitch;
-        const int me = k * (E + 1);
-        const int tot = c * W;       // this column's slots of the totals
+        const int me = lane * (E + 1);
         // This lane's rows of the column's tables (a column past Y reads the
         // last one; its results are not stored), requested first: they do
         // not wait for the staging.
@@ -787,34 +774,12 @@ namespace
         ScanStepUp<0x118, 0xF, PER>(A, B, B2);
         ScanStepUp<0x142, 0xA, PER>(A, B, B2);
         ScanStepUp<0x143, 0xC, PER>(A, B, B2);
-        float cw = 0.0f;
-        float cw2 = 0.0f;
-        if constexpr (W > 1)
-        {
-            if (lane == 63)
-            {
-                sFA[tot + hw] = A;
-                sFB[tot + hw] = B;
-                sFB2[tot + hw] = B2;
-            }
-            __syncthreads();
-            for (int g = 0; g < hw; g++)
-            {
-                cw = fmaf(sFA[tot + g], cw, sFB[tot + g]);
-                if constexpr (PER)
-                {
-                    cw2 = fmaf(sFA[tot + g], cw2, sFB2[tot + g]);
-                }
-            }
-        }
-        // carry into this lane: the preceding lanes' composition applied to
-        // the preceding wavefronts' carry
-        const float Aprev = DppF<0x138>(1.0f, A);
-        float acc = fmaf(Aprev, cw, DppF<0x138>(0.0f, B));
+        // carry into this lane: the preceding lanes' inclusive value
+        float acc = DppF<0x138>(0.0f, B);
         float acc2 = 0.0f;
         if constexpr (PER)
         {
-            acc2 = fmaf(Aprev, cw2, DppF<0x138>(0.0f, B2));
+            acc2 = DppF<0x138>(0.0f, B2);
         }
 #pragma unroll
         for (int e = 0; e < E; e++)
@@ -865,33 +830,12 @@ namespace
                 A *= Ad;
             }
         }
-        cw = 0.0f;
-        cw2 = 0.0f;
-        if constexpr (W > 1)
-        {
-            if (lane == 0)
-            {
-                sBA[tot + hw] = A;
-                sBB[tot + hw] = B;
-                sBB2[tot + hw] = B2;
-            }
-            __syncthreads();
-            for (int g = W - 1; g > hw; g--)
-            {
-                cw = fmaf(sBA[tot + g], cw, sBB[tot + g]);
-                if constexpr (PER)
-                {
-                    cw2 = fmaf(sBA[tot + g], cw2, sBB2[tot + g]);
-                }
-            }
-        }
-        const float An = __shfl_down(A, 1);
         const float Bn = __shfl_down(B, 1);
-        acc = (lane == 63) ? cw : fmaf(An, cw, Bn);
+        acc = (lane == 63) ? 0.0f : Bn;
         if constexpr (PER)
         {
             const float Bn2 = __shfl_down(B2, 1);
-            acc2 = (lane == 63) ? cw2 : fmaf(An, cw2, Bn2);
+            acc2 = (lane == 63) ? 0.0f : Bn2;
         }
 #pragma unroll
         for (int e = E - 1; e >= 0; e--)
@@ -908,33 +852,25 @@ namespace
         if constexpr (PER)
         {
             // x = y - f z with f = (y_0 + vEnd y_{X-1}) / (1 + z_0 + vEnd z_{X-1}):
-            // row 0 is chunk 0's first element, row X - 1 chunk (X-1)/E's
-            const int kLast = (X - 1) / E;
-            const int eLast = (X - 1) - kLast * E;
-            if (k == 0)
-            {
-                sEnd[4 * c] = d[0];
-                sEnd[4 * c + 1] = z[0];
-            }
-            if (k == kLast)
-            {
-                float yl = 0.0f;
-                float zl = 0.0f;
+            // row 0 is lane 0's first element, row X - 1 lane (X-1)/E's
+            const int lLast = (X - 1) / E;
+            const int eLast = (X - 1) - lLast * E;
+            float yl = 0.0f;
+            float zl = 0.0f;
 #pragma unroll
-                for (int e = 0; e < E; e++)
+            for (int e = 0; e < E; e++)
+            {
+                if (e == eLast)
                 {
-                    if (e == eLast)
-                    {
-                        yl = d[e];
-                        zl = z[e];
-                    }
+                    yl = d[e];
+                    zl = z[e];
                 }
-                sEnd[4 * c + 2] = yl;
-                sEnd[4 * c + 3] = zl;
             }
-            __syncthreads();
-            const float f = fmaf(K.z, sEnd[4 * c + 2], sEnd[4 * c]) /
-                            (1.0f + fmaf(K.z, sEnd[4 * c + 3], sEnd[4 * c + 1]));
+            const float y0v = __shfl(d[0], 0);
+            const float z0v = __shfl(z[0], 0);
+            yl = __shfl(yl, lLast);
+            zl = __shfl(zl, lLast);
+            const float f = fmaf(K.z, yl, y0v) / (1.0f + fmaf(K.z, zl, z0v));
 #pragma unroll
             for (int e = 0; e < E; e++)
             {
@@ -1335,39 +1271,36 @@ namespace
         p->prof.End(p->clock.stream);
     }
 
-    template <int C, int W, int E, int MODE, bool PER>
+    template <int C, int E, int MODE, bool PER>
     void LaunchColumnPer(halhip_adjoint2d *p, const ColArgs &a)
     {
         const dim3 grid(static_cast<unsigned>(a.blocks));
-        const dim3 block(static_cast<unsigned>(64 * C * W));
+        const dim3 block(static_cast<unsigned>(64 * C));
         if (a.Y % C == 0)
         {
-            hipLaunchKernelGGL((CellColumnSolve<C, W, E, MODE, true, PER>), grid, block, 0, p->clock.stream, a);
+            hipLaunchKernelGGL((CellColumnSolve<C, E, MODE, true, PER>), grid, block, 0, p->clock.stream, a);
         }
         else
         {
-            hipLaunchKernelGGL((CellColumnSolve<C, W, E, MODE, false, PER>), grid, block, 0, p->clock.stream, a);
+            hipLaunchKernelGGL((CellColumnSolve<C, E, MODE, false, PER>), grid, block, 0, p->clock.stream, a);
         }
     }
 
-    template <int C, int W, int E, int MODE>
+    template <int C, int E, int MODE>
     void LaunchColumnE(halhip_adjoint2d *p, const ColArgs &a)
     {
         if (PeriodicColumns(p))
         {
-            LaunchColumnPer<C, W, E, MODE, true>(p, a);
+            LaunchColumnPer<C, E, MODE, true>(p, a);
         }
         else
         {
-            LaunchColumnPer<C, W, E, MODE, false>(p, a);
+            LaunchColumnPer<C, E, MODE, false>(p, a);
         }
     }
 
-    // One x-line solve per column.  Up to 256 rows one wavefront holds a
-    // column (E = the smallest power of two with 64 E >= X); longer columns
-    // are shared by four wavefronts (E = 2 .. 16 rows per lane), so a
-    // workgroup of four columns runs 16 wavefronts; two columns per
-    // workgroup at E = 16, for LDS.
+    // One x-line solve per column: E = the smallest power of two with
+    // 64 E >= X, four columns per workgroup (two at E = 64).
     template <int MODE>
     void LaunchColumn(halhip_adjoint2d *p, const float *in, float *out, const char *name)
     {
@@ -1392,31 +1325,31 @@ namespace
         p->prof.Begin(p->clock.stream, name);
         if (X <= 64)
         {
-            LaunchColumnE<4, 1, 1, MODE>(p, a);
+            LaunchColumnE<4, 1, MODE>(p, a);
         }
         else if (X <= 128)
         {
-            LaunchColumnE<4, 1, 2, MODE>(p, a);
+            LaunchColumnE<4, 2, MODE>(p, a);
         }
         else if (X <= 256)
         {
-            LaunchColumnE<4, 1, 4, MODE>(p, a);
+            LaunchColumnE<4, 4, MODE>(p, a);
         }
         else if (X <= 512)
         {
-            LaunchColumnE<4, 4, 2, MODE>(p, a);
+            LaunchColumnE<4, 8, MODE>(p, a);
         }
         else if (X <= 1024)
         {
-            LaunchColumnE<4, 4, 4, MODE>(p, a);
+            LaunchColumnE<4, 16, MODE>(p, a);
         }
         else if (X <= 2048)
         {
-            LaunchColumnE<4, 4, 8, MODE>(p, a);
+            LaunchColumnE<4, 32, MODE>(p, a);
         }
         else
         {
-            LaunchColumnE<2, 4, 16, MODE>(p, a);
+            LaunchColumnE<2, 64, MODE>(p, a);
         }
         Check(hipGetLastError(), name);
         p->prof.End(p->clock.stream);
