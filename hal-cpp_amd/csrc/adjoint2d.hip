@@ -1581,7 +1581,7 @@ extern "C"
                 RequireAdj(p, "halhip_adjoint2d_set_problem");
                 Require(diffusivity != nullptr, "halhip_adjoint2d_set_problem: null diffusivity");
                 Require(ticks >= 1 && checkpointEvery >= 1, "halhip_adjoint2d_set_problem: ticks and checkpoint >= 1");
-                Require(std::isfinite(decay), "halhip_adjoint2d_set_problem: bad decay");
+                Require(std::isfinite(static_cast<float>(decay)), "halhip_adjoint2d_set_problem: bad decay");
                 const int32_t X = p->X;
                 const int32_t Y = p->Y;
                 const size_t n = Cells(p);
@@ -1590,8 +1590,9 @@ extern "C"
                 std::vector<double> Dxy(diffusivity, diffusivity + n);
                 for (double v : Dxy)
                 {
-                    Require(std::isfinite(v) && v >= 0.0, "halhip_adjoint2d_set_problem: diffusivity must be finite "
-                                                           "and non-negative");
+                    // finite in FP32, which the device computes in
+                    Require(std::isfinite(static_cast<float>(v)) && v >= 0.0,
+                            "halhip_adjoint2d_set_problem: diffusivity must be finite and non-negative");
                 }
                 const bool wx = wrapX != 0;
                 const bool wy = wrapY != 0;

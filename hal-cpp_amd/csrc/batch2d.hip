@@ -929,12 +929,12 @@ extern "C"
         return Guard(
             [&]
             {
-                RequireBatch(b, "halhip_batch_set_strip_rows");
-                Require(rows != nullptr, "halhip_batch_set_strip_rows: null rows");
+                RequireBatch(b, "halhip_batch_set_class_rows");
+                Require(rows != nullptr, "halhip_batch_set_class_rows: null rows");
                 for (int32_t c = 0; c < kClasses; c++)
                 {
                     Require(rows[c] == 4 || rows[c] == 8 || rows[c] == 16,
-                            "halhip_batch_set_strip_rows: rows must be 4, 8 or 16");
+                            "halhip_batch_set_class_rows: rows must be 4, 8 or 16");
                 }
                 Check(hipStreamSynchronize(b->clock.stream), "hipStreamSynchronize");
                 for (int32_t c = 0; c < kClasses; c++)

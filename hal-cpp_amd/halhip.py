@@ -282,7 +282,8 @@ def make_bc(bc, X, Y):
         s.value = float(bc)
         return s, ()
     arrs = tuple(_f64(a) for a in bc)
-    assert arrs[0].size == Y and arrs[1].size == Y and arrs[2].size == X and arrs[3].size == X
+    _require(len(arrs) == 4 and arrs[0].size == Y and arrs[1].size == Y and arrs[2].size == X and arrs[3].size == X,
+             "bc edge arrays must hold yDim, yDim, xDim and xDim values")
     s.xlo, s.xhi, s.ylo, s.yhi = (a.ctypes.data_as(_dp) for a in arrs)
     return s, arrs
 
@@ -294,6 +295,11 @@ def _check_field_sizes(field, deltas, X, Y, caller):
         raise InvalidArgument(ERR_INVALID_ARGUMENT, caller + ": deltas must be a contiguous float64 array")
     if field.size != X * Y or deltas.size != X * Y:
         raise InvalidArgument(ERR_INVALID_ARGUMENT, caller + ": field and deltas must each hold xDim*yDim values")
+
+
+def _require(ok, msg):
+    if not ok:
+        raise InvalidArgument(ERR_INVALID_ARGUMENT, msg)
 
 
 def _bcp(s):
@@ -600,7 +606,7 @@ class Agents2D(_Plan):
 
     def upload(self, positions, radii):
         positions, radii = _f32(positions), _f32(radii)
-        assert positions.size == 2 * radii.size
+        _require(positions.size == 2 * radii.size, "Agents2D.upload: positions must hold two values per radius")
         _check(self._fn("upload")(self.h, positions.ctypes.data_as(_fp), radii.ctypes.data_as(_fp), radii.size))
         self.n = radii.size
 
@@ -622,6 +628,11 @@ class Agents2D(_Plan):
         _check(self._fn("download_velocities")(self.h, out.ctypes.data_as(_fp), self.n))
         return out
 
+    def set_scan_spin_limit(self, limit):
+        """Polls a bin-scan tile makes for a predecessor's total before the
+        scan gives up (reported as an error by the next download)."""
+        _check(self._fn("set_scan_spin_limit")(self.h, limit))
+
     def set_motion(self, friction, max_vel=0.0):
         """Carry friction * velocity across ticks and cap its norm at max_vel
         (0: no cap), as SphericalAgent2D::ApplyFriction / CapVelocity."""
@@ -629,7 +640,7 @@ class Agents2D(_Plan):
 
     def upload_motion(self, velocities):
         velocities = _f32(velocities)
-        assert velocities.size == 2 * self.n
+        _require(velocities.size == 2 * self.n, "Agents2D.upload_motion: velocities must hold two values per agent")
         _check(self._fn("upload_motion")(self.h, velocities.ctypes.data_as(_fp), self.n))
 
     def motion(self):
@@ -711,13 +722,13 @@ class Batch(_Plan):
         """Workgroup shape option (0..2) of each width class (<= 256, <= 512,
         <= 1024 columns)."""
         a = np.ascontiguousarray(shapes, np.int32)
-        assert a.size == 3
+        _require(a.size == 3, "Batch.set_shapes: one option per width class (3)")
         _check(self._fn("set_shapes")(self.h, a.ctypes.data_as(_ip)))
 
     def set_class_rows(self, rows):
         """Strip height (4, 8 or 16) of each width class."""
         a = np.ascontiguousarray(rows, np.int32)
-        assert a.size == 3
+        _require(a.size == 3, "Batch.set_class_rows: one strip height per width class (3)")
         _check(self._fn("set_class_rows")(self.h, a.ctypes.data_as(_ip)))
 
     def class_rows(self):
@@ -759,7 +770,7 @@ class Adjoint2D(_Plan):
         """bc: None (zero flux), a constant Dirichlet value or edge arrays,
         on the non-wrapped axes."""
         D = _f64(diffusivity)
-        assert D.size == self.X * self.Y
+        _require(D.size == self.X * self.Y, "Adjoint2D.set_problem: diffusivity must hold xDim*yDim values")
         if not wrapX and not wrapY and bc is None:
             _check(self._fn("set_problem")(self.h, D.ctypes.data_as(_dp), decay, ticks, checkpoint_every))
             return

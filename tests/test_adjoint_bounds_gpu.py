@@ -149,3 +149,22 @@ def test_wrapped_one_cell_axis_and_edge_array_lengths(ctx):
     b = p.forward(u0)
     np.testing.assert_array_equal(a, b)
     p.close()
+    # edge arrays of the wrong lengths, diffusivities and decays that FP32
+    # cannot hold, negative or NaN diffusivities and zero ticks are refused,
+    # and the problem set before stands
+    X, Y = 6, 5
+    D, u0 = problem(X, Y, 9)
+    q = halhip.Adjoint2D(ctx, X, Y)
+    q.set_problem(D, 1e-3, 5, 5, 1, 0, 0.25)
+    before = q.forward(u0)
+    bad_edges = (np.zeros(X), np.zeros(Y), np.zeros(X), np.zeros(X))   # x-edges hold yDim values
+    cell = np.arange(X * Y)
+    for args in ((D, 1e-3, 5, 5, 0, 0, bad_edges), (D, 1e-3, 5, 5, 0, 0, bad_edges[:3]),
+                 (np.where(cell == 7, 1e40, D), 1e-3, 5, 5, 1, 0, None),
+                 (np.where(cell == 3, -1.0, D), 1e-3, 5, 5, 1, 0, None),
+                 (np.where(cell == 0, np.nan, D), 1e-3, 5, 5, 0, 1, None),
+                 (D, 1e300, 5, 5, 1, 0, None), (D, 1e-3, 0, 5, 0, 1, None), (D[:-1], 1e-3, 5, 5, 1, 0, None)):
+        with pytest.raises(halhip.InvalidArgument):
+            q.set_problem(*args)
+    np.testing.assert_array_equal(q.forward(u0), before)
+    q.close()

@@ -152,7 +152,9 @@ def test_scan_wait_limit_fails_loudly_never_silently(ctx, oracle):
     rad = np.full(n, 0.5, np.float32)
     ro, rb = oracle.agent_bins(pos.astype(np.float64), X, Y)
     plan = halhip.Agents2D(ctx, X, Y, n)
-    plan._fn("set_scan_spin_limit")(plan.h, 0)
+    with pytest.raises(halhip.InvalidArgument):
+        plan.set_scan_spin_limit(-1)
+    plan.set_scan_spin_limit(0)
     outcomes = []
     for _ in range(6):
         plan.upload(pos, rad)
@@ -167,7 +169,7 @@ def test_scan_wait_limit_fails_loudly_never_silently(ctx, oracle):
         np.testing.assert_array_equal(bucket, rb)
         outcomes.append("exact")
     print("spin limit 0:", outcomes)
-    plan._fn("set_scan_spin_limit")(plan.h, 1 << 20)
+    plan.set_scan_spin_limit(1 << 20)
     plan.upload(pos, rad)
     plan.run_ticks(0, 1.0, 1.0, 0.0, 0, 0)
     offsets, bucket = plan.bins()

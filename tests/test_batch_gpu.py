@@ -107,6 +107,38 @@ def test_every_workgroup_shape_vs_oracle(ctx, oracle, carries, shape, rows):
     b.close()
 
 
+def test_shape_and_rows_misuse(ctx, oracle):
+    # out-of-range shape options and strip heights are refused and change
+    # nothing; 16-row strips with a 4-columns-per-thread shape are refused
+    # by the next run, before any tick
+    xd = np.array([9, 20, 11], np.int32)
+    yd = np.array([200, 400, 900], np.int32)
+    spec = dict(xdims=xd, ydims=yd, rate=np.array([0.3, 2.0, 4.0]), decay=np.array([0.0, 1e-3, 5e-3]),
+                src_well=np.array([0, 2], np.int32), src_cell=np.array([5, 900 * 5 + 7], np.int64),
+                src_val=np.array([1.0, 0.5], np.float32))
+    b = make(ctx, spec, 8)
+    for bad in ([0, 3, 0], [-1, 0, 0], [0, 0]):
+        with pytest.raises(halhip.InvalidArgument):
+            b.set_shapes(bad)
+    for bad in ([8, 12, 8], [8, 8, 0], [8, 8]):
+        with pytest.raises(halhip.InvalidArgument):
+            b.set_class_rows(bad)
+    assert [s[0] for s in b.shapes()] == [0, 0, 0] and b.class_rows() == [8, 8, 8]
+    f0 = np.random.default_rng(21).random(b.cells).astype(np.float32)
+    b.upload(f0)
+    b.set_strip_rows(16)   # the widest class's default shape: 256 threads x 4 columns
+    with pytest.raises(halhip.InvalidArgument, match="16-row"):
+        b.run_ticks(3)
+    np.testing.assert_array_equal(b.download(), f0)
+    b.set_shapes([0, 0, 1])   # 512 threads x 2 columns
+    b.run_ticks(3)
+    got = b.download()
+    ref = oracle_wells(oracle, spec, f0, b, 3)
+    for k in range(3):
+        assert rel_l2(b.well(got, k), ref[k]) < 1e-5, k
+    b.close()
+
+
 @pytest.mark.parametrize("carries", CARRIES)
 def test_per_instance_loop_matches_batched(ctx, carries):
     spec = workloads.wells_batch(n_wells=40, seed=3, lo=16, hi=512)
