@@ -1406,24 +1406,22 @@ namespace
     // (zero flux) or 2 bc - c (Dirichlet).  Only the first and last strips
     // (edge) add the Dirichlet implicit term of rows 0 and X-1 and zero the
     // rows past X.  Same arithmetic as SweepARhs.
-    template <int CPT, int S, int E, bool EDGE>
-    __device__ __forceinline__ void RhsAFromLds(const AdiArgs &a, const float *rows, int rowStride, int s0, int y0,
-                                                bool first, bool last, float (&v)[S][CPT])
+    //
+    // RhsACore takes the y neighbours of the thread's edge columns, per row:
+    // Ln (column y0-1) and Rn (column y0+CPT), with the row's far end on a
+    // periodic y axis; at a non-periodic edge they are replaced by ghosts.
+    template <int CPT, int S, bool EDGE>
+    __device__ __forceinline__ void RhsACore(const AdiArgs &a, int s0, int y0, bool first, bool last,
+                                             float (&v)[S][CPT], const float (&Ln)[S], const float (&Rn)[S])
     {
-        // y neighbours of the thread's columns; on a periodic y axis the
-        // row's far end (the whole row is in LDS)
-        const int yl = (y0 > 0) ? y0 - 1 : (a.wrapY ? a.Y - 1 : 0);
-        const int yr = (y0 + CPT < a.Y) ? y0 + CPT : (a.wrapY ? 0 : a.Y - 1);
-        const int al = RowAddr<E>(yl);
-        const int ar = RowAddr<E>(yr);
         const bool ghostL = first && !a.wrapY;
         const bool ghostR = last && !a.wrapY;
 #pragma unroll
         for (int r = 0; r < S; r++)
         {
             const int x = min(s0 + r, a.X - 1);
-            float L = rows[r * rowStride + al];
-            float R = rows[r * rowStride + ar];
+            float L = Ln[r];
+            float R = Rn[r];
             const float c0 = v[r][0];
             const float cl = v[r][CPT - 1];
             const float gl = a.dirichlet ? fmaf(2.0f, a.bcYlo[x], -c0) : c0;
@@ -1476,6 +1474,54 @@ namespace
                 }
             }
         }
+    }
+
+    // RhsACore with the neighbours read from the rows in LDS.
+    template <int CPT, int S, int E, bool EDGE>
+    __device__ __forceinline__ void RhsAFromLds(const AdiArgs &a, const float *rows, int rowStride, int s0, int y0,
+                                                bool first, bool last, float (&v)[S][CPT])
+    {
+        const int yl = (y0 > 0) ? y0 - 1 : (a.wrapY ? a.Y - 1 : 0);
+        const int yr = (y0 + CPT < a.Y) ? y0 + CPT : (a.wrapY ? 0 : a.Y - 1);
+        const int al = RowAddr<E>(yl);
+        const int ar = RowAddr<E>(yr);
+        float Ln[S];
+        float Rn[S];
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            Ln[r] = rows[r * rowStride + al];
+            Rn[r] = rows[r * rowStride + ar];
+        }
+        RhsACore<CPT, S, EDGE>(a, s0, y0, first, last, v, Ln, Rn);
+    }
+
+    // RhsACore with the neighbours taken from the adjacent lanes (DPP wave
+    // shifts) and, for a wavefront's first and last lane, from the
+    // neighbouring wavefronts' edge columns in sFirst / sLast ([wave][S],
+    // written before a barrier).  The workgroup holds whole rows, so the
+    // first wavefront's left neighbour is the last wavefront's last column
+    // (the row's far end), and the other way round.
+    template <int CPT, int S, bool EDGE>
+    __device__ __forceinline__ void RhsAFromLanes(const AdiArgs &a, const float *sFirst, const float *sLast, int s0,
+                                                  int y0, bool first, bool last, float (&v)[S][CPT])
+    {
+        const int lane = threadIdx.x & 63;
+        const int wave = threadIdx.x >> 6;
+        const int nW = blockDim.x >> 6;
+        const int wl = (wave > 0) ? wave - 1 : nW - 1;
+        const int wr = (wave + 1 < nW) ? wave + 1 : 0;
+        float Ln[S];
+        float Rn[S];
+#pragma unroll
+        for (int r = 0; r < S; r++)
+        {
+            const float fromLeft = sLast[wl * S + r];
+            const float fromRight = sFirst[wr * S + r];
+            Ln[r] = Dpp<0x138>(fromLeft, v[r][CPT - 1]);  // lane 0 keeps fromLeft
+            Rn[r] = Dpp<0x130>(fromRight, v[r][0]);       // lane 63 keeps fromRight
+        }
+        RhsACore<CPT, S, EDGE>(a, s0, y0, first, last, v, Ln, Rn);
     }
 
     // Sweep B right-hand side for a strip away from the x boundaries (rows
@@ -1561,9 +1607,12 @@ namespace
     // a.srcVal, in list order, or none when a.srcStrip is null), and the
     // reduction for the next tick is taken of that u'.
     // rows: blockDim.x * (E+1) floats (S rows x 64W chunks x (E+1)).
+    // sFirst / sLast: [waves][S] floats for the edge columns of each
+    // wavefront.
     template <int CPT, int S, int E, int KW, bool EDGE, bool DECAY = false, bool PX = false>
     __device__ __forceinline__ void StripLdsBody(const AdiArgs &a, float *__restrict__ u, int p, int K,
-                                                 const RowSolve &rs, float *rows, float *sTotF, float *sTotB)
+                                                 const RowSolve &rs, float *rows, float *sTotF, float *sTotB,
+                                                 float *sFirst, float *sLast)
     {
 
         const int s0 = p * S;
@@ -1643,19 +1692,27 @@ namespace
             LoadRow<CPT, true>(a.carry + base, y0, a.Y, cf);
             LoadRow<CPT, true>(a.carry + plane + base, y0, a.Y, dn);
         }
-        // RhsAFromLds reads only the neighbours' edge columns (and, on a
-        // periodic y axis, the row's far ends, which are edge columns too)
-#pragma unroll
-        for (int r = 0; r < S; r++)
+        // The wavefronts' edge columns for RhsAFromLanes.  They live apart
+        // from the rows buffer, so the right-hand sides can go to LDS below
+        // without a second barrier.
+        if (lane == 0)
         {
-            rows[r * rowStride + RowAddr<E>(y0)] = v[r][0];
-            if constexpr (CPT > 1)
+#pragma unroll
+            for (int r = 0; r < S; r++)
             {
-                rows[r * rowStride + RowAddr<E>(y0 + CPT - 1)] = v[r][CPT - 1];
+                sFirst[wave * S + r] = v[r][0];
+            }
+        }
+        if (lane == 63)
+        {
+#pragma unroll
+            for (int r = 0; r < S; r++)
+            {
+                sLast[wave * S + r] = v[r][CPT - 1];
             }
         }
         __syncthreads();
-        RhsAFromLds<CPT, S, E, EDGE>(a, rows, rowStride, s0, y0, first, last, v);
+        RhsAFromLanes<CPT, S, EDGE>(a, sFirst, sLast, s0, y0, first, last, v);
         if constexpr (PX)
         {
             // windowed carries on a periodic x axis: no scan kernel wrote
@@ -1682,7 +1739,6 @@ namespace
                 RhsBInterior<CPT, S>(a, s0, first, last, v, up, dn);
             }
         }
-        __syncthreads(); // every read of the staged u is done
 
         // Right-hand sides to LDS, row-major by chunk.
 #pragma unroll
@@ -1914,14 +1970,16 @@ namespace
         __shared__ float rows[T * (E + 1)];
         __shared__ float sTotF[kMaxWaves];
         __shared__ float sTotB[kMaxWaves];
+        __shared__ float sFirst[kMaxWaves * S];
+        __shared__ float sLast[kMaxWaves * S];
         const int p = StripOfBlock(blockIdx.x, a.P, KW > 0 && (a.P % 8 == 0));
         if (p == 0 || (p + 1) * S >= a.X)
         {
-            StripLdsBody<CPT, S, E, KW, true, DECAY, PX>(a, u, p, K, rs, rows, sTotF, sTotB);
+            StripLdsBody<CPT, S, E, KW, true, DECAY, PX>(a, u, p, K, rs, rows, sTotF, sTotB, sFirst, sLast);
         }
         else
         {
-            StripLdsBody<CPT, S, E, KW, false, DECAY, PX>(a, u, p, K, rs, rows, sTotF, sTotB);
+            StripLdsBody<CPT, S, E, KW, false, DECAY, PX>(a, u, p, K, rs, rows, sTotF, sTotB, sFirst, sLast);
         }
     }
 
