@@ -37,6 +37,7 @@
 #include <cmath>
 #include <mutex>
 #include <string>
+#include <type_traits>
 
 using namespace halhip;
 
@@ -283,7 +284,8 @@ namespace
         }
     }
 
-    // Moves slot k's (id, position, radius) to slot offsets[cell] + ticket.
+    // Moves slot k's (id, position, radius) to slot offsets[cell] + ticket
+    // (no radius with rad null: every agent has the same radius).
     __global__ __launch_bounds__(kThreads) void AgentScatter(int32_t N, const int32_t *__restrict__ cellOf,
                                                              const int32_t *__restrict__ ticketOf,
                                                              const int32_t *__restrict__ offsets,
@@ -303,7 +305,10 @@ namespace
         const int dst = offsets[cellOf[k]] + ticketOf[k];
         nId[dst] = id[k];
         nPos[dst] = pos[k];
-        nRad[dst] = rad[k];
+        if (rad != nullptr)
+        {
+            nRad[dst] = rad[k];
+        }
         if (carry != nullptr)
         {
             nCarry[dst] = carry[k];
@@ -311,8 +316,8 @@ namespace
     }
 
     // Sorts one cell's slice [s, e) of slots by agent index, moving position,
-    // radius and (when kept) carried velocity along (insertion sort: slices
-    // hold a handful of agents).
+    // radius (unless rad is null) and (when kept) carried velocity along
+    // (insertion sort: slices hold a handful of agents).
     __device__ __forceinline__ void OrderSlice(int s, int e, int32_t *__restrict__ id, float2 *__restrict__ pos,
                                                float *__restrict__ rad, float2 *__restrict__ carry)
     {
@@ -324,14 +329,17 @@ namespace
                 continue;
             }
             const float2 kp = pos[a];
-            const float kr = rad[a];
+            const float kr = (rad != nullptr) ? rad[a] : 0.0f;
             const float2 kc = (carry != nullptr) ? carry[a] : make_float2(0.0f, 0.0f);
             int b = a - 1;
             while (b >= s && id[b] > key)
             {
                 id[b + 1] = id[b];
                 pos[b + 1] = pos[b];
-                rad[b + 1] = rad[b];
+                if (rad != nullptr)
+                {
+                    rad[b + 1] = rad[b];
+                }
                 if (carry != nullptr)
                 {
                     carry[b + 1] = carry[b];
@@ -340,7 +348,10 @@ namespace
             }
             id[b + 1] = key;
             pos[b + 1] = kp;
-            rad[b + 1] = kr;
+            if (rad != nullptr)
+            {
+                rad[b + 1] = kr;
+            }
             if (carry != nullptr)
             {
                 carry[b + 1] = kc;
@@ -447,6 +458,8 @@ namespace
         }
     }
 
+    // UR: every agent's radius is ri (srad is not read).
+    template <bool UR>
     __device__ __forceinline__ void AccumulateRange(int s, int e, int k, float2 p, float ri, float R, float kSpring,
                                                     float fX, float fY, bool wrapX, bool wrapY,
                                                     const float2 *__restrict__ spos, const float *__restrict__ srad,
@@ -461,14 +474,14 @@ namespace
         {
             const float2 o0 = spos[q];
             const float2 o1 = spos[q + 1];
-            const float r0 = srad[q];
-            const float r1 = srad[q + 1];
+            const float r0 = UR ? ri : srad[q];
+            const float r1 = UR ? ri : srad[q + 1];
             one(q, o0, r0);
             one(q + 1, o1, r1);
         }
         if (q < e)
         {
-            one(q, spos[q], srad[q]);
+            one(q, spos[q], UR ? ri : srad[q]);
         }
     }
 
@@ -568,12 +581,14 @@ namespace
     // common case (about 2.9 agents per 3-cell row at one agent per cell).
     constexpr int kHead = 4;
 
-    template <bool WX, bool WY>
+    // UR: every agent has radius r0, and no radius is loaded.
+    template <bool WX, bool WY, bool UR>
     __global__ __launch_bounds__(kThreads) void AgentForces(int32_t N, int32_t X, int32_t Y, float R, float kSpring,
                                                             const int32_t *__restrict__ offsets,
                                                             const float2 *__restrict__ spos,
-                                                            const float *__restrict__ srad, float2 *__restrict__ vel,
-                                                            MoveArgs move, const int32_t *__restrict__ scanFault)
+                                                            const float *__restrict__ srad, float r0,
+                                                            float2 *__restrict__ vel, MoveArgs move,
+                                                            const int32_t *__restrict__ scanFault)
     {
         // Each XCD takes a contiguous range of slots: the neighbour rows a
         // slot reads (about Y slots away) then sit in the same L2.
@@ -583,7 +598,7 @@ namespace
             return; // (no neighbour ranges from a scan that gave up)
         }
         const float2 p = spos[k];
-        const float ri = srad[k];
+        const float ri = UR ? r0 : srad[k];
         const float fX = static_cast<float>(X);
         const float fY = static_cast<float>(Y);
         const int x0 = static_cast<int>(floorf(p.x - R));
@@ -627,7 +642,7 @@ namespace
                     if (q < re[r])
                     {
                         o[r][j] = spos[q];
-                        rq[r][j] = srad[q];
+                        rq[r][j] = UR ? ri : srad[q];
                     }
                 }
             }
@@ -643,8 +658,8 @@ namespace
                                       vy);
                     }
                 }
-                AccumulateRange(rs[r] + kHead, re[r], k, p, ri, R, kSpring, fX, fY, false, false, spos, srad, vx,
-                                vy);
+                AccumulateRange<UR>(rs[r] + kHead, re[r], k, p, ri, R, kSpring, fX, fY, false, false, spos, srad,
+                                    vx, vy);
             }
             vel[k] = make_float2(vx, vy);
             if (move.on)
@@ -671,8 +686,8 @@ namespace
                 const int yb = min(y1, Y);
                 if (ya < yb)
                 {
-                    AccumulateRange(offsets[rowBase + ya], offsets[rowBase + yb], k, p, ri, R, kSpring, fX, fY, WX,
-                                    WY, spos, srad, vx, vy);
+                    AccumulateRange<UR>(offsets[rowBase + ya], offsets[rowBase + yb], k, p, ri, R, kSpring, fX, fY,
+                                        WX, WY, spos, srad, vx, vy);
                 }
                 continue;
             }
@@ -680,8 +695,8 @@ namespace
             {
                 const int cy = WrapIndex(ys, Y);
                 const int len = min(y1 - ys, Y - cy);
-                AccumulateRange(offsets[rowBase + cy], offsets[rowBase + cy + len], k, p, ri, R, kSpring, fX, fY, WX,
-                                WY, spos, srad, vx, vy);
+                AccumulateRange<UR>(offsets[rowBase + cy], offsets[rowBase + cy + len], k, p, ri, R, kSpring, fX,
+                                    fY, WX, WY, spos, srad, vx, vy);
                 ys += len;
             }
         }
@@ -848,6 +863,11 @@ struct halhip_agents2d
     DeviceArray<int32_t> id[2];
     DeviceArray<float2> pos[2];
     DeviceArray<float> rad[2];
+    // Radii change only on upload.  When they are all equal (radius0), the
+    // force pass takes that value instead of loading each candidate's, and
+    // the binning moves no radii (rad[] then holds the upload's set only).
+    bool uniformRadius = false;
+    float radius0 = 0.0f;
     int32_t cur = 0;
     DeviceArray<float2> vel;    // slot order of set `cur`
     // Moved positions, slot order of set `cur`: after a tick the resident
@@ -959,13 +979,13 @@ namespace
             Launch(p, "agents_scatter", AgentScatter, Blocks(N, kThreads), kThreads, N,
                    static_cast<const int32_t *>(p->cellOf.get()), static_cast<const int32_t *>(p->ticketOf.get()),
                    static_cast<const int32_t *>(p->offsets.get()), static_cast<const int32_t *>(p->id[c].get()),
-                   positions, static_cast<const float *>(p->rad[c].get()),
+                   positions, static_cast<const float *>(p->uniformRadius ? nullptr : p->rad[c].get()),
                    static_cast<const float2 *>(p->motion ? p->carry[c].get() : nullptr), p->id[n].get(),
                    p->pos[n].get(), p->rad[n].get(), p->motion ? p->carry[n].get() : nullptr,
                    static_cast<const int32_t *>(p->scanWords.get() + 3));
             Launch(p, "agents_order", AgentOrderCells, Blocks((p->C + 3) / 4, kThreads), kThreads, p->C,
                    static_cast<const int32_t *>(p->offsets.get()), p->id[n].get(), p->pos[n].get(),
-                   p->rad[n].get(), p->motion ? p->carry[n].get() : nullptr,
+                   p->uniformRadius ? nullptr : p->rad[n].get(), p->motion ? p->carry[n].get() : nullptr,
                    static_cast<const int32_t *>(p->scanWords.get() + 3));
         }
         p->cur = n;
@@ -991,11 +1011,16 @@ namespace
         m.counts = p->counts.get();
         m.cellOf = p->cellOf.get();
         m.ticketOf = p->ticketOf.get();
-        auto kernel = wrapX ? (wrapY ? AgentForces<true, true> : AgentForces<true, false>)
-                            : (wrapY ? AgentForces<false, true> : AgentForces<false, false>);
+        auto pick = [&](auto ur)
+        {
+            constexpr bool UR = decltype(ur)::value;
+            return wrapX ? (wrapY ? AgentForces<true, true, UR> : AgentForces<true, false, UR>)
+                         : (wrapY ? AgentForces<false, true, UR> : AgentForces<false, false, UR>);
+        };
+        auto kernel = p->uniformRadius ? pick(std::true_type{}) : pick(std::false_type{});
         Launch(p, "agents_force", kernel, Blocks(p->N, kThreads), kThreads, p->N, p->X, p->Y, R, k,
                static_cast<const int32_t *>(p->offsets.get()), static_cast<const float2 *>(p->pos[c].get()),
-               static_cast<const float *>(p->rad[c].get()), p->vel.get(), m,
+               static_cast<const float *>(p->rad[c].get()), p->radius0, p->vel.get(), m,
                static_cast<const int32_t *>(p->scanWords.get() + 3));
         p->moved = move;
     }
@@ -1006,8 +1031,21 @@ namespace
         EnqueueForces(p, R, k, wrapX, wrapY, true, dt);
     }
 
-    // Copies the resident agents in and resets slot k to agent k.
-    void UploadAgents(halhip_agents2d *p, const float *positions, const float *radii, int32_t nAgents)
+    // Whether every radius equals the first (no early exit, so the loop
+    // vectorizes).
+    bool UniformRadii(const float *radii, size_t n)
+    {
+        bool same = true;
+        for (size_t i = 1; i < n; i++)
+        {
+            same &= radii[i] == radii[0];
+        }
+        return same;
+    }
+
+    // Copies the resident agents in and resets slot k to agent k.  uniform:
+    // every radius equals radii[0].
+    void UploadAgents(halhip_agents2d *p, const float *positions, const float *radii, int32_t nAgents, bool uniform)
     {
         if (nAgents != p->N)
         {
@@ -1023,10 +1061,13 @@ namespace
         p->N = nAgents;
         p->binsValid = false;
         p->moved = false;
+        p->uniformRadius = false;
         if (nAgents == 0)
         {
             return;
         }
+        p->uniformRadius = uniform;
+        p->radius0 = radii[0];
         const int c = p->cur;
         Check(hipMemcpyAsync(p->pos[c].get(), positions, sizeof(float2) * nAgents, hipMemcpyHostToDevice,
                              p->clock.stream),
@@ -1101,6 +1142,8 @@ namespace
             AppendKey(key, p->friction);
             AppendKey(key, p->maxVel);
             AppendKey(key, moved0 ? 1 : 0);
+            AppendKey(key, p->uniformRadius ? 1 : 0);
+            AppendKey(key, p->radius0);
             const int32_t savedCur = p->cur;
             const bool savedMoved = p->moved;
             p->cur = c0;
@@ -1236,7 +1279,7 @@ extern "C"
                         "halhip_agents2d_upload: agent count outside [0, construction-time capacity]");
                 Require(nAgents == 0 || (positions != nullptr && radii != nullptr),
                         "halhip_agents2d_upload: null array");
-                UploadAgents(p, positions, radii, nAgents);
+                UploadAgents(p, positions, radii, nAgents, UniformRadii(radii, static_cast<size_t>(nAgents)));
                 Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
             });
     }
@@ -1404,11 +1447,13 @@ extern "C"
                 {
                     p->hostBuf[i] = static_cast<float>(positions[i]);
                 }
+                bool uniform = true;
                 for (size_t i = 0; i < n; i++)
                 {
                     p->hostBuf[2 * n + i] = static_cast<float>(radii[i]);
+                    uniform &= p->hostBuf[2 * n + i] == p->hostBuf[2 * n];
                 }
-                UploadAgents(p, p->hostBuf.data(), p->hostBuf.data() + 2 * n, nAgents);
+                UploadAgents(p, p->hostBuf.data(), p->hostBuf.data() + 2 * n, nAgents, uniform);
                 EnqueueBins(p);
                 EnqueueForces(p, static_cast<float>(interactionRad), static_cast<float>(springConstant), wrapX, wrapY,
                               false, 0.0f);

@@ -178,6 +178,37 @@ def test_scan_wait_limit_fails_loudly_never_silently(ctx, oracle):
     plan.close()
 
 
+def test_equal_radii_path_matches_general_path(ctx):
+    # Equal radii take a force pass that loads no radius and a binning that
+    # moves none.  The same agents plus one lone agent of another radius
+    # (more than 4 cells from every other, beyond any drift in 12 ticks)
+    # take the general path: every other agent's trajectory must be
+    # bit-identical.  Re-uploads on one plan switch between the two and
+    # between two equal radii over graph-replayed ticks.
+    rng = np.random.default_rng(23)
+    X, Y, n, ticks = 40, 36, 1300, 12
+    pos = np.empty(2 * n, np.float32)
+    pos[0::2] = rng.random(n) * 30.0
+    pos[1::2] = rng.random(n) * Y
+    pos_b = np.concatenate([pos, np.array([35.5, 18.0], np.float32)])
+    plan = halhip.Agents2D(ctx, X, Y, n + 1)
+    for r in (0.5, 0.3):
+        rad = np.full(n, r, np.float32)
+        rad_b = np.concatenate([rad, np.array([1.5 * r], np.float32)])
+        plan.upload(pos, rad)
+        plan.run_ticks(ticks, 1.0, 1.0, 0.1, 0, 1)
+        equal = plan.positions()
+        plan.upload(pos_b, rad_b)
+        plan.run_ticks(ticks, 1.0, 1.0, 0.1, 0, 1)
+        general = plan.positions()
+        np.testing.assert_array_equal(equal, general[:2 * n])
+        assert np.abs(general[2 * n:] - pos_b[2 * n:]).max() == 0.0   # the lone agent feels nothing
+        plan.upload(pos, rad)
+        plan.run_ticks(ticks, 1.0, 1.0, 0.1, 0, 1)
+        np.testing.assert_array_equal(plan.positions(), equal)
+    plan.close()
+
+
 def test_graph_chunks_from_fresh_upload_and_forces_only(ctx, oracle):
     # The first graph chunk after an upload (or after a forces-only pass)
     # starts from the binned positions, later chunks from the moved ones.
