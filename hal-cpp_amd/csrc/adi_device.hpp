@@ -880,6 +880,11 @@ namespace
     // X = 0 K strips down (exact at the bottom).
     constexpr int kMaxWindow = 8;
 
+    //
+    // The rows are requested in batches of at most 32 values (clamped to rows
+    // that exist, used only where the window has them), so a window costs a
+    // few memory round trips rather than one per strip; the fold order, and
+    // so every bit, is the strip-by-strip one.
     template <int CPT, bool VEC, int KW>
     __device__ __forceinline__ void WindowCarries(const AdiArgs &a, int p, int y0, int K, float (&cf)[CPT],
                                                   float (&xb)[CPT])
@@ -890,6 +895,8 @@ namespace
         const int qLo = max(0, p - K);
         const int nUp = p - qLo;                 // strips above folded into cf
         const int nDn = min(a.P - 1, p + K) - p; // strips below folded into xb
+        constexpr int kUp = (32 / CPT < KW) ? (32 / CPT > 0 ? 32 / CPT : 1) : KW;        // B rows per batch
+        constexpr int kDn = (16 / CPT < KW) ? (16 / CPT > 0 ? 16 / CPT : 1) : KW;        // (B, D) pairs per batch
         float F[CPT];
         float X[CPT];
 #pragma unroll
@@ -899,17 +906,29 @@ namespace
             X[c] = 0.0f;
         }
 #pragma unroll
-        for (int i = 0; i < KW; i++)
+        for (int i0 = 0; i0 < KW; i0 += kUp)
         {
-            if (i < nUp)
+            if (i0 < nUp)
             {
-                float bq[CPT];
-                LoadRow<CPT, VEC>(B + static_cast<size_t>(qLo + i) * a.Y, y0, a.Y, bq);
-                const float A = a.stripA[qLo + i];
+                float bq[kUp][CPT];
 #pragma unroll
-                for (int c = 0; c < CPT; c++)
+                for (int j = 0; j < kUp; j++)
                 {
-                    F[c] = fmaf(A, F[c], bq[c]);
+                    const int q = qLo + min(i0 + j, nUp - 1);
+                    LoadRow<CPT, VEC>(B + static_cast<size_t>(q) * a.Y, y0, a.Y, bq[j]);
+                }
+#pragma unroll
+                for (int j = 0; j < kUp; j++)
+                {
+                    if (i0 + j < nUp && i0 + j < KW)
+                    {
+                        const float A = a.stripA[qLo + i0 + j];
+#pragma unroll
+                        for (int c = 0; c < CPT; c++)
+                        {
+                            F[c] = fmaf(A, F[c], bq[j][c]);
+                        }
+                    }
                 }
             }
         }
@@ -922,24 +941,36 @@ namespace
         // with F_q = A_{q-1} F_{q-1} + B_{q-1}.
         float coef = 1.0f;
 #pragma unroll
-        for (int i = 0; i < KW; i++)
+        for (int i0 = 0; i0 < KW; i0 += kDn)
         {
-            if (i < nDn)
+            if (i0 < nDn)
             {
-                const int q = p + 1 + i;
-                float bq[CPT];
-                float dq[CPT];
-                LoadRow<CPT, VEC>(B + static_cast<size_t>(q - 1) * a.Y, y0, a.Y, bq);
-                LoadRow<CPT, VEC>(D + static_cast<size_t>(q) * a.Y, y0, a.Y, dq);
-                const float A = a.stripA[q - 1];
-                const float Eq = a.stripE[q];
+                float bq[kDn][CPT];
+                float dq[kDn][CPT];
 #pragma unroll
-                for (int c = 0; c < CPT; c++)
+                for (int j = 0; j < kDn; j++)
                 {
-                    F[c] = fmaf(A, F[c], bq[c]);
-                    X[c] = fmaf(coef, fmaf(Eq, F[c], dq[c]), X[c]);
+                    const int q = p + 1 + min(i0 + j, nDn - 1);
+                    LoadRow<CPT, VEC>(B + static_cast<size_t>(q - 1) * a.Y, y0, a.Y, bq[j]);
+                    LoadRow<CPT, VEC>(D + static_cast<size_t>(q) * a.Y, y0, a.Y, dq[j]);
                 }
-                coef *= a.stripC[q];
+#pragma unroll
+                for (int j = 0; j < kDn; j++)
+                {
+                    if (i0 + j < nDn && i0 + j < KW)
+                    {
+                        const int q = p + 1 + i0 + j;
+                        const float A = a.stripA[q - 1];
+                        const float Eq = a.stripE[q];
+#pragma unroll
+                        for (int c = 0; c < CPT; c++)
+                        {
+                            F[c] = fmaf(A, F[c], bq[j][c]);
+                            X[c] = fmaf(coef, fmaf(Eq, F[c], dq[j][c]), X[c]);
+                        }
+                        coef *= a.stripC[q];
+                    }
+                }
             }
         }
 #pragma unroll
