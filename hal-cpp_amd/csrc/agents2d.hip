@@ -70,42 +70,71 @@ namespace
         ticketOf[i] = atomicAdd(&counts[cell], 1);
     }
 
-    __device__ __forceinline__ int WaveInclusiveSum(int v, int lane)
+    // Inclusive sum over the wavefront's lanes by DPP: shifts within each
+    // 16-lane row, then each row's total broadcast into the rows above it
+    // (row_bcast:15 into rows 1 and 3, row_bcast:31 into rows 2 and 3).
+    // Lanes a shift leaves without a source add 0.
+    __device__ __forceinline__ int WaveInclusiveSum(int v)
     {
+        v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false); // row_shr:1
+        v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false); // row_shr:2
+        v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false); // row_shr:4
+        v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false); // row_shr:8
+        v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false); // row_bcast:15
+        v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false); // row_bcast:31
+        return v;
+    }
+
+    // Exclusive block-wide sums of R sequences at once (value r of every
+    // thread is one sequence); total[r] gets each sequence's sum.  sWave
+    // holds R * (waves per block) ints.
+    template <int R>
+    __device__ __forceinline__ void BlockExclusiveSums(const int (&v)[R], int *sWave, int (&ex)[R], int (&total)[R])
+    {
+        const int lane = threadIdx.x & 63;
+        const int wave = threadIdx.x >> 6;
+        const int nWaves = blockDim.x >> 6;
+        int inc[R];
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1)
+        for (int r = 0; r < R; r++)
         {
-            int u = __shfl_up(v, o);
-            if (lane >= o)
+            inc[r] = WaveInclusiveSum(v[r]);
+        }
+        if (lane == 63)
+        {
+#pragma unroll
+            for (int r = 0; r < R; r++)
             {
-                v += u;
+                sWave[r * nWaves + wave] = inc[r];
             }
         }
-        return v;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; r++)
+        {
+            int before = 0;
+            int sum = 0;
+            for (int w = 0; w < nWaves; w++)
+            {
+                const int t = sWave[r * nWaves + w];
+                before += (w < wave) ? t : 0;
+                sum += t;
+            }
+            ex[r] = before + inc[r] - v[r];
+            total[r] = sum;
+        }
+        __syncthreads();
     }
 
     // Exclusive block-wide sum of one value per thread; also returns the total.
     __device__ __forceinline__ int BlockExclusiveSum(int v, int *sWave, int &total)
     {
-        const int lane = threadIdx.x & 63;
-        const int wave = threadIdx.x >> 6;
-        const int nWaves = blockDim.x >> 6;
-        int inc = WaveInclusiveSum(v, lane);
-        if (lane == 63)
-        {
-            sWave[wave] = inc;
-        }
-        __syncthreads();
-        int before = 0;
-        total = 0;
-        for (int w = 0; w < nWaves; w++)
-        {
-            int s = sWave[w];
-            before += (w < wave) ? s : 0;
-            total += s;
-        }
-        __syncthreads();
-        return before + inc - v;
+        const int in[1] = {v};
+        int ex[1];
+        int tot[1];
+        BlockExclusiveSums<1>(in, sWave, ex, tot);
+        total = tot[0];
+        return ex[0];
     }
 
     // The counts and offsets arrays are padded to whole tiles (padding counts
@@ -200,7 +229,7 @@ namespace
                                                                 int32_t *__restrict__ offsetsTail, int32_t N,
                                                                 int32_t nTiles, ScanState st)
     {
-        __shared__ int sWave[kScanThreads / 64];
+        __shared__ int sWave[kScanRounds * (kScanThreads / 64)];
         __shared__ uint32_t sEpoch;
         if (threadIdx.x == 0)
         {
@@ -217,14 +246,22 @@ namespace
         {
             v[r] = counts[tileBase + r * kScanThreads + threadIdx.x];
         }
+        // the rounds' block sums together, then chained round after round
+        int sums[kScanRounds];
+#pragma unroll
+        for (int r = 0; r < kScanRounds; r++)
+        {
+            sums[r] = (v[r].x + v[r].y) + (v[r].z + v[r].w);
+        }
         int ex[kScanRounds];
+        int roundTotal[kScanRounds];
+        BlockExclusiveSums<kScanRounds>(sums, sWave, ex, roundTotal);
         int tileTotal = 0;
 #pragma unroll
         for (int r = 0; r < kScanRounds; r++)
         {
-            int total;
-            ex[r] = tileTotal + BlockExclusiveSum((v[r].x + v[r].y) + (v[r].z + v[r].w), sWave, total);
-            tileTotal += total;
+            ex[r] += tileTotal;
+            tileTotal += roundTotal[r];
         }
         if (threadIdx.x == 0)
         {
