@@ -881,10 +881,10 @@ namespace
     constexpr int kMaxWindow = 8;
 
     //
-    // The rows are requested in batches of at most 32 values (clamped to rows
-    // that exist, used only where the window has them), so a window costs a
-    // few memory round trips rather than one per strip; the fold order, and
-    // so every bit, is the strip-by-strip one.
+    // The window's rows are requested in batches of at most 32 values before
+    // any of the batch is used, so a window costs a few memory round trips
+    // rather than one per strip; the fold order, and so every bit, is the
+    // strip-by-strip one.
     template <int CPT, bool VEC, int KW>
     __device__ __forceinline__ void WindowCarries(const AdiArgs &a, int p, int y0, int K, float (&cf)[CPT],
                                                   float (&xb)[CPT])
@@ -910,12 +910,14 @@ namespace
         {
             if (i0 < nUp)
             {
-                float bq[kUp][CPT];
+                float bq[kUp][CPT] = {};
 #pragma unroll
                 for (int j = 0; j < kUp; j++)
                 {
-                    const int q = qLo + min(i0 + j, nUp - 1);
-                    LoadRow<CPT, VEC>(B + static_cast<size_t>(q) * a.Y, y0, a.Y, bq[j]);
+                    if (i0 + j < nUp && i0 + j < KW)
+                    {
+                        LoadRow<CPT, VEC>(B + static_cast<size_t>(qLo + i0 + j) * a.Y, y0, a.Y, bq[j]);
+                    }
                 }
 #pragma unroll
                 for (int j = 0; j < kUp; j++)
@@ -945,14 +947,17 @@ namespace
         {
             if (i0 < nDn)
             {
-                float bq[kDn][CPT];
-                float dq[kDn][CPT];
+                float bq[kDn][CPT] = {};
+                float dq[kDn][CPT] = {};
 #pragma unroll
                 for (int j = 0; j < kDn; j++)
                 {
-                    const int q = p + 1 + min(i0 + j, nDn - 1);
-                    LoadRow<CPT, VEC>(B + static_cast<size_t>(q - 1) * a.Y, y0, a.Y, bq[j]);
-                    LoadRow<CPT, VEC>(D + static_cast<size_t>(q) * a.Y, y0, a.Y, dq[j]);
+                    if (i0 + j < nDn && i0 + j < KW)
+                    {
+                        const int q = p + 1 + i0 + j;
+                        LoadRow<CPT, VEC>(B + static_cast<size_t>(q - 1) * a.Y, y0, a.Y, bq[j]);
+                        LoadRow<CPT, VEC>(D + static_cast<size_t>(q) * a.Y, y0, a.Y, dq[j]);
+                    }
                 }
 #pragma unroll
                 for (int j = 0; j < kDn; j++)
