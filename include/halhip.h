@@ -365,6 +365,8 @@ int halhip_agents2d_forces_f64(halhip_agents2d *plan, const double *positions, c
 /* Resident agents: FP32 positions and radii on the device. */
 int halhip_agents2d_upload(halhip_agents2d *plan, const float *positions, const float *radii, int32_t nAgents);
 int halhip_agents2d_download_positions(halhip_agents2d *plan, float *positions, int32_t nAgents);
+/* Velocity deltas (2*nAgents) of the last tick of the last run (or of the
+ * last forces-only pass). */
 int halhip_agents2d_download_velocities(halhip_agents2d *plan, float *velocityDeltas, int32_t nAgents);
 /* Bins of the last tick: offsets (xDim*yDim+1 int32, exclusive prefix of
  * per-bin counts) and bucket (nAgents int32, agent indices in bin order,

@@ -250,6 +250,31 @@ def test_bins_each_tick_match_oracle_on_gpu_positions(ctx, oracle):
         np.testing.assert_array_equal(bucket, rb)
 
 
+@pytest.mark.parametrize("ticks", [20, 13, 1])
+def test_velocities_are_the_last_ticks(ctx, oracle, ticks):
+    # Resident ticks keep only the last tick's velocity deltas (a graph
+    # chunk writes them on its last tick, an eager run on its last): after
+    # run_ticks(n) they are the forces on the positions after n-1 ticks,
+    # bit-identical to stepping n-1 ticks and then one more.
+    rng = np.random.default_rng(41)
+    X, Y, n = 48, 40, 2000
+    pos = (rng.random(2 * n) * np.tile([X, Y], n)).astype(np.float32)
+    rad = np.full(n, 0.5, np.float32)
+    plan = halhip.Agents2D(ctx, X, Y, n)
+    plan.upload(pos, rad)
+    plan.run_ticks(ticks, 1.0, 1.0, 0.1, 0, 1)
+    v_run = plan.velocities()
+    plan.upload(pos, rad)
+    if ticks > 1:
+        plan.run_ticks(ticks - 1, 1.0, 1.0, 0.1, 0, 1)
+    before = plan.positions()
+    plan.run_ticks(1, 1.0, 1.0, 0.1, 0, 1)
+    np.testing.assert_array_equal(v_run, plan.velocities())
+    ref = oracle.agent_forces(before.astype(np.float64), rad.astype(np.float64), 1.0, 1.0, X, Y, 0, 1)
+    assert rel_l2(v_run, ref) < 1e-5
+    plan.close()
+
+
 def test_workload_4m_first_tick(ctx, oracle):
     w = workloads.agents_uniform()
     n = w["rad"].size
