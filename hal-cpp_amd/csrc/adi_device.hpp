@@ -1542,7 +1542,6 @@ namespace
     __device__ __forceinline__ void RhsAFromLanes(const AdiArgs &a, const float *sFirst, const float *sLast, int s0,
                                                   int y0, bool first, bool last, float (&v)[S][CPT])
     {
-        const int lane = threadIdx.x & 63;
         const int wave = threadIdx.x >> 6;
         const int nW = blockDim.x >> 6;
         const int wl = (wave > 0) ? wave - 1 : nW - 1;
