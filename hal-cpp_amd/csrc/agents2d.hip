@@ -54,9 +54,11 @@ namespace
         return c < 0 ? 0 : (c >= dim ? dim - 1 : c);
     }
 
+    // Counts slot i's agent into its clamped cell and takes a ticket there
+    // (the scatter computes the cell again from the same position).
     __global__ __launch_bounds__(kThreads) void AgentCount(int32_t N, int32_t X, int32_t Y,
                                                              const float2 *__restrict__ pos,
-                                                             int32_t *__restrict__ counts, int32_t *__restrict__ cellOf,
+                                                             int32_t *__restrict__ counts,
                                                              int32_t *__restrict__ ticketOf)
     {
         const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -66,7 +68,6 @@ namespace
         }
         const float2 p = pos[i];
         const int cell = ClampCell(p.x, X) * Y + ClampCell(p.y, Y);
-        cellOf[i] = cell;
         ticketOf[i] = atomicAdd(&counts[cell], 1);
     }
 
@@ -321,9 +322,10 @@ namespace
         }
     }
 
-    // Moves slot k's (id, position, radius) to slot offsets[cell] + ticket
-    // (no radius with rad null: every agent has the same radius).
-    __global__ __launch_bounds__(kThreads) void AgentScatter(int32_t N, const int32_t *__restrict__ cellOf,
+    // Moves slot k's (id, position, radius) to slot offsets[cell] + ticket,
+    // with the clamped cell of its position as the count computed it (no
+    // radius with rad null: every agent has the same radius).
+    __global__ __launch_bounds__(kThreads) void AgentScatter(int32_t N, int32_t X, int32_t Y,
                                                              const int32_t *__restrict__ ticketOf,
                                                              const int32_t *__restrict__ offsets,
                                                              const int32_t *__restrict__ id,
@@ -339,9 +341,10 @@ namespace
         {
             return; // (a scan that gave up left offsets that must not be used)
         }
-        const int dst = offsets[cellOf[k]] + ticketOf[k];
+        const float2 q = pos[k];
+        const int dst = offsets[ClampCell(q.x, X) * Y + ClampCell(q.y, Y)] + ticketOf[k];
         nId[dst] = id[k];
-        nPos[dst] = pos[k];
+        nPos[dst] = q;
         if (rad != nullptr)
         {
             nRad[dst] = rad[k];
@@ -540,9 +543,8 @@ namespace
         float friction;
         float maxVel;
         // the next binning's count of the moved positions (AgentCount's
-        // work): clamped cell and a ticket in it
+        // work): a ticket in the clamped cell
         int32_t *counts;
-        int32_t *cellOf;
         int32_t *ticketOf;
     };
 
@@ -604,7 +606,6 @@ namespace
         const float2 q = MovedPosition(m, k, o, v, X, Y, wrapX, wrapY);
         m.posNext[k] = q;
         const int cell = ClampCell(q.x, X) * Y + ClampCell(q.y, Y);
-        m.cellOf[k] = cell;
         m.ticketOf[k] = atomicAdd(&m.counts[cell], 1);
     }
 
@@ -909,7 +910,7 @@ struct halhip_agents2d
     DeviceArray<float2> vel;    // slot order of set `cur`
     // Moved positions, slot order of set `cur`: after a tick the resident
     // positions are here (moved == true), already counted into
-    // counts / cellOf / ticketOf, until the next binning scatters them into
+    // counts / ticketOf, until the next binning scatters them into
     // the other set.
     DeviceArray<float2> posNext;
     bool moved = false;
@@ -927,7 +928,6 @@ struct halhip_agents2d
     DeviceArray<unsigned long long> scanTiles;
     DeviceArray<int32_t> scanWords;
     int32_t scanSpinLimit = 1 << 20;
-    DeviceArray<int32_t> cellOf;
     DeviceArray<int32_t> ticketOf;
     bool binsValid = false;
     StreamClock clock;
@@ -1004,7 +1004,7 @@ namespace
         {
             // (moved positions were counted by the move itself)
             Launch(p, "agents_count", AgentCount, Blocks(N, kThreads), kThreads, N, p->X, p->Y, positions,
-                   p->counts.get(), p->cellOf.get(), p->ticketOf.get());
+                   p->counts.get(), p->ticketOf.get());
         }
         int4 *counts4 = reinterpret_cast<int4 *>(p->counts.get());
         const int64_t cPad = static_cast<int64_t>(p->nTiles) * kScanTile;
@@ -1014,7 +1014,7 @@ namespace
         if (N > 0)
         {
             Launch(p, "agents_scatter", AgentScatter, Blocks(N, kThreads), kThreads, N,
-                   static_cast<const int32_t *>(p->cellOf.get()), static_cast<const int32_t *>(p->ticketOf.get()),
+                   p->X, p->Y, static_cast<const int32_t *>(p->ticketOf.get()),
                    static_cast<const int32_t *>(p->offsets.get()), static_cast<const int32_t *>(p->id[c].get()),
                    positions, static_cast<const float *>(p->uniformRadius ? nullptr : p->rad[c].get()),
                    static_cast<const float2 *>(p->motion ? p->carry[c].get() : nullptr), p->id[n].get(),
@@ -1046,7 +1046,6 @@ namespace
         m.friction = p->friction;
         m.maxVel = p->maxVel;
         m.counts = p->counts.get();
-        m.cellOf = p->cellOf.get();
         m.ticketOf = p->ticketOf.get();
         auto pick = [&](auto ur)
         {
@@ -1265,7 +1264,6 @@ extern "C"
                 p->vel.Allocate(ctx, cap);
                 p->posNext.Allocate(ctx, cap);
                 p->outBuf.Allocate(ctx, cap);
-                p->cellOf.Allocate(ctx, cap);
                 p->ticketOf.Allocate(ctx, cap);
                 // counts and offsets padded to whole scan tiles; counts start
                 // at 0 and the scan resets them after every binning
