@@ -418,10 +418,55 @@ namespace
         const int4 o = *reinterpret_cast<const int4 *>(offsets + c0);
         const int o4 = offsets[c0 + 4];
         const int b[5] = {o.x, o.y, o.z, o.w, o4};
+        // Cells of two agents (most of the cells that need ordering at about
+        // one agent per cell): both ids of all four cells are requested
+        // before any is compared, and a pair out of order is swapped in
+        // straight-line code.  Larger cells take the insertion sort.
+        int first[4];
+        int second[4];
 #pragma unroll
         for (int j = 0; j < 4; j++)
         {
-            if (c0 + j < C && b[j + 1] - b[j] > 1)
+            const bool pair = c0 + j < C && b[j + 1] - b[j] == 2;
+            first[j] = pair ? id[b[j]] : 0;
+            second[j] = pair ? id[b[j] + 1] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+        {
+            if (c0 + j >= C)
+            {
+                continue;
+            }
+            const int n = b[j + 1] - b[j];
+            if (n == 2)
+            {
+                if (first[j] > second[j])
+                {
+                    const int s = b[j];
+                    id[s] = second[j];
+                    id[s + 1] = first[j];
+                    const float2 p0 = pos[s];
+                    const float2 p1 = pos[s + 1];
+                    pos[s] = p1;
+                    pos[s + 1] = p0;
+                    if (rad != nullptr)
+                    {
+                        const float r0 = rad[s];
+                        const float r1 = rad[s + 1];
+                        rad[s] = r1;
+                        rad[s + 1] = r0;
+                    }
+                    if (carry != nullptr)
+                    {
+                        const float2 k0 = carry[s];
+                        const float2 k1 = carry[s + 1];
+                        carry[s] = k1;
+                        carry[s + 1] = k0;
+                    }
+                }
+            }
+            else if (n > 2)
             {
                 OrderSlice(b[j], b[j + 1], id, pos, rad, carry);
             }
