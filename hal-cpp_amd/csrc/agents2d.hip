@@ -225,7 +225,7 @@ namespace
         int32_t spinLimit;
     };
 
-    __global__ __launch_bounds__(kScanThreads) void ScanOnePass(int4 *__restrict__ counts,
+    __global__ __launch_bounds__(kScanThreads) void ScanOnePass(const int4 *__restrict__ counts,
                                                                 int4 *__restrict__ offsets,
                                                                 int32_t *__restrict__ offsetsTail, int32_t N,
                                                                 int32_t nTiles, ScanState st)
@@ -314,7 +314,6 @@ namespace
             const int64_t i = tileBase + r * kScanThreads + threadIdx.x;
             const int o = run + ex[r];
             offsets[i] = make_int4(o, o + v[r].x, o + v[r].x + v[r].y, o + v[r].x + v[r].y + v[r].z);
-            counts[i] = make_int4(0, 0, 0, 0);
         }
         if (tile == 0 && threadIdx.x == 0)
         {
@@ -402,19 +401,24 @@ namespace
     // Sorts every cell's slice, four consecutive cells per thread: their five
     // offsets are one int4 and one int (offsets holds whole scan tiles plus
     // 4 entries, and c0 is a multiple of 4, so both loads are in bounds).
+    // It also resets its four cells' counts for the next binning (the scan
+    // has read them); counts holds whole tiles, so the int4 is in bounds.
     __global__ __launch_bounds__(kThreads) void AgentOrderCells(int64_t C, const int32_t *__restrict__ offsets,
                                                                 int32_t *__restrict__ id, float2 *__restrict__ pos,
                                                                 float *__restrict__ rad, float2 *__restrict__ carry,
+                                                                int32_t *__restrict__ counts,
                                                                 const int32_t *__restrict__ scanFault)
     {
         const int64_t c0 =
             4 * (static_cast<int64_t>(XcdContiguousBlock(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x);
         // Offsets from a scan that gave up are not monotonic: one cell's
-        // range could span most slots, which no insertion sort may walk.
+        // range could span most slots, which no insertion sort may walk
+        // (the host then clears the counts).
         if (c0 >= C || *scanFault != 0)
         {
             return;
         }
+        *reinterpret_cast<int4 *>(counts + c0) = make_int4(0, 0, 0, 0);
         const int4 o = *reinterpret_cast<const int4 *>(offsets + c0);
         const int o4 = offsets[c0 + 4];
         const int b[5] = {o.x, o.y, o.z, o.w, o4};
@@ -1008,6 +1012,10 @@ namespace
         const int32_t words[4] = {0, 0, 1, 0};
         Check(hipMemcpyAsync(p->scanWords.get(), words, sizeof(words), hipMemcpyHostToDevice, p->clock.stream),
               "hipMemcpyAsync");
+        // the order pass, which resets the counts, stopped on the fault
+        Check(hipMemsetAsync(p->counts.get(), 0, static_cast<size_t>(p->nTiles) * kScanTile * sizeof(int32_t),
+                             p->clock.stream),
+              "hipMemsetAsync");
         Check(hipStreamSynchronize(p->clock.stream), "hipStreamSynchronize");
     }
 
@@ -1068,7 +1076,7 @@ namespace
             Launch(p, "agents_order", AgentOrderCells, Blocks((p->C + 3) / 4, kThreads), kThreads, p->C,
                    static_cast<const int32_t *>(p->offsets.get()), p->id[n].get(), p->pos[n].get(),
                    p->uniformRadius ? nullptr : p->rad[n].get(), p->motion ? p->carry[n].get() : nullptr,
-                   static_cast<const int32_t *>(p->scanWords.get() + 3));
+                   p->counts.get(), static_cast<const int32_t *>(p->scanWords.get() + 3));
         }
         p->cur = n;
         p->moved = false;
@@ -1311,7 +1319,7 @@ extern "C"
                 p->outBuf.Allocate(ctx, cap);
                 p->ticketOf.Allocate(ctx, cap);
                 // counts and offsets padded to whole scan tiles; counts start
-                // at 0 and the scan resets them after every binning
+                // at 0 and the order pass resets them after every binning
                 const size_t cPad = static_cast<size_t>(p->nTiles) * kScanTile;
                 p->counts.Allocate(ctx, cPad);
                 p->offsets.Allocate(ctx, cPad + 4);
